@@ -1,0 +1,187 @@
+"""Seeded synthetic inputs for the template-matching workloads.
+
+* Template: an isotropic-Gaussian pseudo-atom volume (T^3 float32), atoms uniform in a ball
+  centred on the box centre (T/2, T/2, T/2) — the reference's box-centre convention
+  (src/core/image.cpp:5019-5034).
+* Micrograph: copies of the template projected at uniform-random SO(3) orientations and
+  non-overlapping positions, multiplied in Fourier space by a CTF (the -sin(chi) model of
+  src/core/ctf.cpp:445-541), plus white Gaussian noise at a stated SNR
+  (SNR = var(signal image) / sigma_noise^2 over the full frame).
+
+Orientation convention. The search extracts the Fourier slice at R(phi, theta, psi) * (kx, ky, 0)
+(src/core/image.cpp:2450-2607 with the ZYZ matrix of src/core/matrix.cpp:490-515), which is the
+transform of p(u) = sum_z V(R (u, z)). A Gaussian atom at a therefore projects to a 2-D Gaussian
+at (R^T a)_xy, with integral along z equal to amplitude * sigma * sqrt(2 pi). Inserted copies use
+exactly that, so the ground-truth angles are directly comparable with the search's best angles.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+PIf = np.float32(3.14159265359)
+
+
+def euler_matrix(phi: float, theta: float, psi: float) -> np.ndarray:
+    """ZYZ rotation matrix m[row][col] from degrees (src/core/matrix.cpp:490-515)."""
+    d2r = np.pi / 180.0
+    cp, sp = np.cos(phi * d2r), np.sin(phi * d2r)
+    ct, st = np.cos(theta * d2r), np.sin(theta * d2r)
+    cs, ss = np.cos(psi * d2r), np.sin(psi * d2r)
+    m = np.empty((3, 3))
+    m[0, 0] = cp * ct * cs - sp * ss
+    m[1, 0] = sp * ct * cs + cp * ss
+    m[2, 0] = -st * cs
+    m[0, 1] = -cp * ct * ss - sp * cs
+    m[1, 1] = -sp * ct * ss + cp * cs
+    m[2, 1] = st * ss
+    m[0, 2] = st * cp
+    m[1, 2] = st * sp
+    m[2, 2] = ct
+    return m
+
+
+def random_euler(rng: np.random.Generator):
+    """Uniform SO(3) sample returned as ZYZ (phi, theta, psi) degrees."""
+    phi = rng.uniform(0.0, 360.0)
+    psi = rng.uniform(0.0, 360.0)
+    theta = np.degrees(np.arccos(rng.uniform(-1.0, 1.0)))
+    return float(phi), float(theta), float(psi)
+
+
+def make_atoms(rng: np.random.Generator, n_atoms: int, radius: float):
+    """Positions uniform in a ball (relative to the box centre) and amplitudes U(0.5, 1)."""
+    d = rng.standard_normal((n_atoms, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r = radius * rng.uniform(0.0, 1.0, n_atoms) ** (1.0 / 3.0)
+    pos = d * r[:, None]
+    amp = rng.uniform(0.5, 1.0, n_atoms)
+    return pos, amp
+
+
+def render_volume(T: int, pos: np.ndarray, amp: np.ndarray, sigma: float = 1.5) -> np.ndarray:
+    """(z, y, x) float32 volume; atom coordinates (x, y, z) are relative to the box centre T/2."""
+    vol = np.zeros((T, T, T), np.float64)
+    c = T // 2
+    half = int(np.ceil(4 * sigma))
+    offs = np.arange(-half, half + 1)
+    for (x, y, z), a in zip(pos, amp):
+        ix, iy, iz = int(np.floor(c + x)), int(np.floor(c + y)), int(np.floor(c + z))
+        xs, ys, zs = ix + offs, iy + offs, iz + offs
+        gx = np.exp(-((xs - (c + x)) ** 2) / (2 * sigma * sigma))
+        gy = np.exp(-((ys - (c + y)) ** 2) / (2 * sigma * sigma))
+        gz = np.exp(-((zs - (c + z)) ** 2) / (2 * sigma * sigma))
+        mx, my, mz = (xs >= 0) & (xs < T), (ys >= 0) & (ys < T), (zs >= 0) & (zs < T)
+        blk = a * gz[mz][:, None, None] * gy[my][None, :, None] * gx[mx][None, None, :]
+        vol[zs[mz][0]:zs[mz][-1] + 1, ys[my][0]:ys[my][-1] + 1, xs[mx][0]:xs[mx][-1] + 1] += blk
+    return vol.astype(np.float32)
+
+
+def ctf_2d(ny: int, nx: int, pixel_size: float, kv: float, cs_mm: float, amp_contrast: float,
+           defocus_angstrom: float, phase_shift_rad: float = 0.0) -> np.ndarray:
+    """-sin(chi) on the rfft grid (ny, nx//2+1), the model of src/core/ctf.cpp:93-150, 445-541."""
+    lam = 12.2639 / np.sqrt(1000.0 * kv + 0.97845e-6 * (1000.0 * kv) ** 2) / pixel_size
+    cs = cs_mm * 1e7 / pixel_size
+    df = defocus_angstrom / pixel_size
+    ac = np.arctan(amp_contrast / np.sqrt(1.0 - amp_contrast ** 2))
+    fy = np.fft.fftfreq(ny)[:, None]
+    fx = np.arange(nx // 2 + 1)[None, :] / nx
+    g2 = fx * fx + fy * fy
+    chi = np.pi * lam * g2 * (df - 0.5 * lam * lam * g2 * cs) + phase_shift_rad + ac
+    return (-np.sin(chi)).astype(np.float32)
+
+
+def place_particles(rng: np.random.Generator, n: int, W: int, H: int, margin: int, min_dist: float):
+    """Random sequential placement with a minimum centre distance (relaxed by 5% when stuck)."""
+    pts = []
+    d = float(min_dist)
+    tries = 0
+    while len(pts) < n:
+        x = rng.uniform(margin, W - margin)
+        y = rng.uniform(margin, H - margin)
+        if all((x - px) ** 2 + (y - py) ** 2 >= d * d for px, py in pts):
+            pts.append((x, y))
+            tries = 0
+        else:
+            tries += 1
+            if tries > 2000:
+                d *= 0.95
+                tries = 0
+    return np.array(pts), d
+
+
+@dataclass
+class Particle:
+    x: float
+    y: float
+    phi: float
+    theta: float
+    psi: float
+
+
+@dataclass
+class Synthetic:
+    micrograph: np.ndarray          # (H, W) float32
+    volume: np.ndarray              # (T, T, T) float32, z-y-x
+    particles: list = field(default_factory=list)
+    noise_sigma: float = 0.0
+    signal_var: float = 0.0
+    min_dist: float = 0.0
+    defocus: float = 0.0
+
+
+def project_copy(pos: np.ndarray, amp: np.ndarray, m: np.ndarray, cx: float, cy: float, W: int, H: int,
+                 box: int, sigma: float, out: np.ndarray) -> None:
+    """Adds the projection of the atom model at rotation m, centred at (cx, cy), into out (H, W)."""
+    proj = pos @ m  # rows: (R^T a)^T = a^T R
+    px = cx + proj[:, 0]
+    py = cy + proj[:, 1]
+    x0 = max(0, int(np.floor(cx)) - box // 2)
+    y0 = max(0, int(np.floor(cy)) - box // 2)
+    x1 = min(W, int(np.floor(cx)) + box // 2 + 1)
+    y1 = min(H, int(np.floor(cy)) + box // 2 + 1)
+    xs = np.arange(x0, x1, dtype=np.float64)
+    ys = np.arange(y0, y1, dtype=np.float64)
+    s2 = 2.0 * sigma * sigma
+    gx = np.exp(-((xs[None, :] - px[:, None]) ** 2) / s2)
+    gy = np.exp(-((ys[None, :] - py[:, None]) ** 2) / s2)
+    w = amp * sigma * np.sqrt(2.0 * np.pi)
+    out[y0:y1, x0:x1] += (gy.T * w[None, :]) @ gx
+
+
+def make_synthetic(seed: int, W: int, H: int, T: int, n_atoms: int, atom_radius: float, n_copies: int,
+                   snr: float, defocus: float, pixel_size: float = 1.0, kv: float = 300.0, cs_mm: float = 2.7,
+                   amp_contrast: float = 0.07, sigma: float = 1.5, min_dist: float | None = None) -> Synthetic:
+    rng = np.random.default_rng(seed)
+    pos, amp = make_atoms(rng, n_atoms, atom_radius)
+    vol = render_volume(T, pos, amp, sigma)
+    if min_dist is None:
+        min_dist = T
+    centres, used = place_particles(rng, n_copies, W, H, T // 2, min_dist)
+    img = np.zeros((H, W), np.float64)
+    parts = []
+    box = int(2 * (atom_radius + 4 * sigma)) + 2
+    for (cx, cy) in centres:
+        phi, theta, psi = random_euler(rng)
+        m = euler_matrix(phi, theta, psi)
+        project_copy(pos, amp, m, cx, cy, W, H, box, sigma, img)
+        parts.append(Particle(float(cx), float(cy), phi, theta, psi))
+    spec = np.fft.rfft2(img)
+    spec *= ctf_2d(H, W, pixel_size, kv, cs_mm, amp_contrast, defocus)
+    signal = np.fft.irfft2(spec, s=(H, W))
+    var = float(signal.var())
+    noise_sigma = float(np.sqrt(var / snr)) if snr > 0 else 0.0
+    noisy = signal + noise_sigma * rng.standard_normal((H, W))
+    return Synthetic(noisy.astype(np.float32), vol, parts, noise_sigma, var, used, defocus)
+
+
+# Workload presets (sizes and counts of the benchmark configurations).
+def config0(seed: int = 0) -> Synthetic:
+    """512^2 micrograph, 64^3 template (200 atoms, r=24), 10 copies, SNR 0.1, defocus 15000 A."""
+    return make_synthetic(seed, 512, 512, 64, 200, 24.0, 10, 0.1, 15000.0, min_dist=64)
+
+
+def config1(seed: int = 1000) -> Synthetic:
+    """4096^2 micrograph, 256^3 template (3000 atoms, r=100), 300 copies, SNR 0.05, defocus 15000 A."""
+    return make_synthetic(seed, 4096, 4096, 256, 3000, 100.0, 300, 0.05, 15000.0, min_dist=200)
