@@ -1,0 +1,203 @@
+"""ctypes binding of libtm_hip.so (include/tm_hip.h) — the host-side mirror of the reference's
+``TemplateMatchingCore`` (src/gpu/TemplateMatchingCore.h:55-179).
+
+``TemplateMatchingCore.init`` / ``run_inner_loop`` keep the reference's method names and argument
+meaning; arrays are numpy, in the reference's FFTW layouts. Errors raise ``TMError`` carrying the
+library's message. There is no CPU fallback: if the HIP library is missing this module raises
+at import of the library, and every call goes to the GPU.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtm_hip.so")
+_lib = None
+
+F32P = ctypes.POINTER(ctypes.c_float)
+F64P = ctypes.POINTER(ctypes.c_double)
+I32P = ctypes.POINTER(ctypes.c_int)
+I64P = ctypes.POINTER(ctypes.c_longlong)
+
+# Every symbol include/tm_hip.h declares.
+EXPORTS = ["tm_device_count", "tm_create", "tm_destroy", "tm_init", "tm_run_inner_loop", "tm_get_results",
+           "tm_reset", "tm_finalize", "tm_debug_projection", "tm_debug_ccf", "tm_set_timing", "tm_get_timing",
+           "tm_last_error"]
+
+PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.c_longlong, ctypes.c_void_p)
+
+
+class TMError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"tm error {code}: {msg}")
+        self.code = code
+
+
+class InitParams(ctypes.Structure):
+    _fields_ = [("template_size", ctypes.c_int), ("projection_size", ctypes.c_int),
+                ("image_x", ctypes.c_int), ("image_y", ctypes.c_int),
+                ("n_search_positions", ctypes.c_int), ("n_psi", ctypes.c_int),
+                ("first_search_position", ctypes.c_int), ("last_search_position", ctypes.c_int),
+                ("histogram_min_scaled", ctypes.c_double), ("histogram_step_scaled", ctypes.c_double),
+                ("histogram_number_of_bins", ctypes.c_int), ("max_padding", ctypes.c_int),
+                ("defocus_step", ctypes.c_float), ("pixel_size_step", ctypes.c_float),
+                ("batch_size", ctypes.c_int)]
+
+
+def lib():
+    """Loads libtm_hip.so; raises if it has not been built (no fallback path exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m cistem_downstream_bah_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        L.tm_last_error.restype = ctypes.c_char_p
+        L.tm_device_count.argtypes = [I32P]
+        L.tm_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+        L.tm_destroy.argtypes = [vp]
+        L.tm_init.argtypes = [vp, ctypes.POINTER(InitParams), F32P, F32P, F32P, F32P]
+        L.tm_run_inner_loop.argtypes = [vp, F32P, ctypes.c_int, ctypes.c_int, PROGRESS_FN, vp, I64P]
+        L.tm_get_results.argtypes = [vp, F32P, F32P, F32P, F32P, F32P, F32P, F64P, F64P, I32P, I64P, I64P]
+        L.tm_reset.argtypes = [vp]
+        L.tm_finalize.argtypes = [vp, ctypes.c_longlong, F32P, F32P, F32P, F32P]
+        L.tm_debug_projection.argtypes = [vp, F32P, ctypes.c_int, ctypes.c_int, F32P]
+        L.tm_debug_ccf.argtypes = [vp, F32P, ctypes.c_int, ctypes.c_int, F32P]
+        L.tm_set_timing.argtypes = [vp, ctypes.c_int]
+        L.tm_get_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_char_p), F64P, I64P, ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise TMError(rc, lib().tm_last_error().decode())
+
+
+def _f32(a):
+    return None if a is None else a.ctypes.data_as(F32P)
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    _check(lib().tm_device_count(ctypes.byref(n)))
+    return n.value
+
+
+@dataclass
+class Results:
+    mip: np.ndarray
+    psi: np.ndarray
+    theta: np.ndarray
+    phi: np.ndarray
+    defocus: np.ndarray
+    pixel_size: np.ndarray
+    sum: np.ndarray
+    sumsq: np.ndarray
+    best_index: np.ndarray
+    hist: np.ndarray
+    nccf: int
+
+
+class TemplateMatchingCore:
+    """One search on one GPU (one per host thread, like the reference's per-thread objects)."""
+
+    def __init__(self, device: int = 0):
+        self.h = ctypes.c_void_p()
+        _check(lib().tm_create(device, ctypes.byref(self.h)))
+        self.W = self.H = self.T = self.P = 0
+
+    def close(self):
+        if self.h:
+            lib().tm_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init(self, template_spectrum: np.ndarray, input_spectrum: np.ndarray, projection_size: int,
+             search_positions: np.ndarray, psi_list: np.ndarray, first_search_position: int,
+             last_search_position: int, histogram_min_scaled: float, histogram_step_scaled: float,
+             histogram_number_of_bins: int = 512, max_padding: int = 0, defocus_step: float = 0.0,
+             pixel_size_step: float = 0.0, batch_size: int = 0) -> None:
+        """TemplateMatchingCore::Init (TemplateMatchingCore.cu:49-138).
+
+        template_spectrum: (P, P, P/2+1) complex64; input_spectrum: (H, W/2+1) complex64;
+        search_positions: (n, 2) float32 {phi, theta}; psi_list: float32 in-plane angles."""
+        tpl = np.ascontiguousarray(template_spectrum, np.complex64)
+        inp = np.ascontiguousarray(input_spectrum, np.complex64)
+        grid = np.ascontiguousarray(search_positions, np.float32)
+        psi = np.ascontiguousarray(psi_list, np.float32)
+        P = tpl.shape[0]
+        H, Wh = inp.shape
+        W = 2 * (Wh - 1)
+        p = InitParams(P, projection_size, W, H, grid.shape[0], psi.size, first_search_position,
+                       last_search_position, histogram_min_scaled, histogram_step_scaled, histogram_number_of_bins,
+                       max_padding, defocus_step, pixel_size_step, batch_size)
+        _check(lib().tm_init(self.h, ctypes.byref(p), _f32(tpl.view(np.float32)), _f32(inp.view(np.float32)),
+                             _f32(grid), _f32(psi)))
+        self.W, self.H, self.T, self.P = W, H, projection_size, P
+        self.n_psi = psi.size
+        self.first, self.last = first_search_position, last_search_position
+
+    def run_inner_loop(self, projection_filter: np.ndarray, size_i: int = 0, defocus_i: int = 0,
+                       progress=None) -> int:
+        """TemplateMatchingCore::RunInnerLoop (TemplateMatchingCore.cu:140-368) plus the host merge."""
+        f = np.ascontiguousarray(projection_filter, np.complex64)
+        cb = PROGRESS_FN(lambda n, u: progress(n)) if progress else PROGRESS_FN()
+        n = ctypes.c_longlong(0)
+        _check(lib().tm_run_inner_loop(self.h, _f32(f.view(np.float32)), size_i, defocus_i, cb, None,
+                                       ctypes.byref(n)))
+        return n.value
+
+    def results(self) -> Results:
+        W, H = self.W, self.H
+        f = [np.zeros((H, W + 2), np.float32) for _ in range(6)]
+        s = np.zeros((H, W + 2), np.float64)
+        q = np.zeros((H, W + 2), np.float64)
+        idx = np.zeros((H, W + 2), np.int32)
+        hist = np.zeros(512, np.int64)
+        n = ctypes.c_longlong(0)
+        _check(lib().tm_get_results(self.h, *[_f32(a) for a in f], s.ctypes.data_as(F64P), q.ctypes.data_as(F64P),
+                                    idx.ctypes.data_as(I32P), hist.ctypes.data_as(I64P), ctypes.byref(n)))
+        crop = lambda a: np.ascontiguousarray(a[:, :W])
+        return Results(*[crop(a) for a in f], crop(s), crop(q), crop(idx), hist, n.value)
+
+    def reset(self) -> None:
+        _check(lib().tm_reset(self.h))
+
+    def finalize(self, total_ccfs: int) -> dict:
+        W, H = self.W, self.H
+        outs = [np.zeros((H, W + 2), np.float32) for _ in range(4)]
+        _check(lib().tm_finalize(self.h, total_ccfs, *[_f32(a) for a in outs]))
+        return {k: np.ascontiguousarray(a[:, :W]) for k, a in zip(["mip_scaled", "zscore", "avg", "std"], outs)}
+
+    def debug_projection(self, projection_filter: np.ndarray, search_position: int, psi_index: int) -> np.ndarray:
+        f = np.ascontiguousarray(projection_filter, np.complex64)
+        out = np.zeros((self.T, self.T + 2), np.float32)
+        _check(lib().tm_debug_projection(self.h, _f32(f.view(np.float32)), search_position, psi_index, _f32(out)))
+        return np.ascontiguousarray(out[:, : self.T])
+
+    def debug_ccf(self, projection_filter: np.ndarray, search_position: int, psi_index: int) -> np.ndarray:
+        f = np.ascontiguousarray(projection_filter, np.complex64)
+        out = np.zeros((self.H, self.W + 2), np.float32)
+        _check(lib().tm_debug_ccf(self.h, _f32(f.view(np.float32)), search_position, psi_index, _f32(out)))
+        return np.ascontiguousarray(out[:, : self.W])
+
+    def set_timing(self, on: bool) -> None:
+        _check(lib().tm_set_timing(self.h, int(on)))
+
+    def timing(self) -> dict:
+        cap = 16
+        names = (ctypes.c_char_p * cap)()
+        ms = np.zeros(cap, np.float64)
+        launches = np.zeros(cap, np.int64)
+        n = lib().tm_get_timing(self.h, names, ms.ctypes.data_as(F64P), launches.ctypes.data_as(I64P), cap)
+        return {names[i].decode(): (float(ms[i]), int(launches[i])) for i in range(n)}

@@ -1,0 +1,753 @@
+// Host side of the gfx950 template-matching inner loop and its C ABI (include/tm_hip.h).
+//
+// One tm_core owns one device stream and all device buffers of one search (template
+// spectrum, transposed input spectrum, orientation matrices, batch work buffers, per-pixel
+// state). tm_run_inner_loop walks the handle's orientations in the reference order
+// (position-major, then psi; match_template.cpp:996-1001) in batches of B, so the argmax
+// tie-break (earliest orientation wins) and the FP64 summation order per pixel are those of
+// the CPU loop.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/tm_hip.h"
+#include "tm_kernels.hpp"
+
+using tmk::cf;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct TmError {
+  int code;
+  std::string msg;
+};
+
+#define TM_HIP(call)                                                                                   \
+  do {                                                                                                 \
+    hipError_t e_ = (call);                                                                            \
+    if (e_ != hipSuccess)                                                                              \
+      throw TmError{TM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)};                  \
+  } while (0)
+
+#define TM_REQUIRE(cond, code, msg)                                                                    \
+  do {                                                                                                 \
+    if (!(cond)) throw TmError{code, msg};                                                             \
+  } while (0)
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return TM_OK;
+  } catch (const TmError& e) {
+    g_err = e.msg;
+    return e.code;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return TM_ERR_INVALID;
+  }
+}
+
+bool pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+// Accurate twiddle table exp(-2 pi i k / n), k < n.
+std::vector<cf> twiddles(int n) {
+  std::vector<cf> t(n);
+  const long double two_pi = 2.0L * acosl(-1.0L);
+  for (int k = 0; k < n; ++k) {
+    long double a = two_pi * (long double)k / (long double)n;
+    t[k].x = (float)cosl(a);
+    t[k].y = (float)-sinl(a);
+  }
+  return t;
+}
+
+// Quadrant-swap phase factors of Image::SwapRealSpaceQuadrants on an n x n image that is not
+// centred (shift n/2 per axis, image.cpp:9757-9822 -> PhaseShift image.cpp:9221-9268), with
+// the reference's float phase arithmetic (functions.h:494-504).
+std::vector<cf> swap_phase(int n) {
+  const double PI = 3.14159265359;
+  const int nh = n / 2 + 1;
+  std::vector<cf> t((size_t)n * nh);
+  const float shift = (float)(n / 2);
+  for (int j = 0; j < n; ++j) {
+    const int ly = j >= n / 2 ? j - n : j;
+    const float py = (float)(shift * (float)ly * 2.0 * PI / (float)n);
+    for (int i = 0; i < nh; ++i) {
+      const float px = (float)(shift * (float)i * 2.0 * PI / (float)n);
+      const float pz = 0.0f;
+      const float ph = -px - py - pz;
+      t[(size_t)j * nh + i].x = cosf(ph);
+      t[(size_t)j * nh + i].y = sinf(ph);
+    }
+  }
+  return t;
+}
+
+// deg_2_rad and the ZYZ Euler matrix of RotationMatrix::SetToEulerRotation (matrix.cpp:490-515),
+// row-major m[3*r + c].
+void euler_matrix(float phi, float theta, float psi, float* m) {
+  const float PIf = 3.14159265359f;
+  auto d2r = [&](float d) { return (float)(d * PIf / 180.); };
+  const float cp = cosf(d2r(phi)), sp = sinf(d2r(phi));
+  const float ct = cosf(d2r(theta)), st = sinf(d2r(theta));
+  const float cs = cosf(d2r(psi)), ss = sinf(d2r(psi));
+  m[0] = cp * ct * cs - sp * ss;
+  m[3] = sp * ct * cs + cp * ss;
+  m[6] = -st * cs;
+  m[1] = -cp * ct * ss - sp * cs;
+  m[4] = -sp * ct * ss + cp * cs;
+  m[7] = st * ss;
+  m[2] = st * cp;
+  m[5] = st * sp;
+  m[8] = ct;
+}
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  void alloc(size_t count) {
+    release();
+    if (count == 0) return;
+    TM_HIP(hipMalloc((void**)&p, count * sizeof(T)));
+    n = count;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DevBuf() { release(); }
+};
+
+enum KernelId { K_EXTRACT, K_SMALLFFT, K_NORMALIZE, K_FWD_ROWS, K_MID, K_LAST, K_COUNT };
+const char* kKernelNames[K_COUNT] = {"extract", "small_fft", "normalize", "fwd_rows", "mid_columns", "last_rows_epilogue"};
+
+}  // namespace
+
+struct tm_core {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool initialised = false;
+  tm_init_params prm{};
+  int W = 0, H = 0, Wh = 0, T = 0, Th = 0, P = 0, Ph = 0, B = 16;
+  int n_pos_local = 0, n_psi = 0, n_orient = 0;
+  std::vector<float> grid, psi;      // host copies ({phi, theta} per position; psi list)
+  std::vector<int2> planes;          // (size_i, defocus_i) per plane index, in call order
+  long long n_ccfs = 0;
+  // device data
+  DevBuf<cf> tpl, inpT, phaseP, phaseT, filt, slice, projT, rows, Z;
+  DevBuf<float> mats;
+  DevBuf<float> mip;
+  DevBuf<int> idx;
+  DevBuf<double> sum, sumsq;
+  DevBuf<unsigned long long> hist;
+  std::map<int, cf*> tw;
+  // timing
+  bool timing = false;
+  hipEvent_t ev[2 * K_COUNT * 2] = {};
+  double kernel_ms[K_COUNT] = {};
+  long long kernel_launches[K_COUNT] = {};
+
+  ~tm_core() {
+    for (auto& kv : tw) (void)hipFree(kv.second);
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  const cf* twiddle(int n) {
+    auto it = tw.find(n);
+    if (it != tw.end()) return it->second;
+    auto host = twiddles(n);
+    cf* d = nullptr;
+    TM_HIP(hipMalloc((void**)&d, sizeof(cf) * n));
+    TM_HIP(hipMemcpy(d, host.data(), sizeof(cf) * n, hipMemcpyHostToDevice));
+    tw[n] = d;
+    return d;
+  }
+};
+
+namespace {
+
+void check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) throw TmError{TM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e)};
+}
+
+// ------------------------------------------------------------------ size dispatch
+template <int N>
+void launch_cols(tm_core* h, cf* data, int ncols, size_t bstride, int nb, int sign, int mode, const cf* filt,
+                 float scale) {
+  constexpr int NT = 64;
+  const cf* tw = h->twiddle(N);
+  dim3 g(ncols, nb);
+  if (mode == 1)
+    hipLaunchKernelGGL((tmk::k_cols<N, NT, -1, 1>), g, dim3(NT), 0, h->stream, data, ncols, bstride, tw, filt, scale);
+  else if (sign < 0)
+    hipLaunchKernelGGL((tmk::k_cols<N, NT, -1, 0>), g, dim3(NT), 0, h->stream, data, ncols, bstride, tw, filt, scale);
+  else
+    hipLaunchKernelGGL((tmk::k_cols<N, NT, +1, 0>), g, dim3(NT), 0, h->stream, data, ncols, bstride, tw, filt, scale);
+  check_launch("k_cols");
+}
+
+void cols(tm_core* h, int N, cf* data, int ncols, size_t bstride, int nb, int sign, int mode, const cf* filt,
+          float scale) {
+  switch (N) {
+    case 32: launch_cols<32>(h, data, ncols, bstride, nb, sign, mode, filt, scale); break;
+    case 64: launch_cols<64>(h, data, ncols, bstride, nb, sign, mode, filt, scale); break;
+    case 128: launch_cols<128>(h, data, ncols, bstride, nb, sign, mode, filt, scale); break;
+    case 256: launch_cols<256>(h, data, ncols, bstride, nb, sign, mode, filt, scale); break;
+    case 512: launch_cols<512>(h, data, ncols, bstride, nb, sign, mode, filt, scale); break;
+    case 1024: launch_cols<1024>(h, data, ncols, bstride, nb, sign, mode, filt, scale); break;
+    default: throw TmError{TM_ERR_UNSUPPORTED, "unsupported column FFT length " + std::to_string(N)};
+  }
+}
+
+template <int N>
+constexpr int row_threads() { return (N / 2) >= 1024 ? 256 : 64; }
+
+template <int N>
+void launch_c2r(tm_core* h, const cf* in, size_t ibs, int irs, float* out, size_t obs, int ors, int nrows, int nb) {
+  constexpr int NT = row_threads<N>();
+  hipLaunchKernelGGL((tmk::k_rows_c2r<N, NT>), dim3(nrows, nb), dim3(NT), 0, h->stream, in, ibs, irs, out, obs, ors,
+                     h->twiddle(N), h->twiddle(N / 2));
+  check_launch("k_rows_c2r");
+}
+
+void rows_c2r(tm_core* h, int N, const cf* in, size_t ibs, int irs, float* out, size_t obs, int ors, int nrows,
+              int nb) {
+  switch (N) {
+    case 32: launch_c2r<32>(h, in, ibs, irs, out, obs, ors, nrows, nb); break;
+    case 64: launch_c2r<64>(h, in, ibs, irs, out, obs, ors, nrows, nb); break;
+    case 128: launch_c2r<128>(h, in, ibs, irs, out, obs, ors, nrows, nb); break;
+    case 256: launch_c2r<256>(h, in, ibs, irs, out, obs, ors, nrows, nb); break;
+    case 512: launch_c2r<512>(h, in, ibs, irs, out, obs, ors, nrows, nb); break;
+    case 1024: launch_c2r<1024>(h, in, ibs, irs, out, obs, ors, nrows, nb); break;
+    default: throw TmError{TM_ERR_UNSUPPORTED, "unsupported C2R row length " + std::to_string(N)};
+  }
+}
+
+template <int N>
+void launch_r2c(tm_core* h, const float* src, size_t sbs, int srs, int r0, int s0, int len, int c0, cf* dst,
+                size_t dbs, int drs, int nrows, int nb, float scale) {
+  constexpr int NT = row_threads<N>();
+  hipLaunchKernelGGL((tmk::k_rows_r2c<N, NT>), dim3(nrows, nb), dim3(NT), 0, h->stream, src, sbs, srs, r0, s0, len, c0,
+                     dst, dbs, drs, h->twiddle(N), h->twiddle(N / 2), scale);
+  check_launch("k_rows_r2c");
+}
+
+void rows_r2c(tm_core* h, int N, const float* src, size_t sbs, int srs, int r0, int s0, int len, int c0, cf* dst,
+              size_t dbs, int drs, int nrows, int nb, float scale) {
+  switch (N) {
+    case 32: launch_r2c<32>(h, src, sbs, srs, r0, s0, len, c0, dst, dbs, drs, nrows, nb, scale); break;
+    case 64: launch_r2c<64>(h, src, sbs, srs, r0, s0, len, c0, dst, dbs, drs, nrows, nb, scale); break;
+    case 128: launch_r2c<128>(h, src, sbs, srs, r0, s0, len, c0, dst, dbs, drs, nrows, nb, scale); break;
+    case 256: launch_r2c<256>(h, src, sbs, srs, r0, s0, len, c0, dst, dbs, drs, nrows, nb, scale); break;
+    case 512: launch_r2c<512>(h, src, sbs, srs, r0, s0, len, c0, dst, dbs, drs, nrows, nb, scale); break;
+    case 1024: launch_r2c<1024>(h, src, sbs, srs, r0, s0, len, c0, dst, dbs, drs, nrows, nb, scale); break;
+    case 2048: launch_r2c<2048>(h, src, sbs, srs, r0, s0, len, c0, dst, dbs, drs, nrows, nb, scale); break;
+    case 4096: launch_r2c<4096>(h, src, sbs, srs, r0, s0, len, c0, dst, dbs, drs, nrows, nb, scale); break;
+    default: throw TmError{TM_ERR_UNSUPPORTED, "unsupported R2C row length " + std::to_string(N)};
+  }
+}
+
+template <int HH, int KC, int NTC>
+void launch_mid_t(tm_core* h, int nb) {
+  const int ntiles = (h->Wh + KC - 1) / KC;
+  const int grid = ((ntiles + 7) / 8) * 8;
+  hipLaunchKernelGGL((tmk::k_mid<HH, KC, NTC>), dim3(grid), dim3(KC * NTC), 0, h->stream, h->rows.p, h->inpT.p, h->Z.p,
+                     h->Wh, h->T, nb, ntiles, h->twiddle(HH));
+  check_launch("k_mid");
+}
+
+void launch_mid(tm_core* h, int nb) {
+  switch (h->H) {
+    case 256: launch_mid_t<256, 16, 16>(h, nb); break;
+    case 512: launch_mid_t<512, 16, 32>(h, nb); break;
+    case 1024: launch_mid_t<1024, 16, 64>(h, nb); break;
+    case 2048: launch_mid_t<2048, 8, 128>(h, nb); break;
+    case 4096: launch_mid_t<4096, 4, 256>(h, nb); break;
+    default: throw TmError{TM_ERR_UNSUPPORTED, "unsupported image height " + std::to_string(h->H)};
+  }
+}
+
+template <int WW, int NT>
+void launch_last_t(tm_core* h, int nb, const tmk::EpiParams& ep) {
+  hipLaunchKernelGGL((tmk::k_last<WW, NT>), dim3(h->H), dim3(NT), 0, h->stream, h->Z.p, h->H, nb, ep, h->twiddle(WW),
+                     h->twiddle(WW / 2));
+  check_launch("k_last");
+}
+
+void launch_last(tm_core* h, int nb, const tmk::EpiParams& ep) {
+  switch (h->W) {
+    case 256: launch_last_t<256, 128>(h, nb, ep); break;
+    case 512: launch_last_t<512, 256>(h, nb, ep); break;
+    case 1024: launch_last_t<1024, 256>(h, nb, ep); break;
+    case 2048: launch_last_t<2048, 256>(h, nb, ep); break;
+    case 4096: launch_last_t<4096, 256>(h, nb, ep); break;
+    default: throw TmError{TM_ERR_UNSUPPORTED, "unsupported image width " + std::to_string(h->W)};
+  }
+}
+
+struct Timer {
+  tm_core* h;
+  int k;
+  Timer(tm_core* hh, int kk) : h(hh), k(kk) {
+    if (h->timing) TM_HIP(hipEventRecord(h->ev[2 * k], h->stream));
+  }
+  // Errors here surface on the next checked HIP call of the same stream.
+  ~Timer() {
+    if (h->timing) {
+      float ms = 0.f;
+      if (hipEventRecord(h->ev[2 * k + 1], h->stream) == hipSuccess &&
+          hipEventSynchronize(h->ev[2 * k + 1]) == hipSuccess &&
+          hipEventElapsedTime(&ms, h->ev[2 * k], h->ev[2 * k + 1]) == hipSuccess) {
+        h->kernel_ms[k] += ms;
+        h->kernel_launches[k] += 1;
+      }
+    }
+  }
+};
+
+// ------------------------------------------------------------------ batch pipeline
+// Normalised real projections for orientations [o0, o0 + nb) into projT (T x (T+2) floats each).
+void run_projections(tm_core* h, int o0, int nb) {
+  const int P = h->P, Ph = h->Ph, T = h->T, Th = h->Th;
+  const size_t slice_bs = (size_t)P * Ph;     // complex
+  const size_t proj_bs = (size_t)T * Th;      // complex (= T * (T+2) floats)
+  {
+    Timer t(h, K_EXTRACT);
+    dim3 g((P * Ph + 255) / 256, nb);
+    const bool fold = (P == T);
+    hipLaunchKernelGGL(tmk::k_extract, g, dim3(256), 0, h->stream, h->tpl.p, P, h->mats.p, o0,
+                       fold ? h->phaseT.p : h->phaseP.p, fold ? h->filt.p : nullptr, fold ? h->projT.p : h->slice.p);
+    check_launch("k_extract");
+  }
+  {
+    Timer t(h, K_SMALLFFT);
+    if (P != T) {
+      // ExtractSlice at P, SwapRealSpaceQuadrants (folded above), BackwardFFT (C2R P^2) in place
+      cols(h, P, h->slice.p, Ph, slice_bs, nb, +1, 0, nullptr, 1.f);
+      rows_c2r(h, P, h->slice.p, slice_bs, Ph, reinterpret_cast<float*>(h->slice.p), 2 * slice_bs, P + 2, P, nb);
+      // ClipInto T^2 (centre) + ForwardFFT rows; columns fused with the 1/T^2 scale, filter and inverse
+      rows_r2c(h, T, reinterpret_cast<const float*>(h->slice.p), 2 * slice_bs, P + 2, P / 2 - T / 2, P / 2 - T / 2, T,
+               0, h->projT.p, proj_bs, Th, T, nb, 1.f);
+      const float scale = (float)(1. / (double)(float)((long long)T * T));
+      cols(h, T, h->projT.p, Th, proj_bs, nb, -1, 1, h->filt.p, scale);
+    } else {
+      cols(h, T, h->projT.p, Th, proj_bs, nb, +1, 0, nullptr, 1.f);
+    }
+    rows_c2r(h, T, h->projT.p, proj_bs, Th, reinterpret_cast<float*>(h->projT.p), 2 * proj_bs, T + 2, T, nb);
+  }
+  {
+    Timer t(h, K_NORMALIZE);
+    hipLaunchKernelGGL((tmk::k_normalize<256>), dim3(nb), dim3(256), 0, h->stream, reinterpret_cast<float*>(h->projT.p),
+                       T, 2 * proj_bs, (long long)h->W * h->H);
+    check_launch("k_normalize");
+  }
+}
+
+// Forward row FFTs of the padded reference: projection row r sits at column W/2 - T/2 of
+// row H/2 - T/2 + r (ClipIntoLargerRealSpace2D); scaled by 1/(W H) (ForwardFFT).
+void run_forward_rows(tm_core* h, int nb) {
+  Timer t(h, K_FWD_ROWS);
+  const int T = h->T;
+  const float scale = (float)(1. / (double)(float)((long long)h->W * h->H));
+  rows_r2c(h, h->W, reinterpret_cast<const float*>(h->projT.p), 2 * (size_t)T * h->Th, T + 2, 0, 0, T, h->W / 2 - T / 2,
+           h->rows.p, (size_t)T * h->Wh, h->Wh, T, nb, scale);
+}
+
+void run_batch(tm_core* h, int o0, int nb, int g0) {
+  run_projections(h, o0, nb);
+  run_forward_rows(h, nb);
+  {
+    Timer t(h, K_MID);
+    launch_mid(h, nb);
+  }
+  {
+    Timer t(h, K_LAST);
+    tmk::EpiParams ep;
+    ep.mip = h->mip.p;
+    ep.idx = h->idx.p;
+    ep.sum = h->sum.p;
+    ep.sumsq = h->sumsq.p;
+    ep.hist = h->hist.p;
+    ep.hmin = h->prm.histogram_min_scaled;
+    ep.hstep = h->prm.histogram_step_scaled;
+    ep.nbins = h->prm.histogram_number_of_bins;
+    ep.g0 = g0;
+    launch_last(h, nb, ep);
+  }
+}
+
+void upload_filter(tm_core* h, const float* projection_filter) {
+  TM_REQUIRE(projection_filter, TM_ERR_INVALID, "projection_filter is NULL");
+  const size_t n = (size_t)h->T * h->Th;
+  TM_HIP(hipMemcpyAsync(h->filt.p, projection_filter, n * sizeof(cf), hipMemcpyHostToDevice, h->stream));
+}
+
+void reset_state(tm_core* h) {
+  const size_t n = (size_t)h->W * h->H;
+  hipLaunchKernelGGL(tmk::k_reset_state, dim3((n + 255) / 256), dim3(256), 0, h->stream, h->mip.p, h->idx.p, h->sum.p,
+                     h->sumsq.p, n);
+  check_launch("k_reset_state");
+  hipLaunchKernelGGL(tmk::k_zero_u64, dim3(2), dim3(256), 0, h->stream, h->hist.p, 512);
+  check_launch("k_zero_u64");
+  h->planes.clear();
+  h->n_ccfs = 0;
+}
+
+void require_ready(tm_core* h) {
+  TM_REQUIRE(h, TM_ERR_INVALID, "handle is NULL");
+  TM_REQUIRE(h->initialised, TM_ERR_STATE, "tm_init has not succeeded on this handle");
+  TM_HIP(hipSetDevice(h->device));
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+const char* tm_last_error(void) { return g_err.c_str(); }
+
+int tm_device_count(int* count) {
+  return guarded([&] {
+    TM_REQUIRE(count, TM_ERR_INVALID, "count is NULL");
+    int n = 0;
+    TM_HIP(hipGetDeviceCount(&n));
+    *count = n;
+  });
+}
+
+int tm_create(int device_ordinal, tm_core** out) {
+  return guarded([&] {
+    TM_REQUIRE(out, TM_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    int n = 0;
+    TM_HIP(hipGetDeviceCount(&n));
+    TM_REQUIRE(device_ordinal >= 0 && device_ordinal < n, TM_ERR_INVALID,
+               "device ordinal " + std::to_string(device_ordinal) + " out of range (" + std::to_string(n) + " devices)");
+    TM_HIP(hipSetDevice(device_ordinal));
+    auto* h = new tm_core();
+    h->device = device_ordinal;
+    hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete h;
+      throw TmError{TM_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)};
+    }
+    *out = h;
+  });
+}
+
+int tm_destroy(tm_core* h) {
+  return guarded([&] {
+    if (!h) return;
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    delete h;
+  });
+}
+
+int tm_init(tm_core* h, const tm_init_params* p, const float* template_spectrum, const float* input_spectrum,
+            const float* search_positions, const float* psi_list) {
+  return guarded([&] {
+    TM_REQUIRE(h && p, TM_ERR_INVALID, "NULL handle or params");
+    TM_REQUIRE(template_spectrum && input_spectrum && search_positions && psi_list, TM_ERR_INVALID,
+               "NULL input array");
+    TM_HIP(hipSetDevice(h->device));
+    h->initialised = false;
+    const int P = p->template_size, T = p->projection_size, W = p->image_x, H = p->image_y;
+    TM_REQUIRE(pow2(W) && W >= 256 && W <= 4096 && pow2(H) && H >= 256 && H <= 4096, TM_ERR_UNSUPPORTED,
+               "image size " + std::to_string(W) + "x" + std::to_string(H) +
+                   " not supported (powers of two from 256 to 4096)");
+    TM_REQUIRE(pow2(T) && T >= 32 && T <= 1024 && pow2(P) && P >= T && P <= 1024, TM_ERR_UNSUPPORTED,
+               "template/projection sizes " + std::to_string(P) + "/" + std::to_string(T) +
+                   " not supported (powers of two, 32 <= T <= P <= 1024)");
+    TM_REQUIRE(T <= W && T <= H, TM_ERR_INVALID, "projection larger than the image");
+    TM_REQUIRE(p->n_search_positions > 0 && p->n_psi > 0, TM_ERR_INVALID, "empty search grid");
+    TM_REQUIRE(p->first_search_position >= 0 && p->last_search_position < p->n_search_positions &&
+                   p->first_search_position <= p->last_search_position,
+               TM_ERR_INVALID, "search position range out of bounds");
+    TM_REQUIRE(p->histogram_number_of_bins > 0 && p->histogram_number_of_bins <= 512, TM_ERR_INVALID,
+               "histogram_number_of_bins must be in 1..512");
+    TM_REQUIRE(p->histogram_step_scaled > 0.0, TM_ERR_INVALID, "histogram_step_scaled must be > 0");
+    h->prm = *p;
+    h->W = W; h->H = H; h->Wh = W / 2 + 1;
+    h->T = T; h->Th = T / 2 + 1;
+    h->P = P; h->Ph = P / 2 + 1;
+    h->B = p->batch_size > 0 ? p->batch_size : 16;
+    TM_REQUIRE(h->B <= 256, TM_ERR_INVALID, "batch_size must be <= 256");
+    h->n_pos_local = p->last_search_position - p->first_search_position + 1;
+    h->n_psi = p->n_psi;
+    const long long n_orient = (long long)h->n_pos_local * h->n_psi;
+    TM_REQUIRE(n_orient < (1LL << 30), TM_ERR_INVALID, "too many orientations for one handle");
+    h->n_orient = (int)n_orient;
+    h->grid.assign(search_positions, search_positions + 2 * (size_t)p->n_search_positions);
+    h->psi.assign(psi_list, psi_list + p->n_psi);
+
+    // orientation matrices in reference loop order: position-major, psi inner
+    std::vector<float> m((size_t)h->n_orient * 9);
+    for (int pl = 0; pl < h->n_pos_local; ++pl) {
+      const int pos = p->first_search_position + pl;
+      for (int k = 0; k < h->n_psi; ++k)
+        euler_matrix(h->grid[2 * pos], h->grid[2 * pos + 1], h->psi[k], &m[((size_t)pl * h->n_psi + k) * 9]);
+    }
+    h->mats.alloc(m.size());
+    TM_HIP(hipMemcpy(h->mats.p, m.data(), m.size() * sizeof(float), hipMemcpyHostToDevice));
+
+    // template spectrum
+    const size_t ntpl = (size_t)P * P * h->Ph;
+    h->tpl.alloc(ntpl);
+    TM_HIP(hipMemcpy(h->tpl.p, template_spectrum, ntpl * sizeof(cf), hipMemcpyHostToDevice));
+
+    // input spectrum, transposed to [kx][ky] so that each column is contiguous
+    std::vector<cf> it((size_t)h->Wh * H);
+    const cf* in = reinterpret_cast<const cf*>(input_spectrum);
+    for (int y = 0; y < H; ++y)
+      for (int x = 0; x < h->Wh; ++x) it[(size_t)x * H + y] = in[(size_t)y * h->Wh + x];
+    h->inpT.alloc(it.size());
+    TM_HIP(hipMemcpy(h->inpT.p, it.data(), it.size() * sizeof(cf), hipMemcpyHostToDevice));
+
+    // phase tables and filter
+    auto pp = swap_phase(P);
+    h->phaseP.alloc(pp.size());
+    TM_HIP(hipMemcpy(h->phaseP.p, pp.data(), pp.size() * sizeof(cf), hipMemcpyHostToDevice));
+    auto pt = swap_phase(T);
+    h->phaseT.alloc(pt.size());
+    TM_HIP(hipMemcpy(h->phaseT.p, pt.data(), pt.size() * sizeof(cf), hipMemcpyHostToDevice));
+    h->filt.alloc((size_t)T * h->Th);
+
+    // batch work buffers
+    const int B = h->B;
+    h->slice.alloc((size_t)B * P * h->Ph);
+    h->projT.alloc((size_t)B * T * h->Th);
+    h->rows.alloc((size_t)B * T * h->Wh);
+    h->Z.alloc((size_t)B * H * h->Wh);
+
+    // state
+    const size_t npx = (size_t)W * H;
+    h->mip.alloc(npx);
+    h->idx.alloc(npx);
+    h->sum.alloc(npx);
+    h->sumsq.alloc(npx);
+    h->hist.alloc(512);
+    for (int k : {P, P / 2, T, T / 2, W, W / 2, H}) h->twiddle(k);
+    for (auto& e : h->ev)
+      if (!e) TM_HIP(hipEventCreate(&e));
+    reset_state(h);
+    TM_HIP(hipStreamSynchronize(h->stream));
+    h->initialised = true;
+  });
+}
+
+int tm_reset(tm_core* h) {
+  return guarded([&] {
+    require_ready(h);
+    reset_state(h);
+    TM_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+int tm_run_inner_loop(tm_core* h, const float* projection_filter, int size_i, int defocus_i, tm_progress_fn progress,
+                      void* user, long long* ccfs) {
+  return guarded([&] {
+    require_ready(h);
+    upload_filter(h, projection_filter);
+    const int plane = (int)h->planes.size();
+    TM_REQUIRE((long long)(plane + 1) * h->n_orient < (1LL << 31), TM_ERR_INVALID,
+               "orientation index space exhausted (too many planes for one handle)");
+    h->planes.push_back(make_int2(size_i, defocus_i));
+    for (int k = 0; k < K_COUNT; ++k) { h->kernel_ms[k] = 0.0; h->kernel_launches[k] = 0; }
+    long long done = 0;
+    for (int o0 = 0; o0 < h->n_orient; o0 += h->B) {
+      const int nb = std::min(h->B, h->n_orient - o0);
+      run_batch(h, o0, nb, plane * h->n_orient + o0);
+      done += nb;
+      if (progress) {
+        TM_HIP(hipStreamSynchronize(h->stream));
+        progress(done, user);
+      }
+    }
+    TM_HIP(hipStreamSynchronize(h->stream));
+    h->n_ccfs += done;
+    if (ccfs) *ccfs = done;
+  });
+}
+
+int tm_get_results(tm_core* h, float* mip, float* psi, float* theta, float* phi, float* defocus, float* pixel_size,
+                   double* sum, double* sum_of_squares, int* best_index, long long* histogram, long long* n_ccfs) {
+  return guarded([&] {
+    require_ready(h);
+    const int W = h->W, H = h->H, RS = W + 2;
+    const size_t npx = (size_t)W * H;
+    std::vector<float> m(npx);
+    std::vector<int> ix(npx);
+    TM_HIP(hipMemcpyAsync(m.data(), h->mip.p, npx * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    TM_HIP(hipMemcpyAsync(ix.data(), h->idx.p, npx * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    std::vector<double> s, q;
+    if (sum) {
+      s.resize(npx);
+      TM_HIP(hipMemcpyAsync(s.data(), h->sum.p, npx * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    }
+    if (sum_of_squares) {
+      q.resize(npx);
+      TM_HIP(hipMemcpyAsync(q.data(), h->sumsq.p, npx * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    }
+    std::vector<unsigned long long> hh(512);
+    TM_HIP(hipMemcpyAsync(hh.data(), h->hist.p, 512 * sizeof(unsigned long long), hipMemcpyDeviceToHost, h->stream));
+    TM_HIP(hipStreamSynchronize(h->stream));
+    const int per_plane = h->n_orient;
+    for (int y = 0; y < H; ++y) {
+      for (int x = 0; x < RS; ++x) {
+        const size_t o = (size_t)y * RS + x;
+        if (x >= W) {
+          if (mip) mip[o] = 0.f;
+          if (psi) psi[o] = 0.f;
+          if (theta) theta[o] = 0.f;
+          if (phi) phi[o] = 0.f;
+          if (defocus) defocus[o] = 0.f;
+          if (pixel_size) pixel_size[o] = 0.f;
+          if (sum) sum[o] = 0.0;
+          if (sum_of_squares) sum_of_squares[o] = 0.0;
+          if (best_index) best_index[o] = -1;
+          continue;
+        }
+        const size_t i = (size_t)y * W + x;
+        const int g = ix[i];
+        if (mip) mip[o] = m[i];
+        if (best_index) best_index[o] = g;
+        if (sum) sum[o] = s[i];
+        if (sum_of_squares) sum_of_squares[o] = q[i];
+        float a_psi = 0.f, a_theta = 0.f, a_phi = 0.f, a_def = 0.f, a_pix = 0.f;
+        if (g >= 0) {
+          const int plane = g / per_plane;
+          const int local = g % per_plane;
+          const int pos = h->prm.first_search_position + local / h->n_psi;
+          const int k = local % h->n_psi;
+          a_psi = h->psi[k];
+          a_theta = h->grid[2 * pos + 1];
+          a_phi = h->grid[2 * pos];
+          a_def = float(h->planes[plane].y) * h->prm.defocus_step;
+          a_pix = float(h->planes[plane].x) * h->prm.pixel_size_step;
+        }
+        if (psi) psi[o] = a_psi;
+        if (theta) theta[o] = a_theta;
+        if (phi) phi[o] = a_phi;
+        if (defocus) defocus[o] = a_def;
+        if (pixel_size) pixel_size[o] = a_pix;
+      }
+    }
+    if (histogram)
+      for (int b = 0; b < h->prm.histogram_number_of_bins; ++b) histogram[b] = (long long)hh[b];
+    if (n_ccfs) *n_ccfs = h->n_ccfs;
+  });
+}
+
+int tm_finalize(tm_core* h, long long total_ccfs, float* mip_scaled, float* zscore, float* average, float* std_dev) {
+  return guarded([&] {
+    require_ready(h);
+    TM_REQUIRE(total_ccfs > 0, TM_ERR_INVALID, "total_ccfs must be > 0");
+    const int W = h->W, H = h->H, RS = W + 2;
+    const size_t npx = (size_t)W * H;
+    DevBuf<float> out;
+    out.alloc(4 * npx);
+    const double sqrt_px = sqrt((double)(W * H));
+    hipLaunchKernelGGL(tmk::k_finalize, dim3((npx + 255) / 256), dim3(256), 0, h->stream, h->mip.p, h->sum.p,
+                       h->sumsq.p, npx, (float)total_ccfs, (float)sqrt_px, out.p, out.p + npx, out.p + 2 * npx,
+                       out.p + 3 * npx);
+    check_launch("k_finalize");
+    std::vector<float> host(4 * npx);
+    TM_HIP(hipMemcpyAsync(host.data(), out.p, 4 * npx * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    TM_HIP(hipStreamSynchronize(h->stream));
+    float* dst[4] = {mip_scaled, zscore, average, std_dev};
+    for (int k = 0; k < 4; ++k) {
+      if (!dst[k]) continue;
+      for (int y = 0; y < H; ++y) {
+        std::memcpy(dst[k] + (size_t)y * RS, host.data() + k * npx + (size_t)y * W, W * sizeof(float));
+        dst[k][(size_t)y * RS + W] = 0.f;
+        dst[k][(size_t)y * RS + W + 1] = 0.f;
+      }
+    }
+  });
+}
+
+int tm_debug_projection(tm_core* h, const float* projection_filter, int search_position, int psi_index, float* out) {
+  return guarded([&] {
+    require_ready(h);
+    TM_REQUIRE(out, TM_ERR_INVALID, "out is NULL");
+    const int pl = search_position - h->prm.first_search_position;
+    TM_REQUIRE(pl >= 0 && pl < h->n_pos_local && psi_index >= 0 && psi_index < h->n_psi, TM_ERR_INVALID,
+               "orientation outside the handle's range");
+    upload_filter(h, projection_filter);
+    run_projections(h, pl * h->n_psi + psi_index, 1);
+    TM_HIP(hipMemcpyAsync(out, h->projT.p, (size_t)h->T * (h->T + 2) * sizeof(float), hipMemcpyDeviceToHost,
+                          h->stream));
+    TM_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+int tm_debug_ccf(tm_core* h, const float* projection_filter, int search_position, int psi_index, float* out) {
+  return guarded([&] {
+    require_ready(h);
+    TM_REQUIRE(out, TM_ERR_INVALID, "out is NULL");
+    const int pl = search_position - h->prm.first_search_position;
+    TM_REQUIRE(pl >= 0 && pl < h->n_pos_local && psi_index >= 0 && psi_index < h->n_psi, TM_ERR_INVALID,
+               "orientation outside the handle's range");
+    upload_filter(h, projection_filter);
+    run_projections(h, pl * h->n_psi + psi_index, 1);
+    run_forward_rows(h, 1);
+    launch_mid(h, 1);
+    // C2R of one plane with a scratch state: MIP of a single CCF is the CCF itself
+    const int W = h->W, H = h->H;
+    const size_t npx = (size_t)W * H;
+    DevBuf<float> m;
+    DevBuf<int> ix;
+    DevBuf<double> s, q;
+    DevBuf<unsigned long long> hs;
+    m.alloc(npx); ix.alloc(npx); s.alloc(npx); q.alloc(npx); hs.alloc(512);
+    hipLaunchKernelGGL(tmk::k_reset_state, dim3((npx + 255) / 256), dim3(256), 0, h->stream, m.p, ix.p, s.p, q.p, npx);
+    check_launch("k_reset_state");
+    hipLaunchKernelGGL(tmk::k_zero_u64, dim3(2), dim3(256), 0, h->stream, hs.p, 512);
+    tmk::EpiParams ep{m.p, ix.p, s.p, q.p, hs.p, h->prm.histogram_min_scaled, h->prm.histogram_step_scaled,
+                     h->prm.histogram_number_of_bins, 0};
+    launch_last(h, 1, ep);
+    std::vector<float> host(npx);
+    TM_HIP(hipMemcpyAsync(host.data(), m.p, npx * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    TM_HIP(hipStreamSynchronize(h->stream));
+    for (int y = 0; y < H; ++y) {
+      std::memcpy(out + (size_t)y * (W + 2), host.data() + (size_t)y * W, W * sizeof(float));
+      out[(size_t)y * (W + 2) + W] = 0.f;
+      out[(size_t)y * (W + 2) + W + 1] = 0.f;
+    }
+  });
+}
+
+int tm_set_timing(tm_core* h, int enabled) {
+  return guarded([&] {
+    TM_REQUIRE(h, TM_ERR_INVALID, "handle is NULL");
+    h->timing = enabled != 0;
+  });
+}
+
+int tm_get_timing(tm_core* h, const char** names, double* ms, long long* launches, int cap) {
+  if (!h) { g_err = "handle is NULL"; return TM_ERR_INVALID; }
+  int n = 0;
+  for (int k = 0; k < K_COUNT && n < cap; ++k, ++n) {
+    if (names) names[n] = kKernelNames[k];
+    if (ms) ms[n] = h->kernel_ms[k];
+    if (launches) launches[n] = h->kernel_launches[k];
+  }
+  return n;
+}
+
+}  // extern "C"

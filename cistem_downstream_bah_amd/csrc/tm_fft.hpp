@@ -1,0 +1,165 @@
+// Device-side FFT building blocks for gfx950 (CDNA4, 64-wide wavefronts).
+//
+// Stockham autosort formulation: for a length-N transform built from radices R_1..R_s, stage s
+// with current sub-transform length Ns has N/R butterflies; butterfly j reads
+// x[j + q*N/R] (q < R), multiplies by W^(k*q*N/(Ns*R)) with k = j mod Ns, does a DFT-R in
+// registers and writes y[(j/Ns)*Ns*R + k + q*Ns]. Twiddles come from a per-N table of
+// exp(-2 pi i k / N) computed on the host in long double (exact to float rounding).
+//
+// Conventions follow FFTW, which the reference uses (src/core/image.cpp:4806-4829,
+// 5085-5116): forward sign -1 unscaled, inverse sign +1 unscaled; the reference's 1/N forward
+// scaling is applied by the callers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace tmk {
+
+struct cf { float x, y; };
+
+__device__ __forceinline__ cf cmake(float a, float b) { cf r; r.x = a; r.y = b; return r; }
+__device__ __forceinline__ cf cadd(cf a, cf b) { return cmake(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ cf csub(cf a, cf b) { return cmake(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ cf cmul(cf a, cf b) { return cmake(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ __forceinline__ cf cconj(cf a) { return cmake(a.x, -a.y); }
+__device__ __forceinline__ cf cscale(cf a, float s) { return cmake(a.x * s, a.y * s); }
+// multiply by -i (forward) or +i (inverse)
+template <int SIGN>
+__device__ __forceinline__ cf cmul_mi(cf a) { return SIGN < 0 ? cmake(a.y, -a.x) : cmake(-a.y, a.x); }
+// conj(a) * b : the reference's conj(reference) * input (match_template.cpp:1083)
+__device__ __forceinline__ cf cconjmul(cf a, cf b) { return cmake(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x); }
+
+// ---------------------------------------------------------------- register DFTs
+template <int SIGN>
+__device__ __forceinline__ void dft2(cf& a, cf& b) { cf t = a; a = cadd(t, b); b = csub(t, b); }
+
+template <int SIGN>
+__device__ __forceinline__ void dft4(cf* v) {
+  cf t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+  cf t2 = cadd(v[1], v[3]), t3 = cmul_mi<SIGN>(csub(v[1], v[3]));
+  v[0] = cadd(t0, t2); v[2] = csub(t0, t2); v[1] = cadd(t1, t3); v[3] = csub(t1, t3);
+}
+
+// radix-8: 2 x 4 with internal twiddles W8^k
+template <int SIGN>
+__device__ __forceinline__ void dft8(cf* v) {
+  const float r = 0.70710678118654752440f;
+  cf a[4] = {v[0], v[2], v[4], v[6]};
+  cf b[4] = {v[1], v[3], v[5], v[7]};
+  dft4<SIGN>(a);
+  dft4<SIGN>(b);
+  // b[k] *= W8^(k), W8 = exp(SIGN * 2 pi i / 8)
+  b[1] = cmake((b[1].x - SIGN * b[1].y) * r, (b[1].y + SIGN * b[1].x) * r);  // * (1 + SIGN i)/sqrt2
+  b[2] = cmul_mi<SIGN>(b[2]);
+  b[3] = cmake((-b[3].x - SIGN * b[3].y) * r, (SIGN * b[3].x - b[3].y) * r);  // * (-1 + SIGN i)/sqrt2
+  for (int k = 0; k < 4; ++k) { v[k] = cadd(a[k], b[k]); v[k + 4] = csub(a[k], b[k]); }
+}
+
+// radix-16: 4 x 4 with internal twiddles W16^(k1*k2)
+template <int SIGN>
+__device__ __forceinline__ void dft16(cf* v) {
+  // column DFT-4 over stride-4 subsequences
+  cf t[4][4];
+#pragma unroll
+  for (int n1 = 0; n1 < 4; ++n1) {
+    cf s[4] = {v[n1], v[n1 + 4], v[n1 + 8], v[n1 + 12]};
+    dft4<SIGN>(s);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) t[n1][k] = s[k];
+  }
+  // twiddles W16^(n1*k2) with W16 = exp(SIGN 2 pi i/16)
+  const float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f, c2 = 0.70710678118654752440f;
+  // n1=1: k=1 (c1, s1), k=2 (c2, c2), k=3 (s1, c1)
+  // n1=2: k=1 (c2, c2), k=2 (0, 1), k=3 (-c2, c2)
+  // n1=3: k=1 (s1, c1), k=2 (-c2, c2), k=3 (-c1, -s1)
+  auto tw = [](cf a, float c, float s) { return cmake(a.x * c - a.y * (SIGN * s), a.x * (SIGN * s) + a.y * c); };
+  t[1][1] = tw(t[1][1], c1, s1);
+  t[1][2] = tw(t[1][2], c2, c2);
+  t[1][3] = tw(t[1][3], s1, c1);
+  t[2][1] = tw(t[2][1], c2, c2);
+  t[2][2] = cmake(-SIGN * t[2][2].y, SIGN * t[2][2].x);
+  t[2][3] = tw(t[2][3], -c2, c2);
+  t[3][1] = tw(t[3][1], s1, c1);
+  t[3][2] = tw(t[3][2], -c2, c2);
+  t[3][3] = tw(t[3][3], -c1, -s1);
+  // row DFT-4 over n1 for each k1; output index k1 + 4*k2
+#pragma unroll
+  for (int k1 = 0; k1 < 4; ++k1) {
+    cf s[4] = {t[0][k1], t[1][k1], t[2][k1], t[3][k1]};
+    dft4<SIGN>(s);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = s[k2];
+  }
+}
+
+template <int R, int SIGN>
+__device__ __forceinline__ void dftR(cf* v) {
+  if constexpr (R == 2) dft2<SIGN>(v[0], v[1]);
+  else if constexpr (R == 4) dft4<SIGN>(v);
+  else if constexpr (R == 8) dft8<SIGN>(v);
+  else if constexpr (R == 16) dft16<SIGN>(v);
+  else static_assert(R == 2 || R == 4 || R == 8 || R == 16, "unsupported radix");
+}
+
+// twiddle W^(idx) from the table (forward table); conjugated for the inverse.
+template <int SIGN>
+__device__ __forceinline__ cf twiddle(const cf* __restrict__ tw, int idx) {
+  cf w = tw[idx];
+  return SIGN < 0 ? w : cconj(w);
+}
+
+// ---------------------------------------------------------------- radix plans (powers of two)
+// Largest-radix-first plan with radices in {16, 8, 4, 2}.
+__host__ __device__ constexpr int log2i(int n) { return n <= 1 ? 0 : 1 + log2i(n / 2); }
+__host__ __device__ constexpr int plan_stages(int N) {
+  int l = log2i(N);
+  return l / 4 + ((l % 4) ? 1 : 0);
+}
+// radix of stage s: 16 for the first l/4 stages, then 2^(l%4)
+__host__ __device__ constexpr int plan_radix(int N, int s) {
+  int l = log2i(N);
+  return (s < l / 4) ? 16 : (1 << (l % 4));
+}
+__host__ __device__ constexpr int plan_ns(int N, int s) {  // sub-transform length before stage s
+  int ns = 1;
+  for (int i = 0; i < s; ++i) ns *= plan_radix(N, i);
+  return ns;
+}
+
+// One Stockham stage executed by `nthreads` threads over LDS buffers (in != out allowed to alias
+// only when the caller separates loads and stores with a barrier; here in/out are distinct).
+template <int N, int R, int NS, int SIGN>
+__device__ __forceinline__ void stage_lds(const cf* __restrict__ in, cf* __restrict__ out, const cf* __restrict__ tw,
+                                          int tid, int nthreads) {
+  constexpr int M = N / R;
+  for (int j = tid; j < M; j += nthreads) {
+    cf v[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) v[q] = in[j + q * M];
+    const int k = j % NS;
+    if constexpr (NS > 1) {
+      constexpr int step = N / (NS * R);
+#pragma unroll
+      for (int q = 1; q < R; ++q) v[q] = cmul(v[q], twiddle<SIGN>(tw, k * q * step));
+    }
+    dftR<R, SIGN>(v);
+    const int base = (j / NS) * NS * R + k;
+#pragma unroll
+    for (int q = 0; q < R; ++q) out[base + q * NS] = v[q];
+  }
+}
+
+// Full power-of-two FFT of one line held in LDS buffer a (scratch b, both N long).
+// Result ends in a if the number of stages is even, else in b; returns the result pointer.
+template <int N, int SIGN, int S = 0>
+__device__ __forceinline__ cf* fft_lds(cf* a, cf* b, const cf* __restrict__ tw, int tid, int nthreads) {
+  if constexpr (S >= plan_stages(N)) {
+    return a;
+  } else {
+    stage_lds<N, plan_radix(N, S), plan_ns(N, S), SIGN>(a, b, tw, tid, nthreads);
+    __syncthreads();
+    return fft_lds<N, SIGN, S + 1>(b, a, tw, tid, nthreads);
+  }
+}
+
+}  // namespace tmk

@@ -1,0 +1,488 @@
+// Hot-path kernels of the template-matching inner loop (gfx950).
+//
+// Per batch of B orientations (one tm_run_inner_loop call runs ceil(n_orient / B) batches):
+//   k_extract      Fourier slice of the P^3 template at R(phi,theta,psi), trilinear
+//                  (src/core/image.cpp:2450-2607, 2899-3020), times the quadrant-swap phase and,
+//                  when P == T, the projection filter (match_template.cpp:1007-1023)
+//   k_cols / k_rows_c2r / k_rows_r2c
+//                  the small 2-D transforms of the projection (C2R P^2, clip T^2, R2C T^2,
+//                  x filter, C2R T^2; match_template.cpp:1009-1030)
+//   k_normalize    edge mean, variance to the padded size, 1/sqrt(var) (match_template.cpp:1060-1068)
+//   k_rows_r2c     forward row FFT of the T non-zero rows of the W x H padded reference
+//                  (ClipIntoLargerRealSpace2D + first half of the R2C, image.cpp:8398-8464)
+//   k_mid          per column tile: forward column FFT, DC zero, conj(ref) * input, inverse
+//                  column FFT (match_template.cpp:1071-1088); writes the inter-pass plane Z
+//   k_last         per row: C2R row FFT of Z for every orientation of the batch, then the MIP /
+//                  argmax / FP64 sum / FP64 sum-of-squares / histogram update with the state of the
+//                  row held in registers across the batch (match_template.cpp:1100-1149)
+#pragma once
+
+#include <cfloat>
+#include <cstdint>
+
+#include "tm_fft.hpp"
+
+namespace tmk {
+
+// LDS index padding: one slot per 16 keeps stride-16 Stockham stores off a single bank group.
+__device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
+__host__ __device__ constexpr int lds_len(int n) { return n + n / 16; }
+
+// In-place Stockham FFT of one line in LDS (padded index), executed by NT threads (tid < NT)
+// of a block in which every thread reaches the barriers. Each stage loads all butterfly inputs
+// into registers, synchronises, then stores.
+template <int N, int SIGN, int NT, int S = 0>
+__device__ __forceinline__ void fft_inplace(cf* buf, const cf* __restrict__ tw, int tid) {
+  if constexpr (S < plan_stages(N)) {
+    constexpr int R = plan_radix(N, S);
+    constexpr int NS = plan_ns(N, S);
+    constexpr int M = N / R;
+    constexpr int CNT = (M + NT - 1) / NT;
+    cf v[CNT][R];
+#pragma unroll
+    for (int c = 0; c < CNT; ++c) {
+      const int j = tid + c * NT;
+      if (tid < NT && j < M) {
+#pragma unroll
+        for (int q = 0; q < R; ++q) v[c][q] = buf[lpad(j + q * M)];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < CNT; ++c) {
+      const int j = tid + c * NT;
+      if (tid < NT && j < M) {
+        const int k = j % NS;
+        if constexpr (NS > 1) {
+          constexpr int step = N / (NS * R);
+#pragma unroll
+          for (int q = 1; q < R; ++q) v[c][q] = cmul(v[c][q], twiddle<SIGN>(tw, k * q * step));
+        }
+        dftR<R, SIGN>(v[c]);
+        const int base = (j / NS) * NS * R + k;
+#pragma unroll
+        for (int q = 0; q < R; ++q) buf[lpad(base + q * NS)] = v[c][q];
+      }
+    }
+    __syncthreads();
+    fft_inplace<N, SIGN, NT, S + 1>(buf, tw, tid);
+  }
+}
+
+// ---------------------------------------------------------------- slice extraction
+// Image::ReturnLinearInterpolatedFourier (image.cpp:2899-3020) on a P^3 FFTW half volume.
+__device__ __forceinline__ cf interp_fourier(const cf* __restrict__ tpl, int P, float x, float y, float z) {
+  const int Ph = P / 2 + 1;
+  const int lo = -P / 2, hi_yz = P / 2 - 1, hi_x = P / 2;
+  cf sum = cmake(0.f, 0.f);
+  const bool pos = x >= 0.0f;
+  const int i0 = (int)floorf(x);
+  if (pos) { if (i0 + 1 > hi_x) return sum; }
+  else { if (i0 < lo) return sum; }
+  const int j0 = (int)floorf(y);
+  if (j0 < lo || j0 + 1 > hi_yz) return sum;
+  const int k0 = (int)floorf(z);
+  if (k0 < lo || k0 + 1 > hi_yz) return sum;
+#pragma unroll
+  for (int dk = 0; dk < 2; ++dk) {
+    const int k = k0 + dk;
+    const int pz = pos ? (k >= 0 ? k : P + k) : (k > 0 ? P - k : -k);
+    const double zw = 1.0 - (double)fabsf(z - (float)k);
+#pragma unroll
+    for (int dj = 0; dj < 2; ++dj) {
+      const int j = j0 + dj;
+      const int py = pos ? (j >= 0 ? j : P + j) : (j > 0 ? P - j : -j);
+      const double yw = 1.0 - (double)fabsf(y - (float)j);
+      const cf* row = tpl + ((size_t)pz * P + py) * Ph;
+#pragma unroll
+      for (int di = 0; di < 2; ++di) {
+        const int i = i0 + di;
+        const float w = (float)((1.0 - (double)fabsf(x - (float)i)) * yw * zw);
+        cf val = pos ? row[i] : cconj(row[-i]);
+        sum = cadd(sum, cscale(val, w));
+      }
+    }
+  }
+  return sum;
+}
+
+// One thread per output Fourier pixel (i, j) of a P x (P/2+1) slice; blockIdx.y = orientation.
+// Special cases of Image::ExtractSlice: the i = 0 column takes Hermitian mates for negative y
+// except the even-size edge row, and the DC term is 0 (image.cpp:2556-2589).
+__global__ void k_extract(const cf* __restrict__ tpl, int P, const float* __restrict__ mats, int o0,
+                          const cf* __restrict__ phase, const cf* __restrict__ filt, cf* __restrict__ out) {
+  const int Ph = P / 2 + 1;
+  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (pix >= P * Ph) return;
+  const int i = pix % Ph;
+  const int j = pix / Ph;
+  const int ly = j >= P / 2 ? j - P : j;
+  const float* m = mats + (size_t)(o0 + b) * 9;
+  cf val = cmake(0.f, 0.f);
+  if (!(i == 0 && ly == 0)) {
+    int sy = ly;
+    bool mate = false;
+    if (i == 0 && ly < 0 && ly != -P / 2) { sy = -ly; mate = true; }
+    const float x2 = (float)i, y2 = (float)sy;
+    const float x3 = m[0] * x2 + m[1] * y2;
+    const float y3 = m[3] * x2 + m[4] * y2;
+    const float z3 = m[6] * x2 + m[7] * y2;
+    val = interp_fourier(tpl, P, x3, y3, z3);
+    if (mate) val = cconj(val);
+  }
+  val = cmul(val, phase[pix]);
+  if (filt) val = cmul(val, filt[pix]);
+  out[(size_t)b * P * Ph + pix] = val;
+}
+
+// ---------------------------------------------------------------- small line transforms
+// Complex FFTs of length N down columns of [b][N][ncols] complex arrays (row stride ncols).
+// MODE 0: one transform of sign SIGN. MODE 1: forward, multiply by scale * filt[row][col], inverse
+// (ForwardFFT scale, MultiplyPixelWise, first half of BackwardFFT; match_template.cpp:1019-1030).
+template <int N, int NT, int SIGN, int MODE>
+__global__ __launch_bounds__(NT) void k_cols(cf* __restrict__ data, int ncols, size_t bstride, const cf* __restrict__ tw,
+                                             const cf* __restrict__ filt, float scale) {
+  __shared__ cf buf[lds_len(N)];
+  const int col = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
+  cf* base = data + (size_t)b * bstride + col;
+  for (int e = tid; e < N; e += NT) buf[lpad(e)] = base[(size_t)e * ncols];
+  __syncthreads();
+  if constexpr (MODE == 0) {
+    fft_inplace<N, SIGN, NT>(buf, tw, tid);
+  } else {
+    fft_inplace<N, -1, NT>(buf, tw, tid);
+    for (int e = tid; e < N; e += NT) buf[lpad(e)] = cmul(cscale(buf[lpad(e)], scale), filt[(size_t)e * ncols + col]);
+    __syncthreads();
+    fft_inplace<N, +1, NT>(buf, tw, tid);
+  }
+  for (int e = tid; e < N; e += NT) base[(size_t)e * ncols] = buf[lpad(e)];
+}
+
+// C2R of rows of length N (FFTW c2r semantics, unscaled) via an N/2 complex inverse FFT.
+// Input row: N/2+1 complex at in + b*ibs + row*irs (complex units); output N reals at
+// out + b*obs + row*ors (float units). In place is allowed (all reads precede writes).
+template <int N, int NT>
+__global__ __launch_bounds__(NT) void k_rows_c2r(const cf* in, size_t ibs, int irs, float* out, size_t obs, int ors,
+                                                 const cf* __restrict__ tw_n, const cf* __restrict__ tw_h) {
+  constexpr int h = N / 2;
+  __shared__ cf buf[lds_len(h + 1)];
+  const int row = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
+  const cf* src = in + (size_t)b * ibs + (size_t)row * irs;
+  for (int e = tid; e <= h; e += NT) buf[lpad(e)] = src[e];
+  __syncthreads();
+  // Z_k = (X_k + conj X_{h-k}) + i (X_k - conj X_{h-k}) conj(W_N^k), pairs (k, h-k)
+  for (int k = tid; k <= h / 2; k += NT) {
+    if (k == 0) {
+      const float x0 = buf[lpad(0)].x, xh = buf[lpad(h)].x;
+      buf[lpad(0)] = cmake(x0 + xh, x0 - xh);
+    } else {
+      const int kk = h - k;
+      const cf Xk = buf[lpad(k)], Xc = cconj(buf[lpad(kk)]);
+      const cf E = cadd(Xk, Xc);
+      const cf D = csub(Xk, Xc);
+      const cf w = tw_n[k];
+      const cf O = cmul(D, cconj(w));
+      const cf Ok = cmul(cconj(D), w);
+      const cf Zk = cmake(E.x - O.y, E.y + O.x);
+      const cf Zkk = cmake(E.x - Ok.y, -E.y + Ok.x);
+      buf[lpad(k)] = Zk;
+      buf[lpad(kk)] = Zkk;
+    }
+  }
+  __syncthreads();
+  fft_inplace<h, +1, NT>(buf, tw_h, tid);
+  float* dst = out + (size_t)b * obs + (size_t)row * ors;
+  for (int m = tid; m < h; m += NT) {
+    const cf z = buf[lpad(m)];
+    dst[2 * m] = z.x;
+    dst[2 * m + 1] = z.y;
+  }
+}
+
+// R2C of rows of length N (unscaled, then * scale) via an N/2 complex forward FFT.
+// The input row is zero except a window of `len` reals starting at column c0 of the N-row,
+// read from src + b*sbs + (r0 + row)*srs + s0 (float units). Output N/2+1 complex at
+// dst + b*dbs + row*drs (complex units).
+template <int N, int NT>
+__global__ __launch_bounds__(NT) void k_rows_r2c(const float* __restrict__ src, size_t sbs, int srs, int r0, int s0,
+                                                 int len, int c0, cf* __restrict__ dst, size_t dbs, int drs,
+                                                 const cf* __restrict__ tw_n, const cf* __restrict__ tw_h,
+                                                 float scale) {
+  constexpr int h = N / 2;
+  __shared__ cf buf[lds_len(h + 1)];
+  const int row = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
+  const float* s = src + (size_t)b * sbs + (size_t)(r0 + row) * srs + s0;
+  for (int m = tid; m < h; m += NT) {
+    const int x0 = 2 * m - c0, x1 = 2 * m + 1 - c0;
+    const float a = (x0 >= 0 && x0 < len) ? s[x0] : 0.f;
+    const float c = (x1 >= 0 && x1 < len) ? s[x1] : 0.f;
+    buf[lpad(m)] = cmake(a, c);
+  }
+  __syncthreads();
+  fft_inplace<h, -1, NT>(buf, tw_h, tid);
+  // X_k = E_k + W^k O_k, X_{h-k} = conj(E_k - W^k O_k); E = (Z_k + conj Z_{h-k})/2, O = (Z_k - conj Z_{h-k})/(2i)
+  cf* d = dst + (size_t)b * dbs + (size_t)row * drs;
+  for (int k = tid; k <= h / 2; k += NT) {
+    if (k == 0) {
+      const cf z0 = buf[lpad(0)];
+      d[0] = cmake((z0.x + z0.y) * scale, 0.f);
+      d[h] = cmake((z0.x - z0.y) * scale, 0.f);
+    } else {
+      const int kk = h - k;
+      const cf zk = buf[lpad(k)], zc = cconj(buf[lpad(kk)]);
+      const cf E = cscale(cadd(zk, zc), 0.5f);
+      const cf D = cscale(csub(zk, zc), 0.5f);
+      const cf O = cmake(D.y, -D.x);
+      const cf wO = cmul(tw_n[k], O);
+      const cf Xk = cadd(E, wO);
+      const cf Xkk = cconj(csub(E, wO));
+      d[k] = cscale(Xk, scale);
+      d[kk] = cscale(Xkk, scale);
+    }
+  }
+}
+
+// ---------------------------------------------------------------- projection normalisation
+// One block per orientation on a T x T real image in the padded layout (row stride T+2).
+// Follows match_template.cpp:1060-1068: subtract the edge mean (image.cpp:5907-6000), then
+// variance = <x^2> * nT/nP - (<x> * nT/nP)^2 in float from FP64 sums, then x * (1/sqrt(var)).
+template <int NT>
+__global__ __launch_bounds__(NT) void k_normalize(float* __restrict__ proj, int T, size_t bstride, long long n_pad) {
+  __shared__ double red[3][NT];
+  const int tid = threadIdx.x;
+  float* p = proj + (size_t)blockIdx.x * bstride;
+  const int rs = T + 2;
+  double e = 0.0;
+  for (int i = tid; i < 4 * T - 4; i += NT) {
+    int x, y;
+    if (i < T) { x = i; y = 0; }
+    else if (i < 2 * T) { x = i - T; y = T - 1; }
+    else { const int r = i - 2 * T; y = 1 + r / 2; x = (r & 1) ? T - 1 : 0; }
+    e += (double)p[y * rs + x];
+  }
+  red[0][tid] = e;
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (tid < s) red[0][tid] += red[0][tid + s];
+    __syncthreads();
+  }
+  const float edge_mean = (float)(red[0][0] / (double)(float)(4 * T - 4));
+  __syncthreads();
+  double s1 = 0.0, s2 = 0.0;
+  for (int i = tid; i < T * T; i += NT) {
+    const int y = i / T, x = i - y * T;
+    const float v = p[y * rs + x] + (-edge_mean);
+    p[y * rs + x] = v;
+    s1 += (double)v;
+    s2 += (double)v * (double)v;
+  }
+  red[1][tid] = s1;
+  red[2][tid] = s2;
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (tid < s) { red[1][tid] += red[1][tid + s]; red[2][tid] += red[2][tid + s]; }
+    __syncthreads();
+  }
+  const float mean_sq = (float)(red[2][0] / T / T);                 // ReturnSumOfSquares (image.cpp:294-301)
+  const float mean = (float)(red[1][0] / ((long long)T * T));       // ReturnAverageOfRealValues (image.cpp:6327-6340)
+  const float nT = (float)((long long)T * T), nP = (float)n_pad;
+  const float t1 = mean_sq * nT / nP;
+  const float t2 = mean * nT / nP;
+  const float variance = t1 - t2 * t2;
+  const float inverse = (float)(1. / (double)sqrtf(variance));       // DivideByConstant (image.cpp:5118-5127)
+  for (int i = tid; i < T * T; i += NT) {
+    const int y = i / T, x = i - y * T;
+    p[y * rs + x] = p[y * rs + x] * inverse;
+  }
+}
+
+// ---------------------------------------------------------------- column pass
+// Tile of KC adjacent columns kx of the W x H spectrum, NTC threads per column. For each
+// orientation b: place the T forward-row-FFT rows at y = H/2 - T/2 + r, forward column FFT,
+// zero the DC term (ZeroCentralPixel, image.cpp:6498-6503), multiply conj(R) * I
+// (match_template.cpp:1083), inverse column FFT, and write Z[b][y][kx].
+// Tiles are mapped so that consecutive tiles run on the same XCD (workgroups are dispatched
+// round-robin over the 8 XCDs), letting partial cache lines of Z merge in that XCD's L2.
+template <int H, int KC, int NTC>
+__global__ __launch_bounds__(KC* NTC) void k_mid(const cf* __restrict__ rows, const cf* __restrict__ inpT,
+                                                 cf* __restrict__ Z, int Wh, int T, int nb, int ntiles,
+                                                 const cf* __restrict__ tw) {
+  __shared__ cf buf[KC][lds_len(H)];
+  const int tid = threadIdx.x;
+  const int c = tid / NTC, ctid = tid % NTC;
+  const int bid = blockIdx.x;
+  const int per_xcd = (ntiles + 7) / 8;
+  const int tile = (bid % 8) * per_xcd + bid / 8;
+  if (tile >= ntiles) return;  // whole block exits together
+  const int kx0 = tile * KC;
+  const int kx = kx0 + c;
+  const bool valid = kx < Wh;
+  const int jlo = H / 2 - T / 2;
+  const size_t rows_bs = (size_t)T * Wh;
+  const size_t z_bs = (size_t)H * Wh;
+  for (int b = 0; b < nb; ++b) {
+    for (int e = ctid; e < H; e += NTC) buf[c][lpad(e)] = cmake(0.f, 0.f);
+    __syncthreads();
+    const cf* rb = rows + b * rows_bs;
+    for (int e = tid; e < KC * T; e += KC * NTC) {
+      const int cc = e % KC, r = e / KC;
+      const int kxx = kx0 + cc;
+      if (kxx < Wh) buf[cc][lpad(jlo + r)] = rb[(size_t)r * Wh + kxx];
+    }
+    __syncthreads();
+    fft_inplace<H, -1, NTC>(buf[c], tw, ctid);
+    if (valid) {
+      const cf* ic = inpT + (size_t)kx * H;
+      for (int e = ctid; e < H; e += NTC) {
+        cf v = buf[c][lpad(e)];
+        if (kx == 0 && e == 0) v = cmake(0.f, 0.f);
+        buf[c][lpad(e)] = cconjmul(v, ic[e]);
+      }
+    }
+    __syncthreads();
+    fft_inplace<H, +1, NTC>(buf[c], tw, ctid);
+    cf* zb = Z + b * z_bs;
+    for (int e = tid; e < KC * H; e += KC * NTC) {
+      const int cc = e % KC, y = e / KC;
+      const int kxx = kx0 + cc;
+      if (kxx < Wh) zb[(size_t)y * Wh + kxx] = buf[cc][lpad(y)];
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- row pass + epilogue
+struct EpiParams {
+  float* mip;
+  int* idx;
+  double* sum;
+  double* sumsq;
+  unsigned long long* hist;
+  double hmin, hstep;
+  int nbins;
+  int g0;   // global orientation index of batch entry 0
+};
+
+// One block per image row y. For each orientation b of the batch: C2R of Z[b][y][*] into W reals,
+// then, per pixel, in orientation order: strict-greater MIP/argmax update, histogram bin
+// int((v - min) / step) over 0..nbins-1, FP64 sum += v and sum-of-squares += (v*v in float).
+template <int W, int NT>
+__global__ __launch_bounds__(NT) void k_last(const cf* __restrict__ Z, int H, int nb, EpiParams ep,
+                                             const cf* __restrict__ tw_n, const cf* __restrict__ tw_h) {
+  constexpr int h = W / 2;
+  constexpr int Wh = W / 2 + 1;
+  constexpr int PAIRS = h / NT;
+  static_assert(h % NT == 0, "row length must be a multiple of 2*NT");
+  __shared__ cf buf[lds_len(h + 1)];
+  __shared__ unsigned int hist[512];
+  const int y = blockIdx.x, tid = threadIdx.x;
+  for (int i = tid; i < 512; i += NT) hist[i] = 0u;
+  float mip[2 * PAIRS];
+  int idx[2 * PAIRS];
+  double s1[2 * PAIRS], s2[2 * PAIRS];
+  const size_t rowoff = (size_t)y * W;
+#pragma unroll
+  for (int u = 0; u < PAIRS; ++u) {
+    const int m = tid + u * NT;
+    const float2 a = *reinterpret_cast<const float2*>(ep.mip + rowoff + 2 * m);
+    const int2 ia = *reinterpret_cast<const int2*>(ep.idx + rowoff + 2 * m);
+    const double2 sa = *reinterpret_cast<const double2*>(ep.sum + rowoff + 2 * m);
+    const double2 qa = *reinterpret_cast<const double2*>(ep.sumsq + rowoff + 2 * m);
+    mip[2 * u] = a.x; mip[2 * u + 1] = a.y;
+    idx[2 * u] = ia.x; idx[2 * u + 1] = ia.y;
+    s1[2 * u] = sa.x; s1[2 * u + 1] = sa.y;
+    s2[2 * u] = qa.x; s2[2 * u + 1] = qa.y;
+  }
+  __syncthreads();
+  for (int b = 0; b < nb; ++b) {
+    const cf* src = Z + ((size_t)b * H + y) * Wh;
+    for (int e = tid; e <= h; e += NT) buf[lpad(e)] = src[e];
+    __syncthreads();
+    for (int k = tid; k <= h / 2; k += NT) {
+      if (k == 0) {
+        const float x0 = buf[lpad(0)].x, xh = buf[lpad(h)].x;
+        buf[lpad(0)] = cmake(x0 + xh, x0 - xh);
+      } else {
+        const int kk = h - k;
+        const cf Xk = buf[lpad(k)], Xc = cconj(buf[lpad(kk)]);
+        const cf E = cadd(Xk, Xc);
+        const cf D = csub(Xk, Xc);
+        const cf w = tw_n[k];
+        const cf O = cmul(D, cconj(w));
+        const cf Ok = cmul(cconj(D), w);
+        buf[lpad(k)] = cmake(E.x - O.y, E.y + O.x);
+        buf[lpad(kk)] = cmake(E.x - Ok.y, -E.y + Ok.x);
+      }
+    }
+    __syncthreads();
+    fft_inplace<h, +1, NT>(buf, tw_h, tid);
+    const int g = ep.g0 + b;
+#pragma unroll
+    for (int u = 0; u < PAIRS; ++u) {
+      const cf z = buf[lpad(tid + u * NT)];
+      const float vv[2] = {z.x, z.y};
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const float v = vv[t];
+        const int p = 2 * u + t;
+        if (v > mip[p]) { mip[p] = v; idx[p] = g; }
+        const double d = ((double)v - ep.hmin) / ep.hstep;
+        if (d > -1.0 && d < (double)ep.nbins) atomicAdd(&hist[(int)d], 1u);
+        s1[p] += (double)v;
+        const float sq = v * v;
+        s2[p] += (double)sq;
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int u = 0; u < PAIRS; ++u) {
+    const int m = tid + u * NT;
+    *reinterpret_cast<float2*>(ep.mip + rowoff + 2 * m) = make_float2(mip[2 * u], mip[2 * u + 1]);
+    *reinterpret_cast<int2*>(ep.idx + rowoff + 2 * m) = make_int2(idx[2 * u], idx[2 * u + 1]);
+    *reinterpret_cast<double2*>(ep.sum + rowoff + 2 * m) = make_double2(s1[2 * u], s1[2 * u + 1]);
+    *reinterpret_cast<double2*>(ep.sumsq + rowoff + 2 * m) = make_double2(s2[2 * u], s2[2 * u + 1]);
+  }
+  for (int i = tid; i < ep.nbins; i += NT)
+    if (hist[i]) atomicAdd(&ep.hist[i], (unsigned long long)hist[i]);
+}
+
+// ---------------------------------------------------------------- state and finalisation
+__global__ void k_reset_state(float* mip, int* idx, double* sum, double* sumsq, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  mip[i] = -FLT_MAX;
+  idx[i] = -1;
+  sum[i] = 0.0;
+  sumsq[i] = 0.0;
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+// match_template.cpp:1221-1243, 1300-1310 with the reference's float/double mix.
+__global__ void k_finalize(const float* __restrict__ mip, const double* __restrict__ sum,
+                           const double* __restrict__ sumsq, size_t n, float n_ccf, float sqrt_px,
+                           float* __restrict__ mip_scaled, float* __restrict__ z, float* __restrict__ avg,
+                           float* __restrict__ sd) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = sum[i] / n_ccf;
+  double q = sumsq[i] / n_ccf - (double)powf((float)s, 2);
+  if (q > 0.0f) q = (double)(sqrtf((float)q) * sqrt_px);
+  else q = 0.0f;
+  s *= sqrt_px;
+  const float m = mip[i] * sqrt_px;
+  float zz = (float)((double)m - s);
+  if (q > 0.0f) zz = (float)((double)zz / q);
+  else zz = 0.0f;
+  mip_scaled[i] = m;
+  z[i] = zz;
+  avg[i] = (float)s;
+  sd[i] = (float)q;
+}
+
+}  // namespace tmk

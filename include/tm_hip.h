@@ -1,0 +1,119 @@
+/*
+ * C ABI of the MI355X (gfx950) template-matching inner loop.
+ *
+ * It replaces the reference's GPU class `TemplateMatchingCore`
+ * (src/gpu/TemplateMatchingCore.h:55-179) and the pieces of `DeviceManager`
+ * (src/gpu/DeviceManager.h:1-28) it uses, behind plain pointers and sizes.
+ *
+ * Layouts (the reference's `Image` memory, src/core/image.cpp:4761-4846):
+ *   - Fourier arrays are FFTW half-complex, interleaved (re, im) float pairs:
+ *       2-D  (W/2+1) * H complex, x fastest;   3-D  (P/2+1) * P * P complex, x fastest.
+ *   - Real maps use the padded layout: rows of W + 2 floats (W even), logical pixels first.
+ * Every call returns TM_OK (0) or a negative TM_ERR_* code; tm_last_error() describes the last
+ * failure on the calling thread. Nothing in the library exits the process. One handle per
+ * host thread; handles share no state.
+ */
+#ifndef TM_HIP_H
+#define TM_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TM_OK 0
+#define TM_ERR_INVALID (-1)
+#define TM_ERR_HIP (-2)
+#define TM_ERR_UNSUPPORTED (-3)
+#define TM_ERR_STATE (-4)
+
+typedef struct tm_core tm_core;
+
+/* Progress callback: number of cross-correlation maps (CCFs) finished so far in this call.
+ * Replaces the per-CCF ProgressBar / JobResult updates (TemplateMatchingCore.cu:324-339). */
+typedef void (*tm_progress_fn)(long long ccfs_done, void* user);
+
+typedef struct tm_init_params {
+  int template_size;          /* P: edge of the padded 3-D template (cube)                         */
+  int projection_size;        /* T: edge of current_projection (defines the projection size)        */
+  int image_x, image_y;       /* W, H: size of the (factorised, possibly rotated) input image        */
+  int n_search_positions;     /* number of {phi, theta} rows in search_positions                      */
+  int n_psi;                  /* number of in-plane angles in psi_list                               */
+  int first_search_position;  /* inclusive range of search positions handled by this handle          */
+  int last_search_position;
+  double histogram_min_scaled;   /* match_template.cpp:633-644                                       */
+  double histogram_step_scaled;
+  int histogram_number_of_bins;  /* <= 512 */
+  int max_padding;               /* kept for interface parity; the CPU path it mirrors does not trim */
+  float defocus_step;            /* best_defocus map stores defocus_i * defocus_step                 */
+  float pixel_size_step;         /* best_pixel_size map stores size_i * pixel_size_step              */
+  int batch_size;                /* orientations per launch; 0 selects the default                   */
+} tm_init_params;
+
+/* Number of visible HIP devices. Replaces DeviceManager::Init's device query
+ * (src/gpu/DeviceManager.cu:27-124). */
+int tm_device_count(int* count);
+
+/* Creates a handle bound to one device. Replaces `new TemplateMatchingCore` plus
+ * DeviceManager::SetGpu (match_template.cpp:821-822, DeviceManager.cu:126-140). */
+int tm_create(int device_ordinal, tm_core** out);
+
+/* Frees every device buffer of the handle. Replaces TemplateMatchingCore::~TemplateMatchingCore
+ * (TemplateMatchingCore.cu:27-38, which frees nothing). */
+int tm_destroy(tm_core* h);
+
+/* Copies the search inputs to the device and allocates the working set.
+ * Replaces TemplateMatchingCore::Init (TemplateMatchingCore.h:147-171, .cu:49-138):
+ *   template_spectrum  <- template_reconstruction (Fourier, quadrant-swapped, DC 0)
+ *   input_spectrum     <- input_image (whitened, normalised, quadrant-swapped spectrum)
+ *   search_positions   <- global_euler_search.list_of_search_parameters ({phi, theta} per row)
+ *   psi_list           <- the in-plane angles of the inclusive float loop psi_start..psi_max
+ * The accumulated maps are reset. */
+int tm_init(tm_core* h, const tm_init_params* params, const float* template_spectrum, const float* input_spectrum,
+            const float* search_positions, const float* psi_list);
+
+/* Searches all orientations of the handle's position range for one (pixel size, defocus) plane
+ * and folds the result into the accumulated maps (max with the earliest orientation winning
+ * ties, FP64 sums, histogram). Replaces TemplateMatchingCore::RunInnerLoop
+ * (TemplateMatchingCore.h:173, .cu:140-368) together with the host merge that follows it
+ * (match_template.cpp:918-985). projection_filter is T x (T/2+1) complex (CTF x whitening).
+ * ccfs (may be NULL) receives the number of CCFs computed by this call. */
+int tm_run_inner_loop(tm_core* h, const float* projection_filter, int size_i, int defocus_i, tm_progress_fn progress,
+                      void* user, long long* ccfs);
+
+/* Copies the accumulated maps to caller arrays in the padded layout ((W+2) * H; padding columns
+ * are written as 0). Any pointer may be NULL. Replaces the reads of d_max_intensity_projection,
+ * d_best_psi/theta/phi, d_sum3, d_sumSq3 and Histogram::CopyToHostAndAdd
+ * (match_template.cpp:922-984, Histogram.cu:188-203). best_index receives the global orientation
+ * index ((plane * n_local_positions + local_position) * n_psi + psi_index), -1 where unset. */
+int tm_get_results(tm_core* h, float* mip, float* psi, float* theta, float* phi, float* defocus, float* pixel_size,
+                   double* sum, double* sum_of_squares, int* best_index, long long* histogram, long long* n_ccfs);
+
+/* Clears the accumulated maps (MIP to -FLT_MAX, sums and histogram to 0). */
+int tm_reset(tm_core* h);
+
+/* Final scaling and z-score on the device (match_template.cpp:1221-1243, 1300-1310):
+ * mean = sum/N, sd = sqrt(sumsq/N - mean^2) * sqrt(W*H), mean *= sqrt(W*H),
+ * mip_scaled = MIP * sqrt(W*H), z = (mip_scaled - mean) / sd (0 where sd <= 0).
+ * Outputs are padded-layout float maps; any pointer may be NULL. */
+int tm_finalize(tm_core* h, long long total_ccfs, float* mip_scaled, float* zscore, float* average, float* std_dev);
+
+/* Test hooks with no counterpart in the reference: the normalised real-space projection
+ * (T x (T+2) floats) and one cross-correlation plane ((W+2) x H floats) for a single orientation,
+ * computed by the same kernels as tm_run_inner_loop (match_template.cpp:1004-1088). */
+int tm_debug_projection(tm_core* h, const float* projection_filter, int search_position, int psi_index, float* out);
+int tm_debug_ccf(tm_core* h, const float* projection_filter, int search_position, int psi_index, float* out);
+
+/* Average duration (ms) of each hot kernel over the last tm_run_inner_loop call, measured with
+ * HIP events on the handle's stream when timing is enabled. names/ms hold up to cap entries;
+ * returns the number of entries written. */
+int tm_set_timing(tm_core* h, int enabled);
+int tm_get_timing(tm_core* h, const char** names, double* ms, long long* launches, int cap);
+
+/* Last error message of the calling thread. */
+const char* tm_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TM_HIP_H */
