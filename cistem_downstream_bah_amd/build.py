@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtm_hip.so")
-SOURCES = ["tm_core.hip"]
+SOURCES = ["tm_core.hip", "tm_prepare.hip"]
 HEADERS = ["tm_fft.hpp", "tm_kernels.hpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -109,7 +109,7 @@ __device__ __forceinline__ cf interp_fourier(const cf* __restrict__ tpl, int P, 
 // One thread per output Fourier pixel (i, j) of a P x (P/2+1) slice; blockIdx.y = orientation.
 // Special cases of Image::ExtractSlice: the i = 0 column takes Hermitian mates for negative y
 // except the even-size edge row, and the DC term is 0 (image.cpp:2556-2589).
-__global__ void k_extract(const cf* __restrict__ tpl, int P, const float* __restrict__ mats, int o0,
+static __global__ void k_extract(const cf* __restrict__ tpl, int P, const float* __restrict__ mats, int o0,
                           const cf* __restrict__ phase, const cf* __restrict__ filt, cf* __restrict__ out) {
   const int Ph = P / 2 + 1;
   const int pix = blockIdx.x * blockDim.x + threadIdx.x;
@@ -449,7 +449,7 @@ __global__ __launch_bounds__(NT) void k_last(const cf* __restrict__ Z, int H, in
 }
 
 // ---------------------------------------------------------------- state and finalisation
-__global__ void k_reset_state(float* mip, int* idx, double* sum, double* sumsq, size_t n) {
+static __global__ void k_reset_state(float* mip, int* idx, double* sum, double* sumsq, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   mip[i] = -FLT_MAX;
@@ -458,13 +458,13 @@ __global__ void k_reset_state(float* mip, int* idx, double* sum, double* sumsq, 
   sumsq[i] = 0.0;
 }
 
-__global__ void k_zero_u64(unsigned long long* p, int n) {
+static __global__ void k_zero_u64(unsigned long long* p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0ull;
 }
 
 // match_template.cpp:1221-1243, 1300-1310 with the reference's float/double mix.
-__global__ void k_finalize(const float* __restrict__ mip, const double* __restrict__ sum,
+static __global__ void k_finalize(const float* __restrict__ mip, const double* __restrict__ sum,
                            const double* __restrict__ sumsq, size_t n, float n_ccf, float sqrt_px,
                            float* __restrict__ mip_scaled, float* __restrict__ z, float* __restrict__ avg,
                            float* __restrict__ sd) {

@@ -6,7 +6,8 @@ Bars (match_template.cpp:996-1169 is the reference semantics):
   * best orientation index: identical wherever the oracle's best value beats its runner-up by more
     than the FP32 noise of the two FFT implementations; identical at every detected peak
   * FP64 sum / sum-of-squares: relative 1e-5 (values differ only by FFT rounding)
-  * histogram: totals identical; per-bin counts differ only by values that sit on a bin edge
+  * histogram: totals identical; cumulative counts agree to 1e-5 of the total (values on a bin
+    edge may fall on either side)
 """
 import os
 
@@ -86,21 +87,51 @@ def test_projection_padding2(c0_pad2, pos, psi):
     assert e <= 1e-4 and ec <= 1e-4
 
 
+def _near_tie_check(mt, ref, got, max_orientations=400):
+    """Where the best index differs, the oracle's own CCF at the GPU's orientation must equal the
+    oracle's MIP within FP32 noise (1e-5 relative): the two orientations tie to within the rounding
+    of two FFT implementations, so either choice is a correct argmax."""
+    I = mt.info
+    bad = np.argwhere(got.best_index != ref["best_index"])
+    if bad.size == 0:
+        return 0
+    g = got.best_index[bad[:, 0], bad[:, 1]]
+    uniq = np.unique(g)[:max_orientations]
+    checked = 0
+    worst = 0.0
+    for gi in uniq:
+        local = int(gi) % (((I.last_pos - I.first_pos) + 1) * I.n_psi)
+        pos = I.first_pos + local // I.n_psi
+        k = local % I.n_psi
+        _, ccf = mt.projection_and_ccf(pos, k, 0)
+        sel = bad[g == gi]
+        v = ccf[sel[:, 0], sel[:, 1]]
+        m = ref["mip"][sel[:, 0], sel[:, 1]]
+        worst = max(worst, float(np.max(np.abs(v - m) / np.abs(m))))
+        checked += len(sel)
+    print(f"near-tie check: {checked} differing pixels over {len(uniq)} orientations, worst gap {worst:.2e}", flush=True)
+    assert worst <= 1e-5
+    return checked
+
+
 def _compare_search(mt, ref, got, label):
     mip_r, mip_g = ref["mip"], got.mip
     rel = np.abs(mip_g - mip_r) / np.maximum(np.abs(mip_r), 1e-30)
     same = got.best_index == ref["best_index"]
     print(f"{label}: MIP max rel {rel.max():.2e}; best index identical at {same.mean() * 100:.4f}% of pixels", flush=True)
     assert rel.max() <= 1e-4
-    assert same.mean() >= 0.999
+    _near_tie_check(mt, ref, got)
     srel = np.abs(got.sum - ref["sum"]).max() / np.abs(ref["sum"]).max()
     qrel = np.abs(got.sumsq - ref["sumsq"]).max() / np.abs(ref["sumsq"]).max()
     print(f"{label}: sum rel {srel:.2e} sumsq rel {qrel:.2e}", flush=True)
     assert srel <= 1e-5 and qrel <= 1e-5
     assert got.hist.sum() == ref["hist"].sum()
-    hd = np.abs(got.hist - ref["hist"]).sum()
-    print(f"{label}: histogram total {got.hist.sum()} abs bin diff {hd}", flush=True)
-    assert hd <= 1e-6 * got.hist.sum() + 10
+    # values within FP32 rounding of a bin edge may land in the neighbouring bin; each such value
+    # moves the cumulative count at exactly one edge, so the cumulative histograms must agree to
+    # within the expected number of edge values (<< 1e-5 of the total)
+    cd = np.abs(np.cumsum(got.hist) - np.cumsum(ref["hist"])).max()
+    print(f"{label}: histogram total {got.hist.sum()} max cumulative diff {cd}", flush=True)
+    assert cd <= 1e-5 * got.hist.sum()
     # angle maps follow from the index
     m = same
     assert np.array_equal(got.psi[m], ref["psi"][m]) and np.array_equal(got.theta[m], ref["theta"][m])
@@ -123,3 +154,46 @@ def test_search_subset_matches_oracle(c0):
     gs.close()
     assert n == (hi - lo + 1) * I.n_psi == ref["nccf"] == got.nccf
     _compare_search(mt_sub, ref, got, "subset")
+
+
+def _peaks(z, thr, radius=10):
+    """Local maxima of z above thr, greedy with an exclusion radius (strongest first)."""
+    zz = z.copy()
+    out = []
+    while True:
+        iy, ix = np.unravel_index(np.argmax(zz), zz.shape)
+        if zz[iy, ix] < thr:
+            break
+        out.append((int(iy), int(ix)))
+        zz[max(0, iy - radius):iy + radius + 1, max(0, ix - radius):ix + radius + 1] = -np.inf
+    return out
+
+
+def test_full_config0_matches_oracle_and_finds_particles(c0):
+    """Config 0 end to end (15,466 CCFs): maps match the oracle, peaks are identical in position and
+    best-angle index, and every inserted particle is found."""
+    s, mt, g = c0
+    I = mt.info
+    ref = mt.run(threads=THREADS)
+    g.reset()
+    n = g.run_inner_loop(mt.projection_filter(0), 0, 0)
+    got = g.results()
+    assert n == ref["nccf"] == I.n_pos * I.n_psi == 15466
+    _compare_search(mt, ref, got, "config0")
+    fo = mt.finalise()
+    fg = g.finalize(n)
+    zr, zg = fo["zscore"], fg["zscore"]
+    print(f"z max oracle {zr.max():.4f} gpu {zg.max():.4f}; threshold {fo['expected_threshold']:.3f}", flush=True)
+    assert np.abs(zg - zr).max() <= 1e-3
+    thr = min(fo["expected_threshold"], 6.0)
+    pr, pg = _peaks(zr, thr), _peaks(zg, thr)
+    print(f"peaks oracle {len(pr)} gpu {len(pg)}", flush=True)
+    assert pr == pg
+    for (iy, ix) in pr:
+        assert got.best_index[iy, ix] == ref["best_index"][iy, ix]
+    found = 0
+    for pt in s.particles:
+        if any(abs(ix - pt.x) <= 2 and abs(iy - pt.y) <= 2 for (iy, ix) in pg):
+            found += 1
+    print(f"inserted particles found: {found}/{len(s.particles)}", flush=True)
+    assert found == len(s.particles)
