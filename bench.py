@@ -1,0 +1,223 @@
+"""Benchmark of the template-matching inner loop on one MI355X.
+
+Workload (the real single-GPU shape): a 4096 x 4096 micrograph with 300 inserted copies at SNR 0.05,
+a 256^3 template of 3000 Gaussian atoms padded 2x in Fourier space (512^3), a C1 grid at 2.5 deg
+out-of-plane / 1.5 deg in-plane (6,620 positions x 241 psi) taken in reference order from the
+first position, and 11 defocus planes (+-1000 A in 200 A steps). One step = one
+tm_run_inner_loop call (all orientations of the handle's position range for one defocus plane);
+steps cycle through the defocus planes. Inputs are prepared on the GPU and resident in HBM before
+the timed region starts. Metric: cross-correlation maps (CCFs) per second.
+
+Prints one JSON line (rank 0, N = 1). `--gpus N` with N > 1 exits: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def kernel_bytes(name: str, nb: int, W: int, H: int, T: int, P: int, s: int = 8) -> float:
+    """Algorithmic HBM bytes of one launch of a hot kernel processing nb orientations."""
+    Wh = W // 2 + 1
+    Nc = Wh * H
+    if name == "mid_columns":       # read T forward rows per column, read input spectrum once, write Z
+        return nb * (T * Wh + Nc) * s + Nc * s
+    if name == "last_rows_epilogue":  # read Z, read+write MIP f32, index i32, sum f64, sumsq f64
+        return nb * Nc * s + 48.0 * W * H
+    if name == "fwd_rows":          # read the T x T projection, write T rows of W/2+1
+        return nb * (T * (T + 2) * 4 + T * Wh * s)
+    return float("nan")
+
+
+def pipeline_bytes_per_ccf(W: int, H: int, B: int, s: int = 8) -> float:
+    """Survey formula: 5 s Nc + 48 Npix / B bytes per CCF for the whole inner loop."""
+    Nc = (W // 2 + 1) * H
+    return 5 * s * Nc + 48.0 * W * H / B
+
+
+def read_traffic(workload: str, kernel: str, batch: int):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled by a copy
+    kernel of known size), from profiles/pmc_<kernel>_<workload>_b<batch>.txt if committed."""
+    path = os.path.join(ROOT, "profiles", f"pmc_{kernel}_{workload}_b{batch}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch="):
+            return float(line.split("=", 1)[1])
+    return None
+
+
+def cpu_baseline(mic, vol, params, threads: int, psi_per_pos: int):
+    """The CPU oracle (the reference's CPU match_template path, one single-threaded worker per
+    core) on positions [0, threads) x psi [0, psi_per_pos) at the same 4096^2 / 512^3 shape."""
+    from oracle import oracle as O
+    p = O.MTParams(pixel_size=1.0, voltage_kV=300.0, spherical_aberration_mm=2.7, amplitude_contrast=0.07,
+                   defocus1=15000.0, defocus2=15000.0, angular_step=params["angular_step"],
+                   in_plane_angular_step=params["psi_step"], padding=2.0,
+                   first_search_position=0, last_search_position=threads - 1)
+    t0 = time.time()
+    mt = O.MatchTemplateOracle(p, mic, vol)
+    setup = time.time() - t0
+    t0 = time.time()
+    r = mt.run(threads=threads, pos_lo=0, pos_hi=threads - 1, psi_lo=0, psi_hi=psi_per_pos - 1)
+    dt = time.time() - t0
+    return r["nccf"] / dt, r["nccf"], dt, setup
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=11)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--positions", type=int, default=8, help="search positions per step (x 241 psi)")
+    ap.add_argument("--batch", type=int, default=16, help="orientations per launch")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-psi", type=int, default=2)
+    ap.add_argument("--small", action="store_true", help="512^2 / 64^3 workload (config 0 shapes)")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="only run a plain device-to-device copy of this many MiB (3 times) and exit; "
+                         "its FETCH_SIZE/WRITE_SIZE counters calibrate the HBM traffic figures")
+    args = ap.parse_args()
+
+    if args.copy_calibration_mib:
+        import torch
+        n = args.copy_calibration_mib * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": 4 * n, "copy_calibration_bytes_written": 4 * n}))
+        return
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs the search on one GPU; multi-GPU runs are not supported "
+              "(the orientation range would be sharded across GPUs with one final RCCL max/sum "
+              "reduction of the per-pixel state).", file=sys.stderr)
+        sys.exit(2)
+
+    from cistem_downstream_bah_amd import core, synth
+    from cistem_downstream_bah_amd import match_template as MT
+
+    if args.small:
+        workload = "config0_512sq_T64"
+        s = synth.config0(0)
+        params = dict(angular_step=10.0, psi_step=10.0, padding=1.0, def_range=0.0, def_step=0.0)
+    else:
+        workload = "config1_4096sq_T256_P512"
+        t0 = time.time()
+        s = synth.config1(1000)
+        log(f"synthetic config-1 inputs in {time.time() - t0:.1f}s")
+        params = dict(angular_step=2.5, psi_step=1.5, padding=2.0, def_range=1000.0, def_step=200.0)
+
+    sp = MT.SearchParams(angular_step=params["angular_step"], in_plane_angular_step=params["psi_step"],
+                         padding=params["padding"], defocus_search_range=params["def_range"],
+                         defocus_step=params["def_step"], first_search_position=0,
+                         last_search_position=args.positions - 1, batch_size=args.batch)
+    t0 = time.time()
+    search = MT.MatchTemplateSearch(s.micrograph, s.volume, sp)
+    log(f"prepared on GPU in {time.time() - t0:.1f}s: W={search.W} H={search.H} T={search.T} "
+        f"P={search.template_spectrum.shape[0]} positions={search.grid.shape[0]} psi={search.psi.size}")
+    g = search.core
+    planes = list(range(-search.n_def_half, search.n_def_half + 1))
+    filters = {d: search.filter_for(d) for d in planes}
+
+    for w in range(args.warmup):
+        g.run_inner_loop(filters[planes[w % len(planes)]], 0, planes[w % len(planes)])
+    g.reset()
+    g.set_timing(True)
+    kms = {}
+    klaunch = {}
+    ccfs = 0
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        d = planes[k % len(planes)]
+        ccfs += g.run_inner_loop(filters[d], 0, d)
+        for name, (ms, nl) in g.timing().items():
+            kms[name] = kms.get(name, 0.0) + ms
+            klaunch[name] = klaunch.get(name, 0) + nl
+    elapsed = time.perf_counter() - t0  # run_inner_loop synchronises its stream before returning
+    g.set_timing(False)
+    value = ccfs / elapsed
+    ms_per_step = 1e3 * elapsed / args.steps
+    log(f"{ccfs} CCFs in {elapsed:.3f}s -> {value:.1f} CCF/s")
+
+    W, H, T = search.W, search.H, search.T
+    P = search.template_spectrum.shape[0]
+    B = args.batch
+    per_kernel = {}
+    for name in kms:
+        avg = kms[name] / max(klaunch[name], 1)
+        per_kernel[name] = {"avg_ms": round(avg, 4), "launches": klaunch[name],
+                            "share": round(kms[name] / max(sum(kms.values()), 1e-12), 4)}
+    dominant = max(kms, key=lambda n: kms[n])
+    nb_avg = ccfs / max(klaunch.get(dominant, 1), 1)
+    avg_s = kms[dominant] / klaunch[dominant] / 1e3
+    algo = kernel_bytes(dominant, int(round(nb_avg)), W, H, T, P)
+    achieved = algo / avg_s
+    for name in per_kernel:
+        kb = kernel_bytes(name, int(round(nb_avg)), W, H, T, P)
+        if not math.isnan(kb):
+            per_kernel[name]["gbps"] = round(kb / (per_kernel[name]["avg_ms"] / 1e3) / 1e9, 1)
+    traffic = read_traffic(workload, dominant, B)
+    roofline = {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic, "kernel": dominant,
+                "algorithmic_bytes_per_launch": algo, "orientations_per_launch": round(nb_avg, 2)}
+    pipe = pipeline_bytes_per_ccf(W, H, B)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        log(f"CPU baseline: oracle on {threads} threads x {args.cpu_psi} psi ...")
+        v, n, dt, setup = cpu_baseline(s.micrograph, s.volume, params, threads, args.cpu_psi)
+        cpu = {"value": round(v, 3), "unit": "CCF/s", "cores": threads, "kind": "port",
+               "sample": f"{n} CCFs at the same {W}x{H} / {P}^3 shape (positions 0-{threads - 1} x psi "
+                         f"0-{args.cpu_psi - 1}, one worker thread per core), {dt:.1f}s timed after "
+                         f"{setup:.1f}s setup",
+               "gpu_over_cpu": round(value / v, 1)}
+        log(f"CPU baseline {v:.2f} CCF/s ({n} CCFs in {dt:.1f}s)")
+
+    out = {
+        "metric": "correlations/sec @4096^2 (CCF/s), HBM roofline fraction of the dominant kernel",
+        "value": round(value, 2),
+        "unit": "CCF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded Gaussian pseudo-atom template and micrograph)",
+        "config": {"workload": workload, "image": [W, H], "template": T, "padded_template": P,
+                   "orientations_per_step": search.core.n_psi * (search.last - search.first + 1),
+                   "defocus_planes": len(planes), "batch": B, "precision": "fp32 storage and butterflies, "
+                   "fp64 sum/sum-of-squares"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": per_kernel,
+        "pipeline": {"algorithmic_bytes_per_ccf": pipe,
+                     "fraction_of_hbm_peak": round(pipe * value / HBM_PEAK, 4)},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

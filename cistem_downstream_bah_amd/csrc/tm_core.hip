@@ -152,17 +152,39 @@ struct tm_core {
   DevBuf<double> sum, sumsq;
   DevBuf<unsigned long long> hist;
   std::map<int, cf*> tw;
-  // timing
+  // timing: event pairs recorded around each kernel group on the handle's stream, resolved after
+  // the call's final synchronisation (no synchronisation inside the loop)
   bool timing = false;
-  hipEvent_t ev[2 * K_COUNT * 2] = {};
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  std::vector<std::pair<int, size_t>> ev_marks;  // (kernel id, index of the start event)
   double kernel_ms[K_COUNT] = {};
   long long kernel_launches[K_COUNT] = {};
 
   ~tm_core() {
     for (auto& kv : tw) (void)hipFree(kv.second);
-    for (auto& e : ev)
-      if (e) (void)hipEventDestroy(e);
+    for (auto& e : ev_pool) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  hipEvent_t next_event() {
+    if (ev_used == ev_pool.size()) {
+      hipEvent_t e;
+      TM_HIP(hipEventCreate(&e));
+      ev_pool.push_back(e);
+    }
+    return ev_pool[ev_used++];
+  }
+
+  void resolve_timing() {
+    for (auto& m : ev_marks) {
+      float ms = 0.f;
+      TM_HIP(hipEventElapsedTime(&ms, ev_pool[m.second], ev_pool[m.second + 1]));
+      kernel_ms[m.first] += ms;
+      kernel_launches[m.first] += 1;
+    }
+    ev_marks.clear();
+    ev_used = 0;
   }
 
   const cf* twiddle(int n) {
@@ -302,19 +324,19 @@ void launch_last(tm_core* h, int nb, const tmk::EpiParams& ep) {
 struct Timer {
   tm_core* h;
   int k;
+  size_t start = 0;
   Timer(tm_core* hh, int kk) : h(hh), k(kk) {
-    if (h->timing) TM_HIP(hipEventRecord(h->ev[2 * k], h->stream));
+    if (h->timing) {
+      start = h->ev_used;
+      TM_HIP(hipEventRecord(h->next_event(), h->stream));
+    }
   }
-  // Errors here surface on the next checked HIP call of the same stream.
+  // A failed record here surfaces on the next checked HIP call of the same stream.
   ~Timer() {
     if (h->timing) {
-      float ms = 0.f;
-      if (hipEventRecord(h->ev[2 * k + 1], h->stream) == hipSuccess &&
-          hipEventSynchronize(h->ev[2 * k + 1]) == hipSuccess &&
-          hipEventElapsedTime(&ms, h->ev[2 * k], h->ev[2 * k + 1]) == hipSuccess) {
-        h->kernel_ms[k] += ms;
-        h->kernel_launches[k] += 1;
-      }
+      hipEvent_t e = nullptr;
+      try { e = h->next_event(); } catch (...) { return; }
+      if (hipEventRecord(e, h->stream) == hipSuccess) h->ev_marks.push_back({k, start});
     }
   }
 };
@@ -542,8 +564,6 @@ int tm_init(tm_core* h, const tm_init_params* p, const float* template_spectrum,
     h->sumsq.alloc(npx);
     h->hist.alloc(512);
     for (int k : {P, P / 2, T, T / 2, W, W / 2, H}) h->twiddle(k);
-    for (auto& e : h->ev)
-      if (!e) TM_HIP(hipEventCreate(&e));
     reset_state(h);
     TM_HIP(hipStreamSynchronize(h->stream));
     h->initialised = true;
@@ -568,6 +588,8 @@ int tm_run_inner_loop(tm_core* h, const float* projection_filter, int size_i, in
                "orientation index space exhausted (too many planes for one handle)");
     h->planes.push_back(make_int2(size_i, defocus_i));
     for (int k = 0; k < K_COUNT; ++k) { h->kernel_ms[k] = 0.0; h->kernel_launches[k] = 0; }
+    h->ev_marks.clear();
+    h->ev_used = 0;
     long long done = 0;
     for (int o0 = 0; o0 < h->n_orient; o0 += h->B) {
       const int nb = std::min(h->B, h->n_orient - o0);
@@ -579,6 +601,7 @@ int tm_run_inner_loop(tm_core* h, const float* projection_filter, int size_i, in
       }
     }
     TM_HIP(hipStreamSynchronize(h->stream));
+    if (h->timing) h->resolve_timing();
     h->n_ccfs += done;
     if (ccfs) *ccfs = done;
   });

@@ -1,0 +1,181 @@
+"""Host driver mirroring the reference's ``match_template`` flow around the GPU inner loop.
+
+``MatchTemplateSearch`` does what MatchTemplateApp::DoCalculation does for one image
+(src/programs/match_template/match_template.cpp:490-1370): factorise/pad/rotate the image and
+whiten it, pad and transform the template, build the Euler grid and the in-plane angles, then for
+each defocus plane build the CTF x whitening filter and call the inner loop, and finally scale the
+maps and compute the z-score. The preparation steps and the search run through the C ABI of
+libtm_hip.so (include/tm_prepare.h, include/tm_hip.h); nothing here computes on the CPU beyond
+bookkeeping and the final un-rotation/cropping of the maps.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import core
+
+F32P = core.F32P
+
+
+def _lib():
+    L = core.lib()
+    if not getattr(L, "_tm_prepare_bound", False):
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.tm_prepare_last_error.restype = ctypes.c_char_p
+        L.tm_prepare_sizes.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ip, ip, ip, ip, ip]
+        L.tm_prepare_input_image.argtypes = [ctypes.c_int, F32P, ctypes.c_int, ctypes.c_int, ctypes.c_int, F32P, F32P,
+                                             F32P]
+        L.tm_prepare_template.argtypes = [ctypes.c_int, F32P, ctypes.c_int, ctypes.c_float, F32P, ip]
+        L.tm_projection_filter.argtypes = [ctypes.c_int, F32P, F32P, ctypes.c_int] + [ctypes.c_float] * 8 + [F32P]
+        L.tm_search_grid.argtypes = [ctypes.c_char, ctypes.c_int, ctypes.c_float, F32P, ctypes.c_int]
+        L.tm_psi_list.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float, F32P, ctypes.c_int]
+        L._tm_prepare_bound = True
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise core.TMError(rc, _lib().tm_prepare_last_error().decode())
+
+
+def _f(a):
+    return a.ctypes.data_as(F32P)
+
+
+PREPARE_EXPORTS = ["tm_prepare_sizes", "tm_prepare_input_image", "tm_prepare_template", "tm_projection_filter",
+                   "tm_search_grid", "tm_psi_list", "tm_prepare_last_error"]
+
+
+def prepare_sizes(mx: int, my: int, template_size: int):
+    v = [ctypes.c_int(0) for _ in range(5)]
+    _check(_lib().tm_prepare_sizes(mx, my, template_size, *[ctypes.byref(x) for x in v]))
+    W, H, rot, ncurve, maxpad = (x.value for x in v)
+    return W, H, bool(rot), ncurve, maxpad
+
+
+def prepare_input_image(micrograph: np.ndarray, template_size: int, device: int = 0):
+    """Returns (spectrum (H, W/2+1) complex64, curve_x, curve_y, W, H, rotated, max_padding)."""
+    m = np.ascontiguousarray(micrograph, np.float32)
+    my, mx = m.shape
+    W, H, rot, ncurve, maxpad = prepare_sizes(mx, my, template_size)
+    spec = np.zeros((H, W // 2 + 1), np.complex64)
+    cx = np.zeros(ncurve, np.float32)
+    cy = np.zeros(ncurve, np.float32)
+    _check(_lib().tm_prepare_input_image(device, _f(m), mx, my, template_size, _f(spec.view(np.float32)), _f(cx),
+                                         _f(cy)))
+    return spec, cx, cy, W, H, rot, maxpad
+
+
+def prepare_template(volume: np.ndarray, padding: float, device: int = 0) -> np.ndarray:
+    v = np.ascontiguousarray(volume, np.float32)
+    t = v.shape[0]
+    P = ctypes.c_int(0)
+    _check(_lib().tm_prepare_template(device, _f(v), t, padding, None, ctypes.byref(P)))
+    P = P.value
+    spec = np.zeros((P, P, P // 2 + 1), np.complex64)
+    _check(_lib().tm_prepare_template(device, _f(v), t, padding, _f(spec.view(np.float32)), None))
+    return spec
+
+
+def projection_filter(T, curve_x, curve_y, voltage_kV, cs_mm, amplitude_contrast, defocus1, defocus2,
+                      defocus_angle, pixel_size, phase_shift_deg=0.0) -> np.ndarray:
+    out = np.zeros((T, T // 2 + 1), np.complex64)
+    _check(_lib().tm_projection_filter(T, _f(curve_x), _f(curve_y), curve_x.size, voltage_kV, cs_mm,
+                                       amplitude_contrast, defocus1, defocus2, defocus_angle, pixel_size,
+                                       phase_shift_deg, _f(out.view(np.float32))))
+    return out
+
+
+def search_grid(symmetry: str, angular_step: float) -> np.ndarray:
+    t, n = symmetry[0].upper().encode(), int(symmetry[1:] or 0)
+    cnt = _lib().tm_search_grid(t, n, angular_step, None, 0)
+    if cnt < 0:
+        _check(cnt)
+    out = np.zeros((cnt, 2), np.float32)
+    _lib().tm_search_grid(t, n, angular_step, _f(out), cnt)
+    return out
+
+
+def psi_list(psi_start: float, psi_max: float, psi_step: float) -> np.ndarray:
+    n = _lib().tm_psi_list(psi_start, psi_max, psi_step, None, 0)
+    out = np.zeros(n, np.float32)
+    _lib().tm_psi_list(psi_start, psi_max, psi_step, _f(out), n)
+    return out
+
+
+@dataclass
+class SearchParams:
+    """match_template arguments used by the path (match_template.cpp:346-387)."""
+    pixel_size: float = 1.0
+    voltage_kV: float = 300.0
+    spherical_aberration_mm: float = 2.7
+    amplitude_contrast: float = 0.07
+    defocus1: float = 15000.0
+    defocus2: float = 15000.0
+    defocus_angle: float = 0.0
+    phase_shift_deg: float = 0.0
+    angular_step: float = 10.0
+    in_plane_angular_step: float = 10.0
+    defocus_search_range: float = 0.0
+    defocus_step: float = 0.0
+    padding: float = 1.0
+    symmetry: str = "C1"
+    first_search_position: int = -1
+    last_search_position: int = -1
+    batch_size: int = 0
+
+
+class MatchTemplateSearch:
+    """Prepared search of one image on one GPU (the local-run path of MatchTemplateApp)."""
+
+    histogram_number_of_points = 512          # match_template.cpp:45-47
+    histogram_min, histogram_max = -12.5, 22.5
+
+    def __init__(self, micrograph: np.ndarray, volume: np.ndarray, p: SearchParams, device: int = 0):
+        self.p = p
+        self.original_shape = micrograph.shape
+        T = volume.shape[0]
+        self.T = T
+        (self.input_spectrum, self.curve_x, self.curve_y, self.W, self.H, self.rotated,
+         self.max_padding) = prepare_input_image(micrograph, T, device)
+        self.template_spectrum = prepare_template(volume, p.padding, device)
+        self.grid = search_grid(p.symmetry, p.angular_step)
+        self.psi = psi_list(0.0, 360.0, p.in_plane_angular_step)
+        n = self.grid.shape[0]
+        self.first = 0 if p.first_search_position < 0 else p.first_search_position
+        self.last = n - 1 if p.last_search_position < 0 else min(p.last_search_position, n - 1)
+        self.defocus_step = p.defocus_step if p.defocus_step > 0 else 100.0
+        rng = p.defocus_search_range if p.defocus_step > 0 else 0.0
+        r = rng / self.defocus_step
+        self.n_def_half = int(r + 0.5) if r > 0 else int(r - 0.5)
+        sq = math.sqrt(float(self.W * self.H))
+        step = (self.histogram_max - self.histogram_min) / float(self.histogram_number_of_points)
+        step = float(np.float32(step))
+        self.histogram_min_scaled = self.histogram_min / sq
+        self.histogram_step_scaled = step / sq
+        self.core = core.TemplateMatchingCore(device)
+        self.core.init(self.template_spectrum, self.input_spectrum, T, self.grid, self.psi, self.first, self.last,
+                       self.histogram_min_scaled, self.histogram_step_scaled, 512, self.max_padding,
+                       self.defocus_step, 0.02, p.batch_size)
+
+    def filter_for(self, defocus_i: int) -> np.ndarray:
+        p = self.p
+        d1 = np.float32(p.defocus1) + np.float32(defocus_i) * np.float32(self.defocus_step)
+        d2 = np.float32(p.defocus2) + np.float32(defocus_i) * np.float32(self.defocus_step)
+        return projection_filter(self.T, self.curve_x, self.curve_y, p.voltage_kV, p.spherical_aberration_mm,
+                                 p.amplitude_contrast, float(d1), float(d2), p.defocus_angle, p.pixel_size,
+                                 p.phase_shift_deg)
+
+    def run(self, progress=None) -> int:
+        n = 0
+        for di in range(-self.n_def_half, self.n_def_half + 1):
+            n += self.core.run_inner_loop(self.filter_for(di), 0, di, progress)
+        return n
+
+    @property
+    def total_correlation_positions(self) -> int:
+        return (self.last - self.first + 1) * self.psi.size * (2 * self.n_def_half + 1)

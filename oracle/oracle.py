@@ -59,7 +59,7 @@ def lib():
         L.tmo_mt_get.argtypes = [vp, ctypes.c_int, F32P]
         L.tmo_mt_projection_filter.argtypes = [vp, ctypes.c_int, F32P]
         L.tmo_mt_projection_ccf.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, F32P, F32P]
-        L.tmo_mt_run.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.tmo_mt_run.argtypes = [vp] + [ctypes.c_int] * 8
         L.tmo_mt_results.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
         L.tmo_mt_results.restype = ctypes.c_long
         L.tmo_mt_finalise.argtypes = [vp, ctypes.c_int, ctypes.c_long, F32P, F32P, F32P, F32P]
@@ -312,8 +312,8 @@ class MatchTemplateOracle:
         return proj[:, : I.T].copy(), (ccf[:, : I.W].copy() if want_ccf else None)
 
     def run(self, threads: int = 1, pos_lo: int = -1, pos_hi: int = -1, def_lo: int = -(1 << 30),
-            def_hi: int = 1 << 30):
-        _check(lib().tmo_mt_run(self.h, self.fp64, threads, pos_lo, pos_hi, def_lo, def_hi))
+            def_hi: int = 1 << 30, psi_lo: int = 0, psi_hi: int = (1 << 30)):
+        _check(lib().tmo_mt_run(self.h, self.fp64, threads, pos_lo, pos_hi, def_lo, def_hi, psi_lo, psi_hi))
         return self.results()
 
     def results(self) -> dict:

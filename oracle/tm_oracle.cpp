@@ -411,7 +411,9 @@ struct MatchTemplate {
   // The hot loop over [first_pos, last_pos] x psi for all defocus planes (match_template.cpp:890-1169).
   // Positions are split over `threads` contiguous slices per plane and merged in slice order,
   // which keeps the sequential tie-break (earliest orientation wins).
-  void Run(MTResults& out, int threads, int pos_lo = -1, int pos_hi = -1, int def_lo = INT32_MIN, int def_hi = INT32_MAX) const {
+  void Run(MTResults& out, int threads, int pos_lo = -1, int pos_hi = -1, int def_lo = INT32_MIN, int def_hi = INT32_MAX,
+           int psi_lo = 0, int psi_hi = INT32_MAX) const {
+    psi_hi = std::min(psi_hi, NumPsi() - 1);
     if (pos_lo < 0) pos_lo = first_pos;
     if (pos_hi < 0) pos_hi = last_pos;
     InitResults(out);
@@ -432,7 +434,7 @@ struct MatchTemplate {
           Work w;
           InitWork(w);
           for (int pos = a; pos <= b; ++pos) {
-            for (int k = 0; k < NumPsi(); ++k) {
+            for (int k = psi_lo; k <= psi_hi; ++k) {
               Projection(w, pf, grid[2 * pos], grid[2 * pos + 1], psi[k]);
               CCF(w);
               Accumulate(w, part[t], pos, k, defocus_i, 0, GlobalIndex(0, defocus_i, pos, k));
@@ -697,10 +699,10 @@ int tmo_mt_projection_ccf(void* hv, int fp64, int pos, int psi_idx, int defocus_
 }
 // Run the search over positions [pos_lo, pos_hi] (or the handle's range when < 0) and all defocus
 // planes in [def_lo, def_hi]. Results stay in the handle.
-int tmo_mt_run(void* hv, int fp64, int threads, int pos_lo, int pos_hi, int def_lo, int def_hi) {
+int tmo_mt_run(void* hv, int fp64, int threads, int pos_lo, int pos_hi, int def_lo, int def_hi, int psi_lo, int psi_hi) {
   return guarded([&] {
-    if (fp64) { auto* h = (Handle64*)hv; h->mt.Run(h->res, threads, pos_lo, pos_hi, def_lo, def_hi); }
-    else { auto* h = (Handle32*)hv; h->mt.Run(h->res, threads, pos_lo, pos_hi, def_lo, def_hi); }
+    if (fp64) { auto* h = (Handle64*)hv; h->mt.Run(h->res, threads, pos_lo, pos_hi, def_lo, def_hi, psi_lo, psi_hi); }
+    else { auto* h = (Handle32*)hv; h->mt.Run(h->res, threads, pos_lo, pos_hi, def_lo, def_hi, psi_lo, psi_hi); }
   });
 }
 // which: 0 mip, 1 psi, 2 theta, 3 phi, 4 defocus, 5 pixel size (float numel); 6 sum, 7 sumsq (double numel);

@@ -1,0 +1,53 @@
+"""GPU: the product's preparation of the search inputs (include/tm_prepare.h) against the oracle's
+restatement of match_template.cpp:490-724, 837-845, 893-898 on the same inputs."""
+import numpy as np
+import pytest
+
+from cistem_downstream_bah_amd import match_template as MT
+from cistem_downstream_bah_amd import synth
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+@pytest.fixture(scope="module")
+def data():
+    return synth.config0(0)
+
+
+def test_grid_and_psi_identical():
+    for step in [10.0, 2.5]:
+        assert np.array_equal(MT.search_grid("C1", step), O.euler_grid("C1", step))
+    for sym in ["C3", "D2", "O", "I"]:
+        assert np.array_equal(MT.search_grid(sym, 5.0), O.euler_grid(sym, 5.0))
+    for st in [10.0, 1.5, 7.0]:
+        assert np.array_equal(MT.psi_list(0.0, 360.0, st), O.psi_list(0.0, 360.0, st))
+
+
+def test_sizes_match_factorisation():
+    for (mx, my) in [(4096, 4096), (3072, 2048), (4096, 2880), (3840, 3840), (2560, 2560)]:
+        W, H, rot, n, mp = MT.prepare_sizes(mx, my, 256)
+        fx, fy = O.factorize_axis(mx, 256), O.factorize_axis(my, 256)
+        assert (W, H) == ((fy, fx) if rot else (fx, fy))
+
+
+@pytest.mark.parametrize("padding", [1.0, 2.0])
+def test_input_and_template_preparation(data, padding):
+    s = data
+    mt = O.MatchTemplateOracle(O.MTParams(padding=padding), s.micrograph, s.volume)
+    spec, cx, cy, W, H, rot, mp = MT.prepare_input_image(s.micrograph, s.volume.shape[0])
+    assert (W, H) == (mt.info.W, mt.info.H)
+    ox, oy = mt.whitening_curve
+    assert np.array_equal(cx, ox)
+    ec = _rel(cy, oy)
+    ei = _rel(spec, mt.input_spectrum)
+    tpl = MT.prepare_template(s.volume, padding)
+    et = _rel(tpl, mt.template_spectrum)
+    pf = MT.projection_filter(mt.info.T, cx, cy, 300.0, 2.7, 0.07, 15000.0, 15000.0, 0.0, 1.0, 0.0)
+    ef = _rel(pf, mt.projection_filter(0))
+    print(f"padding {padding}: curve {ec:.2e} input spectrum {ei:.2e} template {et:.2e} filter {ef:.2e}", flush=True)
+    assert ec <= 1e-5 and ei <= 1e-5 and et <= 1e-5 and ef <= 1e-5
