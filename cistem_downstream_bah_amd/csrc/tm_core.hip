@@ -292,31 +292,45 @@ void launch_mid_t(tm_core* h, int nb) {
   check_launch("k_mid");
 }
 
+// Column-tile width of k_mid (and of the tiled Z layout) for image height H.
+int mid_kc(int H) { return H >= 4096 ? 4 : (H >= 2048 ? 8 : 16); }
+int mid_tiles(const tm_core* h) { return (h->Wh + mid_kc(h->H) - 1) / mid_kc(h->H); }
+
 void launch_mid(tm_core* h, int nb) {
   switch (h->H) {
     case 256: launch_mid_t<256, 16, 16>(h, nb); break;
     case 512: launch_mid_t<512, 16, 32>(h, nb); break;
     case 1024: launch_mid_t<1024, 16, 64>(h, nb); break;
     case 2048: launch_mid_t<2048, 8, 128>(h, nb); break;
-    case 4096: launch_mid_t<4096, 4, 256>(h, nb); break;
+    case 4096: {
+      const int ntiles = mid_tiles(h);
+      const int grid = ((ntiles + 7) / 8) * 8;
+      hipLaunchKernelGGL((tmk::k_mid4096<4>), dim3(grid), dim3(1024), 0, h->stream, h->rows.p, h->inpT.p, h->Z.p,
+                         h->Wh, h->T, nb, ntiles, h->twiddle(4096));
+      check_launch("k_mid4096");
+      break;
+    }
     default: throw TmError{TM_ERR_UNSUPPORTED, "unsupported image height " + std::to_string(h->H)};
   }
 }
 
-template <int WW, int NT>
-void launch_last_t(tm_core* h, int nb, const tmk::EpiParams& ep) {
-  hipLaunchKernelGGL((tmk::k_last<WW, NT>), dim3(h->H), dim3(NT), 0, h->stream, h->Z.p, h->H, nb, ep, h->twiddle(WW),
-                     h->twiddle(WW / 2));
+template <int WW, int NT, int QR>
+void launch_last_t(tm_core* h, int nb, tmk::EpiParams ep) {
+  ep.ztile = mid_kc(h->H);
+  ep.ntiles = mid_tiles(h);
+  ep.hstep_inv = 1.0 / ep.hstep;
+  hipLaunchKernelGGL((tmk::k_last<WW, NT, QR>), dim3(h->H), dim3(QR * NT), 0, h->stream, h->Z.p, h->H, nb, ep,
+                     h->twiddle(WW), h->twiddle(WW / 2));
   check_launch("k_last");
 }
 
 void launch_last(tm_core* h, int nb, const tmk::EpiParams& ep) {
   switch (h->W) {
-    case 256: launch_last_t<256, 128>(h, nb, ep); break;
-    case 512: launch_last_t<512, 256>(h, nb, ep); break;
-    case 1024: launch_last_t<1024, 256>(h, nb, ep); break;
-    case 2048: launch_last_t<2048, 256>(h, nb, ep); break;
-    case 4096: launch_last_t<4096, 256>(h, nb, ep); break;
+    case 256: launch_last_t<256, 32, 2>(h, nb, ep); break;
+    case 512: launch_last_t<512, 64, 2>(h, nb, ep); break;
+    case 1024: launch_last_t<1024, 128, 2>(h, nb, ep); break;
+    case 2048: launch_last_t<2048, 256, 2>(h, nb, ep); break;
+    case 4096: launch_last_t<4096, 256, 4>(h, nb, ep); break;
     default: throw TmError{TM_ERR_UNSUPPORTED, "unsupported image width " + std::to_string(h->W)};
   }
 }
@@ -373,7 +387,7 @@ void run_projections(tm_core* h, int o0, int nb) {
   }
   {
     Timer t(h, K_NORMALIZE);
-    hipLaunchKernelGGL((tmk::k_normalize<256>), dim3(nb), dim3(256), 0, h->stream, reinterpret_cast<float*>(h->projT.p),
+    hipLaunchKernelGGL((tmk::k_normalize<1024>), dim3(nb), dim3(1024), 0, h->stream, reinterpret_cast<float*>(h->projT.p),
                        T, 2 * proj_bs, (long long)h->W * h->H);
     check_launch("k_normalize");
   }
@@ -398,7 +412,7 @@ void run_batch(tm_core* h, int o0, int nb, int g0) {
   }
   {
     Timer t(h, K_LAST);
-    tmk::EpiParams ep;
+    tmk::EpiParams ep{};
     ep.mip = h->mip.p;
     ep.idx = h->idx.p;
     ep.sum = h->sum.p;
@@ -507,7 +521,7 @@ int tm_init(tm_core* h, const tm_init_params* p, const float* template_spectrum,
     h->W = W; h->H = H; h->Wh = W / 2 + 1;
     h->T = T; h->Th = T / 2 + 1;
     h->P = P; h->Ph = P / 2 + 1;
-    h->B = p->batch_size > 0 ? p->batch_size : 16;
+    h->B = p->batch_size > 0 ? p->batch_size : 32;
     TM_REQUIRE(h->B <= 256, TM_ERR_INVALID, "batch_size must be <= 256");
     h->n_pos_local = p->last_search_position - p->first_search_position + 1;
     h->n_psi = p->n_psi;
@@ -554,7 +568,7 @@ int tm_init(tm_core* h, const tm_init_params* p, const float* template_spectrum,
     h->slice.alloc((size_t)B * P * h->Ph);
     h->projT.alloc((size_t)B * T * h->Th);
     h->rows.alloc((size_t)B * T * h->Wh);
-    h->Z.alloc((size_t)B * H * h->Wh);
+    h->Z.alloc((size_t)B * mid_tiles(h) * H * mid_kc(H));
 
     // state
     const size_t npx = (size_t)W * H;
@@ -741,8 +755,12 @@ int tm_debug_ccf(tm_core* h, const float* projection_filter, int search_position
     hipLaunchKernelGGL(tmk::k_reset_state, dim3((npx + 255) / 256), dim3(256), 0, h->stream, m.p, ix.p, s.p, q.p, npx);
     check_launch("k_reset_state");
     hipLaunchKernelGGL(tmk::k_zero_u64, dim3(2), dim3(256), 0, h->stream, hs.p, 512);
-    tmk::EpiParams ep{m.p, ix.p, s.p, q.p, hs.p, h->prm.histogram_min_scaled, h->prm.histogram_step_scaled,
-                     h->prm.histogram_number_of_bins, 0};
+    tmk::EpiParams ep{};
+    ep.mip = m.p; ep.idx = ix.p; ep.sum = s.p; ep.sumsq = q.p; ep.hist = hs.p;
+    ep.hmin = h->prm.histogram_min_scaled;
+    ep.hstep = h->prm.histogram_step_scaled;
+    ep.nbins = h->prm.histogram_number_of_bins;
+    ep.g0 = 0;
     launch_last(h, 1, ep);
     std::vector<float> host(npx);
     TM_HIP(hipMemcpyAsync(host.data(), m.p, npx * sizeof(float), hipMemcpyDeviceToHost, h->stream));

@@ -109,20 +109,20 @@ __device__ __forceinline__ cf twiddle(const cf* __restrict__ tw, int idx) {
 }
 
 // ---------------------------------------------------------------- radix plans (powers of two)
-// Largest-radix-first plan with radices in {16, 8, 4, 2}.
+// Largest-radix-first plan with radices 2^LR (LR = 4: radix 16, LR = 3: radix 8) and one
+// smaller last radix when log2(N) is not a multiple of LR.
 __host__ __device__ constexpr int log2i(int n) { return n <= 1 ? 0 : 1 + log2i(n / 2); }
-__host__ __device__ constexpr int plan_stages(int N) {
+__host__ __device__ constexpr int plan_stages(int N, int LR = 4) {
   int l = log2i(N);
-  return l / 4 + ((l % 4) ? 1 : 0);
+  return l / LR + ((l % LR) ? 1 : 0);
 }
-// radix of stage s: 16 for the first l/4 stages, then 2^(l%4)
-__host__ __device__ constexpr int plan_radix(int N, int s) {
+__host__ __device__ constexpr int plan_radix(int N, int s, int LR = 4) {
   int l = log2i(N);
-  return (s < l / 4) ? 16 : (1 << (l % 4));
+  return (s < l / LR) ? (1 << LR) : (1 << (l % LR));
 }
-__host__ __device__ constexpr int plan_ns(int N, int s) {  // sub-transform length before stage s
+__host__ __device__ constexpr int plan_ns(int N, int s, int LR = 4) {  // sub-transform length before stage s
   int ns = 1;
-  for (int i = 0; i < s; ++i) ns *= plan_radix(N, i);
+  for (int i = 0; i < s; ++i) ns *= plan_radix(N, i, LR);
   return ns;
 }
 

@@ -24,6 +24,8 @@
 
 namespace tmk {
 
+__device__ __forceinline__ cf cmul(cf a, cf b, cf c) { return cmul(a, cmul(b, c)); }
+
 // LDS index padding: one slot per 16 keeps stride-16 Stockham stores off a single bank group.
 __device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
 __host__ __device__ constexpr int lds_len(int n) { return n + n / 16; }
@@ -31,11 +33,11 @@ __host__ __device__ constexpr int lds_len(int n) { return n + n / 16; }
 // In-place Stockham FFT of one line in LDS (padded index), executed by NT threads (tid < NT)
 // of a block in which every thread reaches the barriers. Each stage loads all butterfly inputs
 // into registers, synchronises, then stores.
-template <int N, int SIGN, int NT, int S = 0>
+template <int N, int SIGN, int NT, int LR = 4, int S = 0>
 __device__ __forceinline__ void fft_inplace(cf* buf, const cf* __restrict__ tw, int tid) {
-  if constexpr (S < plan_stages(N)) {
-    constexpr int R = plan_radix(N, S);
-    constexpr int NS = plan_ns(N, S);
+  if constexpr (S < plan_stages(N, LR)) {
+    constexpr int R = plan_radix(N, S, LR);
+    constexpr int NS = plan_ns(N, S, LR);
     constexpr int M = N / R;
     constexpr int CNT = (M + NT - 1) / NT;
     cf v[CNT][R];
@@ -65,7 +67,7 @@ __device__ __forceinline__ void fft_inplace(cf* buf, const cf* __restrict__ tw, 
       }
     }
     __syncthreads();
-    fft_inplace<N, SIGN, NT, S + 1>(buf, tw, tid);
+    fft_inplace<N, SIGN, NT, LR, S + 1>(buf, tw, tid);
   }
 }
 
@@ -301,7 +303,8 @@ __global__ __launch_bounds__(NT) void k_normalize(float* __restrict__ proj, int 
 // Tile of KC adjacent columns kx of the W x H spectrum, NTC threads per column. For each
 // orientation b: place the T forward-row-FFT rows at y = H/2 - T/2 + r, forward column FFT,
 // zero the DC term (ZeroCentralPixel, image.cpp:6498-6503), multiply conj(R) * I
-// (match_template.cpp:1083), inverse column FFT, and write Z[b][y][kx].
+// (match_template.cpp:1083), inverse column FFT, and write Z[b][tile][y][KC] (column tiles
+// contiguous, so every store of the tile is a full cache line).
 // Tiles are mapped so that consecutive tiles run on the same XCD (workgroups are dispatched
 // round-robin over the 8 XCDs), letting partial cache lines of Z merge in that XCD's L2.
 template <int H, int KC, int NTC>
@@ -320,7 +323,7 @@ __global__ __launch_bounds__(KC* NTC) void k_mid(const cf* __restrict__ rows, co
   const bool valid = kx < Wh;
   const int jlo = H / 2 - T / 2;
   const size_t rows_bs = (size_t)T * Wh;
-  const size_t z_bs = (size_t)H * Wh;
+  const size_t z_bs = (size_t)ntiles * H * KC;
   for (int b = 0; b < nb; ++b) {
     for (int e = ctid; e < H; e += NTC) buf[c][lpad(e)] = cmake(0.f, 0.f);
     __syncthreads();
@@ -342,13 +345,145 @@ __global__ __launch_bounds__(KC* NTC) void k_mid(const cf* __restrict__ rows, co
     }
     __syncthreads();
     fft_inplace<H, +1, NTC>(buf[c], tw, ctid);
-    cf* zb = Z + b * z_bs;
+    // Z[b][tile][y][KC]: the tile's output is one contiguous block; streamed past L2
+    cf* zt = Z + b * z_bs + (size_t)tile * H * KC;
     for (int e = tid; e < KC * H; e += KC * NTC) {
       const int cc = e % KC, y = e / KC;
-      const int kxx = kx0 + cc;
-      if (kxx < Wh) zb[(size_t)y * Wh + kxx] = buf[cc][lpad(y)];
+      const cf v = buf[cc][lpad(y)];
+      unsigned long long bits;
+      __builtin_memcpy(&bits, &v, sizeof(bits));
+      __builtin_nontemporal_store(bits, reinterpret_cast<unsigned long long*>(zt + e));
     }
     __syncthreads();
+  }
+}
+
+// Column pass for H = 4096 with the data register-resident between the three radix-16 stages:
+// 1024 threads = 256 butterflies x 4 columns, lane -> (column c = tid % 4, butterfly j = tid / 4).
+//  * forward stage 1 reads its 16 inputs x[j + 256 q] straight from the forward rows (zero outside
+//    the T projection rows), so no LDS zero-fill or scatter is needed;
+//  * forward stage 3, the DC zero, conj(R) * I and inverse stage 1 run on registers;
+//  * inverse stage 3 leaves z[j + 256 q] in registers and a wave stores 16 rows x 4 columns = 512
+//    contiguous bytes of the tiled Z per q.
+// Eight LDS passes per column and orientation (four exchanges, each a write and a read). With the
+// padded LDS index lpad(i) = i + i/16 the three access patterns are a per-thread base plus
+// compile-time offsets: lpad(16 j + q) = 17 j + q, lpad(j + 256 q) = lpad(j) + 272 q and
+// lpad(b2 + 16 q) = lpad(b2) + 17 q for b2 = (j / 16) 256 + j % 16.
+template <int KC>
+__global__ __launch_bounds__(1024) void k_mid4096(const cf* __restrict__ rows, const cf* __restrict__ inpT,
+                                                  cf* __restrict__ Z, int Wh, int T, int nb, int ntiles,
+                                                  const cf* __restrict__ tw) {
+  constexpr int H = 4096, M = 256;
+  constexpr int CS = lds_len(H) + 8;  // column stride: shifts the columns by 16 banks
+  static_assert(KC * M == 1024, "1024 threads cover KC columns of 256 butterflies");
+  __shared__ cf lds[KC * CS];
+  // Twiddles from LDS: W4096^(16 m) = W256^m and W4096^(b q) for b, q < 16; the stage-3 twiddle
+  // W4096^(j q) with j = 16 a + b is W256^(a q) * W4096^(b q).
+  __shared__ cf tw256[256];
+  __shared__ cf twbq[256];
+  const int tid = threadIdx.x;
+  if (tid < 256) {
+    tw256[tid] = tw[16 * tid];
+    twbq[tid] = tw[(tid / 16) * (tid % 16)];
+  }
+  __syncthreads();
+  const int c = tid % KC;
+  const int bid = blockIdx.x;
+  const int per_xcd = (ntiles + 7) / 8;
+  const int tile = (bid % 8) * per_xcd + bid / 8;
+  if (tile >= ntiles) return;  // whole block exits together
+  const int kx = tile * KC + c;
+  const bool valid = kx < Wh;
+  const int jlo = H / 2 - T / 2;
+  const size_t z_bs = (size_t)ntiles * H * KC;
+  for (int b = 0; b < nb; ++b) {
+    // Indices are re-derived every iteration from a register the compiler cannot see through, so
+    // the address and twiddle computations are not hoisted out of the loop (they would spill).
+    int j = tid / KC;
+    asm volatile("" : "+v"(j));
+    const int k2 = j % 16;
+    cf* const colbuf = lds + c * CS;
+    cf* const p_s1 = colbuf + 17 * j;                         // stage-1 stores: + q
+    cf* const p_rd = colbuf + j + (j >> 4);                   // strided reads:  + 272 q
+    cf* const p_s2 = colbuf + (j / 16) * 272 + k2;            // stage-2 stores: + 17 q
+    const cf* ic = inpT + (size_t)(valid ? kx : 0) * H + j;   // input spectrum: + 256 q
+    cf v[16];
+    // forward stage 1 (Ns = 1): inputs from the T non-zero rows placed at y = jlo + r; only the
+    // q with 256 q + [0, 256) overlapping [jlo, jlo + T) can be non-zero (a block-uniform range)
+    const int qmin = jlo / M, qmax = (jlo + T - 1) / M;
+    const cf* rb = rows + (size_t)b * T * Wh + kx;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      v[q] = cmake(0.f, 0.f);
+      if (q >= qmin && q <= qmax) {
+        const int r = j + M * q - jlo;
+        const cf* rq = rb + (ptrdiff_t)(M * q - jlo) * Wh;  // block-uniform base
+        if (valid && r >= 0 && r < T) v[q] = rq[j * Wh];
+      }
+    }
+    dft16<-1>(v);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) p_s1[q] = v[q];
+    __syncthreads();
+    // forward stage 2 (Ns = 16)
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = p_rd[272 * q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 1; q < 16; ++q) v[q] = cmul(v[q], twiddle<-1>(tw256, k2 * q));
+    dft16<-1>(v);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) p_s2[17 * q] = v[q];
+    __syncthreads();
+    // input-spectrum column prefetched before forward stage 3 so its latency overlaps the stage
+    cf iv[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) iv[q] = valid ? ic[M * q] : cmake(0.f, 0.f);
+    // forward stage 3 (Ns = 256): X[j + 256 q] in registers
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = p_rd[272 * q];
+#pragma unroll
+    for (int q = 1; q < 16; ++q)
+      v[q] = cmul(v[q], twiddle<-1>(tw256, (j / 16) * q), twiddle<-1>(twbq, k2 * 16 + q));
+    dft16<-1>(v);
+    // ZeroCentralPixel and conj(reference) * input
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const cf x = (kx == 0 && j == 0 && q == 0) ? cmake(0.f, 0.f) : v[q];
+      v[q] = cconjmul(x, iv[q]);
+    }
+    __syncthreads();
+    // inverse stage 1
+    dft16<+1>(v);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) p_s1[q] = v[q];
+    __syncthreads();
+    // inverse stage 2
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = p_rd[272 * q];
+    __syncthreads();
+#pragma unroll
+    for (int q = 1; q < 16; ++q) v[q] = cmul(v[q], twiddle<+1>(tw256, k2 * q));
+    dft16<+1>(v);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) p_s2[17 * q] = v[q];
+    __syncthreads();
+    // inverse stage 3: z[j + 256 q]
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = p_rd[272 * q];
+#pragma unroll
+    for (int q = 1; q < 16; ++q)
+      v[q] = cmul(v[q], twiddle<+1>(tw256, (j / 16) * q), twiddle<+1>(twbq, k2 * 16 + q));
+    dft16<+1>(v);
+    unsigned long long* zt = reinterpret_cast<unsigned long long*>(Z + b * z_bs + (size_t)tile * H * KC) +
+                             (size_t)j * KC + c;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      unsigned long long bits;
+      __builtin_memcpy(&bits, &v[q], sizeof(bits));
+      __builtin_nontemporal_store(bits, zt + (size_t)M * KC * q);
+    }
+    __syncthreads();  // LDS reads of this orientation complete before the next one writes
   }
 }
 
@@ -359,47 +494,28 @@ struct EpiParams {
   double* sum;
   double* sumsq;
   unsigned long long* hist;
-  double hmin, hstep;
+  double hmin, hstep_inv, hstep;
   int nbins;
-  int g0;   // global orientation index of batch entry 0
+  int g0;      // global orientation index of batch entry 0
+  int ztile;   // column-tile width of Z (power of two)
+  int ntiles;  // column tiles per orientation
 };
 
-// One block per image row y. For each orientation b of the batch: C2R of Z[b][y][*] into W reals,
-// then, per pixel, in orientation order: strict-greater MIP/argmax update, histogram bin
-// int((v - min) / step) over 0..nbins-1, FP64 sum += v and sum-of-squares += (v*v in float).
+// Loads row y of orientation b from the tiled Z into LDS and forms the N/2-point complex input of
+// the C2R: Z_k = (X_k + conj X_{h-k}) + i (X_k - conj X_{h-k}) conj(W_N^k).
 template <int W, int NT>
-__global__ __launch_bounds__(NT) void k_last(const cf* __restrict__ Z, int H, int nb, EpiParams ep,
-                                             const cf* __restrict__ tw_n, const cf* __restrict__ tw_h) {
+__device__ __forceinline__ void last_load_pre(const cf* __restrict__ Z, int H, int b, int y, const EpiParams& ep,
+                                              cf* buf, const cf* __restrict__ tw_n, int t, bool active) {
   constexpr int h = W / 2;
-  constexpr int Wh = W / 2 + 1;
-  constexpr int PAIRS = h / NT;
-  static_assert(h % NT == 0, "row length must be a multiple of 2*NT");
-  __shared__ cf buf[lds_len(h + 1)];
-  __shared__ unsigned int hist[512];
-  const int y = blockIdx.x, tid = threadIdx.x;
-  for (int i = tid; i < 512; i += NT) hist[i] = 0u;
-  float mip[2 * PAIRS];
-  int idx[2 * PAIRS];
-  double s1[2 * PAIRS], s2[2 * PAIRS];
-  const size_t rowoff = (size_t)y * W;
-#pragma unroll
-  for (int u = 0; u < PAIRS; ++u) {
-    const int m = tid + u * NT;
-    const float2 a = *reinterpret_cast<const float2*>(ep.mip + rowoff + 2 * m);
-    const int2 ia = *reinterpret_cast<const int2*>(ep.idx + rowoff + 2 * m);
-    const double2 sa = *reinterpret_cast<const double2*>(ep.sum + rowoff + 2 * m);
-    const double2 qa = *reinterpret_cast<const double2*>(ep.sumsq + rowoff + 2 * m);
-    mip[2 * u] = a.x; mip[2 * u + 1] = a.y;
-    idx[2 * u] = ia.x; idx[2 * u + 1] = ia.y;
-    s1[2 * u] = sa.x; s1[2 * u + 1] = sa.y;
-    s2[2 * u] = qa.x; s2[2 * u + 1] = qa.y;
+  if (active) {
+    const int sh = __ffs(ep.ztile) - 1;
+    const cf* zb = Z + (size_t)b * ep.ntiles * H * ep.ztile;
+    for (int e = t; e <= h; e += NT)
+      buf[lpad(e)] = zb[((size_t)(e >> sh) * H + y) * ep.ztile + (e & (ep.ztile - 1))];
   }
   __syncthreads();
-  for (int b = 0; b < nb; ++b) {
-    const cf* src = Z + ((size_t)b * H + y) * Wh;
-    for (int e = tid; e <= h; e += NT) buf[lpad(e)] = src[e];
-    __syncthreads();
-    for (int k = tid; k <= h / 2; k += NT) {
+  if (active) {
+    for (int k = t; k <= h / 2; k += NT) {
       if (k == 0) {
         const float x0 = buf[lpad(0)].x, xh = buf[lpad(h)].x;
         buf[lpad(0)] = cmake(x0 + xh, x0 - xh);
@@ -415,37 +531,89 @@ __global__ __launch_bounds__(NT) void k_last(const cf* __restrict__ Z, int H, in
         buf[lpad(kk)] = cmake(E.x - Ok.y, -E.y + Ok.x);
       }
     }
-    __syncthreads();
-    fft_inplace<h, +1, NT>(buf, tw_h, tid);
-    const int g = ep.g0 + b;
+  }
+  __syncthreads();
+}
+
+// One block of QR*NT threads per image row y; QR groups of NT threads transform QR orientations of
+// the batch at a time (C2R of row y of Z into W reals), then every thread updates its 2*PAIRS pixels
+// for those orientations in batch order: strict-greater MIP/argmax, histogram bin
+// int((v - min) / step) over 0..nbins-1, FP64 sum += v and FP64 sum-of-squares += (v*v in float)
+// (match_template.cpp:1100-1149). Rows are assigned so that consecutive rows run on the same XCD and
+// share Z's cache lines in L2.
+template <int W, int NT, int QR>
+__global__ __launch_bounds__(QR * NT, 4) void k_last(const cf* __restrict__ Z, int H, int nb, EpiParams ep,
+                                                     const cf* __restrict__ tw_n, const cf* __restrict__ tw_h) {
+  constexpr int h = W / 2;
+  constexpr int BT = QR * NT;
+  constexpr int PAIRS = h / BT;
+  static_assert(h % BT == 0, "row length must be a multiple of 2*QR*NT");
+  __shared__ cf buf[QR][lds_len(h + 1)];
+  // 8 interleaved copies of the histogram (bin * 8 + lane % 8) spread same-bin atomics of a wave
+  constexpr int HC = 8;
+  __shared__ unsigned int hist[512 * HC];
+  const int tid = threadIdx.x;
+  const int half = tid / NT, t = tid % NT;
+  const int bid = blockIdx.x;
+  const int y = (bid % 8) * (H / 8) + bid / 8;
+  for (int i = tid; i < 512 * HC; i += BT) hist[i] = 0u;
+  const int hc = tid % HC;
+  float mip[2 * PAIRS];
+  int idx[2 * PAIRS];
+  double s1[2 * PAIRS], s2[2 * PAIRS];
+  const size_t rowoff = (size_t)y * W;
 #pragma unroll
-    for (int u = 0; u < PAIRS; ++u) {
-      const cf z = buf[lpad(tid + u * NT)];
-      const float vv[2] = {z.x, z.y};
+  for (int u = 0; u < PAIRS; ++u) {
+    const int m = tid + u * BT;
+    const float2 a = *reinterpret_cast<const float2*>(ep.mip + rowoff + 2 * m);
+    const int2 ia = *reinterpret_cast<const int2*>(ep.idx + rowoff + 2 * m);
+    const double2 sa = *reinterpret_cast<const double2*>(ep.sum + rowoff + 2 * m);
+    const double2 qa = *reinterpret_cast<const double2*>(ep.sumsq + rowoff + 2 * m);
+    mip[2 * u] = a.x; mip[2 * u + 1] = a.y;
+    idx[2 * u] = ia.x; idx[2 * u + 1] = ia.y;
+    s1[2 * u] = sa.x; s1[2 * u + 1] = sa.y;
+    s2[2 * u] = qa.x; s2[2 * u + 1] = qa.y;
+  }
+  for (int b0 = 0; b0 < nb; b0 += QR) {
+    const int nbp = min(QR, nb - b0);
+    last_load_pre<W, NT>(Z, H, b0 + half, y, ep, buf[half], tw_n, t, half < nbp);
+    fft_inplace<h, +1, NT, 3>(buf[half], tw_h, t);
+#pragma unroll 1
+    for (int q = 0; q < nbp; ++q) {
+      const int g = ep.g0 + b0 + q;
 #pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        const float v = vv[t];
-        const int p = 2 * u + t;
-        if (v > mip[p]) { mip[p] = v; idx[p] = g; }
-        const double d = ((double)v - ep.hmin) / ep.hstep;
-        if (d > -1.0 && d < (double)ep.nbins) atomicAdd(&hist[(int)d], 1u);
-        s1[p] += (double)v;
-        const float sq = v * v;
-        s2[p] += (double)sq;
+      for (int u = 0; u < PAIRS; ++u) {
+        const cf z = buf[q][lpad(tid + u * BT)];
+        const float vv[2] = {z.x, z.y};
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          const float v = vv[c];
+          const int p = 2 * u + c;
+          if (v > mip[p]) { mip[p] = v; idx[p] = g; }
+          const double d = ((double)v - ep.hmin) * ep.hstep_inv;
+          if (d > -1.0 && d < (double)ep.nbins) atomicAdd(&hist[(int)d * HC + hc], 1u);
+          s1[p] += (double)v;
+          const float sq = v * v;
+          s2[p] += (double)sq;
+        }
       }
     }
     __syncthreads();
   }
 #pragma unroll
   for (int u = 0; u < PAIRS; ++u) {
-    const int m = tid + u * NT;
+    const int m = tid + u * BT;
     *reinterpret_cast<float2*>(ep.mip + rowoff + 2 * m) = make_float2(mip[2 * u], mip[2 * u + 1]);
     *reinterpret_cast<int2*>(ep.idx + rowoff + 2 * m) = make_int2(idx[2 * u], idx[2 * u + 1]);
     *reinterpret_cast<double2*>(ep.sum + rowoff + 2 * m) = make_double2(s1[2 * u], s1[2 * u + 1]);
     *reinterpret_cast<double2*>(ep.sumsq + rowoff + 2 * m) = make_double2(s2[2 * u], s2[2 * u + 1]);
   }
-  for (int i = tid; i < ep.nbins; i += NT)
-    if (hist[i]) atomicAdd(&ep.hist[i], (unsigned long long)hist[i]);
+  for (int i = tid; i < ep.nbins; i += BT) {
+    unsigned long long n = 0;
+#pragma unroll
+    for (int c = 0; c < HC; ++c) n += hist[i * HC + c];
+    if (n) atomicAdd(&ep.hist[i], n);
+  }
 }
 
 // ---------------------------------------------------------------- state and finalisation

@@ -46,7 +46,7 @@ typedef struct tm_init_params {
   int max_padding;               /* kept for interface parity; the CPU path it mirrors does not trim */
   float defocus_step;            /* best_defocus map stores defocus_i * defocus_step                 */
   float pixel_size_step;         /* best_pixel_size map stores size_i * pixel_size_step              */
-  int batch_size;                /* orientations per launch; 0 selects the default                   */
+  int batch_size;                /* orientations per launch; 0 selects the default (32)              */
 } tm_init_params;
 
 /* Number of visible HIP devices. Replaces DeviceManager::Init's device query
