@@ -33,6 +33,17 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed building libtm_hip.so")
     os.replace(LIB + ".tmp", LIB)
+    # the C++ command-line driver over the library
+    drv = os.path.join(HERE, "match_template_hip")
+    cmd = ["g++", "-O2", "-std=c++17", "-o", drv + ".tmp", os.path.join(CSRC, "match_template_main.cpp"),
+           "-L" + HERE, "-ltm_hip", "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("g++ failed building match_template_hip")
+    os.replace(drv + ".tmp", drv)
     return LIB
 
 

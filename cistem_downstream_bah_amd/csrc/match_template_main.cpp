@@ -1,0 +1,258 @@
+// match_template-style command-line driver (C++ host) over libtm_hip.so.
+//
+// Mirrors MatchTemplateApp::DoCalculation for a local run on one GPU
+// (src/programs/match_template/match_template.cpp:283-1370): prepare the image and template,
+// build the Euler grid and in-plane angles, loop over defocus planes through TemplateMatchingCore,
+// then scale, z-score, un-rotate, crop to the original size and write the maps as MRC files,
+// the histogram text file (:1333-1364, master form :1763-1837) and the peaks above the expected
+// threshold (:1839-1980, iterative maximum with radius exclusion).
+//
+//   match_template_hip --image img.mrc --template vol.mrc --out prefix [options]
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/template_matching_core.hpp"
+#include "../../include/tm_prepare.h"
+
+namespace {
+
+struct Vol {
+  int nx = 0, ny = 0, nz = 0;
+  std::vector<float> v;  // x fastest
+};
+
+Vol read_mrc(const std::string& path) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) throw std::runtime_error("cannot open " + path);
+  int32_t hdr[256];
+  f.read(reinterpret_cast<char*>(hdr), 1024);
+  if (hdr[3] != 2) throw std::runtime_error(path + ": only MRC mode 2 is supported");
+  Vol r;
+  r.nx = hdr[0]; r.ny = hdr[1]; r.nz = hdr[2];
+  f.seekg(1024 + hdr[23]);
+  r.v.resize((size_t)r.nx * r.ny * r.nz);
+  f.read(reinterpret_cast<char*>(r.v.data()), r.v.size() * sizeof(float));
+  if (!f) throw std::runtime_error(path + ": truncated data");
+  return r;
+}
+
+void write_mrc(const std::string& path, const float* data, int nx, int ny, float pixel_size) {
+  int32_t hdr[256] = {0};
+  hdr[0] = nx; hdr[1] = ny; hdr[2] = 1; hdr[3] = 2;
+  hdr[7] = nx; hdr[8] = ny; hdr[9] = 1;
+  float cell[3] = {nx * pixel_size, ny * pixel_size, pixel_size};
+  std::memcpy(&hdr[10], cell, sizeof(cell));
+  float ang[3] = {90.f, 90.f, 90.f};
+  std::memcpy(&hdr[13], ang, sizeof(ang));
+  hdr[16] = 1; hdr[17] = 2; hdr[18] = 3;
+  float mn = FLT_MAX, mx = -FLT_MAX;
+  double mean = 0;
+  for (long i = 0; i < (long)nx * ny; ++i) { mn = std::min(mn, data[i]); mx = std::max(mx, data[i]); mean += data[i]; }
+  float stats[3] = {mn, mx, (float)(mean / ((double)nx * ny))};
+  std::memcpy(&hdr[19], stats, sizeof(stats));
+  std::memcpy(&hdr[52], "MAP ", 4);
+  hdr[53] = 0x00004444;
+  std::ofstream f(path, std::ios::binary);
+  f.write(reinterpret_cast<const char*>(hdr), 1024);
+  f.write(reinterpret_cast<const char*>(data), (size_t)nx * ny * sizeof(float));
+}
+
+// Padded-layout W x H map -> dense, un-rotated (Rotate2DInPlaceBy90Degrees(false),
+// image.cpp:11127-11138) and cropped/padded to the original size with a fill value
+// (Image::Resize -> ClipInto, image.cpp:8522-8575, 8825-8838).
+std::vector<float> to_original(const float* padded, int W, int H, bool rotated, int ox, int oy, bool edge_fill,
+                               float fill) {
+  std::vector<float> d((size_t)W * H);
+  for (int y = 0; y < H; ++y) std::memcpy(&d[(size_t)y * W], padded + (size_t)y * (W + 2), W * sizeof(float));
+  int nx = W, ny = H;
+  if (rotated) {
+    std::vector<float> r((size_t)W * H);
+    const int rx = H, ry = W;  // rotated image is H x W
+    for (int yo = 0; yo < H; ++yo)
+      for (int xo = 0; xo < W; ++xo) r[(size_t)(W - xo - 1) * rx + yo] = d[(size_t)yo * W + xo];
+    d.swap(r);
+    nx = rx; ny = ry;
+  }
+  if (edge_fill) {
+    double s = 0; long n = 0;
+    for (int x = 0; x < nx; ++x) { s += d[x]; s += d[(size_t)(ny - 1) * nx + x]; n += 2; }
+    for (int y = 1; y < ny - 1; ++y) { s += d[(size_t)y * nx]; s += d[(size_t)y * nx + nx - 1]; n += 2; }
+    fill = (float)(s / (double)(float)n);
+  }
+  std::vector<float> o((size_t)ox * oy);
+  for (int jj = 0; jj < oy; ++jj) {
+    const int j = ny / 2 + (jj - oy / 2);
+    for (int ii = 0; ii < ox; ++ii) {
+      const int i = nx / 2 + (ii - ox / 2);
+      o[(size_t)jj * ox + ii] = (i < 0 || i >= nx || j < 0 || j >= ny) ? fill : d[(size_t)j * nx + i];
+    }
+  }
+  return o;
+}
+
+double erfcinv_winitzki(double x) {  // functions.cpp:1282-1299
+  const double PI = 3.14159265359;
+  double y = 1.0 - x;
+  double sgn = (y < 0) ? -1.0 : 1.0;
+  y = (1 - y) * (1 + y);
+  double lnx = log(y);
+  double tt1 = 2 / (PI * 0.147) + 0.5 * lnx;
+  double tt2 = 1 / (0.147) * lnx;
+  return sgn * sqrt(-tt1 + sqrt(tt1 * tt1 - tt2));
+}
+
+void usage() {
+  std::fprintf(stderr,
+               "usage: match_template_hip --image F --template F --out PREFIX [--pixel-size 1] [--voltage 300]\n"
+               "  [--cs 2.7] [--amplitude-contrast 0.07] [--defocus1 A] [--defocus2 A] [--defocus-angle 0]\n"
+               "  [--phase-shift 0] [--angular-step 10] [--in-plane-step 10] [--defocus-range 0] [--defocus-step 0]\n"
+               "  [--padding 1] [--symmetry C1] [--min-peak-radius 10] [--device 0] [--batch 0]\n");
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  std::map<std::string, std::string> a;
+  for (int i = 1; i + 1 < argc; i += 2) {
+    if (std::strncmp(argv[i], "--", 2) != 0) { usage(); return 2; }
+    a[argv[i] + 2] = argv[i + 1];
+  }
+  if (!a.count("image") || !a.count("template") || !a.count("out")) { usage(); return 2; }
+  auto f = [&](const char* k, double d) { return a.count(k) ? std::stod(a[k]) : d; };
+  try {
+    const float pixel_size = f("pixel-size", 1.0), kv = f("voltage", 300.0), cs = f("cs", 2.7);
+    const float amp = f("amplitude-contrast", 0.07), df1 = f("defocus1", 15000.0), df2 = f("defocus2", df1);
+    const float dfa = f("defocus-angle", 0.0), phase = f("phase-shift", 0.0);
+    const float ang_step = f("angular-step", 10.0), psi_step = f("in-plane-step", 10.0);
+    float def_range = f("defocus-range", 0.0), def_step = f("defocus-step", 0.0);
+    const float padding = f("padding", 1.0), min_peak_radius = f("min-peak-radius", 10.0);
+    const int device = (int)f("device", 0), batch = (int)f("batch", 0);
+    const std::string sym = a.count("symmetry") ? a["symmetry"] : "C1";
+    const std::string out = a["out"];
+
+    Vol img = read_mrc(a["image"]);
+    Vol vol = read_mrc(a["template"]);
+    if (img.nz != 1 || vol.nx != vol.ny || vol.nx != vol.nz) throw std::runtime_error("need a 2-D image and a cubic template");
+    const int T = vol.nx;
+    int W, H, rot, ncurve, maxpad;
+    if (tm_prepare_sizes(img.nx, img.ny, T, &W, &H, &rot, &ncurve, &maxpad)) throw std::runtime_error(tm_prepare_last_error());
+    std::vector<float> spec((size_t)H * (W / 2 + 1) * 2), cx(ncurve), cy(ncurve);
+    if (tm_prepare_input_image(device, img.v.data(), img.nx, img.ny, T, spec.data(), cx.data(), cy.data()))
+      throw std::runtime_error(tm_prepare_last_error());
+    int P = 0;
+    if (tm_prepare_template(device, vol.v.data(), T, padding, nullptr, &P)) throw std::runtime_error(tm_prepare_last_error());
+    std::vector<float> tpl((size_t)P * P * (P / 2 + 1) * 2);
+    if (tm_prepare_template(device, vol.v.data(), T, padding, tpl.data(), &P)) throw std::runtime_error(tm_prepare_last_error());
+    const int npos = tm_search_grid(sym[0], sym.size() > 1 ? std::stoi(sym.substr(1)) : 0, ang_step, nullptr, 0);
+    std::vector<float> grid(2 * (size_t)npos);
+    tm_search_grid(sym[0], sym.size() > 1 ? std::stoi(sym.substr(1)) : 0, ang_step, grid.data(), npos);
+
+    if (def_step <= 0.0) { def_range = 0.0f; def_step = 100.0f; }
+    const float r = def_range / def_step;
+    const int ndh = r > 0 ? int(r + 0.5) : int(r - 0.5);
+    const double sqrt_px = sqrt((double)(W * H));
+    const float hstep = (22.5f - -12.5f) / float(512);
+    cistem_amd::TemplateMatchingCore core(device);
+    core.Init(tpl.data(), P, spec.data(), W, H, T, 0.02f, def_step, 360.0f, 0.0f, psi_step, grid.data(), npos,
+              -12.5 / sqrt_px, hstep / sqrt_px, 512, maxpad, 0, npos - 1, batch);
+    std::vector<float> pf((size_t)T * (T / 2 + 1) * 2);
+    for (int di = -ndh; di <= ndh; ++di) {
+      if (tm_projection_filter(T, cx.data(), cy.data(), ncurve, kv, cs, amp, df1 + float(di) * def_step,
+                               df2 + float(di) * def_step, dfa, pixel_size, phase, pf.data()))
+        throw std::runtime_error(tm_prepare_last_error());
+      const long long n = core.RunInnerLoop(pf.data(), 0, di);
+      std::fprintf(stderr, "defocus plane %d: %lld CCFs\n", di, n);
+    }
+    const long long total = core.total_number_of_cccs_calculated;
+    const size_t np = (size_t)(W + 2) * H;
+    std::vector<float> mip(np), psi(np), theta(np), phi(np), defo(np), pix(np);
+    std::vector<long> hist(512, 0);
+    core.CopyResults(mip.data(), psi.data(), theta.data(), phi.data(), defo.data(), pix.data(), nullptr, nullptr,
+                     hist.data());
+    std::vector<float> mips(np), z(np), avg(np), sd(np);
+    core.Finalize(total, mips.data(), z.data(), avg.data(), sd.data());
+    const int ox = img.nx, oy = img.ny;
+    const bool r90 = rot != 0;
+    auto mip_o = to_original(mips.data(), W, H, r90, ox, oy, true, 0.f);
+    auto z_o = to_original(z.data(), W, H, r90, ox, oy, true, 0.f);
+    auto avg_o = to_original(avg.data(), W, H, r90, ox, oy, true, 0.f);
+    auto sd_o = to_original(sd.data(), W, H, r90, ox, oy, true, 0.f);
+    auto psi_o = to_original(psi.data(), W, H, r90, ox, oy, false, 0.f);
+    auto theta_o = to_original(theta.data(), W, H, r90, ox, oy, false, 0.f);
+    auto phi_o = to_original(phi.data(), W, H, r90, ox, oy, false, 0.f);
+    auto def_o = to_original(defo.data(), W, H, r90, ox, oy, false, 0.f);
+    auto pix_o = to_original(pix.data(), W, H, r90, ox, oy, false, 0.f);
+    write_mrc(out + "_mip.mrc", mip_o.data(), ox, oy, pixel_size);
+    write_mrc(out + "_scaled_mip.mrc", z_o.data(), ox, oy, pixel_size);
+    write_mrc(out + "_avg.mrc", avg_o.data(), ox, oy, pixel_size);
+    write_mrc(out + "_std.mrc", sd_o.data(), ox, oy, pixel_size);
+    write_mrc(out + "_psi.mrc", psi_o.data(), ox, oy, pixel_size);
+    write_mrc(out + "_theta.mrc", theta_o.data(), ox, oy, pixel_size);
+    write_mrc(out + "_phi.mrc", phi_o.data(), ox, oy, pixel_size);
+    write_mrc(out + "_defocus.mrc", def_o.data(), ox, oy, pixel_size);
+    write_mrc(out + "_pixel_size.mrc", pix_o.data(), ox, oy, pixel_size);
+
+    // expected threshold, master form (match_template.cpp:1799)
+    const double erf_input = 2.0 / (1.0 * (double)ox * (double)oy * (double)total);
+    const float thr = sqrtf(2.0f) * (float)erfcinv_winitzki(erf_input);
+    // histogram file (match_template.cpp:1333-1364), bins 0..511
+    {
+      const float first = -12.5f + hstep / 2.0f;
+      std::vector<double> surv(512, 0.0), expect(512, 0.0);
+      surv[511] = (double)hist[511];
+      for (int i = 510; i >= 0; --i) surv[i] = surv[i + 1] + (double)hist[i];
+      for (int i = 0; i < 512; ++i)
+        expect[i] = (erfc((first + hstep * float(i)) / sqrtf(2.0f)) / 2.0f) * ((float)(sqrt_px * sqrt_px) * float(total));
+      FILE* hf = std::fopen((out + "_histogram.txt").c_str(), "w");
+      std::fprintf(hf, "# Expected threshold = %.2f\n", thr);
+      std::fprintf(hf, "# SNR, histogram, survival histogram, random survival histogram\n");
+      for (int i = 0; i < 512; ++i)
+        std::fprintf(hf, "%f %ld %f %f\n", first + hstep * float(i), hist[i], surv[i], expect[i]);
+      std::fclose(hf);
+    }
+    // peaks (match_template.cpp:1839-1980): integer maximum of the z-score map away from a
+    // T/4 + 1 border; mask a disc of min_peak_radius; stop below the threshold or at 1000 peaks
+    {
+      std::vector<float> zz = z_o;
+      const int edge = T / 4 + 1;
+      const int rad = (int)min_peak_radius;
+      const float rad2 = powf(float(rad), 2);
+      FILE* pf2 = std::fopen((out + "_peaks.txt").c_str(), "w");
+      std::fprintf(pf2, "# x_A y_A psi theta phi defocus pixel_size peak_height\n");
+      int found = 0;
+      while (found < 1000) {
+        float best = -FLT_MAX;
+        int bx = 0, by = 0;
+        for (int y = edge; y < oy - edge; ++y)
+          for (int x = edge; x < ox - edge; ++x)
+            if (zz[(size_t)y * ox + x] > best) { best = zz[(size_t)y * ox + x]; bx = x; by = y; }
+        if (best < thr) break;
+        const size_t at = (size_t)by * ox + bx;
+        std::fprintf(pf2, "%f %f %f %f %f %f %f %f\n", bx * pixel_size, by * pixel_size, psi_o[at], theta_o[at],
+                     phi_o[at], def_o[at], pix_o[at], z_o[at]);
+        for (int j = std::max(by - rad, 0); j < std::min(by + rad, oy); ++j)
+          for (int i = std::max(bx - rad, 0); i < std::min(bx + rad, ox); ++i) {
+            const float dx = float(i) - bx, dy = float(j) - by;
+            if (dx * dx + dy * dy <= rad2) zz[(size_t)j * ox + i] = -FLT_MAX;
+          }
+        ++found;
+      }
+      std::fclose(pf2);
+      std::fprintf(stderr, "%d peaks above the expected threshold %.3f\n", found, thr);
+    }
+    std::fprintf(stderr, "total CCFs %lld\n", total);
+    return 0;
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "match_template_hip: %s\n", e.what());
+    return 1;
+  }
+}

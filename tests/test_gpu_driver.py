@@ -1,0 +1,48 @@
+"""GPU: the C++ match_template driver (cistem_downstream_bah_amd/match_template_hip) end to end on the
+config-0 inputs written as MRC files: its maps equal the C-ABI path's and its peak list contains
+every inserted particle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from cistem_downstream_bah_amd import match_template as MT
+from cistem_downstream_bah_amd import mrc, synth
+
+pytestmark = pytest.mark.gpu
+DRIVER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "cistem_downstream_bah_amd",
+                      "match_template_hip")
+
+
+def test_driver_end_to_end(tmp_path):
+    s = synth.config0(0)
+    img, vol = str(tmp_path / "img.mrc"), str(tmp_path / "vol.mrc")
+    mrc.write_mrc(img, s.micrograph)
+    mrc.write_mrc(vol, s.volume)
+    out = str(tmp_path / "run")
+    r = subprocess.run([DRIVER, "--image", img, "--template", vol, "--out", out, "--angular-step", "10",
+                        "--in-plane-step", "10", "--min-peak-radius", "10"], capture_output=True, text=True,
+                       timeout=240)
+    print(r.stderr, flush=True)
+    assert r.returncode == 0
+    z = mrc.read_mrc(out + "_scaled_mip.mrc")
+    # same search through the Python mirror of the driver
+    search = MT.MatchTemplateSearch(s.micrograph, s.volume, MT.SearchParams(angular_step=10.0,
+                                                                            in_plane_angular_step=10.0))
+    n = search.run()
+    ref = search.core.finalize(n)
+    assert np.abs(z - ref["zscore"]).max() <= 1e-5 * max(1.0, np.abs(ref["zscore"]).max())
+    peaks = np.loadtxt(out + "_peaks.txt", ndmin=2)
+    thr = float(open(out + "_histogram.txt").readline().split("=")[1])
+    detectable = 0
+    for pt in s.particles:
+        x, y = int(round(pt.x)), int(round(pt.y))
+        if z[y - 2:y + 3, x - 2:x + 3].max() < thr:
+            continue  # this copy's correlation peak is below the expected threshold at SNR 0.1
+        detectable += 1
+        assert any(abs(px - pt.x) <= 2 and abs(py - pt.y) <= 2 for px, py in peaks[:, :2]), (pt.x, pt.y)
+    print(f"{len(peaks)} peaks, {detectable} of {len(s.particles)} inserted copies above {thr}", flush=True)
+    assert detectable >= 8
+    hist = np.loadtxt(out + "_histogram.txt")
+    assert hist.shape == (512, 4) and hist[:, 1].sum() == n * 512 * 512
