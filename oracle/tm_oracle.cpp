@@ -9,6 +9,26 @@
 //   finalisation     match_template.cpp:1172-1243, 1275-1284 (master formula :1799)
 // with the Image/CTF/Curve/EulerSearch pieces restated in tm_oracle_image.h and a
 // project-written FFT (tm_oracle_fft.h). FP32 (the reference's arithmetic) and FP64 variants.
+//
+// How each part is pinned (tests/test_oracle_pins.py, tests/test_oracle_pipeline.py):
+//   FFT c2c / r2c / c2r layout   numpy.fft at 12 sizes incl. 2916 and 4096; console_test ForwardFFT
+//                                (constant, sine fixture) and BackwardFFT known answers; Parseval
+//   PhaseShift / quadrant swap   console_test PhaseShift(20,20) values; cross-correlation values and
+//                                integer peak (7,10) (console_test.cpp:644-688)
+//   ExtractSlice + trilinear     exact on a spectrum linear in (x,y,z) incl. the boundary-zero rule and
+//                                the Hermitian i = 0 column; projection-slice theorem at (0,90,0)
+//   Euler matrix                 orthonormality, det 1, closed-form rotations
+//   Euler grid / psi list        418 (10 deg) and 6620 (2.5 deg) positions, ring sizes, poles; 37/241 psi
+//   factorisation                the padded sizes of 2880, 3840, 2560, 3072, 1536, 4096, 512
+//   erfcinv                      scipy.special.erfcinv within the a = 0.147 approximation error
+//   CTF image                    closed-form -sin(chi) in float64
+//   whitening                    white noise gives a flat filter; unit spectrum energy after scaling
+//   projection normalisation     the reference's variance formula equals 1 afterwards; edge mean 0
+//   hot loop                     MIP, argmax (earliest wins), FP64 sums and histogram equal a
+//                                brute-force pass over individually computed CCF planes
+//   finalisation                 z-score equals the closed form from the sums
+//   end to end                   inserted copies peak at their positions (ground truth by construction)
+// No function of the oracle is left without a check.
 #include <atomic>
 #include <cstdint>
 #include <cstdio>

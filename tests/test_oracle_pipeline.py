@@ -1,0 +1,121 @@
+"""CPU: pins of the oracle's match_template pipeline by invariants, brute force and ground truth.
+
+* whitening (match_template.cpp:703-724): white noise in -> a flat filter out, and the whitened
+  spectrum has unit energy (Hermitian-weighted sum of squares 1)
+* projection normalisation (match_template.cpp:1060-1068): edge mean 0 and the reference's
+  variance formula gives exactly 1 after normalisation
+* hot loop (match_template.cpp:1100-1149): MIP, argmax (earliest orientation wins), FP64 sums and
+  histogram equal a brute-force pass over individually computed CCF planes
+* finalisation (match_template.cpp:1221-1243): z-score equals the closed form from the sums
+* ground truth: an inserted copy's CCF at the grid orientation nearest its true orientation peaks
+  at the copy's position
+"""
+import numpy as np
+import pytest
+
+from cistem_downstream_bah_amd import synth
+from oracle import oracle as O
+
+
+@pytest.fixture(scope="module")
+def c0():
+    s = synth.config0(0)
+    mt = O.MatchTemplateOracle(O.MTParams(angular_step=10.0, in_plane_angular_step=10.0, padding=1.0,
+                                          first_search_position=100, last_search_position=101), s.micrograph,
+                               s.volume)
+    return s, mt
+
+
+def test_whitening_of_white_noise_is_flat():
+    rng = np.random.default_rng(5)
+    img = rng.standard_normal((256, 256)).astype(np.float32)
+    vol = np.zeros((32, 32, 32), np.float32)
+    vol[16, 16, 16] = 1
+    mt = O.MatchTemplateOracle(O.MTParams(), img, vol)
+    x, y = mt.whitening_curve
+    band = (x > 0.05) & (x < 0.45)
+    # flat within the statistical scatter of a radially averaged white spectrum
+    assert y[band].min() > 0.6 and y[band].max() <= 1.0
+    assert np.std(y[band]) / np.mean(y[band]) < 0.12
+    # input spectrum normalised to unit energy (DivideByConstant(sqrt(ReturnSumOfSquares)))
+    s = mt.input_spectrum.astype(np.complex128)
+    w = np.ones(s.shape)
+    w[:, 1:-1] = 2
+    w[128:, 0] = 0      # explicit Hermitian mates are not independent terms
+    w[128, 0] = 1
+    assert abs((w * np.abs(s) ** 2).sum() - 1.0) < 2e-3
+
+
+def test_projection_normalisation_invariants(c0):
+    s, mt = c0
+    I = mt.info
+    for pos, k in [(100, 0), (101, 17)]:
+        p, _ = mt.projection_and_ccf(pos, k, 0, want_ccf=False)
+        edges = np.concatenate([p[0], p[-1], p[1:-1, 0], p[1:-1, -1]]).astype(np.float64)
+        nT, nP = I.T * I.T, I.W * I.H
+        ms = (p.astype(np.float64) ** 2).mean()
+        m = p.astype(np.float64).mean()
+        var = ms * nT / nP - (m * nT / nP) ** 2
+        assert abs(var - 1.0) < 1e-4
+        # the edge mean was removed before scaling: after scaling it is 0 up to float rounding
+        assert abs(edges.mean()) < 1e-5 * np.abs(p).max()
+
+
+def test_hot_loop_equals_brute_force(c0):
+    s, mt = c0
+    I = mt.info
+    res = mt.run(threads=4)
+    mip = np.full((I.H, I.W), -np.finfo(np.float32).max, np.float32)
+    idx = np.full((I.H, I.W), -1, np.int32)
+    s1 = np.zeros((I.H, I.W))
+    s2 = np.zeros((I.H, I.W))
+    hist = np.zeros(512, np.int64)
+    g = 0
+    for pos in range(I.first_pos, I.last_pos + 1):
+        for k in range(I.n_psi):
+            _, c = mt.projection_and_ccf(pos, k, 0)
+            better = c > mip                      # strict: the earliest orientation keeps ties
+            mip[better] = c[better]
+            idx[better] = g
+            s1 += c
+            s2 += (c * c).astype(np.float32)
+            d = (c.astype(np.float64) - I.histogram_min_scaled) / I.histogram_step_scaled
+            b = np.trunc(d).astype(np.int64)
+            ok = (b >= 0) & (b < 512)
+            hist += np.bincount(b[ok], minlength=512)
+            g += 1
+    assert res["nccf"] == g
+    assert np.array_equal(res["mip"], mip)
+    assert np.array_equal(res["best_index"], idx)
+    assert np.abs(res["sum"] - s1).max() <= 1e-12 * max(1.0, np.abs(s1).max()) + 1e-9
+    assert np.abs(res["sumsq"] - s2).max() <= 1e-12 * max(1.0, np.abs(s2).max()) + 1e-9
+    assert np.array_equal(res["hist"], hist)
+    # finalisation closed form (double sums, float scaling as match_template.cpp:1221-1243)
+    f = mt.finalise(g)
+    n = np.float32(g)
+    sq = np.float32(np.sqrt(I.W * I.H))
+    mean = s1 / n
+    var = s2 / n - mean.astype(np.float32).astype(np.float64) ** 2
+    sd = np.where(var > 0, np.sqrt(var.astype(np.float32)).astype(np.float32) * sq, 0.0)
+    mean = mean * sq
+    z = np.where(sd > 0, (mip * sq - mean) / np.where(sd > 0, sd, 1), 0.0)
+    assert np.abs(f["zscore"] - z).max() < 1e-3
+
+
+def test_inserted_copy_peaks_at_its_position():
+    s = synth.config0(0)
+    pt = s.particles[7]
+    mt = O.MatchTemplateOracle(O.MTParams(angular_step=10.0, in_plane_angular_step=10.0), s.micrograph, s.volume)
+    grid, psi = mt.grid, mt.psi
+    # nearest grid orientation by rotation angle
+    R0 = synth.euler_matrix(pt.phi, pt.theta, pt.psi)
+    best, arg = -1.0, None
+    for p in range(grid.shape[0]):
+        for k in range(psi.size):
+            R = synth.euler_matrix(float(grid[p, 0]), float(grid[p, 1]), float(psi[k]))
+            tr = np.trace(R.T @ R0)
+            if tr > best:
+                best, arg = tr, (p, k)
+    _, c = mt.projection_and_ccf(arg[0], arg[1], 0)
+    iy, ix = np.unravel_index(np.argmax(c), c.shape)
+    assert abs(ix - pt.x) <= 1.5 and abs(iy - pt.y) <= 1.5
