@@ -293,7 +293,7 @@ void launch_mid_t(tm_core* h, int nb) {
 }
 
 // Column-tile width of k_mid (and of the tiled Z layout) for image height H.
-int mid_kc(int H) { return H >= 4096 ? 4 : (H >= 2048 ? 8 : 16); }
+int mid_kc(int H) { return H >= 4096 ? 2 : (H >= 2048 ? 8 : 16); }
 int mid_tiles(const tm_core* h) { return (h->Wh + mid_kc(h->H) - 1) / mid_kc(h->H); }
 
 void launch_mid(tm_core* h, int nb) {
@@ -305,7 +305,7 @@ void launch_mid(tm_core* h, int nb) {
     case 4096: {
       const int ntiles = mid_tiles(h);
       const int grid = ((ntiles + 7) / 8) * 8;
-      hipLaunchKernelGGL((tmk::k_mid4096<4>), dim3(grid), dim3(1024), 0, h->stream, h->rows.p, h->inpT.p, h->Z.p,
+      hipLaunchKernelGGL((tmk::k_mid4096<2, true>), dim3(grid), dim3(2 * 256), 0, h->stream, h->rows.p, h->inpT.p, h->Z.p,
                          h->Wh, h->T, nb, ntiles, h->twiddle(4096));
       check_launch("k_mid4096");
       break;

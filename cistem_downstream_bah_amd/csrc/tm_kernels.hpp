@@ -359,23 +359,25 @@ __global__ __launch_bounds__(KC* NTC) void k_mid(const cf* __restrict__ rows, co
 }
 
 // Column pass for H = 4096 with the data register-resident between the three radix-16 stages:
-// 1024 threads = 256 butterflies x 4 columns, lane -> (column c = tid % 4, butterfly j = tid / 4).
+// KC * 256 threads = 256 butterflies x KC columns, lane -> (column c = tid % KC, butterfly j = tid / KC).
+// With KC = 2 a block needs ~74 KB of LDS, so two blocks share a CU and hide each other's barrier
+// and load latency.
 //  * forward stage 1 reads its 16 inputs x[j + 256 q] straight from the forward rows (zero outside
 //    the T projection rows), so no LDS zero-fill or scatter is needed;
 //  * forward stage 3, the DC zero, conj(R) * I and inverse stage 1 run on registers;
-//  * inverse stage 3 leaves z[j + 256 q] in registers and a wave stores 16 rows x 4 columns = 512
-//    contiguous bytes of the tiled Z per q.
+//  * inverse stage 3 leaves z[j + 256 q] in registers and a wave stores 64 / KC rows x KC columns
+//    = 512 contiguous bytes of the tiled Z per q.
 // Eight LDS passes per column and orientation (four exchanges, each a write and a read). With the
 // padded LDS index lpad(i) = i + i/16 the three access patterns are a per-thread base plus
 // compile-time offsets: lpad(16 j + q) = 17 j + q, lpad(j + 256 q) = lpad(j) + 272 q and
 // lpad(b2 + 16 q) = lpad(b2) + 17 q for b2 = (j / 16) 256 + j % 16.
-template <int KC>
-__global__ __launch_bounds__(1024) void k_mid4096(const cf* __restrict__ rows, const cf* __restrict__ inpT,
+template <int KC, bool PREFETCH>
+__global__ __launch_bounds__(KC * 256, 4) void k_mid4096(const cf* __restrict__ rows, const cf* __restrict__ inpT,
                                                   cf* __restrict__ Z, int Wh, int T, int nb, int ntiles,
                                                   const cf* __restrict__ tw) {
   constexpr int H = 4096, M = 256;
   constexpr int CS = lds_len(H) + 8;  // column stride: shifts the columns by 16 banks
-  static_assert(KC * M == 1024, "1024 threads cover KC columns of 256 butterflies");
+  static_assert(KC * M <= 1024, "KC columns of 256 butterflies per block");
   __shared__ cf lds[KC * CS];
   // Twiddles from LDS: W4096^(16 m) = W256^m and W4096^(b q) for b, q < 16; the stage-3 twiddle
   // W4096^(j q) with j = 16 a + b is W256^(a q) * W4096^(b q).
@@ -435,10 +437,13 @@ __global__ __launch_bounds__(1024) void k_mid4096(const cf* __restrict__ rows, c
 #pragma unroll
     for (int q = 0; q < 16; ++q) p_s2[17 * q] = v[q];
     __syncthreads();
-    // input-spectrum column prefetched before forward stage 3 so its latency overlaps the stage
+    // input-spectrum column, optionally prefetched before forward stage 3 so that its latency
+    // overlaps the stage (costs 32 registers)
     cf iv[16];
+    if constexpr (PREFETCH) {
 #pragma unroll
-    for (int q = 0; q < 16; ++q) iv[q] = valid ? ic[M * q] : cmake(0.f, 0.f);
+      for (int q = 0; q < 16; ++q) iv[q] = valid ? ic[M * q] : cmake(0.f, 0.f);
+    }
     // forward stage 3 (Ns = 256): X[j + 256 q] in registers
 #pragma unroll
     for (int q = 0; q < 16; ++q) v[q] = p_rd[272 * q];
@@ -449,6 +454,7 @@ __global__ __launch_bounds__(1024) void k_mid4096(const cf* __restrict__ rows, c
     // ZeroCentralPixel and conj(reference) * input
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
+      if constexpr (!PREFETCH) iv[q] = valid ? ic[M * q] : cmake(0.f, 0.f);
       const cf x = (kx == 0 && j == 0 && q == 0) ? cmake(0.f, 0.f) : v[q];
       v[q] = cconjmul(x, iv[q]);
     }
