@@ -292,9 +292,13 @@ void launch_mid_t(tm_core* h, int nb) {
   check_launch("k_mid");
 }
 
-// Column-tile width of k_mid (and of the tiled Z layout) for image height H.
+// Columns per k_mid block for image height H, and the column-tile width of the Z layout
+// Z[b][tile][y][z_tile]. At H = 4096 two 2-column blocks fill one 4-column tile, so that a row
+// read of k_last takes 32 bytes from each cache line instead of 16.
 int mid_kc(int H) { return H >= 4096 ? 2 : (H >= 2048 ? 8 : 16); }
 int mid_tiles(const tm_core* h) { return (h->Wh + mid_kc(h->H) - 1) / mid_kc(h->H); }
+int z_tile(int H) { return H >= 4096 ? 4 : mid_kc(H); }
+int z_tiles(const tm_core* h) { return (h->Wh + z_tile(h->H) - 1) / z_tile(h->H); }
 
 void launch_mid(tm_core* h, int nb) {
   switch (h->H) {
@@ -305,7 +309,7 @@ void launch_mid(tm_core* h, int nb) {
     case 4096: {
       const int ntiles = mid_tiles(h);
       const int grid = ((ntiles + 7) / 8) * 8;
-      hipLaunchKernelGGL((tmk::k_mid4096<2, true>), dim3(grid), dim3(2 * 256), 0, h->stream, h->rows.p, h->inpT.p, h->Z.p,
+      hipLaunchKernelGGL((tmk::k_mid4096<2, true, 4, false>), dim3(grid), dim3(2 * 256), 0, h->stream, h->rows.p, h->inpT.p, h->Z.p,
                          h->Wh, h->T, nb, ntiles, h->twiddle(4096));
       check_launch("k_mid4096");
       break;
@@ -314,23 +318,24 @@ void launch_mid(tm_core* h, int nb) {
   }
 }
 
-template <int WW, int NT, int QR>
+// QR orientations per k_last block; blocks have W/16 threads per orientation.
+template <int WW, int QR>
 void launch_last_t(tm_core* h, int nb, tmk::EpiParams ep) {
-  ep.ztile = mid_kc(h->H);
-  ep.ntiles = mid_tiles(h);
+  ep.ztile = z_tile(h->H);
+  ep.ntiles = z_tiles(h);
   ep.hstep_inv = 1.0 / ep.hstep;
-  hipLaunchKernelGGL((tmk::k_last<WW, NT, QR>), dim3(h->H), dim3(QR * NT), 0, h->stream, h->Z.p, h->H, nb, ep,
+  hipLaunchKernelGGL((tmk::k_last<WW, QR>), dim3(h->H), dim3(QR * (WW / 16)), 0, h->stream, h->Z.p, h->H, nb, ep,
                      h->twiddle(WW), h->twiddle(WW / 2));
   check_launch("k_last");
 }
 
 void launch_last(tm_core* h, int nb, const tmk::EpiParams& ep) {
   switch (h->W) {
-    case 256: launch_last_t<256, 32, 2>(h, nb, ep); break;
-    case 512: launch_last_t<512, 64, 2>(h, nb, ep); break;
-    case 1024: launch_last_t<1024, 128, 2>(h, nb, ep); break;
-    case 2048: launch_last_t<2048, 256, 2>(h, nb, ep); break;
-    case 4096: launch_last_t<4096, 256, 4>(h, nb, ep); break;
+    case 256: launch_last_t<256, 8>(h, nb, ep); break;
+    case 512: launch_last_t<512, 8>(h, nb, ep); break;
+    case 1024: launch_last_t<1024, 4>(h, nb, ep); break;
+    case 2048: launch_last_t<2048, 4>(h, nb, ep); break;
+    case 4096: launch_last_t<4096, 2>(h, nb, ep); break;
     default: throw TmError{TM_ERR_UNSUPPORTED, "unsupported image width " + std::to_string(h->W)};
   }
 }
@@ -568,7 +573,7 @@ int tm_init(tm_core* h, const tm_init_params* p, const float* template_spectrum,
     h->slice.alloc((size_t)B * P * h->Ph);
     h->projT.alloc((size_t)B * T * h->Th);
     h->rows.alloc((size_t)B * T * h->Wh);
-    h->Z.alloc((size_t)B * mid_tiles(h) * H * mid_kc(H));
+    h->Z.alloc((size_t)B * z_tiles(h) * H * z_tile(H));
 
     // state
     const size_t npx = (size_t)W * H;

@@ -371,7 +371,7 @@ __global__ __launch_bounds__(KC* NTC) void k_mid(const cf* __restrict__ rows, co
 // padded LDS index lpad(i) = i + i/16 the three access patterns are a per-thread base plus
 // compile-time offsets: lpad(16 j + q) = 17 j + q, lpad(j + 256 q) = lpad(j) + 272 q and
 // lpad(b2 + 16 q) = lpad(b2) + 17 q for b2 = (j / 16) 256 + j % 16.
-template <int KC, bool PREFETCH>
+template <int KC, bool PREFETCH, int ZT = KC, bool NTS = true>
 __global__ __launch_bounds__(KC * 256, 4) void k_mid4096(const cf* __restrict__ rows, const cf* __restrict__ inpT,
                                                   cf* __restrict__ Z, int Wh, int T, int nb, int ntiles,
                                                   const cf* __restrict__ tw) {
@@ -397,7 +397,8 @@ __global__ __launch_bounds__(KC * 256, 4) void k_mid4096(const cf* __restrict__ 
   const int kx = tile * KC + c;
   const bool valid = kx < Wh;
   const int jlo = H / 2 - T / 2;
-  const size_t z_bs = (size_t)ntiles * H * KC;
+  static_assert(ZT % KC == 0, "Z tiles hold whole column groups");
+  const size_t z_bs = (size_t)((Wh + ZT - 1) / ZT) * H * ZT;
   for (int b = 0; b < nb; ++b) {
     // Indices are re-derived every iteration from a register the compiler cannot see through, so
     // the address and twiddle computations are not hoisted out of the loop (they would spill).
@@ -481,13 +482,14 @@ __global__ __launch_bounds__(KC * 256, 4) void k_mid4096(const cf* __restrict__ 
     for (int q = 1; q < 16; ++q)
       v[q] = cmul(v[q], twiddle<+1>(tw256, (j / 16) * q), twiddle<+1>(twbq, k2 * 16 + q));
     dft16<+1>(v);
-    unsigned long long* zt = reinterpret_cast<unsigned long long*>(Z + b * z_bs + (size_t)tile * H * KC) +
-                             (size_t)j * KC + c;
+    unsigned long long* zt = reinterpret_cast<unsigned long long*>(Z + b * z_bs + (size_t)(kx / ZT) * H * ZT) +
+                             (size_t)j * ZT + kx % ZT;
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       unsigned long long bits;
       __builtin_memcpy(&bits, &v[q], sizeof(bits));
-      __builtin_nontemporal_store(bits, zt + (size_t)M * KC * q);
+      if constexpr (NTS) __builtin_nontemporal_store(bits, zt + (size_t)M * ZT * q);
+      else zt[(size_t)M * ZT * q] = bits;
     }
     __syncthreads();  // LDS reads of this orientation complete before the next one writes
   }
@@ -507,70 +509,43 @@ struct EpiParams {
   int ntiles;  // column tiles per orientation
 };
 
-// Loads row y of orientation b from the tiled Z into LDS and forms the N/2-point complex input of
-// the C2R: Z_k = (X_k + conj X_{h-k}) + i (X_k - conj X_{h-k}) conj(W_N^k).
-template <int W, int NT>
-__device__ __forceinline__ void last_load_pre(const cf* __restrict__ Z, int H, int b, int y, const EpiParams& ep,
-                                              cf* buf, const cf* __restrict__ tw_n, int t, bool active) {
-  constexpr int h = W / 2;
-  if (active) {
-    const int sh = __ffs(ep.ztile) - 1;
-    const cf* zb = Z + (size_t)b * ep.ntiles * H * ep.ztile;
-    for (int e = t; e <= h; e += NT)
-      buf[lpad(e)] = zb[((size_t)(e >> sh) * H + y) * ep.ztile + (e & (ep.ztile - 1))];
-  }
-  __syncthreads();
-  if (active) {
-    for (int k = t; k <= h / 2; k += NT) {
-      if (k == 0) {
-        const float x0 = buf[lpad(0)].x, xh = buf[lpad(h)].x;
-        buf[lpad(0)] = cmake(x0 + xh, x0 - xh);
-      } else {
-        const int kk = h - k;
-        const cf Xk = buf[lpad(k)], Xc = cconj(buf[lpad(kk)]);
-        const cf E = cadd(Xk, Xc);
-        const cf D = csub(Xk, Xc);
-        const cf w = tw_n[k];
-        const cf O = cmul(D, cconj(w));
-        const cf Ok = cmul(cconj(D), w);
-        buf[lpad(k)] = cmake(E.x - O.y, E.y + O.x);
-        buf[lpad(kk)] = cmake(E.x - Ok.y, -E.y + Ok.x);
-      }
-    }
-  }
-  __syncthreads();
-}
-
-// One block of QR*NT threads per image row y; QR groups of NT threads transform QR orientations of
-// the batch at a time (C2R of row y of Z into W reals), then every thread updates its 2*PAIRS pixels
-// for those orientations in batch order: strict-greater MIP/argmax, histogram bin
+// Row pass + epilogue. One block per image row y; QR groups of NT = W/16 threads transform QR
+// orientations of the batch at a time: C2R of row y of Z into W reals (an h = W/2 point complex
+// inverse FFT after the even/odd pre-twist), then every thread updates its 2*PAIRS pixels for
+// those orientations in batch order: strict-greater MIP/argmax, histogram bin
 // int((v - min) / step) over 0..nbins-1, FP64 sum += v and FP64 sum-of-squares += (v*v in float)
-// (match_template.cpp:1100-1149). Rows are assigned so that consecutive rows run on the same XCD and
-// share Z's cache lines in L2.
-template <int W, int NT, int QR>
-__global__ __launch_bounds__(QR * NT, 4) void k_last(const cf* __restrict__ Z, int H, int nb, EpiParams ep,
-                                                     const cf* __restrict__ tw_n, const cf* __restrict__ tw_h) {
+// (match_template.cpp:1100-1149). The per-pixel state stays in registers for the whole batch.
+// Rows are assigned so that consecutive rows run on the same XCD and share Z's cache lines in L2.
+// Twiddles live in LDS; the block is sized so that two blocks share a CU and overlap each other's
+// Z gathers.
+template <int W, int QR>
+__global__ __launch_bounds__(QR * (W / 16)) __attribute__((amdgpu_waves_per_eu(4)))
+void k_last(const cf* __restrict__ Z, int H, int nb, EpiParams ep, const cf* __restrict__ tw_n,
+            const cf* __restrict__ tw_h) {
   constexpr int h = W / 2;
+  constexpr int R1 = plan_radix(h, 0, 3);
+  constexpr int NT = h / R1;  // one first-stage butterfly per thread
   constexpr int BT = QR * NT;
   constexpr int PAIRS = h / BT;
-  static_assert(h % BT == 0, "row length must be a multiple of 2*QR*NT");
+  static_assert(R1 == 8 && h % BT == 0, "row length must be a multiple of 16*QR");
+  constexpr int HC = 4;  // interleaved histogram copies (bin * HC + lane % HC) spread same-bin atomics
   __shared__ cf buf[QR][lds_len(h + 1)];
-  // 8 interleaved copies of the histogram (bin * 8 + lane % 8) spread same-bin atomics of a wave
-  constexpr int HC = 8;
+  __shared__ cf twh[h];
+  __shared__ cf twn[h / 2 + 1];
   __shared__ unsigned int hist[512 * HC];
-  const int tid = threadIdx.x;
-  const int half = tid / NT, t = tid % NT;
+  const int tid0 = threadIdx.x;
   const int bid = blockIdx.x;
   const int y = (bid % 8) * (H / 8) + bid / 8;
-  for (int i = tid; i < 512 * HC; i += BT) hist[i] = 0u;
-  const int hc = tid % HC;
+  for (int i = tid0; i < 512 * HC; i += BT) hist[i] = 0u;
+  for (int i = tid0; i < h; i += BT) twh[i] = tw_h[i];
+  for (int i = tid0; i <= h / 2; i += BT) twn[i] = tw_n[i];
   float mip[2 * PAIRS];
   int idx[2 * PAIRS];
   double s1[2 * PAIRS], s2[2 * PAIRS];
   const size_t rowoff = (size_t)y * W;
 #pragma unroll
   for (int u = 0; u < PAIRS; ++u) {
-    const int m = tid + u * BT;
+    const int m = tid0 + u * BT;
     const float2 a = *reinterpret_cast<const float2*>(ep.mip + rowoff + 2 * m);
     const int2 ia = *reinterpret_cast<const int2*>(ep.idx + rowoff + 2 * m);
     const double2 sa = *reinterpret_cast<const double2*>(ep.sum + rowoff + 2 * m);
@@ -580,10 +555,60 @@ __global__ __launch_bounds__(QR * NT, 4) void k_last(const cf* __restrict__ Z, i
     s1[2 * u] = sa.x; s1[2 * u + 1] = sa.y;
     s2[2 * u] = qa.x; s2[2 * u + 1] = qa.y;
   }
+  const int sh = __ffs(ep.ztile) - 1;
+  const size_t z_bs = (size_t)ep.ntiles * H * ep.ztile;
   for (int b0 = 0; b0 < nb; b0 += QR) {
+    // re-derived every iteration from a register the compiler cannot see through, so address
+    // arithmetic is not hoisted out of the loop into registers (it would spill)
+    int tid = tid0;
+    asm volatile("" : "+v"(tid));
+    const int grp = tid / NT, t = tid % NT;
     const int nbp = min(QR, nb - b0);
-    last_load_pre<W, NT>(Z, H, b0 + half, y, ep, buf[half], tw_n, t, half < nbp);
-    fft_inplace<h, +1, NT, 3>(buf[half], tw_h, t);
+    const bool active = grp < nbp;
+    cf* const bb = buf[grp];
+    // gather row y of orientation b0 + grp (elements 0..h) from the column tiles
+    if (active) {
+      const cf* zb = Z + (size_t)(b0 + grp) * z_bs + (size_t)y * ep.ztile;
+      for (int e = t; e <= h; e += NT) bb[lpad(e)] = zb[(size_t)(e >> sh) * H * ep.ztile + (e & (ep.ztile - 1))];
+    }
+    __syncthreads();
+    // first radix-8 stage (Ns = 1) with the real-to-complex pre-twist folded in:
+    // Z_k = (X_k + conj X_{h-k}) + i (X_k - conj X_{h-k}) conj(W_W^k), with W_W^k = -conj(W_W^{h-k})
+    // for k > h/2, and Z_0 = (Re X_0 + Re X_h) + i (Re X_0 - Re X_h)
+    cf v[R1];
+    if (active) {
+#pragma unroll
+      for (int q = 0; q < R1; ++q) {
+        const int k = t + NT * q;
+        const cf Xk = bb[lpad(k)];
+        const cf Xm = bb[lpad(h - k)];
+        if (k == 0) {
+          v[q] = cmake(Xk.x + Xm.x, Xk.x - Xm.x);
+        } else {
+          const cf Xc = cconj(Xm);
+          const cf E = cadd(Xk, Xc);
+          const cf D = csub(Xk, Xc);
+          cf w;
+          if (k <= h / 2) {
+            w = twn[k];
+          } else {
+            const cf w2 = twn[h - k];
+            w = cmake(-w2.x, w2.y);
+          }
+          const cf O = cmul(D, cconj(w));
+          v[q] = cmake(E.x - O.y, E.y + O.x);
+        }
+      }
+    }
+    __syncthreads();
+    if (active) {
+      dftR<R1, +1>(v);
+#pragma unroll
+      for (int q = 0; q < R1; ++q) bb[lpad(t * R1 + q)] = v[q];
+    }
+    __syncthreads();
+    // remaining stages (the groups past nbp run them on stale data; their result is unused)
+    fft_inplace<h, +1, NT, 3, 1>(bb, twh, t);
 #pragma unroll 1
     for (int q = 0; q < nbp; ++q) {
       const int g = ep.g0 + b0 + q;
@@ -593,13 +618,13 @@ __global__ __launch_bounds__(QR * NT, 4) void k_last(const cf* __restrict__ Z, i
         const float vv[2] = {z.x, z.y};
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
-          const float v = vv[c];
+          const float vx = vv[c];
           const int p = 2 * u + c;
-          if (v > mip[p]) { mip[p] = v; idx[p] = g; }
-          const double d = ((double)v - ep.hmin) * ep.hstep_inv;
-          if (d > -1.0 && d < (double)ep.nbins) atomicAdd(&hist[(int)d * HC + hc], 1u);
-          s1[p] += (double)v;
-          const float sq = v * v;
+          if (vx > mip[p]) { mip[p] = vx; idx[p] = g; }
+          const double d = ((double)vx - ep.hmin) * ep.hstep_inv;
+          if (d > -1.0 && d < (double)ep.nbins) atomicAdd(&hist[(int)d * HC + tid % HC], 1u);
+          s1[p] += (double)vx;
+          const float sq = vx * vx;
           s2[p] += (double)sq;
         }
       }
@@ -608,13 +633,13 @@ __global__ __launch_bounds__(QR * NT, 4) void k_last(const cf* __restrict__ Z, i
   }
 #pragma unroll
   for (int u = 0; u < PAIRS; ++u) {
-    const int m = tid + u * BT;
+    const int m = tid0 + u * BT;
     *reinterpret_cast<float2*>(ep.mip + rowoff + 2 * m) = make_float2(mip[2 * u], mip[2 * u + 1]);
     *reinterpret_cast<int2*>(ep.idx + rowoff + 2 * m) = make_int2(idx[2 * u], idx[2 * u + 1]);
     *reinterpret_cast<double2*>(ep.sum + rowoff + 2 * m) = make_double2(s1[2 * u], s1[2 * u + 1]);
     *reinterpret_cast<double2*>(ep.sumsq + rowoff + 2 * m) = make_double2(s2[2 * u], s2[2 * u + 1]);
   }
-  for (int i = tid; i < ep.nbins; i += BT) {
+  for (int i = tid0; i < ep.nbins; i += BT) {
     unsigned long long n = 0;
 #pragma unroll
     for (int c = 0; c < HC; ++c) n += hist[i * HC + c];
