@@ -318,14 +318,20 @@ void launch_mid(tm_core* h, int nb) {
   }
 }
 
-// QR orientations per k_last block; blocks have W/16 threads per orientation.
+// QR orientations per k_last block; blocks have W/16 threads per orientation. From W = 1024 on
+// the groups are whole wavefronts and the row gather runs as double-buffered direct LDS loads.
 template <int WW, int QR>
 void launch_last_t(tm_core* h, int nb, tmk::EpiParams ep) {
   ep.ztile = z_tile(h->H);
   ep.ntiles = z_tiles(h);
   ep.hstep_inv = 1.0 / ep.hstep;
-  hipLaunchKernelGGL((tmk::k_last<WW, QR>), dim3(h->H), dim3(QR * (WW / 16)), 0, h->stream, h->Z.p, h->H, nb, ep,
-                     h->twiddle(WW), h->twiddle(WW / 2));
+  if constexpr (WW >= 1024) {
+    hipLaunchKernelGGL((tmk::k_last_dma<WW, QR>), dim3(h->H), dim3(QR * (WW / 16)), 0, h->stream, h->Z.p, h->H, nb,
+                       ep, h->twiddle(WW), h->twiddle(WW / 2));
+  } else {
+    hipLaunchKernelGGL((tmk::k_last<WW, QR>), dim3(h->H), dim3(QR * (WW / 16)), 0, h->stream, h->Z.p, h->H, nb, ep,
+                       h->twiddle(WW), h->twiddle(WW / 2));
+  }
   check_launch("k_last");
 }
 

@@ -101,10 +101,45 @@ __device__ __forceinline__ void dftR(cf* v) {
   else static_assert(R == 2 || R == 4 || R == 8 || R == 16, "unsupported radix");
 }
 
+// Workgroup barrier that orders LDS traffic only: this wave's LDS operations complete, then
+// s_barrier. __syncthreads() also waits for every outstanding global load and store of the wave
+// (vmcnt 0), which would stall on Z stores or prefetches that the barrier does not need to order.
+// The memory clobber keeps the compiler from moving memory accesses across it.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// Barrier after direct global-to-LDS loads: they count as vector-memory operations (vmcnt), so
+// each wave waits for its own before the barrier makes all of them visible to the workgroup.
+__device__ __forceinline__ void lds_dma_barrier() {
+  asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// LDS atomic increment without return (ds_add_u32). Written out because the compiler's wait
+// insertion cannot tell an atomic apart from a pending direct global-to-LDS load and would wait for
+// every such load (vmcnt 0) before each histogram update. The instruction has no output register;
+// LDS operations complete in order, so the compiler's own LDS waits stay correct, and an
+// lds_barrier() completes it before the counts are read.
+__device__ __forceinline__ void lds_inc(unsigned int* p) {
+  const unsigned addr = (unsigned)(size_t)((__attribute__((address_space(3))) unsigned int*)p);
+  asm volatile("ds_add_u32 %0, %1" ::"v"(addr), "v"(1u) : "memory");
+}
+
 // twiddle W^(idx) from the table (forward table); conjugated for the inverse.
 template <int SIGN>
 __device__ __forceinline__ cf twiddle(const cf* __restrict__ tw, int idx) {
   cf w = tw[idx];
+  return SIGN < 0 ? w : cconj(w);
+}
+
+// Factored twiddle table: W^m = lo[m mod 64] * hi[m / 64], with lo[b] = W^b and hi[a] = W^(64 a).
+// Two small tables replace one of length N (for tables kept in LDS).
+struct TwSplit {
+  const cf* lo;
+  const cf* hi;
+};
+template <int SIGN>
+__device__ __forceinline__ cf twiddle(const TwSplit& t, int idx) {
+  const cf a = t.lo[idx & 63], b = t.hi[idx >> 6];
+  const cf w = cmake(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
   return SIGN < 0 ? w : cconj(w);
 }
 

@@ -33,8 +33,8 @@ __host__ __device__ constexpr int lds_len(int n) { return n + n / 16; }
 // In-place Stockham FFT of one line in LDS (padded index), executed by NT threads (tid < NT)
 // of a block in which every thread reaches the barriers. Each stage loads all butterfly inputs
 // into registers, synchronises, then stores.
-template <int N, int SIGN, int NT, int LR = 4, int S = 0>
-__device__ __forceinline__ void fft_inplace(cf* buf, const cf* __restrict__ tw, int tid) {
+template <int N, int SIGN, int NT, int LR = 4, int S = 0, typename TW = const cf*>
+__device__ __forceinline__ void fft_inplace(cf* buf, TW tw, int tid) {
   if constexpr (S < plan_stages(N, LR)) {
     constexpr int R = plan_radix(N, S, LR);
     constexpr int NS = plan_ns(N, S, LR);
@@ -49,7 +49,7 @@ __device__ __forceinline__ void fft_inplace(cf* buf, const cf* __restrict__ tw, 
         for (int q = 0; q < R; ++q) v[c][q] = buf[lpad(j + q * M)];
       }
     }
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int c = 0; c < CNT; ++c) {
       const int j = tid + c * NT;
@@ -66,8 +66,8 @@ __device__ __forceinline__ void fft_inplace(cf* buf, const cf* __restrict__ tw, 
         for (int q = 0; q < R; ++q) buf[lpad(base + q * NS)] = v[c][q];
       }
     }
-    __syncthreads();
-    fft_inplace<N, SIGN, NT, LR, S + 1>(buf, tw, tid);
+    lds_barrier();
+    fft_inplace<N, SIGN, NT, LR, S + 1, TW>(buf, tw, tid);
   }
 }
 
@@ -631,6 +631,173 @@ void k_last(const cf* __restrict__ Z, int H, int nb, EpiParams ep, const cf* __r
     }
     __syncthreads();
   }
+#pragma unroll
+  for (int u = 0; u < PAIRS; ++u) {
+    const int m = tid0 + u * BT;
+    *reinterpret_cast<float2*>(ep.mip + rowoff + 2 * m) = make_float2(mip[2 * u], mip[2 * u + 1]);
+    *reinterpret_cast<int2*>(ep.idx + rowoff + 2 * m) = make_int2(idx[2 * u], idx[2 * u + 1]);
+    *reinterpret_cast<double2*>(ep.sum + rowoff + 2 * m) = make_double2(s1[2 * u], s1[2 * u + 1]);
+    *reinterpret_cast<double2*>(ep.sumsq + rowoff + 2 * m) = make_double2(s2[2 * u], s2[2 * u + 1]);
+  }
+  for (int i = tid0; i < ep.nbins; i += BT) {
+    unsigned long long n = 0;
+#pragma unroll
+    for (int c = 0; c < HC; ++c) n += hist[i * HC + c];
+    if (n) atomicAdd(&ep.hist[i], n);
+  }
+}
+
+typedef __attribute__((address_space(3))) void* lds_vptr;
+typedef const __attribute__((address_space(1))) void* global_vptr;
+
+// k_last with the Z gather double-buffered through direct global-to-LDS loads
+// (global_load_lds_dwordx4, 16 bytes = two adjacent columns per lane): while the block transforms
+// and accumulates orientations b0..b0+QR-1, row y of the next QR orientations streams into a second
+// LDS buffer. Arithmetic and pixel ownership are those of k_last; the pre-twist and FFT twiddles come
+// from factored tables so that two blocks still fit in one CU's LDS. Each group of NT = W/16 threads
+// must be whole wavefronts (W >= 1024), because a direct load writes one wavefront's 64 lanes to
+// consecutive LDS addresses.
+template <int W, int QR>
+__global__ __launch_bounds__(QR * (W / 16)) __attribute__((amdgpu_waves_per_eu(4)))
+void k_last_dma(const cf* __restrict__ Z, int H, int nb, EpiParams ep, const cf* __restrict__ tw_n,
+                const cf* __restrict__ tw_h) {
+  constexpr int h = W / 2;
+  constexpr int R1 = plan_radix(h, 0, 3);
+  constexpr int NT = h / R1;
+  constexpr int BT = QR * NT;
+  constexpr int PAIRS = h / BT;
+  constexpr int NL = h / 2 + 1;            // 16-byte loads per row: elements 0..h plus one pad element
+  constexpr int NR = (NL + NT - 1) / NT;   // load rounds per thread
+  static_assert(R1 == 8 && h % BT == 0 && NT % 64 == 0, "groups of whole wavefronts, one butterfly each");
+  constexpr int HC = 4;
+  __shared__ cf gbuf[QR][2 * NL];          // gathered rows in element order
+  __shared__ cf buf[QR][lds_len(h)];       // FFT work buffers (padded index)
+  __shared__ cf fft_lo[64], fft_hi[h / 64];
+  __shared__ cf pre_lo[64], pre_hi[h / 128 + 1];
+  __shared__ unsigned int hist[512 * HC];
+  const int tid0 = threadIdx.x;
+  const int bid = blockIdx.x;
+  const int y = (bid % 8) * (H / 8) + bid / 8;
+  const int sh = __ffs(ep.ztile) - 1;
+  const size_t z_bs = (size_t)ep.ntiles * H * ep.ztile;
+  const size_t tile_stride = (size_t)H * ep.ztile;
+  const cf* const zrow = Z + (size_t)y * ep.ztile;
+  // start the loads of row y of orientations b..b+QR-1 (group grp takes orientation b + grp)
+  auto gather = [&](int b, int tid) {
+    const int grp = tid / NT, t = tid % NT;
+    if (b + grp < nb) {
+      const cf* zb = zrow + (size_t)(b + grp) * z_bs;
+      cf* const dst = gbuf[grp] + 2 * (t & ~63);
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        const int L = r * NT + t;
+        if (L < NL) {
+          const int e = 2 * L;
+          const cf* src = zb + (size_t)(e >> sh) * tile_stride + (e & (ep.ztile - 1));
+          __builtin_amdgcn_global_load_lds((global_vptr)src, (lds_vptr)(dst + 2 * r * NT), 16, 0, 0);
+        }
+      }
+    }
+  };
+  gather(0, tid0);
+  for (int i = tid0; i < 512 * HC; i += BT) hist[i] = 0u;
+  for (int i = tid0; i < 64; i += BT) {
+    fft_lo[i] = tw_h[i];
+    pre_lo[i] = tw_n[i];
+  }
+  for (int i = tid0; i < h / 64; i += BT) fft_hi[i] = tw_h[64 * i];
+  for (int i = tid0; i <= h / 128; i += BT) pre_hi[i] = tw_n[64 * i];
+  float mip[2 * PAIRS];
+  int idx[2 * PAIRS];
+  double s1[2 * PAIRS], s2[2 * PAIRS];
+  const size_t rowoff = (size_t)y * W;
+#pragma unroll
+  for (int u = 0; u < PAIRS; ++u) {
+    const int m = tid0 + u * BT;
+    const float2 a = *reinterpret_cast<const float2*>(ep.mip + rowoff + 2 * m);
+    const int2 ia = *reinterpret_cast<const int2*>(ep.idx + rowoff + 2 * m);
+    const double2 sa = *reinterpret_cast<const double2*>(ep.sum + rowoff + 2 * m);
+    const double2 qa = *reinterpret_cast<const double2*>(ep.sumsq + rowoff + 2 * m);
+    mip[2 * u] = a.x; mip[2 * u + 1] = a.y;
+    idx[2 * u] = ia.x; idx[2 * u + 1] = ia.y;
+    s1[2 * u] = sa.x; s1[2 * u + 1] = sa.y;
+    s2[2 * u] = qa.x; s2[2 * u + 1] = qa.y;
+  }
+  const TwSplit ftw{fft_lo, fft_hi};
+  const TwSplit ptw{pre_lo, pre_hi};
+  for (int b0 = 0; b0 < nb; b0 += QR) {
+    // re-derived every iteration from a register the compiler cannot see through, so address
+    // arithmetic is not hoisted out of the loop into registers (it would spill)
+    int tid = tid0;
+    asm volatile("" : "+v"(tid));
+    const int grp = tid / NT, t = tid % NT;
+    const int nbp = min(QR, nb - b0);
+    const bool active = grp < nbp;
+    // this wave's direct loads have landed (vmcnt 0); the barrier makes every wave's visible
+    lds_dma_barrier();
+    // first radix-8 stage (Ns = 1) with the real-to-complex pre-twist folded in:
+    // Z_k = (X_k + conj X_{h-k}) + i (X_k - conj X_{h-k}) conj(W_W^k), with W_W^k = -conj(W_W^{h-k})
+    // for k > h/2, and Z_0 = (Re X_0 + Re X_h) + i (Re X_0 - Re X_h)
+    cf v[R1];
+    if (active) {
+      const cf* gb = gbuf[grp];
+#pragma unroll
+      for (int q = 0; q < R1; ++q) {
+        const int k = t + NT * q;
+        const cf Xk = gb[k];
+        const cf Xm = gb[h - k];
+        if (k == 0) {
+          v[q] = cmake(Xk.x + Xm.x, Xk.x - Xm.x);
+        } else {
+          const cf Xc = cconj(Xm);
+          const cf E = cadd(Xk, Xc);
+          const cf D = csub(Xk, Xc);
+          cf w;
+          if (k <= h / 2) {
+            w = twiddle<-1>(ptw, k);
+          } else {
+            const cf w2 = twiddle<-1>(ptw, h - k);
+            w = cmake(-w2.x, w2.y);
+          }
+          const cf O = cmul(D, cconj(w));
+          v[q] = cmake(E.x - O.y, E.y + O.x);
+        }
+      }
+    }
+    lds_barrier();  // every read of gbuf is done: the next orientations may overwrite it
+    gather(b0 + QR, tid);
+    if (active) {
+      dftR<R1, +1>(v);
+#pragma unroll
+      for (int q = 0; q < R1; ++q) buf[grp][lpad(t * R1 + q)] = v[q];
+    }
+    lds_barrier();
+    // remaining stages (groups past nbp run them on stale data; their result is unused)
+    fft_inplace<h, +1, NT, 3, 1>(buf[grp], ftw, t);
+#pragma unroll 1
+    for (int q = 0; q < nbp; ++q) {
+      const int g = ep.g0 + b0 + q;
+#pragma unroll
+      for (int u = 0; u < PAIRS; ++u) {
+        const cf z = buf[q][lpad(tid + u * BT)];
+        const float vv[2] = {z.x, z.y};
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          const float vx = vv[c];
+          const int p = 2 * u + c;
+          if (vx > mip[p]) { mip[p] = vx; idx[p] = g; }
+          const double d = ((double)vx - ep.hmin) * ep.hstep_inv;
+          if (d > -1.0 && d < (double)ep.nbins) lds_inc(&hist[(int)d * HC + tid % HC]);
+          s1[p] += (double)vx;
+          const float sq = vx * vx;
+          s2[p] += (double)sq;
+        }
+      }
+    }
+    // no barrier here: the next iteration's first barrier orders these reads of buf before the
+    // next first stage rewrites it
+  }
+  lds_barrier();
 #pragma unroll
   for (int u = 0; u < PAIRS; ++u) {
     const int m = tid0 + u * BT;
