@@ -309,7 +309,7 @@ void launch_mid(tm_core* h, int nb) {
     case 4096: {
       const int ntiles = mid_tiles(h);
       const int grid = ((ntiles + 7) / 8) * 8;
-      hipLaunchKernelGGL((tmk::k_mid4096<2, true, 4, false>), dim3(grid), dim3(2 * 256), 0, h->stream, h->rows.p, h->inpT.p, h->Z.p,
+      hipLaunchKernelGGL((tmk::k_mid4096<2, true, 4, true>), dim3(grid), dim3(2 * 256), 0, h->stream, h->rows.p, h->inpT.p, h->Z.p,
                          h->Wh, h->T, nb, ntiles, h->twiddle(4096));
       check_launch("k_mid4096");
       break;

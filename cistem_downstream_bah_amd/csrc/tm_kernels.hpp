@@ -365,8 +365,11 @@ __global__ __launch_bounds__(KC* NTC) void k_mid(const cf* __restrict__ rows, co
 //  * forward stage 1 reads its 16 inputs x[j + 256 q] straight from the forward rows (zero outside
 //    the T projection rows), so no LDS zero-fill or scatter is needed;
 //  * forward stage 3, the DC zero, conj(R) * I and inverse stage 1 run on registers;
-//  * inverse stage 3 leaves z[j + 256 q] in registers and a wave stores 64 / KC rows x KC columns
-//    = 512 contiguous bytes of the tiled Z per q.
+//  * inverse stage 3 leaves z[j + 256 q] in registers; column kx goes to slot kx % ZT of tile
+//    kx / ZT of Z[b][tile][y][ZT] (ZT a multiple of KC: with ZT = 2 KC two neighbouring blocks,
+//    which run on the same XCD, fill one tile, and a row read of k_last takes ZT * 8 contiguous
+//    bytes per tile); NTS selects non-temporal stores;
+//  * barriers order LDS traffic only, so they do not wait for the Z stores to complete.
 // Eight LDS passes per column and orientation (four exchanges, each a write and a read). With the
 // padded LDS index lpad(i) = i + i/16 the three access patterns are a per-thread base plus
 // compile-time offsets: lpad(16 j + q) = 17 j + q, lpad(j + 256 q) = lpad(j) + 272 q and
@@ -388,7 +391,7 @@ __global__ __launch_bounds__(KC * 256, 4) void k_mid4096(const cf* __restrict__ 
     tw256[tid] = tw[16 * tid];
     twbq[tid] = tw[(tid / 16) * (tid % 16)];
   }
-  __syncthreads();
+  lds_barrier();
   const int c = tid % KC;
   const int bid = blockIdx.x;
   const int per_xcd = (ntiles + 7) / 8;
@@ -427,17 +430,17 @@ __global__ __launch_bounds__(KC * 256, 4) void k_mid4096(const cf* __restrict__ 
     dft16<-1>(v);
 #pragma unroll
     for (int q = 0; q < 16; ++q) p_s1[q] = v[q];
-    __syncthreads();
+    lds_barrier();
     // forward stage 2 (Ns = 16)
 #pragma unroll
     for (int q = 0; q < 16; ++q) v[q] = p_rd[272 * q];
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int q = 1; q < 16; ++q) v[q] = cmul(v[q], twiddle<-1>(tw256, k2 * q));
     dft16<-1>(v);
 #pragma unroll
     for (int q = 0; q < 16; ++q) p_s2[17 * q] = v[q];
-    __syncthreads();
+    lds_barrier();
     // input-spectrum column, optionally prefetched before forward stage 3 so that its latency
     // overlaps the stage (costs 32 registers)
     cf iv[16];
@@ -459,22 +462,22 @@ __global__ __launch_bounds__(KC * 256, 4) void k_mid4096(const cf* __restrict__ 
       const cf x = (kx == 0 && j == 0 && q == 0) ? cmake(0.f, 0.f) : v[q];
       v[q] = cconjmul(x, iv[q]);
     }
-    __syncthreads();
+    lds_barrier();
     // inverse stage 1
     dft16<+1>(v);
 #pragma unroll
     for (int q = 0; q < 16; ++q) p_s1[q] = v[q];
-    __syncthreads();
+    lds_barrier();
     // inverse stage 2
 #pragma unroll
     for (int q = 0; q < 16; ++q) v[q] = p_rd[272 * q];
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int q = 1; q < 16; ++q) v[q] = cmul(v[q], twiddle<+1>(tw256, k2 * q));
     dft16<+1>(v);
 #pragma unroll
     for (int q = 0; q < 16; ++q) p_s2[17 * q] = v[q];
-    __syncthreads();
+    lds_barrier();
     // inverse stage 3: z[j + 256 q]
 #pragma unroll
     for (int q = 0; q < 16; ++q) v[q] = p_rd[272 * q];
@@ -491,7 +494,7 @@ __global__ __launch_bounds__(KC * 256, 4) void k_mid4096(const cf* __restrict__ 
       if constexpr (NTS) __builtin_nontemporal_store(bits, zt + (size_t)M * ZT * q);
       else zt[(size_t)M * ZT * q] = bits;
     }
-    __syncthreads();  // LDS reads of this orientation complete before the next one writes
+    lds_barrier();  // LDS reads of this orientation complete before the next one writes
   }
 }
 
