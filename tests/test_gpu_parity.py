@@ -16,14 +16,11 @@ import pytest
 
 from cistem_downstream_bah_amd import core, synth
 from oracle import oracle as O
+from parity_util import compare_search as _compare_search, rel as _rel
 
 pytestmark = pytest.mark.gpu
 
 THREADS = int(os.environ.get("TM_ORACLE_THREADS", str(min(16, os.cpu_count() or 1))))
-
-
-def _rel(a, b):
-    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30))
 
 
 def _make(params, s):
@@ -85,57 +82,6 @@ def test_projection_padding2(c0_pad2, pos, psi):
     ec = _rel(cg, cref)
     print(f"pad2 projection rel={e:.2e} ccf rel={ec:.2e}", flush=True)
     assert e <= 1e-4 and ec <= 1e-4
-
-
-def _near_tie_check(mt, ref, got, max_orientations=400):
-    """Where the best index differs, the oracle's own CCF at the GPU's orientation must equal the
-    oracle's MIP within FP32 noise (1e-5 relative): the two orientations tie to within the rounding
-    of two FFT implementations, so either choice is a correct argmax."""
-    I = mt.info
-    bad = np.argwhere(got.best_index != ref["best_index"])
-    if bad.size == 0:
-        return 0
-    g = got.best_index[bad[:, 0], bad[:, 1]]
-    uniq = np.unique(g)[:max_orientations]
-    checked = 0
-    worst = 0.0
-    for gi in uniq:
-        local = int(gi) % (((I.last_pos - I.first_pos) + 1) * I.n_psi)
-        pos = I.first_pos + local // I.n_psi
-        k = local % I.n_psi
-        _, ccf = mt.projection_and_ccf(pos, k, 0)
-        sel = bad[g == gi]
-        v = ccf[sel[:, 0], sel[:, 1]]
-        m = ref["mip"][sel[:, 0], sel[:, 1]]
-        worst = max(worst, float(np.max(np.abs(v - m) / np.abs(m))))
-        checked += len(sel)
-    print(f"near-tie check: {checked} differing pixels over {len(uniq)} orientations, worst gap {worst:.2e}", flush=True)
-    assert worst <= 1e-5
-    return checked
-
-
-def _compare_search(mt, ref, got, label):
-    mip_r, mip_g = ref["mip"], got.mip
-    rel = np.abs(mip_g - mip_r) / np.maximum(np.abs(mip_r), 1e-30)
-    same = got.best_index == ref["best_index"]
-    print(f"{label}: MIP max rel {rel.max():.2e}; best index identical at {same.mean() * 100:.4f}% of pixels", flush=True)
-    assert rel.max() <= 1e-4
-    _near_tie_check(mt, ref, got)
-    srel = np.abs(got.sum - ref["sum"]).max() / np.abs(ref["sum"]).max()
-    qrel = np.abs(got.sumsq - ref["sumsq"]).max() / np.abs(ref["sumsq"]).max()
-    print(f"{label}: sum rel {srel:.2e} sumsq rel {qrel:.2e}", flush=True)
-    assert srel <= 1e-5 and qrel <= 1e-5
-    assert got.hist.sum() == ref["hist"].sum()
-    # values within FP32 rounding of a bin edge may land in the neighbouring bin; each such value
-    # moves the cumulative count at exactly one edge, so the cumulative histograms must agree to
-    # within the expected number of edge values (<< 1e-5 of the total)
-    cd = np.abs(np.cumsum(got.hist) - np.cumsum(ref["hist"])).max()
-    print(f"{label}: histogram total {got.hist.sum()} max cumulative diff {cd}", flush=True)
-    assert cd <= 1e-5 * got.hist.sum()
-    # angle maps follow from the index
-    m = same
-    assert np.array_equal(got.psi[m], ref["psi"][m]) and np.array_equal(got.theta[m], ref["theta"][m])
-    assert np.array_equal(got.phi[m], ref["phi"][m])
 
 
 def test_search_subset_matches_oracle(c0):

@@ -1,0 +1,70 @@
+"""GPU parity at the full 4096 x 4096 image size of the benchmark workload (template 256^3 padded
+to 512^3): the kernels specialised for 4096-point columns and rows, and the production batch of 32
+orientations, against the CPU oracle.
+
+* single CCF planes (debug entry point through the same column and row kernels as the search):
+  max |gpu - oracle| / max |oracle| <= 1e-4
+* a search over one grid position (2.5 deg grid, 5 deg in-plane: 73 orientations) and three
+  defocus planes (-200, 0, +200 A), run plane by plane in the reference's loop order: MIP, argmax
+  (near-tie rule), angle and defocus maps, FP64 sums and histogram to the bars of parity_util.
+  73 orientations per plane leave a partial last batch (73 = 2 * 32 + 9).
+The oracle costs about 1.2 s per 4096^2 CCF on one core, so the search is sized to one position.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from cistem_downstream_bah_amd import core, synth
+from oracle import oracle as O
+from parity_util import compare_search, rel
+
+pytestmark = pytest.mark.gpu
+
+THREADS = int(os.environ.get("TM_ORACLE_THREADS", str(min(16, os.cpu_count() or 1))))
+POSITION = 829
+
+
+@pytest.fixture(scope="module")
+def full():
+    s = synth.config1(1000)
+    mt = O.MatchTemplateOracle(O.MTParams(angular_step=2.5, in_plane_angular_step=5.0, padding=2.0,
+                                          defocus_search_range=200.0, defocus_step=200.0,
+                                          first_search_position=POSITION, last_search_position=POSITION),
+                               s.micrograph, s.volume)
+    I = mt.info
+    assert (I.W, I.H, I.T, I.P, I.n_def_half) == (4096, 4096, 256, 512, 1)
+    g = core.TemplateMatchingCore(0)
+    g.init(mt.template_spectrum, mt.input_spectrum, I.T, mt.grid, mt.psi, I.first_pos, I.last_pos,
+           I.histogram_min_scaled, I.histogram_step_scaled, 512, I.max_padding, I.defocus_step, I.pixel_size_step,
+           batch_size=0)
+    yield s, mt, g
+    g.close()
+
+
+@pytest.mark.parametrize("psi,defocus_i", [(0, 0), (40, 1)])
+def test_ccf_4096_matches_oracle(full, psi, defocus_i):
+    s, mt, g = full
+    pf = mt.projection_filter(defocus_i)
+    ref_p, ref = mt.projection_and_ccf(POSITION, psi, defocus_i)
+    got_p = g.debug_projection(pf, POSITION, psi)
+    got = g.debug_ccf(pf, POSITION, psi)
+    ep, ec = rel(got_p, ref_p), rel(got, ref)
+    print(f"4096^2 psi={psi} defocus_i={defocus_i}: projection rel {ep:.2e} ccf rel {ec:.2e}", flush=True)
+    assert ep <= 1e-4 and ec <= 1e-4
+
+
+def test_search_4096_three_planes_matches_oracle(full):
+    s, mt, g = full
+    I = mt.info
+    ref = mt.run(threads=THREADS)
+    g.reset()
+    n = 0
+    for d in range(-I.n_def_half, I.n_def_half + 1):
+        n += g.run_inner_loop(mt.projection_filter(d), 0, d)
+    got = g.results()
+    assert n == ref["nccf"] == 3 * I.n_psi == 3 * 73
+    compare_search(mt, ref, got, "4096^2 three planes", max_tie_orientations=24)
+    # every plane contributed: defocus offsets of all three planes occur in the map
+    offs = np.unique(got.defocus)
+    assert set(np.round(offs).astype(int)) == {-200, 0, 200}
