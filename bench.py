@@ -88,7 +88,7 @@ def main():
     ap.add_argument("--batch", type=int, default=32, help="orientations per launch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--cpu-psi", type=int, default=2)
+    ap.add_argument("--cpu-psi", type=int, default=8)
     ap.add_argument("--small", action="store_true", help="512^2 / 64^3 workload (config 0 shapes)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB (3 times) and exit; "
