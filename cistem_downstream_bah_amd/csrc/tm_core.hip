@@ -18,6 +18,7 @@
 
 #include "../../include/tm_hip.h"
 #include "tm_kernels.hpp"
+#include "tm_sizes.hpp"
 
 using tmk::cf;
 
@@ -57,6 +58,24 @@ int guarded(F&& f) {
 }
 
 bool pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+bool image_size_compiled(int n) {
+  switch (n) {
+#define TM_CASE(k) case k:
+    TM_IMAGE_SIZES(TM_CASE)
+#undef TM_CASE
+    return true;
+    default: return false;
+  }
+}
+
+std::string image_sizes_text() {
+  std::string t;
+#define TM_CASE(k) t += (t.empty() ? "" : ", ") + std::to_string(k);
+  TM_IMAGE_SIZES(TM_CASE)
+#undef TM_CASE
+  return t;
+}
 
 // Accurate twiddle table exp(-2 pi i k / n), k < n.
 std::vector<cf> twiddles(int n) {
@@ -225,12 +244,10 @@ void launch_cols(tm_core* h, cf* data, int ncols, size_t bstride, int nb, int si
 void cols(tm_core* h, int N, cf* data, int ncols, size_t bstride, int nb, int sign, int mode, const cf* filt,
           float scale) {
   switch (N) {
-    case 32: launch_cols<32>(h, data, ncols, bstride, nb, sign, mode, filt, scale); break;
-    case 64: launch_cols<64>(h, data, ncols, bstride, nb, sign, mode, filt, scale); break;
-    case 128: launch_cols<128>(h, data, ncols, bstride, nb, sign, mode, filt, scale); break;
-    case 256: launch_cols<256>(h, data, ncols, bstride, nb, sign, mode, filt, scale); break;
-    case 512: launch_cols<512>(h, data, ncols, bstride, nb, sign, mode, filt, scale); break;
-    case 1024: launch_cols<1024>(h, data, ncols, bstride, nb, sign, mode, filt, scale); break;
+#define TM_CASE(n) \
+  case n: launch_cols<n>(h, data, ncols, bstride, nb, sign, mode, filt, scale); break;
+    TM_TEMPLATE_SIZES(TM_CASE)
+#undef TM_CASE
     default: throw TmError{TM_ERR_UNSUPPORTED, "unsupported column FFT length " + std::to_string(N)};
   }
 }
@@ -249,12 +266,10 @@ void launch_c2r(tm_core* h, const cf* in, size_t ibs, int irs, float* out, size_
 void rows_c2r(tm_core* h, int N, const cf* in, size_t ibs, int irs, float* out, size_t obs, int ors, int nrows,
               int nb) {
   switch (N) {
-    case 32: launch_c2r<32>(h, in, ibs, irs, out, obs, ors, nrows, nb); break;
-    case 64: launch_c2r<64>(h, in, ibs, irs, out, obs, ors, nrows, nb); break;
-    case 128: launch_c2r<128>(h, in, ibs, irs, out, obs, ors, nrows, nb); break;
-    case 256: launch_c2r<256>(h, in, ibs, irs, out, obs, ors, nrows, nb); break;
-    case 512: launch_c2r<512>(h, in, ibs, irs, out, obs, ors, nrows, nb); break;
-    case 1024: launch_c2r<1024>(h, in, ibs, irs, out, obs, ors, nrows, nb); break;
+#define TM_CASE(n) \
+  case n: launch_c2r<n>(h, in, ibs, irs, out, obs, ors, nrows, nb); break;
+    TM_TEMPLATE_SIZES(TM_CASE)
+#undef TM_CASE
     default: throw TmError{TM_ERR_UNSUPPORTED, "unsupported C2R row length " + std::to_string(N)};
   }
 }
@@ -271,77 +286,83 @@ void launch_r2c(tm_core* h, const float* src, size_t sbs, int srs, int r0, int s
 void rows_r2c(tm_core* h, int N, const float* src, size_t sbs, int srs, int r0, int s0, int len, int c0, cf* dst,
               size_t dbs, int drs, int nrows, int nb, float scale) {
   switch (N) {
-    case 32: launch_r2c<32>(h, src, sbs, srs, r0, s0, len, c0, dst, dbs, drs, nrows, nb, scale); break;
-    case 64: launch_r2c<64>(h, src, sbs, srs, r0, s0, len, c0, dst, dbs, drs, nrows, nb, scale); break;
-    case 128: launch_r2c<128>(h, src, sbs, srs, r0, s0, len, c0, dst, dbs, drs, nrows, nb, scale); break;
-    case 256: launch_r2c<256>(h, src, sbs, srs, r0, s0, len, c0, dst, dbs, drs, nrows, nb, scale); break;
-    case 512: launch_r2c<512>(h, src, sbs, srs, r0, s0, len, c0, dst, dbs, drs, nrows, nb, scale); break;
-    case 1024: launch_r2c<1024>(h, src, sbs, srs, r0, s0, len, c0, dst, dbs, drs, nrows, nb, scale); break;
-    case 2048: launch_r2c<2048>(h, src, sbs, srs, r0, s0, len, c0, dst, dbs, drs, nrows, nb, scale); break;
-    case 4096: launch_r2c<4096>(h, src, sbs, srs, r0, s0, len, c0, dst, dbs, drs, nrows, nb, scale); break;
+#define TM_CASE(n) \
+  case n: launch_r2c<n>(h, src, sbs, srs, r0, s0, len, c0, dst, dbs, drs, nrows, nb, scale); break;
+    TM_LINE_SIZES(TM_CASE)
+#undef TM_CASE
     default: throw TmError{TM_ERR_UNSUPPORTED, "unsupported R2C row length " + std::to_string(N)};
   }
 }
 
 template <int HH, int KC, int NTC>
 void launch_mid_t(tm_core* h, int nb) {
+  if constexpr (NTC == 0) {
+    throw TmError{TM_ERR_UNSUPPORTED, "no generic column pass for height " + std::to_string(HH)};
+  } else {
   const int ntiles = (h->Wh + KC - 1) / KC;
   const int grid = ((ntiles + 7) / 8) * 8;
   hipLaunchKernelGGL((tmk::k_mid<HH, KC, NTC>), dim3(grid), dim3(KC * NTC), 0, h->stream, h->rows.p, h->inpT.p, h->Z.p,
                      h->Wh, h->T, nb, ntiles, h->twiddle(HH));
   check_launch("k_mid");
+  }
 }
 
 // Columns per k_mid block for image height H, and the column-tile width of the Z layout
 // Z[b][tile][y][z_tile]. At H = 4096 two 2-column blocks fill one 4-column tile, so that a row
 // read of k_last takes 32 bytes from each cache line instead of 16.
-int mid_kc(int H) { return H >= 4096 ? 2 : (H >= 2048 ? 8 : 16); }
+int mid_kc(int H) { return H == 4096 ? 2 : tmk::mid_cols(H); }
 int mid_tiles(const tm_core* h) { return (h->Wh + mid_kc(h->H) - 1) / mid_kc(h->H); }
 int z_tile(int H) { return H >= 4096 ? 4 : mid_kc(H); }
+// Whether the column pass places the projection rows at y = 0 (with the centring phase folded into
+// the input spectrum) rather than at H/2 - T/2.
+bool column_rows_at_zero(int H) { return H == 4096; }
 int z_tiles(const tm_core* h) { return (h->Wh + z_tile(h->H) - 1) / z_tile(h->H); }
 
 void launch_mid(tm_core* h, int nb) {
+  if (h->H == 4096) {
+    const int ntiles = mid_tiles(h);
+    const int grid = ((ntiles + 7) / 8) * 8;
+    if (h->T <= 256)
+      hipLaunchKernelGGL((tmk::k_mid4096<2, true, 4, true, true>), dim3(grid), dim3(2 * 256), 0, h->stream, h->rows.p,
+                         h->inpT.p, h->Z.p, h->Wh, h->T, nb, ntiles, h->twiddle(4096));
+    else
+      hipLaunchKernelGGL((tmk::k_mid4096<2, true, 4, true, false>), dim3(grid), dim3(2 * 256), 0, h->stream, h->rows.p,
+                         h->inpT.p, h->Z.p, h->Wh, h->T, nb, ntiles, h->twiddle(4096));
+    check_launch("k_mid4096");
+    return;
+  }
   switch (h->H) {
-    case 256: launch_mid_t<256, 16, 16>(h, nb); break;
-    case 512: launch_mid_t<512, 16, 32>(h, nb); break;
-    case 1024: launch_mid_t<1024, 16, 64>(h, nb); break;
-    case 2048: launch_mid_t<2048, 8, 128>(h, nb); break;
-    case 4096: {
-      const int ntiles = mid_tiles(h);
-      const int grid = ((ntiles + 7) / 8) * 8;
-      hipLaunchKernelGGL((tmk::k_mid4096<2, true, 4, true>), dim3(grid), dim3(2 * 256), 0, h->stream, h->rows.p, h->inpT.p, h->Z.p,
-                         h->Wh, h->T, nb, ntiles, h->twiddle(4096));
-      check_launch("k_mid4096");
-      break;
-    }
+#define TM_CASE(n) \
+  case n: launch_mid_t<n, tmk::mid_cols(n), tmk::mid_threads(n)>(h, nb); break;
+    TM_IMAGE_SIZES(TM_CASE)
+#undef TM_CASE
     default: throw TmError{TM_ERR_UNSUPPORTED, "unsupported image height " + std::to_string(h->H)};
   }
 }
 
-// QR orientations per k_last block; blocks have W/16 threads per orientation. From W = 1024 on
-// the groups are whole wavefronts and the row gather runs as double-buffered direct LDS loads.
+// QR orientations per k_last block (tmk::last_qr). For power-of-two widths from 1024 on the
+// groups are whole wavefronts and the row gather runs as double-buffered direct LDS loads.
 template <int WW, int QR>
 void launch_last_t(tm_core* h, int nb, tmk::EpiParams ep) {
   ep.ztile = z_tile(h->H);
   ep.ntiles = z_tiles(h);
   ep.hstep_inv = 1.0 / ep.hstep;
-  if constexpr (WW >= 1024) {
+  if constexpr (WW >= 1024 && (WW & (WW - 1)) == 0) {
     hipLaunchKernelGGL((tmk::k_last_dma<WW, QR>), dim3(h->H), dim3(QR * (WW / 16)), 0, h->stream, h->Z.p, h->H, nb,
                        ep, h->twiddle(WW), h->twiddle(WW / 2));
   } else {
-    hipLaunchKernelGGL((tmk::k_last<WW, QR>), dim3(h->H), dim3(QR * (WW / 16)), 0, h->stream, h->Z.p, h->H, nb, ep,
-                       h->twiddle(WW), h->twiddle(WW / 2));
+    hipLaunchKernelGGL((tmk::k_last<WW, QR>), dim3(h->H), dim3(tmk::last_threads<WW, QR>()), 0, h->stream, h->Z.p,
+                       h->H, nb, ep, h->twiddle(WW), h->twiddle(WW / 2));
   }
   check_launch("k_last");
 }
 
 void launch_last(tm_core* h, int nb, const tmk::EpiParams& ep) {
   switch (h->W) {
-    case 256: launch_last_t<256, 8>(h, nb, ep); break;
-    case 512: launch_last_t<512, 8>(h, nb, ep); break;
-    case 1024: launch_last_t<1024, 4>(h, nb, ep); break;
-    case 2048: launch_last_t<2048, 4>(h, nb, ep); break;
-    case 4096: launch_last_t<4096, 2>(h, nb, ep); break;
+#define TM_CASE(n) \
+  case n: launch_last_t<n, tmk::last_qr(n)>(h, nb, ep); break;
+    TM_IMAGE_SIZES(TM_CASE)
+#undef TM_CASE
     default: throw TmError{TM_ERR_UNSUPPORTED, "unsupported image width " + std::to_string(h->W)};
   }
 }
@@ -514,9 +535,9 @@ int tm_init(tm_core* h, const tm_init_params* p, const float* template_spectrum,
     TM_HIP(hipSetDevice(h->device));
     h->initialised = false;
     const int P = p->template_size, T = p->projection_size, W = p->image_x, H = p->image_y;
-    TM_REQUIRE(pow2(W) && W >= 256 && W <= 4096 && pow2(H) && H >= 256 && H <= 4096, TM_ERR_UNSUPPORTED,
+    TM_REQUIRE(image_size_compiled(W) && image_size_compiled(H), TM_ERR_UNSUPPORTED,
                "image size " + std::to_string(W) + "x" + std::to_string(H) +
-                   " not supported (powers of two from 256 to 4096)");
+                   " not supported (sides " + image_sizes_text() + ")");
     TM_REQUIRE(pow2(T) && T >= 32 && T <= 1024 && pow2(P) && P >= T && P <= 1024, TM_ERR_UNSUPPORTED,
                "template/projection sizes " + std::to_string(P) + "/" + std::to_string(T) +
                    " not supported (powers of two, 32 <= T <= P <= 1024)");
@@ -557,11 +578,34 @@ int tm_init(tm_core* h, const tm_init_params* p, const float* template_spectrum,
     h->tpl.alloc(ntpl);
     TM_HIP(hipMemcpy(h->tpl.p, template_spectrum, ntpl * sizeof(cf), hipMemcpyHostToDevice));
 
-    // input spectrum, transposed to [kx][ky] so that each column is contiguous
+    // input spectrum, transposed to [kx][ky] so that each column is contiguous. At H = 4096 the
+    // column pass places the projection rows at y = 0..T-1 instead of the reference's centred
+    // y0 = H/2 - T/2 (ClipIntoLargerRealSpace2D); the shift multiplies the reference spectrum by
+    // e^{-2 pi i y0 ky / H}, so conj(R) * I keeps its value with I' = I * e^{+2 pi i y0 ky / H}.
     std::vector<cf> it((size_t)h->Wh * H);
     const cf* in = reinterpret_cast<const cf*>(input_spectrum);
+    const bool shifted = column_rows_at_zero(H);
+    std::vector<long double> pc(H, 1.0L), ps(H, 0.0L);
+    if (shifted) {
+      const long double two_pi = 2.0L * acosl(-1.0L);
+      const long long y0 = H / 2 - T / 2;
+      for (int y = 0; y < H; ++y) {
+        const long double a = two_pi * (long double)((y0 * y) % H) / (long double)H;
+        pc[y] = cosl(a);
+        ps[y] = sinl(a);
+      }
+    }
     for (int y = 0; y < H; ++y)
-      for (int x = 0; x < h->Wh; ++x) it[(size_t)x * H + y] = in[(size_t)y * h->Wh + x];
+      for (int x = 0; x < h->Wh; ++x) {
+        const cf v = in[(size_t)y * h->Wh + x];
+        cf& o = it[(size_t)x * H + y];
+        if (shifted) {
+          o.x = (float)(v.x * pc[y] - v.y * ps[y]);
+          o.y = (float)(v.x * ps[y] + v.y * pc[y]);
+        } else {
+          o = v;
+        }
+      }
     h->inpT.alloc(it.size());
     TM_HIP(hipMemcpy(h->inpT.p, it.data(), it.size() * sizeof(cf), hipMemcpyHostToDevice));
 

@@ -40,6 +40,37 @@ __device__ __forceinline__ void dft4(cf* v) {
   v[0] = cadd(t0, t2); v[2] = csub(t0, t2); v[1] = cadd(t1, t3); v[3] = csub(t1, t3);
 }
 
+// radix-3: X1,2 = x0 - (x1 + x2)/2 +- SIGN i sin(2 pi/3) (x1 - x2)
+template <int SIGN>
+__device__ __forceinline__ void dft3(cf* v) {
+  const float s3 = 0.86602540378443864676f;
+  const cf t1 = cadd(v[1], v[2]);
+  const cf t2 = cmake(v[0].x - 0.5f * t1.x, v[0].y - 0.5f * t1.y);
+  const cf u = cscale(cmul_mi<SIGN>(csub(v[1], v[2])), s3);
+  v[0] = cadd(v[0], t1);
+  v[1] = cadd(t2, u);
+  v[2] = csub(t2, u);
+}
+
+// radix-5 with c_k = cos(2 pi k/5), s_k = sin(2 pi k/5), a = x1 + x4, a' = x2 + x3, b = x1 - x4,
+// b' = x2 - x3: X1,4 = x0 + c1 a + c2 a' +- SIGN i (s1 b + s2 b'), X2,3 = x0 + c2 a + c1 a' +- SIGN i (s2 b - s1 b')
+template <int SIGN>
+__device__ __forceinline__ void dft5(cf* v) {
+  const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
+  const float s1 = 0.95105651629515357212f, s2 = 0.58778525229247312917f;
+  const cf a1 = cadd(v[1], v[4]), a2 = cadd(v[2], v[3]);
+  const cf b1 = csub(v[1], v[4]), b2 = csub(v[2], v[3]);
+  const cf p1 = cmake(v[0].x + c1 * a1.x + c2 * a2.x, v[0].y + c1 * a1.y + c2 * a2.y);
+  const cf p2 = cmake(v[0].x + c2 * a1.x + c1 * a2.x, v[0].y + c2 * a1.y + c1 * a2.y);
+  const cf q1 = cmul_mi<SIGN>(cmake(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y));
+  const cf q2 = cmul_mi<SIGN>(cmake(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y));
+  v[0] = cadd(v[0], cadd(a1, a2));
+  v[1] = cadd(p1, q1);
+  v[4] = csub(p1, q1);
+  v[2] = cadd(p2, q2);
+  v[3] = csub(p2, q2);
+}
+
 // radix-8: 2 x 4 with internal twiddles W8^k
 template <int SIGN>
 __device__ __forceinline__ void dft8(cf* v) {
@@ -98,7 +129,9 @@ __device__ __forceinline__ void dftR(cf* v) {
   else if constexpr (R == 4) dft4<SIGN>(v);
   else if constexpr (R == 8) dft8<SIGN>(v);
   else if constexpr (R == 16) dft16<SIGN>(v);
-  else static_assert(R == 2 || R == 4 || R == 8 || R == 16, "unsupported radix");
+  else if constexpr (R == 3) dft3<SIGN>(v);
+  else if constexpr (R == 5) dft5<SIGN>(v);
+  else static_assert(R == 2 || R == 3 || R == 4 || R == 5 || R == 8 || R == 16, "unsupported radix");
 }
 
 // Workgroup barrier that orders LDS traffic only: this wave's LDS operations complete, then
@@ -143,17 +176,25 @@ __device__ __forceinline__ cf twiddle(const TwSplit& t, int idx) {
   return SIGN < 0 ? w : cconj(w);
 }
 
-// ---------------------------------------------------------------- radix plans (powers of two)
-// Largest-radix-first plan with radices 2^LR (LR = 4: radix 16, LR = 3: radix 8) and one
-// smaller last radix when log2(N) is not a multiple of LR.
+// ---------------------------------------------------------------- radix plans
+// N = 2^a 3^b 5^c. The power-of-two part runs first, largest radix first: radices 2^LR (LR = 4:
+// radix 16, LR = 3: radix 8) and one smaller radix when a is not a multiple of LR; then b radix-3
+// and c radix-5 stages. For powers of two this is the plain largest-radix-first plan.
 __host__ __device__ constexpr int log2i(int n) { return n <= 1 ? 0 : 1 + log2i(n / 2); }
+__host__ __device__ constexpr int nfactor(int n, int f) { return (n > 0 && n % f == 0) ? 1 + nfactor(n / f, f) : 0; }
+__host__ __device__ constexpr int ipow(int b, int e) { return e == 0 ? 1 : b * ipow(b, e - 1); }
+__host__ __device__ constexpr bool fft_size_ok(int n) {
+  return n >= 1 && ipow(2, nfactor(n, 2)) * ipow(3, nfactor(n, 3)) * ipow(5, nfactor(n, 5)) == n;
+}
+__host__ __device__ constexpr int plan_stages2(int N, int LR) {
+  return nfactor(N, 2) / LR + ((nfactor(N, 2) % LR) ? 1 : 0);
+}
 __host__ __device__ constexpr int plan_stages(int N, int LR = 4) {
-  int l = log2i(N);
-  return l / LR + ((l % LR) ? 1 : 0);
+  return plan_stages2(N, LR) + nfactor(N, 3) + nfactor(N, 5);
 }
 __host__ __device__ constexpr int plan_radix(int N, int s, int LR = 4) {
-  int l = log2i(N);
-  return (s < l / LR) ? (1 << LR) : (1 << (l % LR));
+  return s < plan_stages2(N, LR) ? ((s < nfactor(N, 2) / LR) ? (1 << LR) : (1 << (nfactor(N, 2) % LR)))
+                                 : (s - plan_stages2(N, LR) < nfactor(N, 3) ? 3 : 5);
 }
 __host__ __device__ constexpr int plan_ns(int N, int s, int LR = 4) {  // sub-transform length before stage s
   int ns = 1;

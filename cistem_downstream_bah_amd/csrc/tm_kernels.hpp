@@ -35,6 +35,7 @@ __host__ __device__ constexpr int lds_len(int n) { return n + n / 16; }
 // into registers, synchronises, then stores.
 template <int N, int SIGN, int NT, int LR = 4, int S = 0, typename TW = const cf*>
 __device__ __forceinline__ void fft_inplace(cf* buf, TW tw, int tid) {
+  static_assert(fft_size_ok(N), "FFT length must be 2^a 3^b 5^c");
   if constexpr (S < plan_stages(N, LR)) {
     constexpr int R = plan_radix(N, S, LR);
     constexpr int NS = plan_ns(N, S, LR);
@@ -363,7 +364,8 @@ __global__ __launch_bounds__(KC* NTC) void k_mid(const cf* __restrict__ rows, co
 // With KC = 2 a block needs ~74 KB of LDS, so two blocks share a CU and hide each other's barrier
 // and load latency.
 //  * forward stage 1 reads its 16 inputs x[j + 256 q] straight from the forward rows (zero outside
-//    the T projection rows), so no LDS zero-fill or scatter is needed;
+//    the T projection rows), so no LDS zero-fill or scatter is needed; with DIRECT (T <= 256) the
+//    stage is trivial and stage 2 loads its inputs from the rows instead (one exchange fewer);
 //  * forward stage 3, the DC zero, conj(R) * I and inverse stage 1 run on registers;
 //  * inverse stage 3 leaves z[j + 256 q] in registers; column kx goes to slot kx % ZT of tile
 //    kx / ZT of Z[b][tile][y][ZT] (ZT a multiple of KC: with ZT = 2 KC two neighbouring blocks,
@@ -374,7 +376,7 @@ __global__ __launch_bounds__(KC* NTC) void k_mid(const cf* __restrict__ rows, co
 // padded LDS index lpad(i) = i + i/16 the three access patterns are a per-thread base plus
 // compile-time offsets: lpad(16 j + q) = 17 j + q, lpad(j + 256 q) = lpad(j) + 272 q and
 // lpad(b2 + 16 q) = lpad(b2) + 17 q for b2 = (j / 16) 256 + j % 16.
-template <int KC, bool PREFETCH, int ZT = KC, bool NTS = true>
+template <int KC, bool PREFETCH, int ZT = KC, bool NTS = true, bool DIRECT = false>
 __global__ __launch_bounds__(KC * 256, 4) void k_mid4096(const cf* __restrict__ rows, const cf* __restrict__ inpT,
                                                   cf* __restrict__ Z, int Wh, int T, int nb, int ntiles,
                                                   const cf* __restrict__ tw) {
@@ -399,7 +401,9 @@ __global__ __launch_bounds__(KC * 256, 4) void k_mid4096(const cf* __restrict__ 
   if (tile >= ntiles) return;  // whole block exits together
   const int kx = tile * KC + c;
   const bool valid = kx < Wh;
-  const int jlo = H / 2 - T / 2;
+  // The T projection rows sit at y = 0..T-1 of the column; the reference centres them at
+  // y = H/2 - T/2 (ClipIntoLargerRealSpace2D), a shift whose phase e^{2 pi i (H/2 - T/2) ky / H} the
+  // caller has folded into the input spectrum inpT.
   static_assert(ZT % KC == 0, "Z tiles hold whole column groups");
   const size_t z_bs = (size_t)((Wh + ZT - 1) / ZT) * H * ZT;
   for (int b = 0; b < nb; ++b) {
@@ -414,27 +418,38 @@ __global__ __launch_bounds__(KC * 256, 4) void k_mid4096(const cf* __restrict__ 
     cf* const p_s2 = colbuf + (j / 16) * 272 + k2;            // stage-2 stores: + 17 q
     const cf* ic = inpT + (size_t)(valid ? kx : 0) * H + j;   // input spectrum: + 256 q
     cf v[16];
-    // forward stage 1 (Ns = 1): inputs from the T non-zero rows placed at y = jlo + r; only the
-    // q with 256 q + [0, 256) overlapping [jlo, jlo + T) can be non-zero (a block-uniform range)
-    const int qmin = jlo / M, qmax = (jlo + T - 1) / M;
     const cf* rb = rows + (size_t)b * T * Wh + kx;
+    if constexpr (DIRECT) {
+      // T <= 256: forward stage 1 (Ns = 1) has the single non-zero input x[j] (q = 0), so its 16
+      // outputs all equal x[j] and element 16 j' + k1 of its result is x[j']. Stage 2 butterfly j
+      // reads elements j + 256 q, which are x[j / 16 + 16 q]: load them from the rows directly.
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      v[q] = cmake(0.f, 0.f);
-      if (q >= qmin && q <= qmax) {
-        const int r = j + M * q - jlo;
-        const cf* rq = rb + (ptrdiff_t)(M * q - jlo) * Wh;  // block-uniform base
-        if (valid && r >= 0 && r < T) v[q] = rq[j * Wh];
+      for (int q = 0; q < 16; ++q) {
+        const int r = j / 16 + 16 * q;
+        v[q] = (valid && r < T) ? rb[(size_t)r * Wh] : cmake(0.f, 0.f);
       }
+    } else {
+      // forward stage 1 (Ns = 1): inputs x[j + 256 q] from the T non-zero rows; only q <= (T-1)/256
+      // can be non-zero (a block-uniform range)
+      const int qmax = (T - 1) / M;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        v[q] = cmake(0.f, 0.f);
+        if (q <= qmax) {
+          const int r = j + M * q;
+          const cf* rq = rb + (ptrdiff_t)(M * q) * Wh;  // block-uniform base
+          if (valid && r < T) v[q] = rq[j * Wh];
+        }
+      }
+      dft16<-1>(v);
+#pragma unroll
+      for (int q = 0; q < 16; ++q) p_s1[q] = v[q];
+      lds_barrier();
+      // forward stage 2 (Ns = 16)
+#pragma unroll
+      for (int q = 0; q < 16; ++q) v[q] = p_rd[272 * q];
+      lds_barrier();
     }
-    dft16<-1>(v);
-#pragma unroll
-    for (int q = 0; q < 16; ++q) p_s1[q] = v[q];
-    lds_barrier();
-    // forward stage 2 (Ns = 16)
-#pragma unroll
-    for (int q = 0; q < 16; ++q) v[q] = p_rd[272 * q];
-    lds_barrier();
 #pragma unroll
     for (int q = 1; q < 16; ++q) v[q] = cmul(v[q], twiddle<-1>(tw256, k2 * q));
     dft16<-1>(v);
@@ -512,7 +527,20 @@ struct EpiParams {
   int ntiles;  // column tiles per orientation
 };
 
-// Row pass + epilogue. One block per image row y; QR groups of NT = W/16 threads transform QR
+// Image row of block bid. Workgroups are dispatched round-robin over the 8 XCDs, so XCD x runs
+// the blocks with bid % 8 == x; it gets the contiguous rows starting at x * (H / 8) + min(x, H % 8),
+// as many as it has blocks, and consecutive rows share Z's cache lines in that XCD's L2.
+__device__ __forceinline__ int xcd_row(int bid, int H) {
+  const int x = bid % 8, q = H / 8, r = H % 8;
+  return x * q + min(x, r) + bid / 8;
+}
+
+// Threads of a k_last block: QR orientations x (W/2) / R1 threads, R1 the first radix of the
+// W/2-point plan (radix 8 wherever W/2 has three factors of two).
+template <int W, int QR>
+__host__ __device__ constexpr int last_threads() { return QR * ((W / 2) / plan_radix(W / 2, 0, 3)); }
+
+// Row pass + epilogue. One block per image row y; QR groups of NT = (W/2)/R1 threads transform QR
 // orientations of the batch at a time: C2R of row y of Z into W reals (an h = W/2 point complex
 // inverse FFT after the even/odd pre-twist), then every thread updates its 2*PAIRS pixels for
 // those orientations in batch order: strict-greater MIP/argmax, histogram bin
@@ -522,7 +550,7 @@ struct EpiParams {
 // Twiddles live in LDS; the block is sized so that two blocks share a CU and overlap each other's
 // Z gathers.
 template <int W, int QR>
-__global__ __launch_bounds__(QR * (W / 16)) __attribute__((amdgpu_waves_per_eu(4)))
+__global__ __launch_bounds__((last_threads<W, QR>())) __attribute__((amdgpu_waves_per_eu(4)))
 void k_last(const cf* __restrict__ Z, int H, int nb, EpiParams ep, const cf* __restrict__ tw_n,
             const cf* __restrict__ tw_h) {
   constexpr int h = W / 2;
@@ -530,7 +558,7 @@ void k_last(const cf* __restrict__ Z, int H, int nb, EpiParams ep, const cf* __r
   constexpr int NT = h / R1;  // one first-stage butterfly per thread
   constexpr int BT = QR * NT;
   constexpr int PAIRS = h / BT;
-  static_assert(R1 == 8 && h % BT == 0, "row length must be a multiple of 16*QR");
+  static_assert(BT == last_threads<W, QR>() && h % BT == 0 && PAIRS >= 1, "QR must divide the first radix");
   constexpr int HC = 4;  // interleaved histogram copies (bin * HC + lane % HC) spread same-bin atomics
   __shared__ cf buf[QR][lds_len(h + 1)];
   __shared__ cf twh[h];
@@ -538,7 +566,7 @@ void k_last(const cf* __restrict__ Z, int H, int nb, EpiParams ep, const cf* __r
   __shared__ unsigned int hist[512 * HC];
   const int tid0 = threadIdx.x;
   const int bid = blockIdx.x;
-  const int y = (bid % 8) * (H / 8) + bid / 8;
+  const int y = xcd_row(bid, H);
   for (int i = tid0; i < 512 * HC; i += BT) hist[i] = 0u;
   for (int i = tid0; i < h; i += BT) twh[i] = tw_h[i];
   for (int i = tid0; i <= h / 2; i += BT) twn[i] = tw_n[i];
@@ -680,7 +708,7 @@ void k_last_dma(const cf* __restrict__ Z, int H, int nb, EpiParams ep, const cf*
   __shared__ unsigned int hist[512 * HC];
   const int tid0 = threadIdx.x;
   const int bid = blockIdx.x;
-  const int y = (bid % 8) * (H / 8) + bid / 8;
+  const int y = xcd_row(bid, H);
   const int sh = __ffs(ep.ztile) - 1;
   const size_t z_bs = (size_t)ep.ntiles * H * ep.ztile;
   const size_t tile_stride = (size_t)H * ep.ztile;

@@ -19,6 +19,7 @@
 #include "../../include/tm_hip.h"
 #include "../../include/tm_prepare.h"
 #include "tm_kernels.hpp"
+#include "tm_sizes.hpp"
 
 using tmk::cf;
 
@@ -59,6 +60,15 @@ inline float deg_2_rad(float d) { return d * PIf / 180.; }
 inline float rad_2_deg(float r) { return r / (PIf / 180.); }
 inline bool even(int n) { return (n & 1) == 0; }
 inline bool pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+inline bool line_size_compiled(int n) {
+  switch (n) {
+#define TM_CASE(k) case k:
+    TM_LINE_SIZES(TM_CASE)
+#undef TM_CASE
+    return true;
+    default: return false;
+  }
+}
 inline bool is_power_of_two(int n) {  // functions.h:422-426
   if (n == 0) return false;
   return std::ceil(std::log2((float)n)) == std::floor(std::log2((float)n));
@@ -193,14 +203,10 @@ void rows_r2c_t(Dev& d, const float* src, size_t sbs, int srs, cf* dst, size_t d
 }
 void rows_r2c(Dev& d, int N, const float* src, size_t sbs, int srs, cf* dst, size_t dbs, int drs, int nrows, int nb) {
   switch (N) {
-    case 32: rows_r2c_t<32>(d, src, sbs, srs, dst, dbs, drs, nrows, nb); break;
-    case 64: rows_r2c_t<64>(d, src, sbs, srs, dst, dbs, drs, nrows, nb); break;
-    case 128: rows_r2c_t<128>(d, src, sbs, srs, dst, dbs, drs, nrows, nb); break;
-    case 256: rows_r2c_t<256>(d, src, sbs, srs, dst, dbs, drs, nrows, nb); break;
-    case 512: rows_r2c_t<512>(d, src, sbs, srs, dst, dbs, drs, nrows, nb); break;
-    case 1024: rows_r2c_t<1024>(d, src, sbs, srs, dst, dbs, drs, nrows, nb); break;
-    case 2048: rows_r2c_t<2048>(d, src, sbs, srs, dst, dbs, drs, nrows, nb); break;
-    case 4096: rows_r2c_t<4096>(d, src, sbs, srs, dst, dbs, drs, nrows, nb); break;
+#define TM_CASE(n) \
+  case n: rows_r2c_t<n>(d, src, sbs, srs, dst, dbs, drs, nrows, nb); break;
+    TM_LINE_SIZES(TM_CASE)
+#undef TM_CASE
     default: throw PErr{TM_ERR_UNSUPPORTED, "unsupported row length " + std::to_string(N)};
   }
 }
@@ -214,22 +220,19 @@ void cols_t(Dev& d, cf* data, int ncols, size_t bstride, int nb) {
 }
 void cols(Dev& d, int N, cf* data, int ncols, size_t bstride, int nb) {
   switch (N) {
-    case 32: cols_t<32>(d, data, ncols, bstride, nb); break;
-    case 64: cols_t<64>(d, data, ncols, bstride, nb); break;
-    case 128: cols_t<128>(d, data, ncols, bstride, nb); break;
-    case 256: cols_t<256>(d, data, ncols, bstride, nb); break;
-    case 512: cols_t<512>(d, data, ncols, bstride, nb); break;
-    case 1024: cols_t<1024>(d, data, ncols, bstride, nb); break;
-    case 2048: cols_t<2048>(d, data, ncols, bstride, nb); break;
-    case 4096: cols_t<4096>(d, data, ncols, bstride, nb); break;
+#define TM_CASE(n) \
+  case n: cols_t<n>(d, data, ncols, bstride, nb); break;
+    TM_LINE_SIZES(TM_CASE)
+#undef TM_CASE
     default: throw PErr{TM_ERR_UNSUPPORTED, "unsupported column length " + std::to_string(N)};
   }
 }
 
 // Unscaled R2C of a real image (padded layout, nz planes) on the device; returns the spectrum.
 std::vector<std::complex<float>> forward_fft(int device, const Img& a) {
-  P_REQ(pow2(a.nx) && pow2(a.ny) && (a.nz == 1 || pow2(a.nz)), TM_ERR_UNSUPPORTED,
-        "device FFT needs power-of-two sizes");
+  P_REQ(line_size_compiled(a.nx) && line_size_compiled(a.ny) && (a.nz == 1 || line_size_compiled(a.nz)),
+        TM_ERR_UNSUPPORTED, "no compiled device FFT for size " + std::to_string(a.nx) + "x" + std::to_string(a.ny) +
+                                "x" + std::to_string(a.nz));
   Dev d(device);
   const int nh = a.nx / 2 + 1;
   const size_t plane = (size_t)nh * a.ny;

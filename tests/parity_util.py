@@ -1,7 +1,9 @@
 """Comparison helpers shared by the GPU parity tests (the HIP path against the CPU oracle).
 
 Bars (match_template.cpp:996-1169 is the reference semantics):
-  * MIP: |gpu - oracle| <= 1e-4 * |oracle| per pixel (relative, FP32)
+  * MIP: |gpu - oracle| <= 1e-4 * |oracle| + 1e-6 * max |oracle| per pixel. The absolute term is
+    the FP32 rounding of a correlation plane (single planes agree to ~3e-7 of their maximum); it
+    matters only where a maximum over few orientations lies near zero
   * best orientation index: identical wherever the oracle's best value beats its runner-up by more
     than the FP32 noise of the two FFT implementations (checked by recomputing the oracle's CCF at
     the GPU's choice); angle maps identical wherever the index is
@@ -26,9 +28,9 @@ def decode_index(info, gi):
 
 def near_tie_check(mt, ref, got, max_orientations=400):
     """Where the best index differs, the oracle's own CCF at the GPU's orientation must equal the
-    oracle's MIP within FP32 noise (1e-5 relative): the two orientations tie to within the rounding
-    of two FFT implementations, so either choice is a correct argmax. At most max_orientations of the
-    GPU's differing choices are recomputed (the most frequent first)."""
+    oracle's MIP m within FP32 noise, |v - m| <= 1e-5 |m| + 1e-6 max|m|: the two orientations tie to
+    within the rounding of two FFT implementations, so either choice is a correct argmax. At most
+    max_orientations of the GPU's differing choices are recomputed (the most frequent first)."""
     I = mt.info
     bad = np.argwhere(got.best_index != ref["best_index"])
     if bad.size == 0:
@@ -36,27 +38,38 @@ def near_tie_check(mt, ref, got, max_orientations=400):
     g = got.best_index[bad[:, 0], bad[:, 1]]
     uniq, counts = np.unique(g, return_counts=True)
     order = np.argsort(-counts, kind="stable")[:max_orientations]
+    scale = float(np.abs(ref["mip"]).max())
     checked = 0
-    worst = 0.0
+    worst, at = 0.0, None
     for gi in uniq[order]:
         d, pos, k = decode_index(I, gi)
         _, ccf = mt.projection_and_ccf(pos, k, d)
         sel = bad[g == gi]
         v = ccf[sel[:, 0], sel[:, 1]]
         m = ref["mip"][sel[:, 0], sel[:, 1]]
-        worst = max(worst, float(np.max(np.abs(v - m) / np.abs(m))))
+        gap = np.abs(v - m) / (np.abs(m) + 0.1 * scale)  # <= 1e-5 <=> the bar above
+        i = int(np.argmax(gap))
+        if gap[i] > worst:
+            worst, at = float(gap[i]), (tuple(sel[i]), float(m[i]), float(v[i]))
         checked += len(sel)
     print(f"near-tie check: {checked} of {len(bad)} differing pixels over {len(order)} of {len(uniq)} "
-          f"orientations, worst gap {worst:.2e}", flush=True)
+          f"orientations, worst gap / (|m| + 0.1 max|m|) {worst:.2e} at {at}; max|m| {scale:.4e}", flush=True)
     assert worst <= 1e-5
     return checked
 
 
 def compare_search(mt, ref, got, label, max_tie_orientations=400):
     mip_r, mip_g = ref["mip"], got.mip
-    r = np.abs(mip_g - mip_r) / np.maximum(np.abs(mip_r), 1e-30)
+    scale = float(np.abs(mip_r).max())
+    r = np.abs(mip_g - mip_r) / (np.abs(mip_r) + 1e-2 * scale)  # <= 1e-4 <=> the bar above
     same = got.best_index == ref["best_index"]
-    print(f"{label}: MIP max rel {r.max():.2e}; best index identical at {same.mean() * 100:.4f}% of pixels", flush=True)
+    print(f"{label}: MIP max err / (|oracle| + 0.01 max|oracle|) {r.max():.2e}; best index identical at "
+          f"{same.mean() * 100:.4f}% of pixels", flush=True)
+    if r.max() > 1e-4:
+        iy, ix = np.unravel_index(np.argmax(r), r.shape)
+        print(f"{label}: worst MIP pixel ({iy}, {ix}): oracle {mip_r[iy, ix]:.6e} gpu {mip_g[iy, ix]:.6e} "
+              f"index oracle {ref['best_index'][iy, ix]} gpu {got.best_index[iy, ix]}; max |oracle MIP| {scale:.6e}",
+              flush=True)
     assert r.max() <= 1e-4
     near_tie_check(mt, ref, got, max_tie_orientations)
     srel = np.abs(got.sum - ref["sum"]).max() / np.abs(ref["sum"]).max()

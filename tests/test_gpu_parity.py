@@ -2,7 +2,7 @@
 
 Bars (match_template.cpp:996-1169 is the reference semantics):
   * normalised projections and single CCF planes: max |gpu - oracle| / max |oracle| <= 1e-4
-  * MIP: |gpu - oracle| <= 1e-4 * |oracle| per pixel (relative, FP32)
+  * MIP: |gpu - oracle| <= 1e-4 * |oracle| + 1e-6 * max |oracle| per pixel (FP32)
   * best orientation index: identical wherever the oracle's best value beats its runner-up by more
     than the FP32 noise of the two FFT implementations; identical at every detected peak
   * FP64 sum / sum-of-squares: relative 1e-5 (values differ only by FFT rounding)

@@ -68,3 +68,33 @@ def test_search_4096_three_planes_matches_oracle(full):
     # every plane contributed: defocus offsets of all three planes occur in the map
     offs = np.unique(got.defocus)
     assert set(np.round(offs).astype(int)) == {-200, 0, 200}
+
+
+@pytest.fixture(scope="module")
+def full_t512():
+    """T = 512 at 4096^2 (the column pass's general first stage): the 256^3 template zero-padded to
+    512^3 in real space, searched with padding 1."""
+    s = synth.config1(1000)
+    vol = np.zeros((512, 512, 512), np.float32)
+    vol[128:384, 128:384, 128:384] = s.volume
+    mt = O.MatchTemplateOracle(O.MTParams(angular_step=2.5, in_plane_angular_step=5.0, padding=1.0,
+                                          first_search_position=POSITION, last_search_position=POSITION),
+                               s.micrograph, vol)
+    I = mt.info
+    assert (I.W, I.H, I.T, I.P) == (4096, 4096, 512, 512)
+    g = core.TemplateMatchingCore(0)
+    g.init(mt.template_spectrum, mt.input_spectrum, I.T, mt.grid, mt.psi, I.first_pos, I.last_pos,
+           I.histogram_min_scaled, I.histogram_step_scaled, 512, I.max_padding, I.defocus_step, I.pixel_size_step,
+           batch_size=0)
+    yield mt, g
+    g.close()
+
+
+def test_ccf_4096_t512_matches_oracle(full_t512):
+    mt, g = full_t512
+    pf = mt.projection_filter(0)
+    _, ref = mt.projection_and_ccf(POSITION, 7, 0)
+    got = g.debug_ccf(pf, POSITION, 7)
+    e = rel(got, ref)
+    print(f"4096^2 T=512 ccf rel {e:.2e}", flush=True)
+    assert e <= 1e-4
