@@ -135,14 +135,17 @@ class MatchTemplateSearch:
     histogram_number_of_points = 512          # match_template.cpp:45-47
     histogram_min, histogram_max = -12.5, 22.5
 
-    def __init__(self, micrograph: np.ndarray, volume: np.ndarray, p: SearchParams, device: int = 0):
+    def __init__(self, micrograph: np.ndarray, volume: np.ndarray, p: SearchParams, device: int = 0,
+                 template_spectrum: np.ndarray | None = None):
         self.p = p
         self.original_shape = micrograph.shape
         T = volume.shape[0]
         self.T = T
         (self.input_spectrum, self.curve_x, self.curve_y, self.W, self.H, self.rotated,
          self.max_padding) = prepare_input_image(micrograph, T, device)
-        self.template_spectrum = prepare_template(volume, p.padding, device)
+        # a spectrum prepared once (prepare_template) can be shared by the searches of several images
+        self.template_spectrum = (prepare_template(volume, p.padding, device) if template_spectrum is None
+                                  else template_spectrum)
         self.grid = search_grid(p.symmetry, p.angular_step)
         self.psi = psi_list(0.0, 360.0, p.in_plane_angular_step)
         n = self.grid.shape[0]
@@ -179,3 +182,16 @@ class MatchTemplateSearch:
     @property
     def total_correlation_positions(self) -> int:
         return (self.last - self.first + 1) * self.psi.size * (2 * self.n_def_half + 1)
+
+
+def run_concurrent(searches: list, workers: int = 2) -> list:
+    """Runs prepared searches (one per image, each with its own GPU stream) on `workers` host threads,
+    so the kernels of different images overlap on the GPU where one image alone leaves it partly
+    idle. The reference runs one image per process; the per-image results do not depend on the
+    schedule (the kernels are deterministic). Returns the correlations computed per search."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    if workers <= 1:
+        return [s.run() for s in searches]
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        return list(ex.map(lambda s: s.run(), searches))

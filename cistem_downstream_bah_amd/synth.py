@@ -152,10 +152,16 @@ def project_copy(pos: np.ndarray, amp: np.ndarray, m: np.ndarray, cx: float, cy:
 
 def make_synthetic(seed: int, W: int, H: int, T: int, n_atoms: int, atom_radius: float, n_copies: int,
                    snr: float, defocus: float, pixel_size: float = 1.0, kv: float = 300.0, cs_mm: float = 2.7,
-                   amp_contrast: float = 0.07, sigma: float = 1.5, min_dist: float | None = None) -> Synthetic:
+                   amp_contrast: float = 0.07, sigma: float = 1.5, min_dist: float | None = None,
+                   template: tuple | None = None) -> Synthetic:
+    """template = (atom positions, amplitudes, rendered volume) to reuse a model; else one is drawn
+    from the seed first."""
     rng = np.random.default_rng(seed)
-    pos, amp = make_atoms(rng, n_atoms, atom_radius)
-    vol = render_volume(T, pos, amp, sigma)
+    if template is None:
+        pos, amp = make_atoms(rng, n_atoms, atom_radius)
+        vol = render_volume(T, pos, amp, sigma)
+    else:
+        pos, amp, vol = template
     if min_dist is None:
         min_dist = T
     centres, used = place_particles(rng, n_copies, W, H, T // 2, min_dist)
@@ -185,3 +191,24 @@ def config0(seed: int = 0) -> Synthetic:
 def config1(seed: int = 1000) -> Synthetic:
     """4096^2 micrograph, 256^3 template (3000 atoms, r=100), 300 copies, SNR 0.05, defocus 15000 A."""
     return make_synthetic(seed, 4096, 4096, 256, 3000, 100.0, 300, 0.05, 15000.0, min_dist=200)
+
+
+# Multi-micrograph workload: original micrograph sizes (x, y) and, after the reference's size policy,
+# 2048^2, 2048 x 3072 (rotated), 4096^2, 4096 x 2916, 2048 x 1536, 4096^2, 4096^2, 2592^2.
+CONFIG2_SIZES = [(2048, 2048), (3072, 2048), (4096, 4096), (4096, 2880), (2048, 1536), (3840, 3840), (4096, 4096),
+                 (2560, 2560)]
+
+
+def config2(seed: int = 2000, sizes=None) -> list:
+    """One 256^3 template (3000 atoms, r=100, as the 4096^2 workload) and one micrograph per size with
+    U(50, 300) copies, SNR U(0.05, 0.1) and defocus U(8000, 25000) A (recorded in .defocus)."""
+    rng = np.random.default_rng(seed)
+    pos, amp = make_atoms(rng, 3000, 100.0)
+    tpl = (pos, amp, render_volume(256, pos, amp, 1.5))
+    out = []
+    for i, (w, h) in enumerate(sizes or CONFIG2_SIZES):
+        n = int(rng.integers(50, 301))
+        snr = float(rng.uniform(0.05, 0.1))
+        df = float(rng.uniform(8000.0, 25000.0))
+        out.append(make_synthetic(seed + 1 + i, w, h, 256, 3000, 100.0, n, snr, df, min_dist=200, template=tpl))
+    return out
