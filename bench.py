@@ -31,23 +31,25 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def kernel_bytes(name: str, nb: int, W: int, H: int, T: int, P: int, s: int = 8) -> float:
-    """Algorithmic HBM bytes of one launch of a hot kernel processing nb orientations."""
+def kernel_bytes(name: str, nb: int, W: int, H: int, T: int, P: int, s: int = 8, zs: int = 8) -> float:
+    """Algorithmic HBM bytes of one launch of a hot kernel processing nb orientations (spectrum
+    elements of s bytes, inter-pass plane elements of zs bytes)."""
     Wh = W // 2 + 1
     Nc = Wh * H
     if name == "mid_columns":       # read T forward rows per column, read input spectrum once, write Z
-        return nb * (T * Wh + Nc) * s + Nc * s
+        return nb * (T * Wh * s + Nc * zs) + Nc * s
     if name == "last_rows_epilogue":  # read Z, read+write MIP f32, index i32, sum f64, sumsq f64
-        return nb * Nc * s + 48.0 * W * H
+        return nb * Nc * zs + 48.0 * W * H
     if name == "fwd_rows":          # read the T x T projection, write T rows of W/2+1
         return nb * (T * (T + 2) * 4 + T * Wh * s)
     return float("nan")
 
 
-def pipeline_bytes_per_ccf(W: int, H: int, B: int, s: int = 8) -> float:
-    """Survey formula: 5 s Nc + 48 Npix / B bytes per CCF for the whole inner loop."""
+def pipeline_bytes_per_ccf(W: int, H: int, B: int, s: int = 8, zs: int = 8) -> float:
+    """Survey formula 5 s Nc + 48 Npix / B bytes per CCF for the whole inner loop, with the
+    inter-pass plane (written and read once) at zs bytes per element: (3 s + 2 zs) Nc + 48 Npix / B."""
     Nc = (W // 2 + 1) * H
-    return 5 * s * Nc + 48.0 * W * H / B
+    return (3 * s + 2 * zs) * Nc + 48.0 * W * H / B
 
 
 def read_traffic(workload: str, kernel: str, batch: int):
@@ -90,6 +92,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-psi", type=int, default=8)
     ap.add_argument("--small", action="store_true", help="512^2 / 64^3 workload (config 0 shapes)")
+    ap.add_argument("--plane-storage", choices=["fp32", "fp16", "bf16"], default="fp32",
+                    help="storage of the inter-pass plane (fp32: the reference's precision, the headline)")
     ap.add_argument("--workload", choices=["config1", "config2"], default="config1",
                     help="config2: eight micrographs of mixed sizes (own JSON line, not the headline)")
     ap.add_argument("--concurrency", type=str, default="1,2,4",
@@ -137,7 +141,8 @@ def main():
     sp = MT.SearchParams(angular_step=params["angular_step"], in_plane_angular_step=params["psi_step"],
                          padding=params["padding"], defocus_search_range=params["def_range"],
                          defocus_step=params["def_step"], first_search_position=0,
-                         last_search_position=args.positions - 1, batch_size=args.batch)
+                         last_search_position=args.positions - 1, batch_size=args.batch,
+                         plane_storage=args.plane_storage)
     t0 = time.time()
     search = MT.MatchTemplateSearch(s.micrograph, s.volume, sp)
     log(f"prepared on GPU in {time.time() - t0:.1f}s: W={search.W} H={search.H} T={search.T} "
@@ -177,17 +182,18 @@ def main():
     dominant = max(kms, key=lambda n: kms[n])
     nb_avg = ccfs / max(klaunch.get(dominant, 1), 1)
     avg_s = kms[dominant] / klaunch[dominant] / 1e3
-    algo = kernel_bytes(dominant, int(round(nb_avg)), W, H, T, P)
+    zs = 8 if args.plane_storage == "fp32" else 4
+    algo = kernel_bytes(dominant, int(round(nb_avg)), W, H, T, P, zs=zs)
     achieved = algo / avg_s
     for name in per_kernel:
-        kb = kernel_bytes(name, int(round(nb_avg)), W, H, T, P)
+        kb = kernel_bytes(name, int(round(nb_avg)), W, H, T, P, zs=zs)
         if not math.isnan(kb):
             per_kernel[name]["gbps"] = round(kb / (per_kernel[name]["avg_ms"] / 1e3) / 1e9, 1)
-    traffic = read_traffic(workload, dominant, B)
+    traffic = read_traffic(workload, dominant, B) if args.plane_storage == "fp32" else None
     roofline = {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic, "kernel": dominant,
                 "algorithmic_bytes_per_launch": algo, "orientations_per_launch": round(nb_avg, 2)}
-    pipe = pipeline_bytes_per_ccf(W, H, B)
+    pipe = pipeline_bytes_per_ccf(W, H, B, zs=zs)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -214,10 +220,12 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic (seeded Gaussian pseudo-atom template and micrograph)",
-        "config": {"workload": workload, "image": [W, H], "template": T, "padded_template": P,
+        "config": {"workload": workload + ("" if args.plane_storage == "fp32" else "_" + args.plane_storage + "_plane"),
+                   "image": [W, H], "template": T, "padded_template": P,
                    "orientations_per_step": search.core.n_psi * (search.last - search.first + 1),
-                   "defocus_planes": len(planes), "batch": B, "precision": "fp32 storage and butterflies, "
-                   "fp64 sum/sum-of-squares"},
+                   "defocus_planes": len(planes), "batch": B,
+                   "precision": f"{args.plane_storage} storage of the inter-pass plane, fp32 spectra and "
+                                "butterflies, fp64 sum/sum-of-squares"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": per_kernel,

@@ -45,7 +45,9 @@ class InitParams(ctypes.Structure):
                 ("histogram_min_scaled", ctypes.c_double), ("histogram_step_scaled", ctypes.c_double),
                 ("histogram_number_of_bins", ctypes.c_int), ("max_padding", ctypes.c_int),
                 ("defocus_step", ctypes.c_float), ("pixel_size_step", ctypes.c_float),
-                ("batch_size", ctypes.c_int)]
+                ("batch_size", ctypes.c_int), ("plane_storage", ctypes.c_int)]
+
+PLANE_STORAGE = {"fp32": 0, "fp16": 1, "bf16": 2}
 
 
 def lib():
@@ -126,11 +128,12 @@ class TemplateMatchingCore:
              search_positions: np.ndarray, psi_list: np.ndarray, first_search_position: int,
              last_search_position: int, histogram_min_scaled: float, histogram_step_scaled: float,
              histogram_number_of_bins: int = 512, max_padding: int = 0, defocus_step: float = 0.0,
-             pixel_size_step: float = 0.0, batch_size: int = 0) -> None:
+             pixel_size_step: float = 0.0, batch_size: int = 0, plane_storage: str = "fp32") -> None:
         """TemplateMatchingCore::Init (TemplateMatchingCore.cu:49-138).
 
         template_spectrum: (P, P, P/2+1) complex64; input_spectrum: (H, W/2+1) complex64;
-        search_positions: (n, 2) float32 {phi, theta}; psi_list: float32 in-plane angles."""
+        search_positions: (n, 2) float32 {phi, theta}; psi_list: float32 in-plane angles;
+        plane_storage: "fp32" (default), "fp16" or "bf16" storage of the inter-pass plane."""
         tpl = np.ascontiguousarray(template_spectrum, np.complex64)
         inp = np.ascontiguousarray(input_spectrum, np.complex64)
         grid = np.ascontiguousarray(search_positions, np.float32)
@@ -140,7 +143,7 @@ class TemplateMatchingCore:
         W = 2 * (Wh - 1)
         p = InitParams(P, projection_size, W, H, grid.shape[0], psi.size, first_search_position,
                        last_search_position, histogram_min_scaled, histogram_step_scaled, histogram_number_of_bins,
-                       max_padding, defocus_step, pixel_size_step, batch_size)
+                       max_padding, defocus_step, pixel_size_step, batch_size, PLANE_STORAGE[plane_storage])
         _check(lib().tm_init(self.h, ctypes.byref(p), _f32(tpl.view(np.float32)), _f32(inp.view(np.float32)),
                              _f32(grid), _f32(psi)))
         self.W, self.H, self.T, self.P = W, H, projection_size, P

@@ -159,6 +159,8 @@ struct tm_core {
   bool initialised = false;
   tm_init_params prm{};
   int W = 0, H = 0, Wh = 0, T = 0, Th = 0, P = 0, Ph = 0, B = 16;
+  int zst = TM_PLANE_FP32;  // storage of the inter-pass plane Z
+  float zscale = 1.0f;      // power of two applied to 16-bit Z values before storing
   int n_pos_local = 0, n_psi = 0, n_orient = 0;
   std::vector<float> grid, psi;      // host copies ({phi, theta} per position; psi list)
   std::vector<int2> planes;          // (size_i, defocus_i) per plane index, in call order
@@ -318,17 +320,26 @@ int z_tile(int H) { return H >= 4096 ? 4 : mid_kc(H); }
 bool column_rows_at_zero(int H) { return H == 4096; }
 int z_tiles(const tm_core* h) { return (h->Wh + z_tile(h->H) - 1) / z_tile(h->H); }
 
+template <int ZST>
+void launch_mid4096(tm_core* h, int nb) {
+  const int ntiles = mid_tiles(h);
+  const int grid = ((ntiles + 7) / 8) * 8;
+  if (h->T <= 256)
+    hipLaunchKernelGGL((tmk::k_mid4096<2, true, 4, true, true, ZST>), dim3(grid), dim3(2 * 256), 0, h->stream,
+                       h->rows.p, h->inpT.p, h->Z.p, h->Wh, h->T, nb, ntiles, h->twiddle(4096), h->zscale);
+  else
+    hipLaunchKernelGGL((tmk::k_mid4096<2, true, 4, true, false, ZST>), dim3(grid), dim3(2 * 256), 0, h->stream,
+                       h->rows.p, h->inpT.p, h->Z.p, h->Wh, h->T, nb, ntiles, h->twiddle(4096), h->zscale);
+  check_launch("k_mid4096");
+}
+
 void launch_mid(tm_core* h, int nb) {
   if (h->H == 4096) {
-    const int ntiles = mid_tiles(h);
-    const int grid = ((ntiles + 7) / 8) * 8;
-    if (h->T <= 256)
-      hipLaunchKernelGGL((tmk::k_mid4096<2, true, 4, true, true>), dim3(grid), dim3(2 * 256), 0, h->stream, h->rows.p,
-                         h->inpT.p, h->Z.p, h->Wh, h->T, nb, ntiles, h->twiddle(4096));
-    else
-      hipLaunchKernelGGL((tmk::k_mid4096<2, true, 4, true, false>), dim3(grid), dim3(2 * 256), 0, h->stream, h->rows.p,
-                         h->inpT.p, h->Z.p, h->Wh, h->T, nb, ntiles, h->twiddle(4096));
-    check_launch("k_mid4096");
+    switch (h->zst) {
+      case TM_PLANE_FP16: launch_mid4096<tmk::ZS_F16>(h, nb); break;
+      case TM_PLANE_BF16: launch_mid4096<tmk::ZS_BF16>(h, nb); break;
+      default: launch_mid4096<tmk::ZS_F32>(h, nb); break;
+    }
     return;
   }
   switch (h->H) {
@@ -347,9 +358,17 @@ void launch_last_t(tm_core* h, int nb, tmk::EpiParams ep) {
   ep.ztile = z_tile(h->H);
   ep.ntiles = z_tiles(h);
   ep.hstep_inv = 1.0 / ep.hstep;
+  ep.zinv = 1.0f / h->zscale;  // a power of two: exact
   if constexpr (WW >= 1024 && (WW & (WW - 1)) == 0) {
-    hipLaunchKernelGGL((tmk::k_last_dma<WW, QR>), dim3(h->H), dim3(QR * (WW / 16)), 0, h->stream, h->Z.p, h->H, nb,
-                       ep, h->twiddle(WW), h->twiddle(WW / 2));
+    if (h->zst == TM_PLANE_FP16)
+      hipLaunchKernelGGL((tmk::k_last_dma<WW, QR, tmk::ZS_F16>), dim3(h->H), dim3(QR * (WW / 16)), 0, h->stream,
+                         h->Z.p, h->H, nb, ep, h->twiddle(WW), h->twiddle(WW / 2));
+    else if (h->zst == TM_PLANE_BF16)
+      hipLaunchKernelGGL((tmk::k_last_dma<WW, QR, tmk::ZS_BF16>), dim3(h->H), dim3(QR * (WW / 16)), 0, h->stream,
+                         h->Z.p, h->H, nb, ep, h->twiddle(WW), h->twiddle(WW / 2));
+    else
+      hipLaunchKernelGGL((tmk::k_last_dma<WW, QR>), dim3(h->H), dim3(QR * (WW / 16)), 0, h->stream, h->Z.p, h->H,
+                         nb, ep, h->twiddle(WW), h->twiddle(WW / 2));
   } else {
     hipLaunchKernelGGL((tmk::k_last<WW, QR>), dim3(h->H), dim3(tmk::last_threads<WW, QR>()), 0, h->stream, h->Z.p,
                        h->H, nb, ep, h->twiddle(WW), h->twiddle(WW / 2));
@@ -554,6 +573,15 @@ int tm_init(tm_core* h, const tm_init_params* p, const float* template_spectrum,
     h->T = T; h->Th = T / 2 + 1;
     h->P = P; h->Ph = P / 2 + 1;
     h->B = p->batch_size > 0 ? p->batch_size : 32;
+    TM_REQUIRE(p->plane_storage == TM_PLANE_FP32 || p->plane_storage == TM_PLANE_FP16 ||
+                   p->plane_storage == TM_PLANE_BF16, TM_ERR_INVALID, "plane_storage must be 0, 1 or 2");
+    TM_REQUIRE(p->plane_storage == TM_PLANE_FP32 || (H == 4096 && pow2(W) && W >= 1024), TM_ERR_UNSUPPORTED,
+               "16-bit plane storage needs image height 4096 and a power-of-two width >= 1024");
+    h->zst = p->plane_storage;
+    // 16-bit values: the plane's elements are ~ 1 / sqrt(W^2 H) (correlations ~ 1 / sqrt(W H) after
+    // the row transform of W points); scale them to ~1/4, leaving FP16 headroom above and below
+    h->zscale = h->zst == TM_PLANE_FP32 ? 1.0f
+                                       : ldexpf(1.0f, (int)lround(0.5 * std::log2((double)W * W * H)) - 2);
     TM_REQUIRE(h->B <= 256, TM_ERR_INVALID, "batch_size must be <= 256");
     h->n_pos_local = p->last_search_position - p->first_search_position + 1;
     h->n_psi = p->n_psi;
@@ -623,7 +651,8 @@ int tm_init(tm_core* h, const tm_init_params* p, const float* template_spectrum,
     h->slice.alloc((size_t)B * P * h->Ph);
     h->projT.alloc((size_t)B * T * h->Th);
     h->rows.alloc((size_t)B * T * h->Wh);
-    h->Z.alloc((size_t)B * z_tiles(h) * H * z_tile(H));
+    const size_t z_elems = (size_t)B * z_tiles(h) * H * z_tile(H);
+    h->Z.alloc(h->zst == TM_PLANE_FP32 ? z_elems : (z_elems + 1) / 2);  // cf units; 16-bit: 4 bytes each
 
     // state
     const size_t npx = (size_t)W * H;

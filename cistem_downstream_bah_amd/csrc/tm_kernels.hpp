@@ -300,6 +300,47 @@ __global__ __launch_bounds__(NT) void k_normalize(float* __restrict__ proj, int 
   }
 }
 
+// ---------------------------------------------------------------- inter-pass plane storage
+// Storage of the plane Z between the column and row transforms: ZS_F32 (FP32 complex, 8 bytes),
+// ZS_F16 or ZS_BF16 (two 16-bit halves packed in 4 bytes, real in the low half). 16-bit values are
+// stored multiplied by a power of two (exact) that brings the plane's typical magnitude near 1, so
+// FP16 keeps its full precision; the row pass multiplies by the inverse after loading.
+enum { ZS_F32 = 0, ZS_F16 = 1, ZS_BF16 = 2 };
+
+template <int ZST>
+__device__ __forceinline__ unsigned short to_half_bits(float x) {
+  if constexpr (ZST == ZS_F16) {
+    const _Float16 h = (_Float16)x;  // round to nearest even
+    unsigned short b;
+    __builtin_memcpy(&b, &h, sizeof(b));
+    return b;
+  } else {
+    // bfloat16, round to nearest even (finite values)
+    unsigned u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+  }
+}
+template <int ZST>
+__device__ __forceinline__ float from_half_bits(unsigned b) {
+  if constexpr (ZST == ZS_F16) {
+    const unsigned short s = (unsigned short)b;
+    _Float16 h;
+    __builtin_memcpy(&h, &s, sizeof(h));
+    return (float)h;
+  } else {
+    return __uint_as_float(b << 16);
+  }
+}
+template <int ZST>
+__device__ __forceinline__ unsigned pack_half2(cf v, float scale) {
+  return (unsigned)to_half_bits<ZST>(v.x * scale) | ((unsigned)to_half_bits<ZST>(v.y * scale) << 16);
+}
+template <int ZST>
+__device__ __forceinline__ cf unpack_half2(unsigned w, float inv_scale) {
+  return cmake(from_half_bits<ZST>(w & 0xFFFFu) * inv_scale, from_half_bits<ZST>(w >> 16) * inv_scale);
+}
+
 // ---------------------------------------------------------------- column pass
 // Tile of KC adjacent columns kx of the W x H spectrum, NTC threads per column. For each
 // orientation b: place the T forward-row-FFT rows at y = H/2 - T/2 + r, forward column FFT,
@@ -376,10 +417,10 @@ __global__ __launch_bounds__(KC* NTC) void k_mid(const cf* __restrict__ rows, co
 // padded LDS index lpad(i) = i + i/16 the three access patterns are a per-thread base plus
 // compile-time offsets: lpad(16 j + q) = 17 j + q, lpad(j + 256 q) = lpad(j) + 272 q and
 // lpad(b2 + 16 q) = lpad(b2) + 17 q for b2 = (j / 16) 256 + j % 16.
-template <int KC, bool PREFETCH, int ZT = KC, bool NTS = true, bool DIRECT = false>
+template <int KC, bool PREFETCH, int ZT = KC, bool NTS = true, bool DIRECT = false, int ZST = ZS_F32>
 __global__ __launch_bounds__(KC * 256, 4) void k_mid4096(const cf* __restrict__ rows, const cf* __restrict__ inpT,
                                                   cf* __restrict__ Z, int Wh, int T, int nb, int ntiles,
-                                                  const cf* __restrict__ tw) {
+                                                  const cf* __restrict__ tw, float zscale) {
   constexpr int H = 4096, M = 256;
   constexpr int CS = lds_len(H) + 8;  // column stride: shifts the columns by 16 banks
   static_assert(KC * M <= 1024, "KC columns of 256 butterflies per block");
@@ -500,14 +541,24 @@ __global__ __launch_bounds__(KC * 256, 4) void k_mid4096(const cf* __restrict__ 
     for (int q = 1; q < 16; ++q)
       v[q] = cmul(v[q], twiddle<+1>(tw256, (j / 16) * q), twiddle<+1>(twbq, k2 * 16 + q));
     dft16<+1>(v);
-    unsigned long long* zt = reinterpret_cast<unsigned long long*>(Z + b * z_bs + (size_t)(kx / ZT) * H * ZT) +
-                             (size_t)j * ZT + kx % ZT;
+    const size_t zoff = b * z_bs + (size_t)(kx / ZT) * H * ZT + (size_t)j * ZT + kx % ZT;  // elements
+    if constexpr (ZST == ZS_F32) {
+      unsigned long long* zt = reinterpret_cast<unsigned long long*>(Z) + zoff;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      unsigned long long bits;
-      __builtin_memcpy(&bits, &v[q], sizeof(bits));
-      if constexpr (NTS) __builtin_nontemporal_store(bits, zt + (size_t)M * ZT * q);
-      else zt[(size_t)M * ZT * q] = bits;
+      for (int q = 0; q < 16; ++q) {
+        unsigned long long bits;
+        __builtin_memcpy(&bits, &v[q], sizeof(bits));
+        if constexpr (NTS) __builtin_nontemporal_store(bits, zt + (size_t)M * ZT * q);
+        else zt[(size_t)M * ZT * q] = bits;
+      }
+    } else {
+      unsigned* zt = reinterpret_cast<unsigned*>(Z) + zoff;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const unsigned w = pack_half2<ZST>(v[q], zscale);
+        if constexpr (NTS) __builtin_nontemporal_store(w, zt + (size_t)M * ZT * q);
+        else zt[(size_t)M * ZT * q] = w;
+      }
     }
     lds_barrier();  // LDS reads of this orientation complete before the next one writes
   }
@@ -525,6 +576,7 @@ struct EpiParams {
   int g0;      // global orientation index of batch entry 0
   int ztile;   // column-tile width of Z (power of two)
   int ntiles;  // column tiles per orientation
+  float zinv;  // inverse of the scale of 16-bit Z storage (1 for FP32)
 };
 
 // Image row of block bid. Workgroups are dispatched round-robin over the 8 XCDs, so XCD x runs
@@ -682,13 +734,14 @@ typedef __attribute__((address_space(3))) void* lds_vptr;
 typedef const __attribute__((address_space(1))) void* global_vptr;
 
 // k_last with the Z gather double-buffered through direct global-to-LDS loads
-// (global_load_lds_dwordx4, 16 bytes = two adjacent columns per lane): while the block transforms
+// (global_load_lds_dwordx4, 16 bytes = two FP32 or four 16-bit adjacent elements of a tile row per
+// lane; 16-bit elements are widened and unscaled when the first stage reads them): while the block transforms
 // and accumulates orientations b0..b0+QR-1, row y of the next QR orientations streams into a second
 // LDS buffer. Arithmetic and pixel ownership are those of k_last; the pre-twist and FFT twiddles come
 // from factored tables so that two blocks still fit in one CU's LDS. Each group of NT = W/16 threads
 // must be whole wavefronts (W >= 1024), because a direct load writes one wavefront's 64 lanes to
 // consecutive LDS addresses.
-template <int W, int QR>
+template <int W, int QR, int ZST = ZS_F32>
 __global__ __launch_bounds__(QR * (W / 16)) __attribute__((amdgpu_waves_per_eu(4)))
 void k_last_dma(const cf* __restrict__ Z, int H, int nb, EpiParams ep, const cf* __restrict__ tw_n,
                 const cf* __restrict__ tw_h) {
@@ -697,11 +750,13 @@ void k_last_dma(const cf* __restrict__ Z, int H, int nb, EpiParams ep, const cf*
   constexpr int NT = h / R1;
   constexpr int BT = QR * NT;
   constexpr int PAIRS = h / BT;
-  constexpr int NL = h / 2 + 1;            // 16-byte loads per row: elements 0..h plus one pad element
-  constexpr int NR = (NL + NT - 1) / NT;   // load rounds per thread
+  constexpr int EPL = ZST == ZS_F32 ? 2 : 4;  // Z elements per 16-byte load (within one tile row)
+  constexpr int ESZ = 16 / EPL;               // bytes per stored element
+  constexpr int NL = h / EPL + 1;             // 16-byte loads per row: elements 0..h plus padding
+  constexpr int NR = (NL + NT - 1) / NT;      // load rounds per thread
   static_assert(R1 == 8 && h % BT == 0 && NT % 64 == 0, "groups of whole wavefronts, one butterfly each");
   constexpr int HC = 4;
-  __shared__ cf gbuf[QR][2 * NL];          // gathered rows in element order
+  __shared__ __align__(16) unsigned char gbuf[QR][16 * NL];  // gathered rows in element order
   __shared__ cf buf[QR][lds_len(h)];       // FFT work buffers (padded index)
   __shared__ cf fft_lo[64], fft_hi[h / 64];
   __shared__ cf pre_lo[64], pre_hi[h / 128 + 1];
@@ -712,23 +767,29 @@ void k_last_dma(const cf* __restrict__ Z, int H, int nb, EpiParams ep, const cf*
   const int sh = __ffs(ep.ztile) - 1;
   const size_t z_bs = (size_t)ep.ntiles * H * ep.ztile;
   const size_t tile_stride = (size_t)H * ep.ztile;
-  const cf* const zrow = Z + (size_t)y * ep.ztile;
-  // start the loads of row y of orientations b..b+QR-1 (group grp takes orientation b + grp)
+  const unsigned char* const zrow = reinterpret_cast<const unsigned char*>(Z) + (size_t)y * ep.ztile * ESZ;
+  // start the loads of row y of orientations b..b+QR-1 (group grp takes orientation b + grp); one
+  // load takes EPL consecutive elements of a tile row (the tile width is at least EPL)
   auto gather = [&](int b, int tid) {
     const int grp = tid / NT, t = tid % NT;
     if (b + grp < nb) {
-      const cf* zb = zrow + (size_t)(b + grp) * z_bs;
-      cf* const dst = gbuf[grp] + 2 * (t & ~63);
+      const unsigned char* zb = zrow + (size_t)(b + grp) * z_bs * ESZ;
+      unsigned char* const dst = gbuf[grp] + 16 * (t & ~63);
 #pragma unroll
       for (int r = 0; r < NR; ++r) {
         const int L = r * NT + t;
         if (L < NL) {
-          const int e = 2 * L;
-          const cf* src = zb + (size_t)(e >> sh) * tile_stride + (e & (ep.ztile - 1));
-          __builtin_amdgcn_global_load_lds((global_vptr)src, (lds_vptr)(dst + 2 * r * NT), 16, 0, 0);
+          const int e = EPL * L;
+          const unsigned char* src = zb + ((size_t)(e >> sh) * tile_stride + (e & (ep.ztile - 1))) * ESZ;
+          __builtin_amdgcn_global_load_lds((global_vptr)src, (lds_vptr)(dst + 16 * r * NT), 16, 0, 0);
         }
       }
     }
+  };
+  // element k of a gathered row
+  auto zload = [&](const unsigned char* gb, int k) -> cf {
+    if constexpr (ZST == ZS_F32) return reinterpret_cast<const cf*>(gb)[k];
+    else return unpack_half2<ZST>(reinterpret_cast<const unsigned*>(gb)[k], ep.zinv);
   };
   gather(0, tid0);
   for (int i = tid0; i < 512 * HC; i += BT) hist[i] = 0u;
@@ -771,12 +832,12 @@ void k_last_dma(const cf* __restrict__ Z, int H, int nb, EpiParams ep, const cf*
     // for k > h/2, and Z_0 = (Re X_0 + Re X_h) + i (Re X_0 - Re X_h)
     cf v[R1];
     if (active) {
-      const cf* gb = gbuf[grp];
+      const unsigned char* gb = gbuf[grp];
 #pragma unroll
       for (int q = 0; q < R1; ++q) {
         const int k = t + NT * q;
-        const cf Xk = gb[k];
-        const cf Xm = gb[h - k];
+        const cf Xk = zload(gb, k);
+        const cf Xm = zload(gb, h - k);
         if (k == 0) {
           v[q] = cmake(Xk.x + Xm.x, Xk.x - Xm.x);
         } else {

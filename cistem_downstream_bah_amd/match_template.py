@@ -127,6 +127,7 @@ class SearchParams:
     first_search_position: int = -1
     last_search_position: int = -1
     batch_size: int = 0
+    plane_storage: str = "fp32"  # "fp16"/"bf16": 16-bit inter-pass plane (image height 4096 only)
 
 
 class MatchTemplateSearch:
@@ -163,7 +164,7 @@ class MatchTemplateSearch:
         self.core = core.TemplateMatchingCore(device)
         self.core.init(self.template_spectrum, self.input_spectrum, T, self.grid, self.psi, self.first, self.last,
                        self.histogram_min_scaled, self.histogram_step_scaled, 512, self.max_padding,
-                       self.defocus_step, 0.02, p.batch_size)
+                       self.defocus_step, 0.02, p.batch_size, p.plane_storage)
 
     def filter_for(self, defocus_i: int) -> np.ndarray:
         p = self.p

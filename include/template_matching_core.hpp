@@ -41,7 +41,8 @@ class TemplateMatchingCore {
             int image_y, int projection_size, float pixel_size_step, float defocus_step, float psi_max,
             float psi_start, float psi_step, const float* search_positions, int number_of_search_positions,
             double histogram_min_scaled, double histogram_step_scaled, int histogram_number_of_bins,
-            int max_padding, int first_search_position, int last_search_position, int batch_size = 0) {
+            int max_padding, int first_search_position, int last_search_position, int batch_size = 0,
+            int plane_storage = TM_PLANE_FP32) {
     psi_.clear();
     for (float p = psi_start; p <= psi_max; p += psi_step) psi_.push_back(p);
     tm_init_params prm{};
@@ -60,6 +61,7 @@ class TemplateMatchingCore {
     prm.defocus_step = defocus_step;
     prm.pixel_size_step = pixel_size_step;
     prm.batch_size = batch_size;
+    prm.plane_storage = plane_storage;
     check(tm_init(h_, &prm, template_reconstruction, input_image, search_positions, psi_.data()));
     image_x_ = image_x;
     image_y_ = image_y;

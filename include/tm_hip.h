@@ -47,7 +47,16 @@ typedef struct tm_init_params {
   float defocus_step;            /* best_defocus map stores defocus_i * defocus_step                 */
   float pixel_size_step;         /* best_pixel_size map stores size_i * pixel_size_step              */
   int batch_size;                /* orientations per launch; 0 selects the default (32)              */
+  int plane_storage;             /* storage of the plane between the column and row transforms:
+                                    TM_PLANE_FP32 (0, the reference's precision), TM_PLANE_FP16 or
+                                    TM_PLANE_BF16 (half the traffic; image height 4096 and a
+                                    power-of-two width >= 1024 only, else TM_ERR_UNSUPPORTED). No
+                                    counterpart in the reference, whose planes are FP32.           */
 } tm_init_params;
+
+#define TM_PLANE_FP32 0
+#define TM_PLANE_FP16 1
+#define TM_PLANE_BF16 2
 
 /* Number of visible HIP devices. Replaces DeviceManager::Init's device query
  * (src/gpu/DeviceManager.cu:27-124). */

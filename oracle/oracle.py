@@ -56,8 +56,8 @@ def lib():
         L.tmo_mt_create.restype = vp
         L.tmo_mt_destroy.argtypes = [vp, ctypes.c_int]
         L.tmo_mt_info.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_long), F64P]
-        L.tmo_mt_get.argtypes = [vp, ctypes.c_int, F32P]
-        L.tmo_mt_projection_filter.argtypes = [vp, ctypes.c_int, F32P]
+        L.tmo_mt_get.argtypes = [vp, ctypes.c_int, ctypes.c_int, F32P]
+        L.tmo_mt_projection_filter.argtypes = [vp, ctypes.c_int, ctypes.c_int, F32P]
         L.tmo_mt_projection_ccf.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, F32P, F32P]
         L.tmo_mt_run.argtypes = [vp] + [ctypes.c_int] * 8
         L.tmo_mt_results.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
@@ -269,7 +269,7 @@ class MatchTemplateOracle:
 
     def _get(self, which, n):
         out = np.zeros(n, np.float32)
-        lib().tmo_mt_get(self.h, which, _fp(out))
+        lib().tmo_mt_get(self.h, self.fp64, which, _fp(out))
         return out
 
     @property
@@ -300,7 +300,7 @@ class MatchTemplateOracle:
     def projection_filter(self, defocus_i: int = 0) -> np.ndarray:
         T = self.info.T
         out = np.zeros((T, T // 2 + 1), np.complex64)
-        _check(lib().tmo_mt_projection_filter(self.h, defocus_i, _fp(out.view(np.float32))))
+        _check(lib().tmo_mt_projection_filter(self.h, self.fp64, defocus_i, _fp(out.view(np.float32))))
         return out
 
     def projection_and_ccf(self, pos: int, psi_idx: int, defocus_i: int = 0, want_ccf: bool = True):

@@ -679,23 +679,31 @@ void tmo_mt_info(void* hv, int fp64, long* info, double* dinfo) {
 }
 // which: 0 input spectrum (numel floats), 1 template spectrum, 2 grid (2*npos), 3 psi list,
 // 4 whitening curve y (npts), 5 whitening curve x
-void tmo_mt_get(void* hv, int which, float* out) {
-  auto& mt = ((Handle32*)hv)->mt;
-  switch (which) {
-    case 0: std::memcpy(out, mt.input_image.v.data(), sizeof(float) * mt.input_image.real_memory_allocated); break;
-    case 1: std::memcpy(out, mt.template_reconstruction.v.data(), sizeof(float) * mt.template_reconstruction.real_memory_allocated); break;
-    case 2: std::memcpy(out, mt.grid.data(), sizeof(float) * mt.grid.size()); break;
-    case 3: std::memcpy(out, mt.psi.data(), sizeof(float) * mt.psi.size()); break;
-    case 4: std::memcpy(out, mt.whitening_filter.y.data(), sizeof(float) * mt.whitening_filter.n()); break;
-    case 5: std::memcpy(out, mt.whitening_filter.x.data(), sizeof(float) * mt.whitening_filter.n()); break;
-  }
+// Arrays of either precision, rounded to float for the caller.
+void tmo_mt_get(void* hv, int fp64, int which, float* out) {
+  auto get = [&](auto& mt) {
+    auto copy = [&](const auto* src, size_t n) { for (size_t i = 0; i < n; ++i) out[i] = (float)src[i]; };
+    switch (which) {
+      case 0: copy(mt.input_image.v.data(), (size_t)mt.input_image.real_memory_allocated); break;
+      case 1: copy(mt.template_reconstruction.v.data(), (size_t)mt.template_reconstruction.real_memory_allocated); break;
+      case 2: copy(mt.grid.data(), mt.grid.size()); break;
+      case 3: copy(mt.psi.data(), mt.psi.size()); break;
+      case 4: copy(mt.whitening_filter.y.data(), (size_t)mt.whitening_filter.n()); break;
+      case 5: copy(mt.whitening_filter.x.data(), (size_t)mt.whitening_filter.n()); break;
+    }
+  };
+  if (fp64) get(((Handle64*)hv)->mt);
+  else get(((Handle32*)hv)->mt);
 }
-int tmo_mt_projection_filter(void* hv, int defocus_i, float* out) {
+int tmo_mt_projection_filter(void* hv, int fp64, int defocus_i, float* out) {
   return guarded([&] {
-    auto& mt = ((Handle32*)hv)->mt;
-    Image<float> pf;
-    mt.ProjectionFilter(defocus_i, pf);
-    std::memcpy(out, pf.v.data(), sizeof(float) * pf.real_memory_allocated);
+    auto filt = [&](auto& mt) {
+      Image<typename std::remove_reference<decltype(mt)>::type::real_type> pf;
+      mt.ProjectionFilter(defocus_i, pf);
+      for (long i = 0; i < pf.real_memory_allocated; ++i) out[i] = (float)pf.v[i];
+    };
+    if (fp64) filt(((Handle64*)hv)->mt);
+    else filt(((Handle32*)hv)->mt);
   });
 }
 // Normalised projection T x T (padded layout, (T+2)*T floats) and optionally the CCF plane (numel floats).

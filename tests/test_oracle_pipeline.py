@@ -119,3 +119,21 @@ def test_inserted_copy_peaks_at_its_position():
     _, c = mt.projection_and_ccf(arg[0], arg[1], 0)
     iy, ix = np.unravel_index(np.argmax(c), c.shape)
     assert abs(ix - pt.x) <= 1.5 and abs(iy - pt.y) <= 1.5
+
+
+def test_fp64_mode_agrees_with_fp32():
+    """The oracle's FP64 instantiation (the reference for storage-precision error reports) exports the
+    same inputs and finds the same maps as the FP32 one, to FP32 rounding."""
+    s = synth.config0(0)
+    p = O.MTParams(angular_step=10.0, in_plane_angular_step=10.0, padding=1.0, first_search_position=100,
+                   last_search_position=101)
+    a = O.MatchTemplateOracle(p, s.micrograph, s.volume)
+    b = O.MatchTemplateOracle(p, s.micrograph, s.volume, fp64=True)
+    for name in ("template_spectrum", "input_spectrum"):
+        x, y = getattr(a, name), getattr(b, name)
+        assert np.abs(x - y).max() <= 1e-6 * np.abs(x).max(), name
+    assert np.abs(a.projection_filter(0) - b.projection_filter(0)).max() <= 1e-6
+    assert np.array_equal(a.grid, b.grid) and np.array_equal(a.psi, b.psi)
+    ra, rb = a.run(threads=4), b.run(threads=4)
+    assert np.abs(ra["mip"] - rb["mip"]).max() <= 1e-5 * np.abs(rb["mip"]).max()
+    assert (ra["best_index"] == rb["best_index"]).mean() > 0.99
