@@ -24,7 +24,7 @@ I32P = ctypes.POINTER(ctypes.c_int)
 I64P = ctypes.POINTER(ctypes.c_longlong)
 
 # Every symbol include/tm_hip.h declares.
-EXPORTS = ["tm_device_count", "tm_create", "tm_destroy", "tm_init", "tm_run_inner_loop", "tm_get_results",
+EXPORTS = ["tm_device_count", "tm_create", "tm_destroy", "tm_init", "tm_run_inner_loop", "tm_get_results", "tm_merge",
            "tm_reset", "tm_finalize", "tm_debug_projection", "tm_debug_ccf", "tm_set_timing", "tm_get_timing",
            "tm_last_error"]
 
@@ -66,6 +66,7 @@ def lib():
         L.tm_run_inner_loop.argtypes = [vp, F32P, ctypes.c_int, ctypes.c_int, PROGRESS_FN, vp, I64P]
         L.tm_get_results.argtypes = [vp, F32P, F32P, F32P, F32P, F32P, F32P, F64P, F64P, I32P, I64P, I64P]
         L.tm_reset.argtypes = [vp]
+        L.tm_merge.argtypes = [vp, vp]
         L.tm_finalize.argtypes = [vp, ctypes.c_longlong, F32P, F32P, F32P, F32P]
         L.tm_debug_projection.argtypes = [vp, F32P, ctypes.c_int, ctypes.c_int, F32P]
         L.tm_debug_ccf.argtypes = [vp, F32P, ctypes.c_int, ctypes.c_int, F32P]
@@ -172,6 +173,11 @@ class TemplateMatchingCore:
                                     idx.ctypes.data_as(I32P), hist.ctypes.data_as(I64P), ctypes.byref(n)))
         crop = lambda a: np.ascontiguousarray(a[:, :W])
         return Results(*[crop(a) for a in f], crop(s), crop(q), crop(idx), hist, n.value)
+
+    def merge(self, other: "TemplateMatchingCore") -> None:
+        """Folds another handle's results (same search, other position range, same device) into
+        this one (tm_merge; AggregatedTemplateResult::AddResult, match_template.cpp:2015-2103)."""
+        _check(lib().tm_merge(self.h, other.h))
 
     def reset(self) -> None:
         _check(lib().tm_reset(self.h))

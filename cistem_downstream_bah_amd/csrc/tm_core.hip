@@ -682,8 +682,12 @@ int tm_run_inner_loop(tm_core* h, const float* projection_filter, int size_i, in
     require_ready(h);
     upload_filter(h, projection_filter);
     const int plane = (int)h->planes.size();
-    TM_REQUIRE((long long)(plane + 1) * h->n_orient < (1LL << 31), TM_ERR_INVALID,
+    // global orientation index: (plane * n_positions + pos) * n_psi + psi over the whole grid, so
+    // handles that search different position ranges produce comparable, loop-ordered indices
+    const long long per_plane = (long long)h->prm.n_search_positions * h->n_psi;
+    TM_REQUIRE((long long)(plane + 1) * per_plane < (1LL << 31), TM_ERR_INVALID,
                "orientation index space exhausted (too many planes for one handle)");
+    const long long first = (long long)h->prm.first_search_position * h->n_psi;
     h->planes.push_back(make_int2(size_i, defocus_i));
     for (int k = 0; k < K_COUNT; ++k) { h->kernel_ms[k] = 0.0; h->kernel_launches[k] = 0; }
     h->ev_marks.clear();
@@ -691,7 +695,7 @@ int tm_run_inner_loop(tm_core* h, const float* projection_filter, int size_i, in
     long long done = 0;
     for (int o0 = 0; o0 < h->n_orient; o0 += h->B) {
       const int nb = std::min(h->B, h->n_orient - o0);
-      run_batch(h, o0, nb, plane * h->n_orient + o0);
+      run_batch(h, o0, nb, (int)(plane * per_plane + first + o0));
       done += nb;
       if (progress) {
         TM_HIP(hipStreamSynchronize(h->stream));
@@ -727,7 +731,7 @@ int tm_get_results(tm_core* h, float* mip, float* psi, float* theta, float* phi,
     std::vector<unsigned long long> hh(512);
     TM_HIP(hipMemcpyAsync(hh.data(), h->hist.p, 512 * sizeof(unsigned long long), hipMemcpyDeviceToHost, h->stream));
     TM_HIP(hipStreamSynchronize(h->stream));
-    const int per_plane = h->n_orient;
+    const int per_plane = h->prm.n_search_positions * h->n_psi;
     for (int y = 0; y < H; ++y) {
       for (int x = 0; x < RS; ++x) {
         const size_t o = (size_t)y * RS + x;
@@ -752,9 +756,9 @@ int tm_get_results(tm_core* h, float* mip, float* psi, float* theta, float* phi,
         float a_psi = 0.f, a_theta = 0.f, a_phi = 0.f, a_def = 0.f, a_pix = 0.f;
         if (g >= 0) {
           const int plane = g / per_plane;
-          const int local = g % per_plane;
-          const int pos = h->prm.first_search_position + local / h->n_psi;
-          const int k = local % h->n_psi;
+          const int rem = g % per_plane;
+          const int pos = rem / h->n_psi;
+          const int k = rem % h->n_psi;
           a_psi = h->psi[k];
           a_theta = h->grid[2 * pos + 1];
           a_phi = h->grid[2 * pos];
@@ -771,6 +775,35 @@ int tm_get_results(tm_core* h, float* mip, float* psi, float* theta, float* phi,
     if (histogram)
       for (int b = 0; b < h->prm.histogram_number_of_bins; ++b) histogram[b] = (long long)hh[b];
     if (n_ccfs) *n_ccfs = h->n_ccfs;
+  });
+}
+
+int tm_merge(tm_core* dst, tm_core* src) {
+  return guarded([&] {
+    require_ready(dst);
+    require_ready(src);
+    TM_REQUIRE(dst != src, TM_ERR_INVALID, "cannot merge a handle into itself");
+    TM_REQUIRE(dst->device == src->device, TM_ERR_INVALID, "handles on different devices");
+    TM_REQUIRE(dst->W == src->W && dst->H == src->H && dst->n_psi == src->n_psi &&
+                   dst->prm.n_search_positions == src->prm.n_search_positions &&
+                   dst->prm.histogram_number_of_bins == src->prm.histogram_number_of_bins &&
+                   dst->prm.histogram_min_scaled == src->prm.histogram_min_scaled &&
+                   dst->prm.histogram_step_scaled == src->prm.histogram_step_scaled,
+               TM_ERR_INVALID, "handles describe different searches (image, grid, psi or histogram)");
+    TM_REQUIRE(dst->planes.size() == src->planes.size(), TM_ERR_INVALID, "handles ran different plane sequences");
+    for (size_t k = 0; k < dst->planes.size(); ++k)
+      TM_REQUIRE(dst->planes[k].x == src->planes[k].x && dst->planes[k].y == src->planes[k].y, TM_ERR_INVALID,
+                 "handles ran different plane sequences");
+    TM_HIP(hipSetDevice(dst->device));
+    TM_HIP(hipStreamSynchronize(src->stream));
+    const size_t n = (size_t)dst->W * dst->H;
+    hipLaunchKernelGGL(tmk::k_merge, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, dst->stream, dst->mip.p,
+                       dst->idx.p, dst->sum.p, dst->sumsq.p, src->mip.p, src->idx.p, src->sum.p, src->sumsq.p, n);
+    check_launch("k_merge");
+    hipLaunchKernelGGL(tmk::k_add_u64, dim3(2), dim3(256), 0, dst->stream, dst->hist.p, src->hist.p, 512);
+    check_launch("k_add_u64");
+    TM_HIP(hipStreamSynchronize(dst->stream));
+    dst->n_ccfs += src->n_ccfs;
   });
 }
 

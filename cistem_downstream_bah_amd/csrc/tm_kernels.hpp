@@ -921,6 +921,31 @@ static __global__ void k_zero_u64(unsigned long long* p, int n) {
   if (i < n) p[i] = 0ull;
 }
 
+// AggregatedTemplateResult::AddResult (match_template.cpp:2015-2103) for two partial searches of
+// one image held on one device: the larger MIP wins with its orientation; with global orientation
+// indices an equal MIP keeps the lower index, the orientation the sequential loop meets first, so
+// merging position ranges reproduces one search over their union. Sums add.
+static __global__ void k_merge(float* __restrict__ mip, int* __restrict__ idx, double* __restrict__ sum,
+                               double* __restrict__ sumsq, const float* __restrict__ smip,
+                               const int* __restrict__ sidx, const double* __restrict__ ssum,
+                               const double* __restrict__ ssumsq, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float a = mip[i], b = smip[i];
+  const int ia = idx[i], ib = sidx[i];
+  if (ib >= 0 && (ia < 0 || b > a || (b == a && ib < ia))) {
+    mip[i] = b;
+    idx[i] = ib;
+  }
+  sum[i] += ssum[i];
+  sumsq[i] += ssumsq[i];
+}
+
+static __global__ void k_add_u64(unsigned long long* __restrict__ a, const unsigned long long* __restrict__ b, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] += b[i];
+}
+
 // match_template.cpp:1221-1243, 1300-1310 with the reference's float/double mix.
 static __global__ void k_finalize(const float* __restrict__ mip, const double* __restrict__ sum,
                            const double* __restrict__ sumsq, size_t n, float n_ccf, float sqrt_px,

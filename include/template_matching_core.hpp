@@ -88,6 +88,13 @@ class TemplateMatchingCore {
       for (int i = 0; i < n_bins_; ++i) histogram_to_add[i] += (long)hist[i];
   }
 
+  // Folds another core's results into this one (same search, another position range, same device),
+  // as AggregatedTemplateResult::AddResult merges job results (match_template.cpp:2015-2103).
+  void Merge(TemplateMatchingCore& other) {
+    check(tm_merge(h_, other.h_));
+    total_number_of_cccs_calculated += other.total_number_of_cccs_calculated;
+  }
+
   // Final scaling and z-score on the device (match_template.cpp:1221-1243, 1300-1310); padded
   // layout outputs, any pointer may be null.
   void Finalize(long long total_correlation_positions, float* mip_scaled, float* zscore, float* average,

@@ -93,12 +93,22 @@ int tm_run_inner_loop(tm_core* h, const float* projection_filter, int size_i, in
  * are written as 0). Any pointer may be NULL. Replaces the reads of d_max_intensity_projection,
  * d_best_psi/theta/phi, d_sum3, d_sumSq3 and Histogram::CopyToHostAndAdd
  * (match_template.cpp:922-984, Histogram.cu:188-203). best_index receives the global orientation
- * index ((plane * n_local_positions + local_position) * n_psi + psi_index), -1 where unset. */
+ * index ((plane * n_search_positions + search_position) * n_psi + psi_index, plane counting this
+ * handle's tm_run_inner_loop calls), -1 where unset. */
 int tm_get_results(tm_core* h, float* mip, float* psi, float* theta, float* phi, float* defocus, float* pixel_size,
                    double* sum, double* sum_of_squares, int* best_index, long long* histogram, long long* n_ccfs);
 
 /* Clears the accumulated maps (MIP to -FLT_MAX, sums and histogram to 0). */
 int tm_reset(tm_core* h);
+
+/* Merges the results of src into dst, as AggregatedTemplateResult::AddResult merges the results of
+ * separate jobs (match_template.cpp:2015-2103): per pixel the larger MIP wins with its orientation
+ * (equal values keep the lower global orientation index, i.e. the one the reference's sequential
+ * loop meets first), sums and histogram counts add, and dst's CCF count grows by src's. Both
+ * handles must be on one device, describe the same search (image size, grid, psi list, histogram)
+ * and have run the same sequence of planes; typically they cover different position ranges.
+ * Merging ranges that partition a search gives the single-handle result. src is unchanged. */
+int tm_merge(tm_core* dst, tm_core* src);
 
 /* Final scaling and z-score on the device (match_template.cpp:1221-1243, 1300-1310):
  * mean = sum/N, sd = sqrt(sumsq/N - mean^2) * sqrt(W*H), mean *= sqrt(W*H),
@@ -112,9 +122,10 @@ int tm_finalize(tm_core* h, long long total_ccfs, float* mip_scaled, float* zsco
 int tm_debug_projection(tm_core* h, const float* projection_filter, int search_position, int psi_index, float* out);
 int tm_debug_ccf(tm_core* h, const float* projection_filter, int search_position, int psi_index, float* out);
 
-/* Average duration (ms) of each hot kernel over the last tm_run_inner_loop call, measured with
- * HIP events on the handle's stream when timing is enabled. names/ms hold up to cap entries;
- * returns the number of entries written. */
+/* Per hot-kernel stage over the last tm_run_inner_loop call, measured with HIP events on the
+ * handle's stream when timing is enabled: total duration in ms (ms) and number of launches
+ * (launches); the average per launch is ms / launches. names/ms/launches hold up to cap entries;
+ * returns the number of entries written. No counterpart in the reference (diagnostics). */
 int tm_set_timing(tm_core* h, int enabled);
 int tm_get_timing(tm_core* h, const char** names, double* ms, long long* launches, int cap);
 

@@ -420,12 +420,13 @@ struct MatchTemplate {
   int NumPsi() const { return (int)psi.size(); }
   int NumPos() const { return (int)grid.size() / 2; }
 
-  // Global orientation index: g = (plane * n_local_pos + (pos - first_pos)) * n_psi + psi_idx,
-  // plane = (size_i + n_size_half) * (2 n_def_half + 1) + (defocus_i + n_def_half).
+  // Global orientation index over the whole search (every grid position, not only this run's
+  // range), in the reference's loop order: g = (plane * n_pos + pos) * n_psi + psi_idx with
+  // plane = (size_i + n_size_half) * (2 n_def_half + 1) + (defocus_i + n_def_half). Runs over
+  // different position ranges share it, so their maps merge with the earliest-orientation tie-break.
   int32_t GlobalIndex(int size_i, int defocus_i, int pos, int psi_idx) const {
     long plane = (long)(size_i + n_size_half) * (2 * n_def_half + 1) + (defocus_i + n_def_half);
-    long nloc = last_pos - first_pos + 1;
-    return (int32_t)((plane * nloc + (pos - first_pos)) * (long)psi.size() + psi_idx);
+    return (int32_t)((plane * NumPos() + pos) * (long)psi.size() + psi_idx);
   }
 
   // The hot loop over [first_pos, last_pos] x psi for all defocus planes (match_template.cpp:890-1169).

@@ -20,10 +20,10 @@ def rel(a, b):
 
 def decode_index(info, gi):
     """Global orientation index -> (defocus_i, pos, psi_idx) for pixel-size index 0, the inverse of
-    ((plane * n_local_pos) + pos - first_pos) * n_psi + psi_idx with plane = defocus_i + n_def_half."""
-    nloc = info.last_pos - info.first_pos + 1
-    plane, local = divmod(int(gi), nloc * info.n_psi)
-    return plane - info.n_def_half, info.first_pos + local // info.n_psi, local % info.n_psi
+    (plane * n_pos + pos) * n_psi + psi_idx with plane = defocus_i + n_def_half (pos over the whole
+    grid)."""
+    plane, rem = divmod(int(gi), info.n_pos * info.n_psi)
+    return plane - info.n_def_half, rem // info.n_psi, rem % info.n_psi
 
 
 def near_tie_check(mt, ref, got, max_orientations=400):

@@ -76,7 +76,7 @@ def test_hot_loop_equals_brute_force(c0):
             _, c = mt.projection_and_ccf(pos, k, 0)
             better = c > mip                      # strict: the earliest orientation keeps ties
             mip[better] = c[better]
-            idx[better] = g
+            idx[better] = pos * I.n_psi + k       # global index (one plane, every grid position)
             s1 += c
             s2 += (c * c).astype(np.float32)
             d = (c.astype(np.float64) - I.histogram_min_scaled) / I.histogram_step_scaled
