@@ -87,7 +87,7 @@ def main():
     ap.add_argument("--steps", type=int, default=11)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--positions", type=int, default=8, help="search positions per step (x 241 psi)")
-    ap.add_argument("--batch", type=int, default=32, help="orientations per launch")
+    ap.add_argument("--batch", type=int, default=64, help="orientations per launch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-psi", type=int, default=8)

@@ -408,31 +408,50 @@ struct Timer {
 
 // ------------------------------------------------------------------ batch pipeline
 // Normalised real projections for orientations [o0, o0 + nb) into projT (T x (T+2) floats each).
+template <int N>
+void launch_extract_cols(tm_core* h, int o0, int nb, const cf* phase, const cf* filt, cf* out) {
+  constexpr int KC = 8;
+  const dim3 g((N / 2 + 1 + KC - 1) / KC, nb);
+  hipLaunchKernelGGL((tmk::k_extract_cols<N, KC>), g, dim3(KC * 64), 0, h->stream, h->tpl.p, h->mats.p, o0, phase,
+                     filt, h->twiddle(N), out);
+  check_launch("k_extract_cols");
+}
+
+// Slice extraction and the inverse column FFT of the slice (P != T) or of the filtered projection
+// (P == T), fused.
+void extract_cols(tm_core* h, int N, int o0, int nb, const cf* phase, const cf* filt, cf* out) {
+  switch (N) {
+#define TM_CASE(n) \
+  case n: launch_extract_cols<n>(h, o0, nb, phase, filt, out); break;
+    TM_TEMPLATE_SIZES(TM_CASE)
+#undef TM_CASE
+    default: throw TmError{TM_ERR_UNSUPPORTED, "unsupported slice size " + std::to_string(N)};
+  }
+}
+
 void run_projections(tm_core* h, int o0, int nb) {
   const int P = h->P, Ph = h->Ph, T = h->T, Th = h->Th;
   const size_t slice_bs = (size_t)P * Ph;     // complex
   const size_t proj_bs = (size_t)T * Th;      // complex (= T * (T+2) floats)
   {
+    // ExtractSlice (x the swap phase, and when P == T the projection filter), then the column half
+    // of the C2R (BackwardFFT)
     Timer t(h, K_EXTRACT);
-    dim3 g((P * Ph + 255) / 256, nb);
-    const bool fold = (P == T);
-    hipLaunchKernelGGL(tmk::k_extract, g, dim3(256), 0, h->stream, h->tpl.p, P, h->mats.p, o0,
-                       fold ? h->phaseT.p : h->phaseP.p, fold ? h->filt.p : nullptr, fold ? h->projT.p : h->slice.p);
-    check_launch("k_extract");
+    if (P != T) extract_cols(h, P, o0, nb, h->phaseP.p, nullptr, h->slice.p);
+    else extract_cols(h, T, o0, nb, h->phaseT.p, h->filt.p, h->projT.p);
   }
   {
     Timer t(h, K_SMALLFFT);
     if (P != T) {
-      // ExtractSlice at P, SwapRealSpaceQuadrants (folded above), BackwardFFT (C2R P^2) in place
-      cols(h, P, h->slice.p, Ph, slice_bs, nb, +1, 0, nullptr, 1.f);
-      rows_c2r(h, P, h->slice.p, slice_bs, Ph, reinterpret_cast<float*>(h->slice.p), 2 * slice_bs, P + 2, P, nb);
+      // row half of the C2R, only for the T rows that ClipInto keeps (centre rows P/2 - T/2 ..)
+      const int r0 = P / 2 - T / 2;
+      rows_c2r(h, P, h->slice.p + (size_t)r0 * Ph, slice_bs, Ph,
+               reinterpret_cast<float*>(h->slice.p) + (size_t)r0 * (P + 2), 2 * slice_bs, P + 2, T, nb);
       // ClipInto T^2 (centre) + ForwardFFT rows; columns fused with the 1/T^2 scale, filter and inverse
-      rows_r2c(h, T, reinterpret_cast<const float*>(h->slice.p), 2 * slice_bs, P + 2, P / 2 - T / 2, P / 2 - T / 2, T,
-               0, h->projT.p, proj_bs, Th, T, nb, 1.f);
+      rows_r2c(h, T, reinterpret_cast<const float*>(h->slice.p), 2 * slice_bs, P + 2, r0, r0, T, 0, h->projT.p,
+               proj_bs, Th, T, nb, 1.f);
       const float scale = (float)(1. / (double)(float)((long long)T * T));
       cols(h, T, h->projT.p, Th, proj_bs, nb, -1, 1, h->filt.p, scale);
-    } else {
-      cols(h, T, h->projT.p, Th, proj_bs, nb, +1, 0, nullptr, 1.f);
     }
     rows_c2r(h, T, h->projT.p, proj_bs, Th, reinterpret_cast<float*>(h->projT.p), 2 * proj_bs, T + 2, T, nb);
   }
@@ -572,7 +591,7 @@ int tm_init(tm_core* h, const tm_init_params* p, const float* template_spectrum,
     h->W = W; h->H = H; h->Wh = W / 2 + 1;
     h->T = T; h->Th = T / 2 + 1;
     h->P = P; h->Ph = P / 2 + 1;
-    h->B = p->batch_size > 0 ? p->batch_size : 32;
+    h->B = p->batch_size > 0 ? p->batch_size : 64;
     TM_REQUIRE(p->plane_storage == TM_PLANE_FP32 || p->plane_storage == TM_PLANE_FP16 ||
                    p->plane_storage == TM_PLANE_BF16, TM_ERR_INVALID, "plane_storage must be 0, 1 or 2");
     TM_REQUIRE(p->plane_storage == TM_PLANE_FP32 || (H == 4096 && pow2(W) && W >= 1024), TM_ERR_UNSUPPORTED,

@@ -1,9 +1,10 @@
 // Hot-path kernels of the template-matching inner loop (gfx950).
 //
 // Per batch of B orientations (one tm_run_inner_loop call runs ceil(n_orient / B) batches):
-//   k_extract      Fourier slice of the P^3 template at R(phi,theta,psi), trilinear
+//   k_extract_cols Fourier slice of the P^3 template at R(phi,theta,psi), trilinear
 //                  (src/core/image.cpp:2450-2607, 2899-3020), times the quadrant-swap phase and,
-//                  when P == T, the projection filter (match_template.cpp:1007-1023)
+//                  when P == T, the projection filter (match_template.cpp:1007-1023), then the
+//                  inverse column FFT of the slice, in LDS
 //   k_cols / k_rows_c2r / k_rows_r2c
 //                  the small 2-D transforms of the projection (C2R P^2, clip T^2, R2C T^2,
 //                  x filter, C2R T^2; match_template.cpp:1009-1030)
@@ -109,19 +110,15 @@ __device__ __forceinline__ cf interp_fourier(const cf* __restrict__ tpl, int P, 
   return sum;
 }
 
-// One thread per output Fourier pixel (i, j) of a P x (P/2+1) slice; blockIdx.y = orientation.
-// Special cases of Image::ExtractSlice: the i = 0 column takes Hermitian mates for negative y
-// except the even-size edge row, and the DC term is 0 (image.cpp:2556-2589).
-static __global__ void k_extract(const cf* __restrict__ tpl, int P, const float* __restrict__ mats, int o0,
-                          const cf* __restrict__ phase, const cf* __restrict__ filt, cf* __restrict__ out) {
+// Slice pixel (i, j) of a P x (P/2+1) slice at orientation matrix m, times the quadrant-swap phase
+// and (when filt is not null) the projection filter. Special cases of Image::ExtractSlice: the
+// i = 0 column takes Hermitian mates for negative y except the even-size edge row, and the DC term
+// is 0 (image.cpp:2556-2589).
+__device__ __forceinline__ cf slice_value(const cf* __restrict__ tpl, int P, const float* __restrict__ m, int i,
+                                          int j, const cf* __restrict__ phase, const cf* __restrict__ filt) {
   const int Ph = P / 2 + 1;
-  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
-  const int b = blockIdx.y;
-  if (pix >= P * Ph) return;
-  const int i = pix % Ph;
-  const int j = pix / Ph;
+  const int pix = j * Ph + i;
   const int ly = j >= P / 2 ? j - P : j;
-  const float* m = mats + (size_t)(o0 + b) * 9;
   cf val = cmake(0.f, 0.f);
   if (!(i == 0 && ly == 0)) {
     int sy = ly;
@@ -136,7 +133,33 @@ static __global__ void k_extract(const cf* __restrict__ tpl, int P, const float*
   }
   val = cmul(val, phase[pix]);
   if (filt) val = cmul(val, filt[pix]);
-  out[(size_t)b * P * Ph + pix] = val;
+  return val;
+}
+
+// Slice extraction fused with the inverse column FFT that starts the slice's C2R (image.cpp:5104-5116):
+// a block builds KC adjacent columns of one orientation's slice in LDS and transforms them there,
+// so the slice is written once, after the transform, in runs of KC columns. Same per-pixel
+// arithmetic and FFT plan (64 threads per column) as extracting the slice and then k_cols<P, 64, +1, 0>.
+template <int P, int KC>
+__global__ __launch_bounds__(KC * 64) void k_extract_cols(const cf* __restrict__ tpl, const float* __restrict__ mats,
+                                                         int o0, const cf* __restrict__ phase,
+                                                         const cf* __restrict__ filt, const cf* __restrict__ tw,
+                                                         cf* __restrict__ out) {
+  constexpr int Ph = P / 2 + 1, NTP = 64;
+  __shared__ cf buf[KC][lds_len(P)];
+  const int b = blockIdx.y, i0 = blockIdx.x * KC, tid = threadIdx.x;
+  const float* m = mats + (size_t)(o0 + b) * 9;
+  for (int e = tid; e < KC * P; e += KC * NTP) {
+    const int c = e % KC, j = e / KC, i = i0 + c;
+    buf[c][lpad(j)] = i < Ph ? slice_value(tpl, P, m, i, j, phase, filt) : cmake(0.f, 0.f);
+  }
+  __syncthreads();
+  fft_inplace<P, +1, NTP>(buf[tid / NTP], tw, tid % NTP);
+  cf* o = out + (size_t)b * P * Ph;
+  for (int e = tid; e < KC * P; e += KC * NTP) {
+    const int c = e % KC, j = e / KC, i = i0 + c;
+    if (i < Ph) o[(size_t)j * Ph + i] = buf[c][lpad(j)];
+  }
 }
 
 // ---------------------------------------------------------------- small line transforms

@@ -46,7 +46,7 @@ typedef struct tm_init_params {
   int max_padding;               /* kept for interface parity; the CPU path it mirrors does not trim */
   float defocus_step;            /* best_defocus map stores defocus_i * defocus_step                 */
   float pixel_size_step;         /* best_pixel_size map stores size_i * pixel_size_step              */
-  int batch_size;                /* orientations per launch; 0 selects the default (32)              */
+  int batch_size;                /* orientations per launch; 0 selects the default (64)              */
   int plane_storage;             /* storage of the plane between the column and row transforms:
                                     TM_PLANE_FP32 (0, the reference's precision), TM_PLANE_FP16 or
                                     TM_PLANE_BF16 (half the traffic; image height 4096 and a

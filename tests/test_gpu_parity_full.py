@@ -7,7 +7,7 @@ orientations, against the CPU oracle.
 * a search over one grid position (2.5 deg grid, 5 deg in-plane: 73 orientations) and three
   defocus planes (-200, 0, +200 A), run plane by plane in the reference's loop order: MIP, argmax
   (near-tie rule), angle and defocus maps, FP64 sums and histogram to the bars of parity_util.
-  73 orientations per plane leave a partial last batch (73 = 2 * 32 + 9).
+  73 orientations per plane leave a partial last batch (73 = 64 + 9).
 The oracle costs about 1.2 s per 4096^2 CCF on one core, so the search is sized to one position.
 """
 import os
