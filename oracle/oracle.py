@@ -64,8 +64,35 @@ def lib():
         L.tmo_mt_results.restype = ctypes.c_long
         L.tmo_mt_finalise.argtypes = [vp, ctypes.c_int, ctypes.c_long, F32P, F32P, F32P, F32P]
         L.tmo_mt_finalise.restype = ctypes.c_float
+        L.tmo_rt_create.argtypes = [F32P, F32P, ctypes.c_int, ctypes.c_int, F32P, ctypes.c_int, ctypes.c_int]
+        L.tmo_rt_create.restype = vp
+        L.tmo_rt_destroy.argtypes = [vp, ctypes.c_int]
+        L.tmo_rt_info.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_long)]
+        L.tmo_rt_get.argtypes = [vp, ctypes.c_int, ctypes.c_int, F32P]
+        L.tmo_rt_window.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.tmo_rt_window_from_real.argtypes = [vp, ctypes.c_int, F32P]
+        L.tmo_rt_score.argtypes = [vp, ctypes.c_int] + [ctypes.c_float] * 5 + [F32P, F32P]
+        L.tmo_rt_peak_list.argtypes = [vp, ctypes.c_int, F32P, F32P, ctypes.c_long]
+        L.tmo_rt_peak_list.restype = ctypes.c_long
+        L.tmo_rt_run.argtypes = [vp, ctypes.c_int, ctypes.POINTER(F32P), ctypes.c_int, ctypes.c_float, ctypes.c_int,
+                                 ctypes.POINTER(F32P), ctypes.POINTER(RefinedPeak), ctypes.c_long]
+        L.tmo_rt_run.restype = ctypes.c_long
         _lib = L
     return _lib
+
+
+class RefinedPeak(ctypes.Structure):
+    """One refined peak: the found peak and the search's values there, then the refined values
+    (refine_template.cpp:1091-1113). Positions are pixels relative to the image centre."""
+    _fields_ = [(n, ctypes.c_float) for n in
+                ("x", "y", "value", "psi", "theta", "phi", "defocus", "pixel_size", "x_pos", "y_pos", "shift_x",
+                 "shift_y", "r_psi", "r_theta", "r_phi", "r_defocus", "r_pixel_size", "peak_height", "mip",
+                 "best_score", "starting_score", "offset")] + \
+              [("best_address", ctypes.c_longlong), ("evaluations", ctypes.c_int), ("pad_", ctypes.c_int)]
+
+
+def peaks_to_dicts(arr) -> list:
+    return [{n: getattr(p, n) for n, _ in RefinedPeak._fields_ if n != "pad_"} for p in arr]
 
 
 def _fp(a):
@@ -341,3 +368,129 @@ class MatchTemplateOracle:
         out = {n: a.reshape(I.H, I.W + 2)[:, : I.W].copy() for n, a in zip(names, arrs)}
         out["expected_threshold"] = float(thr)
         return out
+
+
+# ---------------------------------------------------------------- refine_template
+@dataclass
+class RTParams:
+    """refine_template arguments (refine_template.cpp:260-300) used by the restatement."""
+    pixel_size: float = 1.0
+    voltage_kV: float = 300.0
+    spherical_aberration_mm: float = 2.7
+    amplitude_contrast: float = 0.07
+    defocus1: float = 15000.0
+    defocus2: float = 15000.0
+    defocus_angle: float = 0.0
+    phase_shift_deg: float = 0.0
+    angular_step: float = 1.5
+    in_plane_angular_step: float = 1.5
+    defocus_search_range: float = 0.0
+    defocus_search_step: float = 0.0
+    padding: float = 1.0
+    wanted_threshold: float = 7.5
+    min_peak_radius: float = 10.0
+
+    def packed(self):
+        return np.array([self.pixel_size, self.voltage_kV, self.spherical_aberration_mm, self.amplitude_contrast,
+                         self.defocus1, self.defocus2, self.defocus_angle, self.phase_shift_deg, self.angular_step,
+                         self.in_plane_angular_step, self.defocus_search_range, self.defocus_search_step,
+                         self.padding, self.wanted_threshold, self.min_peak_radius], np.float32)
+
+
+MAP_NAMES = ("mip", "scaled_mip", "psi", "theta", "phi", "defocus", "pixel_size")
+
+
+class RefineTemplateOracle:
+    """The reference refine_template program for one image, on the CPU."""
+
+    def __init__(self, params: RTParams, micrograph: np.ndarray, volume: np.ndarray, fp64: bool = False):
+        L = lib()
+        m = np.ascontiguousarray(micrograph, np.float32)
+        v = np.ascontiguousarray(volume, np.float32)
+        self.fp64 = int(fp64)
+        self.h = L.tmo_rt_create(_fp(params.packed()), _fp(m), m.shape[1], m.shape[0], _fp(v), v.shape[0], self.fp64)
+        if not self.h:
+            raise RuntimeError("oracle: " + L.tmo_last_error().decode())
+        info = np.zeros(8, np.int64)
+        L.tmo_rt_info(self.h, self.fp64, info.ctypes.data_as(ctypes.POINTER(ctypes.c_long)))
+        self.mx, self.my, self.t, self.P, self.n_curve = [int(x) for x in info[:5]]
+        self.params = params
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().tmo_rt_destroy(self.h, self.fp64)
+            self.h = None
+
+    def _get(self, which, n):
+        out = np.zeros(n, np.float32)
+        lib().tmo_rt_get(self.h, self.fp64, which, _fp(out))
+        return out
+
+    def _pad(self):
+        return self.mx + (2 if self.mx % 2 == 0 else 1)
+
+    @property
+    def whitened_image(self) -> np.ndarray:
+        """(my, mx) whitened, normalised input image in real space (refine_template.cpp:498-510)."""
+        return self._get(0, self._pad() * self.my).reshape(self.my, self._pad())[:, : self.mx].copy()
+
+    @property
+    def template_spectrum(self) -> np.ndarray:
+        P = self.P
+        return self._get(1, P * P * (P + 2)).view(np.complex64).reshape(P, P, P // 2 + 1)
+
+    @property
+    def whitening_curve(self):
+        n = self.n_curve
+        return self._get(3, n), self._get(2, n)
+
+    @property
+    def window_spectrum(self) -> np.ndarray:
+        t = self.t
+        return self._get(4, t * (t + 2)).view(np.complex64).reshape(t, t // 2 + 1)
+
+    @property
+    def projection_filter(self) -> np.ndarray:
+        t = self.t
+        return self._get(5, t * (t + 2)).view(np.complex64).reshape(t, t // 2 + 1)
+
+    def window(self, peak_x: int, peak_y: int) -> None:
+        _check(lib().tmo_rt_window(self.h, self.fp64, int(peak_x), int(peak_y)))
+
+    def window_from_real(self, img: np.ndarray) -> None:
+        p = to_padded(np.asarray(img, np.float32))
+        _check(lib().tmo_rt_window_from_real(self.h, self.fp64, _fp(p)))
+
+    def score(self, phi, theta, psi, current_defocus=0.0, step=0.0, want_projection=False):
+        """TemplateScore: (x, y, value) of the correlation peak; with want_projection also the
+        real-space normalised filtered projection (t, t)."""
+        out = np.zeros(3, np.float32)
+        t = self.t
+        proj = np.zeros((t, t + 2), np.float32) if want_projection else None
+        _check(lib().tmo_rt_score(self.h, self.fp64, current_defocus, step, phi, theta, psi, _fp(out),
+                                  _fp(proj) if want_projection else None))
+        res = (float(out[0]), float(out[1]), float(out[2]))
+        return (res, proj[:, :t].copy()) if want_projection else res
+
+    def peak_list(self, scaled_mip: np.ndarray, cap: int = 100000) -> np.ndarray:
+        sm = to_padded(np.asarray(scaled_mip, np.float32))
+        out = np.zeros((cap, 3), np.float32)
+        n = lib().tmo_rt_peak_list(self.h, self.fp64, _fp(sm), _fp(out), cap)
+        if n < 0:
+            raise RuntimeError("oracle: " + lib().tmo_last_error().decode())
+        return out[: min(n, cap)].copy()
+
+    def run(self, maps: dict, shared_local_maps: bool = False, xy_change_threshold: float = 10.0,
+            exclude_above_xy_threshold: bool = False, cap: int = 10000):
+        """Refines every peak. maps: dense (my, mx) arrays named as MAP_NAMES. Returns
+        (list of peak dicts, dict of output maps best_*)."""
+        ins = [to_padded(np.asarray(maps[n], np.float32)) for n in MAP_NAMES]
+        outs = [np.zeros_like(a) for a in ins]
+        in_ptrs = (F32P * 7)(*[_fp(a) for a in ins])
+        out_ptrs = (F32P * 7)(*[_fp(a) for a in outs])
+        peaks = (RefinedPeak * cap)()
+        n = lib().tmo_rt_run(self.h, self.fp64, in_ptrs, int(shared_local_maps), xy_change_threshold,
+                             int(exclude_above_xy_threshold), out_ptrs, peaks, cap)
+        if n < 0:
+            raise RuntimeError("oracle: " + lib().tmo_last_error().decode())
+        return peaks_to_dicts(peaks[: min(n, cap)]), {n: a[:, : self.mx].copy() for n, a in zip(MAP_NAMES, outs)}

@@ -530,6 +530,426 @@ void unpack_params(MTParams& p, const float* f, const int* i) {
 }
 }  // namespace
 
+// =============================================================================================
+// refine_template (src/programs/refine_template/refine_template.cpp) for one image: the peak list
+// from the scaled MIP (:561-600), the per-peak particle window (:738-744), TemplateScore (:25-80)
+// and the coordinate descent over defocus and angles (:748-1000, 1016-1072), with the result maps
+// written and read at best_address as the program does. Not restated: pixel-size refinement
+// (pixel_size_search_range must be 0, so TemplateScore's ChangePixelSize has factor 1, which is
+// ClipInto + ForwardFFT, image.cpp:8705-8722) and the particle mask (mask_radius must be 0, the
+// program's default).
+struct RefineParams {
+  float pixel_size, voltage_kV, spherical_aberration_mm, amplitude_contrast, defocus1, defocus2, defocus_angle,
+      phase_shift_deg, angular_step, in_plane_angular_step, defocus_search_range, defocus_search_step, padding,
+      wanted_threshold, min_peak_radius;
+};
+
+struct PeakI {
+  float x = 0.f, y = 0.f, value = -FLT_MAX;  // x, y relative to the box centre
+  long address = 0;
+};
+
+// Image::FindPeakWithIntegerCoordinates with absolute radii (image.cpp:9961-10105): strict greater
+// in scan order over the pixels at least `edge` from the borders (upper bound inclusive).
+template <class R>
+PeakI FindPeakWithIntegerCoordinates(const Image<R>& im, int edge) {
+  PeakI pk;
+  const int i_min = edge, i_max = im.nx - edge, j_min = edge, j_max = im.ny - edge;
+  long counter = 0;
+  for (int j = 0; j < im.ny; ++j) {
+    for (int i = 0; i < im.nx; ++i) {
+      if (i >= i_min && i <= i_max && j >= j_min && j <= j_max) {
+        if (im.v[counter] > pk.value) {
+          pk.value = (float)im.v[counter];
+          pk.x = float(i - im.box_cx());
+          pk.y = float(j - im.box_cy());
+          pk.address = counter;
+        }
+      }
+      ++counter;
+    }
+    counter += im.padding_jump_value;
+  }
+  return pk;
+}
+
+// Image::RealSpaceIntegerShift (image.cpp:9064-9107) of a 2-D image: a circular shift of the
+// linear pixel index by -(sx + nx sy), so x shifts wrap into the neighbouring row.
+template <class R>
+void RealSpaceIntegerShift(Image<R>& im, int sx, int sy) {
+  const long n = (long)im.nx * im.ny;
+  std::vector<R> buffer(n);
+  long shifted = -sx - (long)im.nx * sy;
+  shifted = (long)remainderf(float(shifted), float(n));
+  if (shifted < 0) shifted += n;
+  long counter = 0;
+  for (int j = 0; j < im.ny; ++j) {
+    for (int i = 0; i < im.nx; ++i) {
+      buffer[shifted] = im.v[counter];
+      ++counter;
+      if (++shifted >= n) shifted -= n;
+    }
+    counter += im.padding_jump_value;
+  }
+  counter = 0;
+  shifted = 0;
+  for (int j = 0; j < im.ny; ++j) {
+    for (int i = 0; i < im.nx; ++i) im.v[counter++] = buffer[shifted++];
+    counter += im.padding_jump_value;
+  }
+}
+
+struct RefinedPeak {
+  // found peak (x, y relative to the image centre, value from the scaled MIP) and the search's values there
+  float x, y, value, psi, theta, phi, defocus, pixel_size;
+  // refined: position (relative to the centre), shift of the peak, angles, defocus, pixel size,
+  // scaled peak height (best_scaled_mip), best_mip, the best TemplateScore value, the starting
+  // score and the distance the peak moved (pixels)
+  float x_pos, y_pos, shift_x, shift_y, r_psi, r_theta, r_phi, r_defocus, r_pixel_size, peak_height, mip, best_score,
+      starting_score, offset;
+  long long best_address;  // padded-layout address of the refined peak in the maps
+  int evaluations;         // TemplateScore calls for this peak
+  int pad_;
+};
+
+template <class R>
+struct RefineOracle {
+  typedef R Real;
+  RefineParams p;
+  int mx = 0, my = 0, t = 0, P = 0;
+  Image<R> input_image;           // whitened, real space (refine_template.cpp:498-510)
+  Image<R> input_reconstruction;  // padded template spectrum, DC 0, quadrant-swapped (:482-488)
+  Curve<R> whitening_filter, number_of_terms;
+  CTFModel<R> ctf;
+  Image<R> windowed_particle, projection_filter;
+  int evaluations = 0;
+
+  void Setup(const RefineParams& prm, const float* micrograph, int x, int y, const float* volume, int tt) {
+    p = prm;
+    mx = x; my = y; t = tt;
+    if (p.defocus_search_step < 0.f) throw std::runtime_error("oracle refine: negative defocus step");
+    input_reconstruction.Allocate(t, t, t, true);
+    for (long z = 0; z < t; ++z)
+      for (long yy = 0; yy < t; ++yy)
+        for (long xx = 0; xx < t; ++xx)
+          input_reconstruction.v[input_reconstruction.real_addr(xx, yy, z)] = volume[(z * t + yy) * t + xx];
+    if (p.padding != 1.0f) {
+      const int np = int(input_reconstruction.nx * p.padding);
+      input_reconstruction.Resize(np, np, np, input_reconstruction.ReturnAverageOfRealValuesOnEdges());
+    }
+    P = input_reconstruction.nx;
+    input_reconstruction.ForwardFFT();
+    input_reconstruction.ZeroCentralPixel();
+    input_reconstruction.SwapRealSpaceQuadrants();
+
+    input_image.Allocate(mx, my, 1, true);
+    for (int yy = 0; yy < my; ++yy)
+      for (int xx = 0; xx < mx; ++xx) input_image.v[input_image.real_addr(xx, yy, 0)] = micrograph[(long)yy * mx + xx];
+    const int npts = int((input_image.nx / 2.0 + 1.0) * sqrtf(2.0) + 1.0);
+    whitening_filter.SetupXAxis(0.0, 0.5 * sqrtf(2.0), npts);
+    number_of_terms.SetupXAxis(0.0, 0.5 * sqrtf(2.0), npts);
+    input_image.ReplaceOutliersWithMean(5.0f);
+    input_image.ForwardFFT();
+    input_image.ZeroCentralPixel();
+    input_image.Compute1DPowerSpectrumCurve(&whitening_filter, &number_of_terms);
+    whitening_filter.SquareRoot();
+    whitening_filter.Reciprocal();
+    whitening_filter.MultiplyByConstant(1.0f / whitening_filter.ReturnMaximumValue());
+    input_image.ApplyCurveFilter(whitening_filter);
+    input_image.ZeroCentralPixel();
+    input_image.DivideByConstant((float)sqrt((double)input_image.ReturnSumOfSquares()));
+    input_image.BackwardFFT();
+
+    ctf.Init(p.voltage_kV, p.spherical_aberration_mm, p.amplitude_contrast, p.defocus1, p.defocus2, p.defocus_angle,
+             p.pixel_size, deg_2_rad(p.phase_shift_deg));
+    windowed_particle.Allocate(t, t, 1, true);
+    projection_filter.Allocate(t, t, 1, false);
+  }
+
+  // :738-744 (mask_radius 0): the whitened image shifted so the peak sits at the centre, clipped to
+  // the template box, transformed and quadrant-swapped
+  void Window(int peak_x, int peak_y) {
+    Image<R> padded_reference = input_image;
+    RealSpaceIntegerShift(padded_reference, peak_x, peak_y);
+    windowed_particle.Allocate(t, t, 1, true);
+    padded_reference.ClipInto(&windowed_particle);
+    windowed_particle.ForwardFFT();
+    windowed_particle.SwapRealSpaceQuadrants();
+  }
+
+  // CTF x whitening at defocus1|2 + current + step, summed in the program's order (:757-759, :866)
+  void Filter(float current_defocus, float step) {
+    ctf.SetDefocus((p.defocus1 + current_defocus + step) / p.pixel_size, (p.defocus2 + current_defocus + step) / p.pixel_size,
+                   deg_2_rad(p.defocus_angle));
+    projection_filter.Allocate(t, t, 1, false);
+    projection_filter.CalculateCTFImage(ctf);
+    projection_filter.ApplyCurveFilter(whitening_filter);
+  }
+
+  // TemplateScore (:25-56) up to the normalised filtered projection spectrum, pixel_size_factor 1
+  void Projection(float phi, float theta, float psi, Image<R>& cp) const {
+    float m[3][3];
+    EulerMatrix(phi, theta, psi, m);
+    cp.Allocate(t, t, 1, false);
+    if (P != t) {
+      Image<R> pp;
+      pp.Allocate(P, P, 1, false);
+      input_reconstruction.ExtractSlice(pp, m);
+      pp.SwapRealSpaceQuadrants();
+      pp.BackwardFFT();
+      pp.ClipInto(&cp);
+      cp.ForwardFFT();
+    } else {
+      input_reconstruction.ExtractSlice(cp, m);
+      cp.SwapRealSpaceQuadrants();
+      cp.BackwardFFT();
+      cp.ForwardFFT();
+    }
+    cp.MultiplyPixelWise(projection_filter);
+    cp.ZeroCentralPixel();
+    cp.DivideByConstant(sqrtf(cp.ReturnSumOfSquares()));
+  }
+
+  // TemplateScore (:25-80): correlation with the window and its integer peak
+  PeakI Score(float phi, float theta, float psi) {
+    ++evaluations;
+    Image<R> cp;
+    Projection(phi, theta, psi, cp);
+    auto* a = cp.c();
+    const auto* w = windowed_particle.c();
+    for (long i = 0; i < cp.real_memory_allocated / 2; ++i) a[i] = std::conj(a[i]) * w[i];
+    cp.BackwardFFT();
+    return FindPeakWithIntegerCoordinates(cp, 0);
+  }
+
+  // :561-600: peaks of the scaled MIP above the threshold, each masking the pixels whose squared
+  // distance is <= min_peak_radius (the program compares the squared distance with the radius)
+  std::vector<PeakI> PeakList(Image<R> scaled) const {
+    std::vector<PeakI> out;
+    while (true) {
+      PeakI pk = FindPeakWithIntegerCoordinates(scaled, 50);
+      if (pk.value < p.wanted_threshold) break;
+      out.push_back(pk);
+      const float cx = pk.x + scaled.box_cx(), cy = pk.y + scaled.box_cy();
+      long address = 0;
+      for (int j = 0; j < scaled.ny; ++j) {
+        const float sy = float(pow(j - cy, 2));
+        for (int i = 0; i < scaled.nx; ++i) {
+          const float sx = float(pow(i - cx, 2));
+          if (sx + sy <= p.min_peak_radius) scaled.v[address] = -FLT_MAX;
+          ++address;
+        }
+        address += scaled.padding_jump_value;
+      }
+    }
+    return out;
+  }
+
+  // The search's result maps for this image (padded layout) and the program's output maps.
+  struct Maps {
+    Image<R> mip, scaled_mip, psi, theta, phi, defocus, pixel_size;
+  };
+
+  // Refines every peak of PeakList. shared_local_maps: the per-thread copies best_*_local of the
+  // program (:682-689) carry over from one peak to the next, as in a run with one thread; otherwise
+  // every peak starts from the input maps, as a thread that handles a single peak does. out (may be
+  // null) receives best_mip, best_scaled_mip, best_psi, ... (:641-648, 1093-1113).
+  std::vector<RefinedPeak> Run(const Maps& in, bool shared_local_maps, float xy_change_threshold,
+                               bool exclude_above_xy_threshold, Maps* out) {
+    for (const Image<R>* m : {&in.mip, &in.psi, &in.theta, &in.phi, &in.defocus, &in.pixel_size})
+      if (m->nx != in.scaled_mip.nx || m->ny != in.scaled_mip.ny) throw std::runtime_error("oracle refine: map sizes differ");
+    for (long i = 0; i < in.pixel_size.real_memory_allocated; ++i)
+      if (in.pixel_size.v[i] != 0) throw std::runtime_error("oracle refine: non-zero pixel-size map (ChangePixelSize != 1) is not restated");
+    std::vector<PeakI> peaks = PeakList(in.scaled_mip);
+    float defocus_search_range = p.defocus_search_range;
+    float defocus_refine_step = 2.0f * p.defocus_search_step;
+    if (defocus_refine_step <= 0.0) { defocus_search_range = 0.0f; defocus_refine_step = 100.0f; }
+    const float defocus_step = std::max(defocus_refine_step, 100.0f);
+    const float pixel_size_refine_step = 100.0f;  // pixel_size_search_range 0 (:619-623)
+    const float sqrt_t = sqrtf(float(t) * float(t));
+    const long rs = (long)in.scaled_mip.nx + in.scaled_mip.padding_jump_value;
+    if (out) {
+      *out = in;
+      out->mip.SetToConstant(0.0f);
+      out->scaled_mip.SetToConstant(0.0f);
+    }
+    std::vector<RefinedPeak> result;
+    Image<R> best_psi_local, best_theta_local, best_phi_local, best_defocus_local, best_pixel_size_local;
+    auto copy_maps = [&] {
+      best_psi_local = in.psi; best_theta_local = in.theta; best_phi_local = in.phi;
+      best_defocus_local = in.defocus; best_pixel_size_local = in.pixel_size;
+    };
+    copy_maps();
+    for (size_t peak_number = 0; peak_number < peaks.size(); ++peak_number) {
+      const PeakI& found = peaks[peak_number];
+      if (!shared_local_maps && peak_number > 0) copy_maps();
+      Window((int)found.x, (int)found.y);
+      evaluations = 0;
+      const float px = found.x + in.scaled_mip.box_cx(), py = found.y + in.scaled_mip.box_cy();
+      const long current_address = (long)myroundint(py) * rs + myroundint(px);
+      float current_phi = (float)best_phi_local.v[current_address];
+      float current_theta = (float)best_theta_local.v[current_address];
+      float current_psi = (float)best_psi_local.v[current_address];
+      float current_defocus = (float)best_defocus_local.v[current_address];
+      float current_pixel_size = (float)best_pixel_size_local.v[current_address];
+      float best_score = -FLT_MAX, score;
+      long best_address = current_address;
+      PeakI best_peak, template_peak;
+      float offset_distance = 0.f;
+      const float score_adjustment = 1.0f;
+      const float scaled_here = (float)in.scaled_mip.v[current_address], mip_here = (float)in.mip.v[current_address];
+      Filter(current_defocus, 0.0f);
+      template_peak = Score(current_phi, current_theta, current_psi);
+      float starting_score = template_peak.value * sqrt_t;
+      starting_score = score_adjustment * scaled_here * starting_score / mip_here;
+      auto improve = [&](const PeakI& tp) {
+        best_peak = tp;
+        best_score = tp.value;
+        const long address_offset = rs * myroundint(tp.y) + myroundint(tp.x);
+        best_address = current_address + address_offset;
+        offset_distance = sqrtf(powf(tp.x, 2) + powf(tp.y, 2));
+      };
+      if (defocus_search_range != 0.0f) {
+        const int nd = myroundint(float(defocus_search_range) / float(defocus_step));
+        for (int defocus_is = -nd; defocus_is <= nd; ++defocus_is) {
+          Filter(current_defocus, defocus_is * defocus_step);
+          template_peak = Score(current_phi, current_theta, current_psi);
+          score = template_peak.value;
+          if (score > best_score) {
+            improve(template_peak);
+            best_defocus_local.v[best_address] = current_defocus + defocus_is * defocus_step;
+          }
+        }
+        current_defocus = (float)best_defocus_local.v[best_address];
+      }
+      int phi_i = 0, theta_i = 0, psi_i = 0, defocus_i = 0;
+      // one descent over phi, theta, psi (:870-935); on improvement the block's defocus and pixel
+      // size values are written beside the angles
+      auto angles_descent = [&](float written_defocus, float written_pixel_size) {
+        for (int kk = 0; kk < 2; kk = -2 * kk + 1) {
+          float best_phi_score;
+          do {
+            best_phi_score = best_score;
+            phi_i += kk;
+            for (int jj = 0; jj < 2; jj = -2 * jj + 1) {
+              float best_theta_score;
+              do {
+                best_theta_score = best_score;
+                theta_i += jj;
+                for (int ii = 0; ii < 2; ii = -2 * ii + 1) {
+                  float best_psi_score;
+                  do {
+                    best_psi_score = best_score;
+                    psi_i += ii;
+                    template_peak = Score(current_phi + phi_i * p.angular_step, current_theta + theta_i * p.angular_step,
+                                          current_psi + psi_i * p.in_plane_angular_step);
+                    score = template_peak.value;
+                    if (score > best_score) {
+                      improve(template_peak);
+                      best_psi_local.v[best_address] = current_psi + psi_i * p.in_plane_angular_step;
+                      best_theta_local.v[best_address] = current_theta + theta_i * p.angular_step;
+                      best_phi_local.v[best_address] = current_phi + phi_i * p.angular_step;
+                      best_defocus_local.v[best_address] = written_defocus;
+                      best_pixel_size_local.v[best_address] = written_pixel_size;
+                    }
+                  } while (best_score > best_psi_score);
+                  psi_i -= ii;
+                }
+              } while (best_score > best_theta_score);
+              theta_i -= jj;
+            }
+          } while (best_score > best_phi_score);
+          phi_i -= kk;
+        }
+      };
+      // :851-948 defocus refinement with nested angle descents
+      const float mult_i_start = defocus_step / defocus_refine_step;
+      for (float mult_i = mult_i_start; mult_i > 0.5f; mult_i /= 2.0f) {
+        for (int ll = 0; ll < 2; ll = -2 * ll + 1) {
+          if ((ll != 0) && (defocus_refine_step == 0.0f)) break;
+          float best_defocus_score;
+          do {
+            best_defocus_score = best_score;
+            if (defocus_search_range != 0.0f) defocus_i += myroundint(mult_i * ll);
+            Filter(current_defocus, defocus_i * defocus_refine_step);
+            angles_descent(current_defocus + defocus_i * defocus_refine_step, current_pixel_size);
+          } while (best_score > best_defocus_score);
+          if (defocus_search_range != 0.0f) defocus_i -= ll;
+        }
+      }
+      // :950-962 restart from the values stored at the best address
+      current_phi = (float)best_phi_local.v[best_address];
+      current_theta = (float)best_theta_local.v[best_address];
+      current_psi = (float)best_psi_local.v[best_address];
+      current_defocus = (float)best_defocus_local.v[best_address];
+      current_pixel_size = (float)best_pixel_size_local.v[best_address];
+      phi_i = theta_i = psi_i = 0;
+      Filter(current_defocus, 0.0f);
+      // :1000-1086 pixel-size block with pixel_size_search_range 0: size_i stays 0, the factor stays 1
+      // and three more angle descents run around the restart values
+      for (float mult_i = 1.0f; mult_i > 0.5f; mult_i /= 2.0f) {
+        for (int ll = 0; ll < 2; ll = -2 * ll + 1) {
+          if ((ll != 0) && (pixel_size_refine_step == 0.0f)) break;
+          float best_pixel_size_score;
+          do {
+            best_pixel_size_score = best_score;
+            angles_descent(current_defocus, current_pixel_size);
+          } while (best_score > best_pixel_size_score);
+        }
+      }
+      // :1091-1113 results
+      const float best_scaled = score_adjustment * scaled_here * best_score / mip_here * sqrt_t;
+      const float best_mip = score_adjustment * best_score * sqrt_t;
+      RefinedPeak r{};
+      r.x = found.x; r.y = found.y; r.value = found.value;
+      r.psi = (float)in.psi.v[current_address]; r.theta = (float)in.theta.v[current_address];
+      r.phi = (float)in.phi.v[current_address]; r.defocus = (float)in.defocus.v[current_address];
+      r.pixel_size = (float)in.pixel_size.v[current_address];
+      r.x_pos = found.x + best_peak.x; r.y_pos = found.y + best_peak.y;
+      r.shift_x = best_peak.x; r.shift_y = best_peak.y;
+      r.r_psi = (float)best_psi_local.v[best_address]; r.r_theta = (float)best_theta_local.v[best_address];
+      r.r_phi = (float)best_phi_local.v[best_address]; r.r_defocus = (float)best_defocus_local.v[best_address];
+      r.r_pixel_size = (float)best_pixel_size_local.v[best_address];
+      r.peak_height = best_scaled;
+      r.mip = best_mip;
+      r.best_score = best_score;
+      r.starting_score = starting_score;
+      r.offset = offset_distance;
+      r.best_address = best_address;
+      r.evaluations = evaluations;
+      result.push_back(r);
+      if (out && (offset_distance * p.pixel_size <= xy_change_threshold || !exclude_above_xy_threshold)) {
+        out->scaled_mip.v[best_address] = best_scaled;
+        out->mip.v[best_address] = best_mip;
+        out->psi.v[best_address] = best_psi_local.v[best_address];
+        out->theta.v[best_address] = best_theta_local.v[best_address];
+        out->phi.v[best_address] = best_phi_local.v[best_address];
+        out->defocus.v[best_address] = best_defocus_local.v[best_address];
+        out->pixel_size.v[best_address] = best_pixel_size_local.v[best_address];
+      }
+    }
+    return result;
+  }
+};
+
+struct RHandle32 { RefineOracle<float> rt; };
+struct RHandle64 { RefineOracle<double> rt; };
+
+template <class F>
+void with_rt(void* hv, int fp64, F f) {
+  if (fp64) f(((RHandle64*)hv)->rt);
+  else f(((RHandle32*)hv)->rt);
+}
+template <class R>
+void padded_from(Image<R>& im, int nx, int ny, const float* src) {
+  im.Allocate(nx, ny, 1, true);
+  for (long i = 0; i < im.real_memory_allocated; ++i) im.v[i] = src[i];
+}
+template <class R>
+void padded_to(const Image<R>& im, float* dst) {
+  for (long i = 0; i < im.real_memory_allocated; ++i) dst[i] = (float)im.v[i];
+}
+
 extern "C" {
 
 const char* tmo_last_error() { return g_err.c_str(); }
@@ -766,4 +1186,114 @@ float tmo_mt_finalise(void* hv, int fp64, long total_ccfs, float* mip_scaled, fl
   return f.expected_threshold;
 }
 
+
+// ---- refine_template ----
+// fparams[15] in RefineParams order; micrograph mx x my (dense), volume t^3 (dense).
+void* tmo_rt_create(const float* fp, const float* micrograph, int mx, int my, const float* volume, int t, int fp64) {
+  try {
+    RefineParams p{fp[0], fp[1], fp[2], fp[3], fp[4], fp[5], fp[6], fp[7], fp[8], fp[9], fp[10], fp[11], fp[12], fp[13], fp[14]};
+    if (fp64) { auto* h = new RHandle64(); h->rt.Setup(p, micrograph, mx, my, volume, t); return h; }
+    auto* h = new RHandle32();
+    h->rt.Setup(p, micrograph, mx, my, volume, t);
+    return h;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return nullptr;
+  }
+}
+void tmo_rt_destroy(void* h, int fp64) {
+  if (fp64) delete (RHandle64*)h; else delete (RHandle32*)h;
+}
+// info: mx, my, t, P, n_curve
+void tmo_rt_info(void* hv, int fp64, long* info) {
+  with_rt(hv, fp64, [&](auto& rt) {
+    info[0] = rt.mx; info[1] = rt.my; info[2] = rt.t; info[3] = rt.P; info[4] = rt.whitening_filter.n();
+  });
+}
+// which: 0 whitened input image (padded real), 1 template spectrum (P^3), 2 whitening curve y, 3 curve x,
+// 4 windowed particle spectrum of the last window, 5 projection filter of the last Filter call
+void tmo_rt_get(void* hv, int fp64, int which, float* out) {
+  with_rt(hv, fp64, [&](auto& rt) {
+    auto copy = [&](const auto& v, size_t n) { for (size_t i = 0; i < n; ++i) out[i] = (float)v[i]; };
+    switch (which) {
+      case 0: copy(rt.input_image.v, rt.input_image.v.size()); break;
+      case 1: copy(rt.input_reconstruction.v, rt.input_reconstruction.v.size()); break;
+      case 2: copy(rt.whitening_filter.y, rt.whitening_filter.y.size()); break;
+      case 3: copy(rt.whitening_filter.x, rt.whitening_filter.x.size()); break;
+      case 4: copy(rt.windowed_particle.v, rt.windowed_particle.v.size()); break;
+      case 5: copy(rt.projection_filter.v, rt.projection_filter.v.size()); break;
+    }
+  });
+}
+int tmo_rt_window(void* hv, int fp64, int peak_x, int peak_y) {
+  return guarded([&] { with_rt(hv, fp64, [&](auto& rt) { rt.Window(peak_x, peak_y); }); });
+}
+// The window from a t x t real image (padded layout): ForwardFFT + SwapRealSpaceQuadrants as Window does.
+int tmo_rt_window_from_real(void* hv, int fp64, const float* real_padded) {
+  return guarded([&] {
+    with_rt(hv, fp64, [&](auto& rt) {
+      padded_from(rt.windowed_particle, rt.t, rt.t, real_padded);
+      rt.windowed_particle.ForwardFFT();
+      rt.windowed_particle.SwapRealSpaceQuadrants();
+    });
+  });
+}
+// TemplateScore at (phi, theta, psi) with the filter at defocus1|2 + current + step; out = x, y, value.
+// proj_real (may be null) receives the real-space normalised filtered projection (C2R of the spectrum
+// TemplateScore correlates, before the product with the window; padded t x t).
+int tmo_rt_score(void* hv, int fp64, float current_defocus, float step, float phi, float theta, float psi, float* out,
+                 float* proj_real) {
+  return guarded([&] {
+    with_rt(hv, fp64, [&](auto& rt) {
+      rt.Filter(current_defocus, step);
+      if (proj_real) {
+        Image<typename std::remove_reference<decltype(rt)>::type::Real> cp;
+        rt.Projection(phi, theta, psi, cp);
+        cp.BackwardFFT();
+        padded_to(cp, proj_real);
+      }
+      PeakI pk = rt.Score(phi, theta, psi);
+      out[0] = pk.x; out[1] = pk.y; out[2] = pk.value;
+    });
+  });
+}
+// Peak list of a scaled MIP (padded mx x my): out = (x, y, value) per peak; returns the count or -1.
+long tmo_rt_peak_list(void* hv, int fp64, const float* scaled_mip, float* out, long cap) {
+  long n = -1;
+  guarded([&] {
+    with_rt(hv, fp64, [&](auto& rt) {
+      typedef typename std::remove_reference<decltype(rt)>::type::Real RR;
+      Image<RR> sm;
+      padded_from(sm, rt.mx, rt.my, scaled_mip);
+      auto pk = rt.PeakList(sm);
+      for (long i = 0; i < (long)pk.size() && i < cap; ++i) { out[3 * i] = pk[i].x; out[3 * i + 1] = pk[i].y; out[3 * i + 2] = pk[i].value; }
+      n = (long)pk.size();
+    });
+  });
+  return n;
+}
+// Full refinement. maps[7]: mip, scaled_mip, psi, theta, phi, defocus, pixel_size (padded); out_maps[7]
+// (or null) receive best_mip, best_scaled_mip, best_psi, ... ; peaks receive up to cap records.
+// Returns the number of peaks or -1.
+long tmo_rt_run(void* hv, int fp64, const float* const* maps, int shared_local_maps, float xy_change_threshold,
+                int exclude_above_xy_threshold, float* const* out_maps, RefinedPeak* peaks, long cap) {
+  long n = -1;
+  guarded([&] {
+    with_rt(hv, fp64, [&](auto& rt) {
+      typedef typename std::remove_reference<decltype(rt)>::type RT;
+      typename RT::Maps in, out;
+      Image<typename RT::Real>* dst[7] = {&in.mip, &in.scaled_mip, &in.psi, &in.theta, &in.phi, &in.defocus, &in.pixel_size};
+      for (int k = 0; k < 7; ++k) padded_from(*dst[k], rt.mx, rt.my, maps[k]);
+      auto res = rt.Run(in, shared_local_maps != 0, xy_change_threshold, exclude_above_xy_threshold != 0,
+                        out_maps ? &out : nullptr);
+      if (out_maps) {
+        const Image<typename RT::Real>* src[7] = {&out.mip, &out.scaled_mip, &out.psi, &out.theta, &out.phi, &out.defocus, &out.pixel_size};
+        for (int k = 0; k < 7; ++k) if (out_maps[k]) padded_to(*src[k], out_maps[k]);
+      }
+      for (long i = 0; i < (long)res.size() && i < cap; ++i) peaks[i] = res[i];
+      n = (long)res.size();
+    });
+  });
+  return n;
+}
 }  // extern "C"
