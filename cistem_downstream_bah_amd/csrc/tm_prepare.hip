@@ -3,6 +3,7 @@
 //   tm_prepare_sizes / tm_prepare_input_image   match_template.cpp:490-598, 703-724
 //   tm_prepare_template                         match_template.cpp:621-625, 837-845
 //   tm_projection_filter                        match_template.cpp:893-898 (CTF x whitening)
+//   tm_prepare_refine_image                     refine_template.cpp:476-510 (whitened real-space image)
 //   tm_search_grid / tm_psi_list                euler_search.cpp:180-282, 353-443; match_template.cpp:658-697, 1001
 // Host loops restate the reference's Image/Curve/CTF arithmetic; the 2-D and 3-D R2C transforms
 // run on the device with the same kernels as the search.
@@ -211,20 +212,40 @@ void rows_r2c(Dev& d, int N, const float* src, size_t sbs, int srs, cf* dst, siz
   }
 }
 template <int N>
-void cols_t(Dev& d, cf* data, int ncols, size_t bstride, int nb) {
+void cols_t(Dev& d, cf* data, int ncols, size_t bstride, int nb, int sign) {
   constexpr int NT = N >= 2048 ? 256 : 64;
   cf* tw = d.twiddles(N);
-  hipLaunchKernelGGL((tmk::k_cols<N, NT, -1, 0>), dim3(ncols, nb), dim3(NT), 0, d.s, data, ncols, bstride, tw,
-                     (const cf*)nullptr, 1.0f);
+  if (sign < 0)
+    hipLaunchKernelGGL((tmk::k_cols<N, NT, -1, 0>), dim3(ncols, nb), dim3(NT), 0, d.s, data, ncols, bstride, tw,
+                       (const cf*)nullptr, 1.0f);
+  else
+    hipLaunchKernelGGL((tmk::k_cols<N, NT, +1, 0>), dim3(ncols, nb), dim3(NT), 0, d.s, data, ncols, bstride, tw,
+                       (const cf*)nullptr, 1.0f);
   chk("k_cols");
 }
-void cols(Dev& d, int N, cf* data, int ncols, size_t bstride, int nb) {
+void cols(Dev& d, int N, cf* data, int ncols, size_t bstride, int nb, int sign = -1) {
   switch (N) {
 #define TM_CASE(n) \
-  case n: cols_t<n>(d, data, ncols, bstride, nb); break;
+  case n: cols_t<n>(d, data, ncols, bstride, nb, sign); break;
     TM_LINE_SIZES(TM_CASE)
 #undef TM_CASE
     default: throw PErr{TM_ERR_UNSUPPORTED, "unsupported column length " + std::to_string(N)};
+  }
+}
+template <int N>
+void rows_c2r_t(Dev& d, const cf* in, size_t ibs, int irs, float* out, size_t obs, int ors, int nrows, int nb) {
+  constexpr int NT = (N / 2) >= 1024 ? 256 : 64;
+  hipLaunchKernelGGL((tmk::k_rows_c2r<N, NT>), dim3(nrows, nb), dim3(NT), 0, d.s, in, ibs, irs, out, obs, ors,
+                     d.twiddles(N), d.twiddles(N / 2));
+  chk("k_rows_c2r");
+}
+void rows_c2r(Dev& d, int N, const cf* in, size_t ibs, int irs, float* out, size_t obs, int ors, int nrows, int nb) {
+  switch (N) {
+#define TM_CASE(n) \
+  case n: rows_c2r_t<n>(d, in, ibs, irs, out, obs, ors, nrows, nb); break;
+    TM_LINE_SIZES(TM_CASE)
+#undef TM_CASE
+    default: throw PErr{TM_ERR_UNSUPPORTED, "unsupported row length " + std::to_string(N)};
   }
 }
 
@@ -245,6 +266,24 @@ std::vector<std::complex<float>> forward_fft(int device, const Img& a) {
   P_HIP(hipStreamSynchronize(d.s));
   std::vector<std::complex<float>> out(plane * a.nz);
   P_HIP(hipMemcpy(out.data(), spec, out.size() * sizeof(cf), hipMemcpyDeviceToHost));
+  return out;
+}
+
+// Unscaled C2R of a 2-D spectrum (H x (W/2+1)) on the device (BackwardFFT, image.cpp:5104-5116);
+// returns the real image in the padded layout.
+Img backward_fft(int device, const std::vector<std::complex<float>>& spec, int nx, int ny) {
+  P_REQ(line_size_compiled(nx) && line_size_compiled(ny), TM_ERR_UNSUPPORTED,
+        "no compiled device FFT for size " + std::to_string(nx) + "x" + std::to_string(ny));
+  Dev d(device);
+  const int nh = nx / 2 + 1;
+  cf* buf = d.alloc<cf>((size_t)nh * ny);
+  P_HIP(hipMemcpy(buf, spec.data(), spec.size() * sizeof(cf), hipMemcpyHostToDevice));
+  cols(d, ny, buf, nh, 0, 1, +1);
+  rows_c2r(d, nx, buf, 0, nh, reinterpret_cast<float*>(buf), 0, nx + 2, ny, 1);
+  P_HIP(hipStreamSynchronize(d.s));
+  Img out;
+  out.alloc(nx, ny, 1);
+  P_HIP(hipMemcpy(out.v.data(), buf, out.v.size() * sizeof(float), hipMemcpyDeviceToHost));
   return out;
 }
 
@@ -338,6 +377,54 @@ float spectrum_sum_of_squares(const std::complex<float>* c, int nx, int ny) {
   return 2.0 * sum;
 }
 
+// ReplaceOutliersWithMean(5) (image.cpp:986-1100) over the logical pixels of a 2-D image
+void replace_outliers(Img& img) {
+  const int W = img.nx, H = img.ny;
+  double s = 0.0, s2 = 0.0;
+  for (int y = 0; y < H; ++y)
+    for (int x = 0; x < W; ++x) { const float v = img.at(x, y); s += v; s2 += powf(v, 2); }
+  const long n = (long)W * H;
+  const float sigma = sqrtf(fabsf(float(s2 / n - powf(s / n, 2))));
+  const float mean = float(s / n);
+  const float mx_ = mean + 5.0f * sigma, mn_ = mean - 5.0f * sigma;
+  for (int y = 0; y < H; ++y)
+    for (int x = 0; x < W; ++x) {
+      float& v = img.at(x, y);
+      if (v > mx_) v = mean;
+      else if (v < mn_) v = mean;
+    }
+}
+
+// Whitening curve of a spectrum (H x (W/2+1), DC already 0): Compute1DPowerSpectrumCurve
+// (image.cpp:7135-7205; intensities, Hermitian mates skipped), SquareRoot, Reciprocal, scaled to a
+// maximum of 1 (match_template.cpp:711-716, refine_template.cpp:501-505).
+Curve whitening_curve(const std::complex<float>* spec, int W, int H, int ncurve) {
+  Curve wf, nt;
+  wf.setup(0.0, 0.5 * sqrtf(2.0), ncurve);
+  nt.setup(0.0, 0.5 * sqrtf(2.0), ncurve);
+  const int nh = W / 2 + 1;
+  const int fny = H / 2;
+  for (int j = 0; j < H; ++j) {
+    const float sy = powf((j >= fny ? j - H : j) * (1.0 / double(H)), 2);
+    for (int i = 0; i < nh; ++i) {
+      if (i == 0 && j >= fny) continue;
+      const float sx = powf(i * (1.0 / double(W)), 2);
+      const float sf = sqrtf(sx + sy + 0.0f);
+      const std::complex<float> c = spec[(size_t)j * nh + i];
+      wf.add(sf, c.real() * c.real() + c.imag() * c.imag());
+      nt.add(sf, 1.0);
+    }
+  }
+  for (int c = 0; c < ncurve; ++c) wf.y[c] = nt.y[c] > 0.0 ? wf.y[c] / nt.y[c] : 0.0f;
+  for (auto& a : wf.y) a = sqrtf(a);
+  for (auto& a : wf.y) if (a != 0.0) a = 1.0 / a;
+  float mxv = -FLT_MAX;
+  for (float a : wf.y) mxv = std::max(mxv, a);
+  const float k = 1.0f / mxv;
+  for (auto& a : wf.y) a *= k;
+  return wf;
+}
+
 }  // namespace
 
 extern "C" {
@@ -377,51 +464,13 @@ int tm_prepare_input_image(int device, const float* micrograph, int mx, int my, 
         for (int xo = 0; xo < img.nx; ++xo) b.at(img.ny - yo - 1, xo) = img.at(xo, yo);
       img = std::move(b);
     }
-    // ReplaceOutliersWithMean(5) (image.cpp:986-1100) over the logical pixels
-    {
-      double s = 0.0, s2 = 0.0;
-      for (int y = 0; y < H; ++y)
-        for (int x = 0; x < W; ++x) { const float v = img.at(x, y); s += v; s2 += powf(v, 2); }
-      const long n = (long)W * H;
-      const float sigma = sqrtf(fabsf(float(s2 / n - powf(s / n, 2))));
-      const float mean = float(s / n);
-      const float mx_ = mean + 5.0f * sigma, mn_ = mean - 5.0f * sigma;
-      for (int y = 0; y < H; ++y)
-        for (int x = 0; x < W; ++x) {
-          float& v = img.at(x, y);
-          if (v > mx_) v = mean;
-          else if (v < mn_) v = mean;
-        }
-    }
+    replace_outliers(img);
     auto spec = forward_fft(device, img);
     const float inv_n = 1. / (float)((long)W * H);  // ForwardFFT scaling (image.cpp:5096-5099)
     for (auto& c : spec) c *= inv_n;
     swap_quadrants(spec.data(), W, H, 1);
     spec[0] = 0.f;
-    // Compute1DPowerSpectrumCurve (image.cpp:7135-7205), intensities, Hermitian mates skipped
-    Curve wf, nt;
-    wf.setup(0.0, 0.5 * sqrtf(2.0), ncurve);
-    nt.setup(0.0, 0.5 * sqrtf(2.0), ncurve);
-    const int nh = W / 2 + 1;
-    const int fny = H / 2;
-    for (int j = 0; j < H; ++j) {
-      const float sy = powf((j >= fny ? j - H : j) * (1.0 / double(H)), 2);
-      for (int i = 0; i < nh; ++i) {
-        if (i == 0 && j >= fny) continue;
-        const float sx = powf(i * (1.0 / double(W)), 2);
-        const float sf = sqrtf(sx + sy + 0.0f);
-        const std::complex<float> c = spec[(size_t)j * nh + i];
-        wf.add(sf, c.real() * c.real() + c.imag() * c.imag());
-        nt.add(sf, 1.0);
-      }
-    }
-    for (int c = 0; c < ncurve; ++c) wf.y[c] = nt.y[c] > 0.0 ? wf.y[c] / nt.y[c] : 0.0f;
-    for (auto& a : wf.y) a = sqrtf(a);
-    for (auto& a : wf.y) if (a != 0.0) a = 1.0 / a;
-    float mxv = -FLT_MAX;
-    for (float a : wf.y) mxv = std::max(mxv, a);
-    const float k = 1.0f / mxv;
-    for (auto& a : wf.y) a *= k;
+    Curve wf = whitening_curve(spec.data(), W, H, ncurve);
     apply_curve(spec.data(), W, H, wf);
     spec[0] = 0.f;
     const float inv = 1. / sqrtf(spectrum_sum_of_squares(spec.data(), W, H));
@@ -529,6 +578,36 @@ int tm_psi_list(float psi_start, float psi_max, float psi_step, float* out, int 
     ++n;
   }
   return n;
+}
+
+
+int tm_prepare_refine_image(int device, const float* micrograph, int mx, int my, float* whitened, float* curve_x,
+                            float* curve_y, int* n_curve) {
+  return pguard([&] {
+    P_REQ(micrograph && mx > 0 && my > 0 && even(mx), TM_ERR_INVALID, "bad arguments (the image width must be even)");
+    const int ncurve = int((mx / 2.0 + 1.0) * sqrtf(2.0) + 1.0);
+    if (n_curve) *n_curve = ncurve;
+    if (!whitened) return;
+    P_REQ(curve_x && curve_y, TM_ERR_INVALID, "NULL curve argument");
+    Img img;
+    img.alloc(mx, my, 1);
+    for (int y = 0; y < my; ++y) std::memcpy(&img.at(0, y), micrograph + (size_t)y * mx, mx * sizeof(float));
+    replace_outliers(img);
+    auto spec = forward_fft(device, img);
+    const float inv_n = 1. / (float)((long)mx * my);  // ForwardFFT scaling (image.cpp:5096-5099)
+    for (auto& c : spec) c *= inv_n;
+    spec[0] = 0.f;
+    Curve wf = whitening_curve(spec.data(), mx, my, ncurve);
+    apply_curve(spec.data(), mx, my, wf);
+    spec[0] = 0.f;
+    // DivideByConstant(sqrt(ReturnSumOfSquares())) (refine_template.cpp:508)
+    const float inv = 1. / (float)sqrt((double)spectrum_sum_of_squares(spec.data(), mx, my));
+    for (auto& c : spec) c *= inv;
+    Img out = backward_fft(device, spec, mx, my);
+    for (int y = 0; y < my; ++y) std::memcpy(whitened + (size_t)y * mx, &out.at(0, y), mx * sizeof(float));
+    std::memcpy(curve_x, wf.x.data(), ncurve * sizeof(float));
+    std::memcpy(curve_y, wf.y.data(), ncurve * sizeof(float));
+  });
 }
 
 }  // extern "C"

@@ -1,0 +1,1029 @@
+// Local refinement of template-matching peaks on the GPU and its C ABI (include/tm_refine.h):
+// the per-image work of refine_template (src/programs/refine_template/refine_template.cpp).
+//
+// Each peak's coordinate descent (refine_template.cpp:748-1087) is a C++20 coroutine that suspends
+// at every TemplateScore call. The driver keeps up to max_in_flight peaks in flight, collects their
+// pending TemplateScore requests into one batch, runs the batch as a few device launches (slice
+// extraction, the slice's C2R and the projection's R2C, the CTF x whitening product fused with the
+// spectral energy and the conjugate product with the particle window, the correlation's C2R and a
+// per-item argmax), and resumes every peak with its result. Items of a batch share nothing, so a
+// peak's result does not depend on which peaks were in flight with it.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cfloat>
+#include <climits>
+#include <cmath>
+#include <coroutine>
+#include <cstring>
+#include <exception>
+#include <map>
+#include <memory>
+#include <string>
+#include <unordered_map>
+#include <utility>
+#include <vector>
+
+#include "../../include/tm_refine.h"
+#include "tm_host.hpp"
+
+namespace rtk {
+
+using tmk::cf;
+using tmk::cmake;
+using tmk::cmul;
+using tmk::cconj;
+
+// CTF constants of CTF::Init (ctf.cpp:93-150) in pixel units.
+struct CtfConst {
+  float wavelength, squared_wavelength, spherical_aberration, phase_shift, amplitude_term, astigmatism_azimuth;
+};
+
+// Whitening curve lookup, Curve::ReturnLinearInterpolationFromX (curve.cpp:377-396) with the bin of
+// ReturnIndexOfNearestPreviousBin (curve.cpp:834-870): the c with x[c] <= wx < x[c+1].
+__device__ float curve_value(const float* __restrict__ cx, const float* __restrict__ cy, int n, float wx) {
+#pragma clang fp contract(off)
+  int c;
+  if (wx < cx[0]) {
+    c = 0;
+  } else {
+    c = (int)((wx - cx[0]) / (cx[1] - cx[0]));
+    c = c < 0 ? 0 : (c > n - 1 ? n - 1 : c);
+    while (c > 0 && wx < cx[c]) --c;
+    while (c < n - 1 && wx >= cx[c + 1]) ++c;
+  }
+  float r = 0.0f;
+  if (c == n - 1) {
+    r = cy[n - 1];
+  } else {
+    const float d = (wx - cx[c]) / (cx[c + 1] - cx[c]);
+    r = (float)((double)r + (double)cy[c] * (1.0 - (double)d));
+    r += cy[c + 1] * d;
+  }
+  return r;
+}
+
+// Projection filter value at pixel (i, j) of a t x (t/2+1) spectrum: Image::CalculateCTFImage
+// (image.cpp:3113-3165, CTF::Evaluate ctf.cpp:445-461 with ctf.cpp:497-500 and DefocusGivenAzimuth
+// ctf.cpp:538-541) times Image::ApplyCurveFilter (image.cpp:7284-7320), in the reference's float and
+// double steps, without contraction into fused multiply-adds.
+__device__ float filter_value(int i, int j, int t, float d1, float d2, const CtfConst& k, const float* __restrict__ cx,
+                              const float* __restrict__ cy, int n_curve) {
+#pragma clang fp contract(off)
+  const double fv = 1.0 / (double)t;
+  const int ly = j >= t / 2 ? j - t : j;
+  const float y2 = (float)(ly * fv);
+  const float x2 = (float)(i * fv);
+  const float az = (i == 0 && j == 0) ? 0.0f : atan2f(y2, x2);
+  const float f2 = x2 * x2 + y2 * y2;
+  float ctf;
+  if (d1 == 0.0f && d2 == 0.0f) {
+    ctf = -0.7f;
+  } else {
+    const float df = 0.5f * (d1 + d2 + cosf(2.0f * (az - k.astigmatism_azimuth)) * (d1 - d2));
+    const float chi = 3.14159265359f * k.wavelength * f2 * (df - 0.5f * k.squared_wavelength * f2 * k.spherical_aberration) +
+                      k.phase_shift + k.amplitude_term;
+    ctf = -sinf(chi);
+  }
+  const double dx = i * fv, dy = ly * fv;
+  const float sx = (float)(dx * dx), sy = (float)(dy * dy);
+  const float s2 = sx + sy + 0.0f;
+  const float sf = (float)sqrt((double)s2);
+  if (!(sf <= 1.0f)) return 0.0f;
+  return ctf * curve_value(cx, cy, n_curve, sf);
+}
+
+// Particle windows (refine_template.cpp:738-741): window p is the t x t centre clip of the
+// whitened image after RealSpaceIntegerShift (image.cpp:9064-9107), i.e. pixel (ii, jj) reads the
+// image at linear index (k - shift0[p]) mod n with k the index of (mx/2 + ii - t/2, my/2 + jj - t/2).
+// Output rows of t + 2 floats (padded layout).
+__global__ void k_window_gather(const float* __restrict__ img, int mx, int my, int t,
+                                const long long* __restrict__ shift0, float* __restrict__ out) {
+  const int jj = blockIdx.x, p = blockIdx.y;
+  const long long n = (long long)mx * my;
+  const int j = my / 2 + jj - t / 2;
+  float* o = out + ((size_t)p * t + jj) * (t + 2);
+  for (int ii = threadIdx.x; ii < t + 2; ii += blockDim.x) {
+    float v = 0.0f;
+    const int i = mx / 2 + ii - t / 2;
+    if (ii < t && i >= 0 && i < mx && j >= 0 && j < my) {
+      long long k = (long long)j * mx + i - shift0[p];
+      if (k < 0) k += n;
+      v = img[k];
+    }
+    o[ii] = v;
+  }
+}
+
+// Window spectrum times the quadrant-swap phase (SwapRealSpaceQuadrants), scattered to its slot.
+__global__ void k_window_phase(const cf* __restrict__ stage, const cf* __restrict__ phase, int per,
+                               const int* __restrict__ slots, cf* __restrict__ win) {
+  const int p = blockIdx.y;
+  const cf* s = stage + (size_t)p * per;
+  cf* d = win + (size_t)slots[p] * per;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < per; e += gridDim.x * blockDim.x) d[e] = cmul(s[e], phase[e]);
+}
+
+// TemplateScore (refine_template.cpp:49-66) on the projection spectra of a batch, item b:
+// c = C x filter (MultiplyPixelWise), DC 0 (ZeroCentralPixel), the Hermitian-weighted sum of |c|^2
+// of ReturnSumOfSquares' Fourier branch (image.cpp:185-311) into one partial per block, and
+// C <- conj(c) x window. The division by sqrt(sum) is applied to the peak value afterwards (a
+// positive scale does not move the peak).
+template <int NT>
+__global__ __launch_bounds__(NT) void k_score_product(cf* __restrict__ C, int t, const cf* __restrict__ win,
+                                                      const int* __restrict__ slots, const float* __restrict__ dfs,
+                                                      CtfConst k, const float* __restrict__ cx,
+                                                      const float* __restrict__ cy, int n_curve,
+                                                      double* __restrict__ partial) {
+  __shared__ double red[NT];
+  const int th = t / 2 + 1, per = t * th, b = blockIdx.y;
+  cf* c = C + (size_t)b * per;
+  const cf* w = win + (size_t)slots[b] * per;
+  const float d1 = dfs[2 * b], d2 = dfs[2 * b + 1];
+  double sum = 0.0;
+  for (int e = blockIdx.x * NT + threadIdx.x; e < per; e += gridDim.x * NT) {
+    const int j = e / th, i = e - j * th;
+    const float f = filter_value(i, j, t, d1, d2, k, cx, cy, n_curve);
+    cf v = c[e];
+    v = cmake(v.x * f, v.y * f);
+    if (e == 0) v = cmake(0.0f, 0.0f);
+    const int jj = j >= t / 2 ? j - t : j;
+    const bool edge_x = (i == 0 || i == t / 2);
+    const double a = (double)hypotf(v.x, v.y);
+    const double a2 = a * a;
+    if (edge_x && (jj == 0 || jj == -t / 2)) sum += a2 * 0.5;
+    else if (!edge_x || jj >= 0) sum += a2;
+    c[e] = cmul(cconj(v), w[e]);
+  }
+  red[threadIdx.x] = sum;
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[(size_t)b * gridDim.x + blockIdx.x] = red[0];
+}
+
+// Peak of each correlation image (Image::FindPeakWithIntegerCoordinates with no radius or edge
+// limit, image.cpp:9961-10105): the first maximum in scan order, reported relative to the box centre,
+// its value divided by sqrtf(ReturnSumOfSquares) of the filtered projection.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_score_peak(const float* __restrict__ R, int t, const double* __restrict__ partial,
+                                                   int nparts, float* __restrict__ out) {
+  __shared__ float rv[NT];
+  __shared__ int ri[NT];
+  const int b = blockIdx.x;
+  const float* r = R + (size_t)b * t * (t + 2);
+  float best = -FLT_MAX;
+  int bi = 0x7fffffff;
+  for (int e = threadIdx.x; e < t * t; e += NT) {
+    const int j = e / t, i = e - j * t;
+    const float v = r[(size_t)j * (t + 2) + i];
+    if (v > best) { best = v; bi = e; }
+  }
+  rv[threadIdx.x] = best;
+  ri[threadIdx.x] = bi;
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      const float v = rv[threadIdx.x + s];
+      const int q = ri[threadIdx.x + s];
+      if (v > rv[threadIdx.x] || (v == rv[threadIdx.x] && q < ri[threadIdx.x])) { rv[threadIdx.x] = v; ri[threadIdx.x] = q; }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double ss = 0.0;
+    for (int q = 0; q < nparts; ++q) ss += partial[(size_t)b * nparts + q];
+    const float sos = (float)(2.0 * ss);
+    const float inv = (float)(1. / sqrtf(sos));
+    const int e = ri[0] == 0x7fffffff ? 0 : ri[0];
+    out[3 * b] = (float)(e % t - t / 2);
+    out[3 * b + 1] = (float)(e / t - t / 2);
+    out[3 * b + 2] = ri[0] == 0x7fffffff ? -FLT_MAX : rv[0] * inv;
+  }
+}
+
+// Peak list on the scaled MIP (refine_template.cpp:561-600): first maximum in scan order over the
+// pixels at least `edge` from the borders (upper bounds inclusive), as (value, dense index) partials
+// per block, then one block combines them.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_map_argmax(const float* __restrict__ a, int mx, int my, int edge,
+                                                   float* __restrict__ pv, long long* __restrict__ pi) {
+  __shared__ float rv[NT];
+  __shared__ long long ri[NT];
+  const long long n = (long long)mx * my;
+  float best = -FLT_MAX;
+  long long bi = LLONG_MAX;
+  for (long long e = (long long)blockIdx.x * NT + threadIdx.x; e < n; e += (long long)gridDim.x * NT) {
+    const int j = (int)(e / mx), i = (int)(e - (long long)j * mx);
+    if (i >= edge && i <= mx - edge && j >= edge && j <= my - edge) {
+      const float v = a[e];
+      if (v > best) { best = v; bi = e; }
+    }
+  }
+  rv[threadIdx.x] = best;
+  ri[threadIdx.x] = bi;
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      const float v = rv[threadIdx.x + s];
+      const long long q = ri[threadIdx.x + s];
+      if (v > rv[threadIdx.x] || (v == rv[threadIdx.x] && q < ri[threadIdx.x])) { rv[threadIdx.x] = v; ri[threadIdx.x] = q; }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { pv[blockIdx.x] = rv[0]; pi[blockIdx.x] = ri[0]; }
+}
+
+template <int NT>
+__global__ __launch_bounds__(NT) void k_argmax_final(const float* __restrict__ pv, const long long* __restrict__ pi, int n,
+                                                     float* __restrict__ ov, long long* __restrict__ oi) {
+  __shared__ float rv[NT];
+  __shared__ long long ri[NT];
+  float best = -FLT_MAX;
+  long long bi = LLONG_MAX;
+  for (int e = threadIdx.x; e < n; e += NT) {
+    const float v = pv[e];
+    const long long q = pi[e];
+    if (v > best || (v == best && q < bi)) { best = v; bi = q; }
+  }
+  rv[threadIdx.x] = best;
+  ri[threadIdx.x] = bi;
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      const float v = rv[threadIdx.x + s];
+      const long long q = ri[threadIdx.x + s];
+      if (v > rv[threadIdx.x] || (v == rv[threadIdx.x] && q < ri[threadIdx.x])) { rv[threadIdx.x] = v; ri[threadIdx.x] = q; }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { *ov = rv[0]; *oi = ri[0]; }
+}
+
+// Pixels whose squared distance from (cx, cy) is <= radius become -FLT_MAX (refine_template.cpp:583-590;
+// the program compares the squared distance with min_peak_radius). Launched over a box that holds
+// every such pixel.
+__global__ void k_map_mask(float* __restrict__ a, int mx, int x0, int y0, int w, int hgt, float cx, float cy,
+                           float radius) {
+#pragma clang fp contract(off)
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= w * hgt) return;
+  const int j = y0 + e / w, i = x0 + e % w;
+  const float dy = (float)j - cy, dx = (float)i - cx;
+  const float sq = (float)((double)dx * dx) + (float)((double)dy * dy);
+  if (sq <= radius) a[(size_t)j * mx + i] = -FLT_MAX;
+}
+
+}  // namespace rtk
+
+namespace {
+
+using tmk::cf;
+thread_local std::string g_rerr;
+
+// ------------------------------------------------------------------ descent coroutine
+struct ScoreRequest {
+  float phi, theta, psi;  // degrees
+  float df1, df2;         // SetDefocus arguments (pixels)
+};
+struct ScoreResult {
+  float x, y, value;
+};
+
+struct DescentTask {
+  struct promise_type {
+    ScoreRequest req{};
+    ScoreResult res{};
+    std::exception_ptr error;
+    DescentTask get_return_object() { return DescentTask{std::coroutine_handle<promise_type>::from_promise(*this)}; }
+    std::suspend_always initial_suspend() noexcept { return {}; }
+    std::suspend_always final_suspend() noexcept { return {}; }
+    void return_void() noexcept {}
+    void unhandled_exception() noexcept { error = std::current_exception(); }
+  };
+  std::coroutine_handle<promise_type> h;
+  explicit DescentTask(std::coroutine_handle<promise_type> c) : h(c) {}
+  DescentTask(DescentTask&& o) noexcept : h(std::exchange(o.h, {})) {}
+  DescentTask& operator=(DescentTask&& o) noexcept {
+    if (this != &o) {
+      if (h) h.destroy();
+      h = std::exchange(o.h, {});
+    }
+    return *this;
+  }
+  DescentTask(const DescentTask&) = delete;
+  ~DescentTask() {
+    if (h) h.destroy();
+  }
+};
+
+// co_await Score{...} suspends the peak until the batch holding its request has run.
+struct Score {
+  ScoreRequest r;
+  std::coroutine_handle<DescentTask::promise_type> c{};
+  bool await_ready() const noexcept { return false; }
+  void await_suspend(std::coroutine_handle<DescentTask::promise_type> cc) noexcept {
+    c = cc;
+    c.promise().req = r;
+  }
+  ScoreResult await_resume() const noexcept { return c.promise().res; }
+};
+
+// The per-thread result maps best_*_local of the program (refine_template.cpp:682-689) as the input
+// maps plus the addresses written so far. Addresses are those of the padded layout (rows of mx + 2);
+// the two padding columns read as 0.
+enum { M_PSI, M_THETA, M_PHI, M_DEFOCUS, M_PIXEL_SIZE };
+struct LocalMaps {
+  const float* in[5] = {};  // psi, theta, phi, defocus, pixel_size (dense mx * my)
+  int mx = 0, my = 0;
+  std::unordered_map<long long, std::array<float, 5>> written;
+  float base(long long addr, int f) const {
+    const long long rs = mx + 2;
+    if (addr < 0 || addr >= rs * my) throw TmError{TM_ERR_INVALID, "refined peak address outside the maps"};
+    const long long row = addr / rs, col = addr % rs;
+    return col < mx ? in[f][row * mx + col] : 0.0f;
+  }
+  float get(long long addr, int f) const {
+    auto it = written.find(addr);
+    return it != written.end() ? it->second[f] : base(addr, f);
+  }
+  void set(long long addr, int f, float v) {
+    auto it = written.find(addr);
+    if (it == written.end()) {
+      std::array<float, 5> a;
+      for (int q = 0; q < 5; ++q) a[q] = base(addr, q);
+      it = written.emplace(addr, a).first;
+    }
+    it->second[f] = v;
+  }
+};
+
+struct RunConst {
+  tm_refine_params p;
+  const float* mip = nullptr;         // dense
+  const float* scaled_mip = nullptr;  // dense
+  float defocus_search_range = 0, defocus_refine_step = 0, defocus_step = 0, pixel_size_refine_step = 0, sqrt_t = 0;
+  long long rs = 0;  // padded row stride mx + 2
+};
+
+struct PeakState {
+  float x = 0, y = 0, value = 0;  // found peak, relative to the image centre
+  LocalMaps* maps = nullptr;
+  tm_refined_peak rec{};
+  int evaluations = 0;
+};
+
+inline int myroundint(float a) { return a > 0 ? int(a + 0.5) : int(a - 0.5); }
+
+float dense_at(const float* m, int mx, long long rs, long long addr) {
+  const long long row = addr / rs, col = addr % rs;
+  return col < mx ? m[row * mx + col] : 0.0f;
+}
+
+// The program's per-peak body (refine_template.cpp:738-1113 with pixel_size_search_range 0).
+DescentTask refine_peak(const RunConst& K, PeakState& st) {
+  const tm_refine_params& p = K.p;
+  LocalMaps& L = *st.maps;
+  const float box_cx = (float)(p.image_x / 2), box_cy = (float)(p.image_y / 2);
+  const float px = st.x + box_cx, py = st.y + box_cy;
+  const long long current_address = (long long)myroundint(py) * K.rs + myroundint(px);
+  float current_phi = L.get(current_address, M_PHI);
+  float current_theta = L.get(current_address, M_THETA);
+  float current_psi = L.get(current_address, M_PSI);
+  float current_defocus = L.get(current_address, M_DEFOCUS);
+  float current_pixel_size = L.get(current_address, M_PIXEL_SIZE);
+  const float scaled_here = dense_at(K.scaled_mip, p.image_x, K.rs, current_address);
+  const float mip_here = dense_at(K.mip, p.image_x, K.rs, current_address);
+  float best_score = -FLT_MAX, score;
+  long long best_address = current_address;
+  ScoreResult best_peak{0, 0, 0}, tp{0, 0, 0};
+  float offset_distance = 0.f;
+  const float score_adjustment = 1.0f;
+  auto req = [&](float cur, float step, float phi, float theta, float psi) {
+    ++st.evaluations;
+    return ScoreRequest{phi, theta, psi, (p.defocus1 + cur + step) / p.pixel_size, (p.defocus2 + cur + step) / p.pixel_size};
+  };
+  auto improve = [&](const ScoreResult& r) {
+    best_peak = r;
+    best_score = r.value;
+    const long long address_offset = K.rs * myroundint(r.y) + myroundint(r.x);
+    best_address = current_address + address_offset;
+    offset_distance = sqrtf(powf(r.x, 2) + powf(r.y, 2));
+  };
+  tp = co_await Score{req(current_defocus, 0.0f, current_phi, current_theta, current_psi)};
+  float starting_score = tp.value * K.sqrt_t;
+  starting_score = score_adjustment * scaled_here * starting_score / mip_here;
+  if (K.defocus_search_range != 0.0f) {
+    const int nd = myroundint(float(K.defocus_search_range) / float(K.defocus_step));
+    for (int defocus_is = -nd; defocus_is <= nd; ++defocus_is) {
+      tp = co_await Score{req(current_defocus, defocus_is * K.defocus_step, current_phi, current_theta, current_psi)};
+      score = tp.value;
+      if (score > best_score) {
+        improve(tp);
+        L.set(best_address, M_DEFOCUS, current_defocus + defocus_is * K.defocus_step);
+      }
+    }
+    current_defocus = L.get(best_address, M_DEFOCUS);
+  }
+  int phi_i = 0, theta_i = 0, psi_i = 0, defocus_i = 0;
+  // block 0: defocus refinement with nested angle descents (:851-948); block 1: the pixel-size
+  // block with size_i fixed at 0 (:1000-1086), after restarting from the values at best_address
+  for (int block = 0; block < 2; ++block) {
+    float filter_step = 0.0f;
+    const float mult_start = block == 0 ? K.defocus_step / K.defocus_refine_step : 1.0f;
+    const float outer_step = block == 0 ? K.defocus_refine_step : K.pixel_size_refine_step;
+    for (float mult_i = mult_start; mult_i > 0.5f; mult_i /= 2.0f) {
+      for (int ll = 0; ll < 2; ll = -2 * ll + 1) {
+        if ((ll != 0) && (outer_step == 0.0f)) break;
+        float best_outer_score;
+        do {
+          best_outer_score = best_score;
+          float written_defocus = current_defocus;
+          if (block == 0) {
+            if (K.defocus_search_range != 0.0f) defocus_i += myroundint(mult_i * ll);
+            filter_step = defocus_i * K.defocus_refine_step;
+            written_defocus = current_defocus + defocus_i * K.defocus_refine_step;
+          }
+          for (int kk = 0; kk < 2; kk = -2 * kk + 1) {
+            float best_phi_score;
+            do {
+              best_phi_score = best_score;
+              phi_i += kk;
+              for (int jj = 0; jj < 2; jj = -2 * jj + 1) {
+                float best_theta_score;
+                do {
+                  best_theta_score = best_score;
+                  theta_i += jj;
+                  for (int ii = 0; ii < 2; ii = -2 * ii + 1) {
+                    float best_psi_score;
+                    do {
+                      best_psi_score = best_score;
+                      psi_i += ii;
+                      tp = co_await Score{req(current_defocus, filter_step, current_phi + phi_i * p.angular_step,
+                                              current_theta + theta_i * p.angular_step,
+                                              current_psi + psi_i * p.in_plane_angular_step)};
+                      score = tp.value;
+                      if (score > best_score) {
+                        improve(tp);
+                        L.set(best_address, M_PSI, current_psi + psi_i * p.in_plane_angular_step);
+                        L.set(best_address, M_THETA, current_theta + theta_i * p.angular_step);
+                        L.set(best_address, M_PHI, current_phi + phi_i * p.angular_step);
+                        L.set(best_address, M_DEFOCUS, written_defocus);
+                        L.set(best_address, M_PIXEL_SIZE, current_pixel_size);
+                      }
+                    } while (best_score > best_psi_score);
+                    psi_i -= ii;
+                  }
+                } while (best_score > best_theta_score);
+                theta_i -= jj;
+              }
+            } while (best_score > best_phi_score);
+            phi_i -= kk;
+          }
+        } while (best_score > best_outer_score);
+        if (block == 0 && K.defocus_search_range != 0.0f) defocus_i -= ll;
+      }
+    }
+    if (block == 0) {
+      current_phi = L.get(best_address, M_PHI);
+      current_theta = L.get(best_address, M_THETA);
+      current_psi = L.get(best_address, M_PSI);
+      current_defocus = L.get(best_address, M_DEFOCUS);
+      current_pixel_size = L.get(best_address, M_PIXEL_SIZE);
+      phi_i = theta_i = psi_i = 0;
+    }
+  }
+  tm_refined_peak& r = st.rec;
+  r.x = st.x;
+  r.y = st.y;
+  r.value = st.value;
+  r.psi = L.base(current_address, M_PSI);
+  r.theta = L.base(current_address, M_THETA);
+  r.phi = L.base(current_address, M_PHI);
+  r.defocus = L.base(current_address, M_DEFOCUS);
+  r.pixel_size = L.base(current_address, M_PIXEL_SIZE);
+  r.x_pos = st.x + best_peak.x;
+  r.y_pos = st.y + best_peak.y;
+  r.shift_x = best_peak.x;
+  r.shift_y = best_peak.y;
+  r.r_psi = L.get(best_address, M_PSI);
+  r.r_theta = L.get(best_address, M_THETA);
+  r.r_phi = L.get(best_address, M_PHI);
+  r.r_defocus = L.get(best_address, M_DEFOCUS);
+  r.r_pixel_size = L.get(best_address, M_PIXEL_SIZE);
+  r.peak_height = score_adjustment * scaled_here * best_score / mip_here * K.sqrt_t;
+  r.mip = score_adjustment * best_score * K.sqrt_t;
+  r.best_score = best_score;
+  r.starting_score = starting_score;
+  r.offset = offset_distance;
+  r.best_address = best_address;
+  r.evaluations = st.evaluations;
+}
+
+template <class T>
+struct PinnedBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  void alloc(size_t count) {
+    release();
+    if (count == 0) return;
+    TM_HIP(hipHostMalloc((void**)&p, count * sizeof(T), hipHostMallocDefault));
+    n = count;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~PinnedBuf() { release(); }
+};
+
+}  // namespace
+
+struct tm_refine {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool initialised = false;
+  tm_refine_params prm{};
+  int t = 0, th = 0, P = 0, Ph = 0, K = 0;  // K: max_in_flight
+  int n_curve = 0;
+  rtk::CtfConst ctf{};
+  DevBuf<float> img, curve_x, curve_y, mats, dfs, results, smip, win_real, arg_v;
+  DevBuf<cf> tpl, phaseP, phaseT, slice, proj, win, win_stage;
+  DevBuf<double> partial;
+  DevBuf<int> slots;
+  DevBuf<long long> shifts, arg_i;
+  PinnedBuf<float> h_mats, h_dfs, h_results, h_arg_v;
+  PinnedBuf<int> h_slots;
+  PinnedBuf<long long> h_shifts, h_arg_i;
+  int score_parts = 1;
+  std::map<int, cf*> tw;
+
+  ~tm_refine() {
+    for (auto& kv : tw) (void)hipFree(kv.second);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  const cf* twiddle(int n) {
+    auto it = tw.find(n);
+    if (it != tw.end()) return it->second;
+    auto host = twiddles(n);
+    cf* d = nullptr;
+    TM_HIP(hipMalloc((void**)&d, sizeof(cf) * n));
+    TM_HIP(hipMemcpy(d, host.data(), sizeof(cf) * n, hipMemcpyHostToDevice));
+    tw[n] = d;
+    return d;
+  }
+};
+
+namespace {
+
+template <class F>
+int rguard(F&& f) {
+  try {
+    f();
+    return TM_OK;
+  } catch (const TmError& e) {
+    g_rerr = e.msg;
+    return e.code;
+  } catch (const std::exception& e) {
+    g_rerr = e.what();
+    return TM_ERR_INVALID;
+  }
+}
+
+bool template_size_compiled(int n) {
+  switch (n) {
+#define TM_CASE(k) case k:
+    TM_TEMPLATE_SIZES(TM_CASE)
+#undef TM_CASE
+    return true;
+    default: return false;
+  }
+}
+
+void require_ready(tm_refine* h) {
+  TM_REQUIRE(h, TM_ERR_INVALID, "handle is NULL");
+  TM_REQUIRE(h->initialised, TM_ERR_STATE, "tm_refine_init has not succeeded on this handle");
+  TM_HIP(hipSetDevice(h->device));
+}
+
+// RealSpaceIntegerShift's starting index (image.cpp:9072-9074), with its float remainder.
+long long shift_start(int mx, int my, int sx, int sy) {
+  const long long n = (long long)mx * my;
+  long long s = -(long long)sx - (long long)mx * sy;
+  s = (long long)remainderf(float(s), float(n));
+  if (s < 0) s += n;
+  return s;
+}
+
+// Particle windows for the peaks at (xs[q], ys[q]) into window slots slot_of[q].
+void make_windows(tm_refine* h, const std::vector<int>& xs, const std::vector<int>& ys, const std::vector<int>& slot_of) {
+  const int n = (int)xs.size();
+  if (n == 0) return;
+  const int t = h->t, th = h->th;
+  for (int q0 = 0; q0 < n; q0 += h->K) {
+    const int m = std::min(h->K, n - q0);
+    for (int q = 0; q < m; ++q) {
+      h->h_shifts.p[q] = shift_start(h->prm.image_x, h->prm.image_y, xs[q0 + q], ys[q0 + q]);
+      h->h_slots.p[q] = slot_of[q0 + q];
+    }
+    TM_HIP(hipMemcpyAsync(h->shifts.p, h->h_shifts.p, m * sizeof(long long), hipMemcpyHostToDevice, h->stream));
+    TM_HIP(hipMemcpyAsync(h->slots.p, h->h_slots.p, m * sizeof(int), hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(rtk::k_window_gather, dim3(t, m), dim3(256), 0, h->stream, h->img.p, h->prm.image_x,
+                       h->prm.image_y, t, h->shifts.p, h->win_real.p);
+    check_launch("k_window_gather");
+    // ForwardFFT (scaled 1/N) of each window
+    const float scale = (float)(1. / (double)(float)((long long)t * t));
+    rows_r2c(h, t, h->win_real.p, (size_t)t * (t + 2), t + 2, 0, 0, t, 0, h->win_stage.p, (size_t)t * th, th, t, m, scale);
+    cols(h, t, h->win_stage.p, th, (size_t)t * th, m, -1, 0, nullptr, 1.0f);
+    hipLaunchKernelGGL(rtk::k_window_phase, dim3(std::max(1, (t * th + 255) / 256), m), dim3(256), 0, h->stream,
+                       h->win_stage.p, h->phaseT.p, t * th, h->slots.p, h->win.p);
+    check_launch("k_window_phase");
+    // the staging buffers are reused by the next chunk only after these launches (same stream)
+    TM_HIP(hipStreamSynchronize(h->stream));
+  }
+}
+
+// One batch of TemplateScore calls: item b correlates the projection at reqs[b] with window slots[b].
+void run_scores(tm_refine* h, const std::vector<ScoreRequest>& reqs, const std::vector<int>& slot_of,
+                std::vector<ScoreResult>& out) {
+  const int n = (int)reqs.size();
+  const int t = h->t, th = h->th, P = h->P, Ph = h->Ph;
+  TM_REQUIRE(n <= h->K, TM_ERR_INVALID, "batch larger than max_in_flight");
+  for (int b = 0; b < n; ++b) {
+    euler_matrix(reqs[b].phi, reqs[b].theta, reqs[b].psi, &h->h_mats.p[9 * b]);
+    h->h_dfs.p[2 * b] = reqs[b].df1;
+    h->h_dfs.p[2 * b + 1] = reqs[b].df2;
+    h->h_slots.p[b] = slot_of[b];
+  }
+  TM_HIP(hipMemcpyAsync(h->mats.p, h->h_mats.p, 9 * n * sizeof(float), hipMemcpyHostToDevice, h->stream));
+  TM_HIP(hipMemcpyAsync(h->dfs.p, h->h_dfs.p, 2 * n * sizeof(float), hipMemcpyHostToDevice, h->stream));
+  TM_HIP(hipMemcpyAsync(h->slots.p, h->h_slots.p, n * sizeof(int), hipMemcpyHostToDevice, h->stream));
+  const size_t proj_bs = (size_t)t * th;
+  const float scale = (float)(1. / (double)(float)((long long)t * t));
+  if (P != t) {
+    // ExtractSlice, SwapRealSpaceQuadrants, BackwardFFT (columns, then the t centre rows that
+    // ClipInto keeps), ClipInto t^2 + ForwardFFT (rows, scaled 1/N; columns below)
+    const size_t slice_bs = (size_t)P * Ph;
+    extract_cols(h, P, h->tpl.p, h->mats.p, 0, n, h->phaseP.p, nullptr, h->slice.p);
+    const int r0 = P / 2 - t / 2;
+    rows_c2r(h, P, h->slice.p + (size_t)r0 * Ph, slice_bs, Ph, reinterpret_cast<float*>(h->slice.p) + (size_t)r0 * (P + 2),
+             2 * slice_bs, P + 2, t, n);
+    rows_r2c(h, t, reinterpret_cast<const float*>(h->slice.p), 2 * slice_bs, P + 2, r0, r0, t, 0, h->proj.p, proj_bs, th, t,
+             n, scale);
+  } else {
+    // ExtractSlice, SwapRealSpaceQuadrants, BackwardFFT, ForwardFFT (rows in place)
+    extract_cols(h, t, h->tpl.p, h->mats.p, 0, n, h->phaseT.p, nullptr, h->proj.p);
+    rows_c2r(h, t, h->proj.p, proj_bs, th, reinterpret_cast<float*>(h->proj.p), 2 * proj_bs, t + 2, t, n);
+    rows_r2c(h, t, reinterpret_cast<const float*>(h->proj.p), 2 * proj_bs, t + 2, 0, 0, t, 0, h->proj.p, proj_bs, th, t, n,
+             scale);
+  }
+  cols(h, t, h->proj.p, th, proj_bs, n, -1, 0, nullptr, 1.0f);
+  constexpr int NT = 256;
+  hipLaunchKernelGGL((rtk::k_score_product<NT>), dim3(h->score_parts, n), dim3(NT), 0, h->stream, h->proj.p, t, h->win.p,
+                     h->slots.p, h->dfs.p, h->ctf, h->curve_x.p, h->curve_y.p, h->n_curve, h->partial.p);
+  check_launch("k_score_product");
+  cols(h, t, h->proj.p, th, proj_bs, n, +1, 0, nullptr, 1.0f);
+  rows_c2r(h, t, h->proj.p, proj_bs, th, reinterpret_cast<float*>(h->proj.p), 2 * proj_bs, t + 2, t, n);
+  hipLaunchKernelGGL((rtk::k_score_peak<NT>), dim3(n), dim3(NT), 0, h->stream, reinterpret_cast<const float*>(h->proj.p), t,
+                     h->partial.p, h->score_parts, h->results.p);
+  check_launch("k_score_peak");
+  TM_HIP(hipMemcpyAsync(h->h_results.p, h->results.p, 3 * n * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  TM_HIP(hipStreamSynchronize(h->stream));
+  out.resize(n);
+  for (int b = 0; b < n; ++b) out[b] = ScoreResult{h->h_results.p[3 * b], h->h_results.p[3 * b + 1], h->h_results.p[3 * b + 2]};
+}
+
+struct FoundPeak {
+  float x, y, value;
+};
+
+// The peak list (refine_template.cpp:561-600) on the device copy of the scaled MIP (consumed).
+std::vector<FoundPeak> peak_list(tm_refine* h, long long cap) {
+  const int mx = h->prm.image_x, my = h->prm.image_y;
+  constexpr int NT = 256, NB = 1024;
+  const float radius = h->prm.min_peak_radius;
+  std::vector<FoundPeak> out;
+  while ((long long)out.size() < cap) {
+    hipLaunchKernelGGL((rtk::k_map_argmax<NT>), dim3(NB), dim3(NT), 0, h->stream, h->smip.p, mx, my, 50, h->arg_v.p,
+                       h->arg_i.p);
+    check_launch("k_map_argmax");
+    hipLaunchKernelGGL((rtk::k_argmax_final<NT>), dim3(1), dim3(NT), 0, h->stream, h->arg_v.p, h->arg_i.p, NB,
+                       h->arg_v.p + NB, h->arg_i.p + NB);
+    check_launch("k_argmax_final");
+    TM_HIP(hipMemcpyAsync(h->h_arg_v.p, h->arg_v.p + NB, sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    TM_HIP(hipMemcpyAsync(h->h_arg_i.p, h->arg_i.p + NB, sizeof(long long), hipMemcpyDeviceToHost, h->stream));
+    TM_HIP(hipStreamSynchronize(h->stream));
+    const float v = h->h_arg_v.p[0];
+    const long long e = h->h_arg_i.p[0];
+    FoundPeak pk{0.0f, 0.0f, v};
+    if (e != LLONG_MAX) {
+      pk.x = (float)((int)(e % mx) - mx / 2);
+      pk.y = (float)((int)(e / mx) - my / 2);
+    }
+    if (pk.value < h->prm.wanted_threshold) break;
+    out.push_back(pk);
+    const float cx = pk.x + (float)(mx / 2), cy = pk.y + (float)(my / 2);
+    if (radius >= 0.0f) {
+      const double rr = std::sqrt((double)radius) + 2.0;
+      const int x0 = std::max(0, (int)std::floor(cx - rr)), x1 = std::min(mx - 1, (int)std::ceil(cx + rr));
+      const int y0 = std::max(0, (int)std::floor(cy - rr)), y1 = std::min(my - 1, (int)std::ceil(cy + rr));
+      const int w = x1 - x0 + 1, hg = y1 - y0 + 1;
+      if (w > 0 && hg > 0) {
+        const long long cnt = (long long)w * hg;
+        hipLaunchKernelGGL(rtk::k_map_mask, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, h->stream, h->smip.p, mx,
+                           x0, y0, w, hg, cx, cy, radius);
+        check_launch("k_map_mask");
+      }
+    }
+  }
+  return out;
+}
+
+void upload_scaled_mip(tm_refine* h, const float* scaled_mip) {
+  const size_t n = (size_t)h->prm.image_x * h->prm.image_y;
+  TM_HIP(hipMemcpyAsync(h->smip.p, scaled_mip, n * sizeof(float), hipMemcpyHostToDevice, h->stream));
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+const char* tm_refine_last_error(void) { return g_rerr.c_str(); }
+
+int tm_refine_create(int device_ordinal, tm_refine** out) {
+  return rguard([&] {
+    TM_REQUIRE(out, TM_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    int n = 0;
+    TM_HIP(hipGetDeviceCount(&n));
+    TM_REQUIRE(device_ordinal >= 0 && device_ordinal < n, TM_ERR_INVALID,
+               "device ordinal " + std::to_string(device_ordinal) + " out of range (" + std::to_string(n) + " devices)");
+    TM_HIP(hipSetDevice(device_ordinal));
+    auto* h = new tm_refine();
+    h->device = device_ordinal;
+    hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete h;
+      throw TmError{TM_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)};
+    }
+    *out = h;
+  });
+}
+
+int tm_refine_destroy(tm_refine* h) {
+  return rguard([&] {
+    if (!h) return;
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    delete h;
+  });
+}
+
+int tm_refine_init(tm_refine* h, const tm_refine_params* p, const float* whitened_image, const float* template_spectrum,
+                   const float* curve_x, const float* curve_y, int n_curve) {
+  return rguard([&] {
+    TM_REQUIRE(h && p && whitened_image && template_spectrum && curve_x && curve_y, TM_ERR_INVALID, "NULL argument");
+    TM_HIP(hipSetDevice(h->device));
+    h->initialised = false;
+    const int mx = p->image_x, my = p->image_y, t = p->template_size, P = p->padded_size;
+    TM_REQUIRE(mx > 0 && my > 0 && mx % 2 == 0, TM_ERR_INVALID, "image_x must be positive and even, image_y positive");
+    TM_REQUIRE(template_size_compiled(t) && template_size_compiled(P) && P >= t, TM_ERR_UNSUPPORTED,
+               "template size " + std::to_string(t) + " / padded size " + std::to_string(P) +
+                   " not supported (powers of two 32..1024, padded >= template)");
+    TM_REQUIRE(n_curve >= 2, TM_ERR_INVALID, "the whitening curve needs at least two points");
+    TM_REQUIRE(p->pixel_size > 0.0f, TM_ERR_INVALID, "pixel_size must be positive");
+    TM_REQUIRE(p->max_in_flight >= 0, TM_ERR_INVALID, "max_in_flight must be >= 0");
+    h->prm = *p;
+    h->t = t;
+    h->th = t / 2 + 1;
+    h->P = P;
+    h->Ph = P / 2 + 1;
+    h->K = p->shared_local_maps ? 1 : (p->max_in_flight > 0 ? p->max_in_flight : 256);
+    h->n_curve = n_curve;
+    // CTF::Init (ctf.cpp:93-150) in pixel units
+    const double PI = 3.14159265359;
+    const float wl =
+        12.2639f / sqrtf(1000.0f * p->voltage_kV + 0.97845e-6 * powf(1000.0f * p->voltage_kV, 2)) / p->pixel_size;
+    h->ctf.wavelength = wl;
+    h->ctf.squared_wavelength = powf(wl, 2);
+    h->ctf.spherical_aberration = p->spherical_aberration_mm * 10000000.0 / p->pixel_size;
+    h->ctf.phase_shift = (float)(p->phase_shift_deg * 3.14159265359f / 180.);
+    h->ctf.amplitude_term = (fabs(p->amplitude_contrast - 1.0) < 1e-3)
+                                ? (float)(PI / 2.0f)
+                                : atanf(p->amplitude_contrast / sqrtf(1.0 - powf(p->amplitude_contrast, 2)));
+    h->ctf.astigmatism_azimuth = (float)(p->defocus_angle * 3.14159265359f / 180.);
+    const size_t npx = (size_t)mx * my;
+    h->img.alloc(npx);
+    TM_HIP(hipMemcpy(h->img.p, whitened_image, npx * sizeof(float), hipMemcpyHostToDevice));
+    h->smip.alloc(npx);
+    h->tpl.alloc((size_t)P * P * h->Ph);
+    TM_HIP(hipMemcpy(h->tpl.p, template_spectrum, h->tpl.n * sizeof(cf), hipMemcpyHostToDevice));
+    h->curve_x.alloc(n_curve);
+    h->curve_y.alloc(n_curve);
+    TM_HIP(hipMemcpy(h->curve_x.p, curve_x, n_curve * sizeof(float), hipMemcpyHostToDevice));
+    TM_HIP(hipMemcpy(h->curve_y.p, curve_y, n_curve * sizeof(float), hipMemcpyHostToDevice));
+    auto pp = swap_phase(P);
+    h->phaseP.alloc(pp.size());
+    TM_HIP(hipMemcpy(h->phaseP.p, pp.data(), pp.size() * sizeof(cf), hipMemcpyHostToDevice));
+    auto pt = swap_phase(t);
+    h->phaseT.alloc(pt.size());
+    TM_HIP(hipMemcpy(h->phaseT.p, pt.data(), pt.size() * sizeof(cf), hipMemcpyHostToDevice));
+    const int K = h->K;
+    const size_t per = (size_t)t * h->th;
+    h->score_parts = (int)std::max<size_t>(1, std::min<size_t>(32, per / 2048));
+    h->mats.alloc(9 * (size_t)K);
+    h->dfs.alloc(2 * (size_t)K);
+    h->results.alloc(3 * (size_t)K);
+    h->slots.alloc(K);
+    h->shifts.alloc(K);
+    h->partial.alloc((size_t)K * h->score_parts);
+    if (P != t) h->slice.alloc((size_t)K * P * h->Ph);
+    h->proj.alloc((size_t)K * per);
+    h->win.alloc((size_t)K * per);
+    h->win_stage.alloc((size_t)K * per);
+    h->win_real.alloc((size_t)K * t * (t + 2));
+    h->arg_v.alloc(1024 + 1);
+    h->arg_i.alloc(1024 + 1);
+    h->h_mats.alloc(9 * (size_t)K);
+    h->h_dfs.alloc(2 * (size_t)K);
+    h->h_results.alloc(3 * (size_t)K);
+    h->h_slots.alloc(K);
+    h->h_shifts.alloc(K);
+    h->h_arg_v.alloc(1);
+    h->h_arg_i.alloc(1);
+    for (int n : {t, t / 2, P, P / 2}) h->twiddle(n);
+    TM_HIP(hipDeviceSynchronize());
+    h->initialised = true;
+  });
+}
+
+int tm_refine_peak_list(tm_refine* h, const float* scaled_mip, float* out, int capacity, int* n_peaks) {
+  return rguard([&] {
+    require_ready(h);
+    TM_REQUIRE(scaled_mip && n_peaks && (out || capacity == 0), TM_ERR_INVALID, "NULL argument");
+    upload_scaled_mip(h, scaled_mip);
+    auto pk = peak_list(h, (long long)h->prm.image_x * h->prm.image_y / 100);
+    for (int q = 0; q < (int)pk.size() && q < capacity; ++q) {
+      out[3 * q] = pk[q].x;
+      out[3 * q + 1] = pk[q].y;
+      out[3 * q + 2] = pk[q].value;
+    }
+    *n_peaks = (int)pk.size();
+  });
+}
+
+int tm_refine_score(tm_refine* h, int peak_x, int peak_y, float current_defocus, float step, float phi, float theta,
+                    float psi, float* out) {
+  return rguard([&] {
+    require_ready(h);
+    TM_REQUIRE(out, TM_ERR_INVALID, "out is NULL");
+    make_windows(h, {peak_x}, {peak_y}, {0});
+    const tm_refine_params& p = h->prm;
+    std::vector<ScoreResult> res;
+    run_scores(h, {ScoreRequest{phi, theta, psi, (p.defocus1 + current_defocus + step) / p.pixel_size,
+                                (p.defocus2 + current_defocus + step) / p.pixel_size}},
+               {0}, res);
+    out[0] = res[0].x;
+    out[1] = res[0].y;
+    out[2] = res[0].value;
+  });
+}
+
+int tm_refine_run(tm_refine* h, const float* const* maps, float* const* out_maps, tm_refined_peak* peaks, int capacity,
+                  int* n_peaks) {
+  return rguard([&] {
+    require_ready(h);
+    TM_REQUIRE(maps && n_peaks && (peaks || capacity == 0), TM_ERR_INVALID, "NULL argument");
+    for (int k = 0; k < 7; ++k) TM_REQUIRE(maps[k], TM_ERR_INVALID, "NULL map");
+    const tm_refine_params& p = h->prm;
+    const int mx = p.image_x, my = p.image_y;
+    const size_t npx = (size_t)mx * my;
+    for (size_t i = 0; i < npx; ++i)
+      TM_REQUIRE(maps[6][i] == 0.0f, TM_ERR_UNSUPPORTED,
+                 "the pixel-size map must be zero (pixel-size refinement, ChangePixelSize != 1, is not supported)");
+    RunConst K;
+    K.p = p;
+    K.mip = maps[0];
+    K.scaled_mip = maps[1];
+    K.defocus_search_range = p.defocus_search_range;
+    K.defocus_refine_step = 2.0f * p.defocus_search_step;
+    if (K.defocus_refine_step <= 0.0) {
+      K.defocus_search_range = 0.0f;
+      K.defocus_refine_step = 100.0f;
+    }
+    K.defocus_step = std::max(K.defocus_refine_step, 100.0f);
+    K.pixel_size_refine_step = 100.0f;
+    K.sqrt_t = sqrtf(float(h->t) * float(h->t));
+    K.rs = mx + 2;
+    upload_scaled_mip(h, maps[1]);
+    const auto found = peak_list(h, (long long)npx / 100);
+    const int npk = (int)found.size();
+    std::vector<PeakState> states(npk);
+    std::vector<std::unique_ptr<LocalMaps>> own;
+    LocalMaps shared;
+    auto init_maps = [&](LocalMaps& L) {
+      L.in[M_PSI] = maps[2];
+      L.in[M_THETA] = maps[3];
+      L.in[M_PHI] = maps[4];
+      L.in[M_DEFOCUS] = maps[5];
+      L.in[M_PIXEL_SIZE] = maps[6];
+      L.mx = mx;
+      L.my = my;
+    };
+    init_maps(shared);
+    // up to K peaks in flight; slot s holds the window of the peak in flight there
+    struct Flight {
+      int peak = -1;
+      std::unique_ptr<DescentTask> task;
+    };
+    std::vector<Flight> slot(h->K);
+    int next = 0, active = 0;
+    std::vector<ScoreRequest> reqs;
+    std::vector<int> req_slot;
+    std::vector<ScoreResult> res;
+    while (true) {
+      // admit peaks into free slots, windows first, then run each to its first TemplateScore call
+      std::vector<int> xs, ys, ws;
+      std::vector<int> admitted;
+      for (int s = 0; s < h->K && next < npk; ++s) {
+        if (slot[s].peak >= 0) continue;
+        PeakState& st = states[next];
+        st.x = found[next].x;
+        st.y = found[next].y;
+        st.value = found[next].value;
+        if (p.shared_local_maps) {
+          st.maps = &shared;
+        } else {
+          own.emplace_back(new LocalMaps());
+          init_maps(*own.back());
+          st.maps = own.back().get();
+        }
+        slot[s].peak = next;
+        xs.push_back((int)st.x);
+        ys.push_back((int)st.y);
+        ws.push_back(s);
+        admitted.push_back(s);
+        ++next;
+        ++active;
+      }
+      make_windows(h, xs, ys, ws);
+      for (int s : admitted) {
+        slot[s].task.reset(new DescentTask(refine_peak(K, states[slot[s].peak])));
+        slot[s].task->h.resume();
+      }
+      // retire finished peaks (their records are complete)
+      bool retired = false;
+      for (int s = 0; s < h->K; ++s) {
+        if (slot[s].peak < 0 || !slot[s].task->h.done()) continue;
+        if (slot[s].task->h.promise().error) std::rethrow_exception(slot[s].task->h.promise().error);
+        if (!p.shared_local_maps) own[slot[s].peak].reset();
+        slot[s].task.reset();
+        slot[s].peak = -1;
+        --active;
+        retired = true;
+      }
+      if (retired && next < npk) continue;
+      if (active == 0) break;
+      reqs.clear();
+      req_slot.clear();
+      for (int s = 0; s < h->K; ++s) {
+        if (slot[s].peak < 0) continue;
+        reqs.push_back(slot[s].task->h.promise().req);
+        req_slot.push_back(s);
+      }
+      run_scores(h, reqs, req_slot, res);
+      for (size_t q = 0; q < reqs.size(); ++q) {
+        auto& c = slot[req_slot[q]].task->h;
+        c.promise().res = res[q];
+        c.resume();
+      }
+    }
+    // output maps (refine_template.cpp:641-648, 1093-1113), peaks applied in peak-list order
+    if (out_maps) {
+      for (int k = 0; k < 7; ++k) {
+        if (!out_maps[k]) continue;
+        if (k < 2) std::fill(out_maps[k], out_maps[k] + npx, 0.0f);
+        else std::memcpy(out_maps[k], maps[k], npx * sizeof(float));
+      }
+      for (int q = 0; q < npk; ++q) {
+        const tm_refined_peak& r = states[q].rec;
+        if (!(r.offset * p.pixel_size <= p.xy_change_threshold || !p.exclude_above_xy_threshold)) continue;
+        const long long row = r.best_address / K.rs, col = r.best_address % K.rs;
+        if (col >= mx) continue;
+        const size_t i = (size_t)row * mx + col;
+        const float v[7] = {r.mip, r.peak_height, r.r_psi, r.r_theta, r.r_phi, r.r_defocus, r.r_pixel_size};
+        for (int k = 0; k < 7; ++k)
+          if (out_maps[k]) out_maps[k][i] = v[k];
+      }
+    }
+    for (int q = 0; q < npk && q < capacity; ++q) peaks[q] = states[q].rec;
+    *n_peaks = npk;
+  });
+}
+
+}  // extern "C"

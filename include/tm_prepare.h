@@ -40,6 +40,14 @@ int tm_search_grid(char symmetry_type, int symmetry_number, float angular_step, 
 /* In-plane angles of the inclusive float loop psi_start..psi_max (match_template.cpp:1001). */
 int tm_psi_list(float psi_start, float psi_max, float psi_step, float* out, int capacity);
 
+/* refine_template's input image: micrograph (image_x x image_y, x fastest, image_x even) -> outliers
+ * replaced, whitened, normalised to unit spectral energy and transformed back to real space
+ * (whitened, image_x x image_y dense), plus the whitening curve (*n_curve points into curve_x/curve_y).
+ * refine_template.cpp:476-510. With whitened == NULL only *n_curve is set. The image sides must be
+ * sizes with compiled transforms (TM_LINE_SIZES in tm_sizes.hpp), else TM_ERR_UNSUPPORTED. */
+int tm_prepare_refine_image(int device, const float* micrograph, int image_x, int image_y, float* whitened,
+                            float* curve_x, float* curve_y, int* n_curve);
+
 const char* tm_prepare_last_error(void);
 
 #ifdef __cplusplus

@@ -1,0 +1,101 @@
+/*
+ * C ABI of the MI355X (gfx950) local refinement of template-matching peaks.
+ *
+ * It replaces the per-image work of the reference program refine_template
+ * (src/programs/refine_template/refine_template.cpp:298-1130): the peak list taken from the scaled
+ * MIP, one particle window per peak, TemplateScore (refine_template.cpp:25-80) and the coordinate
+ * descent over defocus and the three Euler angles. TemplateScore calls of all peaks in flight run
+ * together as one batch of device transforms; the descent logic of each peak runs on the host.
+ *
+ * Scope: pixel-size refinement is not included (pixel_size_search_range 0, and the pixel-size map
+ * must be zero, so TemplateScore's ChangePixelSize has factor 1), and the particle mask is off
+ * (mask_radius 0, the program's default). The input image, template and maps are the program's
+ * inputs after its preparation (tm_prepare_refine_image and tm_prepare_template in tm_prepare.h).
+ *
+ * Every call returns TM_OK (0) or a negative TM_ERR_* code from tm_hip.h; tm_refine_last_error()
+ * describes the last failure on the calling thread. One handle per host thread.
+ */
+#ifndef TM_REFINE_H
+#define TM_REFINE_H
+
+#include "tm_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tm_refine tm_refine;
+
+typedef struct tm_refine_params {
+  int image_x, image_y;         /* micrograph and map size (x even)                                   */
+  int template_size;            /* t: windows and projections are t x t                                */
+  int padded_size;              /* P: edge of the padded template spectrum (t * padding)               */
+  float pixel_size;             /* Angstrom                                                            */
+  float voltage_kV, spherical_aberration_mm, amplitude_contrast;
+  float defocus1, defocus2, defocus_angle;  /* Angstrom, Angstrom, degrees                             */
+  float phase_shift_deg;
+  float angular_step, in_plane_angular_step;       /* descent steps (degrees)                          */
+  float defocus_search_range, defocus_search_step; /* refine step = 2 * defocus_search_step            */
+  float wanted_threshold;       /* peaks of the scaled MIP at or above this value are refined          */
+  float min_peak_radius;        /* compared with the squared pixel distance, as the program does       */
+  float xy_change_threshold;    /* Angstrom; output maps skip peaks that moved further when            */
+  int exclude_above_xy_threshold; /* this flag is set                                                  */
+  int shared_local_maps;        /* 1: the program's per-thread maps carry over from peak to peak, as in
+                                   a run with one thread (peaks are then refined one at a time);
+                                   0: every peak starts from the input maps, as a thread that handles
+                                   one peak does (peaks are refined together)                          */
+  int max_in_flight;            /* peaks refined together (the TemplateScore batch); 0 = 256           */
+} tm_refine_params;
+
+/* One refined peak (TemplateMatchFoundPeakInfo of all_peak_infos / all_peak_changes,
+ * refine_template.cpp:1091-1113). Positions in pixels relative to the image centre. */
+typedef struct tm_refined_peak {
+  float x, y, value;                           /* found peak and its scaled-MIP value               */
+  float psi, theta, phi, defocus, pixel_size;  /* the search's values at the found peak             */
+  float x_pos, y_pos;                          /* refined position (found + shift)                  */
+  float shift_x, shift_y;                      /* TemplateScore peak offset of the best score       */
+  float r_psi, r_theta, r_phi, r_defocus, r_pixel_size; /* refined values (maps at best_address)    */
+  float peak_height;                           /* scaled_mip * best / mip * t (best_scaled_mip)     */
+  float mip;                                   /* best * t (best_mip)                               */
+  float best_score;                            /* best TemplateScore value                          */
+  float starting_score;                        /* score at the search's values, on the scaled scale */
+  float offset;                                /* distance the peak moved (pixels)                  */
+  long long best_address;                      /* padded-layout address ((x + 2) per row)            */
+  int evaluations;                             /* TemplateScore calls for this peak                 */
+  int reserved;
+} tm_refined_peak;
+
+/* Creates a handle bound to one device. */
+int tm_refine_create(int device_ordinal, tm_refine** out);
+int tm_refine_destroy(tm_refine* h);
+
+/* Copies the refinement inputs to the device. whitened_image: image_x x image_y dense real (the
+ * program's input_image after refine_template.cpp:498-510); template_spectrum: P^3 half-complex,
+ * scaled 1/N, DC 0, quadrant-swapped (refine_template.cpp:476-488); curve_x/curve_y: the whitening
+ * curve (n_curve points) that multiplies the CTF in each projection filter. */
+int tm_refine_init(tm_refine* h, const tm_refine_params* params, const float* whitened_image,
+                   const float* template_spectrum, const float* curve_x, const float* curve_y, int n_curve);
+
+/* Refines every peak of the scaled MIP. maps[7]: mip, scaled_mip, psi, theta, phi, defocus,
+ * pixel_size, each image_x x image_y dense (the search's output maps). out_maps[7] (the array or
+ * any entry may be NULL) receive best_mip, best_scaled_mip, best_psi, best_theta, best_phi,
+ * best_defocus, best_pixel_size (dense) as the program writes them. peaks receives up to capacity
+ * records in peak-list order; *n_peaks the number of peaks found. */
+int tm_refine_run(tm_refine* h, const float* const* maps, float* const* out_maps, tm_refined_peak* peaks, int capacity,
+                  int* n_peaks);
+
+/* Test hooks with no counterpart in the reference: the peak list of a scaled MIP (dense;
+ * out receives x, y, value per peak, at most capacity), and TemplateScore for the window at
+ * (peak_x, peak_y) with the filter at defocus1|2 + current_defocus + step (out: x, y, value). */
+int tm_refine_peak_list(tm_refine* h, const float* scaled_mip, float* out, int capacity, int* n_peaks);
+int tm_refine_score(tm_refine* h, int peak_x, int peak_y, float current_defocus, float step, float phi, float theta,
+                    float psi, float* out);
+
+/* Last error message of the calling thread. */
+const char* tm_refine_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TM_REFINE_H */

@@ -6,7 +6,7 @@ import re
 
 import numpy as np
 
-from cistem_downstream_bah_amd import core, mrc, synth
+from cistem_downstream_bah_amd import core, mrc, refine, synth
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -18,6 +18,31 @@ def test_library_exports_every_declared_symbol():
     L = ctypes.CDLL(core.LIB_PATH)
     for name in declared:
         assert hasattr(L, name), name
+
+
+def test_refine_exports_and_structs_match_header():
+    header = open(os.path.join(ROOT, "include", "tm_refine.h")).read()
+    declared = set(re.findall(r"^(?:int|const char\*)\s+(tm_\w+)\(", header, re.M))
+    prep = open(os.path.join(ROOT, "include", "tm_prepare.h")).read()
+    assert "tm_prepare_refine_image" in set(re.findall(r"^(?:int|const char\*)\s+(tm_\w+)\(", prep, re.M))
+    assert declared | {"tm_prepare_refine_image"} == set(refine.REFINE_EXPORTS)
+    L = ctypes.CDLL(core.LIB_PATH)
+    for name in refine.REFINE_EXPORTS:
+        assert hasattr(L, name), name
+    for struct, cls in (("tm_refine_params", refine.RefineParams), ("tm_refined_peak", refine.RefinedPeak)):
+        body = header[header.index("typedef struct " + struct):header.index("} " + struct + ";")]
+        fields = re.findall(r"^\s+(?:int|double|float|long long)\s+([\w, ]+);", body, re.M)
+        names = [f.strip() for group in fields for f in group.split(",")]
+        assert names == [f[0] for f in cls._fields_], struct
+    assert ctypes.sizeof(refine.RefinedPeak) == 104
+
+
+def test_refine_errors_are_returned_not_fatal():
+    L = refine._lib()
+    assert L.tm_refine_destroy(None) == 0
+    n = ctypes.c_int(0)
+    rc = L.tm_refine_run(None, None, None, None, 0, ctypes.byref(n))
+    assert rc != 0 and L.tm_refine_last_error()
 
 
 def test_init_params_struct_matches_header():

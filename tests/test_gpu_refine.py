@@ -1,0 +1,183 @@
+"""GPU: the refine_template path (include/tm_refine.h) against the oracle's restatement.
+
+* preparation (refine_template.cpp:476-510): whitened real-space image within 1e-5 of its maximum,
+  whitening curve within 1e-5 relative
+* TemplateScore (refine_template.cpp:25-80), template 64^3 unpadded (P = t) and padded (P = 2t),
+  several windows, orientations and defocus offsets, with the oracle's inputs: peak position
+  identical, value within 2e-5 relative
+* peak list (refine_template.cpp:561-600): identical to the oracle on a map with border cases,
+  ties and neighbours inside the exclusion radius
+* whole refinement of planted peaks (config 0: 512^2 image, copies started 3 degrees off), with and
+  without the defocus scan, per-peak and shared local maps: every discrete result (peak positions,
+  shifts, refined angles and defocus, best addresses, number of TemplateScore calls) identical to
+  the oracle, scores within 2e-5 relative, output maps equal away from the score maps
+* full size (4096^2 image, template 256^3 padded to 512^3): three planted peaks refine to the
+  oracle's discrete results
+"""
+import numpy as np
+import pytest
+
+from cistem_downstream_bah_amd import refine, synth
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+DISCRETE = ("x", "y", "value", "psi", "theta", "phi", "defocus", "pixel_size", "x_pos", "y_pos", "shift_x",
+            "shift_y", "r_psi", "r_theta", "r_phi", "r_defocus", "r_pixel_size", "best_address", "evaluations",
+            "offset")
+SCORES = ("peak_height", "mip", "best_score", "starting_score")
+
+
+def _planted(s, off=(3.0, -3.0, 3.0)):
+    H, W = s.micrograph.shape
+    maps = {n: np.zeros((H, W), np.float32) for n in O.MAP_NAMES}
+    maps["mip"][:] = 1.0
+    for q in s.particles:
+        ix, iy = int(round(q.x)), int(round(q.y))
+        maps["scaled_mip"][iy, ix] = 10.0
+        maps["phi"][iy, ix] = q.phi + off[0]
+        maps["theta"][iy, ix] = q.theta + off[1]
+        maps["psi"][iy, ix] = q.psi + off[2]
+    return maps
+
+
+def _settings(rp: O.RTParams, **kw):
+    return refine.RefineSettings(pixel_size=rp.pixel_size, voltage_kV=rp.voltage_kV,
+                                 spherical_aberration_mm=rp.spherical_aberration_mm,
+                                 amplitude_contrast=rp.amplitude_contrast, defocus1=rp.defocus1, defocus2=rp.defocus2,
+                                 defocus_angle=rp.defocus_angle, phase_shift_deg=rp.phase_shift_deg,
+                                 angular_step=rp.angular_step, in_plane_angular_step=rp.in_plane_angular_step,
+                                 defocus_search_range=rp.defocus_search_range,
+                                 defocus_search_step=rp.defocus_search_step, padding=rp.padding,
+                                 wanted_threshold=rp.wanted_threshold, min_peak_radius=rp.min_peak_radius, **kw)
+
+
+def _pair(s, rp, **kw):
+    """Oracle and GPU refiner on the same prepared inputs (the oracle's)."""
+    rt = O.RefineTemplateOracle(rp, s.micrograph, s.volume)
+    x, y = rt.whitening_curve
+    g = refine.TemplateRefiner(s.micrograph, s.volume, _settings(rp, **kw), template_spectrum=rt.template_spectrum,
+                               whitened=(rt.whitened_image, x, y))
+    return rt, g
+
+
+@pytest.fixture(scope="module")
+def c0():
+    return synth.config0(0)
+
+
+def test_prepare_refine_image(c0):
+    print("case: prepare_refine_image 512^2", flush=True)
+    rt = O.RefineTemplateOracle(O.RTParams(), c0.micrograph, c0.volume)
+    img, cx, cy = refine.prepare_refine_image(c0.micrograph)
+    ref = rt.whitened_image
+    x, y = rt.whitening_curve
+    assert np.array_equal(cx, x)
+    assert np.abs(cy - y).max() <= 1e-5 * np.abs(y).max()
+    assert np.abs(img - ref).max() <= 1e-5 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("padding", [1.0, 2.0])
+def test_template_score(c0, padding):
+    print(f"case: TemplateScore t=64 padding {padding}", flush=True)
+    rp = O.RTParams(padding=padding)
+    rt, g = _pair(c0, rp)
+    try:
+        H, W = c0.micrograph.shape
+        worst = 0.0
+        for q in c0.particles[:4]:
+            px, py = int(round(q.x)) - W // 2, int(round(q.y)) - H // 2
+            rt.window(px, py)
+            for (dphi, dth, dpsi, cur, step) in [(0, 0, 0, 0.0, 0.0), (2, -1, 3, 0.0, 200.0), (-7, 5, 11, 150.0, -100.0)]:
+                a = (q.phi + dphi, q.theta + dth, q.psi + dpsi)
+                ref = rt.score(*a, current_defocus=cur, step=step)
+                got = g.score(px, py, *a, current_defocus=cur, step=step)
+                assert got[:2] == ref[:2], (q, a, got, ref)
+                worst = max(worst, abs(got[2] - ref[2]) / abs(ref[2]))
+        print(f"TemplateScore max rel value error {worst:.2e}", flush=True)
+        assert worst <= 2e-5
+    finally:
+        g.close()
+
+
+def test_peak_list(c0):
+    print("case: peak list 256 x 192", flush=True)
+    rng = np.random.default_rng(3)
+    img = rng.standard_normal((192, 256)).astype(np.float32)
+    vol = np.zeros((32, 32, 32), np.float32)
+    vol[16, 16, 16] = 1
+    rp = O.RTParams(wanted_threshold=3.0, min_peak_radius=10.0)
+    rt = O.RefineTemplateOracle(rp, img, vol)
+    sm = rng.uniform(0, 2.5, (192, 256)).astype(np.float32)
+    for (y, x, v) in [(100, 120, 9.0), (60, 50, 8.0), (70, 206, 7.5), (30, 128, 20.0), (120, 49, 19.0),
+                      (90, 90, 6.0), (90, 150, 6.0), (101, 122, 8.5), (101, 124, 5.0), (141, 200, 4.0)]:
+        sm[y, x] = v
+    x, y = rt.whitening_curve
+    g = refine.TemplateRefiner(img, vol, _settings(rp), template_spectrum=rt.template_spectrum,
+                               whitened=(rt.whitened_image, x, y))
+    try:
+        got = g.peak_list(sm)
+        ref = rt.peak_list(sm)
+        assert len(ref) >= 6 and np.array_equal(got, ref)
+    finally:
+        g.close()
+
+
+def _compare(got, ref, out_got=None, out_ref=None):
+    assert len(got) == len(ref) and len(ref) > 0
+    worst = 0.0
+    for a, b in zip(got, ref):
+        for k in DISCRETE:
+            assert (a[k] == b[k]) if k != "offset" else abs(a[k] - b[k]) <= 1e-6, (k, a, b)
+        for k in SCORES:
+            worst = max(worst, abs(a[k] - b[k]) / abs(b[k]))
+    print(f"{len(ref)} peaks, discrete results identical, max rel score error {worst:.2e}, "
+          f"TemplateScore calls {sum(p['evaluations'] for p in ref)}", flush=True)
+    assert worst <= 2e-5
+    if out_got is not None:
+        for gname, oname in zip(refine.OUTPUT_NAMES, O.MAP_NAMES):
+            a, b = out_got[gname], out_ref[oname]
+            if oname in ("mip", "scaled_mip"):
+                assert np.array_equal(a != 0, b != 0)
+                nz = b != 0
+                assert np.abs(a[nz] - b[nz]).max() <= 2e-5 * np.abs(b[nz]).max()
+            else:
+                assert np.array_equal(a, b), oname
+
+
+@pytest.mark.parametrize("case", ["angles", "defocus", "padded", "shared"])
+def test_refine_matches_oracle(c0, case):
+    print(f"case: refine config0 planted peaks, {case}", flush=True)
+    kw = dict(angular_step=1.0, in_plane_angular_step=1.0, wanted_threshold=5.0)
+    if case == "defocus":
+        kw.update(defocus_search_range=200.0, defocus_search_step=50.0)
+    if case == "padded":
+        kw.update(padding=2.0, angular_step=1.5, in_plane_angular_step=1.5)
+    rp = O.RTParams(**kw)
+    shared = case == "shared"
+    rt, g = _pair(c0, rp, shared_local_maps=shared)
+    try:
+        maps = _planted(c0)
+        ref, out_ref = rt.run(maps, shared_local_maps=shared)
+        got, out_got = g.run(maps)
+        _compare(got, ref, out_got, out_ref)
+    finally:
+        g.close()
+
+
+@pytest.mark.timeout(400)
+def test_refine_full_size():
+    print("case: refine 4096^2, template 256 padded 512, 3 planted peaks", flush=True)
+    s = synth.config1(1000)
+    rp = O.RTParams(angular_step=1.5, in_plane_angular_step=1.5, padding=2.0, wanted_threshold=5.0)
+    rt, g = _pair(s, rp)
+    try:
+        H, W = s.micrograph.shape
+        keep = [q for q in s.particles if 300 < q.x < W - 300 and 300 < q.y < H - 300][:3]
+        s3 = synth.Synthetic(s.micrograph, s.volume, keep)
+        maps = _planted(s3)
+        ref, _ = rt.run(maps)
+        got, _ = g.run(maps, want_maps=False)
+        _compare(got, ref)
+    finally:
+        g.close()
