@@ -94,7 +94,8 @@ def main():
     ap.add_argument("--small", action="store_true", help="512^2 / 64^3 workload (config 0 shapes)")
     ap.add_argument("--plane-storage", choices=["fp32", "fp16", "bf16"], default="fp32",
                     help="storage of the inter-pass plane (fp32: the reference's precision, the headline)")
-    ap.add_argument("--workload", choices=["config1", "config2"], default="config1",
+    ap.add_argument("--in-flight", type=int, default=0, help="refine: peaks refined together (0 = 256)")
+    ap.add_argument("--workload", choices=["config1", "config2", "refine"], default="config1",
                     help="config2: eight micrographs of mixed sizes (own JSON line, not the headline)")
     ap.add_argument("--concurrency", type=str, default="1,2,4",
                     help="config2: numbers of images searched concurrently (one stream each)")
@@ -125,6 +126,9 @@ def main():
 
     if args.workload == "config2":
         bench_config2(args)
+        return
+    if args.workload == "refine":
+        bench_refine(args)
         return
 
     if args.small:
@@ -281,6 +285,125 @@ def bench_config2(args):
                    "orientations_per_image": [m.total_correlation_positions for m in searches],
                    "defocus_planes_per_image": 1, "batch": args.batch, "best_concurrency": int(best)},
         "runs": runs,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def planted_maps(s, off=(3.0, -3.0, 3.0), margin=60):
+    """Search maps with one peak (scaled MIP 10, MIP 1) at every inserted copy at least `margin`
+    pixels from the edges, its angles the copy's true angles plus `off` degrees."""
+    H, W = s.micrograph.shape
+    names = ("mip", "scaled_mip", "psi", "theta", "phi", "defocus", "pixel_size")
+    maps = {n: np.zeros((H, W), np.float32) for n in names}
+    maps["mip"][:] = 1.0
+    for q in s.particles:
+        ix, iy = int(round(q.x)), int(round(q.y))
+        if margin < ix < W - margin and margin < iy < H - margin:
+            maps["scaled_mip"][iy, ix] = 10.0
+            maps["phi"][iy, ix] = q.phi + off[0]
+            maps["theta"][iy, ix] = q.theta + off[1]
+            maps["psi"][iy, ix] = q.psi + off[2]
+    return maps
+
+
+def bench_refine(args):
+    """refine_template on the 4096^2 workload: every inserted copy is a peak of the scaled MIP,
+    started 3 degrees off its true orientation on each angle, refined with 1.5 deg steps against the
+    256^3 template padded to 512^3. One step = one tm_refine_run over all peaks. Metric: TemplateScore
+    calls per second (peaks per second alongside)."""
+    from cistem_downstream_bah_amd import refine, synth
+
+    t0 = time.time()
+    s = synth.config1(1000)
+    log(f"synthetic config-1 inputs in {time.time() - t0:.1f}s")
+    maps = planted_maps(s)
+    st = refine.RefineSettings(angular_step=1.5, in_plane_angular_step=1.5, padding=2.0, wanted_threshold=5.0,
+                               max_in_flight=args.in_flight)
+    t0 = time.time()
+    g = refine.TemplateRefiner(s.micrograph, s.volume, st)
+    log(f"prepared on GPU in {time.time() - t0:.1f}s")
+    for _ in range(max(1, min(args.warmup, 2))):
+        recs, _ = g.run(maps, want_maps=False)
+    steps = max(1, args.steps)
+    g.set_timing(True)
+    stage = {}
+    calls = peaks = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        recs, _ = g.run(maps, want_maps=False)   # returns after its final stream synchronisation
+        calls += sum(r["evaluations"] for r in recs)
+        peaks += len(recs)
+        for k, (ms, nl, it) in g.timing().items():
+            a = stage.setdefault(k, [0.0, 0, 0])
+            a[0] += ms
+            a[1] += nl
+            a[2] += it
+    elapsed = time.perf_counter() - t0
+    g.set_timing(False)
+    value = calls / elapsed
+    log(f"{peaks} peaks, {calls} TemplateScore calls in {elapsed:.3f}s -> {value:.0f} calls/s")
+    t, P = s.volume.shape[0], g.P
+    Ph = P // 2 + 1
+    # extraction: every slice pixel interpolates 8 complex template samples (64 B) and writes 8 B
+    th = t // 2 + 1
+    per_item = {"extract_slice_cols": P * Ph * (8 * 8 + 8),               # 8 samples read, one written
+                "slice_rows_c2r": t * (Ph * 8 + P * 4),                   # t rows of the slice
+                "projection_r2c": t * t * 4 + t * th * 8 + 2 * t * th * 8,  # crop rows, then columns
+                "score_product": 3 * t * th * 8,                          # projection, window, product
+                "correlation_c2r": 3 * t * th * 8 + t * t * 4,            # columns, then rows
+                "score_peak": t * t * 4}
+    stages = {}
+    for k, (ms, nl, it) in stage.items():
+        avg = ms / max(nl, 1)
+        items = it / max(nl, 1)
+        stages[k] = {"avg_ms": round(avg, 4), "launches": nl, "items_per_launch": round(items, 1),
+                     "share": round(ms / max(sum(v[0] for v in stage.values()), 1e-12), 4),
+                     "gbps": round(per_item[k] * items / (avg / 1e3) / 1e9, 1) if avg > 0 else None}
+    dominant = max(stage, key=lambda k: stage[k][0])
+    d = stages[dominant]
+    algo = per_item[dominant] * d["items_per_launch"]
+    achieved = algo / (d["avg_ms"] / 1e3)
+    K = args.in_flight or 256
+    roofline = {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK, 4), "traffic": read_traffic("refine_config1_4096sq_T256_P512",
+                                                                               dominant, K),
+                "kernel": dominant, "algorithmic_bytes_per_launch": algo,
+                "items_per_launch": d["items_per_launch"]}
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        log("CPU baseline: oracle refine_template on 2 peaks, 1 thread ...")
+        t0 = time.time()
+        rt = O.RefineTemplateOracle(O.RTParams(angular_step=1.5, in_plane_angular_step=1.5, padding=2.0,
+                                               wanted_threshold=5.0), s.micrograph, s.volume)
+        setup = time.time() - t0
+        two = {k: v.copy() for k, v in maps.items()}
+        keep = np.argwhere(two["scaled_mip"] > 0)[:2]
+        two["scaled_mip"][:] = 0
+        for (y, x) in keep:
+            two["scaled_mip"][y, x] = 10.0
+        t0 = time.time()
+        ref, _ = rt.run(two)
+        dt = time.time() - t0
+        n = sum(r["evaluations"] for r in ref)
+        cpu = {"value": round(n / dt, 2), "unit": "TemplateScore/s", "cores": 1, "kind": "port",
+               "sample": f"oracle refine_template of {len(ref)} of the same peaks ({n} TemplateScore calls at "
+                         f"t={t}, P={P}), {dt:.1f}s timed after {setup:.1f}s setup",
+               "gpu_over_cpu": round(value / (n / dt), 1)}
+        log(f"CPU baseline {n / dt:.2f} calls/s")
+    out = {
+        "metric": "refine_template TemplateScore calls/sec @4096^2, 256^3 template padded to 512^3",
+        "value": round(value, 1), "unit": "TemplateScore/s", "n_gpus": 1, "steps": steps,
+        "warmup": max(1, min(args.warmup, 2)), "ms_per_step": round(1e3 * elapsed / steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (seeded Gaussian pseudo-atom template and micrograph; peaks planted at the inserted copies)",
+        "config": {"workload": "refine_config1_4096sq_T256_P512", "image": [s.micrograph.shape[1], s.micrograph.shape[0]],
+                   "template": t, "padded_template": P, "peaks": peaks // steps, "in_flight": K,
+                   "angular_step": 1.5, "in_plane_angular_step": 1.5,
+                   "peaks_per_s": round(peaks / elapsed, 1)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "stages": stages,
     }
     print(json.dumps(out), flush=True)
 

@@ -561,10 +561,41 @@ struct tm_refine {
   PinnedBuf<long long> h_shifts, h_arg_i;
   int score_parts = 1;
   std::map<int, cf*> tw;
+  // timing: event pairs around each stage of a TemplateScore batch, resolved after the batch's
+  // synchronisation
+  enum { S_EXTRACT, S_SLICE_C2R, S_PROJ_R2C, S_PRODUCT, S_CCF_C2R, S_PEAK, S_COUNT };
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  std::vector<std::pair<int, size_t>> ev_marks;
+  double stage_ms[S_COUNT] = {};
+  long long stage_launches[S_COUNT] = {};
+  long long stage_items[S_COUNT] = {};
+  int batch_items = 0;
 
   ~tm_refine() {
     for (auto& kv : tw) (void)hipFree(kv.second);
+    for (auto& e : ev_pool) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
+  }
+  hipEvent_t next_event() {
+    if (ev_used == ev_pool.size()) {
+      hipEvent_t e;
+      TM_HIP(hipEventCreate(&e));
+      ev_pool.push_back(e);
+    }
+    return ev_pool[ev_used++];
+  }
+  void resolve_timing() {
+    for (auto& m : ev_marks) {
+      float ms = 0.f;
+      TM_HIP(hipEventElapsedTime(&ms, ev_pool[m.second], ev_pool[m.second + 1]));
+      stage_ms[m.first] += ms;
+      stage_launches[m.first] += 1;
+      stage_items[m.first] += batch_items;
+    }
+    ev_marks.clear();
+    ev_used = 0;
   }
   const cf* twiddle(int n) {
     auto it = tw.find(n);
@@ -579,6 +610,29 @@ struct tm_refine {
 };
 
 namespace {
+
+const char* kStageNames[tm_refine::S_COUNT] = {"extract_slice_cols", "slice_rows_c2r", "projection_r2c",
+                                               "score_product",      "correlation_c2r", "score_peak"};
+
+// Records an event pair around one stage on the handle's stream when timing is on.
+struct Stage {
+  tm_refine* h;
+  int k;
+  size_t start = 0;
+  Stage(tm_refine* hh, int kk) : h(hh), k(kk) {
+    if (h->timing) {
+      start = h->ev_used;
+      TM_HIP(hipEventRecord(h->next_event(), h->stream));
+    }
+  }
+  ~Stage() {
+    if (h->timing) {
+      hipEvent_t e = nullptr;
+      try { e = h->next_event(); } catch (...) { return; }
+      if (hipEventRecord(e, h->stream) == hipSuccess) h->ev_marks.push_back({k, start});
+    }
+  }
+};
 
 template <class F>
 int rguard(F&& f) {
@@ -664,35 +718,64 @@ void run_scores(tm_refine* h, const std::vector<ScoreRequest>& reqs, const std::
   TM_HIP(hipMemcpyAsync(h->slots.p, h->h_slots.p, n * sizeof(int), hipMemcpyHostToDevice, h->stream));
   const size_t proj_bs = (size_t)t * th;
   const float scale = (float)(1. / (double)(float)((long long)t * t));
+  using R = tm_refine;
   if (P != t) {
     // ExtractSlice, SwapRealSpaceQuadrants, BackwardFFT (columns, then the t centre rows that
     // ClipInto keeps), ClipInto t^2 + ForwardFFT (rows, scaled 1/N; columns below)
     const size_t slice_bs = (size_t)P * Ph;
-    extract_cols(h, P, h->tpl.p, h->mats.p, 0, n, h->phaseP.p, nullptr, h->slice.p);
     const int r0 = P / 2 - t / 2;
-    rows_c2r(h, P, h->slice.p + (size_t)r0 * Ph, slice_bs, Ph, reinterpret_cast<float*>(h->slice.p) + (size_t)r0 * (P + 2),
-             2 * slice_bs, P + 2, t, n);
+    {
+      Stage st(h, R::S_EXTRACT);
+      extract_cols(h, P, h->tpl.p, h->mats.p, 0, n, h->phaseP.p, nullptr, h->slice.p);
+    }
+    {
+      Stage st(h, R::S_SLICE_C2R);
+      rows_c2r(h, P, h->slice.p + (size_t)r0 * Ph, slice_bs, Ph,
+               reinterpret_cast<float*>(h->slice.p) + (size_t)r0 * (P + 2), 2 * slice_bs, P + 2, t, n);
+    }
+    Stage st(h, R::S_PROJ_R2C);
     rows_r2c(h, t, reinterpret_cast<const float*>(h->slice.p), 2 * slice_bs, P + 2, r0, r0, t, 0, h->proj.p, proj_bs, th, t,
              n, scale);
+    cols(h, t, h->proj.p, th, proj_bs, n, -1, 0, nullptr, 1.0f);
   } else {
     // ExtractSlice, SwapRealSpaceQuadrants, BackwardFFT, ForwardFFT (rows in place)
-    extract_cols(h, t, h->tpl.p, h->mats.p, 0, n, h->phaseT.p, nullptr, h->proj.p);
-    rows_c2r(h, t, h->proj.p, proj_bs, th, reinterpret_cast<float*>(h->proj.p), 2 * proj_bs, t + 2, t, n);
+    {
+      Stage st(h, R::S_EXTRACT);
+      extract_cols(h, t, h->tpl.p, h->mats.p, 0, n, h->phaseT.p, nullptr, h->proj.p);
+    }
+    {
+      Stage st(h, R::S_SLICE_C2R);
+      rows_c2r(h, t, h->proj.p, proj_bs, th, reinterpret_cast<float*>(h->proj.p), 2 * proj_bs, t + 2, t, n);
+    }
+    Stage st(h, R::S_PROJ_R2C);
     rows_r2c(h, t, reinterpret_cast<const float*>(h->proj.p), 2 * proj_bs, t + 2, 0, 0, t, 0, h->proj.p, proj_bs, th, t, n,
              scale);
+    cols(h, t, h->proj.p, th, proj_bs, n, -1, 0, nullptr, 1.0f);
   }
-  cols(h, t, h->proj.p, th, proj_bs, n, -1, 0, nullptr, 1.0f);
   constexpr int NT = 256;
-  hipLaunchKernelGGL((rtk::k_score_product<NT>), dim3(h->score_parts, n), dim3(NT), 0, h->stream, h->proj.p, t, h->win.p,
-                     h->slots.p, h->dfs.p, h->ctf, h->curve_x.p, h->curve_y.p, h->n_curve, h->partial.p);
-  check_launch("k_score_product");
-  cols(h, t, h->proj.p, th, proj_bs, n, +1, 0, nullptr, 1.0f);
-  rows_c2r(h, t, h->proj.p, proj_bs, th, reinterpret_cast<float*>(h->proj.p), 2 * proj_bs, t + 2, t, n);
-  hipLaunchKernelGGL((rtk::k_score_peak<NT>), dim3(n), dim3(NT), 0, h->stream, reinterpret_cast<const float*>(h->proj.p), t,
-                     h->partial.p, h->score_parts, h->results.p);
-  check_launch("k_score_peak");
+  {
+    Stage st(h, R::S_PRODUCT);
+    hipLaunchKernelGGL((rtk::k_score_product<NT>), dim3(h->score_parts, n), dim3(NT), 0, h->stream, h->proj.p, t, h->win.p,
+                       h->slots.p, h->dfs.p, h->ctf, h->curve_x.p, h->curve_y.p, h->n_curve, h->partial.p);
+    check_launch("k_score_product");
+  }
+  {
+    Stage st(h, R::S_CCF_C2R);
+    cols(h, t, h->proj.p, th, proj_bs, n, +1, 0, nullptr, 1.0f);
+    rows_c2r(h, t, h->proj.p, proj_bs, th, reinterpret_cast<float*>(h->proj.p), 2 * proj_bs, t + 2, t, n);
+  }
+  {
+    Stage st(h, R::S_PEAK);
+    hipLaunchKernelGGL((rtk::k_score_peak<NT>), dim3(n), dim3(NT), 0, h->stream, reinterpret_cast<const float*>(h->proj.p),
+                       t, h->partial.p, h->score_parts, h->results.p);
+    check_launch("k_score_peak");
+  }
   TM_HIP(hipMemcpyAsync(h->h_results.p, h->results.p, 3 * n * sizeof(float), hipMemcpyDeviceToHost, h->stream));
   TM_HIP(hipStreamSynchronize(h->stream));
+  if (h->timing) {
+    h->batch_items = n;
+    h->resolve_timing();
+  }
   out.resize(n);
   for (int b = 0; b < n; ++b) out[b] = ScoreResult{h->h_results.p[3 * b], h->h_results.p[3 * b + 1], h->h_results.p[3 * b + 2]};
 }
@@ -894,6 +977,28 @@ int tm_refine_score(tm_refine* h, int peak_x, int peak_y, float current_defocus,
   });
 }
 
+int tm_refine_set_timing(tm_refine* h, int enabled) {
+  return rguard([&] {
+    TM_REQUIRE(h, TM_ERR_INVALID, "handle is NULL");
+    h->timing = enabled != 0;
+  });
+}
+
+int tm_refine_get_timing(tm_refine* h, const char** names, double* ms, long long* launches, long long* items, int cap) {
+  if (!h) {
+    g_rerr = "handle is NULL";
+    return TM_ERR_INVALID;
+  }
+  int n = 0;
+  for (int k = 0; k < tm_refine::S_COUNT && n < cap; ++k, ++n) {
+    if (names) names[n] = kStageNames[k];
+    if (ms) ms[n] = h->stage_ms[k];
+    if (launches) launches[n] = h->stage_launches[k];
+    if (items) items[n] = h->stage_items[k];
+  }
+  return n;
+}
+
 int tm_refine_run(tm_refine* h, const float* const* maps, float* const* out_maps, tm_refined_peak* peaks, int capacity,
                   int* n_peaks) {
   return rguard([&] {
@@ -920,6 +1025,7 @@ int tm_refine_run(tm_refine* h, const float* const* maps, float* const* out_maps
     K.pixel_size_refine_step = 100.0f;
     K.sqrt_t = sqrtf(float(h->t) * float(h->t));
     K.rs = mx + 2;
+    for (int k = 0; k < tm_refine::S_COUNT; ++k) h->stage_ms[k] = 0.0, h->stage_launches[k] = 0, h->stage_items[k] = 0;
     upload_scaled_mip(h, maps[1]);
     const auto found = peak_list(h, (long long)npx / 100);
     const int npk = (int)found.size();

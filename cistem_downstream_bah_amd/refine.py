@@ -23,7 +23,8 @@ MAP_NAMES = ("mip", "scaled_mip", "psi", "theta", "phi", "defocus", "pixel_size"
 OUTPUT_NAMES = ("best_mip", "best_scaled_mip", "best_psi", "best_theta", "best_phi", "best_defocus",
                 "best_pixel_size")
 REFINE_EXPORTS = ["tm_prepare_refine_image", "tm_refine_create", "tm_refine_destroy", "tm_refine_init",
-                  "tm_refine_run", "tm_refine_peak_list", "tm_refine_score", "tm_refine_last_error"]
+                  "tm_refine_run", "tm_refine_peak_list", "tm_refine_score", "tm_refine_set_timing",
+                  "tm_refine_get_timing", "tm_refine_last_error"]
 
 
 class RefineParams(ctypes.Structure):
@@ -62,6 +63,10 @@ def _lib():
                                     ctypes.c_int, ip]
         L.tm_refine_peak_list.argtypes = [vp, F32P, F32P, ctypes.c_int, ip]
         L.tm_refine_score.argtypes = [vp, ctypes.c_int, ctypes.c_int] + [ctypes.c_float] * 5 + [F32P]
+        L.tm_refine_set_timing.argtypes = [vp, ctypes.c_int]
+        i64p = ctypes.POINTER(ctypes.c_longlong)
+        L.tm_refine_get_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_double), i64p,
+                                           i64p, ctypes.c_int]
         L._tm_refine_bound = True
     return L
 
@@ -163,6 +168,19 @@ class TemplateRefiner:
         _check(_lib().tm_refine_score(self.h, int(peak_x), int(peak_y), current_defocus, step, phi, theta, psi,
                                       _f(out)))
         return float(out[0]), float(out[1]), float(out[2])
+
+    def set_timing(self, on: bool) -> None:
+        _check(_lib().tm_refine_set_timing(self.h, int(on)))
+
+    def timing(self) -> dict:
+        """{stage: (total ms, launches, items)} over the last run."""
+        cap = 8
+        names = (ctypes.c_char_p * cap)()
+        ms = (ctypes.c_double * cap)()
+        nl = (ctypes.c_longlong * cap)()
+        it = (ctypes.c_longlong * cap)()
+        n = _lib().tm_refine_get_timing(self.h, names, ms, nl, it, cap)
+        return {names[i].decode(): (ms[i], nl[i], it[i]) for i in range(n)}
 
     def run(self, maps: dict, cap: int = 100000, want_maps: bool = True):
         """Refines every peak. maps: (my, mx) arrays named as MAP_NAMES (the search's outputs).

@@ -91,6 +91,14 @@ int tm_refine_peak_list(tm_refine* h, const float* scaled_mip, float* out, int c
 int tm_refine_score(tm_refine* h, int peak_x, int peak_y, float current_defocus, float step, float phi, float theta,
                     float psi, float* out);
 
+/* Per stage of the TemplateScore batches of the last tm_refine_run (extract_slice_cols,
+ * slice_rows_c2r, projection_r2c, score_product, correlation_c2r, score_peak), measured with HIP
+ * events on the handle's stream when timing is enabled: total ms, number of launches and items
+ * (TemplateScore calls) summed over the launches. Returns the number of entries written (at most cap).
+ * No counterpart in the reference (diagnostics). */
+int tm_refine_set_timing(tm_refine* h, int enabled);
+int tm_refine_get_timing(tm_refine* h, const char** names, double* ms, long long* launches, long long* items, int cap);
+
 /* Last error message of the calling thread. */
 const char* tm_refine_last_error(void);
 
