@@ -136,7 +136,7 @@ void launch_cols(Ctx* h, cf* data, int ncols, size_t bstride, int nb, int sign, 
                  float scale) {
   constexpr int NT = 64;
   const cf* tw = h->twiddle(N);
-  dim3 g(ncols, nb);
+  const dim3 g(ncols * nb);
   if (mode == 1)
     hipLaunchKernelGGL((tmk::k_cols<N, NT, -1, 1>), g, dim3(NT), 0, h->stream, data, ncols, bstride, tw, filt, scale);
   else if (sign < 0)
@@ -202,22 +202,24 @@ void rows_r2c(Ctx* h, int N, const float* src, size_t sbs, int srs, int r0, int 
   }
 }
 
-template <int N, class Ctx>
-void launch_extract_cols(Ctx* h, const cf* tpl, const float* mats, int o0, int nb, const cf* phase, const cf* filt, cf* out) {
+template <int N, int LAYOUT, class Ctx>
+void launch_extract_cols(Ctx* h, const cf* tpl, const float* mats, int o0, int nb, const cf* phase, const cf* filt,
+                         cf* out) {
   constexpr int KC = 8;
   const dim3 g((N / 2 + 1 + KC - 1) / KC, nb);
-  hipLaunchKernelGGL((tmk::k_extract_cols<N, KC>), g, dim3(KC * 64), 0, h->stream, tpl, mats, o0, phase,
-                     filt, h->twiddle(N), out);
+  hipLaunchKernelGGL((tmk::k_extract_cols<N, KC, LAYOUT>), g, dim3(KC * 64), 0, h->stream, tpl, mats, o0, phase, filt,
+                     h->twiddle(N), out);
   check_launch("k_extract_cols");
 }
 
 // Slice extraction and the inverse column FFT of the slice (P != T) or of the filtered projection
-// (P == T), fused.
-template <class Ctx>
-void extract_cols(Ctx* h, int N, const cf* tpl, const float* mats, int o0, int nb, const cf* phase, const cf* filt, cf* out) {
+// (P == T), fused. LAYOUT: the template spectrum's layout (tpl_index in tm_kernels.hpp).
+template <int LAYOUT = 0, class Ctx>
+void extract_cols(Ctx* h, int N, const cf* tpl, const float* mats, int o0, int nb, const cf* phase, const cf* filt,
+                  cf* out) {
   switch (N) {
 #define TM_CASE(n) \
-  case n: launch_extract_cols<n, Ctx>(h, tpl, mats, o0, nb, phase, filt, out); break;
+  case n: launch_extract_cols<n, LAYOUT, Ctx>(h, tpl, mats, o0, nb, phase, filt, out); break;
     TM_TEMPLATE_SIZES(TM_CASE)
 #undef TM_CASE
     default: throw TmError{TM_ERR_UNSUPPORTED, "unsupported slice size " + std::to_string(N)};

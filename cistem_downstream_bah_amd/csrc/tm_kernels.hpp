@@ -73,10 +73,32 @@ __device__ __forceinline__ void fft_inplace(cf* buf, TW tw, int tid) {
   }
 }
 
+// Work item of block bid out of n. Workgroups are dispatched round-robin over the 8 XCDs, so XCD x
+// runs the blocks with bid % 8 == x; it gets the contiguous items starting at x * (n / 8) + min(x, n % 8),
+// as many as it has blocks, so neighbouring items (image rows, columns) share that XCD's L2.
+__device__ __forceinline__ int xcd_row(int bid, int n) {
+  const int x = bid % 8, q = n / 8, r = n % 8;
+  return x * q + min(x, r) + bid / 8;
+}
+
 // ---------------------------------------------------------------- slice extraction
+// Element index of (x, y, z) in the template spectrum (x < P/2+1, y and z physical indices < P).
+// Layout 0: FFTW order [z][y][x]. Layout 1: bricks of 128 bytes holding 4 (x) x 2 (y) x 2 (z) complex
+// values, bricks in [z/2][y/2][x/4] order with the x brick count padded to (P/2+4)/4: a thin slab of
+// any orientation touches fewer cache lines than with rows of consecutive x.
+template <int LAYOUT>
+__device__ __forceinline__ size_t tpl_index(int P, int x, int y, int z) {
+  if constexpr (LAYOUT == 0) {
+    return ((size_t)z * P + y) * (P / 2 + 1) + x;
+  } else {
+    const int nxb = (P / 2 + 4) / 4;
+    return ((((size_t)(z >> 1) * (P / 2) + (y >> 1)) * nxb + (x >> 2)) << 4) + (((z & 1) << 3) | ((y & 1) << 2) | (x & 3));
+  }
+}
+
 // Image::ReturnLinearInterpolatedFourier (image.cpp:2899-3020) on a P^3 FFTW half volume.
+template <int LAYOUT = 0>
 __device__ __forceinline__ cf interp_fourier(const cf* __restrict__ tpl, int P, float x, float y, float z) {
-  const int Ph = P / 2 + 1;
   const int lo = -P / 2, hi_yz = P / 2 - 1, hi_x = P / 2;
   cf sum = cmake(0.f, 0.f);
   const bool pos = x >= 0.0f;
@@ -97,12 +119,11 @@ __device__ __forceinline__ cf interp_fourier(const cf* __restrict__ tpl, int P, 
       const int j = j0 + dj;
       const int py = pos ? (j >= 0 ? j : P + j) : (j > 0 ? P - j : -j);
       const double yw = 1.0 - (double)fabsf(y - (float)j);
-      const cf* row = tpl + ((size_t)pz * P + py) * Ph;
 #pragma unroll
       for (int di = 0; di < 2; ++di) {
         const int i = i0 + di;
         const float w = (float)((1.0 - (double)fabsf(x - (float)i)) * yw * zw);
-        cf val = pos ? row[i] : cconj(row[-i]);
+        cf val = pos ? tpl[tpl_index<LAYOUT>(P, i, py, pz)] : cconj(tpl[tpl_index<LAYOUT>(P, -i, py, pz)]);
         sum = cadd(sum, cscale(val, w));
       }
     }
@@ -114,6 +135,7 @@ __device__ __forceinline__ cf interp_fourier(const cf* __restrict__ tpl, int P, 
 // and (when filt is not null) the projection filter. Special cases of Image::ExtractSlice: the
 // i = 0 column takes Hermitian mates for negative y except the even-size edge row, and the DC term
 // is 0 (image.cpp:2556-2589).
+template <int LAYOUT = 0>
 __device__ __forceinline__ cf slice_value(const cf* __restrict__ tpl, int P, const float* __restrict__ m, int i,
                                           int j, const cf* __restrict__ phase, const cf* __restrict__ filt) {
   const int Ph = P / 2 + 1;
@@ -128,7 +150,7 @@ __device__ __forceinline__ cf slice_value(const cf* __restrict__ tpl, int P, con
     const float x3 = m[0] * x2 + m[1] * y2;
     const float y3 = m[3] * x2 + m[4] * y2;
     const float z3 = m[6] * x2 + m[7] * y2;
-    val = interp_fourier(tpl, P, x3, y3, z3);
+    val = interp_fourier<LAYOUT>(tpl, P, x3, y3, z3);
     if (mate) val = cconj(val);
   }
   val = cmul(val, phase[pix]);
@@ -140,7 +162,7 @@ __device__ __forceinline__ cf slice_value(const cf* __restrict__ tpl, int P, con
 // a block builds KC adjacent columns of one orientation's slice in LDS and transforms them there,
 // so the slice is written once, after the transform, in runs of KC columns. Same per-pixel
 // arithmetic and FFT plan (64 threads per column) as extracting the slice and then k_cols<P, 64, +1, 0>.
-template <int P, int KC>
+template <int P, int KC, int LAYOUT = 0>
 __global__ __launch_bounds__(KC * 64) void k_extract_cols(const cf* __restrict__ tpl, const float* __restrict__ mats,
                                                          int o0, const cf* __restrict__ phase,
                                                          const cf* __restrict__ filt, const cf* __restrict__ tw,
@@ -151,7 +173,7 @@ __global__ __launch_bounds__(KC * 64) void k_extract_cols(const cf* __restrict__
   const float* m = mats + (size_t)(o0 + b) * 9;
   for (int e = tid; e < KC * P; e += KC * NTP) {
     const int c = e % KC, j = e / KC, i = i0 + c;
-    buf[c][lpad(j)] = i < Ph ? slice_value(tpl, P, m, i, j, phase, filt) : cmake(0.f, 0.f);
+    buf[c][lpad(j)] = i < Ph ? slice_value<LAYOUT>(tpl, P, m, i, j, phase, filt) : cmake(0.f, 0.f);
   }
   __syncthreads();
   fft_inplace<P, +1, NTP>(buf[tid / NTP], tw, tid % NTP);
@@ -162,15 +184,31 @@ __global__ __launch_bounds__(KC * 64) void k_extract_cols(const cf* __restrict__
   }
 }
 
+// Copies a P^3 half spectrum from FFTW order into layout 1 (padding bricks entries stay unset and
+// are never read: x < P/2+1 always).
+static __global__ void k_tpl_to_bricks(const cf* __restrict__ src, int P, cf* __restrict__ dst) {
+  const int Ph = P / 2 + 1;
+  const size_t n = (size_t)P * P * Ph;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    const int x = (int)(e % Ph);
+    const size_t r = e / Ph;
+    const int y = (int)(r % P), z = (int)(r / P);
+    dst[tpl_index<1>(P, x, y, z)] = src[e];
+  }
+}
+
 // ---------------------------------------------------------------- small line transforms
 // Complex FFTs of length N down columns of [b][N][ncols] complex arrays (row stride ncols).
 // MODE 0: one transform of sign SIGN. MODE 1: forward, multiply by scale * filt[row][col], inverse
 // (ForwardFFT scale, MultiplyPixelWise, first half of BackwardFFT; match_template.cpp:1019-1030).
+// Launched as a one-dimensional grid of ncols * nb blocks.
 template <int N, int NT, int SIGN, int MODE>
 __global__ __launch_bounds__(NT) void k_cols(cf* __restrict__ data, int ncols, size_t bstride, const cf* __restrict__ tw,
                                              const cf* __restrict__ filt, float scale) {
   __shared__ cf buf[lds_len(N)];
-  const int col = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
+  // one-dimensional grid of ncols x nb blocks; neighbouring columns run on one XCD and share the
+  // cache lines of their rows there
+  const int item = xcd_row(blockIdx.x, gridDim.x), col = item % ncols, b = item / ncols, tid = threadIdx.x;
   cf* base = data + (size_t)b * bstride + col;
   for (int e = tid; e < N; e += NT) buf[lpad(e)] = base[(size_t)e * ncols];
   __syncthreads();
@@ -601,14 +639,6 @@ struct EpiParams {
   int ntiles;  // column tiles per orientation
   float zinv;  // inverse of the scale of 16-bit Z storage (1 for FP32)
 };
-
-// Image row of block bid. Workgroups are dispatched round-robin over the 8 XCDs, so XCD x runs
-// the blocks with bid % 8 == x; it gets the contiguous rows starting at x * (H / 8) + min(x, H % 8),
-// as many as it has blocks, and consecutive rows share Z's cache lines in that XCD's L2.
-__device__ __forceinline__ int xcd_row(int bid, int H) {
-  const int x = bid % 8, q = H / 8, r = H % 8;
-  return x * q + min(x, r) + bid / 8;
-}
 
 // Threads of a k_last block: QR orientations x (W/2) / R1 threads, R1 the first radix of the
 // W/2-point plan (radix 8 wherever W/2 has three factors of two).

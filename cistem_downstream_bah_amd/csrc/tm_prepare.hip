@@ -216,10 +216,10 @@ void cols_t(Dev& d, cf* data, int ncols, size_t bstride, int nb, int sign) {
   constexpr int NT = N >= 2048 ? 256 : 64;
   cf* tw = d.twiddles(N);
   if (sign < 0)
-    hipLaunchKernelGGL((tmk::k_cols<N, NT, -1, 0>), dim3(ncols, nb), dim3(NT), 0, d.s, data, ncols, bstride, tw,
+    hipLaunchKernelGGL((tmk::k_cols<N, NT, -1, 0>), dim3(ncols * nb), dim3(NT), 0, d.s, data, ncols, bstride, tw,
                        (const cf*)nullptr, 1.0f);
   else
-    hipLaunchKernelGGL((tmk::k_cols<N, NT, +1, 0>), dim3(ncols, nb), dim3(NT), 0, d.s, data, ncols, bstride, tw,
+    hipLaunchKernelGGL((tmk::k_cols<N, NT, +1, 0>), dim3(ncols * nb), dim3(NT), 0, d.s, data, ncols, bstride, tw,
                        (const cf*)nullptr, 1.0f);
   chk("k_cols");
 }

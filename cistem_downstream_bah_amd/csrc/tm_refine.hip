@@ -166,18 +166,19 @@ __global__ __launch_bounds__(NT) void k_score_product(cf* __restrict__ C, int t,
 }
 
 // Peak of each correlation image (Image::FindPeakWithIntegerCoordinates with no radius or edge
-// limit, image.cpp:9961-10105): the first maximum in scan order, reported relative to the box centre,
-// its value divided by sqrtf(ReturnSumOfSquares) of the filtered projection.
+// limit, image.cpp:9961-10105): the first maximum in scan order. Stage 1: block q of item b scans
+// rows [q t / nq, (q + 1) t / nq) and writes its (value, index) partial.
 template <int NT>
-__global__ __launch_bounds__(NT) void k_score_peak(const float* __restrict__ R, int t, const double* __restrict__ partial,
-                                                   int nparts, float* __restrict__ out) {
+__global__ __launch_bounds__(NT) void k_score_peak_rows(const float* __restrict__ R, int t, float* __restrict__ pv,
+                                                        int* __restrict__ pi) {
   __shared__ float rv[NT];
   __shared__ int ri[NT];
-  const int b = blockIdx.x;
+  const int q = blockIdx.x, nq = gridDim.x, b = blockIdx.y;
   const float* r = R + (size_t)b * t * (t + 2);
+  const int e0 = q * t / nq * t, e1 = (q + 1) * t / nq * t;
   float best = -FLT_MAX;
   int bi = 0x7fffffff;
-  for (int e = threadIdx.x; e < t * t; e += NT) {
+  for (int e = e0 + threadIdx.x; e < e1; e += NT) {
     const int j = e / t, i = e - j * t;
     const float v = r[(size_t)j * (t + 2) + i];
     if (v > best) { best = v; bi = e; }
@@ -188,21 +189,39 @@ __global__ __launch_bounds__(NT) void k_score_peak(const float* __restrict__ R, 
   for (int s = NT / 2; s > 0; s >>= 1) {
     if (threadIdx.x < s) {
       const float v = rv[threadIdx.x + s];
-      const int q = ri[threadIdx.x + s];
-      if (v > rv[threadIdx.x] || (v == rv[threadIdx.x] && q < ri[threadIdx.x])) { rv[threadIdx.x] = v; ri[threadIdx.x] = q; }
+      const int k = ri[threadIdx.x + s];
+      if (v > rv[threadIdx.x] || (v == rv[threadIdx.x] && k < ri[threadIdx.x])) { rv[threadIdx.x] = v; ri[threadIdx.x] = k; }
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    double ss = 0.0;
-    for (int q = 0; q < nparts; ++q) ss += partial[(size_t)b * nparts + q];
-    const float sos = (float)(2.0 * ss);
-    const float inv = (float)(1. / sqrtf(sos));
-    const int e = ri[0] == 0x7fffffff ? 0 : ri[0];
-    out[3 * b] = (float)(e % t - t / 2);
-    out[3 * b + 1] = (float)(e / t - t / 2);
-    out[3 * b + 2] = ri[0] == 0x7fffffff ? -FLT_MAX : rv[0] * inv;
+    pv[(size_t)b * nq + q] = rv[0];
+    pi[(size_t)b * nq + q] = ri[0];
   }
+}
+
+// Stage 2, one thread per item: the partials in row order, the peak reported relative to the box
+// centre, its value divided by sqrtf(ReturnSumOfSquares) of the filtered projection (the division
+// TemplateScore applies to the spectrum, refine_template.cpp:63).
+__global__ void k_score_peak_final(const float* __restrict__ pv, const int* __restrict__ pi, int nq, int t,
+                                   const double* __restrict__ partial, int nparts, int n, float* __restrict__ out) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n) return;
+  float best = -FLT_MAX;
+  int bi = 0x7fffffff;
+  for (int q = 0; q < nq; ++q) {
+    const float v = pv[(size_t)b * nq + q];
+    const int k = pi[(size_t)b * nq + q];
+    if (v > best || (v == best && k < bi)) { best = v; bi = k; }
+  }
+  double ss = 0.0;
+  for (int q = 0; q < nparts; ++q) ss += partial[(size_t)b * nparts + q];
+  const float sos = (float)(2.0 * ss);
+  const float inv = (float)(1. / sqrtf(sos));
+  const int e = bi == 0x7fffffff ? 0 : bi;
+  out[3 * b] = (float)(e % t - t / 2);
+  out[3 * b + 1] = (float)(e / t - t / 2);
+  out[3 * b + 2] = bi == 0x7fffffff ? -FLT_MAX : best * inv;
 }
 
 // Peak list on the scaled MIP (refine_template.cpp:561-600): first maximum in scan order over the
@@ -554,12 +573,13 @@ struct tm_refine {
   DevBuf<float> img, curve_x, curve_y, mats, dfs, results, smip, win_real, arg_v;
   DevBuf<cf> tpl, phaseP, phaseT, slice, proj, win, win_stage;
   DevBuf<double> partial;
-  DevBuf<int> slots;
+  DevBuf<float> peak_v;
+  DevBuf<int> slots, peak_i;
   DevBuf<long long> shifts, arg_i;
   PinnedBuf<float> h_mats, h_dfs, h_results, h_arg_v;
   PinnedBuf<int> h_slots;
   PinnedBuf<long long> h_shifts, h_arg_i;
-  int score_parts = 1;
+  int score_parts = 1, peak_parts = 1;
   std::map<int, cf*> tw;
   // timing: event pairs around each stage of a TemplateScore batch, resolved after the batch's
   // synchronisation
@@ -726,7 +746,7 @@ void run_scores(tm_refine* h, const std::vector<ScoreRequest>& reqs, const std::
     const int r0 = P / 2 - t / 2;
     {
       Stage st(h, R::S_EXTRACT);
-      extract_cols(h, P, h->tpl.p, h->mats.p, 0, n, h->phaseP.p, nullptr, h->slice.p);
+      extract_cols<1>(h, P, h->tpl.p, h->mats.p, 0, n, h->phaseP.p, nullptr, h->slice.p);
     }
     {
       Stage st(h, R::S_SLICE_C2R);
@@ -741,7 +761,7 @@ void run_scores(tm_refine* h, const std::vector<ScoreRequest>& reqs, const std::
     // ExtractSlice, SwapRealSpaceQuadrants, BackwardFFT, ForwardFFT (rows in place)
     {
       Stage st(h, R::S_EXTRACT);
-      extract_cols(h, t, h->tpl.p, h->mats.p, 0, n, h->phaseT.p, nullptr, h->proj.p);
+      extract_cols<1>(h, t, h->tpl.p, h->mats.p, 0, n, h->phaseT.p, nullptr, h->proj.p);
     }
     {
       Stage st(h, R::S_SLICE_C2R);
@@ -766,9 +786,13 @@ void run_scores(tm_refine* h, const std::vector<ScoreRequest>& reqs, const std::
   }
   {
     Stage st(h, R::S_PEAK);
-    hipLaunchKernelGGL((rtk::k_score_peak<NT>), dim3(n), dim3(NT), 0, h->stream, reinterpret_cast<const float*>(h->proj.p),
-                       t, h->partial.p, h->score_parts, h->results.p);
-    check_launch("k_score_peak");
+    const int nq = h->peak_parts;
+    hipLaunchKernelGGL((rtk::k_score_peak_rows<NT>), dim3(nq, n), dim3(NT), 0, h->stream,
+                       reinterpret_cast<const float*>(h->proj.p), t, h->peak_v.p, h->peak_i.p);
+    check_launch("k_score_peak_rows");
+    hipLaunchKernelGGL(rtk::k_score_peak_final, dim3((n + 63) / 64), dim3(64), 0, h->stream, h->peak_v.p, h->peak_i.p, nq,
+                       t, h->partial.p, h->score_parts, n, h->results.p);
+    check_launch("k_score_peak_final");
   }
   TM_HIP(hipMemcpyAsync(h->h_results.p, h->results.p, 3 * n * sizeof(float), hipMemcpyDeviceToHost, h->stream));
   TM_HIP(hipStreamSynchronize(h->stream));
@@ -904,8 +928,16 @@ int tm_refine_init(tm_refine* h, const tm_refine_params* p, const float* whitene
     h->img.alloc(npx);
     TM_HIP(hipMemcpy(h->img.p, whitened_image, npx * sizeof(float), hipMemcpyHostToDevice));
     h->smip.alloc(npx);
-    h->tpl.alloc((size_t)P * P * h->Ph);
-    TM_HIP(hipMemcpy(h->tpl.p, template_spectrum, h->tpl.n * sizeof(cf), hipMemcpyHostToDevice));
+    {
+      // the template spectrum in the bricked layout the extraction reads (tpl_index<1>)
+      DevBuf<cf> linear;
+      linear.alloc((size_t)P * P * h->Ph);
+      TM_HIP(hipMemcpy(linear.p, template_spectrum, linear.n * sizeof(cf), hipMemcpyHostToDevice));
+      h->tpl.alloc((size_t)(P / 2) * (P / 2) * ((P / 2 + 4) / 4) * 16);
+      hipLaunchKernelGGL(tmk::k_tpl_to_bricks, dim3(4096), dim3(256), 0, h->stream, linear.p, P, h->tpl.p);
+      check_launch("k_tpl_to_bricks");
+      TM_HIP(hipStreamSynchronize(h->stream));
+    }
     h->curve_x.alloc(n_curve);
     h->curve_y.alloc(n_curve);
     TM_HIP(hipMemcpy(h->curve_x.p, curve_x, n_curve * sizeof(float), hipMemcpyHostToDevice));
@@ -925,6 +957,9 @@ int tm_refine_init(tm_refine* h, const tm_refine_params* p, const float* whitene
     h->slots.alloc(K);
     h->shifts.alloc(K);
     h->partial.alloc((size_t)K * h->score_parts);
+    h->peak_parts = std::max(1, std::min(t, t * t / 4096));
+    h->peak_v.alloc((size_t)K * h->peak_parts);
+    h->peak_i.alloc((size_t)K * h->peak_parts);
     if (P != t) h->slice.alloc((size_t)K * P * h->Ph);
     h->proj.alloc((size_t)K * per);
     h->win.alloc((size_t)K * per);
