@@ -3,7 +3,7 @@
 //   tm_prepare_sizes / tm_prepare_input_image   match_template.cpp:490-598, 703-724
 //   tm_prepare_template                         match_template.cpp:621-625, 837-845
 //   tm_projection_filter                        match_template.cpp:893-898 (CTF x whitening)
-//   tm_prepare_refine_image                     refine_template.cpp:476-510 (whitened real-space image)
+//   tm_prepare_refine_image                     refine_template.cpp:526-546 (whitened real-space image)
 //   tm_search_grid / tm_psi_list                euler_search.cpp:180-282, 353-443; match_template.cpp:658-697, 1001
 // Host loops restate the reference's Image/Curve/CTF arithmetic; the 2-D and 3-D R2C transforms
 // run on the device with the same kernels as the search.
@@ -397,7 +397,7 @@ void replace_outliers(Img& img) {
 
 // Whitening curve of a spectrum (H x (W/2+1), DC already 0): Compute1DPowerSpectrumCurve
 // (image.cpp:7135-7205; intensities, Hermitian mates skipped), SquareRoot, Reciprocal, scaled to a
-// maximum of 1 (match_template.cpp:711-716, refine_template.cpp:501-505).
+// maximum of 1 (match_template.cpp:711-716, refine_template.cpp:536-540).
 Curve whitening_curve(const std::complex<float>* spec, int W, int H, int ncurve) {
   Curve wf, nt;
   wf.setup(0.0, 0.5 * sqrtf(2.0), ncurve);
@@ -600,7 +600,7 @@ int tm_prepare_refine_image(int device, const float* micrograph, int mx, int my,
     Curve wf = whitening_curve(spec.data(), mx, my, ncurve);
     apply_curve(spec.data(), mx, my, wf);
     spec[0] = 0.f;
-    // DivideByConstant(sqrt(ReturnSumOfSquares())) (refine_template.cpp:508)
+    // DivideByConstant(sqrt(ReturnSumOfSquares())) (refine_template.cpp:544)
     const float inv = 1. / (float)sqrt((double)spectrum_sum_of_squares(spec.data(), mx, my));
     for (auto& c : spec) c *= inv;
     Img out = backward_fft(device, spec, mx, my);

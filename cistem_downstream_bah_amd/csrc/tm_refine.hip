@@ -1,7 +1,7 @@
 // Local refinement of template-matching peaks on the GPU and its C ABI (include/tm_refine.h):
 // the per-image work of refine_template (src/programs/refine_template/refine_template.cpp).
 //
-// Each peak's coordinate descent (refine_template.cpp:748-1087) is a C++20 coroutine that suspends
+// Each peak's coordinate descent (refine_template.cpp:746-1086) is a C++20 coroutine that suspends
 // at every TemplateScore call. The driver keeps up to max_in_flight peaks in flight, collects their
 // pending TemplateScore requests into one batch, runs the batch as a few device launches (slice
 // extraction, the slice's C2R and the projection's R2C, the CTF x whitening product fused with the
@@ -94,7 +94,7 @@ __device__ float filter_value(int i, int j, int t, float d1, float d2, const Ctf
   return ctf * curve_value(cx, cy, n_curve, sf);
 }
 
-// Particle windows (refine_template.cpp:738-741): window p is the t x t centre clip of the
+// Particle windows (refine_template.cpp:713-718): window p is the t x t centre clip of the
 // whitened image after RealSpaceIntegerShift (image.cpp:9064-9107), i.e. pixel (ii, jj) reads the
 // image at linear index (k - shift0[p]) mod n with k the index of (mx/2 + ii - t/2, my/2 + jj - t/2).
 // Output rows of t + 2 floats (padded layout).
@@ -125,7 +125,7 @@ __global__ void k_window_phase(const cf* __restrict__ stage, const cf* __restric
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < per; e += gridDim.x * blockDim.x) d[e] = cmul(s[e], phase[e]);
 }
 
-// TemplateScore (refine_template.cpp:49-66) on the projection spectra of a batch, item b:
+// TemplateScore (refine_template.cpp:54-72) on the projection spectra of a batch, item b:
 // c = C x filter (MultiplyPixelWise), DC 0 (ZeroCentralPixel), the Hermitian-weighted sum of |c|^2
 // of ReturnSumOfSquares' Fourier branch (image.cpp:185-311) into one partial per block, and
 // C <- conj(c) x window. The division by sqrt(sum) is applied to the peak value afterwards (a
@@ -202,7 +202,7 @@ __global__ __launch_bounds__(NT) void k_score_peak_rows(const float* __restrict_
 
 // Stage 2, one thread per item: the partials in row order, the peak reported relative to the box
 // centre, its value divided by sqrtf(ReturnSumOfSquares) of the filtered projection (the division
-// TemplateScore applies to the spectrum, refine_template.cpp:63).
+// TemplateScore applies to the spectrum, refine_template.cpp:60).
 __global__ void k_score_peak_final(const float* __restrict__ pv, const int* __restrict__ pi, int nq, int t,
                                    const double* __restrict__ partial, int nparts, int n, float* __restrict__ out) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -224,7 +224,7 @@ __global__ void k_score_peak_final(const float* __restrict__ pv, const int* __re
   out[3 * b + 2] = bi == 0x7fffffff ? -FLT_MAX : best * inv;
 }
 
-// Peak list on the scaled MIP (refine_template.cpp:561-600): first maximum in scan order over the
+// Peak list on the scaled MIP (refine_template.cpp:561-601): first maximum in scan order over the
 // pixels at least `edge` from the borders (upper bounds inclusive), as (value, dense index) partials
 // per block, then one block combines them.
 template <int NT>
@@ -282,7 +282,7 @@ __global__ __launch_bounds__(NT) void k_argmax_final(const float* __restrict__ p
   if (threadIdx.x == 0) { *ov = rv[0]; *oi = ri[0]; }
 }
 
-// Pixels whose squared distance from (cx, cy) is <= radius become -FLT_MAX (refine_template.cpp:583-590;
+// Pixels whose squared distance from (cx, cy) is <= radius become -FLT_MAX (refine_template.cpp:580-592;
 // the program compares the squared distance with min_peak_radius). Launched over a box that holds
 // every such pixel.
 __global__ void k_map_mask(float* __restrict__ a, int mx, int x0, int y0, int w, int hgt, float cx, float cy,
@@ -351,7 +351,7 @@ struct Score {
   ScoreResult await_resume() const noexcept { return c.promise().res; }
 };
 
-// The per-thread result maps best_*_local of the program (refine_template.cpp:682-689) as the input
+// The per-thread result maps best_*_local of the program (refine_template.cpp:680-689) as the input
 // maps plus the addresses written so far. Addresses are those of the padded layout (rows of mx + 2);
 // the two padding columns read as 0.
 enum { M_PSI, M_THETA, M_PHI, M_DEFOCUS, M_PIXEL_SIZE };
@@ -402,7 +402,7 @@ float dense_at(const float* m, int mx, long long rs, long long addr) {
   return col < mx ? m[row * mx + col] : 0.0f;
 }
 
-// The program's per-peak body (refine_template.cpp:738-1113 with pixel_size_search_range 0).
+// The program's per-peak body (refine_template.cpp:704-1113 with pixel_size_search_range 0).
 DescentTask refine_peak(const RunConst& K, PeakState& st) {
   const tm_refine_params& p = K.p;
   LocalMaps& L = *st.maps;
@@ -448,8 +448,8 @@ DescentTask refine_peak(const RunConst& K, PeakState& st) {
     current_defocus = L.get(best_address, M_DEFOCUS);
   }
   int phi_i = 0, theta_i = 0, psi_i = 0, defocus_i = 0;
-  // block 0: defocus refinement with nested angle descents (:851-948); block 1: the pixel-size
-  // block with size_i fixed at 0 (:1000-1086), after restarting from the values at best_address
+  // block 0: defocus refinement with nested angle descents (:848-929); block 1: the pixel-size
+  // block with size_i fixed at 0 (:946-1086), after restarting from the values at best_address
   for (int block = 0; block < 2; ++block) {
     float filter_step = 0.0f;
     const float mult_start = block == 0 ? K.defocus_step / K.defocus_refine_step : 1.0f;
@@ -808,7 +808,7 @@ struct FoundPeak {
   float x, y, value;
 };
 
-// The peak list (refine_template.cpp:561-600) on the device copy of the scaled MIP (consumed).
+// The peak list (refine_template.cpp:561-601) on the device copy of the scaled MIP (consumed).
 std::vector<FoundPeak> peak_list(tm_refine* h, long long cap) {
   const int mx = h->prm.image_x, my = h->prm.image_y;
   constexpr int NT = 256, NB = 1024;
@@ -1144,7 +1144,7 @@ int tm_refine_run(tm_refine* h, const float* const* maps, float* const* out_maps
         c.resume();
       }
     }
-    // output maps (refine_template.cpp:641-648, 1093-1113), peaks applied in peak-list order
+    // output maps (refine_template.cpp:641-648, 1092-1132), peaks applied in peak-list order
     if (out_maps) {
       for (int k = 0; k < 7; ++k) {
         if (!out_maps[k]) continue;

@@ -1,7 +1,7 @@
 """Host driver mirroring the reference's ``refine_template`` program for one image on the GPU.
 
 ``TemplateRefiner`` does what RefineTemplateApp::DoCalculation does
-(src/programs/refine_template/refine_template.cpp:298-1130): whiten the image and transform it back
+(src/programs/refine_template/refine_template.cpp:264-1229): whiten the image and transform it back
 to real space, pad and transform the template, take the peaks of the scaled MIP, window each peak
 and run the coordinate descent over defocus and the Euler angles with TemplateScore. Everything
 runs through the C ABI of libtm_hip.so (include/tm_refine.h, include/tm_prepare.h); there is no CPU
@@ -81,7 +81,7 @@ def _f(a):
 
 
 def prepare_refine_image(micrograph: np.ndarray, device: int = 0):
-    """Whitened real-space image and whitening curve (refine_template.cpp:476-510)."""
+    """Whitened real-space image and whitening curve (refine_template.cpp:526-546)."""
     L = _lib()
     m = np.ascontiguousarray(micrograph, np.float32)
     my, mx = m.shape
@@ -96,7 +96,7 @@ def prepare_refine_image(micrograph: np.ndarray, device: int = 0):
 
 @dataclass
 class RefineSettings:
-    """refine_template arguments used by the path (refine_template.cpp:260-300)."""
+    """refine_template arguments used by the path (refine_template.cpp:264-312)."""
     pixel_size: float = 1.0
     voltage_kV: float = 300.0
     spherical_aberration_mm: float = 2.7

@@ -43,7 +43,7 @@ int tm_psi_list(float psi_start, float psi_max, float psi_step, float* out, int 
 /* refine_template's input image: micrograph (image_x x image_y, x fastest, image_x even) -> outliers
  * replaced, whitened, normalised to unit spectral energy and transformed back to real space
  * (whitened, image_x x image_y dense), plus the whitening curve (*n_curve points into curve_x/curve_y).
- * refine_template.cpp:476-510. With whitened == NULL only *n_curve is set. The image sides must be
+ * refine_template.cpp:526-546. With whitened == NULL only *n_curve is set. The image sides must be
  * sizes with compiled transforms (TM_LINE_SIZES in tm_sizes.hpp), else TM_ERR_UNSUPPORTED. */
 int tm_prepare_refine_image(int device, const float* micrograph, int image_x, int image_y, float* whitened,
                             float* curve_x, float* curve_y, int* n_curve);

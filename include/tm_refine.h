@@ -2,7 +2,7 @@
  * C ABI of the MI355X (gfx950) local refinement of template-matching peaks.
  *
  * It replaces the per-image work of the reference program refine_template
- * (src/programs/refine_template/refine_template.cpp:298-1130): the peak list taken from the scaled
+ * (src/programs/refine_template/refine_template.cpp:264-1229): the peak list taken from the scaled
  * MIP, one particle window per peak, TemplateScore (refine_template.cpp:25-80) and the coordinate
  * descent over defocus and the three Euler angles. TemplateScore calls of all peaks in flight run
  * together as one batch of device transforms; the descent logic of each peak runs on the host.
@@ -48,7 +48,7 @@ typedef struct tm_refine_params {
 } tm_refine_params;
 
 /* One refined peak (TemplateMatchFoundPeakInfo of all_peak_infos / all_peak_changes,
- * refine_template.cpp:1091-1113). Positions in pixels relative to the image centre. */
+ * refine_template.cpp:1092-1113). Positions in pixels relative to the image centre. */
 typedef struct tm_refined_peak {
   float x, y, value;                           /* found peak and its scaled-MIP value               */
   float psi, theta, phi, defocus, pixel_size;  /* the search's values at the found peak             */
@@ -70,8 +70,8 @@ int tm_refine_create(int device_ordinal, tm_refine** out);
 int tm_refine_destroy(tm_refine* h);
 
 /* Copies the refinement inputs to the device. whitened_image: image_x x image_y dense real (the
- * program's input_image after refine_template.cpp:498-510); template_spectrum: P^3 half-complex,
- * scaled 1/N, DC 0, quadrant-swapped (refine_template.cpp:476-488); curve_x/curve_y: the whitening
+ * program's input_image after refine_template.cpp:526-546); template_spectrum: P^3 half-complex,
+ * scaled 1/N, DC 0, quadrant-swapped (refine_template.cpp:497-507); curve_x/curve_y: the whitening
  * curve (n_curve points) that multiplies the CTF in each projection filter. */
 int tm_refine_init(tm_refine* h, const tm_refine_params* params, const float* whitened_image,
                    const float* template_spectrum, const float* curve_x, const float* curve_y, int n_curve);

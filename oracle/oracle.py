@@ -83,7 +83,7 @@ def lib():
 
 class RefinedPeak(ctypes.Structure):
     """One refined peak: the found peak and the search's values there, then the refined values
-    (refine_template.cpp:1091-1113). Positions are pixels relative to the image centre."""
+    (refine_template.cpp:1092-1113). Positions are pixels relative to the image centre."""
     _fields_ = [(n, ctypes.c_float) for n in
                 ("x", "y", "value", "psi", "theta", "phi", "defocus", "pixel_size", "x_pos", "y_pos", "shift_x",
                  "shift_y", "r_psi", "r_theta", "r_phi", "r_defocus", "r_pixel_size", "peak_height", "mip",
@@ -373,7 +373,7 @@ class MatchTemplateOracle:
 # ---------------------------------------------------------------- refine_template
 @dataclass
 class RTParams:
-    """refine_template arguments (refine_template.cpp:260-300) used by the restatement."""
+    """refine_template arguments (refine_template.cpp:264-312) used by the restatement."""
     pixel_size: float = 1.0
     voltage_kV: float = 300.0
     spherical_aberration_mm: float = 2.7
@@ -431,7 +431,7 @@ class RefineTemplateOracle:
 
     @property
     def whitened_image(self) -> np.ndarray:
-        """(my, mx) whitened, normalised input image in real space (refine_template.cpp:498-510)."""
+        """(my, mx) whitened, normalised input image in real space (refine_template.cpp:526-546)."""
         return self._get(0, self._pad() * self.my).reshape(self.my, self._pad())[:, : self.mx].copy()
 
     @property

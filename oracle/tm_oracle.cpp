@@ -532,8 +532,8 @@ void unpack_params(MTParams& p, const float* f, const int* i) {
 
 // =============================================================================================
 // refine_template (src/programs/refine_template/refine_template.cpp) for one image: the peak list
-// from the scaled MIP (:561-600), the per-peak particle window (:738-744), TemplateScore (:25-80)
-// and the coordinate descent over defocus and angles (:748-1000, 1016-1072), with the result maps
+// from the scaled MIP (:561-601), the per-peak particle window (:713-718), TemplateScore (:25-80)
+// and the coordinate descent over defocus and angles (:746-1086), with the result maps
 // written and read at best_address as the program does. Not restated: pixel-size refinement
 // (pixel_size_search_range must be 0, so TemplateScore's ChangePixelSize has factor 1, which is
 // ClipInto + ForwardFFT, image.cpp:8705-8722) and the particle mask (mask_radius must be 0, the
@@ -617,7 +617,7 @@ struct RefineOracle {
   typedef R Real;
   RefineParams p;
   int mx = 0, my = 0, t = 0, P = 0;
-  Image<R> input_image;           // whitened, real space (refine_template.cpp:498-510)
+  Image<R> input_image;           // whitened, real space (refine_template.cpp:526-546)
   Image<R> input_reconstruction;  // padded template spectrum, DC 0, quadrant-swapped (:482-488)
   Curve<R> whitening_filter, number_of_terms;
   CTFModel<R> ctf;
@@ -666,7 +666,7 @@ struct RefineOracle {
     projection_filter.Allocate(t, t, 1, false);
   }
 
-  // :738-744 (mask_radius 0): the whitened image shifted so the peak sits at the centre, clipped to
+  // :713-718 (mask_radius 0): the whitened image shifted so the peak sits at the centre, clipped to
   // the template box, transformed and quadrant-swapped
   void Window(int peak_x, int peak_y) {
     Image<R> padded_reference = input_image;
@@ -677,7 +677,7 @@ struct RefineOracle {
     windowed_particle.SwapRealSpaceQuadrants();
   }
 
-  // CTF x whitening at defocus1|2 + current + step, summed in the program's order (:757-759, :866)
+  // CTF x whitening at defocus1|2 + current + step, summed in the program's order (:759, :810, :866)
   void Filter(float current_defocus, float step) {
     ctf.SetDefocus((p.defocus1 + current_defocus + step) / p.pixel_size, (p.defocus2 + current_defocus + step) / p.pixel_size,
                    deg_2_rad(p.defocus_angle));
@@ -722,7 +722,7 @@ struct RefineOracle {
     return FindPeakWithIntegerCoordinates(cp, 0);
   }
 
-  // :561-600: peaks of the scaled MIP above the threshold, each masking the pixels whose squared
+  // :561-601: peaks of the scaled MIP above the threshold, each masking the pixels whose squared
   // distance is <= min_peak_radius (the program compares the squared distance with the radius)
   std::vector<PeakI> PeakList(Image<R> scaled) const {
     std::vector<PeakI> out;
@@ -751,7 +751,7 @@ struct RefineOracle {
   };
 
   // Refines every peak of PeakList. shared_local_maps: the per-thread copies best_*_local of the
-  // program (:682-689) carry over from one peak to the next, as in a run with one thread; otherwise
+  // program (:680-689) carry over from one peak to the next, as in a run with one thread; otherwise
   // every peak starts from the input maps, as a thread that handles a single peak does. out (may be
   // null) receives best_mip, best_scaled_mip, best_psi, ... (:641-648, 1093-1113).
   std::vector<RefinedPeak> Run(const Maps& in, bool shared_local_maps, float xy_change_threshold,
@@ -823,7 +823,7 @@ struct RefineOracle {
         current_defocus = (float)best_defocus_local.v[best_address];
       }
       int phi_i = 0, theta_i = 0, psi_i = 0, defocus_i = 0;
-      // one descent over phi, theta, psi (:870-935); on improvement the block's defocus and pixel
+      // one descent over phi, theta, psi (:870-925); on improvement the block's defocus and pixel
       // size values are written beside the angles
       auto angles_descent = [&](float written_defocus, float written_pixel_size) {
         for (int kk = 0; kk < 2; kk = -2 * kk + 1) {
@@ -862,7 +862,7 @@ struct RefineOracle {
           phi_i -= kk;
         }
       };
-      // :851-948 defocus refinement with nested angle descents
+      // :848-929 defocus refinement with nested angle descents
       const float mult_i_start = defocus_step / defocus_refine_step;
       for (float mult_i = mult_i_start; mult_i > 0.5f; mult_i /= 2.0f) {
         for (int ll = 0; ll < 2; ll = -2 * ll + 1) {
@@ -877,7 +877,7 @@ struct RefineOracle {
           if (defocus_search_range != 0.0f) defocus_i -= ll;
         }
       }
-      // :950-962 restart from the values stored at the best address
+      // :931-944 restart from the values stored at the best address
       current_phi = (float)best_phi_local.v[best_address];
       current_theta = (float)best_theta_local.v[best_address];
       current_psi = (float)best_psi_local.v[best_address];
@@ -885,7 +885,7 @@ struct RefineOracle {
       current_pixel_size = (float)best_pixel_size_local.v[best_address];
       phi_i = theta_i = psi_i = 0;
       Filter(current_defocus, 0.0f);
-      // :1000-1086 pixel-size block with pixel_size_search_range 0: size_i stays 0, the factor stays 1
+      // :946-1086 pixel-size block with pixel_size_search_range 0: size_i stays 0, the factor stays 1
       // and three more angle descents run around the restart values
       for (float mult_i = 1.0f; mult_i > 0.5f; mult_i /= 2.0f) {
         for (int ll = 0; ll < 2; ll = -2 * ll + 1) {
@@ -897,7 +897,7 @@ struct RefineOracle {
           } while (best_score > best_pixel_size_score);
         }
       }
-      // :1091-1113 results
+      // :1092-1113 results
       const float best_scaled = score_adjustment * scaled_here * best_score / mip_here * sqrt_t;
       const float best_mip = score_adjustment * best_score * sqrt_t;
       RefinedPeak r{};

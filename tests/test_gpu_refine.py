@@ -1,11 +1,11 @@
 """GPU: the refine_template path (include/tm_refine.h) against the oracle's restatement.
 
-* preparation (refine_template.cpp:476-510): whitened real-space image within 1e-5 of its maximum,
+* preparation (refine_template.cpp:526-546): whitened real-space image within 1e-5 of its maximum,
   whitening curve within 1e-5 relative
 * TemplateScore (refine_template.cpp:25-80), template 64^3 unpadded (P = t) and padded (P = 2t),
   several windows, orientations and defocus offsets, with the oracle's inputs: peak position
   identical, value within 2e-5 relative
-* peak list (refine_template.cpp:561-600): identical to the oracle on a map with border cases,
+* peak list (refine_template.cpp:561-601): identical to the oracle on a map with border cases,
   ties and neighbours inside the exclusion radius
 * whole refinement of planted peaks (config 0: 512^2 image, copies started 3 degrees off), with and
   without the defocus scan, per-peak and shared local maps: every discrete result (peak positions,

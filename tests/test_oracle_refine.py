@@ -1,16 +1,16 @@
 """CPU: pins of the oracle's refine_template restatement (refine_template.cpp) by brute force,
 closed forms, invariants and ground truth.
 
-* peak list (refine_template.cpp:561-600, image.cpp:9961-10105): equals a numpy brute force —
+* peak list (refine_template.cpp:561-601, image.cpp:9961-10105): equals a numpy brute force —
   first maximum in scan order inside the 50-pixel border (upper bounds inclusive), then every pixel
   whose squared distance is <= min_peak_radius set to -FLT_MAX, until below the threshold
-* particle window (:738-744, image.cpp:9064-9107): the inverse of the window spectrum is the whitened
+* particle window (:713-718, image.cpp:9064-9107): the inverse of the window spectrum is the whitened
   image clipped around the peak; near the x edge the linear-index shift wraps into the next row
-* whitening (:498-510): white noise gives a flat curve, the real-space image has mean square 1
+* whitening (:526-546): white noise gives a flat curve, the real-space image has mean square 1
 * TemplateScore (:25-80): with the window set to the score's own normalised projection shifted by
   (dx, dy), the peak is at (dx, dy) with value 1 (Cauchy-Schwarz equality); a wrong conjugate,
   shift sign, swap or normalisation fails it
-* coordinate descent (:838-1087): the refined score is TemplateScore at the refined angles, and no
+* coordinate descent (:848-1086): the refined score is TemplateScore at the refined angles, and no
   single-axis angular step from the refined angles scores higher (the loops end on a pass without
   improvement)
 * ground truth: copies inserted at known orientations and positions, started 3 degrees off on every
