@@ -51,3 +51,22 @@ def test_input_and_template_preparation(data, padding):
     ef = _rel(pf, mt.projection_filter(0))
     print(f"padding {padding}: curve {ec:.2e} input spectrum {ei:.2e} template {et:.2e} filter {ef:.2e}", flush=True)
     assert ec <= 1e-5 and ei <= 1e-5 and et <= 1e-5 and ef <= 1e-5
+
+
+@pytest.mark.parametrize("defocus_i", [-1, 0, 2])
+def test_projection_filter_astigmatic_phase_plate(data, defocus_i):
+    """CTF x whitening with astigmatism (defocus1 != defocus2, azimuth 30 deg), a 20 deg phase shift,
+    200 kV, 1.2 A pixels and 10% amplitude contrast (ctf.cpp:92-150, 445-541; image.cpp:3113-3165)."""
+    s = data
+    prm = O.MTParams(pixel_size=1.2, voltage_kV=200.0, amplitude_contrast=0.1, defocus1=14000.0, defocus2=16500.0,
+                     defocus_angle=30.0, phase_shift_deg=20.0, defocus_search_range=300.0, defocus_step=150.0)
+    mt = O.MatchTemplateOracle(prm, s.micrograph, s.volume)
+    cx, cy = mt.whitening_curve
+    d1 = float(np.float32(prm.defocus1) + np.float32(defocus_i) * np.float32(prm.defocus_step))
+    d2 = float(np.float32(prm.defocus2) + np.float32(defocus_i) * np.float32(prm.defocus_step))
+    pf = MT.projection_filter(mt.info.T, cx, cy, prm.voltage_kV, prm.spherical_aberration_mm, prm.amplitude_contrast,
+                              d1, d2, prm.defocus_angle, prm.pixel_size, prm.phase_shift_deg)
+    ref = mt.projection_filter(defocus_i)
+    ef = _rel(pf, ref)
+    print(f"astigmatic filter, defocus_i {defocus_i}: {ef:.2e}", flush=True)
+    assert ef <= 1e-5

@@ -100,6 +100,31 @@ def test_template_score(c0, padding):
         g.close()
 
 
+def test_template_score_astigmatic_phase_plate(c0):
+    """TemplateScore with the CTF computed on the device: astigmatism (azimuth 30 deg), a 20 deg phase
+    shift, 200 kV, 1.2 A pixels, 10% amplitude contrast (ctf.cpp:445-541, image.cpp:3113-3165, 7284-7320)."""
+    print("case: TemplateScore astigmatic / phase shift", flush=True)
+    rp = O.RTParams(pixel_size=1.2, voltage_kV=200.0, amplitude_contrast=0.1, defocus1=14000.0, defocus2=16500.0,
+                    defocus_angle=30.0, phase_shift_deg=20.0, padding=2.0)
+    rt, g = _pair(c0, rp)
+    try:
+        H, W = c0.micrograph.shape
+        worst = 0.0
+        for q in c0.particles[:3]:
+            px, py = int(round(q.x)) - W // 2, int(round(q.y)) - H // 2
+            rt.window(px, py)
+            for (cur, step) in [(0.0, 0.0), (250.0, -100.0), (-300.0, 200.0)]:
+                a = (q.phi + 1.0, q.theta - 2.0, q.psi + 1.5)
+                ref = rt.score(*a, current_defocus=cur, step=step)
+                got = g.score(px, py, *a, current_defocus=cur, step=step)
+                assert got[:2] == ref[:2], (q, a, got, ref)
+                worst = max(worst, abs(got[2] - ref[2]) / abs(ref[2]))
+        print(f"astigmatic TemplateScore max rel value error {worst:.2e}", flush=True)
+        assert worst <= 2e-5
+    finally:
+        g.close()
+
+
 def test_peak_list(c0):
     print("case: peak list 256 x 192", flush=True)
     rng = np.random.default_rng(3)
@@ -145,7 +170,7 @@ def _compare(got, ref, out_got=None, out_ref=None):
                 assert np.array_equal(a, b), oname
 
 
-@pytest.mark.parametrize("case", ["angles", "defocus", "padded", "shared"])
+@pytest.mark.parametrize("case", ["angles", "defocus", "padded", "shared", "astigmatic"])
 def test_refine_matches_oracle(c0, case):
     print(f"case: refine config0 planted peaks, {case}", flush=True)
     kw = dict(angular_step=1.0, in_plane_angular_step=1.0, wanted_threshold=5.0)
@@ -153,6 +178,9 @@ def test_refine_matches_oracle(c0, case):
         kw.update(defocus_search_range=200.0, defocus_search_step=50.0)
     if case == "padded":
         kw.update(padding=2.0, angular_step=1.5, in_plane_angular_step=1.5)
+    if case == "astigmatic":
+        kw.update(defocus1=14200.0, defocus2=15800.0, defocus_angle=-40.0, phase_shift_deg=10.0,
+                  defocus_search_range=300.0, defocus_search_step=25.0)
     rp = O.RTParams(**kw)
     shared = case == "shared"
     rt, g = _pair(c0, rp, shared_local_maps=shared)
