@@ -25,7 +25,7 @@ I64P = ctypes.POINTER(ctypes.c_longlong)
 
 # Every symbol include/tm_hip.h declares.
 EXPORTS = ["tm_device_count", "tm_create", "tm_destroy", "tm_init", "tm_run_inner_loop", "tm_get_results", "tm_merge",
-           "tm_reset", "tm_finalize", "tm_debug_projection", "tm_debug_ccf", "tm_set_timing", "tm_get_timing",
+           "tm_set_template", "tm_reset", "tm_finalize", "tm_debug_projection", "tm_debug_ccf", "tm_set_timing", "tm_get_timing",
            "tm_last_error"]
 
 PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.c_longlong, ctypes.c_void_p)
@@ -67,6 +67,7 @@ def lib():
         L.tm_get_results.argtypes = [vp, F32P, F32P, F32P, F32P, F32P, F32P, F64P, F64P, I32P, I64P, I64P]
         L.tm_reset.argtypes = [vp]
         L.tm_merge.argtypes = [vp, vp]
+        L.tm_set_template.argtypes = [vp, F32P]
         L.tm_finalize.argtypes = [vp, ctypes.c_longlong, F32P, F32P, F32P, F32P]
         L.tm_debug_projection.argtypes = [vp, F32P, ctypes.c_int, ctypes.c_int, F32P]
         L.tm_debug_ccf.argtypes = [vp, F32P, ctypes.c_int, ctypes.c_int, F32P]
@@ -178,6 +179,14 @@ class TemplateMatchingCore:
         """Folds another handle's results (same search, other position range, same device) into
         this one (tm_merge; AggregatedTemplateResult::AddResult, match_template.cpp:2015-2103)."""
         _check(lib().tm_merge(self.h, other.h))
+
+    def set_template(self, template_spectrum: np.ndarray) -> None:
+        """Replaces the template spectrum for the next pixel-size plane; the maps keep accumulating
+        (the reference's per-plane template copy, match_template.cpp:883-886)."""
+        tpl = np.ascontiguousarray(template_spectrum, np.complex64)
+        if tpl.shape != (self.P, self.P, self.P // 2 + 1):
+            raise ValueError(f"template shape {tpl.shape} != {(self.P, self.P, self.P // 2 + 1)}")
+        _check(lib().tm_set_template(self.h, _f32(tpl.view(np.float32))))
 
     def reset(self) -> None:
         _check(lib().tm_reset(self.h))

@@ -484,6 +484,15 @@ int tm_init(tm_core* h, const tm_init_params* p, const float* template_spectrum,
   });
 }
 
+int tm_set_template(tm_core* h, const float* template_spectrum) {
+  return guarded([&] {
+    require_ready(h);
+    TM_REQUIRE(template_spectrum, TM_ERR_INVALID, "template_spectrum is NULL");
+    TM_HIP(hipMemcpyAsync(h->tpl.p, template_spectrum, h->tpl.n * sizeof(cf), hipMemcpyHostToDevice, h->stream));
+    TM_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
 int tm_reset(tm_core* h) {
   return guarded([&] {
     require_ready(h);

@@ -197,6 +197,123 @@ static __global__ void k_tpl_to_bricks(const cf* __restrict__ src, int P, cf* __
   }
 }
 
+// ---------------------------------------------------------------- pixel-size change of a volume
+// Bluestein transform of any length N <= M/2 along strided lines (one block per line, lines laid out
+// as k_cols): X_k = sum_n x_n exp(s 2 pi i n k / N) = c_k (a * b)_k with a_n = x_n c_n, c_n =
+// exp(s i pi n^2 / N), b_m = conj(c_|m|) for |m| < N; the circular convolution of length M runs as
+// two M-point FFTs. chirp holds c_n (n < N); bhat holds FFT_M(b) / M.
+template <int M, int NT>
+__global__ __launch_bounds__(NT) void k_bluestein_cols(cf* __restrict__ data, int N, int ncols, size_t bstride,
+                                                       const cf* __restrict__ chirp, const cf* __restrict__ bhat,
+                                                       const cf* __restrict__ tw) {
+  __shared__ cf buf[lds_len(M)];
+  const int item = xcd_row(blockIdx.x, gridDim.x), col = item % ncols, b = item / ncols, tid = threadIdx.x;
+  cf* base = data + (size_t)b * bstride + col;
+  for (int e = tid; e < M; e += NT) buf[lpad(e)] = e < N ? cmul(base[(size_t)e * ncols], chirp[e]) : cmake(0.f, 0.f);
+  __syncthreads();
+  fft_inplace<M, -1, NT>(buf, tw, tid);
+  for (int e = tid; e < M; e += NT) buf[lpad(e)] = cmul(buf[lpad(e)], bhat[e]);
+  __syncthreads();
+  fft_inplace<M, +1, NT>(buf, tw, tid);
+  for (int e = tid; e < N; e += NT) base[(size_t)e * ncols] = cmul(buf[lpad(e)], chirp[e]);
+}
+
+// C2R of rows of any even length N <= M/2 (FFTW c2r semantics: the imaginary parts of the DC and
+// Nyquist terms are ignored): the Hermitian extension of the N/2+1 inputs, then the inverse Bluestein
+// transform above (chirp and bhat for s = +1), real parts out. Rows as k_rows_c2r; in place allowed.
+template <int M, int NT>
+__global__ __launch_bounds__(NT) void k_bluestein_c2r_rows(const cf* in, int N, size_t ibs, int irs, float* out,
+                                                           size_t obs, int ors, const cf* __restrict__ chirp,
+                                                           const cf* __restrict__ bhat, const cf* __restrict__ tw) {
+  __shared__ cf buf[lds_len(M)];
+  const int row = blockIdx.x, b = blockIdx.y, tid = threadIdx.x, h = N / 2;
+  const cf* src = in + (size_t)b * ibs + (size_t)row * irs;
+  for (int e = tid; e < M; e += NT) {
+    cf v = cmake(0.f, 0.f);
+    if (e < N) {
+      cf X = e <= h ? src[e] : cconj(src[N - e]);
+      if (e == 0 || e == h) X.y = 0.f;
+      v = cmul(X, chirp[e]);
+    }
+    buf[lpad(e)] = v;
+  }
+  __syncthreads();
+  fft_inplace<M, -1, NT>(buf, tw, tid);
+  for (int e = tid; e < M; e += NT) buf[lpad(e)] = cmul(buf[lpad(e)], bhat[e]);
+  __syncthreads();
+  fft_inplace<M, +1, NT>(buf, tw, tid);
+  float* dst = out + (size_t)b * obs + (size_t)row * ors;
+  for (int e = tid; e < N; e += NT) dst[e] = cmul(buf[lpad(e)], chirp[e]).x;
+}
+
+// x *= k over n complex values (the ForwardFFT scale of a large spectrum).
+static __global__ void k_scale_cf(cf* __restrict__ x, size_t n, float k) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+    x[e] = cmake(x[e].x * k, x[e].y * k);
+}
+
+// Image::ClipInto, real space, centred (image.cpp:8556-8586), between cubes in the padded layout:
+// out (edge b) from in (edge a), padding value outside.
+static __global__ void k_real_clip3(const float* __restrict__ in, int a, float* __restrict__ out, int b, float pad) {
+  const size_t n = (size_t)b * b * (b + 2);
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    const int ii = (int)(e % (b + 2));
+    const size_t r = e / (b + 2);
+    const int jj = (int)(r % b), kk = (int)(r / b);
+    float v = 0.0f;
+    if (ii < b) {
+      const int i = a / 2 + (ii - b / 2), j = a / 2 + (jj - b / 2), k = a / 2 + (kk - b / 2);
+      v = (i < 0 || i >= a || j < 0 || j >= a || k < 0 || k >= a) ? pad : in[((size_t)k * a + j) * (a + 2) + i];
+    }
+    out[e] = v;
+  }
+}
+
+// Image::ClipInto, Fourier space (image.cpp:8588-8620), between cubic half spectra (edges a -> b,
+// both even): out's physical (ii, jj, kk) takes in's value at the same logical coordinate, 0 outside
+// in's logical bounds.
+static __global__ void k_fourier_clip3(const cf* __restrict__ in, int a, cf* __restrict__ out, int b) {
+  const int ah = a / 2 + 1, bh = b / 2 + 1;
+  const size_t n = (size_t)bh * b * b;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    const int ii = (int)(e % bh);
+    const size_t r = e / bh;
+    const int jj = (int)(r % b), kk = (int)(r / b);
+    const int ly = jj >= b / 2 ? jj - b : jj, lz = kk >= b / 2 ? kk - b : kk;
+    cf v = cmake(0.f, 0.f);
+    if (ii <= a / 2 && ly >= -a / 2 && ly <= a / 2 - 1 && lz >= -a / 2 && lz <= a / 2 - 1) {
+      const int py = ly >= 0 ? ly : a + ly, pz = lz >= 0 ? lz : a + lz;
+      v = in[((size_t)pz * a + py) * ah + ii];
+    }
+    out[e] = v;
+  }
+}
+
+// The Nyquist planes ClipInto fills in when a half spectrum grows (image.cpp:8626-8690), step 1: the
+// positive y Nyquist row of every logical z of the smaller cube takes the negative one.
+static __global__ void k_fourier_grow_y(const cf* __restrict__ in, int a, cf* __restrict__ out, int b) {
+  const int ah = a / 2 + 1, bh = b / 2 + 1;
+  const int n = ah * a;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    const int ii = e % ah, kl = e / ah - a / 2;
+    const int pzo = kl >= 0 ? kl : b + kl, pzi = kl >= 0 ? kl : a + kl;
+    out[((size_t)pzo * b + a / 2) * bh + ii] = in[((size_t)pzi * a + a / 2) * ah + ii];
+  }
+}
+
+// Step 2: the positive z Nyquist plane (physical a/2 of the larger cube) takes the rows of the
+// negative one (physical b - a/2) for physical y in 0..a/2 and b-a/2..b-1.
+static __global__ void k_fourier_grow_z(cf* __restrict__ out, int a, int b) {
+  const int ah = a / 2 + 1, bh = b / 2 + 1;
+  const int rows = a + 1;  // jj = 0..a/2 and b-a/2..b-1
+  const int n = ah * rows;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    const int ii = e % ah, q = e / ah;
+    const int jj = q <= a / 2 ? q : b - a / 2 + (q - a / 2 - 1);
+    out[((size_t)(a / 2) * b + jj) * bh + ii] = out[((size_t)(b - a / 2) * b + jj) * bh + ii];
+  }
+}
+
 // ---------------------------------------------------------------- small line transforms
 // Complex FFTs of length N down columns of [b][N][ncols] complex arrays (row stride ncols).
 // MODE 0: one transform of sign SIGN. MODE 1: forward, multiply by scale * filt[row][col], inverse

@@ -2,6 +2,8 @@
 // the arguments of TemplateMatchingCore::Init / RunInnerLoop), with the large transforms on the GPU:
 //   tm_prepare_sizes / tm_prepare_input_image   match_template.cpp:490-598, 703-724
 //   tm_prepare_template                         match_template.cpp:621-625, 837-845
+//   tm_prepare_template_scaled                  match_template.cpp:837-845 with ChangePixelSize != 1
+//                                               (image.cpp:8705-8792)
 //   tm_projection_filter                        match_template.cpp:893-898 (CTF x whitening)
 //   tm_prepare_refine_image                     refine_template.cpp:526-546 (whitened real-space image)
 //   tm_search_grid / tm_psi_list                euler_search.cpp:180-282, 353-443; match_template.cpp:658-697, 1001
@@ -287,6 +289,176 @@ Img backward_fft(int device, const std::vector<std::complex<float>>& spec, int n
   return out;
 }
 
+// ---------------------------------------------------------------- transforms of any length
+// Bluestein tables for length N and sign s on the device: chirp c_n = exp(s i pi n^2 / N) (angles
+// from n^2 mod 2N in long double) and bhat = FFT_M(b) / M, b_m = conj(c_|m|) for |m| < N.
+struct Bluestein {
+  int N = 0, M = 0;
+  cf* chirp = nullptr;
+  cf* bhat = nullptr;
+};
+Bluestein bluestein(Dev& d, int N, int sign) {
+  Bluestein b;
+  b.N = N;
+  b.M = 1;
+  while (b.M < 2 * N - 1) b.M *= 2;
+  P_REQ(b.M == 2048 || b.M == 4096, TM_ERR_UNSUPPORTED, "transform length " + std::to_string(N) + " not supported");
+  const long double pi = acosl(-1.0L);
+  std::vector<cf> c(N), w(b.M, cf{0.f, 0.f});
+  for (long n = 0; n < N; ++n) {
+    const long double a = sign * pi * (long double)((n * n) % (2L * N)) / (long double)N;
+    c[n].x = (float)cosl(a);
+    c[n].y = (float)sinl(a);
+  }
+  for (int m = 0; m < N; ++m) {
+    const cf cc{c[m].x, -c[m].y};
+    w[m] = cc;
+    if (m > 0) w[b.M - m] = cc;
+  }
+  b.chirp = d.alloc<cf>(N);
+  b.bhat = d.alloc<cf>(b.M);
+  P_HIP(hipMemcpy(b.chirp, c.data(), N * sizeof(cf), hipMemcpyHostToDevice));
+  P_HIP(hipMemcpy(b.bhat, w.data(), b.M * sizeof(cf), hipMemcpyHostToDevice));
+  cols(d, b.M, b.bhat, 1, 0, 1, -1);
+  std::vector<cf> h(b.M);
+  P_HIP(hipStreamSynchronize(d.s));
+  P_HIP(hipMemcpy(h.data(), b.bhat, b.M * sizeof(cf), hipMemcpyDeviceToHost));
+  const float inv = (float)(1.0 / b.M);
+  for (auto& v : h) { v.x *= inv; v.y *= inv; }
+  P_HIP(hipMemcpy(b.bhat, h.data(), b.M * sizeof(cf), hipMemcpyHostToDevice));
+  return b;
+}
+// Complex transforms of length N and sign s down columns (k_cols layout), any N.
+void cols_any(Dev& d, int N, cf* data, int ncols, size_t bstride, int nb, int sign) {
+  if (line_size_compiled(N)) { cols(d, N, data, ncols, bstride, nb, sign); return; }
+  Bluestein b = bluestein(d, N, sign);
+  const dim3 g((unsigned)((size_t)ncols * nb));
+  if (b.M == 2048)
+    hipLaunchKernelGGL((tmk::k_bluestein_cols<2048, 256>), g, dim3(256), 0, d.s, data, N, ncols, bstride, b.chirp, b.bhat,
+                       d.twiddles(2048));
+  else
+    hipLaunchKernelGGL((tmk::k_bluestein_cols<4096, 256>), g, dim3(256), 0, d.s, data, N, ncols, bstride, b.chirp, b.bhat,
+                       d.twiddles(4096));
+  chk("k_bluestein_cols");
+}
+// C2R rows of any even length N.
+void c2r_rows_any(Dev& d, int N, const cf* in, size_t ibs, int irs, float* out, size_t obs, int ors, int nrows, int nb) {
+  if (line_size_compiled(N)) { rows_c2r(d, N, in, ibs, irs, out, obs, ors, nrows, nb); return; }
+  P_REQ(N % 2 == 0, TM_ERR_UNSUPPORTED, "odd C2R length");
+  Bluestein b = bluestein(d, N, +1);
+  if (b.M == 2048)
+    hipLaunchKernelGGL((tmk::k_bluestein_c2r_rows<2048, 256>), dim3(nrows, nb), dim3(256), 0, d.s, in, N, ibs, irs, out, obs,
+                       ors, b.chirp, b.bhat, d.twiddles(2048));
+  else
+    hipLaunchKernelGGL((tmk::k_bluestein_c2r_rows<4096, 256>), dim3(nrows, nb), dim3(256), 0, d.s, in, N, ibs, irs, out, obs,
+                       ors, b.chirp, b.bhat, d.twiddles(4096));
+  chk("k_bluestein_c2r_rows");
+}
+
+// Image::ReturnAverageOfRealValues with a mask radius (image.cpp:6263-6322): the mean of the voxels
+// whose squared distance from the box centre is within the squared radius.
+float masked_average(const Img& a, float radius) {
+  double sum = 0.0;
+  long n = 0;
+  const float r2 = powf(radius, 2);
+  for (int k = 0; k < a.nz; ++k) {
+    const float z = powf(k - a.nz / 2, 2);
+    for (int j = 0; j < a.ny; ++j) {
+      const float y = powf(j - a.ny / 2, 2);
+      for (int i = 0; i < a.nx; ++i) {
+        const float x = powf(i - a.nx / 2, 2);
+        if (x + y + z <= r2) { sum += a.at(i, j, k); ++n; }
+      }
+    }
+  }
+  return n > 0 ? float(sum / n) : 0.0f;
+}
+
+// Image::ChangePixelSize(other of the same size, factor, tolerance, return_fft = true)
+// (image.cpp:8705-8792) of a real cube on the device: pad to the intermediate size with the masked
+// average, forward transform (scaled), crop or pad the spectrum to the new size (with the Nyquist
+// planes ClipInto fills in), inverse transform, clip back to the cube's size with the same padding
+// value, forward transform (scaled). Returns the half spectrum.
+std::vector<std::complex<float>> change_pixel_size_cube(int device, const Img& vol, float factor, float tol) {
+  const int P = vol.nx;
+  const int min_dimension = P;
+  int pad, nw;
+  float padding_value = 0.0f;
+  if (tol < 1.0f / (min_dimension * factor)) {
+    const float inter = 1.0f / (tol * min_dimension);
+    pad = P * inter;
+    if (even(P) != even(pad)) pad++;
+    nw = int(pad / factor);
+    if (even(P) != even(nw)) nw++;
+  } else {
+    pad = P * factor;
+    if (even(P) != even(pad)) pad++;
+    nw = P;
+  }
+  P_REQ(line_size_compiled(pad), TM_ERR_UNSUPPORTED,
+        "pixel-size change pads to " + std::to_string(pad) + ", a length without a compiled transform");
+  if (pad > P) padding_value = masked_average(vol, 0.45f * min_dimension);
+  Dev d(device);
+  // real cube, then the padded cube
+  float* vd = d.alloc<float>(vol.v.size());
+  P_HIP(hipMemcpy(vd, vol.v.data(), vol.v.size() * sizeof(float), hipMemcpyHostToDevice));
+  const size_t pad_floats = (size_t)(pad + 2) * pad * pad;
+  float* pb = d.alloc<float>(pad_floats);
+  hipLaunchKernelGGL(tmk::k_real_clip3, dim3(8192), dim3(256), 0, d.s, vd, P, pb, pad, padding_value);
+  chk("k_real_clip3");
+  const int ph = pad / 2 + 1;
+  cf* ps = d.alloc<cf>((size_t)ph * pad * pad);
+  const float inv_pad = 1. / (float)((long)pad * pad * pad);
+  rows_r2c(d, pad, pb, (size_t)(pad + 2) * pad, pad + 2, ps, (size_t)ph * pad, ph, pad, pad);
+  P_HIP(hipStreamSynchronize(d.s));
+  cols(d, pad, ps, ph, (size_t)ph * pad, pad);
+  cols(d, pad, ps, ph * pad, 0, 1);
+  // the ForwardFFT scale 1/N, applied to the spectrum
+  hipLaunchKernelGGL(tmk::k_scale_cf, dim3(8192), dim3(256), 0, d.s, ps, (size_t)ph * pad * pad, inv_pad);
+  chk("k_scale_cf");
+  // Resize in Fourier space to the new size
+  const int nh = nw / 2 + 1;
+  const size_t new_floats = (size_t)(nw + 2) * nw * nw;
+  float* nbf = d.alloc<float>(new_floats);
+  cf* ns = reinterpret_cast<cf*>(nbf);
+  hipLaunchKernelGGL(tmk::k_fourier_clip3, dim3(8192), dim3(256), 0, d.s, ps, pad, ns, nw);
+  chk("k_fourier_clip3");
+  if (nw > pad) {
+    hipLaunchKernelGGL(tmk::k_fourier_grow_y, dim3((ph * pad + 255) / 256), dim3(256), 0, d.s, ps, pad, ns, nw);
+    chk("k_fourier_grow_y");
+    hipLaunchKernelGGL(tmk::k_fourier_grow_z, dim3((ph * (pad + 1) + 255) / 256), dim3(256), 0, d.s, ns, pad, nw);
+    chk("k_fourier_grow_z");
+  }
+  std::vector<std::complex<float>> out;
+  if (nw == P) {
+    // the new size is the cube's: the cropped spectrum is the result (other_image->CopyFrom)
+    P_HIP(hipStreamSynchronize(d.s));
+    out.resize((size_t)nh * nw * nw);
+    P_HIP(hipMemcpy(out.data(), ns, out.size() * sizeof(cf), hipMemcpyDeviceToHost));
+    return out;
+  }
+  // BackwardFFT at the new size (z, y columns, then the C2R rows)
+  cols_any(d, nw, ns, nh * nw, 0, 1, +1);
+  cols_any(d, nw, ns, nh, (size_t)nh * nw, nw, +1);
+  c2r_rows_any(d, nw, ns, (size_t)nh * nw, nh, nbf, (size_t)(nw + 2) * nw, nw + 2, nw, nw);
+  // ClipInto the cube's size with the padding value, then ForwardFFT
+  const int Ph = P / 2 + 1;
+  float* cb = d.alloc<float>((size_t)(P + 2) * P * P);
+  hipLaunchKernelGGL(tmk::k_real_clip3, dim3(8192), dim3(256), 0, d.s, nbf, nw, cb, P, padding_value);
+  chk("k_real_clip3");
+  cf* cs = d.alloc<cf>((size_t)Ph * P * P);
+  rows_r2c(d, P, cb, (size_t)(P + 2) * P, P + 2, cs, (size_t)Ph * P, Ph, P, P);
+  P_HIP(hipStreamSynchronize(d.s));
+  cols(d, P, cs, Ph, (size_t)Ph * P, P);
+  cols(d, P, cs, Ph * P, 0, 1);
+  P_HIP(hipStreamSynchronize(d.s));
+  out.resize((size_t)Ph * P * P);
+  P_HIP(hipMemcpy(out.data(), cs, out.size() * sizeof(cf), hipMemcpyDeviceToHost));
+  const float inv_n = 1. / (float)((long)P * P * P);
+  for (auto& c : out) c *= inv_n;
+  return out;
+}
+
 // Image::SwapRealSpaceQuadrants on a non-centred even-size spectrum (phase shift n/2 per axis,
 // image.cpp:9221-9268, functions.h:494-504), applied in place.
 void swap_quadrants(std::complex<float>* c, int nx, int ny, int nz) {
@@ -495,6 +667,32 @@ int tm_prepare_template(int device, const float* volume, int t, float padding, f
     auto spec = forward_fft(device, v);
     const float inv_n = 1. / (float)((long)P * P * P);
     for (auto& c : spec) c *= inv_n;
+    spec[0] = 0.f;
+    swap_quadrants(spec.data(), P, P, P);
+    std::memcpy(spectrum, spec.data(), spec.size() * sizeof(cf));
+  });
+}
+
+int tm_prepare_template_scaled(int device, const float* volume, int t, float padding, float pixel_size_factor,
+                               float* spectrum, int* P_out) {
+  return pguard([&] {
+    P_REQ(volume && t > 0 && pixel_size_factor > 0.0f, TM_ERR_INVALID, "bad arguments");
+    const int P = padding != 1.0f ? int(t * padding) : t;
+    if (P_out) *P_out = P;
+    if (!spectrum) return;
+    Img v;
+    v.alloc(t, t, t);
+    for (int z = 0; z < t; ++z)
+      for (int y = 0; y < t; ++y) std::memcpy(&v.at(0, y, z), volume + ((size_t)z * t + y) * t, t * sizeof(float));
+    if (P != t) v = clip_into(v, P, P, P, edge_mean(v));
+    std::vector<std::complex<float>> spec;
+    if (fabsf(pixel_size_factor - 1.0f) < 0.001f) {
+      spec = forward_fft(device, v);
+      const float inv_n = 1. / (float)((long)P * P * P);
+      for (auto& c : spec) c *= inv_n;
+    } else {
+      spec = change_pixel_size_cube(device, v, pixel_size_factor, 0.001f);
+    }
     spec[0] = 0.f;
     swap_quadrants(spec.data(), P, P, P);
     std::memcpy(spectrum, spec.data(), spec.size() * sizeof(cf));

@@ -12,9 +12,10 @@
   X(256) X(270) X(324) X(384) X(512) X(1024) X(1536) X(2048) X(2592) X(2916) X(3072) X(4096)
 #define TM_TEMPLATE_SIZES(X) X(32) X(64) X(128) X(256) X(512) X(1024)
 // every line length a row or column transform of the input image, the template or the projection
-// takes (the union of the two lists)
+// takes (the union of the two lists), and 1000, the intermediate edge of a template's pixel-size change
+// (1 / tolerance 0.001)
 #define TM_LINE_SIZES(X) \
-  X(32) X(64) X(128) X(256) X(270) X(324) X(384) X(512) X(1024) X(1536) X(2048) X(2592) X(2916) X(3072) X(4096)
+  X(32) X(64) X(128) X(256) X(270) X(324) X(384) X(512) X(1000) X(1024) X(1536) X(2048) X(2592) X(2916) X(3072) X(4096)
 
 namespace tmk {
 

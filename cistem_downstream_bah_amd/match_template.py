@@ -30,6 +30,8 @@ def _lib():
         L.tm_prepare_input_image.argtypes = [ctypes.c_int, F32P, ctypes.c_int, ctypes.c_int, ctypes.c_int, F32P, F32P,
                                              F32P]
         L.tm_prepare_template.argtypes = [ctypes.c_int, F32P, ctypes.c_int, ctypes.c_float, F32P, ip]
+        L.tm_prepare_template_scaled.argtypes = [ctypes.c_int, F32P, ctypes.c_int, ctypes.c_float, ctypes.c_float, F32P,
+                                                 ip]
         L.tm_projection_filter.argtypes = [ctypes.c_int, F32P, F32P, ctypes.c_int] + [ctypes.c_float] * 8 + [F32P]
         L.tm_search_grid.argtypes = [ctypes.c_char, ctypes.c_int, ctypes.c_float, F32P, ctypes.c_int]
         L.tm_psi_list.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float, F32P, ctypes.c_int]
@@ -46,8 +48,8 @@ def _f(a):
     return a.ctypes.data_as(F32P)
 
 
-PREPARE_EXPORTS = ["tm_prepare_sizes", "tm_prepare_input_image", "tm_prepare_template", "tm_projection_filter",
-                   "tm_search_grid", "tm_psi_list", "tm_prepare_last_error"]
+PREPARE_EXPORTS = ["tm_prepare_sizes", "tm_prepare_input_image", "tm_prepare_template", "tm_prepare_template_scaled",
+                   "tm_projection_filter", "tm_search_grid", "tm_psi_list", "tm_prepare_last_error"]
 
 
 def prepare_sizes(mx: int, my: int, template_size: int):
@@ -70,14 +72,20 @@ def prepare_input_image(micrograph: np.ndarray, template_size: int, device: int 
     return spec, cx, cy, W, H, rot, maxpad
 
 
-def prepare_template(volume: np.ndarray, padding: float, device: int = 0) -> np.ndarray:
+def prepare_template(volume: np.ndarray, padding: float, device: int = 0, pixel_size_factor: float = 1.0) -> np.ndarray:
+    """(P, P, P/2+1) complex64 template spectrum (match_template.cpp:621-625, 837-845); with
+    pixel_size_factor != 1 the padded volume is first resampled by ChangePixelSize (image.cpp:8705-8792)."""
     v = np.ascontiguousarray(volume, np.float32)
     t = v.shape[0]
     P = ctypes.c_int(0)
-    _check(_lib().tm_prepare_template(device, _f(v), t, padding, None, ctypes.byref(P)))
-    P = P.value
-    spec = np.zeros((P, P, P // 2 + 1), np.complex64)
-    _check(_lib().tm_prepare_template(device, _f(v), t, padding, _f(spec.view(np.float32)), None))
+    L = _lib()
+    _check(L.tm_prepare_template(device, _f(v), t, padding, None, ctypes.byref(P)))
+    spec = np.zeros((P.value, P.value, P.value // 2 + 1), np.complex64)
+    if pixel_size_factor == 1.0:
+        _check(L.tm_prepare_template(device, _f(v), t, padding, _f(spec.view(np.float32)), None))
+    else:
+        _check(L.tm_prepare_template_scaled(device, _f(v), t, padding, pixel_size_factor, _f(spec.view(np.float32)),
+                                            None))
     return spec
 
 
@@ -122,6 +130,8 @@ class SearchParams:
     in_plane_angular_step: float = 10.0
     defocus_search_range: float = 0.0
     defocus_step: float = 0.0
+    pixel_size_search_range: float = 0.0
+    pixel_size_step: float = 0.0
     padding: float = 1.0
     symmetry: str = "C1"
     first_search_position: int = -1
@@ -156,6 +166,13 @@ class MatchTemplateSearch:
         rng = p.defocus_search_range if p.defocus_step > 0 else 0.0
         r = rng / self.defocus_step
         self.n_def_half = int(r + 0.5) if r > 0 else int(r - 0.5)
+        # match_template.cpp:755-762: no pixel-size search without a positive step
+        self.pixel_size_step = p.pixel_size_step if p.pixel_size_step > 0 else 0.02
+        prng = p.pixel_size_search_range if p.pixel_size_step > 0 else 0.0
+        r = float(np.float32(prng) / np.float32(self.pixel_size_step))
+        self.n_size_half = int(r + 0.5) if r > 0 else int(r - 0.5)
+        self.volume = volume
+        self.device = device
         sq = math.sqrt(float(self.W * self.H))
         step = (self.histogram_max - self.histogram_min) / float(self.histogram_number_of_points)
         step = float(np.float32(step))
@@ -164,7 +181,7 @@ class MatchTemplateSearch:
         self.core = core.TemplateMatchingCore(device)
         self.core.init(self.template_spectrum, self.input_spectrum, T, self.grid, self.psi, self.first, self.last,
                        self.histogram_min_scaled, self.histogram_step_scaled, 512, self.max_padding,
-                       self.defocus_step, 0.02, p.batch_size, p.plane_storage)
+                       self.defocus_step, self.pixel_size_step, p.batch_size, p.plane_storage)
 
     def filter_for(self, defocus_i: int) -> np.ndarray:
         p = self.p
@@ -174,10 +191,20 @@ class MatchTemplateSearch:
                                  p.amplitude_contrast, float(d1), float(d2), p.defocus_angle, p.pixel_size,
                                  p.phase_shift_deg)
 
+    def size_factor(self, size_i: int) -> float:
+        """(pixel_size + size_i * pixel_size_step) / pixel_size in float (match_template.cpp:842)."""
+        f32 = np.float32
+        return float((f32(self.p.pixel_size) + f32(size_i) * f32(self.pixel_size_step)) / f32(self.p.pixel_size))
+
     def run(self, progress=None) -> int:
+        """The pixel-size and defocus loops of match_template.cpp:837-914: each pixel-size plane's
+        template replaces the previous one in the same core, whose maps keep accumulating."""
         n = 0
-        for di in range(-self.n_def_half, self.n_def_half + 1):
-            n += self.core.run_inner_loop(self.filter_for(di), 0, di, progress)
+        for si in range(-self.n_size_half, self.n_size_half + 1):
+            if self.n_size_half:
+                self.core.set_template(prepare_template(self.volume, self.p.padding, self.device, self.size_factor(si)))
+            for di in range(-self.n_def_half, self.n_def_half + 1):
+                n += self.core.run_inner_loop(self.filter_for(di), si, di, progress)
         return n
 
     @property

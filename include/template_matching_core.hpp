@@ -88,6 +88,11 @@ class TemplateMatchingCore {
       for (int i = 0; i < n_bins_; ++i) histogram_to_add[i] += (long)hist[i];
   }
 
+  // The next pixel-size plane's template (same size and layout as at Init); the maps keep
+  // accumulating, as after the reference's GPU[tIDX].template_reconstruction.CopyFrom
+  // (match_template.cpp:883-886).
+  void SetTemplate(const float* template_spectrum) { check(tm_set_template(h_, template_spectrum)); }
+
   // Folds another core's results into this one (same search, another position range, same device),
   // as AggregatedTemplateResult::AddResult merges job results (match_template.cpp:2015-2103).
   void Merge(TemplateMatchingCore& other) {

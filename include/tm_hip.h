@@ -98,6 +98,11 @@ int tm_run_inner_loop(tm_core* h, const float* projection_filter, int size_i, in
 int tm_get_results(tm_core* h, float* mip, float* psi, float* theta, float* phi, float* defocus, float* pixel_size,
                    double* sum, double* sum_of_squares, int* best_index, long long* histogram, long long* n_ccfs);
 
+/* Replaces the template spectrum (same P, layout as in tm_init) and keeps the accumulated maps, for
+ * the next pixel-size plane: the reference copies each plane's template into its TemplateMatchingCore
+ * (match_template.cpp:837-845, 883-886). */
+int tm_set_template(tm_core* h, const float* template_spectrum);
+
 /* Clears the accumulated maps (MIP to -FLT_MAX, sums and histogram to 0). */
 int tm_reset(tm_core* h);
 

@@ -26,6 +26,15 @@ int tm_prepare_input_image(int device, const float* micrograph, int image_x, int
  * quadrant-swapped (match_template.cpp:621-625, 837-845). With spectrum == NULL only *P is set. */
 int tm_prepare_template(int device, const float* volume, int t, float padding, float* spectrum, int* P);
 
+/* The template spectrum of one pixel-size plane: as tm_prepare_template, with the padded volume
+ * resampled by Image::ChangePixelSize(pixel_size_factor, tolerance 0.001) before the transform
+ * (match_template.cpp:837-845, image.cpp:8705-8792); pixel_size_factor = (pixel_size + size_i *
+ * pixel_size_step) / pixel_size. A factor within 0.001 of 1 gives tm_prepare_template's result. The
+ * intermediate edge (1000 for P * factor < 1000) must have a compiled transform, else
+ * TM_ERR_UNSUPPORTED; the new edge may be any even length up to 2048. */
+int tm_prepare_template_scaled(int device, const float* volume, int t, float padding, float pixel_size_factor,
+                               float* spectrum, int* P);
+
 /* CTF x whitening projection filter, T x (T/2+1) complex (match_template.cpp:893-898). Defocus
  * values in Angstrom (defocus1/2 already include defocus_i * defocus_step). */
 int tm_projection_filter(int T, const float* curve_x, const float* curve_y, int n_curve, float voltage_kV,

@@ -58,7 +58,13 @@ def lib():
         L.tmo_mt_info.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_long), F64P]
         L.tmo_mt_get.argtypes = [vp, ctypes.c_int, ctypes.c_int, F32P]
         L.tmo_mt_projection_filter.argtypes = [vp, ctypes.c_int, ctypes.c_int, F32P]
-        L.tmo_mt_projection_ccf.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, F32P, F32P]
+        L.tmo_mt_projection_ccf.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, F32P,
+                                             F32P]
+        L.tmo_mt_template_for_size.argtypes = [vp, ctypes.c_int, ctypes.c_int, F32P]
+        L.tmo_change_pixel_size.argtypes = [F32P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                            ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int, F32P, ctypes.POINTER(ctypes.c_int)]
+        L.tmo_set_threads.argtypes = [ctypes.c_int]
         L.tmo_mt_run.argtypes = [vp] + [ctypes.c_int] * 8
         L.tmo_mt_results.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
         L.tmo_mt_results.restype = ctypes.c_long
@@ -93,6 +99,37 @@ class RefinedPeak(ctypes.Structure):
 
 def peaks_to_dicts(arr) -> list:
     return [{n: getattr(p, n) for n, _ in RefinedPeak._fields_ if n != "pad_"} for p in arr]
+
+
+def set_threads(n: int) -> None:
+    """Worker threads of the oracle's large transforms (results do not depend on it)."""
+    lib().tmo_set_threads(int(n))
+
+
+def change_pixel_size(img: np.ndarray, factor: float, out_shape=None, tolerance: float = 0.001,
+                      return_fft: bool = False, fp64: bool = False):
+    """Image::ChangePixelSize (image.cpp:8705-8792) of a dense real 2-D (ny, nx) or 3-D (nz, ny, nx)
+    image into an image of out_shape (default: the input's). Returns the dense real result, or the
+    (.., n/2+1) complex spectrum with return_fft; and the (pad, new) edge sizes used."""
+    a = np.ascontiguousarray(img, np.float32)
+    shp = a.shape if a.ndim == 3 else (1,) + a.shape
+    out_shape = tuple(out_shape or a.shape)
+    oshp = out_shape if len(out_shape) == 3 else (1,) + out_shape
+    nz, ny, nx = shp
+    oz, oy, ox = oshp
+    pad = 2 if ox % 2 == 0 else 1
+    out = np.zeros(oz * oy * (ox + pad), np.float32)
+    dims = (ctypes.c_int * 6)()
+    _check(lib().tmo_change_pixel_size(_fp(a), nx, ny, nz, factor, tolerance, ox, oy, oz, int(return_fft), int(fp64),
+                                       _fp(out), dims))
+    o = out.reshape(oz, oy, ox + pad)
+    if return_fft:
+        res = o.view(np.complex64)
+    else:
+        res = np.ascontiguousarray(o[..., :ox])
+    if len(out_shape) == 2:
+        res = res[0]
+    return res, (tuple(dims[:3]), tuple(dims[3:]))
 
 
 def _fp(a):
@@ -266,6 +303,7 @@ class MTInfo:
     histogram_step_scaled: float
     defocus_step: float
     pixel_size_step: float
+    n_size_half: int = 0
 
 
 class MatchTemplateOracle:
@@ -286,6 +324,7 @@ class MatchTemplateOracle:
         L.tmo_mt_info(self.h, self.fp64, info.ctypes.data_as(ctypes.POINTER(ctypes.c_long)),
                       dinfo.ctypes.data_as(F64P))
         self.info = MTInfo(*[int(x) for x in info[:14]], *[float(x) for x in dinfo[:5]])
+        self.info.n_size_half = int(info[14])
         self.info.rotated = bool(self.info.rotated)
         self.params = params
 
@@ -330,11 +369,18 @@ class MatchTemplateOracle:
         _check(lib().tmo_mt_projection_filter(self.h, self.fp64, defocus_i, _fp(out.view(np.float32))))
         return out
 
-    def projection_and_ccf(self, pos: int, psi_idx: int, defocus_i: int = 0, want_ccf: bool = True):
+    def template_for_size(self, size_i: int) -> np.ndarray:
+        """(P, P, P/2+1) complex64 template spectrum of pixel-size plane size_i (match_template.cpp:837-845)."""
+        P = self.info.P
+        out = np.zeros(P * P * (P + 2), np.float32)
+        _check(lib().tmo_mt_template_for_size(self.h, self.fp64, size_i, _fp(out)))
+        return out.view(np.complex64).reshape(P, P, P // 2 + 1)
+
+    def projection_and_ccf(self, pos: int, psi_idx: int, defocus_i: int = 0, want_ccf: bool = True, size_i: int = 0):
         I = self.info
         proj = np.zeros((I.T, I.T + 2), np.float32)
         ccf = np.zeros((I.H, I.W + 2), np.float32) if want_ccf else None
-        _check(lib().tmo_mt_projection_ccf(self.h, self.fp64, pos, psi_idx, defocus_i, _fp(proj),
+        _check(lib().tmo_mt_projection_ccf(self.h, self.fp64, pos, psi_idx, defocus_i, size_i, _fp(proj),
                                            _fp(ccf) if want_ccf else None))
         return proj[:, : I.T].copy(), (ccf[:, : I.W].copy() if want_ccf else None)
 

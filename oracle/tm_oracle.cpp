@@ -199,7 +199,8 @@ struct MatchTemplate {
   int max_padding = 0;
   bool rotated = false;
   Image<R> input_image;              // whitened, normalised, quadrant-swapped spectrum
-  Image<R> template_reconstruction;  // P^3 spectrum, DC 0, quadrant-swapped
+  Image<R> template_reconstruction;  // P^3 spectrum, DC 0, quadrant-swapped (pixel-size plane size_i = 0)
+  std::vector<Image<R>> size_templates;  // the same for every pixel-size plane, size_i = -n_size_half ..
   Curve<R> whitening_filter, number_of_terms;
   CTFModel<R> ctf;
   std::vector<float> grid;           // {phi, theta} per search position
@@ -293,15 +294,20 @@ struct MatchTemplate {
     if (pixel_size_step <= 0.0f) { pixel_size_search_range = 0.0f; pixel_size_step = 0.02f; }
     n_def_half = myroundint(float(defocus_search_range) / float(defocus_step));
     n_size_half = myroundint(float(pixel_size_search_range) / float(pixel_size_step));
-    if (n_size_half != 0) throw std::runtime_error("oracle: pixel-size search (ChangePixelSize != 1) is not restated");
     total_correlation_positions = (long)(last_pos - first_pos + 1) * (long)psi.size() * (2 * n_def_half + 1);
 
-    // template prep (match_template.cpp:837-845): ChangePixelSize(factor 1) = ClipInto + scaled FFT
-    template_reconstruction.Allocate(P, P, P, true);
-    input_reconstruction.ClipInto(&template_reconstruction);
-    template_reconstruction.ForwardFFT();
-    template_reconstruction.ZeroCentralPixel();
-    template_reconstruction.SwapRealSpaceQuadrants();
+    // template prep per pixel-size plane (match_template.cpp:837-845): ChangePixelSize by
+    // (pixel_size + size_i * step) / pixel_size with tolerance 0.001, DC 0, quadrant swap
+    size_templates.resize(2 * n_size_half + 1);
+    for (int size_i = -n_size_half; size_i <= n_size_half; ++size_i) {
+      Image<R>& tr = size_templates[size_i + n_size_half];
+      tr.Allocate(P, P, P, true);
+      input_reconstruction.ChangePixelSize(&tr, (p.pixel_size + float(size_i) * pixel_size_step) / p.pixel_size, 0.001f,
+                                           true);
+      tr.ZeroCentralPixel();
+      tr.SwapRealSpaceQuadrants();
+    }
+    template_reconstruction = size_templates[n_size_half];
   }
 
   // match_template.cpp:893-898
@@ -325,17 +331,17 @@ struct MatchTemplate {
   }
 
   // match_template.cpp:1004-1068 — normalised real-space projection (T x T, padded layout)
-  void Projection(Work& w, const Image<R>& pf, float phi, float theta, float psi_v) const {
+  void Projection(Work& w, const Image<R>& pf, float phi, float theta, float psi_v, int size_i = 0) const {
     float m[3][3];
     EulerMatrix(phi, theta, psi_v, m);
     if (p.padding != 1.0f) {
-      template_reconstruction.ExtractSlice(w.padded_projection, m);
+      size_templates[size_i + n_size_half].ExtractSlice(w.padded_projection, m);
       w.padded_projection.SwapRealSpaceQuadrants();
       w.padded_projection.BackwardFFT();
       w.padded_projection.ClipInto(&w.current_projection);
       w.current_projection.ForwardFFT();
     } else {
-      template_reconstruction.ExtractSlice(w.current_projection, m);
+      size_templates[size_i + n_size_half].ExtractSlice(w.current_projection, m);
       w.current_projection.SwapRealSpaceQuadrants();
     }
     w.current_projection.MultiplyPixelWise(pf);
@@ -439,6 +445,7 @@ struct MatchTemplate {
     if (pos_hi < 0) pos_hi = last_pos;
     InitResults(out);
     threads = std::max(1, threads);
+    for (int size_i = -n_size_half; size_i <= n_size_half; ++size_i) {
     for (int defocus_i = -n_def_half; defocus_i <= n_def_half; ++defocus_i) {
       if (defocus_i < def_lo || defocus_i > def_hi) continue;
       Image<R> pf;
@@ -456,15 +463,16 @@ struct MatchTemplate {
           InitWork(w);
           for (int pos = a; pos <= b; ++pos) {
             for (int k = psi_lo; k <= psi_hi; ++k) {
-              Projection(w, pf, grid[2 * pos], grid[2 * pos + 1], psi[k]);
+              Projection(w, pf, grid[2 * pos], grid[2 * pos + 1], psi[k], size_i);
               CCF(w);
-              Accumulate(w, part[t], pos, k, defocus_i, 0, GlobalIndex(0, defocus_i, pos, k));
+              Accumulate(w, part[t], pos, k, defocus_i, size_i, GlobalIndex(size_i, defocus_i, pos, k));
             }
           }
         });
       }
       for (auto& x : th) x.join();
       for (int t = 0; t < nt; ++t) Merge(out, part[t]);
+    }
     }
   }
 };
@@ -932,6 +940,12 @@ struct RefineOracle {
   }
 };
 
+template <class F>
+void with_mt(void* hv, int fp64, F f) {
+  if (fp64) f(((Handle64*)hv)->mt);
+  else f(((Handle32*)hv)->mt);
+}
+
 struct RHandle32 { RefineOracle<float> rt; };
 struct RHandle64 { RefineOracle<double> rt; };
 
@@ -1092,7 +1106,7 @@ void tmo_mt_info(void* hv, int fp64, long* info, double* dinfo) {
     info[0] = mt.W; info[1] = mt.H; info[2] = mt.T; info[3] = mt.P; info[4] = mt.NumPos(); info[5] = mt.NumPsi();
     info[6] = mt.first_pos; info[7] = mt.last_pos; info[8] = mt.n_def_half; info[9] = mt.rotated;
     info[10] = mt.max_padding; info[11] = mt.input_image.real_memory_allocated; info[12] = mt.whitening_filter.n();
-    info[13] = mt.total_correlation_positions;
+    info[13] = mt.total_correlation_positions; info[14] = mt.n_size_half;
     dinfo[0] = mt.sqrt_input_pixels; dinfo[1] = mt.histogram_min_scaled; dinfo[2] = mt.histogram_step_scaled;
     dinfo[3] = mt.defocus_step; dinfo[4] = mt.pixel_size_step;
   };
@@ -1128,7 +1142,8 @@ int tmo_mt_projection_filter(void* hv, int fp64, int defocus_i, float* out) {
   });
 }
 // Normalised projection T x T (padded layout, (T+2)*T floats) and optionally the CCF plane (numel floats).
-int tmo_mt_projection_ccf(void* hv, int fp64, int pos, int psi_idx, int defocus_i, float* proj_out, float* ccf_out) {
+int tmo_mt_projection_ccf(void* hv, int fp64, int pos, int psi_idx, int defocus_i, int size_i, float* proj_out,
+                          float* ccf_out) {
   return guarded([&] {
     auto run = [&](auto& mt) {
       typedef typename std::remove_reference<decltype(mt)>::type MT;
@@ -1136,7 +1151,8 @@ int tmo_mt_projection_ccf(void* hv, int fp64, int pos, int psi_idx, int defocus_
       mt.InitWork(w);
       Image<typename MT::real_type> pf;
       mt.ProjectionFilter(defocus_i, pf);
-      mt.Projection(w, pf, mt.grid[2 * pos], mt.grid[2 * pos + 1], mt.psi[psi_idx]);
+      if (size_i < -mt.n_size_half || size_i > mt.n_size_half) throw std::runtime_error("oracle: size_i out of range");
+      mt.Projection(w, pf, mt.grid[2 * pos], mt.grid[2 * pos + 1], mt.psi[psi_idx], size_i);
       if (proj_out) for (long i = 0; i < w.current_projection.real_memory_allocated; ++i) proj_out[i] = (float)w.current_projection.v[i];
       if (ccf_out) {
         mt.CCF(w);
@@ -1186,6 +1202,40 @@ float tmo_mt_finalise(void* hv, int fp64, long total_ccfs, float* mip_scaled, fl
   return f.expected_threshold;
 }
 
+
+// Template spectrum of pixel-size plane size_i (P^3 padded layout, numel floats).
+int tmo_mt_template_for_size(void* hv, int fp64, int size_i, float* out) {
+  return guarded([&] {
+    with_mt(hv, fp64, [&](auto& mt) {
+      if (size_i < -mt.n_size_half || size_i > mt.n_size_half) throw std::runtime_error("oracle: size_i out of range");
+      const auto& tr = mt.size_templates[size_i + mt.n_size_half];
+      for (long i = 0; i < tr.real_memory_allocated; ++i) out[i] = (float)tr.v[i];
+    });
+  });
+}
+// Image::ChangePixelSize of a dense real image in (nx, ny, nz) into an (ox, oy, oz) image; out is
+// the padded layout of the result (Fourier when return_fft). dims (may be null) receives pad and new
+// edges (6 ints).
+int tmo_change_pixel_size(const float* in, int nx, int ny, int nz, float factor, float tol, int ox, int oy, int oz,
+                          int return_fft, int fp64, float* out, int* dims) {
+  return guarded([&] {
+    auto run = [&](auto zero) {
+      typedef decltype(zero) RR;
+      Image<RR> a, o;
+      a.Allocate(nx, ny, nz, true);
+      for (long z = 0; z < nz; ++z)
+        for (long y = 0; y < ny; ++y)
+          for (long x = 0; x < nx; ++x) a.v[a.real_addr(x, y, z)] = in[(z * ny + y) * nx + x];
+      if (dims) a.ChangePixelSizeDims(factor, tol, dims, dims + 3);
+      if (!out) return;
+      o.Allocate(ox, oy, oz, true);
+      a.ChangePixelSize(&o, factor, tol, return_fft != 0);
+      for (long i = 0; i < o.real_memory_allocated; ++i) out[i] = (float)o.v[i];
+    };
+    if (fp64) run(0.0); else run(0.0f);
+  });
+}
+void tmo_set_threads(int n) { fft_threads() = std::max(1, n); }
 
 // ---- refine_template ----
 // fparams[15] in RefineParams order; micrograph mx x my (dense), volume t^3 (dense).

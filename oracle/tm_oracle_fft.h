@@ -2,7 +2,7 @@
 // Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load this code,
 // and only as the checker; the product never links or calls it.
 //
-// Mixed-radix Stockham FFT (radices 4, 2, 3, 5, 7 and a generic odd-prime DFT) with
+// Mixed-radix Stockham FFT (radices 4, 2 and a generic odd-prime DFT of any size) with
 // FFTW3's conventions for the reference's in-place "padded" real layout:
 //   * R2C output of a row of nx reals is nx/2+1 complex values in the same row
 //     (row stride nx+2 floats), src/core/image.cpp:4806-4829 (fftwf_plan_dft_r2c_2d/3d).
@@ -12,6 +12,7 @@
 // (configure.ac:289-320); this file restates the DFT, not FFTW's operation order.
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstring>
@@ -88,7 +89,9 @@ inline void small_dft(std::complex<R>* v, int r, int sign, const std::complex<R>
     return;
   }
   // generic odd radix: O(r^2) with the plan's exact roots
-  C out[16];
+  C small[16];
+  std::vector<C> big(r > 16 ? r : 0);
+  C* out = r > 16 ? big.data() : small;
   for (int k = 0; k < r; ++k) {
     C acc(0, 0);
     for (int q = 0; q < r; ++q) {
@@ -110,9 +113,11 @@ inline void fft_c2c(const FFTPlan<R>& p, std::complex<R>* x, std::complex<R>* y,
   C* src = x;
   C* dst = y;
   int Ns = 1;
-  C v[16];
+  int rmax = 16;
+  for (int r : p.radix) rmax = std::max(rmax, r);
+  std::vector<C> vbuf(rmax);
+  C* v = vbuf.data();
   for (int r : p.radix) {
-    if (r > 16) throw std::runtime_error("fft radix > 16 not supported");
     const C* rt = p.roots.at(r).data();
     const int m = n / r;
     const int step = n / (Ns * r);

@@ -14,11 +14,28 @@
 #include <cmath>
 #include <complex>
 #include <stdexcept>
+#include <thread>
 #include <vector>
 
 #include "tm_oracle_fft.h"
 
 namespace tmo {
+
+// Worker threads of the oracle's large transforms (1 = serial). The result does not depend on it:
+// every line is transformed independently.
+inline int& fft_threads() {
+  static int n = 1;
+  return n;
+}
+template <class F>
+void parallel_for(long n, F f) {
+  const int nt = (int)std::min<long>(std::max(1, fft_threads()), std::max(1L, n));
+  if (nt <= 1) { for (long i = 0; i < n; ++i) f(i); return; }
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; ++t)
+    th.emplace_back([&, t] { for (long i = n * t / nt; i < n * (t + 1) / nt; ++i) f(i); });
+  for (auto& x : th) x.join();
+}
 
 static const double PI = 3.14159265359;   // src/core/defines.h:6
 static const float PIf = 3.14159265359f;  // src/core/defines.h:7
@@ -125,20 +142,22 @@ struct Image {
   void fft_r2c_inplace() {
     if (!IsEven(nx)) throw std::runtime_error("oracle FFT needs even x dimension");
     const int hc = nx / 2 + 1;
-    std::vector<C> scratch(std::max(std::max(nx, ny), nz) + 2);
-    for (int z = 0; z < nz; ++z)
-      for (int y = 0; y < ny; ++y) rfft_row(&v[real_addr(0, y, z)], nx, scratch.data());
+    parallel_for((long)nz * ny, [&](long r) {
+      std::vector<C> scratch(nx + 2);
+      rfft_row(&v[real_addr(0, (int)(r % ny), (int)(r / ny))], nx, scratch.data());
+    });
     if (ny > 1) lines_c2c(ny, (long)hc, /*dist*/ 1, /*count per plane*/ hc, (long)hc * ny, -1);
     if (nz > 1) lines_c2c(nz, (long)hc * ny, 1, hc * ny, 0, -1);
   }
   void fft_c2r_inplace() {
     if (!IsEven(nx)) throw std::runtime_error("oracle FFT needs even x dimension");
     const int hc = nx / 2 + 1;
-    std::vector<C> scratch(std::max(std::max(nx, ny), nz) + 2);
     if (nz > 1) lines_c2c(nz, (long)hc * ny, 1, hc * ny, 0, +1);
     if (ny > 1) lines_c2c(ny, (long)hc, 1, hc, (long)hc * ny, +1);
-    for (int z = 0; z < nz; ++z)
-      for (int y = 0; y < ny; ++y) irfft_row(&v[real_addr(0, y, z)], nx, scratch.data());
+    parallel_for((long)nz * ny, [&](long r) {
+      std::vector<C> scratch(nx + 2);
+      irfft_row(&v[real_addr(0, (int)(r % ny), (int)(r / ny))], nx, scratch.data());
+    });
   }
   // Complex FFTs of length n along a strided axis. Lines: for each plane p (only when
   // plane_stride != 0, nz planes) and each of `count` lines starting at offset `l*dist`,
@@ -147,23 +166,24 @@ struct Image {
     auto plan = fft_plan<R>(n);
     const int planes = plane_stride ? nz : 1;
     const int BL = 8;
-    std::vector<C> buf((size_t)BL * n), scratch(n);
+    const long blocks = (count + BL - 1) / BL;
     C* d = c();
-    for (int p = 0; p < planes; ++p) {
-      C* base = d + (long)p * plane_stride;
-      for (int l0 = 0; l0 < count; l0 += BL) {
-        const int nb = std::min(BL, count - l0);
-        for (int e = 0; e < n; ++e) {
-          C* src = base + (long)e * stride + (long)l0 * dist;
-          for (int b = 0; b < nb; ++b) buf[(size_t)b * n + e] = src[b * dist];
-        }
-        for (int b = 0; b < nb; ++b) fft_c2c(*plan, &buf[(size_t)b * n], scratch.data(), sign);
-        for (int e = 0; e < n; ++e) {
-          C* dst = base + (long)e * stride + (long)l0 * dist;
-          for (int b = 0; b < nb; ++b) dst[b * dist] = buf[(size_t)b * n + e];
-        }
+    parallel_for((long)planes * blocks, [&](long q) {
+      std::vector<C> buf((size_t)BL * n), scratch(n);
+      const long p = q / blocks;
+      const int l0 = (int)(q % blocks) * BL;
+      C* base = d + p * plane_stride;
+      const int nb = std::min(BL, count - l0);
+      for (int e = 0; e < n; ++e) {
+        C* src = base + (long)e * stride + (long)l0 * dist;
+        for (int b = 0; b < nb; ++b) buf[(size_t)b * n + e] = src[b * dist];
       }
-    }
+      for (int b = 0; b < nb; ++b) fft_c2c(*plan, &buf[(size_t)b * n], scratch.data(), sign);
+      for (int e = 0; e < n; ++e) {
+        C* dst = base + (long)e * stride + (long)l0 * dist;
+        for (int b = 0; b < nb; ++b) dst[b * dist] = buf[(size_t)b * n + e];
+      }
+    });
   }
 
   // --- simple arithmetic (image.cpp:5118-5127, 5633-5641, 5188-5196, 6498-6525) ---
@@ -271,6 +291,28 @@ struct Image {
       }
     }
     return 2.0 * sum;
+  }
+  // image.cpp:6263-6340 with a mask radius: the pixels whose squared distance from the box centre is
+  // within the squared radius
+  float ReturnAverageOfRealValues(float wanted_mask_radius) const {
+    if (!(wanted_mask_radius > 0.0)) return ReturnAverageOfRealValues();
+    double sum = 0.0;
+    long address = 0, number_of_pixels = 0;
+    const float mask_radius_squared = powf(wanted_mask_radius, 2);
+    for (int k = 0; k < nz; ++k) {
+      const float z = powf(k - box_cz(), 2);
+      for (int j = 0; j < ny; ++j) {
+        const float y = powf(j - box_cy(), 2);
+        for (int i = 0; i < nx; ++i) {
+          const float x = powf(i - box_cx(), 2);
+          const float d2 = x + y + z;
+          if (d2 <= mask_radius_squared) { sum += v[address]; number_of_pixels++; }
+          address++;
+        }
+        address += padding_jump_value;
+      }
+    }
+    return number_of_pixels > 0 ? float(sum / number_of_pixels) : 0.0f;
   }
   // image.cpp:6263-6340 (no mask)
   float ReturnAverageOfRealValues() const {
@@ -381,23 +423,64 @@ struct Image {
       if (j >= jlo) as += padding_jump_value;
     }
   }
-  // Real-space branch only (the path never clips in Fourier space).
+  // image.cpp:8522-8700 (centred, no noise fill)
   void ClipInto(Image* o, float pad = 0.0f) const {
-    if (!is_in_real_space) throw std::runtime_error("oracle ClipInto: Fourier branch not restated");
     o->is_in_real_space = is_in_real_space;
     o->object_is_centred_in_box = object_is_centred_in_box;
     long pc = 0;
-    for (int kk = 0; kk < o->nz; ++kk) {
-      int k = box_cz() + (kk - o->box_cz());
-      for (int jj = 0; jj < o->ny; ++jj) {
-        int j = box_cy() + (jj - o->box_cy());
-        for (int ii = 0; ii < o->nx; ++ii) {
-          int i = box_cx() + (ii - o->box_cx());
-          if (k < 0 || k >= nz || j < 0 || j >= ny || i < 0 || i >= nx) o->v[pc] = pad;
-          else o->v[pc] = v[real_addr(i, j, k)];
+    if (is_in_real_space) {
+      for (int kk = 0; kk < o->nz; ++kk) {
+        int k = box_cz() + (kk - o->box_cz());
+        for (int jj = 0; jj < o->ny; ++jj) {
+          int j = box_cy() + (jj - o->box_cy());
+          for (int ii = 0; ii < o->nx; ++ii) {
+            int i = box_cx() + (ii - o->box_cx());
+            if (k < 0 || k >= nz || j < 0 || j >= ny || i < 0 || i >= nx) o->v[pc] = pad;
+            else o->v[pc] = v[real_addr(i, j, k)];
+            pc++;
+          }
+          pc += o->padding_jump_value;
+        }
+      }
+      return;
+    }
+    // Fourier space: other's physical (ii, jj, kk) takes this image's value at the same logical
+    // coordinate (x = ii >= 0), or the padding value outside this image's logical bounds
+    // (ReturnComplexPixelFromLogicalCoord, image.h:274-282)
+    C* od = o->c();
+    const C* d = c();
+    for (int kk = 0; kk <= o->pubz(); ++kk) {
+      const int lz = o->logical_z(kk);
+      for (int jj = 0; jj <= o->puby(); ++jj) {
+        const int ly = o->logical_y(jj);
+        for (int ii = 0; ii <= o->pubx(); ++ii) {
+          const int lx = ii;
+          if (lx < lcx_lo() || lx > lcx_hi() || ly < lcy_lo() || ly > lcy_hi() || lz < lcz_lo() || lz > lcz_hi())
+            od[pc] = C((R)pad, (R)0);
+          else
+            od[pc] = d[four_addr_logical(lx, ly, lz)];
           pc++;
         }
-        pc += o->padding_jump_value;
+      }
+    }
+    // growing in y or z: the positive Nyquist planes FFTW leaves implicit are filled in
+    if (ny < o->ny || nz < o->nz) {
+      if (nz == 1) {
+        const int jj = first_neg_y();
+        for (int ii = 0; ii <= pubx(); ++ii) od[o->four_addr_phys(ii, jj, 0)] = d[four_addr_phys(ii, jj, 0)];
+      } else {
+        for (int kl = lcz_lo(); kl <= lcz_hi(); ++kl) {
+          const int jj = first_neg_y();  // used as a logical y in the larger image, as the reference does
+          const int jl = lcy_lo();
+          for (int ii = 0; ii <= pubx(); ++ii) od[o->four_addr_logical(ii, jj, kl)] = d[four_addr_logical(ii, jl, kl)];
+        }
+        const int kk = first_neg_z();
+        const int km = o->nz - first_neg_z();
+        for (int jj = 1; jj <= first_neg_y(); ++jj)
+          for (int ii = 0; ii <= pubx(); ++ii) od[o->four_addr_phys(ii, jj, kk)] = od[o->four_addr_phys(ii, jj, km)];
+        for (int jj = o->ny - first_neg_y(); jj <= o->ny - 1; ++jj)
+          for (int ii = 0; ii <= pubx(); ++ii) od[o->four_addr_phys(ii, jj, kk)] = od[o->four_addr_phys(ii, jj, km)];
+        for (int ii = 0; ii <= pubx(); ++ii) od[o->four_addr_phys(ii, 0, kk)] = od[o->four_addr_phys(ii, 0, km)];
       }
     }
   }
@@ -407,6 +490,86 @@ struct Image {
     t.Allocate(x, y, z, is_in_real_space);
     ClipInto(&t, pad);
     *this = std::move(t);
+  }
+  // image.cpp:8705-8792: resample by wanted_factor (new pixel size / old) through a Fourier crop or
+  // pad of an intermediately padded copy; the result is clipped into other's size.
+  void ChangePixelSize(Image* o, float wanted_factor, float wanted_tolerance, bool return_fft) const {
+    if (!is_in_real_space) throw std::runtime_error("oracle ChangePixelSize: image must be in real space");
+    if (fabsf(wanted_factor - 1.0f) < wanted_tolerance && return_fft) {
+      if (o == this) throw std::runtime_error("oracle ChangePixelSize: in place not restated");
+      ClipInto(o);
+      o->ForwardFFT();
+      return;
+    }
+    if (fabsf(wanted_factor - 1.0f) < wanted_tolerance) {
+      if (o != this) ClipInto(o);
+      return;
+    }
+    int old_x = nx, old_y = ny, old_z = nz, new_x, new_y, new_z = 1, pad_x, pad_y, pad_z = 1;
+    int min_dimension = std::min(nx, ny);
+    if (nz > 1) min_dimension = std::min(min_dimension, nz);
+    float intermediate_factor;
+    float padding_value = 0.0f;
+    if (wanted_tolerance < 1.0f / (min_dimension * wanted_factor)) {
+      intermediate_factor = 1.0f / (wanted_tolerance * min_dimension);
+      pad_x = old_x * intermediate_factor;
+      if (IsEven(old_x) != IsEven(pad_x)) pad_x++;
+      new_x = int(pad_x / wanted_factor);
+      if (IsEven(old_x) != IsEven(new_x)) new_x++;
+      pad_y = old_y * intermediate_factor;
+      if (IsEven(old_y) != IsEven(pad_y)) pad_y++;
+      new_y = int(pad_y / wanted_factor);
+      if (IsEven(old_y) != IsEven(new_y)) new_y++;
+      if (nz > 1) {
+        pad_z = old_z * intermediate_factor;
+        if (IsEven(old_z) != IsEven(pad_z)) pad_z++;
+        new_z = int(pad_z / wanted_factor);
+        if (IsEven(old_z) != IsEven(new_z)) new_z++;
+      }
+    } else {
+      intermediate_factor = wanted_factor;
+      pad_x = old_x * intermediate_factor;
+      if (IsEven(old_x) != IsEven(pad_x)) pad_x++;
+      new_x = old_x;
+      pad_y = old_y * intermediate_factor;
+      if (IsEven(old_y) != IsEven(pad_y)) pad_y++;
+      new_y = old_y;
+      if (nz > 1) {
+        pad_z = old_z * intermediate_factor;
+        if (IsEven(old_z) != IsEven(pad_z)) pad_z++;
+        new_z = old_z;
+      }
+    }
+    Image t = *this;
+    if (pad_x > old_x || pad_y > old_y || pad_z > old_z) padding_value = t.ReturnAverageOfRealValues(0.45f * min_dimension);
+    t.Resize(pad_x, pad_y, pad_z, padding_value);
+    t.ForwardFFT();
+    t.Resize(new_x, new_y, new_z, 0.0f);
+    if (new_x == o->nx && new_y == o->ny && new_z == o->nz) {
+      *o = std::move(t);
+      if (!return_fft) o->BackwardFFT();
+      return;
+    }
+    t.BackwardFFT();
+    t.ClipInto(o, padding_value);
+    if (return_fft) o->ForwardFFT();
+  }
+  // the sizes ChangePixelSize works at: pad and new edge per axis
+  void ChangePixelSizeDims(float wanted_factor, float wanted_tolerance, int* pad, int* nw) const {
+    int min_dimension = std::min(nx, ny);
+    if (nz > 1) min_dimension = std::min(min_dimension, nz);
+    const int dims[3] = {nx, ny, nz};
+    const bool small = wanted_tolerance < 1.0f / (min_dimension * wanted_factor);
+    const float intermediate_factor = small ? 1.0f / (wanted_tolerance * min_dimension) : wanted_factor;
+    for (int a = 0; a < 3; ++a) {
+      if (a == 2 && nz == 1) { pad[a] = nw[a] = 1; continue; }
+      int p = dims[a] * intermediate_factor;
+      if (IsEven(dims[a]) != IsEven(p)) p++;
+      int q = small ? int(p / wanted_factor) : dims[a];
+      if (small && IsEven(dims[a]) != IsEven(q)) q++;
+      pad[a] = p;
+      nw[a] = q;
+    }
   }
   void Rotate2DInPlaceBy90Degrees(bool positive) {
     Image b;

@@ -19,11 +19,13 @@ def rel(a, b):
 
 
 def decode_index(info, gi):
-    """Global orientation index -> (defocus_i, pos, psi_idx) for pixel-size index 0, the inverse of
-    (plane * n_pos + pos) * n_psi + psi_idx with plane = defocus_i + n_def_half (pos over the whole
-    grid)."""
+    """Global orientation index -> (size_i, defocus_i, pos, psi_idx), the inverse of
+    (plane * n_pos + pos) * n_psi + psi_idx with plane = (size_i + n_size_half) * (2 n_def_half + 1)
+    + defocus_i + n_def_half (pos over the whole grid)."""
     plane, rem = divmod(int(gi), info.n_pos * info.n_psi)
-    return plane - info.n_def_half, rem // info.n_psi, rem % info.n_psi
+    size_plane, def_plane = divmod(plane, 2 * info.n_def_half + 1)
+    return (size_plane - getattr(info, "n_size_half", 0), def_plane - info.n_def_half, rem // info.n_psi,
+            rem % info.n_psi)
 
 
 def near_tie_check(mt, ref, got, max_orientations=400):
@@ -42,8 +44,8 @@ def near_tie_check(mt, ref, got, max_orientations=400):
     checked = 0
     worst, at = 0.0, None
     for gi in uniq[order]:
-        d, pos, k = decode_index(I, gi)
-        _, ccf = mt.projection_and_ccf(pos, k, d)
+        si, d, pos, k = decode_index(I, gi)
+        _, ccf = mt.projection_and_ccf(pos, k, d, size_i=si)
         sel = bad[g == gi]
         v = ccf[sel[:, 0], sel[:, 1]]
         m = ref["mip"][sel[:, 0], sel[:, 1]]
