@@ -115,7 +115,8 @@ void usage() {
                "usage: match_template_hip --image F --template F --out PREFIX [--pixel-size 1] [--voltage 300]\n"
                "  [--cs 2.7] [--amplitude-contrast 0.07] [--defocus1 A] [--defocus2 A] [--defocus-angle 0]\n"
                "  [--phase-shift 0] [--angular-step 10] [--in-plane-step 10] [--defocus-range 0] [--defocus-step 0]\n"
-               "  [--padding 1] [--symmetry C1] [--min-peak-radius 10] [--device 0] [--batch 0]\n");
+               "  [--pixel-size-range 0] [--pixel-size-step 0] [--padding 1] [--symmetry C1] [--min-peak-radius 10]\n"
+               "  [--device 0] [--batch 0]\n");
 }
 
 }  // namespace
@@ -134,6 +135,7 @@ int main(int argc, char** argv) {
     const float dfa = f("defocus-angle", 0.0), phase = f("phase-shift", 0.0);
     const float ang_step = f("angular-step", 10.0), psi_step = f("in-plane-step", 10.0);
     float def_range = f("defocus-range", 0.0), def_step = f("defocus-step", 0.0);
+    float pix_range = f("pixel-size-range", 0.0), pix_step = f("pixel-size-step", 0.0);
     const float padding = f("padding", 1.0), min_peak_radius = f("min-peak-radius", 10.0);
     const int device = (int)f("device", 0), batch = (int)f("batch", 0);
     const std::string sym = a.count("symmetry") ? a["symmetry"] : "C1";
@@ -159,20 +161,35 @@ int main(int argc, char** argv) {
     if (def_step <= 0.0) { def_range = 0.0f; def_step = 100.0f; }
     const float r = def_range / def_step;
     const int ndh = r > 0 ? int(r + 0.5) : int(r - 0.5);
+    // match_template.cpp:755-762
+    if (pix_step <= 0.0f) { pix_range = 0.0f; pix_step = 0.02f; }
+    const float rs = pix_range / pix_step;
+    const int nsh = rs > 0 ? int(rs + 0.5) : int(rs - 0.5);
     const double sqrt_px = sqrt((double)(W * H));
     const float hstep = (22.5f - -12.5f) / float(512);
     cistem_amd::TemplateMatchingCore core(device);
-    core.Init(tpl.data(), P, spec.data(), W, H, T, 0.02f, def_step, 360.0f, 0.0f, psi_step, grid.data(), npos,
+    core.Init(tpl.data(), P, spec.data(), W, H, T, pix_step, def_step, 360.0f, 0.0f, psi_step, grid.data(), npos,
               -12.5 / sqrt_px, hstep / sqrt_px, 512, maxpad, 0, npos - 1, batch);
     std::vector<float> pf((size_t)T * (T / 2 + 1) * 2);
-    for (int di = -ndh; di <= ndh; ++di) {
-      if (tm_projection_filter(T, cx.data(), cy.data(), ncurve, kv, cs, amp, df1 + float(di) * def_step,
-                               df2 + float(di) * def_step, dfa, pixel_size, phase, pf.data()))
-        throw std::runtime_error(tm_prepare_last_error());
-      const long long n = core.RunInnerLoop(pf.data(), 0, di);
-      std::fprintf(stderr, "defocus plane %d: %lld CCFs\n", di, n);
+    // pixel-size planes, then defocus planes (match_template.cpp:837-914)
+    for (int si = -nsh; si <= nsh; ++si) {
+      if (nsh) {
+        const float factor = (pixel_size + float(si) * pix_step) / pixel_size;
+        if (tm_prepare_template_scaled(device, vol.v.data(), T, padding, factor, tpl.data(), &P))
+          throw std::runtime_error(tm_prepare_last_error());
+        core.SetTemplate(tpl.data());
+      }
+      for (int di = -ndh; di <= ndh; ++di) {
+        if (tm_projection_filter(T, cx.data(), cy.data(), ncurve, kv, cs, amp, df1 + float(di) * def_step,
+                                 df2 + float(di) * def_step, dfa, pixel_size, phase, pf.data()))
+          throw std::runtime_error(tm_prepare_last_error());
+        const long long n = core.RunInnerLoop(pf.data(), si, di);
+        std::fprintf(stderr, "pixel-size plane %d, defocus plane %d: %lld CCFs\n", si, di, n);
+      }
     }
-    const long long total = core.total_number_of_cccs_calculated;
+    // the count the scaling divides by: positions x psi x defocus planes, without the pixel-size
+    // planes, as the reference's total_correlation_positions (match_template.cpp:730-764)
+    const long long total = core.total_number_of_cccs_calculated / (2 * nsh + 1);
     const size_t np = (size_t)(W + 2) * H;
     std::vector<float> mip(np), psi(np), theta(np), phi(np), defo(np), pix(np);
     std::vector<long> hist(512, 0);
