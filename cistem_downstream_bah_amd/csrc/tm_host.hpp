@@ -226,4 +226,16 @@ void extract_cols(Ctx* h, int N, const cf* tpl, const float* mats, int o0, int n
   }
 }
 
+// Column transforms of any compiled line length (the template sizes and the image sides).
+template <class Ctx>
+void cols_line(Ctx* h, int N, cf* data, int ncols, size_t bstride, int nb, int sign) {
+  switch (N) {
+#define TM_CASE(n) \
+  case n: launch_cols<n, Ctx>(h, data, ncols, bstride, nb, sign, 0, nullptr, 1.0f); break;
+    TM_LINE_SIZES(TM_CASE)
+#undef TM_CASE
+    default: throw TmError{TM_ERR_UNSUPPORTED, "unsupported column FFT length " + std::to_string(N)};
+  }
+}
+
 }  // namespace

@@ -314,6 +314,79 @@ static __global__ void k_fourier_grow_z(cf* __restrict__ out, int a, int b) {
   }
 }
 
+// ---------------------------------------------------------------- pixel-size change of projections
+// The same steps for a batch of square images (one per item, edges a -> b): masked mean inside
+// 0.45 a (image.cpp:6263-6322) per item, real clips with a per-item padding value, Fourier clips
+// with the 2-D Nyquist row ClipInto fills in when an image grows (image.cpp:8628-8635).
+template <int NT>
+__global__ __launch_bounds__(NT) void k_masked_mean2(const float* __restrict__ in, int a, size_t ibs, float radius,
+                                                     float* __restrict__ out) {
+  __shared__ double rs[NT];
+  __shared__ long long rn[NT];
+  const float* im = in + (size_t)blockIdx.x * ibs;
+  const float r2 = radius * radius;
+  double sum = 0.0;
+  long long n = 0;
+  for (int e = threadIdx.x; e < a * a; e += NT) {
+    const int j = e / a, i = e - j * a;
+    const float dy = (float)(j - a / 2), dx = (float)(i - a / 2);
+    const float d2 = (float)((double)dx * dx) + (float)((double)dy * dy) + 0.0f;
+    if (d2 <= r2) { sum += im[(size_t)j * (a + 2) + i]; ++n; }
+  }
+  rs[threadIdx.x] = sum;
+  rn[threadIdx.x] = n;
+  __syncthreads();
+  for (int st = NT / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) { rs[threadIdx.x] += rs[threadIdx.x + st]; rn[threadIdx.x] += rn[threadIdx.x + st]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = rn[0] > 0 ? (float)(rs[0] / rn[0]) : 0.0f;
+}
+
+static __global__ void k_real_clip2(const float* __restrict__ in, int a, size_t ibs, float* __restrict__ out, int b,
+                                    size_t obs, const float* __restrict__ pad) {
+  const int item = blockIdx.y;
+  const float pv = pad[item];
+  const float* im = in + (size_t)item * ibs;
+  float* om = out + (size_t)item * obs;
+  const int n = b * (b + 2);
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    const int ii = e % (b + 2), jj = e / (b + 2);
+    float v = 0.0f;
+    if (ii < b) {
+      const int i = a / 2 + (ii - b / 2), j = a / 2 + (jj - b / 2);
+      v = (i < 0 || i >= a || j < 0 || j >= a) ? pv : im[(size_t)j * (a + 2) + i];
+    }
+    om[e] = v;
+  }
+}
+
+static __global__ void k_fourier_clip2(const cf* __restrict__ in, int a, size_t ibs, cf* __restrict__ out, int b,
+                                       size_t obs) {
+  const int item = blockIdx.y, ah = a / 2 + 1, bh = b / 2 + 1;
+  const cf* im = in + (size_t)item * ibs;
+  cf* om = out + (size_t)item * obs;
+  const int n = bh * b;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    const int ii = e % bh, jj = e / bh;
+    const int ly = jj >= b / 2 ? jj - b : jj;
+    cf v = cmake(0.f, 0.f);
+    if (ii <= a / 2 && ly >= -a / 2 && ly <= a / 2 - 1) v = im[(size_t)(ly >= 0 ? ly : a + ly) * ah + ii];
+    if (b > a && jj == a / 2 && ii <= a / 2) v = im[(size_t)(a / 2) * ah + ii];  // the positive Nyquist row
+    om[e] = v;
+  }
+}
+
+// The spectra of the t x t projections scaled by k (ForwardFFT), copied to their batch slots.
+static __global__ void k_copy_scale_cf(const cf* __restrict__ in, size_t ibs, cf* __restrict__ out, size_t obs, int n,
+                                       float k) {
+  const int item = blockIdx.y;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    const cf v = in[(size_t)item * ibs + e];
+    out[(size_t)item * obs + e] = cmake(v.x * k, v.y * k);
+  }
+}
+
 // ---------------------------------------------------------------- small line transforms
 // Complex FFTs of length N down columns of [b][N][ncols] complex arrays (row stride ncols).
 // MODE 0: one transform of sign SIGN. MODE 1: forward, multiply by scale * filt[row][col], inverse

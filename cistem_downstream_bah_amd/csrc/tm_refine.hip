@@ -307,6 +307,7 @@ thread_local std::string g_rerr;
 struct ScoreRequest {
   float phi, theta, psi;  // degrees
   float df1, df2;         // SetDefocus arguments (pixels)
+  float factor;           // TemplateScore's pixel_size_factor
 };
 struct ScoreResult {
   float x, y, value;
@@ -384,7 +385,8 @@ struct RunConst {
   tm_refine_params p;
   const float* mip = nullptr;         // dense
   const float* scaled_mip = nullptr;  // dense
-  float defocus_search_range = 0, defocus_refine_step = 0, defocus_step = 0, pixel_size_refine_step = 0, sqrt_t = 0;
+  float defocus_search_range = 0, defocus_refine_step = 0, defocus_step = 0, sqrt_t = 0;
+  float pixel_size_search_range = 0, pixel_size_refine_step = 0;
   long long rs = 0;  // padded row stride mx + 2
 };
 
@@ -421,9 +423,11 @@ DescentTask refine_peak(const RunConst& K, PeakState& st) {
   ScoreResult best_peak{0, 0, 0}, tp{0, 0, 0};
   float offset_distance = 0.f;
   const float score_adjustment = 1.0f;
+  float factor = 1.0f;  // template_object.pixel_size_factor (refine_template.cpp:720)
   auto req = [&](float cur, float step, float phi, float theta, float psi) {
     ++st.evaluations;
-    return ScoreRequest{phi, theta, psi, (p.defocus1 + cur + step) / p.pixel_size, (p.defocus2 + cur + step) / p.pixel_size};
+    return ScoreRequest{phi,    theta, psi, (p.defocus1 + cur + step) / p.pixel_size, (p.defocus2 + cur + step) / p.pixel_size,
+                        factor};
   };
   auto improve = [&](const ScoreResult& r) {
     best_peak = r;
@@ -447,9 +451,10 @@ DescentTask refine_peak(const RunConst& K, PeakState& st) {
     }
     current_defocus = L.get(best_address, M_DEFOCUS);
   }
-  int phi_i = 0, theta_i = 0, psi_i = 0, defocus_i = 0;
-  // block 0: defocus refinement with nested angle descents (:848-929); block 1: the pixel-size
-  // block with size_i fixed at 0 (:946-1086), after restarting from the values at best_address
+  int phi_i = 0, theta_i = 0, psi_i = 0, defocus_i = 0, size_i = 0;
+  // block 0: defocus refinement with nested angle descents (:848-929) at pixel_size_factor 1;
+  // block 1: the pixel-size block (:986-1086), after restarting from the values at best_address and
+  // the pixel-size scan (:931-984)
   for (int block = 0; block < 2; ++block) {
     float filter_step = 0.0f;
     const float mult_start = block == 0 ? K.defocus_step / K.defocus_refine_step : 1.0f;
@@ -460,11 +465,15 @@ DescentTask refine_peak(const RunConst& K, PeakState& st) {
         float best_outer_score;
         do {
           best_outer_score = best_score;
-          float written_defocus = current_defocus;
+          float written_defocus = current_defocus, written_pixel_size = current_pixel_size;
           if (block == 0) {
             if (K.defocus_search_range != 0.0f) defocus_i += myroundint(mult_i * ll);
             filter_step = defocus_i * K.defocus_refine_step;
             written_defocus = current_defocus + defocus_i * K.defocus_refine_step;
+          } else {
+            if (K.pixel_size_search_range != 0.0f) size_i += myroundint(mult_i * ll);
+            factor = (p.pixel_size + current_pixel_size + float(size_i) * K.pixel_size_refine_step) / p.pixel_size;
+            written_pixel_size = current_pixel_size + size_i * K.pixel_size_refine_step;
           }
           for (int kk = 0; kk < 2; kk = -2 * kk + 1) {
             float best_phi_score;
@@ -491,7 +500,7 @@ DescentTask refine_peak(const RunConst& K, PeakState& st) {
                         L.set(best_address, M_THETA, current_theta + theta_i * p.angular_step);
                         L.set(best_address, M_PHI, current_phi + phi_i * p.angular_step);
                         L.set(best_address, M_DEFOCUS, written_defocus);
-                        L.set(best_address, M_PIXEL_SIZE, current_pixel_size);
+                        L.set(best_address, M_PIXEL_SIZE, written_pixel_size);
                       }
                     } while (best_score > best_psi_score);
                     psi_i -= ii;
@@ -504,6 +513,7 @@ DescentTask refine_peak(const RunConst& K, PeakState& st) {
           }
         } while (best_score > best_outer_score);
         if (block == 0 && K.defocus_search_range != 0.0f) defocus_i -= ll;
+        if (block == 1 && K.pixel_size_search_range != 0.0f) size_i -= ll;
       }
     }
     if (block == 0) {
@@ -513,6 +523,23 @@ DescentTask refine_peak(const RunConst& K, PeakState& st) {
       current_defocus = L.get(best_address, M_DEFOCUS);
       current_pixel_size = L.get(best_address, M_PIXEL_SIZE);
       phi_i = theta_i = psi_i = 0;
+      filter_step = 0.0f;
+      // the pixel-size scan at the restart values (:946-984)
+      if (K.pixel_size_search_range != 0.0f) {
+        const int ns = myroundint(float(K.pixel_size_search_range) / float(K.pixel_size_refine_step));
+        for (int size_is = -ns; size_is <= ns; ++size_is) {
+          factor = (p.pixel_size + current_pixel_size + float(size_is) * K.pixel_size_refine_step) / p.pixel_size;
+          tp = co_await Score{req(current_defocus, 0.0f, current_phi, current_theta, current_psi)};
+          score = tp.value;
+          if (score > best_score) {
+            improve(tp);
+            L.set(best_address, M_PIXEL_SIZE, current_pixel_size + size_is * K.pixel_size_refine_step);
+          }
+        }
+        current_pixel_size = L.get(best_address, M_PIXEL_SIZE);
+      }
+      phi_i = theta_i = psi_i = 0;
+      size_i = 0;
     }
   }
   tm_refined_peak& r = st.rec;
@@ -581,6 +608,14 @@ struct tm_refine {
   PinnedBuf<long long> h_shifts, h_arg_i;
   int score_parts = 1, peak_parts = 1;
   std::map<int, cf*> tw;
+  // pixel-size resampling of projections (allocated at the first item that needs it)
+  DevBuf<float> rs_pad, rs_big, rs_clip;
+  DevBuf<cf> rs_spec, rs_new;
+  struct BluesteinTables {
+    int M = 0;
+    DevBuf<cf> chirp, bhat;
+  };
+  std::map<std::pair<int, int>, std::unique_ptr<BluesteinTables>> bluestein;
   // timing: event pairs around each stage of a TemplateScore batch, resolved after the batch's
   // synchronisation
   enum { S_EXTRACT, S_SLICE_C2R, S_PROJ_R2C, S_PRODUCT, S_CCF_C2R, S_PEAK, S_COUNT };
@@ -721,17 +756,204 @@ void make_windows(tm_refine* h, const std::vector<int>& xs, const std::vector<in
   }
 }
 
+// Edges ChangePixelSize works at for a square image of edge a (image.cpp:8731-8775): the
+// intermediate (pad) and the new edge.
+void change_pixel_size_dims(int a, float factor, float tol, int* pad, int* nw) {
+  auto ev = [](int n) { return (n & 1) == 0; };
+  if (tol < 1.0f / (a * factor)) {
+    const float inter = 1.0f / (tol * a);
+    int p = a * inter;
+    if (ev(a) != ev(p)) p++;
+    int q = int(p / factor);
+    if (ev(a) != ev(q)) q++;
+    *pad = p;
+    *nw = q;
+  } else {
+    int p = a * factor;
+    if (ev(a) != ev(p)) p++;
+    *pad = p;
+    *nw = a;
+  }
+}
+
+bool line_compiled(int n) {
+  switch (n) {
+#define TM_CASE(k) case k:
+    TM_LINE_SIZES(TM_CASE)
+#undef TM_CASE
+    return true;
+    default: return false;
+  }
+}
+
+// Bluestein tables of length N and sign s (chirp c_n = exp(s i pi n^2 / N), bhat = FFT_M(b) / M with
+// b_m = conj(c_|m|), |m| < N), computed on the host in double and cached on the handle.
+tm_refine::BluesteinTables& bluestein_tables(tm_refine* h, int N, int sign) {
+  auto key = std::make_pair(N, sign);
+  auto it = h->bluestein.find(key);
+  if (it != h->bluestein.end()) return *it->second;
+  auto bt = std::make_unique<tm_refine::BluesteinTables>();
+  int M = 1;
+  while (M < 2 * N - 1) M *= 2;
+  TM_REQUIRE(M == 2048 || M == 4096, TM_ERR_UNSUPPORTED, "transform length " + std::to_string(N) + " not supported");
+  bt->M = M;
+  const long double pi = acosl(-1.0L);
+  std::vector<cf> c(N);
+  std::vector<std::complex<double>> b(M, 0.0), wt(M);
+  for (long n = 0; n < N; ++n) {
+    const long double a = sign * pi * (long double)((n * n) % (2L * N)) / (long double)N;
+    c[n].x = (float)cosl(a);
+    c[n].y = (float)sinl(a);
+    const std::complex<double> cc((double)cosl(a), -(double)sinl(a));
+    b[n] = cc;
+    if (n > 0) b[M - n] = cc;
+  }
+  for (int k = 0; k < M; ++k) {
+    const long double a = -2.0L * pi * (long double)k / (long double)M;
+    wt[k] = std::complex<double>((double)cosl(a), (double)sinl(a));
+  }
+  std::vector<cf> bh(M);
+  for (int k = 0; k < M; ++k) {
+    std::complex<double> acc = 0.0;
+    for (int m = 0; m < M; ++m)
+      if (m < N || m > M - N) acc += b[m] * wt[(int)(((long long)m * k) % M)];
+    bh[k].x = (float)(acc.real() / M);
+    bh[k].y = (float)(acc.imag() / M);
+  }
+  bt->chirp.alloc(N);
+  bt->bhat.alloc(M);
+  TM_HIP(hipMemcpy(bt->chirp.p, c.data(), N * sizeof(cf), hipMemcpyHostToDevice));
+  TM_HIP(hipMemcpy(bt->bhat.p, bh.data(), M * sizeof(cf), hipMemcpyHostToDevice));
+  auto& ref = *bt;
+  h->bluestein[key] = std::move(bt);
+  return ref;
+}
+
+// Inverse column transforms (k_cols layout) of any length N.
+void inverse_cols_any(tm_refine* h, int N, cf* data, int ncols, size_t bstride, int nb) {
+  if (line_compiled(N)) { cols_line(h, N, data, ncols, bstride, nb, +1); return; }
+  auto& bt = bluestein_tables(h, N, +1);
+  const dim3 g((unsigned)((size_t)ncols * nb));
+  if (bt.M == 2048)
+    hipLaunchKernelGGL((tmk::k_bluestein_cols<2048, 256>), g, dim3(256), 0, h->stream, data, N, ncols, bstride, bt.chirp.p,
+                       bt.bhat.p, h->twiddle(2048));
+  else
+    hipLaunchKernelGGL((tmk::k_bluestein_cols<4096, 256>), g, dim3(256), 0, h->stream, data, N, ncols, bstride, bt.chirp.p,
+                       bt.bhat.p, h->twiddle(4096));
+  check_launch("k_bluestein_cols");
+}
+
+// C2R rows of any even length N (in place allowed).
+void c2r_rows_any(tm_refine* h, int N, cf* in, size_t ibs, int irs, float* out, size_t obs, int ors, int nrows, int nb) {
+  if (template_size_compiled(N)) { rows_c2r(h, N, in, ibs, irs, out, obs, ors, nrows, nb); return; }
+  auto& bt = bluestein_tables(h, N, +1);
+  if (bt.M == 2048)
+    hipLaunchKernelGGL((tmk::k_bluestein_c2r_rows<2048, 256>), dim3(nrows, nb), dim3(256), 0, h->stream, in, N, ibs, irs, out,
+                       obs, ors, bt.chirp.p, bt.bhat.p, h->twiddle(2048));
+  else
+    hipLaunchKernelGGL((tmk::k_bluestein_c2r_rows<4096, 256>), dim3(nrows, nb), dim3(256), 0, h->stream, in, N, ibs, irs, out,
+                       obs, ors, bt.chirp.p, bt.bhat.p, h->twiddle(4096));
+  check_launch("k_bluestein_c2r_rows");
+}
+
+// TemplateScore's ChangePixelSize (refine_template.cpp:39, 49; image.cpp:8705-8792) for the items
+// [first, first + count) of a batch: their a x a real projections (src, rows of a + 2 floats, item
+// stride sstride floats) are resampled by factors[i] into t x t spectra (scaled like ForwardFFT)
+// in their projection slots.
+void resample_projections(tm_refine* h, const float* src, size_t sstride, int a, int first, int count,
+                          const std::vector<float>& factors) {
+  const int t = h->t, th = h->th;
+  const size_t proj_bs = (size_t)t * th;
+  const int K = h->K;
+  constexpr int BIG = 1000, BH = BIG / 2 + 1;
+  if (!h->rs_pad.p) {
+    h->rs_pad.alloc(K);
+    h->rs_big.alloc((size_t)K * BIG * (BIG + 2));
+    h->rs_spec.alloc((size_t)K * BH * BIG);
+    h->rs_clip.alloc((size_t)K * t * (t + 2));
+  }
+  int g0 = 0;
+  while (g0 < count) {
+    int pad, nw;
+    change_pixel_size_dims(a, factors[g0], 0.001f, &pad, &nw);
+    int m = 1;
+    while (g0 + m < count) {
+      int p2, n2;
+      change_pixel_size_dims(a, factors[g0 + m], 0.001f, &p2, &n2);
+      if (p2 != pad || n2 != nw) break;
+      ++m;
+    }
+    TM_REQUIRE(pad == BIG, TM_ERR_UNSUPPORTED,
+               "pixel-size change pads a " + std::to_string(a) + "^2 projection to " + std::to_string(pad) +
+                   "; only the intermediate edge 1000 (the reference's tolerance for edges below 1000 / factor) is compiled");
+    TM_REQUIRE(nw % 2 == 0 && nw <= 2048, TM_ERR_UNSUPPORTED, "pixel-size change to edge " + std::to_string(nw));
+    const int nh = nw / 2 + 1;
+    if (h->rs_new.n < (size_t)K * nh * nw) h->rs_new.alloc((size_t)K * nh * nw);
+    const float* s0 = src + (size_t)g0 * sstride;
+    hipLaunchKernelGGL((tmk::k_masked_mean2<256>), dim3(m), dim3(256), 0, h->stream, s0, a, sstride, 0.45f * a, h->rs_pad.p);
+    check_launch("k_masked_mean2");
+    hipLaunchKernelGGL(tmk::k_real_clip2, dim3(1024, m), dim3(256), 0, h->stream, s0, a, sstride, h->rs_big.p, BIG,
+                       (size_t)BIG * (BIG + 2), h->rs_pad.p);
+    check_launch("k_real_clip2");
+    const float sbig = (float)(1. / (double)(float)((long long)BIG * BIG));
+    rows_r2c(h, BIG, h->rs_big.p, (size_t)BIG * (BIG + 2), BIG + 2, 0, 0, BIG, 0, h->rs_spec.p, (size_t)BH * BIG, BH, BIG, m,
+             sbig);
+    cols_line(h, BIG, h->rs_spec.p, BH, (size_t)BH * BIG, m, -1);
+    hipLaunchKernelGGL(tmk::k_fourier_clip2, dim3(512, m), dim3(256), 0, h->stream, h->rs_spec.p, BIG, (size_t)BH * BIG,
+                       h->rs_new.p, nw, (size_t)nh * nw);
+    check_launch("k_fourier_clip2");
+    cf* dst = h->proj.p + (size_t)(first + g0) * proj_bs;
+    if (nw == t) {
+      // the new edge is the box's: the resampled spectrum is the projection (other_image->CopyFrom)
+      hipLaunchKernelGGL(tmk::k_copy_scale_cf, dim3(64, m), dim3(256), 0, h->stream, h->rs_new.p, (size_t)nh * nw, dst,
+                         proj_bs, (int)proj_bs, 1.0f);
+      check_launch("k_copy_scale_cf");
+    } else {
+      inverse_cols_any(h, nw, h->rs_new.p, nh, (size_t)nh * nw, m);
+      float* nr = reinterpret_cast<float*>(h->rs_new.p);
+      c2r_rows_any(h, nw, h->rs_new.p, (size_t)nh * nw, nh, nr, (size_t)(nw + 2) * nw, nw + 2, nw, m);
+      hipLaunchKernelGGL(tmk::k_real_clip2, dim3(64, m), dim3(256), 0, h->stream, nr, nw, (size_t)(nw + 2) * nw,
+                         h->rs_clip.p, t, (size_t)t * (t + 2), h->rs_pad.p);
+      check_launch("k_real_clip2");
+      const float st = (float)(1. / (double)(float)((long long)t * t));
+      rows_r2c(h, t, h->rs_clip.p, (size_t)t * (t + 2), t + 2, 0, 0, t, 0, dst, proj_bs, th, t, m, st);
+      cols(h, t, dst, th, proj_bs, m, -1, 0, nullptr, 1.0f);
+    }
+    g0 += m;
+  }
+}
+
 // One batch of TemplateScore calls: item b correlates the projection at reqs[b] with window slots[b].
-void run_scores(tm_refine* h, const std::vector<ScoreRequest>& reqs, const std::vector<int>& slot_of,
+// Items whose pixel-size factor is not within the tolerance of 1 go last, ordered by their resampled
+// edge, and take the resampling path after the slice's C2R.
+void run_scores(tm_refine* h, const std::vector<ScoreRequest>& reqs_in, const std::vector<int>& slot_in,
                 std::vector<ScoreResult>& out) {
-  const int n = (int)reqs.size();
+  const int n = (int)reqs_in.size();
   const int t = h->t, th = h->th, P = h->P, Ph = h->Ph;
   TM_REQUIRE(n <= h->K, TM_ERR_INVALID, "batch larger than max_in_flight");
-  for (int b = 0; b < n; ++b) {
-    euler_matrix(reqs[b].phi, reqs[b].theta, reqs[b].psi, &h->h_mats.p[9 * b]);
-    h->h_dfs.p[2 * b] = reqs[b].df1;
-    h->h_dfs.p[2 * b + 1] = reqs[b].df2;
-    h->h_slots.p[b] = slot_of[b];
+  std::vector<int> ord(n);
+  for (int b = 0; b < n; ++b) ord[b] = b;
+  auto resampled = [&](int b) { return !(fabsf(reqs_in[b].factor - 1.0f) < 0.001f); };
+  auto new_edge = [&](int b) {
+    int pad, nw;
+    change_pixel_size_dims(P != t ? P : t, reqs_in[b].factor, 0.001f, &pad, &nw);
+    return nw;
+  };
+  std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
+    const bool rx = resampled(x), ry = resampled(y);
+    if (rx != ry) return !rx;
+    return rx && new_edge(x) < new_edge(y);
+  });
+  int n0 = 0;
+  while (n0 < n && !resampled(ord[n0])) ++n0;
+  std::vector<float> factors;
+  for (int q = n0; q < n; ++q) factors.push_back(reqs_in[ord[q]].factor);
+  for (int q = 0; q < n; ++q) {
+    const ScoreRequest& r = reqs_in[ord[q]];
+    euler_matrix(r.phi, r.theta, r.psi, &h->h_mats.p[9 * q]);
+    h->h_dfs.p[2 * q] = r.df1;
+    h->h_dfs.p[2 * q + 1] = r.df2;
+    h->h_slots.p[q] = slot_in[ord[q]];
   }
   TM_HIP(hipMemcpyAsync(h->mats.p, h->h_mats.p, 9 * n * sizeof(float), hipMemcpyHostToDevice, h->stream));
   TM_HIP(hipMemcpyAsync(h->dfs.p, h->h_dfs.p, 2 * n * sizeof(float), hipMemcpyHostToDevice, h->stream));
@@ -741,7 +963,8 @@ void run_scores(tm_refine* h, const std::vector<ScoreRequest>& reqs, const std::
   using R = tm_refine;
   if (P != t) {
     // ExtractSlice, SwapRealSpaceQuadrants, BackwardFFT (columns, then the t centre rows that
-    // ClipInto keeps), ClipInto t^2 + ForwardFFT (rows, scaled 1/N; columns below)
+    // ClipInto keeps, or every row for a resampled item), ClipInto t^2 + ForwardFFT (rows, scaled
+    // 1/N; columns below)
     const size_t slice_bs = (size_t)P * Ph;
     const int r0 = P / 2 - t / 2;
     {
@@ -750,13 +973,22 @@ void run_scores(tm_refine* h, const std::vector<ScoreRequest>& reqs, const std::
     }
     {
       Stage st(h, R::S_SLICE_C2R);
-      rows_c2r(h, P, h->slice.p + (size_t)r0 * Ph, slice_bs, Ph,
-               reinterpret_cast<float*>(h->slice.p) + (size_t)r0 * (P + 2), 2 * slice_bs, P + 2, t, n);
+      if (n0 > 0)
+        rows_c2r(h, P, h->slice.p + (size_t)r0 * Ph, slice_bs, Ph,
+                 reinterpret_cast<float*>(h->slice.p) + (size_t)r0 * (P + 2), 2 * slice_bs, P + 2, t, n0);
+      if (n > n0)
+        rows_c2r(h, P, h->slice.p + (size_t)n0 * slice_bs, slice_bs, Ph,
+                 reinterpret_cast<float*>(h->slice.p + (size_t)n0 * slice_bs), 2 * slice_bs, P + 2, P, n - n0);
     }
     Stage st(h, R::S_PROJ_R2C);
-    rows_r2c(h, t, reinterpret_cast<const float*>(h->slice.p), 2 * slice_bs, P + 2, r0, r0, t, 0, h->proj.p, proj_bs, th, t,
-             n, scale);
-    cols(h, t, h->proj.p, th, proj_bs, n, -1, 0, nullptr, 1.0f);
+    if (n0 > 0) {
+      rows_r2c(h, t, reinterpret_cast<const float*>(h->slice.p), 2 * slice_bs, P + 2, r0, r0, t, 0, h->proj.p, proj_bs, th,
+               t, n0, scale);
+      cols(h, t, h->proj.p, th, proj_bs, n0, -1, 0, nullptr, 1.0f);
+    }
+    if (n > n0)
+      resample_projections(h, reinterpret_cast<const float*>(h->slice.p + (size_t)n0 * slice_bs), 2 * slice_bs, P, n0,
+                           n - n0, factors);
   } else {
     // ExtractSlice, SwapRealSpaceQuadrants, BackwardFFT, ForwardFFT (rows in place)
     {
@@ -768,9 +1000,14 @@ void run_scores(tm_refine* h, const std::vector<ScoreRequest>& reqs, const std::
       rows_c2r(h, t, h->proj.p, proj_bs, th, reinterpret_cast<float*>(h->proj.p), 2 * proj_bs, t + 2, t, n);
     }
     Stage st(h, R::S_PROJ_R2C);
-    rows_r2c(h, t, reinterpret_cast<const float*>(h->proj.p), 2 * proj_bs, t + 2, 0, 0, t, 0, h->proj.p, proj_bs, th, t, n,
-             scale);
-    cols(h, t, h->proj.p, th, proj_bs, n, -1, 0, nullptr, 1.0f);
+    if (n0 > 0) {
+      rows_r2c(h, t, reinterpret_cast<const float*>(h->proj.p), 2 * proj_bs, t + 2, 0, 0, t, 0, h->proj.p, proj_bs, th, t,
+               n0, scale);
+      cols(h, t, h->proj.p, th, proj_bs, n0, -1, 0, nullptr, 1.0f);
+    }
+    if (n > n0)
+      resample_projections(h, reinterpret_cast<const float*>(h->proj.p + (size_t)n0 * proj_bs), 2 * proj_bs, t, n0, n - n0,
+                           factors);
   }
   constexpr int NT = 256;
   {
@@ -801,7 +1038,8 @@ void run_scores(tm_refine* h, const std::vector<ScoreRequest>& reqs, const std::
     h->resolve_timing();
   }
   out.resize(n);
-  for (int b = 0; b < n; ++b) out[b] = ScoreResult{h->h_results.p[3 * b], h->h_results.p[3 * b + 1], h->h_results.p[3 * b + 2]};
+  for (int q = 0; q < n; ++q)
+    out[ord[q]] = ScoreResult{h->h_results.p[3 * q], h->h_results.p[3 * q + 1], h->h_results.p[3 * q + 2]};
 }
 
 struct FoundPeak {
@@ -996,7 +1234,7 @@ int tm_refine_peak_list(tm_refine* h, const float* scaled_mip, float* out, int c
 }
 
 int tm_refine_score(tm_refine* h, int peak_x, int peak_y, float current_defocus, float step, float phi, float theta,
-                    float psi, float* out) {
+                    float psi, float pixel_size_factor, float* out) {
   return rguard([&] {
     require_ready(h);
     TM_REQUIRE(out, TM_ERR_INVALID, "out is NULL");
@@ -1004,7 +1242,7 @@ int tm_refine_score(tm_refine* h, int peak_x, int peak_y, float current_defocus,
     const tm_refine_params& p = h->prm;
     std::vector<ScoreResult> res;
     run_scores(h, {ScoreRequest{phi, theta, psi, (p.defocus1 + current_defocus + step) / p.pixel_size,
-                                (p.defocus2 + current_defocus + step) / p.pixel_size}},
+                                (p.defocus2 + current_defocus + step) / p.pixel_size, pixel_size_factor}},
                {0}, res);
     out[0] = res[0].x;
     out[1] = res[0].y;
@@ -1043,9 +1281,6 @@ int tm_refine_run(tm_refine* h, const float* const* maps, float* const* out_maps
     const tm_refine_params& p = h->prm;
     const int mx = p.image_x, my = p.image_y;
     const size_t npx = (size_t)mx * my;
-    for (size_t i = 0; i < npx; ++i)
-      TM_REQUIRE(maps[6][i] == 0.0f, TM_ERR_UNSUPPORTED,
-                 "the pixel-size map must be zero (pixel-size refinement, ChangePixelSize != 1, is not supported)");
     RunConst K;
     K.p = p;
     K.mip = maps[0];
@@ -1057,7 +1292,12 @@ int tm_refine_run(tm_refine* h, const float* const* maps, float* const* out_maps
       K.defocus_refine_step = 100.0f;
     }
     K.defocus_step = std::max(K.defocus_refine_step, 100.0f);
-    K.pixel_size_refine_step = 100.0f;
+    K.pixel_size_search_range = p.pixel_size_search_range;
+    K.pixel_size_refine_step = 2.0f * p.pixel_size_search_step;
+    if (K.pixel_size_refine_step <= 0.0) {
+      K.pixel_size_search_range = 0.0f;
+      K.pixel_size_refine_step = 100.0f;
+    }
     K.sqrt_t = sqrtf(float(h->t) * float(h->t));
     K.rs = mx + 2;
     for (int k = 0; k < tm_refine::S_COUNT; ++k) h->stage_ms[k] = 0.0, h->stage_launches[k] = 0, h->stage_items[k] = 0;

@@ -5,7 +5,7 @@
 to real space, pad and transform the template, take the peaks of the scaled MIP, window each peak
 and run the coordinate descent over defocus and the Euler angles with TemplateScore. Everything
 runs through the C ABI of libtm_hip.so (include/tm_refine.h, include/tm_prepare.h); there is no CPU
-path. Pixel-size refinement and the particle mask are not supported (the pixel-size map must be 0).
+path. The particle mask is not supported (mask_radius 0, the program's default).
 """
 from __future__ import annotations
 
@@ -37,7 +37,8 @@ class RefineParams(ctypes.Structure):
                                                "defocus_search_range", "defocus_search_step", "wanted_threshold",
                                                "min_peak_radius", "xy_change_threshold")] +
                 [("exclude_above_xy_threshold", ctypes.c_int), ("shared_local_maps", ctypes.c_int),
-                 ("max_in_flight", ctypes.c_int)])
+                 ("max_in_flight", ctypes.c_int), ("pixel_size_search_range", ctypes.c_float),
+                 ("pixel_size_search_step", ctypes.c_float)])
 
 
 class RefinedPeak(ctypes.Structure):
@@ -62,7 +63,7 @@ def _lib():
         L.tm_refine_run.argtypes = [vp, ctypes.POINTER(F32P), ctypes.POINTER(F32P), ctypes.POINTER(RefinedPeak),
                                     ctypes.c_int, ip]
         L.tm_refine_peak_list.argtypes = [vp, F32P, F32P, ctypes.c_int, ip]
-        L.tm_refine_score.argtypes = [vp, ctypes.c_int, ctypes.c_int] + [ctypes.c_float] * 5 + [F32P]
+        L.tm_refine_score.argtypes = [vp, ctypes.c_int, ctypes.c_int] + [ctypes.c_float] * 6 + [F32P]
         L.tm_refine_set_timing.argtypes = [vp, ctypes.c_int]
         i64p = ctypes.POINTER(ctypes.c_longlong)
         L.tm_refine_get_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_double), i64p,
@@ -116,6 +117,8 @@ class RefineSettings:
     exclude_above_xy_threshold: bool = False
     shared_local_maps: bool = False
     max_in_flight: int = 0
+    pixel_size_search_range: float = 0.0
+    pixel_size_search_step: float = 0.0
 
 
 class TemplateRefiner:
@@ -138,7 +141,8 @@ class TemplateRefiner:
                          s.amplitude_contrast, s.defocus1, s.defocus2, s.defocus_angle, s.phase_shift_deg,
                          s.angular_step, s.in_plane_angular_step, s.defocus_search_range, s.defocus_search_step,
                          s.wanted_threshold, s.min_peak_radius, s.xy_change_threshold,
-                         int(s.exclude_above_xy_threshold), int(s.shared_local_maps), s.max_in_flight)
+                         int(s.exclude_above_xy_threshold), int(s.shared_local_maps), s.max_in_flight,
+                         s.pixel_size_search_range, s.pixel_size_search_step)
         t32 = np.ascontiguousarray(tpl, np.complex64)
         _check(L.tm_refine_init(self.h, ctypes.byref(p), _f(np.ascontiguousarray(img, np.float32)),
                                 _f(t32.view(np.float32)), _f(np.ascontiguousarray(cx, np.float32)),
@@ -162,11 +166,11 @@ class TemplateRefiner:
         _check(_lib().tm_refine_peak_list(self.h, _f(sm), _f(out), cap, ctypes.byref(n)))
         return out[: min(n.value, cap)].copy()
 
-    def score(self, peak_x: int, peak_y: int, phi, theta, psi, current_defocus=0.0, step=0.0):
+    def score(self, peak_x: int, peak_y: int, phi, theta, psi, current_defocus=0.0, step=0.0, pixel_size_factor=1.0):
         """TemplateScore for the window at the peak: (x, y, value)."""
         out = np.zeros(3, np.float32)
         _check(_lib().tm_refine_score(self.h, int(peak_x), int(peak_y), current_defocus, step, phi, theta, psi,
-                                      _f(out)))
+                                      pixel_size_factor, _f(out)))
         return float(out[0]), float(out[1]), float(out[2])
 
     def set_timing(self, on: bool) -> None:

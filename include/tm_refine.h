@@ -7,9 +7,9 @@
  * descent over defocus and the three Euler angles. TemplateScore calls of all peaks in flight run
  * together as one batch of device transforms; the descent logic of each peak runs on the host.
  *
- * Scope: pixel-size refinement is not included (pixel_size_search_range 0, and the pixel-size map
- * must be zero, so TemplateScore's ChangePixelSize has factor 1), and the particle mask is off
- * (mask_radius 0, the program's default). The input image, template and maps are the program's
+ * TemplateScore resamples the projection with ChangePixelSize wherever the pixel-size factor differs
+ * from 1 (the pixel-size scan and block, refine_template.cpp:931-1086, or a non-zero pixel-size map).
+ * Scope: the particle mask is off (mask_radius 0, the program's default). The input image, template and maps are the program's
  * inputs after its preparation (tm_prepare_refine_image and tm_prepare_template in tm_prepare.h).
  *
  * Every call returns TM_OK (0) or a negative TM_ERR_* code from tm_hip.h; tm_refine_last_error()
@@ -45,6 +45,9 @@ typedef struct tm_refine_params {
                                    0: every peak starts from the input maps, as a thread that handles
                                    one peak does (peaks are refined together)                          */
   int max_in_flight;            /* peaks refined together (the TemplateScore batch); 0 = 256           */
+  float pixel_size_search_range, pixel_size_search_step; /* Angstrom; refine step = 2 * search step;
+                                   TemplateScore resamples the projection by ChangePixelSize with
+                                   factor (pixel_size + pixel size offset) / pixel_size               */
 } tm_refine_params;
 
 /* One refined peak (TemplateMatchFoundPeakInfo of all_peak_infos / all_peak_changes,
@@ -86,10 +89,11 @@ int tm_refine_run(tm_refine* h, const float* const* maps, float* const* out_maps
 
 /* Test hooks with no counterpart in the reference: the peak list of a scaled MIP (dense;
  * out receives x, y, value per peak, at most capacity), and TemplateScore for the window at
- * (peak_x, peak_y) with the filter at defocus1|2 + current_defocus + step (out: x, y, value). */
+ * (peak_x, peak_y) with the filter at defocus1|2 + current_defocus + step and the projection
+ * resampled by pixel_size_factor (out: x, y, value). */
 int tm_refine_peak_list(tm_refine* h, const float* scaled_mip, float* out, int capacity, int* n_peaks);
 int tm_refine_score(tm_refine* h, int peak_x, int peak_y, float current_defocus, float step, float phi, float theta,
-                    float psi, float* out);
+                    float psi, float pixel_size_factor, float* out);
 
 /* Per stage of the TemplateScore batches of the last tm_refine_run (extract_slice_cols,
  * slice_rows_c2r, projection_r2c, score_product, correlation_c2r, score_peak), measured with HIP

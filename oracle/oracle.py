@@ -77,7 +77,7 @@ def lib():
         L.tmo_rt_get.argtypes = [vp, ctypes.c_int, ctypes.c_int, F32P]
         L.tmo_rt_window.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.tmo_rt_window_from_real.argtypes = [vp, ctypes.c_int, F32P]
-        L.tmo_rt_score.argtypes = [vp, ctypes.c_int] + [ctypes.c_float] * 5 + [F32P, F32P]
+        L.tmo_rt_score.argtypes = [vp, ctypes.c_int] + [ctypes.c_float] * 6 + [F32P, F32P]
         L.tmo_rt_peak_list.argtypes = [vp, ctypes.c_int, F32P, F32P, ctypes.c_long]
         L.tmo_rt_peak_list.restype = ctypes.c_long
         L.tmo_rt_run.argtypes = [vp, ctypes.c_int, ctypes.POINTER(F32P), ctypes.c_int, ctypes.c_float, ctypes.c_int,
@@ -435,12 +435,15 @@ class RTParams:
     padding: float = 1.0
     wanted_threshold: float = 7.5
     min_peak_radius: float = 10.0
+    pixel_size_search_range: float = 0.0
+    pixel_size_search_step: float = 0.0
 
     def packed(self):
         return np.array([self.pixel_size, self.voltage_kV, self.spherical_aberration_mm, self.amplitude_contrast,
                          self.defocus1, self.defocus2, self.defocus_angle, self.phase_shift_deg, self.angular_step,
                          self.in_plane_angular_step, self.defocus_search_range, self.defocus_search_step,
-                         self.padding, self.wanted_threshold, self.min_peak_radius], np.float32)
+                         self.padding, self.wanted_threshold, self.min_peak_radius, self.pixel_size_search_range,
+                         self.pixel_size_search_step], np.float32)
 
 
 MAP_NAMES = ("mip", "scaled_mip", "psi", "theta", "phi", "defocus", "pixel_size")
@@ -507,13 +510,13 @@ class RefineTemplateOracle:
         p = to_padded(np.asarray(img, np.float32))
         _check(lib().tmo_rt_window_from_real(self.h, self.fp64, _fp(p)))
 
-    def score(self, phi, theta, psi, current_defocus=0.0, step=0.0, want_projection=False):
+    def score(self, phi, theta, psi, current_defocus=0.0, step=0.0, want_projection=False, pixel_size_factor=1.0):
         """TemplateScore: (x, y, value) of the correlation peak; with want_projection also the
         real-space normalised filtered projection (t, t)."""
         out = np.zeros(3, np.float32)
         t = self.t
         proj = np.zeros((t, t + 2), np.float32) if want_projection else None
-        _check(lib().tmo_rt_score(self.h, self.fp64, current_defocus, step, phi, theta, psi, _fp(out),
+        _check(lib().tmo_rt_score(self.h, self.fp64, current_defocus, step, phi, theta, psi, pixel_size_factor, _fp(out),
                                   _fp(proj) if want_projection else None))
         res = (float(out[0]), float(out[1]), float(out[2]))
         return (res, proj[:, :t].copy()) if want_projection else res

@@ -542,14 +542,13 @@ void unpack_params(MTParams& p, const float* f, const int* i) {
 // refine_template (src/programs/refine_template/refine_template.cpp) for one image: the peak list
 // from the scaled MIP (:561-601), the per-peak particle window (:713-718), TemplateScore (:25-80)
 // and the coordinate descent over defocus and angles (:746-1086), with the result maps
-// written and read at best_address as the program does. Not restated: pixel-size refinement
-// (pixel_size_search_range must be 0, so TemplateScore's ChangePixelSize has factor 1, which is
-// ClipInto + ForwardFFT, image.cpp:8705-8722) and the particle mask (mask_radius must be 0, the
-// program's default).
+// written and read at best_address as the program does, including the pixel-size scan and the
+// pixel-size block (:946-1086; TemplateScore resamples the projection with ChangePixelSize,
+// image.cpp:8705-8792). Not restated: the particle mask (mask_radius must be 0, the program's default).
 struct RefineParams {
   float pixel_size, voltage_kV, spherical_aberration_mm, amplitude_contrast, defocus1, defocus2, defocus_angle,
       phase_shift_deg, angular_step, in_plane_angular_step, defocus_search_range, defocus_search_step, padding,
-      wanted_threshold, min_peak_radius;
+      wanted_threshold, min_peak_radius, pixel_size_search_range, pixel_size_search_step;
 };
 
 struct PeakI {
@@ -694,8 +693,9 @@ struct RefineOracle {
     projection_filter.ApplyCurveFilter(whitening_filter);
   }
 
-  // TemplateScore (:25-56) up to the normalised filtered projection spectrum, pixel_size_factor 1
-  void Projection(float phi, float theta, float psi, Image<R>& cp) const {
+  // TemplateScore (:25-56) up to the normalised filtered projection spectrum; the projection is
+  // resampled by ChangePixelSize(pixel_size_factor, 0.001, return_fft) into the t x t box
+  void Projection(float phi, float theta, float psi, Image<R>& cp, float pixel_size_factor = 1.0f) const {
     float m[3][3];
     EulerMatrix(phi, theta, psi, m);
     cp.Allocate(t, t, 1, false);
@@ -705,13 +705,18 @@ struct RefineOracle {
       input_reconstruction.ExtractSlice(pp, m);
       pp.SwapRealSpaceQuadrants();
       pp.BackwardFFT();
-      pp.ClipInto(&cp);
-      cp.ForwardFFT();
+      cp.is_in_real_space = true;
+      pp.ChangePixelSize(&cp, pixel_size_factor, 0.001f, true);
     } else {
       input_reconstruction.ExtractSlice(cp, m);
       cp.SwapRealSpaceQuadrants();
       cp.BackwardFFT();
-      cp.ForwardFFT();
+      if (fabsf(pixel_size_factor - 1.0f) < 0.001f) {
+        cp.ForwardFFT();  // ChangePixelSize into itself within the tolerance (image.cpp:8712-8716)
+      } else {
+        Image<R> src = cp;
+        src.ChangePixelSize(&cp, pixel_size_factor, 0.001f, true);
+      }
     }
     cp.MultiplyPixelWise(projection_filter);
     cp.ZeroCentralPixel();
@@ -719,10 +724,10 @@ struct RefineOracle {
   }
 
   // TemplateScore (:25-80): correlation with the window and its integer peak
-  PeakI Score(float phi, float theta, float psi) {
+  PeakI Score(float phi, float theta, float psi, float pixel_size_factor = 1.0f) {
     ++evaluations;
     Image<R> cp;
-    Projection(phi, theta, psi, cp);
+    Projection(phi, theta, psi, cp, pixel_size_factor);
     auto* a = cp.c();
     const auto* w = windowed_particle.c();
     for (long i = 0; i < cp.real_memory_allocated / 2; ++i) a[i] = std::conj(a[i]) * w[i];
@@ -766,14 +771,14 @@ struct RefineOracle {
                                bool exclude_above_xy_threshold, Maps* out) {
     for (const Image<R>* m : {&in.mip, &in.psi, &in.theta, &in.phi, &in.defocus, &in.pixel_size})
       if (m->nx != in.scaled_mip.nx || m->ny != in.scaled_mip.ny) throw std::runtime_error("oracle refine: map sizes differ");
-    for (long i = 0; i < in.pixel_size.real_memory_allocated; ++i)
-      if (in.pixel_size.v[i] != 0) throw std::runtime_error("oracle refine: non-zero pixel-size map (ChangePixelSize != 1) is not restated");
     std::vector<PeakI> peaks = PeakList(in.scaled_mip);
     float defocus_search_range = p.defocus_search_range;
     float defocus_refine_step = 2.0f * p.defocus_search_step;
     if (defocus_refine_step <= 0.0) { defocus_search_range = 0.0f; defocus_refine_step = 100.0f; }
     const float defocus_step = std::max(defocus_refine_step, 100.0f);
-    const float pixel_size_refine_step = 100.0f;  // pixel_size_search_range 0 (:619-623)
+    float pixel_size_search_range = p.pixel_size_search_range;
+    float pixel_size_refine_step = 2.0f * p.pixel_size_search_step;
+    if (pixel_size_refine_step <= 0.0) { pixel_size_search_range = 0.0f; pixel_size_refine_step = 100.0f; }  // :619-623
     const float sqrt_t = sqrtf(float(t) * float(t));
     const long rs = (long)in.scaled_mip.nx + in.scaled_mip.padding_jump_value;
     if (out) {
@@ -831,6 +836,7 @@ struct RefineOracle {
         current_defocus = (float)best_defocus_local.v[best_address];
       }
       int phi_i = 0, theta_i = 0, psi_i = 0, defocus_i = 0;
+      float factor = 1.0f;  // template_object.pixel_size_factor (:720)
       // one descent over phi, theta, psi (:870-925); on improvement the block's defocus and pixel
       // size values are written beside the angles
       auto angles_descent = [&](float written_defocus, float written_pixel_size) {
@@ -850,7 +856,7 @@ struct RefineOracle {
                     best_psi_score = best_score;
                     psi_i += ii;
                     template_peak = Score(current_phi + phi_i * p.angular_step, current_theta + theta_i * p.angular_step,
-                                          current_psi + psi_i * p.in_plane_angular_step);
+                                          current_psi + psi_i * p.in_plane_angular_step, factor);
                     score = template_peak.value;
                     if (score > best_score) {
                       improve(template_peak);
@@ -892,17 +898,36 @@ struct RefineOracle {
       current_defocus = (float)best_defocus_local.v[best_address];
       current_pixel_size = (float)best_pixel_size_local.v[best_address];
       phi_i = theta_i = psi_i = 0;
+      int size_i = 0;
       Filter(current_defocus, 0.0f);
-      // :946-1086 pixel-size block with pixel_size_search_range 0: size_i stays 0, the factor stays 1
-      // and three more angle descents run around the restart values
+      // :946-984 pixel-size scan at the restart angles
+      if (pixel_size_search_range != 0.0f) {
+        const int ns = myroundint(float(pixel_size_search_range) / float(pixel_size_refine_step));
+        for (int size_is = -ns; size_is <= ns; ++size_is) {
+          factor = (p.pixel_size + current_pixel_size + float(size_is) * pixel_size_refine_step) / p.pixel_size;
+          template_peak = Score(current_phi, current_theta, current_psi, factor);
+          score = template_peak.value;
+          if (score > best_score) {
+            improve(template_peak);
+            best_pixel_size_local.v[best_address] = current_pixel_size + size_is * pixel_size_refine_step;
+          }
+        }
+        current_pixel_size = (float)best_pixel_size_local.v[best_address];
+      }
+      phi_i = theta_i = psi_i = 0;
+      size_i = 0;
+      // :986-1086 pixel-size block: size_i steps when the range is non-zero, one angle descent per step
       for (float mult_i = 1.0f; mult_i > 0.5f; mult_i /= 2.0f) {
         for (int ll = 0; ll < 2; ll = -2 * ll + 1) {
           if ((ll != 0) && (pixel_size_refine_step == 0.0f)) break;
           float best_pixel_size_score;
           do {
             best_pixel_size_score = best_score;
-            angles_descent(current_defocus, current_pixel_size);
+            if (pixel_size_search_range != 0.0f) size_i += myroundint(mult_i * ll);
+            factor = (p.pixel_size + current_pixel_size + float(size_i) * pixel_size_refine_step) / p.pixel_size;
+            angles_descent(current_defocus, current_pixel_size + size_i * pixel_size_refine_step);
           } while (best_score > best_pixel_size_score);
+          if (pixel_size_search_range != 0.0f) size_i -= ll;
         }
       }
       // :1092-1113 results
@@ -1241,7 +1266,8 @@ void tmo_set_threads(int n) { fft_threads() = std::max(1, n); }
 // fparams[15] in RefineParams order; micrograph mx x my (dense), volume t^3 (dense).
 void* tmo_rt_create(const float* fp, const float* micrograph, int mx, int my, const float* volume, int t, int fp64) {
   try {
-    RefineParams p{fp[0], fp[1], fp[2], fp[3], fp[4], fp[5], fp[6], fp[7], fp[8], fp[9], fp[10], fp[11], fp[12], fp[13], fp[14]};
+    RefineParams p{fp[0], fp[1], fp[2], fp[3], fp[4], fp[5], fp[6], fp[7], fp[8], fp[9], fp[10], fp[11], fp[12], fp[13], fp[14],
+                   fp[15], fp[16]};
     if (fp64) { auto* h = new RHandle64(); h->rt.Setup(p, micrograph, mx, my, volume, t); return h; }
     auto* h = new RHandle32();
     h->rt.Setup(p, micrograph, mx, my, volume, t);
@@ -1291,18 +1317,18 @@ int tmo_rt_window_from_real(void* hv, int fp64, const float* real_padded) {
 // TemplateScore at (phi, theta, psi) with the filter at defocus1|2 + current + step; out = x, y, value.
 // proj_real (may be null) receives the real-space normalised filtered projection (C2R of the spectrum
 // TemplateScore correlates, before the product with the window; padded t x t).
-int tmo_rt_score(void* hv, int fp64, float current_defocus, float step, float phi, float theta, float psi, float* out,
-                 float* proj_real) {
+int tmo_rt_score(void* hv, int fp64, float current_defocus, float step, float phi, float theta, float psi,
+                 float pixel_size_factor, float* out, float* proj_real) {
   return guarded([&] {
     with_rt(hv, fp64, [&](auto& rt) {
       rt.Filter(current_defocus, step);
       if (proj_real) {
         Image<typename std::remove_reference<decltype(rt)>::type::Real> cp;
-        rt.Projection(phi, theta, psi, cp);
+        rt.Projection(phi, theta, psi, cp, pixel_size_factor);
         cp.BackwardFFT();
         padded_to(cp, proj_real);
       }
-      PeakI pk = rt.Score(phi, theta, psi);
+      PeakI pk = rt.Score(phi, theta, psi, pixel_size_factor);
       out[0] = pk.x; out[1] = pk.y; out[2] = pk.value;
     });
   });

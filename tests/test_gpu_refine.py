@@ -43,6 +43,8 @@ def _planted(s, off=(3.0, -3.0, 3.0)):
 
 def _settings(rp: O.RTParams, **kw):
     return refine.RefineSettings(pixel_size=rp.pixel_size, voltage_kV=rp.voltage_kV,
+                                 pixel_size_search_range=rp.pixel_size_search_range,
+                                 pixel_size_search_step=rp.pixel_size_search_step,
                                  spherical_aberration_mm=rp.spherical_aberration_mm,
                                  amplitude_contrast=rp.amplitude_contrast, defocus1=rp.defocus1, defocus2=rp.defocus2,
                                  defocus_angle=rp.defocus_angle, phase_shift_deg=rp.phase_shift_deg,
@@ -125,6 +127,34 @@ def test_template_score_astigmatic_phase_plate(c0):
         g.close()
 
 
+@pytest.mark.parametrize("padding", [1.0, 2.0])
+def test_template_score_pixel_size(c0, padding):
+    """TemplateScore with the projection resampled by ChangePixelSize (refine_template.cpp:39, 49;
+    image.cpp:8705-8792): factors 0.98 (1000 -> 1020), 1.02 (-> 980) and 0.96 (-> 1042 = 2 x 521,
+    4096-point Bluestein) of a 64^2 (P = t) or 128^2 (P = 2t) projection."""
+    print(f"case: TemplateScore with pixel-size factors, padding {padding}", flush=True)
+    O.set_threads(16)
+    rp = O.RTParams(padding=padding)
+    rt, g = _pair(c0, rp)
+    try:
+        H, W = c0.micrograph.shape
+        worst = 0.0
+        for q in c0.particles[:2]:
+            px, py = int(round(q.x)) - W // 2, int(round(q.y)) - H // 2
+            rt.window(px, py)
+            for f in (0.98, 1.02, 0.96):
+                a = (q.phi + 1.0, q.theta - 1.0, q.psi + 2.0)
+                ref = rt.score(*a, pixel_size_factor=f)
+                got = g.score(px, py, *a, pixel_size_factor=f)
+                assert got[:2] == ref[:2], (q, f, got, ref)
+                worst = max(worst, abs(got[2] - ref[2]) / abs(ref[2]))
+        print(f"pixel-size TemplateScore max rel value error {worst:.2e}", flush=True)
+        assert worst <= 2e-5
+    finally:
+        g.close()
+        O.set_threads(1)
+
+
 def test_peak_list(c0):
     print("case: peak list 256 x 192", flush=True)
     rng = np.random.default_rng(3)
@@ -170,7 +200,7 @@ def _compare(got, ref, out_got=None, out_ref=None):
                 assert np.array_equal(a, b), oname
 
 
-@pytest.mark.parametrize("case", ["angles", "defocus", "padded", "shared", "astigmatic"])
+@pytest.mark.parametrize("case", ["angles", "defocus", "padded", "shared", "astigmatic", "pixel_size"])
 def test_refine_matches_oracle(c0, case):
     print(f"case: refine config0 planted peaks, {case}", flush=True)
     kw = dict(angular_step=1.0, in_plane_angular_step=1.0, wanted_threshold=5.0)
@@ -178,6 +208,8 @@ def test_refine_matches_oracle(c0, case):
         kw.update(defocus_search_range=200.0, defocus_search_step=50.0)
     if case == "padded":
         kw.update(padding=2.0, angular_step=1.5, in_plane_angular_step=1.5)
+    if case == "pixel_size":
+        kw.update(pixel_size_search_range=0.04, pixel_size_search_step=0.01, padding=2.0)
     if case == "astigmatic":
         kw.update(defocus1=14200.0, defocus2=15800.0, defocus_angle=-40.0, phase_shift_deg=10.0,
                   defocus_search_range=300.0, defocus_search_step=25.0)
@@ -186,11 +218,18 @@ def test_refine_matches_oracle(c0, case):
     rt, g = _pair(c0, rp, shared_local_maps=shared)
     try:
         maps = _planted(c0)
+        if case == "pixel_size":
+            O.set_threads(16)
+            keep = [q for q in c0.particles if 50 <= round(q.x) <= 462 and 50 <= round(q.y) <= 462][:3]
+            maps = _planted(synth.Synthetic(c0.micrograph, c0.volume, keep))
+            for q in keep:
+                maps["pixel_size"][int(round(q.y)), int(round(q.x))] = 0.02
         ref, out_ref = rt.run(maps, shared_local_maps=shared)
         got, out_got = g.run(maps)
         _compare(got, ref, out_got, out_ref)
     finally:
         g.close()
+        O.set_threads(1)
 
 
 @pytest.mark.timeout(400)
