@@ -116,6 +116,40 @@ __global__ void k_window_gather(const float* __restrict__ img, int mx, int my, i
   }
 }
 
+// Particle mask of each window (Image::CosineMask, image.cpp:3934-4209, as refine_template.cpp:716
+// calls it): one block per window. w[e] for pixel e of the t x t box is the cosine edge weight in
+// [0, 1] inside the edge band, 2 inside the mask radius (kept) and -1 outside (set to the mean).
+// The band mean is a double sum over the band pixels, as the reference's; band pixels become
+// v e + (1 - e) mean in double, rounded to float once.
+template <int NT>
+__global__ __launch_bounds__(NT) void k_window_mask(float* __restrict__ win_real, int t, const float* __restrict__ w,
+                                                    int n_band) {
+#pragma clang fp contract(off)
+  __shared__ double red[NT];
+  float* o = win_real + (size_t)blockIdx.x * t * (t + 2);
+  const int n = t * t;
+  double s = 0.0;
+  for (int e = threadIdx.x; e < n; e += NT) {
+    const float m = w[e];
+    if (m >= 0.0f && m <= 1.0f) s += (double)o[(e / t) * (t + 2) + e % t];
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = NT / 2; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  const double mean = red[0] / (double)n_band;
+  for (int e = threadIdx.x; e < n; e += NT) {
+    const float m = w[e];
+    float* v = o + (e / t) * (t + 2) + e % t;
+    if (m < 0.0f)
+      *v = (float)mean;
+    else if (m <= 1.0f)
+      *v = (float)((double)(*v * m) + (1.0 - (double)m) * mean);
+  }
+}
+
 // Window spectrum times the quadrant-swap phase (SwapRealSpaceQuadrants), scattered to its slot.
 __global__ void k_window_phase(const cf* __restrict__ stage, const cf* __restrict__ phase, int per,
                                const int* __restrict__ slots, cf* __restrict__ win) {
@@ -598,6 +632,8 @@ struct tm_refine {
   int n_curve = 0;
   rtk::CtfConst ctf{};
   DevBuf<float> img, curve_x, curve_y, mats, dfs, results, smip, win_real, arg_v;
+  DevBuf<float> mask_w;  // particle-mask weights of the t x t window (empty: no mask)
+  int mask_band = 0;     // pixels in the mask's edge band
   DevBuf<cf> tpl, phaseP, phaseT, slice, proj, win, win_stage;
   DevBuf<double> partial;
   DevBuf<float> peak_v;
@@ -728,6 +764,37 @@ long long shift_start(int mx, int my, int sx, int sy) {
   return s;
 }
 
+// Weights of the particle mask (Image::CosineMask, image.cpp:3934-4209, centred real-space branch,
+// not inverted) for the t x t window, in the reference's float / double expression order: the
+// cosine edge weight in the band r0^2 <= d^2 <= (r0 + edge)^2 (r0 = radius - edge / 2, at least
+// 0), 2 inside (kept), -1 outside (the band mean). *n_band receives the band's pixel count.
+std::vector<float> mask_weights(int t, float wanted_mask_radius, float wanted_mask_edge, int* n_band) {
+  const double PI = 3.14159265359;
+  float mask_radius = wanted_mask_radius - wanted_mask_edge * 0.5;
+  if (mask_radius < 0.0) mask_radius = 0.0;
+  const float mask_radius_plus_edge = mask_radius + wanted_mask_edge;
+  const float r2 = powf(mask_radius, 2), re2 = powf(mask_radius_plus_edge, 2);
+  const int c = t / 2;
+  std::vector<float> w((size_t)t * t);
+  int nb = 0;
+  for (int j = 0; j < t; ++j) {
+    const float y = powf(j - c, 2);
+    for (int i = 0; i < t; ++i) {
+      const float d2 = powf(i - c, 2) + y + 0.0f;
+      float& o = w[(size_t)j * t + i];
+      if (d2 >= r2 && d2 <= re2) {
+        const float d = sqrtf(d2);
+        o = (1.0 + cosf(PI * (d - mask_radius) / wanted_mask_edge)) / 2.0;
+        ++nb;
+      } else {
+        o = d2 >= re2 ? -1.0f : 2.0f;
+      }
+    }
+  }
+  *n_band = nb;
+  return w;
+}
+
 // Particle windows for the peaks at (xs[q], ys[q]) into window slots slot_of[q].
 void make_windows(tm_refine* h, const std::vector<int>& xs, const std::vector<int>& ys, const std::vector<int>& slot_of) {
   const int n = (int)xs.size();
@@ -744,6 +811,11 @@ void make_windows(tm_refine* h, const std::vector<int>& xs, const std::vector<in
     hipLaunchKernelGGL(rtk::k_window_gather, dim3(t, m), dim3(256), 0, h->stream, h->img.p, h->prm.image_x,
                        h->prm.image_y, t, h->shifts.p, h->win_real.p);
     check_launch("k_window_gather");
+    if (h->mask_band > 0) {
+      hipLaunchKernelGGL(rtk::k_window_mask<256>, dim3(m), dim3(256), 0, h->stream, h->win_real.p, t, h->mask_w.p,
+                         h->mask_band);
+      check_launch("k_window_mask");
+    }
     // ForwardFFT (scaled 1/N) of each window
     const float scale = (float)(1. / (double)(float)((long long)t * t));
     rows_r2c(h, t, h->win_real.p, (size_t)t * (t + 2), t + 2, 0, 0, t, 0, h->win_stage.p, (size_t)t * th, th, t, m, scale);
@@ -1203,6 +1275,22 @@ int tm_refine_init(tm_refine* h, const tm_refine_params* p, const float* whitene
     h->win.alloc((size_t)K * per);
     h->win_stage.alloc((size_t)K * per);
     h->win_real.alloc((size_t)K * t * (t + 2));
+    h->mask_w.release();
+    h->mask_band = 0;
+    if (p->mask_radius > 0.0f) {
+      // refine_template.cpp:520-521 (the radius is at most half the template box less one pixel),
+      // :461 (20 A falloff) and :716 (radius and falloff in pixels)
+      float mask_radius = p->mask_radius;
+      const float temp_float = (float(t) / 2.0f - 1.0f) * p->pixel_size;
+      if (mask_radius > temp_float) mask_radius = temp_float;
+      const float mask_falloff = 20.0;
+      int nb = 0;
+      auto w = mask_weights(t, mask_radius / p->pixel_size, mask_falloff / p->pixel_size, &nb);
+      TM_REQUIRE(nb > 0, TM_ERR_INVALID, "the particle mask's edge band holds no pixel (falloff below one pixel)");
+      h->mask_w.alloc(w.size());
+      TM_HIP(hipMemcpy(h->mask_w.p, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice));
+      h->mask_band = nb;
+    }
     h->arg_v.alloc(1024 + 1);
     h->arg_i.alloc(1024 + 1);
     h->h_mats.alloc(9 * (size_t)K);

@@ -5,7 +5,7 @@
 to real space, pad and transform the template, take the peaks of the scaled MIP, window each peak
 and run the coordinate descent over defocus and the Euler angles with TemplateScore. Everything
 runs through the C ABI of libtm_hip.so (include/tm_refine.h, include/tm_prepare.h); there is no CPU
-path. The particle mask is not supported (mask_radius 0, the program's default).
+path. ``mask_radius`` > 0 masks each particle window with the program's CosineMask (:716).
 """
 from __future__ import annotations
 
@@ -38,7 +38,7 @@ class RefineParams(ctypes.Structure):
                                                "min_peak_radius", "xy_change_threshold")] +
                 [("exclude_above_xy_threshold", ctypes.c_int), ("shared_local_maps", ctypes.c_int),
                  ("max_in_flight", ctypes.c_int), ("pixel_size_search_range", ctypes.c_float),
-                 ("pixel_size_search_step", ctypes.c_float)])
+                 ("pixel_size_search_step", ctypes.c_float), ("mask_radius", ctypes.c_float)])
 
 
 class RefinedPeak(ctypes.Structure):
@@ -119,6 +119,7 @@ class RefineSettings:
     max_in_flight: int = 0
     pixel_size_search_range: float = 0.0
     pixel_size_search_step: float = 0.0
+    mask_radius: float = 0.0
 
 
 class TemplateRefiner:
@@ -142,7 +143,7 @@ class TemplateRefiner:
                          s.angular_step, s.in_plane_angular_step, s.defocus_search_range, s.defocus_search_step,
                          s.wanted_threshold, s.min_peak_radius, s.xy_change_threshold,
                          int(s.exclude_above_xy_threshold), int(s.shared_local_maps), s.max_in_flight,
-                         s.pixel_size_search_range, s.pixel_size_search_step)
+                         s.pixel_size_search_range, s.pixel_size_search_step, s.mask_radius)
         t32 = np.ascontiguousarray(tpl, np.complex64)
         _check(L.tm_refine_init(self.h, ctypes.byref(p), _f(np.ascontiguousarray(img, np.float32)),
                                 _f(t32.view(np.float32)), _f(np.ascontiguousarray(cx, np.float32)),

@@ -9,8 +9,9 @@
  *
  * TemplateScore resamples the projection with ChangePixelSize wherever the pixel-size factor differs
  * from 1 (the pixel-size scan and block, refine_template.cpp:931-1086, or a non-zero pixel-size map).
- * Scope: the particle mask is off (mask_radius 0, the program's default). The input image, template and maps are the program's
- * inputs after its preparation (tm_prepare_refine_image and tm_prepare_template in tm_prepare.h).
+ * mask_radius > 0 masks each particle window as the program's CosineMask (refine_template.cpp:716).
+ * The input image, template and maps are the program's inputs after its preparation
+ * (tm_prepare_refine_image and tm_prepare_template in tm_prepare.h).
  *
  * Every call returns TM_OK (0) or a negative TM_ERR_* code from tm_hip.h; tm_refine_last_error()
  * describes the last failure on the calling thread. One handle per host thread.
@@ -48,6 +49,8 @@ typedef struct tm_refine_params {
   float pixel_size_search_range, pixel_size_search_step; /* Angstrom; refine step = 2 * search step;
                                    TemplateScore resamples the projection by ChangePixelSize with
                                    factor (pixel_size + pixel size offset) / pixel_size               */
+  float mask_radius;            /* Angstrom; > 0: each particle window gets the program's cosine mask
+                                   (radius at most (t/2 - 1) pixels, 20 A falloff); 0: no mask         */
 } tm_refine_params;
 
 /* One refined peak (TemplateMatchFoundPeakInfo of all_peak_infos / all_peak_changes,

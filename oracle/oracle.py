@@ -65,6 +65,8 @@ def lib():
                                             ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                             ctypes.c_int, F32P, ctypes.POINTER(ctypes.c_int)]
         L.tmo_set_threads.argtypes = [ctypes.c_int]
+        L.tmo_cosine_mask.argtypes = [F32P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                                      ctypes.POINTER(ctypes.c_float)]
         L.tmo_mt_run.argtypes = [vp] + [ctypes.c_int] * 8
         L.tmo_mt_results.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
         L.tmo_mt_results.restype = ctypes.c_long
@@ -104,6 +106,16 @@ def peaks_to_dicts(arr) -> list:
 def set_threads(n: int) -> None:
     """Worker threads of the oracle's large transforms (results do not depend on it)."""
     lib().tmo_set_threads(int(n))
+
+
+def cosine_mask(img: np.ndarray, radius: float, edge: float):
+    """Image::CosineMask (image.cpp:3934-4209; centred real space, not inverted) of a dense 2-D or 3-D
+    image. Returns (masked image, mask volume)."""
+    a = np.array(img, np.float32, order="C", copy=True)
+    shp = a.shape if a.ndim == 3 else (1,) + a.shape
+    vol = ctypes.c_float(0)
+    _check(lib().tmo_cosine_mask(_fp(a), shp[2], shp[1], shp[0], radius, edge, ctypes.byref(vol)))
+    return a, vol.value
 
 
 def change_pixel_size(img: np.ndarray, factor: float, out_shape=None, tolerance: float = 0.001,
@@ -437,13 +449,14 @@ class RTParams:
     min_peak_radius: float = 10.0
     pixel_size_search_range: float = 0.0
     pixel_size_search_step: float = 0.0
+    mask_radius: float = 0.0
 
     def packed(self):
         return np.array([self.pixel_size, self.voltage_kV, self.spherical_aberration_mm, self.amplitude_contrast,
                          self.defocus1, self.defocus2, self.defocus_angle, self.phase_shift_deg, self.angular_step,
                          self.in_plane_angular_step, self.defocus_search_range, self.defocus_search_step,
                          self.padding, self.wanted_threshold, self.min_peak_radius, self.pixel_size_search_range,
-                         self.pixel_size_search_step], np.float32)
+                         self.pixel_size_search_step, self.mask_radius], np.float32)
 
 
 MAP_NAMES = ("mip", "scaled_mip", "psi", "theta", "phi", "defocus", "pixel_size")

@@ -544,11 +544,12 @@ void unpack_params(MTParams& p, const float* f, const int* i) {
 // and the coordinate descent over defocus and angles (:746-1086), with the result maps
 // written and read at best_address as the program does, including the pixel-size scan and the
 // pixel-size block (:946-1086; TemplateScore resamples the projection with ChangePixelSize,
-// image.cpp:8705-8792). Not restated: the particle mask (mask_radius must be 0, the program's default).
+// image.cpp:8705-8792) and the particle mask of the window (:716, CosineMask with the 20 A falloff
+// of :461 when mask_radius > 0).
 struct RefineParams {
   float pixel_size, voltage_kV, spherical_aberration_mm, amplitude_contrast, defocus1, defocus2, defocus_angle,
       phase_shift_deg, angular_step, in_plane_angular_step, defocus_search_range, defocus_search_step, padding,
-      wanted_threshold, min_peak_radius, pixel_size_search_range, pixel_size_search_step;
+      wanted_threshold, min_peak_radius, pixel_size_search_range, pixel_size_search_step, mask_radius;
 };
 
 struct PeakI {
@@ -635,6 +636,9 @@ struct RefineOracle {
     p = prm;
     mx = x; my = y; t = tt;
     if (p.defocus_search_step < 0.f) throw std::runtime_error("oracle refine: negative defocus step");
+    // :520-521: the mask radius is at most half the template box less one pixel
+    const float temp_float = (float(t) / 2.0f - 1.0f) * p.pixel_size;
+    if (p.mask_radius > temp_float) p.mask_radius = temp_float;
     input_reconstruction.Allocate(t, t, t, true);
     for (long z = 0; z < t; ++z)
       for (long yy = 0; yy < t; ++yy)
@@ -673,13 +677,15 @@ struct RefineOracle {
     projection_filter.Allocate(t, t, 1, false);
   }
 
-  // :713-718 (mask_radius 0): the whitened image shifted so the peak sits at the centre, clipped to
-  // the template box, transformed and quadrant-swapped
+  // :713-718: the whitened image shifted so the peak sits at the centre, clipped to the template box,
+  // masked when mask_radius > 0 (falloff 20 A, :461), transformed and quadrant-swapped
   void Window(int peak_x, int peak_y) {
     Image<R> padded_reference = input_image;
     RealSpaceIntegerShift(padded_reference, peak_x, peak_y);
     windowed_particle.Allocate(t, t, 1, true);
     padded_reference.ClipInto(&windowed_particle);
+    const float mask_falloff = 20.0;
+    if (p.mask_radius > 0.0f) windowed_particle.CosineMask(p.mask_radius / p.pixel_size, mask_falloff / p.pixel_size);
     windowed_particle.ForwardFFT();
     windowed_particle.SwapRealSpaceQuadrants();
   }
@@ -1261,13 +1267,29 @@ int tmo_change_pixel_size(const float* in, int nx, int ny, int nz, float factor,
   });
 }
 void tmo_set_threads(int n) { fft_threads() = std::max(1, n); }
+// Image::CosineMask (centred real space, not inverted) of a dense nx x ny x nz image in place;
+// *volume receives the returned mask volume.
+int tmo_cosine_mask(float* img, int nx, int ny, int nz, float radius, float edge, float* volume) {
+  return guarded([&] {
+    Image<float> a;
+    a.Allocate(nx, ny, nz, true);
+    for (long z = 0; z < nz; ++z)
+      for (long y = 0; y < ny; ++y)
+        for (long x = 0; x < nx; ++x) a.v[a.real_addr(x, y, z)] = img[(z * ny + y) * nx + x];
+    const float vol = a.CosineMask(radius, edge);
+    if (volume) *volume = vol;
+    for (long z = 0; z < nz; ++z)
+      for (long y = 0; y < ny; ++y)
+        for (long x = 0; x < nx; ++x) img[(z * ny + y) * nx + x] = a.v[a.real_addr(x, y, z)];
+  });
+}
 
 // ---- refine_template ----
-// fparams[15] in RefineParams order; micrograph mx x my (dense), volume t^3 (dense).
+// fparams[18] in RefineParams order; micrograph mx x my (dense), volume t^3 (dense).
 void* tmo_rt_create(const float* fp, const float* micrograph, int mx, int my, const float* volume, int t, int fp64) {
   try {
     RefineParams p{fp[0], fp[1], fp[2], fp[3], fp[4], fp[5], fp[6], fp[7], fp[8], fp[9], fp[10], fp[11], fp[12], fp[13], fp[14],
-                   fp[15], fp[16]};
+                   fp[15], fp[16], fp[17]};
     if (fp64) { auto* h = new RHandle64(); h->rt.Setup(p, micrograph, mx, my, volume, t); return h; }
     auto* h = new RHandle32();
     h->rt.Setup(p, micrograph, mx, my, volume, t);

@@ -314,6 +314,58 @@ struct Image {
     }
     return number_of_pixels > 0 ? float(sum / number_of_pixels) : 0.0f;
   }
+  // Image::CosineMask (image.cpp:3934-4209), real space, object centred in the box, not inverted, mask
+  // value not forced (the call of refine_template.cpp:716): the pixels whose squared distance from
+  // the box centre lies in [r0^2, (r0 + edge)^2], r0 = radius - edge / 2 (at least 0), are averaged
+  // (double sum); inside r0 the pixels are kept, in the band they become v e + (1 - e) mean with
+  // e = (1 + cos(pi (d - r0) / edge)) / 2, outside they become the mean. Returns the mask volume.
+  float CosineMask(float wanted_mask_radius, float wanted_mask_edge) {
+    if (!is_in_real_space || !object_is_centred_in_box)
+      throw std::runtime_error("oracle CosineMask: only the centred real-space branch is restated");
+    float mask_radius = wanted_mask_radius - wanted_mask_edge * 0.5;
+    if (mask_radius < 0.0) mask_radius = 0.0;
+    const float mask_radius_plus_edge = mask_radius + wanted_mask_edge;
+    const float r2 = powf(mask_radius, 2), re2 = powf(mask_radius_plus_edge, 2);
+    double pixel_sum = 0.0, mask_volume = 0.0;
+    long number_of_pixels = 0, address = 0;
+    for (int k = 0; k < nz; ++k) {
+      const float z = powf(k - box_cz(), 2);
+      for (int j = 0; j < ny; ++j) {
+        const float y = powf(j - box_cy(), 2);
+        for (int i = 0; i < nx; ++i) {
+          const float d2 = powf(i - box_cx(), 2) + y + z;
+          if (d2 >= r2 && d2 <= re2) { pixel_sum += v[address]; number_of_pixels++; }
+          address++;
+        }
+        address += padding_jump_value;
+      }
+    }
+    if (number_of_pixels == 0) throw std::runtime_error("oracle CosineMask: no pixels in the edge band");
+    pixel_sum /= number_of_pixels;
+    address = 0;
+    for (int k = 0; k < nz; ++k) {
+      const float z = powf(k - box_cz(), 2);
+      for (int j = 0; j < ny; ++j) {
+        const float y = powf(j - box_cy(), 2);
+        for (int i = 0; i < nx; ++i) {
+          const float d2 = powf(i - box_cx(), 2) + y + z;
+          if (d2 >= r2 && d2 <= re2) {
+            const float d = sqrtf(d2);
+            const float edge = (1.0 + cosf(PI * (d - mask_radius) / wanted_mask_edge)) / 2.0;
+            v[address] = (R)(float)(float(v[address]) * edge + (1.0 - edge) * pixel_sum);
+            mask_volume += powf(edge, 2);
+          } else if (d2 >= re2) {
+            v[address] = (R)(float)pixel_sum;
+          } else {
+            mask_volume += 1.0;
+          }
+          address++;
+        }
+        address += padding_jump_value;
+      }
+    }
+    return float(mask_volume);
+  }
   // image.cpp:6263-6340 (no mask)
   float ReturnAverageOfRealValues() const {
     double sum = 0.0;

@@ -3,14 +3,16 @@
 * preparation (refine_template.cpp:526-546): whitened real-space image within 1e-5 of its maximum,
   whitening curve within 1e-5 relative
 * TemplateScore (refine_template.cpp:25-80), template 64^3 unpadded (P = t) and padded (P = 2t),
-  several windows, orientations and defocus offsets, with the oracle's inputs: peak position
-  identical, value within 2e-5 relative
+  several windows, orientations and defocus offsets, without and with the particle mask
+  (CosineMask, :716; radius 25 A and a radius clamped to t/2 - 1 pixels), with the oracle's inputs:
+  peak position identical, value within 2e-5 relative
 * peak list (refine_template.cpp:561-601): identical to the oracle on a map with border cases,
   ties and neighbours inside the exclusion radius
 * whole refinement of planted peaks (config 0: 512^2 image, copies started 3 degrees off), with and
-  without the defocus scan, per-peak and shared local maps: every discrete result (peak positions,
-  shifts, refined angles and defocus, best addresses, number of TemplateScore calls) identical to
-  the oracle, scores within 2e-5 relative, output maps equal away from the score maps
+  without the defocus scan, per-peak and shared local maps, with the particle mask: every discrete
+  result (peak positions, shifts, refined angles and defocus, best addresses, number of
+  TemplateScore calls) identical to the oracle, scores within 2e-5 relative, output maps equal away
+  from the score maps
 * full size (4096^2 image, template 256^3 padded to 512^3): three planted peaks refine to the
   oracle's discrete results
 """
@@ -51,7 +53,8 @@ def _settings(rp: O.RTParams, **kw):
                                  angular_step=rp.angular_step, in_plane_angular_step=rp.in_plane_angular_step,
                                  defocus_search_range=rp.defocus_search_range,
                                  defocus_search_step=rp.defocus_search_step, padding=rp.padding,
-                                 wanted_threshold=rp.wanted_threshold, min_peak_radius=rp.min_peak_radius, **kw)
+                                 wanted_threshold=rp.wanted_threshold, min_peak_radius=rp.min_peak_radius,
+                                 mask_radius=rp.mask_radius, **kw)
 
 
 def _pair(s, rp, **kw):
@@ -79,10 +82,10 @@ def test_prepare_refine_image(c0):
     assert np.abs(img - ref).max() <= 1e-5 * np.abs(ref).max()
 
 
-@pytest.mark.parametrize("padding", [1.0, 2.0])
-def test_template_score(c0, padding):
-    print(f"case: TemplateScore t=64 padding {padding}", flush=True)
-    rp = O.RTParams(padding=padding)
+@pytest.mark.parametrize("padding,mask_radius", [(1.0, 0.0), (2.0, 0.0), (2.0, 25.0), (1.0, 400.0)])
+def test_template_score(c0, padding, mask_radius):
+    print(f"case: TemplateScore t=64 padding {padding} mask radius {mask_radius}", flush=True)
+    rp = O.RTParams(padding=padding, mask_radius=mask_radius)
     rt, g = _pair(c0, rp)
     try:
         H, W = c0.micrograph.shape
@@ -200,7 +203,7 @@ def _compare(got, ref, out_got=None, out_ref=None):
                 assert np.array_equal(a, b), oname
 
 
-@pytest.mark.parametrize("case", ["angles", "defocus", "padded", "shared", "astigmatic", "pixel_size"])
+@pytest.mark.parametrize("case", ["angles", "defocus", "padded", "shared", "astigmatic", "pixel_size", "masked"])
 def test_refine_matches_oracle(c0, case):
     print(f"case: refine config0 planted peaks, {case}", flush=True)
     kw = dict(angular_step=1.0, in_plane_angular_step=1.0, wanted_threshold=5.0)
@@ -210,6 +213,8 @@ def test_refine_matches_oracle(c0, case):
         kw.update(padding=2.0, angular_step=1.5, in_plane_angular_step=1.5)
     if case == "pixel_size":
         kw.update(pixel_size_search_range=0.04, pixel_size_search_step=0.01, padding=2.0)
+    if case == "masked":
+        kw.update(mask_radius=24.0, defocus_search_range=100.0, defocus_search_step=50.0)
     if case == "astigmatic":
         kw.update(defocus1=14200.0, defocus2=15800.0, defocus_angle=-40.0, phase_shift_deg=10.0,
                   defocus_search_range=300.0, defocus_search_step=25.0)
