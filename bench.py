@@ -88,6 +88,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--positions", type=int, default=8, help="search positions per step (x 241 psi)")
     ap.add_argument("--batch", type=int, default=64, help="orientations per launch")
+    ap.add_argument("--autotune", type=str, default="",
+                    help="comma-separated batch candidates: tm_autotune_batch picks the batch before timing")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-psi", type=int, default=8)
@@ -151,6 +153,15 @@ def main():
     search = MT.MatchTemplateSearch(s.micrograph, s.volume, sp)
     log(f"prepared on GPU in {time.time() - t0:.1f}s: W={search.W} H={search.H} T={search.T} "
         f"P={search.template_spectrum.shape[0]} positions={search.grid.shape[0]} psi={search.psi.size}")
+    tuned = None
+    if args.autotune:
+        cands = [int(c) for c in args.autotune.split(",")]
+        t0 = time.time()
+        best, us = search.autotune_batch(cands, sample_orientations=4 * max(cands))
+        args.batch = best
+        tuned = {"chosen": best, "us_per_ccf": {str(k): round(v, 3) for k, v in us.items()},
+                 "sample_orientations": 4 * max(cands), "seconds": round(time.time() - t0, 1)}
+        log(f"autotune: batch {best} ({tuned['us_per_ccf']} us per CCF)")
     g = search.core
     planes = list(range(-search.n_def_half, search.n_def_half + 1))
     filters = {d: search.filter_for(d) for d in planes}
@@ -233,6 +244,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": per_kernel,
+        "autotune": tuned,
         "pipeline": {"algorithmic_bytes_per_ccf": pipe,
                      "fraction_of_hbm_peak": round(pipe * value / HBM_PEAK, 4)},
     }

@@ -26,7 +26,7 @@ I64P = ctypes.POINTER(ctypes.c_longlong)
 # Every symbol include/tm_hip.h declares.
 EXPORTS = ["tm_device_count", "tm_create", "tm_destroy", "tm_init", "tm_run_inner_loop", "tm_get_results", "tm_merge",
            "tm_set_template", "tm_reset", "tm_finalize", "tm_debug_projection", "tm_debug_ccf", "tm_set_timing", "tm_get_timing",
-           "tm_last_error"]
+           "tm_last_error", "tm_autotune_batch"]
 
 PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.c_longlong, ctypes.c_void_p)
 
@@ -73,6 +73,8 @@ def lib():
         L.tm_debug_ccf.argtypes = [vp, F32P, ctypes.c_int, ctypes.c_int, F32P]
         L.tm_set_timing.argtypes = [vp, ctypes.c_int]
         L.tm_get_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_char_p), F64P, I64P, ctypes.c_int]
+        L.tm_autotune_batch.argtypes = [ctypes.c_int, ctypes.POINTER(InitParams), F32P, F32P, F32P, F32P, F32P, I32P,
+                                        ctypes.c_int, ctypes.c_int, I32P, F64P]
         _lib = L
     return _lib
 
@@ -90,6 +92,26 @@ def device_count() -> int:
     n = ctypes.c_int(0)
     _check(lib().tm_device_count(ctypes.byref(n)))
     return n.value
+
+
+def autotune_batch(params: InitParams, template_spectrum: np.ndarray, input_spectrum: np.ndarray,
+                   search_positions: np.ndarray, psi_list: np.ndarray, projection_filter: np.ndarray,
+                   candidates=(16, 32, 64, 128), sample_orientations: int = 1024, device: int = 0):
+    """tm_autotune_batch: times one plane of a sample of the search per candidate batch size.
+    Returns (best batch size, {candidate: microseconds per CCF})."""
+    tpl = np.ascontiguousarray(template_spectrum, np.complex64)
+    inp = np.ascontiguousarray(input_spectrum, np.complex64)
+    grid = np.ascontiguousarray(search_positions, np.float32)
+    psi = np.ascontiguousarray(psi_list, np.float32)
+    f = np.ascontiguousarray(projection_filter, np.complex64)
+    cand = np.ascontiguousarray(candidates, np.int32)
+    us = np.zeros(cand.size, np.float64)
+    best = ctypes.c_int(0)
+    _check(lib().tm_autotune_batch(device, ctypes.byref(params), _f32(tpl.view(np.float32)), _f32(inp.view(np.float32)),
+                                   _f32(grid), _f32(psi), _f32(f.view(np.float32)), cand.ctypes.data_as(I32P),
+                                   int(cand.size), int(sample_orientations), ctypes.byref(best),
+                                   us.ctypes.data_as(F64P)))
+    return best.value, {int(c): float(u) for c, u in zip(cand, us)}
 
 
 @dataclass

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -731,6 +732,55 @@ int tm_get_timing(tm_core* h, const char** names, double* ms, long long* launche
     if (launches) launches[n] = h->kernel_launches[k];
   }
   return n;
+}
+
+int tm_autotune_batch(int device_ordinal, const tm_init_params* params, const float* template_spectrum,
+                      const float* input_spectrum, const float* search_positions, const float* psi_list,
+                      const float* projection_filter, const int* candidates, int n_candidates, int sample_orientations,
+                      int* best_batch, double* us_per_ccf) {
+  return guarded([&] {
+    TM_REQUIRE(params && template_spectrum && input_spectrum && search_positions && psi_list && projection_filter &&
+                   candidates && best_batch,
+               TM_ERR_INVALID, "NULL argument");
+    TM_REQUIRE(n_candidates > 0 && sample_orientations > 0, TM_ERR_INVALID,
+               "n_candidates and sample_orientations must be positive");
+    for (int c = 0; c < n_candidates; ++c)
+      TM_REQUIRE(candidates[c] >= 1 && candidates[c] <= 256, TM_ERR_INVALID, "batch candidates must be in 1..256");
+    TM_REQUIRE(params->n_psi > 0 && params->first_search_position >= 0 &&
+                   params->first_search_position <= params->last_search_position,
+               TM_ERR_INVALID, "empty or invalid search position range");
+    // the sample: the first search positions of the range, enough for sample_orientations
+    tm_init_params p = *params;
+    const int npos = (sample_orientations + p.n_psi - 1) / p.n_psi;
+    p.last_search_position = std::min(params->last_search_position, params->first_search_position + npos - 1);
+    auto ok = [](int rc) {
+      if (rc != TM_OK) throw TmError{rc, g_err};
+    };
+    double best = 0.0;
+    int best_b = 0;
+    for (int c = 0; c < n_candidates; ++c) {
+      p.batch_size = candidates[c];
+      tm_core* h = nullptr;
+      ok(tm_create(device_ordinal, &h));
+      struct Drop {
+        tm_core* h;
+        ~Drop() { tm_destroy(h); }
+      } drop{h};
+      ok(tm_init(h, &p, template_spectrum, input_spectrum, search_positions, psi_list));
+      ok(tm_run_inner_loop(h, projection_filter, 0, 0, nullptr, nullptr, nullptr));  // warm-up plane
+      long long n = 0;
+      const auto t0 = std::chrono::steady_clock::now();
+      ok(tm_run_inner_loop(h, projection_filter, 0, 1, nullptr, nullptr, &n));
+      const double us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+      const double per = us / (double)std::max(1LL, n);
+      if (us_per_ccf) us_per_ccf[c] = per;
+      if (best_b == 0 || per < best) {
+        best = per;
+        best_b = candidates[c];
+      }
+    }
+    *best_batch = best_b;
+  });
 }
 
 }  // extern "C"

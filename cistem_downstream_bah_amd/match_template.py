@@ -11,6 +11,7 @@ bookkeeping and the final un-rotation/cropping of the maps.
 from __future__ import annotations
 
 import ctypes
+import dataclasses
 import math
 from dataclasses import dataclass
 
@@ -182,6 +183,26 @@ class MatchTemplateSearch:
         self.core.init(self.template_spectrum, self.input_spectrum, T, self.grid, self.psi, self.first, self.last,
                        self.histogram_min_scaled, self.histogram_step_scaled, 512, self.max_padding,
                        self.defocus_step, self.pixel_size_step, p.batch_size, p.plane_storage)
+
+    def init_params(self, batch_size: int | None = None) -> core.InitParams:
+        """The tm_init_params of this search (batch_size: the search's own unless given)."""
+        p = self.p
+        return core.InitParams(self.template_spectrum.shape[0], self.T, self.W, self.H, self.grid.shape[0],
+                               self.psi.size, self.first, self.last, self.histogram_min_scaled,
+                               self.histogram_step_scaled, 512, self.max_padding, self.defocus_step,
+                               self.pixel_size_step, p.batch_size if batch_size is None else batch_size,
+                               core.PLANE_STORAGE[p.plane_storage])
+
+    def autotune_batch(self, candidates=(16, 32, 64, 128), sample_orientations: int = 1024):
+        """Times a sample of this search at each candidate batch depth (tm_autotune_batch) and
+        re-initialises the search at the fastest. Returns (batch size, {candidate: us per CCF})."""
+        best, us = core.autotune_batch(self.init_params(), self.template_spectrum, self.input_spectrum, self.grid,
+                                       self.psi, self.filter_for(0), candidates, sample_orientations, self.device)
+        self.p = dataclasses.replace(self.p, batch_size=best)
+        self.core.init(self.template_spectrum, self.input_spectrum, self.T, self.grid, self.psi, self.first, self.last,
+                       self.histogram_min_scaled, self.histogram_step_scaled, 512, self.max_padding,
+                       self.defocus_step, self.pixel_size_step, best, self.p.plane_storage)
+        return best, us
 
     def filter_for(self, defocus_i: int) -> np.ndarray:
         p = self.p

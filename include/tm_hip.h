@@ -137,6 +137,19 @@ int tm_get_timing(tm_core* h, const char** names, double* ms, long long* launche
 /* Last error message of the calling thread. */
 const char* tm_last_error(void);
 
+/* Autotuner of the batch depth (orientations per launch), with no counterpart in the reference.
+ * For each candidate batch size it builds a handle over a sample of the search (the first search
+ * positions of params' range, at least sample_orientations orientations), runs one warm-up plane
+ * and one timed plane with projection_filter, and writes the time per CCF in microseconds to
+ * us_per_ccf[c] (may be NULL). *best_batch receives the fastest candidate (the earlier one on a
+ * tie). The batch depth changes no result: each pixel's state is updated in orientation order
+ * whatever the batch, so every candidate gives bit-identical maps, sums and histograms. Kernel tile
+ * shapes are fixed per compiled size (they are template parameters), not tuned here. */
+int tm_autotune_batch(int device_ordinal, const tm_init_params* params, const float* template_spectrum,
+                      const float* input_spectrum, const float* search_positions, const float* psi_list,
+                      const float* projection_filter, const int* candidates, int n_candidates, int sample_orientations,
+                      int* best_batch, double* us_per_ccf);
+
 #ifdef __cplusplus
 }
 #endif
