@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtm_hip.so")
 # (source, C++ standard): the refinement's descent is written as C++20 coroutines
-SOURCES = [("tm_core.hip", "c++17"), ("tm_prepare.hip", "c++17"), ("tm_refine.hip", "c++20")]
+SOURCES = [("tm_core.hip", "c++17"), ("tm_prepare.hip", "c++17"), ("tm_refine.hip", "c++20"), ("tm_cnn.hip", "c++17")]
 HEADERS = ["tm_fft.hpp", "tm_kernels.hpp", "tm_host.hpp", "tm_sizes.hpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

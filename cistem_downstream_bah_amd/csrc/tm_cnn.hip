@@ -1,0 +1,813 @@
+// Proposal network (include/tm_cnn.h): 3x3 convolutions with ReLU and a 1x1 head, forward,
+// backward and Adam on the device. No counterpart in the reference.
+//
+// Every product is one kernel, k_gemm_nt: C[m][n] = sum_k A[m][k] B[n][k] on bf16 operands with
+// FP32 accumulation on MFMA 16x16x16 tiles (a 64 x 64 block tile, four waves of 32 x 32), both
+// operands K-contiguous, optionally split over K into FP32 partials that a second kernel adds in
+// a fixed order (results do not depend on scheduling). With channels-last activations:
+//   forward        Y[pixel][cout]  = col[pixel][k] . W[cout][k]^T          (col: im2col, bf16)
+//   weight grad    dW[cout][k]     = dZ^T[cout][pixel] . col^T[k][pixel]^T (split over pixels)
+//   data grad      dcol[pixel][k]  = dZ[pixel][cout] . W^T[k][cout]^T, then a gather (col2im)
+// im2col is recomputed in the backward pass instead of being kept per layer.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/tm_cnn.h"
+
+namespace {
+
+thread_local std::string g_cerr;
+
+struct CnnError {
+  int code;
+  std::string msg;
+};
+
+#define CNN_HIP(call)                                                                                  \
+  do {                                                                                                 \
+    hipError_t e_ = (call);                                                                            \
+    if (e_ != hipSuccess) throw CnnError{TM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)}; \
+  } while (0)
+#define CNN_REQUIRE(cond, code, msg)                                                                   \
+  do {                                                                                                 \
+    if (!(cond)) throw CnnError{code, msg};                                                            \
+  } while (0)
+
+template <class F>
+int cguard(F&& f) {
+  try {
+    f();
+    return TM_OK;
+  } catch (const CnnError& e) {
+    g_cerr = e.msg;
+    return e.code;
+  } catch (const std::exception& e) {
+    g_cerr = e.what();
+    return TM_ERR_INVALID;
+  }
+}
+
+void check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) throw CnnError{TM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e)};
+}
+
+typedef unsigned short bf16_t;
+typedef short short4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+// float -> bf16, round to nearest even (NaN kept quiet)
+__device__ inline bf16_t f2bf(float f) {
+  unsigned u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (bf16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (bf16_t)(u >> 16);
+}
+__device__ inline float bf2f(bf16_t b) { return __uint_as_float((unsigned)b << 16); }
+__device__ inline float load_f(const float* p) { return *p; }
+__device__ inline float load_f(const bf16_t* p) { return bf2f(*p); }
+
+constexpr int round16(int x) { return (x + 15) / 16 * 16; }
+inline long long round16ll(long long x) { return (x + 15) / 16 * 16; }
+inline unsigned grid1(long long n, int bs = 256, long long cap = 1 << 20) {
+  long long g = (n + bs - 1) / bs;
+  return (unsigned)std::max(1LL, std::min(g, cap));
+}
+
+// ------------------------------------------------------------------ kernels
+// C (+ z * split_stride) = A[M][lda] . B[N][ldb]^T over k in [z * kchunk, min(K, (z + 1) * kchunk)),
+// K and kchunk multiples of 16, lda and ldb multiples of 4. Block (x: 64 rows, y: 64 columns);
+// wave w owns rows 32 (w / 2) .. and columns 32 (w % 2) .. as 2 x 2 MFMA tiles. MFMA 16x16x16
+// bf16 lane layout: A[m = lane % 16][k = 4 (lane / 16) + i], B[n = lane % 16][k = 4 (lane / 16) + i],
+// D[m = 4 (lane / 16) + i][n = lane % 16].
+template <bool BIAS, bool RELU>
+__global__ __launch_bounds__(256) void k_gemm_nt(const bf16_t* __restrict__ A, int lda, const bf16_t* __restrict__ B,
+                                                 int ldb, float* __restrict__ C, long long ldc, long long M, int N,
+                                                 int K, int kchunk, long long split_stride,
+                                                 const float* __restrict__ bias) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long long m0 = (long long)blockIdx.x * 64 + (wave >> 1) * 32;
+  const int n0 = blockIdx.y * 64 + (wave & 1) * 32;
+  const int r = lane & 15, kq = (lane >> 4) * 4;
+  const int k_lo = blockIdx.z * kchunk, k_hi = min(K, k_lo + kchunk);
+  const bf16_t* ap[2];
+  const bf16_t* bp[2];
+  bool aok[2], bok[2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const long long m = m0 + i * 16 + r;
+    aok[i] = m < M;
+    ap[i] = A + (aok[i] ? m : 0) * (long long)lda + kq;
+    const int n = n0 + i * 16 + r;
+    bok[i] = n < N;
+    bp[i] = B + (size_t)(bok[i] ? n : 0) * ldb + kq;
+  }
+  float4_t acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
+  const short4_t zero = {0, 0, 0, 0};
+  for (int k = k_lo; k < k_hi; k += 16) {
+    short4_t a[2], b[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      a[i] = aok[i] ? *reinterpret_cast<const short4_t*>(ap[i] + k) : zero;
+      b[i] = bok[i] ? *reinterpret_cast<const short4_t*>(bp[i] + k) : zero;
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+  }
+  float* Cz = C + (long long)blockIdx.z * split_stride;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const long long m = m0 + i * 16 + kq + e;
+        const int n = n0 + j * 16 + r;
+        if (m < M && n < N) {
+          float v = acc[i][j][e];
+          if (BIAS) v += bias[n];
+          if (RELU) v = fmaxf(v, 0.f);
+          Cz[m * ldc + n] = v;
+        }
+      }
+}
+
+// out[m][n] = sum over z of parts[z][m][n] in z order (+ bias, ReLU)
+template <bool BIAS, bool RELU>
+__global__ void k_reduce_splits(const float* __restrict__ parts, int nsplit, long long M, int N,
+                                long long split_stride, float* __restrict__ out, long long ldo,
+                                const float* __restrict__ bias) {
+  const long long total = M * N;
+  for (long long e = blockIdx.x * 256LL + threadIdx.x; e < total; e += (long long)gridDim.x * 256) {
+    const long long m = e / N;
+    const int n = (int)(e % N);
+    float s = 0.f;
+    for (int z = 0; z < nsplit; ++z) s += parts[z * split_stride + m * N + n];
+    if (BIAS) s += bias[n];
+    if (RELU) s = fmaxf(s, 0.f);
+    out[m * ldo + n] = s;
+  }
+}
+
+// col[row][k] (bf16, Kp per row): row = (b, oy, ox) of the So x So output, k = (ky * 3 + kx) * Cin + c
+// reads X[b][oy * stride + ky - 1][ox * stride + kx - 1][c] (0 outside the image and for k >= 9 Cin)
+__global__ void k_im2col(const float* __restrict__ X, int Si, int Cin, int So, int stride, long long rows, int Kp,
+                         bf16_t* __restrict__ col) {
+  const long long total = rows * Kp;
+  const int K9 = 9 * Cin;
+  for (long long e = blockIdx.x * 256LL + threadIdx.x; e < total; e += (long long)gridDim.x * 256) {
+    const long long row = e / Kp;
+    const int k = (int)(e % Kp);
+    float v = 0.f;
+    if (k < K9) {
+      const int tap = k / Cin, c = k % Cin;
+      const int ky = tap / 3, kx = tap % 3;
+      const long long b = row / ((long long)So * So);
+      const int p = (int)(row % ((long long)So * So));
+      const int iy = (p / So) * stride + ky - 1, ix = (p % So) * stride + kx - 1;
+      if (iy >= 0 && iy < Si && ix >= 0 && ix < Si) v = X[((b * Si + iy) * Si + ix) * Cin + c];
+    }
+    col[e] = f2bf(v);
+  }
+}
+
+// dX[b][iy][ix][c] = sum of dcol[(b, oy, ox)][(ky * 3 + kx) * Cin + c] over the taps that read it
+__global__ void k_col2im(const float* __restrict__ dcol, int ldk, int Si, int Cin, int So, int stride, long long total,
+                         float* __restrict__ dX) {
+  for (long long e = blockIdx.x * 256LL + threadIdx.x; e < total; e += (long long)gridDim.x * 256) {
+    const int c = (int)(e % Cin);
+    const long long p = e / Cin;
+    const int ix = (int)(p % Si);
+    const int iy = (int)((p / Si) % Si);
+    const long long b = p / ((long long)Si * Si);
+    float s = 0.f;
+    for (int ky = 0; ky < 3; ++ky) {
+      const int ty = iy + 1 - ky;
+      if (ty < 0 || ty % stride) continue;
+      const int oy = ty / stride;
+      if (oy >= So) continue;
+      for (int kx = 0; kx < 3; ++kx) {
+        const int tx = ix + 1 - kx;
+        if (tx < 0 || tx % stride) continue;
+        const int ox = tx / stride;
+        if (ox >= So) continue;
+        s += dcol[((b * So + oy) * So + ox) * ldk + (ky * 3 + kx) * Cin + c];
+      }
+    }
+    dX[e] = s;
+  }
+}
+
+// T[c][r] = bf16(X[r][c]) for c < Cp, r < Rp (0 outside R x C); 32 x 32 tiles through LDS
+template <class TIn>
+__global__ __launch_bounds__(256) void k_transpose_bf16(const TIn* __restrict__ X, long long R, int C, long long ldx,
+                                                        bf16_t* __restrict__ T, long long ldt, int Cp, long long Rp) {
+  __shared__ float tile[32][33];
+  const long long r0 = (long long)blockIdx.x * 32;
+  const int c0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int j = ty; j < 32; j += 8) {
+    const long long r = r0 + j;
+    const int c = c0 + tx;
+    tile[j][tx] = (r < R && c < C) ? load_f(X + r * ldx + c) : 0.f;
+  }
+  __syncthreads();
+  for (int j = ty; j < 32; j += 8) {
+    const int c = c0 + j;
+    const long long r = r0 + tx;
+    if (c < Cp && r < Rp) T[(long long)c * ldt + r] = f2bf(tile[tx][j]);
+  }
+}
+
+// Y[r][k] = bf16(X[r][k]) for k < C, 0 for C <= k < Cp
+__global__ void k_to_bf16_pad(const float* __restrict__ X, long long R, int C, long long ldx, bf16_t* __restrict__ Y,
+                              int Cp) {
+  const long long total = R * Cp;
+  for (long long e = blockIdx.x * 256LL + threadIdx.x; e < total; e += (long long)gridDim.x * 256) {
+    const long long r = e / Cp;
+    const int k = (int)(e % Cp);
+    Y[e] = f2bf(k < C ? X[r * ldx + k] : 0.f);
+  }
+}
+
+// dA *= (A > 0): the ReLU's backward from its output
+__global__ void k_relu_bwd(float* __restrict__ dA, const float* __restrict__ A, long long n) {
+  for (long long e = blockIdx.x * 256LL + threadIdx.x; e < n; e += (long long)gridDim.x * 256)
+    if (!(A[e] > 0.f)) dA[e] = 0.f;
+}
+
+// Column sums of X[R][C] over row chunks of `chunk` rows: part[chunk index][c]. Threads split the
+// chunk's rows into 256 / Cg interleaved subsets per column group of Cg = min(C, 256) columns and
+// add the subsets in a fixed order.
+__global__ __launch_bounds__(256) void k_colsum_part(const float* __restrict__ X, long long R, int C, long long chunk,
+                                                     float* __restrict__ part) {
+  __shared__ float red[256];
+  const int Cg = min(C, 256), per = 256 / Cg;
+  const int t = threadIdx.x, c_off = t % Cg, sub = t / Cg;
+  const long long r_lo = blockIdx.x * chunk, r_hi = min(R, r_lo + chunk);
+  for (int cb = 0; cb < C; cb += Cg) {
+    const int c = cb + c_off;
+    float s = 0.f;
+    if (sub < per && c < C)
+      for (long long r = r_lo + sub; r < r_hi; r += per) s += X[r * C + c];
+    red[t] = s;
+    __syncthreads();
+    if (t < Cg && c < C) {
+      float a = 0.f;
+      for (int q = 0; q < per; ++q) a += red[q * Cg + t];
+      part[(long long)blockIdx.x * C + c] = a;
+    }
+    __syncthreads();
+  }
+}
+__global__ void k_colsum_final(const float* __restrict__ part, int nparts, int C, float* __restrict__ out) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= C) return;
+  float s = 0.f;
+  for (int q = 0; q < nparts; ++q) s += part[(long long)q * C + c];
+  out[c] = s;
+}
+
+// Per-image standardisation: (x - mean) / std over the S x S pixels (FP64 sums), one block per image.
+__global__ __launch_bounds__(256) void k_standardize(float* __restrict__ X, long long npx) {
+  __shared__ double s1[256], s2[256];
+  float* x = X + blockIdx.x * npx;
+  double a = 0.0, q = 0.0;
+  for (long long e = threadIdx.x; e < npx; e += 256) {
+    const double v = x[e];
+    a += v;
+    q += v * v;
+  }
+  s1[threadIdx.x] = a;
+  s2[threadIdx.x] = q;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) {
+      s1[threadIdx.x] += s1[threadIdx.x + k];
+      s2[threadIdx.x] += s2[threadIdx.x + k];
+    }
+    __syncthreads();
+  }
+  const double mean = s1[0] / (double)npx;
+  const double var = fmax(s2[0] / (double)npx - mean * mean, 1e-30);
+  const float m = (float)mean, inv = (float)(1.0 / sqrt(var));
+  for (long long e = threadIdx.x; e < npx; e += 256) x[e] = (x[e] - m) * inv;
+}
+
+__device__ inline float softplus(float x) { return fmaxf(x, 0.f) + log1pf(expf(-fabsf(x))); }
+__device__ inline float sigmoid(float x) { return 1.f / (1.f + expf(-x)); }
+
+// One wave per cell: binary cross-entropy of the location logit (positives weighted by pos_w) and
+// the softmax cross-entropy of the orientation logits when the cell has a bin. Gradients are
+// scaled by heat_scale and bin_scale (the means' divisors); cell_loss receives the unscaled terms.
+__global__ __launch_bounds__(64) void k_loss(const float* __restrict__ logits, int Nh, const float* __restrict__ heat_t,
+                                             const int* __restrict__ bin_t, float pos_w, float heat_scale,
+                                             float bin_scale, float* __restrict__ dlog, float* __restrict__ cell_loss) {
+  const long long cell = blockIdx.x;
+  const float* o = logits + cell * Nh;
+  float* d = dlog + cell * Nh;
+  const int lane = threadIdx.x;
+  const int nb = Nh - 1;
+  const int tb = bin_t[cell];
+  if (lane == 0) {
+    const float z = o[0], y = heat_t[cell];
+    cell_loss[2 * cell] = pos_w * y * softplus(-z) + (1.f - y) * softplus(z);
+    d[0] = heat_scale * (pos_w * y * (sigmoid(z) - 1.f) + (1.f - y) * sigmoid(z));
+  }
+  if (tb < 0) {
+    for (int k = lane; k < nb; k += 64) d[1 + k] = 0.f;
+    if (lane == 0) cell_loss[2 * cell + 1] = 0.f;
+    return;
+  }
+  float mx = -INFINITY;
+  for (int k = lane; k < nb; k += 64) mx = fmaxf(mx, o[1 + k]);
+  for (int s = 32; s > 0; s >>= 1) mx = fmaxf(mx, __shfl_xor(mx, s));
+  float se = 0.f;
+  for (int k = lane; k < nb; k += 64) se += expf(o[1 + k] - mx);
+  for (int s = 32; s > 0; s >>= 1) se += __shfl_xor(se, s);
+  const float lse = mx + logf(se);
+  for (int k = lane; k < nb; k += 64) d[1 + k] = bin_scale * (expf(o[1 + k] - lse) - (k == tb ? 1.f : 0.f));
+  if (lane == 0) cell_loss[2 * cell + 1] = lse - o[1 + tb];
+}
+
+__global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                       long long n, float lr_t, float b1, float b2, float eps) {
+  for (long long e = blockIdx.x * 256LL + threadIdx.x; e < n; e += (long long)gridDim.x * 256) {
+    const float gi = g[e];
+    const float mi = b1 * m[e] + (1.f - b1) * gi;
+    const float vi = b2 * v[e] + (1.f - b2) * gi * gi;
+    m[e] = mi;
+    v[e] = vi;
+    w[e] -= lr_t * mi / (sqrtf(vi) + eps);
+  }
+}
+
+// ------------------------------------------------------------------ host
+template <class T>
+struct Buf {
+  T* p = nullptr;
+  size_t n = 0;
+  void ensure(size_t count) {
+    if (count <= n) return;
+    release();
+    CNN_HIP(hipMalloc((void**)&p, count * sizeof(T)));
+    n = count;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~Buf() { release(); }
+};
+
+struct Layer {
+  int cin, cout, stride, si, so;  // channels, stride, input and output edge
+  int K, Kp;                      // 9 cin (1 cin for the head) and its multiple of 16
+  long long w_off, b_off;         // offsets in the flat parameter array
+};
+
+// C[M][N] (ldc) = A[M][lda] . B[N][ldb]^T (+ bias, ReLU), split over K when the output tile grid is
+// too small to fill the GPU.
+void gemm(hipStream_t st, Buf<float>& parts, const bf16_t* A, int lda, const bf16_t* B, int ldb, float* C,
+          long long ldc, long long M, int N, int K, const float* bias, bool relu) {
+  CNN_REQUIRE(K % 16 == 0 && lda % 4 == 0 && ldb % 4 == 0, TM_ERR_INVALID, "gemm: K, lda, ldb not padded");
+  const long long gm = (M + 63) / 64;
+  const int gn = (N + 63) / 64;
+  CNN_REQUIRE(gm < (1LL << 31) && gn <= 65535, TM_ERR_INVALID, "gemm: output too large");
+  const long long tiles = gm * gn;
+  int nsplit = 1;
+  if (tiles < 1024 && K >= 2048) {
+    nsplit = (int)std::min<long long>(std::min<long long>(K / 1024, 2048), std::max<long long>(1, 4096 / tiles));
+    const long long cap = (64LL << 20) / std::max<long long>(1, M * N);  // partials <= 256 MB
+    nsplit = (int)std::max<long long>(1, std::min<long long>(nsplit, cap));
+  }
+  if (nsplit <= 1) {
+    dim3 g((unsigned)gm, gn, 1);
+    if (bias && relu)
+      hipLaunchKernelGGL((k_gemm_nt<true, true>), g, dim3(256), 0, st, A, lda, B, ldb, C, ldc, M, N, K, K, 0LL, bias);
+    else if (bias)
+      hipLaunchKernelGGL((k_gemm_nt<true, false>), g, dim3(256), 0, st, A, lda, B, ldb, C, ldc, M, N, K, K, 0LL, bias);
+    else
+      hipLaunchKernelGGL((k_gemm_nt<false, false>), g, dim3(256), 0, st, A, lda, B, ldb, C, ldc, M, N, K, K, 0LL,
+                         nullptr);
+    check_launch("k_gemm_nt");
+    return;
+  }
+  const int kchunk = (K / 16 + nsplit - 1) / nsplit * 16;
+  nsplit = (K + kchunk - 1) / kchunk;
+  const long long ss = M * N;
+  parts.ensure((size_t)ss * nsplit);
+  hipLaunchKernelGGL((k_gemm_nt<false, false>), dim3((unsigned)gm, gn, nsplit), dim3(256), 0, st, A, lda, B, ldb, parts.p,
+                     (long long)N, M, N, K, kchunk, ss, nullptr);
+  check_launch("k_gemm_nt split");
+  const unsigned g = grid1(M * N);
+  if (bias && relu)
+    hipLaunchKernelGGL((k_reduce_splits<true, true>), dim3(g), dim3(256), 0, st, parts.p, nsplit, M, N, ss, C, ldc, bias);
+  else if (bias)
+    hipLaunchKernelGGL((k_reduce_splits<true, false>), dim3(g), dim3(256), 0, st, parts.p, nsplit, M, N, ss, C, ldc, bias);
+  else
+    hipLaunchKernelGGL((k_reduce_splits<false, false>), dim3(g), dim3(256), 0, st, parts.p, nsplit, M, N, ss, C, ldc,
+                       nullptr);
+  check_launch("k_reduce_splits");
+}
+
+void transpose_bf16(hipStream_t st, const float* X, long long R, int C, long long ldx, bf16_t* T, long long ldt, int Cp,
+                    long long Rp) {
+  hipLaunchKernelGGL(k_transpose_bf16<float>, dim3((unsigned)((Rp + 31) / 32), (Cp + 31) / 32), dim3(256), 0, st, X, R,
+                     C, ldx, T, ldt, Cp, Rp);
+  check_launch("k_transpose_bf16");
+}
+void transpose_bf16(hipStream_t st, const bf16_t* X, long long R, int C, long long ldx, bf16_t* T, long long ldt, int Cp,
+                    long long Rp) {
+  hipLaunchKernelGGL(k_transpose_bf16<bf16_t>, dim3((unsigned)((Rp + 31) / 32), (Cp + 31) / 32), dim3(256), 0, st, X, R,
+                     C, ldx, T, ldt, Cp, Rp);
+  check_launch("k_transpose_bf16");
+}
+void to_bf16_pad(hipStream_t st, const float* X, long long R, int C, long long ldx, bf16_t* Y, int Cp) {
+  hipLaunchKernelGGL(k_to_bf16_pad, dim3(grid1(R * Cp)), dim3(256), 0, st, X, R, C, ldx, Y, Cp);
+  check_launch("k_to_bf16_pad");
+}
+void colsum(hipStream_t st, Buf<float>& part, const float* X, long long R, int C, float* out) {
+  const long long chunk = 4096;
+  const int np = (int)((R + chunk - 1) / chunk);
+  part.ensure((size_t)np * C);
+  hipLaunchKernelGGL(k_colsum_part, dim3(np), dim3(256), 0, st, X, R, C, chunk, part.p);
+  check_launch("k_colsum_part");
+  hipLaunchKernelGGL(k_colsum_final, dim3((C + 255) / 256), dim3(256), 0, st, part.p, np, C, out);
+  check_launch("k_colsum_final");
+}
+
+}  // namespace
+
+struct tm_cnn {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  tm_cnn_config cfg{};
+  std::vector<Layer> layers;  // conv layers, then the head (a 1x1 "layer" with K = C_L)
+  int G = 0, Nh = 0;
+  long long n_params = 0;
+  int batch = 0;  // images of the last forward pass
+  long long adam_t = 0;
+  Buf<float> w, g, m, v;          // parameters, gradients, Adam moments (flat)
+  std::vector<Buf<float>> act;    // act[0]: standardised input; act[l + 1]: layer l output; last: logits
+  Buf<bf16_t> col, colT, wb, wbT, dzb, dzT;
+  Buf<float> dA0, dA1, dcol, parts, csum, dlog, cell_loss, heat_t;
+  Buf<int> bin_t;
+  std::vector<float> h_loss;
+};
+
+namespace {
+
+void require(tm_cnn* h) {
+  CNN_REQUIRE(h, TM_ERR_INVALID, "handle is NULL");
+  CNN_HIP(hipSetDevice(h->device));
+}
+
+long long rows_of(const tm_cnn* h, const Layer& L) { return (long long)h->batch * L.so * L.so; }
+
+// forward of layer l from act[l] into act[l + 1]
+void forward_layer(tm_cnn* h, int l) {
+  const Layer& L = h->layers[l];
+  hipStream_t st = h->stream;
+  const bool head = l == (int)h->layers.size() - 1;
+  const long long M = rows_of(h, L);
+  if (head)
+    to_bf16_pad(st, h->act[l].p, M, L.cin, L.cin, h->col.p, L.Kp);
+  else {
+    hipLaunchKernelGGL(k_im2col, dim3(grid1(M * L.Kp)), dim3(256), 0, st, h->act[l].p, L.si, L.cin, L.so, L.stride, M,
+                       L.Kp, h->col.p);
+    check_launch("k_im2col");
+  }
+  to_bf16_pad(st, h->w.p + L.w_off, L.cout, L.K, L.K, h->wb.p, L.Kp);
+  gemm(st, h->parts, h->col.p, L.Kp, h->wb.p, L.Kp, h->act[l + 1].p, L.cout, M, L.cout, L.Kp, h->w.p + L.b_off, !head);
+}
+
+void alloc_for_batch(tm_cnn* h, int batch) {
+  h->batch = batch;
+  size_t col = 0, colT = 0, dzT = 0, dzb = 0, grad = 0, dcol = 0, wmax = 0;
+  for (size_t l = 0; l < h->layers.size(); ++l) {
+    const Layer& L = h->layers[l];
+    const long long M = (long long)batch * L.so * L.so, Mp = round16ll(M);
+    col = std::max(col, (size_t)(M * L.Kp));
+    colT = std::max(colT, (size_t)(Mp * L.Kp));
+    dzT = std::max(dzT, (size_t)(Mp * L.cout));
+    dzb = std::max(dzb, (size_t)(M * round16(L.cout)));
+    grad = std::max(grad, (size_t)(M * L.cout));
+    grad = std::max(grad, (size_t)((long long)batch * L.si * L.si * L.cin));
+    dcol = std::max(dcol, (size_t)(M * L.K));
+    wmax = std::max(wmax, (size_t)round16(L.cout) * L.Kp);
+    h->act[l + 1].ensure((size_t)(M * L.cout));
+  }
+  const Layer& L0 = h->layers[0];
+  h->act[0].ensure((size_t)batch * L0.si * L0.si);
+  h->col.ensure(col);
+  h->colT.ensure(colT);
+  h->dzT.ensure(dzT);
+  h->dzb.ensure(dzb);
+  h->dA0.ensure(grad);
+  h->dA1.ensure(grad);
+  h->dcol.ensure(dcol);
+  h->wb.ensure(wmax);
+  h->wbT.ensure(wmax);
+  const long long cells = (long long)batch * h->G * h->G;
+  h->dlog.ensure((size_t)cells * h->Nh);
+  h->cell_loss.ensure((size_t)cells * 2);
+  h->heat_t.ensure((size_t)cells);
+  h->bin_t.ensure((size_t)cells);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* tm_cnn_last_error(void) { return g_cerr.c_str(); }
+
+int tm_cnn_create(int device_ordinal, const tm_cnn_config* c, tm_cnn** out) {
+  return cguard([&] {
+    CNN_REQUIRE(c && out, TM_ERR_INVALID, "NULL argument");
+    *out = nullptr;
+    CNN_REQUIRE(c->n_layers >= 1 && c->n_layers <= TM_CNN_MAX_LAYERS, TM_ERR_INVALID, "n_layers must be 1..16");
+    CNN_REQUIRE(c->n_bins >= 1 && c->max_batch >= 1, TM_ERR_INVALID, "n_bins and max_batch must be positive");
+    CNN_REQUIRE(c->input_size >= 4, TM_ERR_INVALID, "input_size too small");
+    int dev_count = 0;
+    CNN_HIP(hipGetDeviceCount(&dev_count));
+    CNN_REQUIRE(device_ordinal >= 0 && device_ordinal < dev_count, TM_ERR_INVALID, "no such device");
+    auto h = std::make_unique<tm_cnn>();
+    h->device = device_ordinal;
+    h->cfg = *c;
+    CNN_HIP(hipSetDevice(device_ordinal));
+    int s = c->input_size, cin = 1;
+    long long off = 0;
+    for (int l = 0; l < c->n_layers; ++l) {
+      CNN_REQUIRE(c->strides[l] == 1 || c->strides[l] == 2, TM_ERR_INVALID, "strides must be 1 or 2");
+      CNN_REQUIRE(c->channels[l] >= 1 && c->channels[l] <= 4096, TM_ERR_INVALID, "channels must be 1..4096");
+      CNN_REQUIRE(s % c->strides[l] == 0, TM_ERR_INVALID, "input_size not divisible by the strides");
+      Layer L;
+      L.cin = cin;
+      L.cout = c->channels[l];
+      L.stride = c->strides[l];
+      L.si = s;
+      L.so = s / L.stride;
+      L.K = 9 * cin;
+      L.Kp = round16(L.K);
+      L.w_off = off;
+      off += (long long)L.cout * L.K;
+      L.b_off = off;
+      off += L.cout;
+      h->layers.push_back(L);
+      s = L.so;
+      cin = L.cout;
+    }
+    h->G = s;
+    h->Nh = 1 + c->n_bins;
+    Layer hd;
+    hd.cin = cin;
+    hd.cout = h->Nh;
+    hd.stride = 1;
+    hd.si = hd.so = s;
+    hd.K = cin;
+    hd.Kp = round16(cin);
+    hd.w_off = off;
+    off += (long long)hd.cout * hd.K;
+    hd.b_off = off;
+    off += hd.cout;
+    h->layers.push_back(hd);
+    h->n_params = off;
+    h->act.resize(h->layers.size() + 1);
+    CNN_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    h->w.ensure(off);
+    h->g.ensure(off);
+    h->m.ensure(off);
+    h->v.ensure(off);
+    CNN_HIP(hipMemsetAsync(h->g.p, 0, off * sizeof(float), h->stream));
+    alloc_for_batch(h.get(), c->max_batch);
+    h->batch = 0;
+    CNN_HIP(hipStreamSynchronize(h->stream));
+    *out = h.release();
+  });
+}
+
+int tm_cnn_destroy(tm_cnn* h) {
+  return cguard([&] {
+    if (!h) return;
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+  });
+}
+
+int tm_cnn_shape(tm_cnn* h, long long* n_params, int* grid) {
+  return cguard([&] {
+    CNN_REQUIRE(h, TM_ERR_INVALID, "handle is NULL");
+    if (n_params) *n_params = h->n_params;
+    if (grid) *grid = h->G;
+  });
+}
+
+int tm_cnn_set_params(tm_cnn* h, const float* params) {
+  return cguard([&] {
+    require(h);
+    CNN_REQUIRE(params, TM_ERR_INVALID, "params is NULL");
+    CNN_HIP(hipMemcpy(h->w.p, params, h->n_params * sizeof(float), hipMemcpyHostToDevice));
+    CNN_HIP(hipMemset(h->m.p, 0, h->n_params * sizeof(float)));
+    CNN_HIP(hipMemset(h->v.p, 0, h->n_params * sizeof(float)));
+    h->adam_t = 0;
+  });
+}
+
+int tm_cnn_get_params(tm_cnn* h, float* params) {
+  return cguard([&] {
+    require(h);
+    CNN_REQUIRE(params, TM_ERR_INVALID, "params is NULL");
+    CNN_HIP(hipStreamSynchronize(h->stream));
+    CNN_HIP(hipMemcpy(params, h->w.p, h->n_params * sizeof(float), hipMemcpyDeviceToHost));
+  });
+}
+
+int tm_cnn_init_params(tm_cnn* h, unsigned long long seed) {
+  return cguard([&] {
+    require(h);
+    std::vector<float> p((size_t)h->n_params, 0.f);
+    std::mt19937_64 rng(seed);
+    for (const Layer& L : h->layers) {
+      std::normal_distribution<double> nd(0.0, std::sqrt(2.0 / (double)L.K));
+      for (long long e = 0; e < (long long)L.cout * L.K; ++e) p[L.w_off + e] = (float)nd(rng);
+    }
+    CNN_HIP(hipMemcpy(h->w.p, p.data(), p.size() * sizeof(float), hipMemcpyHostToDevice));
+    CNN_HIP(hipMemset(h->m.p, 0, h->n_params * sizeof(float)));
+    CNN_HIP(hipMemset(h->v.p, 0, h->n_params * sizeof(float)));
+    h->adam_t = 0;
+  });
+}
+
+int tm_cnn_forward(tm_cnn* h, const float* images, int batch, float* logits) {
+  return cguard([&] {
+    require(h);
+    CNN_REQUIRE(images, TM_ERR_INVALID, "images is NULL");
+    CNN_REQUIRE(batch >= 1 && batch <= h->cfg.max_batch, TM_ERR_INVALID, "batch must be 1..max_batch");
+    h->batch = batch;
+    const long long npx = (long long)h->cfg.input_size * h->cfg.input_size;
+    CNN_HIP(hipMemcpyAsync(h->act[0].p, images, batch * npx * sizeof(float), hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_standardize, dim3(batch), dim3(256), 0, h->stream, h->act[0].p, npx);
+    check_launch("k_standardize");
+    for (int l = 0; l < (int)h->layers.size(); ++l) forward_layer(h, l);
+    if (logits) {
+      const long long cells = (long long)batch * h->G * h->G;
+      CNN_HIP(hipMemcpyAsync(logits, h->act.back().p, cells * h->Nh * sizeof(float), hipMemcpyDeviceToHost,
+                             h->stream));
+    }
+    CNN_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+int tm_cnn_backward(tm_cnn* h, const float* heat_target, const int* bin_target, float pos_weight, float* loss,
+                    float* grads) {
+  return cguard([&] {
+    require(h);
+    CNN_REQUIRE(heat_target && bin_target, TM_ERR_INVALID, "NULL target");
+    CNN_REQUIRE(h->batch > 0, TM_ERR_STATE, "no forward pass to differentiate");
+    hipStream_t st = h->stream;
+    const int B = h->batch, Nh = h->Nh;
+    const long long cells = (long long)B * h->G * h->G;
+    long long n_bin = 0;
+    for (long long c = 0; c < cells; ++c) {
+      CNN_REQUIRE(bin_target[c] < h->cfg.n_bins, TM_ERR_INVALID, "bin target out of range");
+      n_bin += bin_target[c] >= 0;
+    }
+    CNN_HIP(hipMemcpyAsync(h->heat_t.p, heat_target, cells * sizeof(float), hipMemcpyHostToDevice, st));
+    CNN_HIP(hipMemcpyAsync(h->bin_t.p, bin_target, cells * sizeof(int), hipMemcpyHostToDevice, st));
+    const float heat_scale = 1.f / (float)cells, bin_scale = 1.f / (float)std::max(1LL, n_bin);
+    hipLaunchKernelGGL(k_loss, dim3((unsigned)cells), dim3(64), 0, st, h->act.back().p, Nh, h->heat_t.p, h->bin_t.p,
+                       pos_weight, heat_scale, bin_scale, h->dlog.p, h->cell_loss.p);
+    check_launch("k_loss");
+    // walk the layers backwards; dcur holds dL/d(output of layer l) (pre-activation for the head)
+    float* dcur = h->dlog.p;
+    float* dnext = h->dA0.p;
+    for (int l = (int)h->layers.size() - 1; l >= 0; --l) {
+      const Layer& L = h->layers[l];
+      const bool head = l == (int)h->layers.size() - 1;
+      const long long M = rows_of(h, L), Mp = round16ll(M);
+      if (!head) {
+        hipLaunchKernelGGL(k_relu_bwd, dim3(grid1(M * L.cout)), dim3(256), 0, st, dcur, h->act[l + 1].p, M * L.cout);
+        check_launch("k_relu_bwd");
+      }
+      colsum(st, h->csum, dcur, M, L.cout, h->g.p + L.b_off);
+      // the layer's input as the product operand (recomputed), then its transpose [Kp][Mp]
+      if (head)
+        to_bf16_pad(st, h->act[l].p, M, L.cin, L.cin, h->col.p, L.Kp);
+      else {
+        hipLaunchKernelGGL(k_im2col, dim3(grid1(M * L.Kp)), dim3(256), 0, st, h->act[l].p, L.si, L.cin, L.so, L.stride,
+                           M, L.Kp, h->col.p);
+        check_launch("k_im2col");
+      }
+      transpose_bf16(st, h->col.p, M, L.Kp, L.Kp, h->colT.p, Mp, L.Kp, Mp);
+      // dZ^T [cout][Mp] and dW[cout][K] = dZ^T . col^T^T
+      transpose_bf16(st, dcur, M, L.cout, L.cout, h->dzT.p, Mp, L.cout, Mp);
+      gemm(st, h->parts, h->dzT.p, (int)Mp, h->colT.p, (int)Mp, h->g.p + L.w_off, L.K, L.cout, L.K, (int)Mp, nullptr,
+           false);
+      if (l == 0) break;
+      // dcol[M][K] = dZ[M][cout16] . W^T[K][cout16]^T, then the gather into the input's gradient
+      const int c16 = round16(L.cout);
+      to_bf16_pad(st, dcur, M, L.cout, L.cout, h->dzb.p, c16);
+      transpose_bf16(st, h->w.p + L.w_off, L.cout, L.K, L.K, h->wbT.p, c16, L.K, c16);
+      if (head) {
+        gemm(st, h->parts, h->dzb.p, c16, h->wbT.p, c16, dnext, L.K, M, L.K, c16, nullptr, false);
+      } else {
+        gemm(st, h->parts, h->dzb.p, c16, h->wbT.p, c16, h->dcol.p, L.K, M, L.K, c16, nullptr, false);
+        const long long total = (long long)B * L.si * L.si * L.cin;
+        hipLaunchKernelGGL(k_col2im, dim3(grid1(total)), dim3(256), 0, st, h->dcol.p, L.K, L.si, L.cin, L.so, L.stride,
+                           total, dnext);
+        check_launch("k_col2im");
+      }
+      dcur = dnext;
+      dnext = (dnext == h->dA0.p) ? h->dA1.p : h->dA0.p;
+    }
+    h->h_loss.resize((size_t)cells * 2);
+    CNN_HIP(hipMemcpyAsync(h->h_loss.data(), h->cell_loss.p, cells * 2 * sizeof(float), hipMemcpyDeviceToHost, st));
+    if (grads) CNN_HIP(hipMemcpyAsync(grads, h->g.p, h->n_params * sizeof(float), hipMemcpyDeviceToHost, st));
+    CNN_HIP(hipStreamSynchronize(st));
+    if (loss) {
+      double lh = 0.0, lb = 0.0;
+      for (long long c = 0; c < cells; ++c) {
+        lh += h->h_loss[2 * c];
+        lb += h->h_loss[2 * c + 1];
+      }
+      loss[0] = (float)(lh / (double)cells);
+      loss[1] = (float)(lb / (double)std::max(1LL, n_bin));
+    }
+  });
+}
+
+int tm_cnn_adam_step(tm_cnn* h, float lr, float beta1, float beta2, float eps) {
+  return cguard([&] {
+    require(h);
+    CNN_REQUIRE(lr > 0.f && beta1 >= 0.f && beta1 < 1.f && beta2 >= 0.f && beta2 < 1.f && eps > 0.f, TM_ERR_INVALID,
+                "invalid Adam hyper-parameters");
+    h->adam_t += 1;
+    const double c1 = 1.0 - std::pow((double)beta1, (double)h->adam_t);
+    const double c2 = 1.0 - std::pow((double)beta2, (double)h->adam_t);
+    const float lr_t = (float)(lr * std::sqrt(c2) / c1);
+    hipLaunchKernelGGL(k_adam, dim3(grid1(h->n_params)), dim3(256), 0, h->stream, h->w.p, h->g.p, h->m.p, h->v.p,
+                       h->n_params, lr_t, beta1, beta2, eps);
+    check_launch("k_adam");
+    CNN_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+int tm_cnn_get_activation(tm_cnn* h, int layer, float* out) {
+  return cguard([&] {
+    require(h);
+    CNN_REQUIRE(out, TM_ERR_INVALID, "out is NULL");
+    CNN_REQUIRE(h->batch > 0, TM_ERR_STATE, "no forward pass");
+    CNN_REQUIRE(layer >= 0 && layer < (int)h->layers.size(), TM_ERR_INVALID, "layer out of range");
+    const Layer& L = h->layers[layer];
+    const size_t n = (size_t)rows_of(h, L) * L.cout;
+    CNN_HIP(hipStreamSynchronize(h->stream));
+    CNN_HIP(hipMemcpy(out, h->act[layer + 1].p, n * sizeof(float), hipMemcpyDeviceToHost));
+  });
+}
+
+int tm_cnn_debug_gemm(int device_ordinal, const float* A, const float* B, int M, int N, int K, float* C) {
+  return cguard([&] {
+    CNN_REQUIRE(A && B && C && M > 0 && N > 0 && K > 0, TM_ERR_INVALID, "invalid arguments");
+    CNN_HIP(hipSetDevice(device_ordinal));
+    hipStream_t st;
+    CNN_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    struct S {
+      hipStream_t s;
+      ~S() { (void)hipStreamDestroy(s); }
+    } guard{st};
+    const int Kp = round16(K);
+    Buf<float> a, b, c, parts;
+    Buf<bf16_t> ab, bb;
+    a.ensure((size_t)M * K);
+    b.ensure((size_t)N * K);
+    c.ensure((size_t)M * N);
+    ab.ensure((size_t)M * Kp);
+    bb.ensure((size_t)N * Kp);
+    CNN_HIP(hipMemcpyAsync(a.p, A, (size_t)M * K * sizeof(float), hipMemcpyHostToDevice, st));
+    CNN_HIP(hipMemcpyAsync(b.p, B, (size_t)N * K * sizeof(float), hipMemcpyHostToDevice, st));
+    to_bf16_pad(st, a.p, M, K, K, ab.p, Kp);
+    to_bf16_pad(st, b.p, N, K, K, bb.p, Kp);
+    gemm(st, parts, ab.p, Kp, bb.p, Kp, c.p, N, M, N, Kp, nullptr, false);
+    CNN_HIP(hipMemcpyAsync(C, c.p, (size_t)M * N * sizeof(float), hipMemcpyDeviceToHost, st));
+    CNN_HIP(hipStreamSynchronize(st));
+  });
+}
+
+}  // extern "C"

@@ -6,7 +6,7 @@ import re
 
 import numpy as np
 
-from cistem_downstream_bah_amd import core, mrc, refine, synth
+from cistem_downstream_bah_amd import cnn, core, mrc, refine, synth
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -35,6 +35,22 @@ def test_refine_exports_and_structs_match_header():
         names = [f.strip() for group in fields for f in group.split(",")]
         assert names == [f[0] for f in cls._fields_], struct
     assert ctypes.sizeof(refine.RefinedPeak) == 104
+
+
+def test_cnn_exports_and_config_match_header():
+    header = open(os.path.join(ROOT, "include", "tm_cnn.h")).read()
+    declared = set(re.findall(r"^(?:int|const char\*)\s+(tm_\w+)\(", header, re.M))
+    assert declared == set(cnn.CNN_EXPORTS)
+    L = ctypes.CDLL(core.LIB_PATH)
+    for name in declared:
+        assert hasattr(L, name), name
+    body = header[header.index("typedef struct tm_cnn_config"):header.index("} tm_cnn_config;")]
+    fields = re.findall(r"^\s+int\s+(\w+)", body, re.M)
+    assert fields == [f[0] for f in cnn.CnnConfig._fields_]
+    assert ctypes.sizeof(cnn.CnnConfig) == 4 * (4 + 2 * cnn.MAX_LAYERS)
+    Lc = cnn._lib()
+    assert Lc.tm_cnn_destroy(None) == 0
+    assert Lc.tm_cnn_forward(None, None, 1, None) != 0 and Lc.tm_cnn_last_error()
 
 
 def test_refine_errors_are_returned_not_fatal():

@@ -1,0 +1,125 @@
+"""GPU: the proposal network's kernels (include/tm_cnn.h) against the oracle (oracle/cnn_oracle.py).
+
+* bf16 MFMA product C = A . B^T: ragged M, N, K, a 601-column head shape and a split-K case, against
+  float64 on the same bf16-rounded operands, within 1e-5 of sum |a| |b| per element
+* forward pass: every layer's activations and the logits within 1e-5 (relative L2) of the oracle
+  with the device's bf16 rounding points (the remainder is FP32 accumulation order)
+* backward pass: the loss terms within 1e-4 relative and every parameter tensor's gradient within
+  1e-4 (relative L2) of the oracle; the oracle's own gradient is pinned by finite differences on the CPU
+* training: 40 Adam steps on a fixed batch lower the loss by more than half; parameters round-trip
+  through set/get; the seeded initialisation is deterministic
+"""
+import numpy as np
+import pytest
+
+from cistem_downstream_bah_amd import cnn
+from oracle import cnn_oracle as C
+
+pytestmark = pytest.mark.gpu
+
+
+def _bf(x):
+    return C.bf16(x)
+
+
+@pytest.mark.parametrize("M,N,K", [(37, 50, 23), (130, 70, 64), (200, 601, 128), (5, 3, 5000), (64, 64, 16)])
+def test_gemm(M, N, K):
+    print(f"case: gemm {M}x{N}x{K}", flush=True)
+    rng = np.random.default_rng(M * N + K)
+    A = rng.standard_normal((M, K)).astype(np.float32)
+    B = rng.standard_normal((N, K)).astype(np.float32)
+    got = cnn.debug_gemm(A, B)
+    a, b = _bf(A), _bf(B)
+    ref = a @ b.T
+    bound = np.abs(a) @ np.abs(b).T
+    assert np.all(np.abs(got - ref) <= 1e-5 * bound + 1e-30)
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def _case():
+    spec = cnn.NetSpec(input_size=64, channels=[8, 16, 16], strides=[2, 2, 1], n_bins=20)
+    cfg = C.Config(64, spec.channels, spec.strides, spec.n_bins)
+    rng = np.random.default_rng(5)
+    flat = C.init_params(cfg, 7)
+    flat[:] += 0.02 * rng.standard_normal(flat.size).astype(np.float32)
+    images = rng.standard_normal((3, 64, 64)).astype(np.float32)
+    images[:, 20:30, 30:40] += 2.0
+    G = cfg.grid
+    heat = np.zeros((3, G, G), np.float32)
+    bins = -np.ones((3, G, G), np.int32)
+    for b in range(3):
+        for (y, x, t) in [(3, 5, 2 + b), (10, 12, 19 - b), (14, 1, 7)]:
+            heat[b, y, x] = 1
+            bins[b, y, x] = t
+    return spec, cfg, flat, images, heat, bins
+
+
+def test_forward_matches_oracle():
+    print("case: cnn forward 64^2, 3 layers, 20 bins, batch 3", flush=True)
+    spec, cfg, flat, images, *_ = _case()
+    net = cnn.ProposalNet(spec, max_batch=4, seed=None)
+    try:
+        net.set_params(flat)
+        logits = net.forward(images)
+        acts = C.forward(cfg, flat, images, use_bf16=True)
+        for l in range(len(spec.channels)):
+            e = _rel(net.activation(l, 3), acts[l + 1])
+            print(f"layer {l}: rel {e:.2e}", flush=True)
+            assert e <= 1e-5
+        e = _rel(logits, acts[-1])
+        print(f"logits: rel {e:.2e}", flush=True)
+        assert e <= 1e-5
+    finally:
+        net.close()
+
+
+def test_backward_matches_oracle():
+    print("case: cnn backward 64^2, 3 layers, 20 bins, batch 3", flush=True)
+    spec, cfg, flat, images, heat, bins = _case()
+    net = cnn.ProposalNet(spec, max_batch=4, seed=None)
+    try:
+        net.set_params(flat)
+        net.forward(images, want_logits=False)
+        (lh, lb), g = net.backward(heat, bins, pos_weight=5.0, want_grads=True)
+        (rh, rb), rg = C.backward(cfg, flat, images, heat, bins, 5.0, use_bf16=True)
+        assert lh == pytest.approx(rh, rel=1e-4) and lb == pytest.approx(rb, rel=1e-4)
+        o = 0
+        worst = 0.0
+        for ws, bs in cfg.shapes():
+            for shp in (ws, bs):
+                n = int(np.prod(shp))
+                e = _rel(g[o:o + n], rg[o:o + n])
+                worst = max(worst, e)
+                assert e <= 1e-4, (shp, e)
+                o += n
+        print(f"gradients: worst tensor rel {worst:.2e}, overall {_rel(g, rg):.2e}", flush=True)
+        assert _rel(g, rg) <= 1e-5
+    finally:
+        net.close()
+
+
+def test_training_lowers_the_loss_and_params_round_trip():
+    print("case: cnn training 40 Adam steps", flush=True)
+    spec, cfg, flat, images, heat, bins = _case()
+    net = cnn.ProposalNet(spec, max_batch=4, seed=11)
+    other = cnn.ProposalNet(spec, max_batch=4, seed=11)
+    try:
+        assert np.array_equal(net.params(), other.params())
+        losses = []
+        for _ in range(40):
+            net.forward(images, want_logits=False)
+            (lh, lb), _ = net.backward(heat, bins, pos_weight=5.0)
+            losses.append(lh + lb)
+            net.adam_step(lr=3e-3)
+        print(f"loss {losses[0]:.4f} -> {losses[-1]:.4f}", flush=True)
+        assert losses[-1] < 0.5 * losses[0]
+        p = net.params()
+        other.set_params(p)
+        assert np.array_equal(other.params(), p)
+        assert np.array_equal(other.forward(images), net.forward(images))
+    finally:
+        net.close()
+        other.close()
