@@ -1338,6 +1338,63 @@ int tm_refine_score(tm_refine* h, int peak_x, int peak_y, float current_defocus,
   });
 }
 
+int tm_refine_score_batch(tm_refine* h, int n, const int* peak_xy, const float* angles, float current_defocus,
+                          float* out) {
+  return rguard([&] {
+    require_ready(h);
+    TM_REQUIRE(n >= 0 && (n == 0 || (peak_xy && angles && out)), TM_ERR_INVALID, "NULL argument");
+    // windows in order of first appearance, each with its requests in input order
+    std::vector<std::pair<int, int>> wins;
+    std::vector<std::vector<int>> reqs_of;
+    std::map<std::pair<int, int>, int> index;
+    for (int i = 0; i < n; ++i) {
+      const auto key = std::make_pair(peak_xy[2 * i], peak_xy[2 * i + 1]);
+      auto it = index.find(key);
+      if (it == index.end()) {
+        it = index.emplace(key, (int)wins.size()).first;
+        wins.push_back(key);
+        reqs_of.emplace_back();
+      }
+      reqs_of[it->second].push_back(i);
+    }
+    const tm_refine_params& p = h->prm;
+    const float df1 = (p.defocus1 + current_defocus) / p.pixel_size, df2 = (p.defocus2 + current_defocus) / p.pixel_size;
+    std::vector<ScoreRequest> reqs;
+    std::vector<int> slots, which;
+    std::vector<ScoreResult> res;
+    auto flush = [&] {
+      if (reqs.empty()) return;
+      run_scores(h, reqs, slots, res);
+      for (size_t q = 0; q < reqs.size(); ++q) {
+        out[3 * which[q]] = res[q].x;
+        out[3 * which[q] + 1] = res[q].y;
+        out[3 * which[q] + 2] = res[q].value;
+      }
+      reqs.clear();
+      slots.clear();
+      which.clear();
+    };
+    for (size_t w0 = 0; w0 < wins.size(); w0 += h->K) {
+      const size_t w1 = std::min(wins.size(), w0 + (size_t)h->K);
+      std::vector<int> xs, ys, ss;
+      for (size_t w = w0; w < w1; ++w) {
+        xs.push_back(wins[w].first);
+        ys.push_back(wins[w].second);
+        ss.push_back((int)(w - w0));
+      }
+      make_windows(h, xs, ys, ss);
+      for (size_t w = w0; w < w1; ++w)
+        for (int i : reqs_of[w]) {
+          reqs.push_back(ScoreRequest{angles[3 * i], angles[3 * i + 1], angles[3 * i + 2], df1, df2, 1.0f});
+          slots.push_back((int)(w - w0));
+          which.push_back(i);
+          if ((int)reqs.size() == h->K) flush();
+        }
+      flush();  // the next group of windows reuses the slots
+    }
+  });
+}
+
 int tm_refine_set_timing(tm_refine* h, int enabled) {
   return rguard([&] {
     TM_REQUIRE(h, TM_ERR_INVALID, "handle is NULL");

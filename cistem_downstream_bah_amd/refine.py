@@ -24,7 +24,7 @@ OUTPUT_NAMES = ("best_mip", "best_scaled_mip", "best_psi", "best_theta", "best_p
                 "best_pixel_size")
 REFINE_EXPORTS = ["tm_prepare_refine_image", "tm_refine_create", "tm_refine_destroy", "tm_refine_init",
                   "tm_refine_run", "tm_refine_peak_list", "tm_refine_score", "tm_refine_set_timing",
-                  "tm_refine_get_timing", "tm_refine_last_error"]
+                  "tm_refine_get_timing", "tm_refine_last_error", "tm_refine_score_batch"]
 
 
 class RefineParams(ctypes.Structure):
@@ -64,6 +64,7 @@ def _lib():
                                     ctypes.c_int, ip]
         L.tm_refine_peak_list.argtypes = [vp, F32P, F32P, ctypes.c_int, ip]
         L.tm_refine_score.argtypes = [vp, ctypes.c_int, ctypes.c_int] + [ctypes.c_float] * 6 + [F32P]
+        L.tm_refine_score_batch.argtypes = [vp, ctypes.c_int, ip, F32P, ctypes.c_float, F32P]
         L.tm_refine_set_timing.argtypes = [vp, ctypes.c_int]
         i64p = ctypes.POINTER(ctypes.c_longlong)
         L.tm_refine_get_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_double), i64p,
@@ -173,6 +174,18 @@ class TemplateRefiner:
         _check(_lib().tm_refine_score(self.h, int(peak_x), int(peak_y), current_defocus, step, phi, theta, psi,
                                       pixel_size_factor, _f(out)))
         return float(out[0]), float(out[1]), float(out[2])
+
+    def score_batch(self, peak_xy: np.ndarray, angles: np.ndarray, current_defocus: float = 0.0) -> np.ndarray:
+        """TemplateScore for each (window, orientation) request: peak_xy (n, 2) int pixels from the
+        image centre, angles (n, 3) phi, theta, psi. Returns (n, 3): x, y offset and value."""
+        xy = np.ascontiguousarray(peak_xy, np.int32).reshape(-1, 2)
+        ang = np.ascontiguousarray(angles, np.float32).reshape(-1, 3)
+        if xy.shape[0] != ang.shape[0]:
+            raise ValueError("peak_xy and angles differ in length")
+        out = np.zeros((xy.shape[0], 3), np.float32)
+        _check(_lib().tm_refine_score_batch(self.h, xy.shape[0], xy.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                            _f(ang), current_defocus, _f(out)))
+        return out
 
     def set_timing(self, on: bool) -> None:
         _check(_lib().tm_refine_set_timing(self.h, int(on)))

@@ -98,6 +98,16 @@ int tm_refine_peak_list(tm_refine* h, const float* scaled_mip, float* out, int c
 int tm_refine_score(tm_refine* h, int peak_x, int peak_y, float current_defocus, float step, float phi, float theta,
                     float psi, float pixel_size_factor, float* out);
 
+/* Local exhaustive scoring, with no counterpart in the reference (the batched form of the
+ * TemplateScore calls of a refinement around proposed locations): request i scores the window at
+ * (peak_xy[2i], peak_xy[2i+1]) (pixels from the image centre) against the projection at
+ * angles[3i..3i+2] (phi, theta, psi) with the filter at defocus1|2 + current_defocus and writes the
+ * correlation peak's offset and value (x, y, value) to out[3i..3i+2], as tm_refine_score does.
+ * Requests that share a window share one window transform; max_in_flight windows and requests
+ * run per batch. */
+int tm_refine_score_batch(tm_refine* h, int n, const int* peak_xy, const float* angles, float current_defocus,
+                          float* out);
+
 /* Per stage of the TemplateScore batches of the last tm_refine_run (extract_slice_cols,
  * slice_rows_c2r, projection_r2c, score_product, correlation_c2r, score_peak), measured with HIP
  * events on the handle's stream when timing is enabled: total ms, number of launches and items
