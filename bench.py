@@ -102,6 +102,9 @@ def main():
     ap.add_argument("--eval", type=int, default=40, help="proposal: held-out micrographs")
     ap.add_argument("--proposals", type=str, default="16,20", help="proposal: locations proposed per micrograph")
     ap.add_argument("--bins", type=str, default="10,20,40", help="proposal: orientation bins per location")
+    ap.add_argument("--weights", type=str, default="",
+                    help="proposal: load the network from this .npy instead of training (labels the eval set only)")
+    ap.add_argument("--save-weights", type=str, default="", help="proposal: save the trained network (.npy)")
     ap.add_argument("--workload", choices=["config1", "config2", "refine", "proposal"], default="config1",
                     help="config2: eight micrographs of mixed sizes (own JSON line, not the headline)")
     ap.add_argument("--concurrency", type=str, default="1,2,4",
@@ -334,15 +337,23 @@ def bench_proposal(args):
 
     t0 = time.time()
     study = PR.ProposalStudy(PR.StudyConfig())
-    train = study.labelled(range(args.train), workers=4, log=log)
-    t_label = time.time() - t0
-    npk = sum(len(d.peaks) for d in train)
-    log(f"labelled {len(train)} micrographs ({npk} peaks) in {t_label:.1f}s")
     net = cnn.ProposalNet(cnn.NetSpec(), max_batch=16, seed=1)
-    t0 = time.time()
-    hist = study.train(net, train, epochs=args.epochs, batch=16, lr=1e-3, log=log)
-    t_train = time.time() - t0
-    del train
+    t_label = t_train = 0.0
+    hist = None
+    if args.weights:
+        net.set_params(np.load(args.weights))
+        log(f"network loaded from {args.weights}")
+    else:
+        train = study.labelled(range(args.train), workers=4, log=log)
+        t_label = time.time() - t0
+        npk = sum(len(d.peaks) for d in train)
+        log(f"labelled {len(train)} micrographs ({npk} peaks) in {t_label:.1f}s")
+        t0 = time.time()
+        hist = study.train(net, train, epochs=args.epochs, batch=16, lr=1e-3, log=log)
+        t_train = time.time() - t0
+        del train
+        if args.save_weights:
+            np.save(args.save_weights, net.params())
     bins = [int(b) for b in args.bins.split(",")]
     evalset = [study.micrograph(100000 + i) for i in range(args.eval)]
     # warm-up of both paths (first-call allocations)
@@ -354,6 +365,7 @@ def bench_proposal(args):
     variants += [("random_locations_union", Ks[-1], mb) for mb in bins]
     rows = {v: [0, 0, 0.0, 0] for v in variants}
     t_ex, n_peaks = 0.0, 0
+    misses = []
     for i, m in enumerate(evalset):
         peaks, dt = study.exhaustive(m)
         t_ex += dt
@@ -363,6 +375,14 @@ def bench_proposal(args):
             props = study.random_proposals(rng, net.grid, k, mb) if kind.startswith("random") else None
             dets, n, t = study.detect(net, m, k, mb, proposals=props, bins=kind.replace("random_locations_", ""))
             rec, same, _ = PR.recall(dets, peaks)
+            if v == ("network", 16, 40):
+                for p in peaks:
+                    dd = min(((d.x - p["x"]) ** 2 + (d.y - p["y"]) ** 2) ** 0.5 for d in dets)
+                    if dd > 2.0:
+                        dp = min([((q["x"] - p["x"]) ** 2 + (q["y"] - p["y"]) ** 2) ** 0.5 for q in peaks if q is not p]
+                                 or [1e9])
+                        misses.append({"z": round(p["z"], 3), "nearest_detection_px": round(dd, 1),
+                                       "nearest_other_peak_px": round(dp, 1)})
             r = rows[v]
             r[0] += int(round(rec * len(peaks)))
             r[1] += int(round(same * rec * len(peaks)))
@@ -396,9 +416,11 @@ def bench_proposal(args):
         "recall_at_chosen": pick["recall"],
         "speedup_vs_exhaustive": pick["speedup_vs_exhaustive"],
         "table": table,
-        "training": {"label_seconds": round(t_label, 1), "train_seconds": round(t_train, 1),
-                     "loss_first_epoch": [round(v, 4) for v in hist[0]],
-                     "loss_last_epoch": [round(v, 4) for v in hist[-1]]},
+        "misses_network_16_40": misses,
+        "training": ({"loaded_from": args.weights} if hist is None else
+                     {"label_seconds": round(t_label, 1), "train_seconds": round(t_train, 1),
+                      "loss_first_epoch": [round(v, 4) for v in hist[0]],
+                      "loss_last_epoch": [round(v, 4) for v in hist[-1]]}),
     }
     net.close()
     print(json.dumps(out), flush=True)
