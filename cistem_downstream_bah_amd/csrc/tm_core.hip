@@ -122,8 +122,10 @@ void launch_mid_t(tm_core* h, int nb) {
   } else {
   const int ntiles = (h->Wh + KC - 1) / KC;
   const int grid = ((ntiles + 7) / 8) * 8;
-  hipLaunchKernelGGL((tmk::k_mid<HH, KC, NTC>), dim3(grid), dim3(KC * NTC), 0, h->stream, h->rows.p, h->inpT.p, h->Z.p,
-                     h->Wh, h->T, nb, ntiles, h->twiddle(HH));
+  // orientations spread over grid rows until about 2048 blocks are in flight
+  const int rows_y = std::max(1, std::min(nb, (2048 + grid - 1) / grid));
+  hipLaunchKernelGGL((tmk::k_mid<HH, KC, NTC>), dim3(grid, rows_y), dim3(KC * NTC), 0, h->stream, h->rows.p, h->inpT.p,
+                     h->Z.p, h->Wh, h->T, nb, ntiles, h->twiddle(HH));
   check_launch("k_mid");
   }
 }

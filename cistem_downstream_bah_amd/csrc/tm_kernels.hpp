@@ -599,7 +599,9 @@ __device__ __forceinline__ cf unpack_half2(unsigned w, float inv_scale) {
 // (match_template.cpp:1083), inverse column FFT, and write Z[b][tile][y][KC] (column tiles
 // contiguous, so every store of the tile is a full cache line).
 // Tiles are mapped so that consecutive tiles run on the same XCD (workgroups are dispatched
-// round-robin over the 8 XCDs), letting partial cache lines of Z merge in that XCD's L2.
+// round-robin over the 8 XCDs), letting partial cache lines of Z merge in that XCD's L2. The
+// orientations of the batch are spread over gridDim.y (block row y takes b = y, y + gridDim.y, ..),
+// so that small images, with few column tiles, still fill the GPU.
 template <int H, int KC, int NTC>
 __global__ __launch_bounds__(KC* NTC) void k_mid(const cf* __restrict__ rows, const cf* __restrict__ inpT,
                                                  cf* __restrict__ Z, int Wh, int T, int nb, int ntiles,
@@ -617,7 +619,7 @@ __global__ __launch_bounds__(KC* NTC) void k_mid(const cf* __restrict__ rows, co
   const int jlo = H / 2 - T / 2;
   const size_t rows_bs = (size_t)T * Wh;
   const size_t z_bs = (size_t)ntiles * H * KC;
-  for (int b = 0; b < nb; ++b) {
+  for (int b = blockIdx.y; b < nb; b += gridDim.y) {
     for (int e = ctid; e < H; e += NTC) buf[c][lpad(e)] = cmake(0.f, 0.f);
     __syncthreads();
     const cf* rb = rows + b * rows_bs;
