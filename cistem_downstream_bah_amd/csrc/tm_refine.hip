@@ -1306,6 +1306,20 @@ int tm_refine_init(tm_refine* h, const tm_refine_params* p, const float* whitene
   });
 }
 
+int tm_refine_set_image(tm_refine* h, const float* whitened_image, const float* curve_x, const float* curve_y,
+                        int n_curve) {
+  return rguard([&] {
+    require_ready(h);
+    TM_REQUIRE(whitened_image && curve_x && curve_y, TM_ERR_INVALID, "NULL argument");
+    TM_REQUIRE(n_curve == h->n_curve, TM_ERR_INVALID, "the whitening curve length differs from the handle's");
+    const size_t npx = (size_t)h->prm.image_x * h->prm.image_y;
+    TM_HIP(hipMemcpyAsync(h->img.p, whitened_image, npx * sizeof(float), hipMemcpyHostToDevice, h->stream));
+    TM_HIP(hipMemcpyAsync(h->curve_x.p, curve_x, n_curve * sizeof(float), hipMemcpyHostToDevice, h->stream));
+    TM_HIP(hipMemcpyAsync(h->curve_y.p, curve_y, n_curve * sizeof(float), hipMemcpyHostToDevice, h->stream));
+    TM_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
 int tm_refine_peak_list(tm_refine* h, const float* scaled_mip, float* out, int capacity, int* n_peaks) {
   return rguard([&] {
     require_ready(h);

@@ -102,6 +102,7 @@ class ProposalStudy:
         self.dirs = fibonacci_directions()
         self.orient_bin = orientation_bin(self.orients[:, 0], self.orients[:, 1], self.orients[:, 2], self.dirs)
         self.members = [np.nonzero(self.orient_bin == b)[0] for b in range(N_BINS)]
+        self._refiner = None  # one refinement handle, its image replaced per micrograph
         cgrid = MT.search_grid("C1", 2 * cfg.angular_step)
         cpsi = MT.psi_list(0.0, 360.0, 2 * cfg.in_plane_angular_step)
         self.coarse = np.array([(g[0], g[1], s) for g in cgrid for s in cpsi], np.float32)
@@ -273,9 +274,16 @@ class ProposalStudy:
         if proposals is None:
             logits = net.forward(micrograph[None])[0]
             proposals = self.propose(logits, k, m)
-        refiner = TemplateRefiner(micrograph, self.volume, RefineSettings(defocus1=self.cfg.defocus,
-                                                                          defocus2=self.cfg.defocus, padding=1.0),
-                                  template_spectrum=self.tpl_spectrum)
+        t1 = time.perf_counter()
+        refiner = self._refiner
+        if refiner is None:
+            refiner = self._refiner = TemplateRefiner(micrograph, self.volume,
+                                                      RefineSettings(defocus1=self.cfg.defocus, defocus2=self.cfg.defocus,
+                                                                     padding=1.0, max_in_flight=2048),
+                                                      template_spectrum=self.tpl_spectrum, device=self.device)
+        else:
+            refiner.set_image(micrograph, device=self.device)
+        t2 = time.perf_counter()
         n0 = 0
         if bins in ("coarse", "union"):
             cb, n0 = self.coarse_bins(refiner, proposals, net.grid, m)
@@ -284,28 +292,40 @@ class ProposalStudy:
             else:
                 proposals = [(c, np.unique(np.concatenate([nb, b]))) for (c, nb), b in zip(proposals, cb)]
         dets, n = self.refine(refiner, proposals, net.grid)
-        refiner.close()
-        return dets, n + n0, time.perf_counter() - t0
+        t3 = time.perf_counter()
+        self.last_times = {"network_and_proposals": t1 - t0, "image_preparation": t2 - t1, "scores": t3 - t2}
+        return dets, n + n0, t3 - t0
 
     def random_proposals(self, rng, grid: int, k: int, m: int):
         cells = rng.choice(grid * grid, size=k, replace=False)
         return [((int(c) // grid, int(c) % grid), rng.choice(N_BINS, size=m, replace=False)) for c in cells]
 
 
-def recall(dets: list, peaks: list, radius: float = 2.0, grid_orients=None):
-    """Fraction of exhaustive peaks with a detection within radius pixels, and the fraction of those
-    whose detection has the peak's orientation."""
-    if not peaks:
-        return 1.0, 1.0, 0
-    hit = same = 0
+def match(dets: list, peaks: list, radius: float = 2.0) -> list:
+    """Per exhaustive peak: (found, same orientation), found if a detection lies within radius
+    pixels (the nearest one is compared for the orientation)."""
+    out = []
     for p in peaks:
         best = None
         for d in dets:
             r2 = (d.x - p["x"]) ** 2 + (d.y - p["y"]) ** 2
             if r2 <= radius * radius and (best is None or r2 < best[0]):
                 best = (r2, d)
-        if best is not None:
-            hit += 1
+        if best is None:
+            out.append((False, False))
+        else:
             d = best[1]
-            same += (abs(d.phi - p["phi"]) < 1e-3 and abs(d.theta - p["theta"]) < 1e-3 and abs(d.psi - p["psi"]) < 1e-3)
+            out.append((True, abs(d.phi - p["phi"]) < 1e-3 and abs(d.theta - p["theta"]) < 1e-3
+                        and abs(d.psi - p["psi"]) < 1e-3))
+    return out
+
+
+def recall(dets: list, peaks: list, radius: float = 2.0):
+    """Fraction of exhaustive peaks with a detection within radius pixels, and the fraction of those
+    whose detection has the peak's orientation."""
+    if not peaks:
+        return 1.0, 1.0, 0
+    m = match(dets, peaks, radius)
+    hit = sum(f for f, _ in m)
+    same = sum(s for f, s in m if f)
     return hit / len(peaks), (same / hit if hit else 0.0), len(peaks)

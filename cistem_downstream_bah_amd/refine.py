@@ -24,7 +24,7 @@ OUTPUT_NAMES = ("best_mip", "best_scaled_mip", "best_psi", "best_theta", "best_p
                 "best_pixel_size")
 REFINE_EXPORTS = ["tm_prepare_refine_image", "tm_refine_create", "tm_refine_destroy", "tm_refine_init",
                   "tm_refine_run", "tm_refine_peak_list", "tm_refine_score", "tm_refine_set_timing",
-                  "tm_refine_get_timing", "tm_refine_last_error", "tm_refine_score_batch"]
+                  "tm_refine_get_timing", "tm_refine_last_error", "tm_refine_score_batch", "tm_refine_set_image"]
 
 
 class RefineParams(ctypes.Structure):
@@ -65,6 +65,7 @@ def _lib():
         L.tm_refine_peak_list.argtypes = [vp, F32P, F32P, ctypes.c_int, ip]
         L.tm_refine_score.argtypes = [vp, ctypes.c_int, ctypes.c_int] + [ctypes.c_float] * 6 + [F32P]
         L.tm_refine_score_batch.argtypes = [vp, ctypes.c_int, ip, F32P, ctypes.c_float, F32P]
+        L.tm_refine_set_image.argtypes = [vp, F32P, F32P, F32P, ctypes.c_int]
         L.tm_refine_set_timing.argtypes = [vp, ctypes.c_int]
         i64p = ctypes.POINTER(ctypes.c_longlong)
         L.tm_refine_get_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_double), i64p,
@@ -160,6 +161,16 @@ class TemplateRefiner:
             self.close()
         except Exception:
             pass
+
+    def set_image(self, micrograph: np.ndarray, whitened=None, device: int = 0) -> None:
+        """Replaces the micrograph (same size), keeping the template: tm_refine_set_image."""
+        if micrograph.shape != (self.my, self.mx):
+            raise ValueError(f"micrograph shape {micrograph.shape} != {(self.my, self.mx)}")
+        img, cx, cy = whitened if whitened is not None else prepare_refine_image(micrograph, device)
+        self.whitened, self.curve_x, self.curve_y = img, cx, cy
+        _check(_lib().tm_refine_set_image(self.h, _f(np.ascontiguousarray(img, np.float32)),
+                                          _f(np.ascontiguousarray(cx, np.float32)),
+                                          _f(np.ascontiguousarray(cy, np.float32)), int(cx.size)))
 
     def peak_list(self, scaled_mip: np.ndarray, cap: int = 100000) -> np.ndarray:
         sm = np.ascontiguousarray(scaled_mip, np.float32)

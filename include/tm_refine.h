@@ -82,6 +82,13 @@ int tm_refine_destroy(tm_refine* h);
 int tm_refine_init(tm_refine* h, const tm_refine_params* params, const float* whitened_image,
                    const float* template_spectrum, const float* curve_x, const float* curve_y, int n_curve);
 
+/* Replaces the whitened image and the whitening curve of an initialised handle (same image size
+ * and curve length), keeping the template and the work buffers: the per-micrograph part of
+ * tm_refine_init, for callers that score many micrographs of one size with one template. No
+ * counterpart in the reference (its program handles one image per run). */
+int tm_refine_set_image(tm_refine* h, const float* whitened_image, const float* curve_x, const float* curve_y,
+                        int n_curve);
+
 /* Refines every peak of the scaled MIP. maps[7]: mip, scaled_mip, psi, theta, phi, defocus,
  * pixel_size, each image_x x image_y dense (the search's output maps). out_maps[7] (the array or
  * any entry may be NULL) receive best_mip, best_scaled_mip, best_psi, best_theta, best_phi,

@@ -1,7 +1,8 @@
 """GPU: learned proposals + local exhaustive refinement (cistem_downstream_bah_amd/proposal.py).
 
 * tm_refine_score_batch equals tm_refine_score request by request (bit-identical), with more
-  windows and requests than one batch holds
+  windows and requests than one batch holds; after tm_refine_set_image a handle scores exactly as a
+  handle made for that micrograph
 * with the exhaustive search's own peaks as proposals and every orientation bin, the local
   refinement finds every peak (recall 1 within 2 pixels) and mostly its orientation: the window
   correlation and the search's full-image correlation agree on position
@@ -40,6 +41,22 @@ def test_score_batch_equals_single_scores(study):
             assert tuple(got[k]) == tuple(np.float32(v) for v in ref), (k, got[k], ref)
     finally:
         r.close()
+
+
+def test_set_image_equals_a_fresh_handle(study):
+    print("case: tm_refine_set_image vs a new handle", flush=True)
+    a, b = study.micrograph(7), study.micrograph(8)
+    settings = RefineSettings(padding=1.0)
+    reused = TemplateRefiner(a, study.volume, settings, template_spectrum=study.tpl_spectrum)
+    fresh = TemplateRefiner(b, study.volume, settings, template_spectrum=study.tpl_spectrum)
+    try:
+        reused.set_image(b)
+        xy = np.array([(-60, 20), (31, -90), (5, 5)] * 4)
+        ang = study.orients[np.arange(12) * 997 % len(study.orients)]
+        assert np.array_equal(reused.score_batch(xy, ang), fresh.score_batch(xy, ang))
+    finally:
+        reused.close()
+        fresh.close()
 
 
 def test_true_cells_with_all_bins_recall_every_peak(study):
