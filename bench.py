@@ -98,7 +98,7 @@ def main():
                     help="storage of the inter-pass plane (fp32: the reference's precision, the headline)")
     ap.add_argument("--in-flight", type=int, default=0, help="refine: peaks refined together (0 = 256)")
     ap.add_argument("--train", type=int, default=2000, help="proposal: training micrographs")
-    ap.add_argument("--epochs", type=int, default=10, help="proposal: training epochs")
+    ap.add_argument("--epochs", type=int, default=20, help="proposal: training epochs")
     ap.add_argument("--eval", type=int, default=40, help="proposal: held-out micrographs")
     ap.add_argument("--proposals", type=str, default="16,20", help="proposal: locations proposed per micrograph")
     ap.add_argument("--bins", type=str, default="10,20,40", help="proposal: orientation bins per location")
