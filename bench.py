@@ -105,7 +105,7 @@ def main():
     ap.add_argument("--weights", type=str, default="",
                     help="proposal: load the network (raw float32 parameters) instead of training")
     ap.add_argument("--save-weights", type=str, default="", help="proposal: save the trained network as raw float32 parameters")
-    ap.add_argument("--workload", choices=["config1", "config2", "refine", "proposal"], default="config1",
+    ap.add_argument("--workload", choices=["config1", "config2", "refine", "proposal", "rocfft"], default="config1",
                     help="config2: eight micrographs of mixed sizes (own JSON line, not the headline)")
     ap.add_argument("--concurrency", type=str, default="1,2,4",
                     help="config2: numbers of images searched concurrently (one stream each)")
@@ -142,6 +142,9 @@ def main():
         return
     if args.workload == "proposal":
         bench_proposal(args)
+        return
+    if args.workload == "rocfft":
+        bench_rocfft(args)
         return
 
     if args.small:
@@ -327,6 +330,60 @@ def planted_maps(s, off=(3.0, -3.0, 3.0), margin=60):
             maps["theta"][iy, ix] = q.theta + off[1]
             maps["psi"][iy, ix] = q.psi + off[2]
     return maps
+
+
+def bench_rocfft(args):
+    """Comparison only: the same correlation through the vendor FFT library (torch.fft on ROCm calls
+    hipFFT/rocFFT), at the 4096^2 / T = 256 shape, batch args.batch: the T x T projection in the
+    corner of a zero 4096^2 image, rfft2, times conj(input spectrum), irfft2; then the same plus the
+    running MIP / argmax / FP64 sum and sum-of-squares update in torch. Reported beside this build's
+    own path; not the headline."""
+    import torch
+
+    dev = torch.device("cuda:0")
+    W = H = 4096
+    T, B = 256, args.batch
+    g = torch.Generator(device="cpu").manual_seed(0)
+    x = torch.zeros(B, H, W, device=dev)
+    x[:, :T, :T] = torch.randn(B, T, T, generator=g).to(dev)
+    inp = torch.randn(H, W // 2 + 1, dtype=torch.complex64, generator=g).to(dev)
+    mip = torch.full((H, W), -float("inf"), device=dev)
+    idx = torch.zeros((H, W), dtype=torch.int32, device=dev)
+    s1 = torch.zeros((H, W), dtype=torch.float64, device=dev)
+    s2 = torch.zeros((H, W), dtype=torch.float64, device=dev)
+    conj_inp = inp.conj()
+
+    def fft_only():
+        X = torch.fft.rfft2(x)
+        X.mul_(conj_inp)
+        return torch.fft.irfft2(X, s=(H, W))
+
+    def full(g0):
+        y = fft_only()
+        m, a = y.max(dim=0)
+        better = m > mip
+        mip.copy_(torch.where(better, m, mip))
+        idx.copy_(torch.where(better, a.to(torch.int32) + g0, idx))
+        s1.add_(y.sum(dim=0, dtype=torch.float64))
+        s2.add_((y * y).sum(dim=0, dtype=torch.float64))
+
+    out = {}
+    for name, fn in (("fft_only", lambda k: fft_only()), ("with_epilogue", full)):
+        for w in range(args.warmup):
+            fn(w * B)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            fn(k * B)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        out[name] = {"ccf_per_s": round(B * args.steps / dt, 1), "ms_per_batch": round(1e3 * dt / args.steps, 3)}
+        log(f"rocFFT path {name}: {out[name]}")
+    print(json.dumps({"metric": "comparison: correlations/sec @4096^2 through torch.fft (hipFFT/rocFFT)",
+                      "value": out["with_epilogue"]["ccf_per_s"], "unit": "CCF/s", "n_gpus": 1,
+                      "steps": args.steps, "warmup": args.warmup, "higher_is_better": True, "dtype": "fp32",
+                      "data": "synthetic", "config": {"workload": "rocfft_compare_4096sq_T256", "batch": B},
+                      "rocfft": out}), flush=True)
 
 
 def bench_proposal(args):
