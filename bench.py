@@ -88,6 +88,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--positions", type=int, default=8, help="search positions per step (x 241 psi)")
     ap.add_argument("--batch", type=int, default=64, help="orientations per launch")
+    ap.add_argument("--graph", action="store_true",
+                    help="also time the same steps with each plane replayed from a captured hipGraph")
     ap.add_argument("--autotune", type=str, default="",
                     help="comma-separated batch candidates: tm_autotune_batch picks the batch before timing")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -199,6 +201,23 @@ def main():
     value = ccfs / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
     log(f"{ccfs} CCFs in {elapsed:.3f}s -> {value:.1f} CCF/s")
+    graph = None
+    if args.graph:
+        # the same steps with each plane replayed from one captured hipGraph (capture in the warm-up)
+        g.set_graph(True)
+        g.reset()
+        g.run_inner_loop(filters[planes[0]], 0, planes[0])
+        g.reset()
+        gccfs = 0
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            d = planes[k % len(planes)]
+            gccfs += g.run_inner_loop(filters[d], 0, d)
+        gel = time.perf_counter() - t0
+        g.set_graph(False)
+        graph = {"ccf_per_s": round(gccfs / gel, 2), "ms_per_step": round(1e3 * gel / args.steps, 3),
+                 "launch_by_launch_ccf_per_s": round(value, 2)}
+        log(f"graph mode: {gccfs} CCFs in {gel:.3f}s -> {gccfs / gel:.1f} CCF/s")
 
     W, H, T = search.W, search.H, search.T
     P = search.template_spectrum.shape[0]
@@ -259,6 +278,7 @@ def main():
         "cpu_baseline": cpu,
         "kernels": per_kernel,
         "autotune": tuned,
+        "graph": graph,
         "pipeline": {"algorithmic_bytes_per_ccf": pipe,
                      "fraction_of_hbm_peak": round(pipe * value / HBM_PEAK, 4)},
     }

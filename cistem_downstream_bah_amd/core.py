@@ -26,7 +26,7 @@ I64P = ctypes.POINTER(ctypes.c_longlong)
 # Every symbol include/tm_hip.h declares.
 EXPORTS = ["tm_device_count", "tm_create", "tm_destroy", "tm_init", "tm_run_inner_loop", "tm_get_results", "tm_merge",
            "tm_set_template", "tm_reset", "tm_finalize", "tm_debug_projection", "tm_debug_ccf", "tm_set_timing", "tm_get_timing",
-           "tm_last_error", "tm_autotune_batch"]
+           "tm_last_error", "tm_autotune_batch", "tm_set_graph"]
 
 PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.c_longlong, ctypes.c_void_p)
 
@@ -72,6 +72,7 @@ def lib():
         L.tm_debug_projection.argtypes = [vp, F32P, ctypes.c_int, ctypes.c_int, F32P]
         L.tm_debug_ccf.argtypes = [vp, F32P, ctypes.c_int, ctypes.c_int, F32P]
         L.tm_set_timing.argtypes = [vp, ctypes.c_int]
+        L.tm_set_graph.argtypes = [vp, ctypes.c_int]
         L.tm_get_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_char_p), F64P, I64P, ctypes.c_int]
         L.tm_autotune_batch.argtypes = [ctypes.c_int, ctypes.POINTER(InitParams), F32P, F32P, F32P, F32P, F32P, I32P,
                                         ctypes.c_int, ctypes.c_int, I32P, F64P]
@@ -233,6 +234,10 @@ class TemplateMatchingCore:
 
     def set_timing(self, on: bool) -> None:
         _check(lib().tm_set_timing(self.h, int(on)))
+
+    def set_graph(self, on: bool) -> None:
+        """Replays one captured hipGraph per plane instead of launching kernel by kernel (tm_set_graph)."""
+        _check(lib().tm_set_graph(self.h, int(on)))
 
     def timing(self) -> dict:
         cap = 16

@@ -74,8 +74,24 @@ struct tm_core {
   std::vector<std::pair<int, size_t>> ev_marks;  // (kernel id, index of the start event)
   double kernel_ms[K_COUNT] = {};
   long long kernel_launches[K_COUNT] = {};
+  // graph mode: the launches of one plane captured once into a hipGraph and replayed per plane;
+  // the plane's index offset lives in device memory (plane_base), written before each replay
+  bool use_graph = false, capturing = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  DevBuf<int> plane_base;
+  int* h_plane_base = nullptr;  // pinned host staging of the offset
+
+  void drop_graph() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+  }
 
   ~tm_core() {
+    drop_graph();
+    if (h_plane_base) (void)hipHostFree(h_plane_base);
     for (auto& kv : tw) (void)hipFree(kv.second);
     for (auto& e : ev_pool) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
@@ -292,6 +308,7 @@ void run_batch(tm_core* h, int o0, int nb, int g0) {
     ep.hstep = h->prm.histogram_step_scaled;
     ep.nbins = h->prm.histogram_number_of_bins;
     ep.g0 = g0;
+    ep.plane_g0 = h->capturing ? h->plane_base.p : nullptr;
     launch_last(h, nb, ep);
   }
 }
@@ -387,6 +404,7 @@ int tm_init(tm_core* h, const tm_init_params* p, const float* template_spectrum,
     TM_REQUIRE(p->histogram_number_of_bins > 0 && p->histogram_number_of_bins <= 512, TM_ERR_INVALID,
                "histogram_number_of_bins must be in 1..512");
     TM_REQUIRE(p->histogram_step_scaled > 0.0, TM_ERR_INVALID, "histogram_step_scaled must be > 0");
+    h->drop_graph();
     h->prm = *p;
     h->W = W; h->H = H; h->Wh = W / 2 + 1;
     h->T = T; h->Th = T / 2 + 1;
@@ -521,13 +539,43 @@ int tm_run_inner_loop(tm_core* h, const float* projection_filter, int size_i, in
     h->ev_marks.clear();
     h->ev_used = 0;
     long long done = 0;
-    for (int o0 = 0; o0 < h->n_orient; o0 += h->B) {
-      const int nb = std::min(h->B, h->n_orient - o0);
-      run_batch(h, o0, nb, (int)(plane * per_plane + first + o0));
-      done += nb;
+    if (h->use_graph && !h->timing) {
+      // graph mode: the plane's offset goes to device memory; the batches (indices relative to the
+      // plane) are captured once and replayed for every plane
+      *h->h_plane_base = (int)(plane * per_plane);
+      TM_HIP(hipMemcpyAsync(h->plane_base.p, h->h_plane_base, sizeof(int), hipMemcpyHostToDevice, h->stream));
+      if (!h->graph_exec) {
+        TM_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        h->capturing = true;
+        try {
+          for (int o0 = 0; o0 < h->n_orient; o0 += h->B)
+            run_batch(h, o0, std::min(h->B, h->n_orient - o0), (int)(first + o0));
+        } catch (...) {
+          hipGraph_t g = nullptr;
+          (void)hipStreamEndCapture(h->stream, &g);
+          if (g) (void)hipGraphDestroy(g);
+          h->capturing = false;
+          throw;
+        }
+        h->capturing = false;
+        TM_HIP(hipStreamEndCapture(h->stream, &h->graph));
+        TM_HIP(hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
+      }
+      TM_HIP(hipGraphLaunch(h->graph_exec, h->stream));
+      done = h->n_orient;
       if (progress) {
         TM_HIP(hipStreamSynchronize(h->stream));
         progress(done, user);
+      }
+    } else {
+      for (int o0 = 0; o0 < h->n_orient; o0 += h->B) {
+        const int nb = std::min(h->B, h->n_orient - o0);
+        run_batch(h, o0, nb, (int)(plane * per_plane + first + o0));
+        done += nb;
+        if (progress) {
+          TM_HIP(hipStreamSynchronize(h->stream));
+          progress(done, user);
+        }
       }
     }
     TM_HIP(hipStreamSynchronize(h->stream));
@@ -714,6 +762,20 @@ int tm_debug_ccf(tm_core* h, const float* projection_filter, int search_position
       std::memcpy(out + (size_t)y * (W + 2), host.data() + (size_t)y * W, W * sizeof(float));
       out[(size_t)y * (W + 2) + W] = 0.f;
       out[(size_t)y * (W + 2) + W + 1] = 0.f;
+    }
+  });
+}
+
+int tm_set_graph(tm_core* h, int enabled) {
+  return guarded([&] {
+    TM_REQUIRE(h, TM_ERR_INVALID, "handle is NULL");
+    TM_HIP(hipSetDevice(h->device));
+    TM_HIP(hipStreamSynchronize(h->stream));
+    h->drop_graph();
+    h->use_graph = enabled != 0;
+    if (h->use_graph && !h->plane_base.p) {
+      h->plane_base.alloc(1);
+      TM_HIP(hipHostMalloc((void**)&h->h_plane_base, sizeof(int), hipHostMallocDefault));
     }
   });
 }

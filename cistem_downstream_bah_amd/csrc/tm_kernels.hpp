@@ -827,6 +827,7 @@ struct EpiParams {
   double hmin, hstep_inv, hstep;
   int nbins;
   int g0;      // global orientation index of batch entry 0
+  const int* plane_g0;  // when set, *plane_g0 is added to g0 (a replayed graph reads its plane here)
   int ztile;   // column-tile width of Z (power of two)
   int ntiles;  // column tiles per orientation
   float zinv;  // inverse of the scale of 16-bit Z storage (1 for FP32)
@@ -864,6 +865,7 @@ void k_last(const cf* __restrict__ Z, int H, int nb, EpiParams ep, const cf* __r
   const int tid0 = threadIdx.x;
   const int bid = blockIdx.x;
   const int y = xcd_row(bid, H);
+  const int gbase = ep.g0 + (ep.plane_g0 ? *ep.plane_g0 : 0);
   for (int i = tid0; i < 512 * HC; i += BT) hist[i] = 0u;
   for (int i = tid0; i < h; i += BT) twh[i] = tw_h[i];
   for (int i = tid0; i <= h / 2; i += BT) twn[i] = tw_n[i];
@@ -939,7 +941,7 @@ void k_last(const cf* __restrict__ Z, int H, int nb, EpiParams ep, const cf* __r
     fft_inplace<h, +1, NT, 3, 1>(bb, twh, t);
 #pragma unroll 1
     for (int q = 0; q < nbp; ++q) {
-      const int g = ep.g0 + b0 + q;
+      const int g = gbase + b0 + q;
 #pragma unroll
       for (int u = 0; u < PAIRS; ++u) {
         const cf z = buf[q][lpad(tid + u * BT)];
@@ -1009,6 +1011,7 @@ void k_last_dma(const cf* __restrict__ Z, int H, int nb, EpiParams ep, const cf*
   const int tid0 = threadIdx.x;
   const int bid = blockIdx.x;
   const int y = xcd_row(bid, H);
+  const int gbase = ep.g0 + (ep.plane_g0 ? *ep.plane_g0 : 0);
   const int sh = __ffs(ep.ztile) - 1;
   const size_t z_bs = (size_t)ep.ntiles * H * ep.ztile;
   const size_t tile_stride = (size_t)H * ep.ztile;
@@ -1113,7 +1116,7 @@ void k_last_dma(const cf* __restrict__ Z, int H, int nb, EpiParams ep, const cf*
     fft_inplace<h, +1, NT, 3, 1>(buf[grp], ftw, t);
 #pragma unroll 1
     for (int q = 0; q < nbp; ++q) {
-      const int g = ep.g0 + b0 + q;
+      const int g = gbase + b0 + q;
 #pragma unroll
       for (int u = 0; u < PAIRS; ++u) {
         const cf z = buf[q][lpad(tid + u * BT)];

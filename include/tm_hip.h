@@ -134,6 +134,13 @@ int tm_debug_ccf(tm_core* h, const float* projection_filter, int search_position
 int tm_set_timing(tm_core* h, int enabled);
 int tm_get_timing(tm_core* h, const char** names, double* ms, long long* launches, int cap);
 
+/* Graph mode, with no counterpart in the reference (its loop launches every kernel): when enabled,
+ * the first tm_run_inner_loop after it captures the launches of one plane (all batches) into a
+ * hipGraph and every later plane replays it; the plane's orientation-index offset is read from
+ * device memory, written before each replay. Results equal the launch-by-launch path bit for bit.
+ * Not used while timing is enabled. tm_init, and enabling or disabling, drop the captured graph. */
+int tm_set_graph(tm_core* h, int enabled);
+
 /* Last error message of the calling thread. */
 const char* tm_last_error(void);
 

@@ -1,0 +1,58 @@
+"""GPU: graph mode of the search (tm_set_graph; no counterpart in the reference). One plane's
+launches are captured into a hipGraph and replayed for every later plane, with the plane's index
+offset read from device memory.
+
+* five planes (four defocus values, one repeated) through a graph-mode handle equal, plane by plane,
+  a launch-by-launch handle bit for bit: MIP, best index, defocus map, FP64 sum and sum of squares,
+  histogram; eager work on the graph handle's stream (a debug correlation) and a synchronisation sit
+  between the replays
+* a second tm_init drops the graph; the handle captures again and stays equal
+"""
+import numpy as np
+import pytest
+
+from cistem_downstream_bah_amd import synth
+from cistem_downstream_bah_amd import match_template as MT
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c0():
+    return synth.config0(0)
+
+
+def _search(s, graph):
+    se = MT.MatchTemplateSearch(s.micrograph, s.volume,
+                                MT.SearchParams(angular_step=10.0, in_plane_angular_step=10.0, padding=1.0,
+                                                defocus_search_range=200.0, defocus_step=100.0,
+                                                first_search_position=0, last_search_position=59, batch_size=64))
+    se.core.set_graph(graph)
+    return se
+
+
+def _same(a, b):
+    ra, rb = a.core.results(), b.core.results()
+    for f in ("mip", "best_index", "defocus", "sum", "sumsq", "hist"):
+        assert np.array_equal(getattr(ra, f), getattr(rb, f)), f
+    assert ra.nccf == rb.nccf
+
+
+def test_graph_replays_equal_eager(c0):
+    eager, graph = _search(c0, False), _search(c0, True)
+    for k, d in enumerate((-1, 0, 2, 0, 1)):
+        print(f"case: config0 positions 0-59, plane {k} defocus {d}, graph vs eager", flush=True)
+        f = eager.filter_for(d)
+        assert eager.core.run_inner_loop(f, 0, d) == graph.core.run_inner_loop(f, 0, d)
+        _same(eager, graph)
+        graph.core.debug_ccf(f, 3, 2)   # eager launches on the graph handle's stream between replays
+    print("case: re-init drops the graph", flush=True)
+    for se in (eager, graph):
+        se.core.init(se.template_spectrum, se.input_spectrum, se.T, se.grid, se.psi, se.first, se.last,
+                     se.histogram_min_scaled, se.histogram_step_scaled, 512, se.max_padding, se.defocus_step,
+                     se.pixel_size_step, 64, "fp32")
+    for d in (0, -2):
+        f = eager.filter_for(d)
+        eager.core.run_inner_loop(f, 0, d)
+        graph.core.run_inner_loop(f, 0, d)
+        _same(eager, graph)
