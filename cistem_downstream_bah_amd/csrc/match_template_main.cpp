@@ -116,7 +116,7 @@ void usage() {
                "  [--cs 2.7] [--amplitude-contrast 0.07] [--defocus1 A] [--defocus2 A] [--defocus-angle 0]\n"
                "  [--phase-shift 0] [--angular-step 10] [--in-plane-step 10] [--defocus-range 0] [--defocus-step 0]\n"
                "  [--pixel-size-range 0] [--pixel-size-step 0] [--padding 1] [--symmetry C1] [--min-peak-radius 10]\n"
-               "  [--device 0] [--batch 0]\n");
+               "  [--device 0] [--batch 0] [--graph 0|1]\n");
 }
 
 }  // namespace
@@ -137,7 +137,7 @@ int main(int argc, char** argv) {
     float def_range = f("defocus-range", 0.0), def_step = f("defocus-step", 0.0);
     float pix_range = f("pixel-size-range", 0.0), pix_step = f("pixel-size-step", 0.0);
     const float padding = f("padding", 1.0), min_peak_radius = f("min-peak-radius", 10.0);
-    const int device = (int)f("device", 0), batch = (int)f("batch", 0);
+    const int device = (int)f("device", 0), batch = (int)f("batch", 0), graph = (int)f("graph", 0);
     const std::string sym = a.count("symmetry") ? a["symmetry"] : "C1";
     const std::string out = a["out"];
 
@@ -170,6 +170,7 @@ int main(int argc, char** argv) {
     cistem_amd::TemplateMatchingCore core(device);
     core.Init(tpl.data(), P, spec.data(), W, H, T, pix_step, def_step, 360.0f, 0.0f, psi_step, grid.data(), npos,
               -12.5 / sqrt_px, hstep / sqrt_px, 512, maxpad, 0, npos - 1, batch);
+    if (graph) core.SetGraph(true);  // each plane replayed from one captured hipGraph
     std::vector<float> pf((size_t)T * (T / 2 + 1) * 2);
     // pixel-size planes, then defocus planes (match_template.cpp:837-914)
     for (int si = -nsh; si <= nsh; ++si) {

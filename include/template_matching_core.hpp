@@ -92,6 +92,8 @@ class TemplateMatchingCore {
   // accumulating, as after the reference's GPU[tIDX].template_reconstruction.CopyFrom
   // (match_template.cpp:883-886).
   void SetTemplate(const float* template_spectrum) { check(tm_set_template(h_, template_spectrum)); }
+  // Replays each plane from one captured hipGraph (no counterpart in the reference).
+  void SetGraph(bool enabled) { check(tm_set_graph(h_, enabled ? 1 : 0)); }
 
   // Folds another core's results into this one (same search, another position range, same device),
   // as AggregatedTemplateResult::AddResult merges job results (match_template.cpp:2015-2103).
