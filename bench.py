@@ -185,39 +185,47 @@ def main():
     for w in range(args.warmup):
         g.run_inner_loop(filters[planes[w % len(planes)]], 0, planes[w % len(planes)])
     g.reset()
-    g.set_timing(True)
-    kms = {}
-    klaunch = {}
+    # the timed steps run without per-kernel events (they cost launch gaps on small images)
     ccfs = 0
     t0 = time.perf_counter()
     for k in range(args.steps):
         d = planes[k % len(planes)]
         ccfs += g.run_inner_loop(filters[d], 0, d)
-        for name, (ms, nl) in g.timing().items():
-            kms[name] = kms.get(name, 0.0) + ms
-            klaunch[name] = klaunch.get(name, 0) + nl
     elapsed = time.perf_counter() - t0  # run_inner_loop synchronises its stream before returning
-    g.set_timing(False)
     value = ccfs / elapsed
     ms_per_step = 1e3 * elapsed / args.steps
     log(f"{ccfs} CCFs in {elapsed:.3f}s -> {value:.1f} CCF/s")
+    # the same steps again with HIP events around each kernel group: per-kernel averages
+    g.reset()
+    g.set_timing(True)
+    kms = {}
+    klaunch = {}
+    for k in range(args.steps):
+        d = planes[k % len(planes)]
+        g.run_inner_loop(filters[d], 0, d)
+        for name, (ms, nl) in g.timing().items():
+            kms[name] = kms.get(name, 0.0) + ms
+            klaunch[name] = klaunch.get(name, 0) + nl
+    g.set_timing(False)
     graph = None
     if args.graph:
-        # the same steps with each plane replayed from one captured hipGraph (capture in the warm-up)
-        g.set_graph(True)
-        g.reset()
-        g.run_inner_loop(filters[planes[0]], 0, planes[0])
-        g.reset()
-        gccfs = 0
-        t0 = time.perf_counter()
-        for k in range(args.steps):
-            d = planes[k % len(planes)]
-            gccfs += g.run_inner_loop(filters[d], 0, d)
-        gel = time.perf_counter() - t0
+        # the same steps launch by launch and with each plane replayed from one captured hipGraph,
+        # both without the per-kernel events of the timed run above (capture in the warm-up)
+        res = {}
+        for mode in (False, True):
+            g.set_graph(mode)
+            g.reset()
+            g.run_inner_loop(filters[planes[0]], 0, planes[0])
+            g.reset()
+            n_m = 0
+            t0 = time.perf_counter()
+            for k in range(args.steps):
+                d = planes[k % len(planes)]
+                n_m += g.run_inner_loop(filters[d], 0, d)
+            res[mode] = n_m / (time.perf_counter() - t0)
         g.set_graph(False)
-        graph = {"ccf_per_s": round(gccfs / gel, 2), "ms_per_step": round(1e3 * gel / args.steps, 3),
-                 "launch_by_launch_ccf_per_s": round(value, 2)}
-        log(f"graph mode: {gccfs} CCFs in {gel:.3f}s -> {gccfs / gel:.1f} CCF/s")
+        graph = {"graph_ccf_per_s": round(res[True], 2), "launch_by_launch_ccf_per_s": round(res[False], 2)}
+        log(f"without kernel events: launch by launch {res[False]:.1f} CCF/s, graph {res[True]:.1f} CCF/s")
 
     W, H, T = search.W, search.H, search.T
     P = search.template_spectrum.shape[0]
@@ -537,22 +545,25 @@ def bench_refine(args):
     for _ in range(max(1, min(args.warmup, 2))):
         recs, _ = g.run(maps, want_maps=False)
     steps = max(1, args.steps)
-    g.set_timing(True)
-    stage = {}
     calls = peaks = 0
     t0 = time.perf_counter()
     for _ in range(steps):
         recs, _ = g.run(maps, want_maps=False)   # returns after its final stream synchronisation
         calls += sum(r["evaluations"] for r in recs)
         peaks += len(recs)
+    elapsed = time.perf_counter() - t0
+    value = calls / elapsed
+    # stage averages from a second pass with HIP events around each stage (not in the timed steps)
+    g.set_timing(True)
+    stage = {}
+    for _ in range(steps):
+        g.run(maps, want_maps=False)
         for k, (ms, nl, it) in g.timing().items():
             a = stage.setdefault(k, [0.0, 0, 0])
             a[0] += ms
             a[1] += nl
             a[2] += it
-    elapsed = time.perf_counter() - t0
     g.set_timing(False)
-    value = calls / elapsed
     log(f"{peaks} peaks, {calls} TemplateScore calls in {elapsed:.3f}s -> {value:.0f} calls/s")
     t, P = s.volume.shape[0], g.P
     Ph = P // 2 + 1

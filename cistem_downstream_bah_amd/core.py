@@ -26,7 +26,7 @@ I64P = ctypes.POINTER(ctypes.c_longlong)
 # Every symbol include/tm_hip.h declares.
 EXPORTS = ["tm_device_count", "tm_create", "tm_destroy", "tm_init", "tm_run_inner_loop", "tm_get_results", "tm_merge",
            "tm_set_template", "tm_reset", "tm_finalize", "tm_debug_projection", "tm_debug_ccf", "tm_set_timing", "tm_get_timing",
-           "tm_last_error", "tm_autotune_batch", "tm_set_graph"]
+           "tm_last_error", "tm_autotune_batch", "tm_set_graph", "tm_set_input"]
 
 PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.c_longlong, ctypes.c_void_p)
 
@@ -73,6 +73,7 @@ def lib():
         L.tm_debug_ccf.argtypes = [vp, F32P, ctypes.c_int, ctypes.c_int, F32P]
         L.tm_set_timing.argtypes = [vp, ctypes.c_int]
         L.tm_set_graph.argtypes = [vp, ctypes.c_int]
+        L.tm_set_input.argtypes = [vp, F32P]
         L.tm_get_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_char_p), F64P, I64P, ctypes.c_int]
         L.tm_autotune_batch.argtypes = [ctypes.c_int, ctypes.POINTER(InitParams), F32P, F32P, F32P, F32P, F32P, I32P,
                                         ctypes.c_int, ctypes.c_int, I32P, F64P]
@@ -210,6 +211,13 @@ class TemplateMatchingCore:
         if tpl.shape != (self.P, self.P, self.P // 2 + 1):
             raise ValueError(f"template shape {tpl.shape} != {(self.P, self.P, self.P // 2 + 1)}")
         _check(lib().tm_set_template(self.h, _f32(tpl.view(np.float32))))
+
+    def set_input(self, input_spectrum: np.ndarray) -> None:
+        """Replaces the input spectrum (same image size) and resets the maps (tm_set_input)."""
+        inp = np.ascontiguousarray(input_spectrum, np.complex64)
+        if inp.shape != (self.H, self.W // 2 + 1):
+            raise ValueError(f"input spectrum shape {inp.shape} != {(self.H, self.W // 2 + 1)}")
+        _check(lib().tm_set_input(self.h, _f32(inp.view(np.float32))))
 
     def reset(self) -> None:
         _check(lib().tm_reset(self.h))

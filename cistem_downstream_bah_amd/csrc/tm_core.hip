@@ -319,6 +319,41 @@ void upload_filter(tm_core* h, const float* projection_filter) {
   TM_HIP(hipMemcpyAsync(h->filt.p, projection_filter, n * sizeof(cf), hipMemcpyHostToDevice, h->stream));
 }
 
+// The input spectrum, transposed to [kx][ky] so that each column is contiguous. At H = 4096 the
+// column pass places the projection rows at y = 0..T-1 instead of the reference's centred
+// y0 = H/2 - T/2 (ClipIntoLargerRealSpace2D); the shift multiplies the reference spectrum by
+// e^{-2 pi i y0 ky / H}, so conj(R) * I keeps its value with I' = I * e^{+2 pi i y0 ky / H}.
+void upload_input(tm_core* h, const float* input_spectrum) {
+  const int H = h->H, T = h->T;
+  std::vector<cf> it((size_t)h->Wh * H);
+  const cf* in = reinterpret_cast<const cf*>(input_spectrum);
+  const bool shifted = column_rows_at_zero(H);
+  std::vector<long double> pc(H, 1.0L), ps(H, 0.0L);
+  if (shifted) {
+    const long double two_pi = 2.0L * acosl(-1.0L);
+    const long long y0 = H / 2 - T / 2;
+    for (int y = 0; y < H; ++y) {
+      const long double a = two_pi * (long double)((y0 * y) % H) / (long double)H;
+      pc[y] = cosl(a);
+      ps[y] = sinl(a);
+    }
+  }
+  for (int y = 0; y < H; ++y)
+    for (int x = 0; x < h->Wh; ++x) {
+      const cf v = in[(size_t)y * h->Wh + x];
+      cf& o = it[(size_t)x * H + y];
+      if (shifted) {
+        o.x = (float)(v.x * pc[y] - v.y * ps[y]);
+        o.y = (float)(v.x * ps[y] + v.y * pc[y]);
+      } else {
+        o = v;
+      }
+    }
+  if (h->inpT.n != it.size()) h->inpT.alloc(it.size());
+  TM_HIP(hipMemcpyAsync(h->inpT.p, it.data(), it.size() * sizeof(cf), hipMemcpyHostToDevice, h->stream));
+  TM_HIP(hipStreamSynchronize(h->stream));
+}
+
 void reset_state(tm_core* h) {
   const size_t n = (size_t)h->W * h->H;
   hipLaunchKernelGGL(tmk::k_reset_state, dim3((n + 255) / 256), dim3(256), 0, h->stream, h->mip.p, h->idx.p, h->sum.p,
@@ -443,36 +478,7 @@ int tm_init(tm_core* h, const tm_init_params* p, const float* template_spectrum,
     h->tpl.alloc(ntpl);
     TM_HIP(hipMemcpy(h->tpl.p, template_spectrum, ntpl * sizeof(cf), hipMemcpyHostToDevice));
 
-    // input spectrum, transposed to [kx][ky] so that each column is contiguous. At H = 4096 the
-    // column pass places the projection rows at y = 0..T-1 instead of the reference's centred
-    // y0 = H/2 - T/2 (ClipIntoLargerRealSpace2D); the shift multiplies the reference spectrum by
-    // e^{-2 pi i y0 ky / H}, so conj(R) * I keeps its value with I' = I * e^{+2 pi i y0 ky / H}.
-    std::vector<cf> it((size_t)h->Wh * H);
-    const cf* in = reinterpret_cast<const cf*>(input_spectrum);
-    const bool shifted = column_rows_at_zero(H);
-    std::vector<long double> pc(H, 1.0L), ps(H, 0.0L);
-    if (shifted) {
-      const long double two_pi = 2.0L * acosl(-1.0L);
-      const long long y0 = H / 2 - T / 2;
-      for (int y = 0; y < H; ++y) {
-        const long double a = two_pi * (long double)((y0 * y) % H) / (long double)H;
-        pc[y] = cosl(a);
-        ps[y] = sinl(a);
-      }
-    }
-    for (int y = 0; y < H; ++y)
-      for (int x = 0; x < h->Wh; ++x) {
-        const cf v = in[(size_t)y * h->Wh + x];
-        cf& o = it[(size_t)x * H + y];
-        if (shifted) {
-          o.x = (float)(v.x * pc[y] - v.y * ps[y]);
-          o.y = (float)(v.x * ps[y] + v.y * pc[y]);
-        } else {
-          o = v;
-        }
-      }
-    h->inpT.alloc(it.size());
-    TM_HIP(hipMemcpy(h->inpT.p, it.data(), it.size() * sizeof(cf), hipMemcpyHostToDevice));
+    upload_input(h, input_spectrum);
 
     // phase tables and filter
     auto pp = swap_phase(P);
@@ -510,6 +516,17 @@ int tm_set_template(tm_core* h, const float* template_spectrum) {
     require_ready(h);
     TM_REQUIRE(template_spectrum, TM_ERR_INVALID, "template_spectrum is NULL");
     TM_HIP(hipMemcpyAsync(h->tpl.p, template_spectrum, h->tpl.n * sizeof(cf), hipMemcpyHostToDevice, h->stream));
+    TM_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+int tm_set_input(tm_core* h, const float* input_spectrum) {
+  return guarded([&] {
+    require_ready(h);
+    TM_REQUIRE(input_spectrum, TM_ERR_INVALID, "input_spectrum is NULL");
+    TM_HIP(hipStreamSynchronize(h->stream));
+    upload_input(h, input_spectrum);
+    reset_state(h);
     TM_HIP(hipStreamSynchronize(h->stream));
   });
 }

@@ -184,6 +184,16 @@ class MatchTemplateSearch:
                        self.histogram_min_scaled, self.histogram_step_scaled, 512, self.max_padding,
                        self.defocus_step, self.pixel_size_step, p.batch_size, p.plane_storage)
 
+    def set_micrograph(self, micrograph: np.ndarray) -> None:
+        """Searches another micrograph of the same size with this handle (tm_set_input): the template,
+        grid, work buffers and a captured graph are kept; the maps restart from zero."""
+        spec, cx, cy, W, H, rotated, max_padding = prepare_input_image(micrograph, self.T, self.device)
+        if (W, H, rotated) != (self.W, self.H, self.rotated):
+            raise ValueError("the micrograph's search size differs from this search's")
+        self.input_spectrum, self.curve_x, self.curve_y, self.max_padding = spec, cx, cy, max_padding
+        self.original_shape = micrograph.shape
+        self.core.set_input(spec)
+
     def init_params(self, batch_size: int | None = None) -> core.InitParams:
         """The tm_init_params of this search (batch_size: the search's own unless given)."""
         p = self.p

@@ -103,6 +103,7 @@ class ProposalStudy:
         self.orient_bin = orientation_bin(self.orients[:, 0], self.orients[:, 1], self.orients[:, 2], self.dirs)
         self.members = [np.nonzero(self.orient_bin == b)[0] for b in range(N_BINS)]
         self._refiner = None  # one refinement handle, its image replaced per micrograph
+        self._exhaustive_search = None  # one search handle, its input replaced per micrograph
         cgrid = MT.search_grid("C1", 2 * cfg.angular_step)
         cpsi = MT.psi_list(0.0, 360.0, 2 * cfg.in_plane_angular_step)
         self.coarse = np.array([(g[0], g[1], s) for g in cgrid for s in cpsi], np.float32)
@@ -133,12 +134,17 @@ class ProposalStudy:
 
     def exhaustive(self, micrograph: np.ndarray):
         """The full search: its peaks above the threshold (with their best angles) and the time
-        taken (preparation of the image, every orientation, z-score and peak search)."""
+        taken (preparation of the image, every orientation, z-score and peak search). One handle
+        serves every micrograph (tm_set_input), replaying its plane from a captured hipGraph."""
         t0 = time.perf_counter()
-        search = self._search(micrograph)
+        search = self._exhaustive_search
+        if search is None:
+            search = self._exhaustive_search = self._search(micrograph)
+            search.core.set_graph(True)
+        else:
+            search.set_micrograph(micrograph)
         pk = self._peaks(search, search.run())
         dt = time.perf_counter() - t0
-        search.core.close()
         return pk, dt
 
     def labelled(self, seeds, workers: int = 4, log=None) -> list:

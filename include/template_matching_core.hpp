@@ -92,6 +92,8 @@ class TemplateMatchingCore {
   // accumulating, as after the reference's GPU[tIDX].template_reconstruction.CopyFrom
   // (match_template.cpp:883-886).
   void SetTemplate(const float* template_spectrum) { check(tm_set_template(h_, template_spectrum)); }
+  // Replaces the input spectrum and resets the maps (no counterpart in the reference).
+  void SetInput(const float* input_spectrum) { check(tm_set_input(h_, input_spectrum)); }
   // Replays each plane from one captured hipGraph (no counterpart in the reference).
   void SetGraph(bool enabled) { check(tm_set_graph(h_, enabled ? 1 : 0)); }
 

@@ -134,6 +134,12 @@ int tm_debug_ccf(tm_core* h, const float* projection_filter, int search_position
 int tm_set_timing(tm_core* h, int enabled);
 int tm_get_timing(tm_core* h, const char** names, double* ms, long long* launches, int cap);
 
+/* Replaces the input spectrum (same image size; layout as in tm_init) and resets the maps, keeping
+ * the template, the grid, the work buffers and a captured graph: one handle searches many
+ * micrographs of one size with one template. No counterpart in the reference (one image per
+ * TemplateMatchingCore). */
+int tm_set_input(tm_core* h, const float* input_spectrum);
+
 /* Graph mode, with no counterpart in the reference (its loop launches every kernel): when enabled,
  * the first tm_run_inner_loop after it captures the launches of one plane (all batches) into a
  * hipGraph and every later plane replays it; the plane's orientation-index offset is read from

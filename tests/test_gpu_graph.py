@@ -7,6 +7,8 @@ offset read from device memory.
   histogram; eager work on the graph handle's stream (a debug correlation) and a synchronisation sit
   between the replays
 * a second tm_init drops the graph; the handle captures again and stays equal
+* tm_set_input: after searching one micrograph, a graph-mode handle searches another of the same
+  size exactly as a handle made for that micrograph
 """
 import numpy as np
 import pytest
@@ -56,3 +58,25 @@ def test_graph_replays_equal_eager(c0):
         eager.core.run_inner_loop(f, 0, d)
         graph.core.run_inner_loop(f, 0, d)
         _same(eager, graph)
+
+
+def test_set_input_equals_a_fresh_search(c0):
+    """tm_set_input: a handle (in graph mode) that searched one micrograph searches another of the
+    same size exactly as a handle made for it."""
+    print("case: set_micrograph + graph vs a fresh handle", flush=True)
+    other = synth.config0(7).micrograph
+    reused, fresh = _search(c0, True), None
+    try:
+        f0 = reused.filter_for(0)
+        reused.core.run_inner_loop(f0, 0, 0)
+        reused.set_micrograph(other)
+        fresh = MT.MatchTemplateSearch(other, c0.volume, reused.p)
+        for d in (0, 1):
+            reused.core.run_inner_loop(reused.filter_for(d), 0, d)
+            fresh.core.run_inner_loop(fresh.filter_for(d), 0, d)
+            assert np.array_equal(reused.filter_for(d), fresh.filter_for(d))
+        _same(reused, fresh)
+    finally:
+        reused.core.close()
+        if fresh is not None:
+            fresh.core.close()
