@@ -46,6 +46,50 @@ def orientation_bin(phi, theta, psi, dirs: np.ndarray | None = None) -> np.ndarr
     return d * N_PSI_BINS + np.minimum(q, N_PSI_BINS - 1)
 
 
+class OrientationTable:
+    """The search grid's orientations (phi, theta, psi), their bins, the grid orientations of each bin,
+    and the coarse grid (twice the angular steps) with its bins."""
+
+    def __init__(self, angular_step: float, in_plane_angular_step: float, symmetry: str = "C1"):
+        grid = MT.search_grid(symmetry, angular_step)
+        psi = MT.psi_list(0.0, 360.0, in_plane_angular_step)
+        self.orients = np.array([(g[0], g[1], s) for g in grid for s in psi], np.float32)
+        self.dirs = fibonacci_directions()
+        self.bin = orientation_bin(self.orients[:, 0], self.orients[:, 1], self.orients[:, 2], self.dirs)
+        self.members = [np.nonzero(self.bin == b)[0] for b in range(N_BINS)]
+        cgrid = MT.search_grid(symmetry, 2 * angular_step)
+        cpsi = MT.psi_list(0.0, 360.0, 2 * in_plane_angular_step)
+        self.coarse = np.array([(g[0], g[1], s) for g in cgrid for s in cpsi], np.float32)
+        self.coarse_bin = orientation_bin(self.coarse[:, 0], self.coarse[:, 1], self.coarse[:, 2], self.dirs)
+
+
+def targets(peaks: list, grid: int, size: int, dirs: np.ndarray | None = None):
+    """Location (1 at each peak's cell) and orientation-bin (-1 elsewhere) targets of one micrograph."""
+    cell = size // grid
+    heat = np.zeros((grid, grid), np.float32)
+    bins = -np.ones((grid, grid), np.int32)
+    for p in peaks:
+        cy, cx = int(p["y"]) // cell, int(p["x"]) // cell
+        heat[cy, cx] = 1.0
+        bins[cy, cx] = int(orientation_bin(p["phi"], p["theta"], p["psi"], dirs))
+    return heat, bins
+
+
+def propose(logits: np.ndarray, k: int, m: int, min_cells: int = 3):
+    """Top-k cells by location logit, skipping cells within min_cells (Chebyshev) of a taken one,
+    each with its top-m orientation bins: [((cell y, cell x), bins)]."""
+    G = logits.shape[0]
+    order = np.argsort(-logits[..., 0].ravel(), kind="stable")
+    taken = []
+    for c in order:
+        cy, cx = divmod(int(c), G)
+        if all(max(abs(cy - y), abs(cx - x)) >= min_cells for y, x in taken):
+            taken.append((cy, cx))
+            if len(taken) == k:
+                break
+    return [((cy, cx), np.argsort(-logits[cy, cx, 1:], kind="stable")[:m]) for cy, cx in taken]
+
+
 @dataclass
 class Labelled:
     micrograph: np.ndarray
@@ -96,18 +140,11 @@ class ProposalStudy:
         self.params = MT.SearchParams(angular_step=cfg.angular_step, in_plane_angular_step=cfg.in_plane_angular_step,
                                       padding=1.0, defocus1=cfg.defocus, defocus2=cfg.defocus)
         self.tpl_spectrum = MT.prepare_template(self.volume, 1.0, device)
-        grid = MT.search_grid("C1", cfg.angular_step)
-        psi = MT.psi_list(0.0, 360.0, cfg.in_plane_angular_step)
-        self.orients = np.array([(g[0], g[1], s) for g in grid for s in psi], np.float32)  # phi, theta, psi
-        self.dirs = fibonacci_directions()
-        self.orient_bin = orientation_bin(self.orients[:, 0], self.orients[:, 1], self.orients[:, 2], self.dirs)
-        self.members = [np.nonzero(self.orient_bin == b)[0] for b in range(N_BINS)]
+        tab = OrientationTable(cfg.angular_step, cfg.in_plane_angular_step)
+        self.orients, self.dirs, self.members = tab.orients, tab.dirs, tab.members
+        self.coarse, self.coarse_bin = tab.coarse, tab.coarse_bin
         self._refiner = None  # one refinement handle, its image replaced per micrograph
         self._exhaustive_search = None  # one search handle, its input replaced per micrograph
-        cgrid = MT.search_grid("C1", 2 * cfg.angular_step)
-        cpsi = MT.psi_list(0.0, 360.0, 2 * cfg.in_plane_angular_step)
-        self.coarse = np.array([(g[0], g[1], s) for g in cgrid for s in cpsi], np.float32)
-        self.coarse_bin = orientation_bin(self.coarse[:, 0], self.coarse[:, 1], self.coarse[:, 2], self.dirs)
 
     # ---------------------------------------------------------------- data
     def micrograph(self, seed: int) -> np.ndarray:
@@ -168,15 +205,7 @@ class ProposalStudy:
         return out
 
     def targets(self, peaks: list, grid: int):
-        """Location (1 at each peak's cell) and orientation-bin (-1 elsewhere) targets."""
-        cell = self.cfg.size // grid
-        heat = np.zeros((grid, grid), np.float32)
-        bins = -np.ones((grid, grid), np.int32)
-        for p in peaks:
-            cy, cx = int(p["y"]) // cell, int(p["x"]) // cell
-            heat[cy, cx] = 1.0
-            bins[cy, cx] = int(orientation_bin(p["phi"], p["theta"], p["psi"], self.dirs))
-        return heat, bins
+        return targets(peaks, grid, self.cfg.size, self.dirs)
 
     # ---------------------------------------------------------------- training
     def train(self, net: cnn.ProposalNet, data: list, epochs: int, batch: int = 16, lr: float = 1e-3,
@@ -202,23 +231,7 @@ class ProposalStudy:
 
     # ---------------------------------------------------------------- inference
     def propose(self, logits: np.ndarray, k: int, m: int, min_cells: int = 3):
-        """Top-k cells by location logit (cells within min_cells of a taken one skipped) and the top-m
-        orientation bins of each."""
-        G = logits.shape[0]
-        heat = logits[..., 0].ravel()
-        order = np.argsort(-heat, kind="stable")
-        taken = []
-        for c in order:
-            cy, cx = divmod(int(c), G)
-            if all(max(abs(cy - y), abs(cx - x)) >= min_cells for y, x in taken):
-                taken.append((cy, cx))
-                if len(taken) == k:
-                    break
-        out = []
-        for cy, cx in taken:
-            b = np.argsort(-logits[cy, cx, 1:], kind="stable")[:m]
-            out.append(((cy, cx), b))
-        return out
+        return propose(logits, k, m, min_cells)
 
     def _windows(self, proposals: list, grid: int):
         cell = self.cfg.size // grid
