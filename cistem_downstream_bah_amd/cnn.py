@@ -16,7 +16,7 @@ I32P = core.I32P
 MAX_LAYERS = 16
 CNN_EXPORTS = ["tm_cnn_create", "tm_cnn_destroy", "tm_cnn_shape", "tm_cnn_init_params", "tm_cnn_set_params",
                "tm_cnn_get_params", "tm_cnn_forward", "tm_cnn_backward", "tm_cnn_adam_step", "tm_cnn_get_activation",
-               "tm_cnn_debug_gemm", "tm_cnn_last_error"]
+               "tm_cnn_debug_gemm", "tm_cnn_last_error", "tm_cnn_heat", "tm_cnn_cell_logits"]
 
 
 class CnnConfig(ctypes.Structure):
@@ -39,6 +39,8 @@ def _lib():
         L.tm_cnn_forward.argtypes = [vp, F32P, ctypes.c_int, F32P]
         L.tm_cnn_backward.argtypes = [vp, F32P, I32P, ctypes.c_float, F32P, F32P]
         L.tm_cnn_adam_step.argtypes = [vp] + [ctypes.c_float] * 4
+        L.tm_cnn_heat.argtypes = [vp, F32P]
+        L.tm_cnn_cell_logits.argtypes = [vp, I32P, ctypes.c_int, F32P]
         L.tm_cnn_get_activation.argtypes = [vp, ctypes.c_int, F32P]
         L.tm_cnn_debug_gemm.argtypes = [ctypes.c_int, F32P, F32P, ctypes.c_int, ctypes.c_int, ctypes.c_int, F32P]
         L._tm_cnn_bound = True
@@ -124,6 +126,19 @@ class ProposalNet:
         B = x.shape[0]
         out = np.zeros((B, self.grid, self.grid, 1 + self.spec.n_bins), np.float32) if want_logits else None
         _check(_lib().tm_cnn_forward(self.h, _f(x), B, _f(out)))
+        return out
+
+    def heat(self, batch: int = 1) -> np.ndarray:
+        """Location logits of the last forward pass, (batch, grid, grid)."""
+        out = np.zeros((batch, self.grid, self.grid), np.float32)
+        _check(_lib().tm_cnn_heat(self.h, _f(out)))
+        return out
+
+    def cell_logits(self, cells) -> np.ndarray:
+        """Logits (1 + n_bins) of the given flat cells (b grid^2 + y grid + x)."""
+        c = np.ascontiguousarray(cells, np.int32)
+        out = np.zeros((c.size, 1 + self.spec.n_bins), np.float32)
+        _check(_lib().tm_cnn_cell_logits(self.h, c.ctypes.data_as(I32P), c.size, _f(out)))
         return out
 
     def backward(self, heat_target: np.ndarray, bin_target: np.ndarray, pos_weight: float = 1.0,

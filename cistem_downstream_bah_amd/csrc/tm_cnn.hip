@@ -343,6 +343,21 @@ __global__ __launch_bounds__(64) void k_loss(const float* __restrict__ logits, i
   if (lane == 0) cell_loss[2 * cell + 1] = lse - o[1 + tb];
 }
 
+// out[c] = logits[c][0] (the location logit of every cell)
+__global__ void k_column0(const float* __restrict__ logits, int Nh, long long cells, float* __restrict__ out) {
+  for (long long c = blockIdx.x * 256LL + threadIdx.x; c < cells; c += (long long)gridDim.x * 256)
+    out[c] = logits[c * Nh];
+}
+
+// out[q][*] = logits[cells[q]][*] (the logits of chosen cells)
+__global__ void k_gather_cells(const float* __restrict__ logits, int Nh, const int* __restrict__ cells, int n,
+                               float* __restrict__ out) {
+  const int q = blockIdx.x;
+  if (q >= n) return;
+  const float* src = logits + (long long)cells[q] * Nh;
+  for (int k = threadIdx.x; k < Nh; k += blockDim.x) out[(long long)q * Nh + k] = src[k];
+}
+
 __global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
                        long long n, float lr_t, float b1, float b2, float eps) {
   for (long long e = blockIdx.x * 256LL + threadIdx.x; e < n; e += (long long)gridDim.x * 256) {
@@ -466,8 +481,16 @@ struct tm_cnn {
   std::vector<Buf<float>> act;    // act[0]: standardised input; act[l + 1]: layer l output; last: logits
   Buf<bf16_t> col, colT, wb, wbT, dzb, dzT;
   Buf<float> dA0, dA1, dcol, parts, csum, dlog, cell_loss, heat_t;
-  Buf<int> bin_t;
+  Buf<int> bin_t, cells;
+  Buf<float> gathered;
   std::vector<float> h_loss;
+
+  ~tm_cnn() {
+    if (stream) {
+      (void)hipStreamSynchronize(stream);
+      (void)hipStreamDestroy(stream);
+    }
+  }
 };
 
 namespace {
@@ -605,9 +628,7 @@ int tm_cnn_destroy(tm_cnn* h) {
   return cguard([&] {
     if (!h) return;
     (void)hipSetDevice(h->device);
-    if (h->stream) (void)hipStreamSynchronize(h->stream);
-    if (h->stream) (void)hipStreamDestroy(h->stream);
-    delete h;
+    delete h;  // the destructor waits for and destroys the stream, the buffers free themselves
   });
 }
 
@@ -750,6 +771,41 @@ int tm_cnn_backward(tm_cnn* h, const float* heat_target, const int* bin_target, 
       loss[0] = (float)(lh / (double)cells);
       loss[1] = (float)(lb / (double)std::max(1LL, n_bin));
     }
+  });
+}
+
+int tm_cnn_heat(tm_cnn* h, float* heat) {
+  return cguard([&] {
+    require(h);
+    CNN_REQUIRE(heat, TM_ERR_INVALID, "heat is NULL");
+    CNN_REQUIRE(h->batch > 0, TM_ERR_STATE, "no forward pass");
+    const long long cells = (long long)h->batch * h->G * h->G;
+    // column 0 of the cell-major logits, gathered on the device and copied in one piece
+    h->gathered.ensure((size_t)cells);
+    hipLaunchKernelGGL(k_column0, dim3(grid1(cells)), dim3(256), 0, h->stream, h->act.back().p, h->Nh, cells,
+                       h->gathered.p);
+    check_launch("k_column0");
+    CNN_HIP(hipMemcpyAsync(heat, h->gathered.p, cells * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    CNN_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+int tm_cnn_cell_logits(tm_cnn* h, const int* cells, int n, float* out) {
+  return cguard([&] {
+    require(h);
+    CNN_REQUIRE(h->batch > 0, TM_ERR_STATE, "no forward pass");
+    CNN_REQUIRE(n >= 0 && (n == 0 || (cells && out)), TM_ERR_INVALID, "NULL argument");
+    if (n == 0) return;
+    const long long total = (long long)h->batch * h->G * h->G;
+    for (int q = 0; q < n; ++q) CNN_REQUIRE(cells[q] >= 0 && cells[q] < total, TM_ERR_INVALID, "cell out of range");
+    h->cells.ensure(n);
+    h->gathered.ensure((size_t)n * h->Nh);
+    CNN_HIP(hipMemcpyAsync(h->cells.p, cells, n * sizeof(int), hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(k_gather_cells, dim3(n), dim3(256), 0, h->stream, h->act.back().p, h->Nh, h->cells.p, n,
+                       h->gathered.p);
+    check_launch("k_gather_cells");
+    CNN_HIP(hipMemcpyAsync(out, h->gathered.p, (size_t)n * h->Nh * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    CNN_HIP(hipStreamSynchronize(h->stream));
   });
 }
 

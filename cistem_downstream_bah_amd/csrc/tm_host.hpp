@@ -181,6 +181,19 @@ void rows_c2r(Ctx* h, int N, const cf* in, size_t ibs, int irs, float* out, size
   }
 }
 
+// C2R rows of any compiled line length (the template sizes and the image sides).
+template <class Ctx>
+void rows_c2r_line(Ctx* h, int N, const cf* in, size_t ibs, int irs, float* out, size_t obs, int ors, int nrows,
+                   int nb) {
+  switch (N) {
+#define TM_CASE(n) \
+  case n: launch_c2r<n, Ctx>(h, in, ibs, irs, out, obs, ors, nrows, nb); break;
+    TM_LINE_SIZES(TM_CASE)
+#undef TM_CASE
+    default: throw TmError{TM_ERR_UNSUPPORTED, "unsupported C2R row length " + std::to_string(N)};
+  }
+}
+
 template <int N, class Ctx>
 void launch_r2c(Ctx* h, const float* src, size_t sbs, int srs, int r0, int s0, int len, int c0, cf* dst,
                 size_t dbs, int drs, int nrows, int nb, float scale) {

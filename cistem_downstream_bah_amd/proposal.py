@@ -75,11 +75,11 @@ def targets(peaks: list, grid: int, size: int, dirs: np.ndarray | None = None):
     return heat, bins
 
 
-def propose(logits: np.ndarray, k: int, m: int, min_cells: int = 3):
-    """Top-k cells by location logit, skipping cells within min_cells (Chebyshev) of a taken one,
-    each with its top-m orientation bins: [((cell y, cell x), bins)]."""
-    G = logits.shape[0]
-    order = np.argsort(-logits[..., 0].ravel(), kind="stable")
+def propose_cells(heat: np.ndarray, k: int, min_cells: int = 3) -> list:
+    """Top-k cells by location logit (heat: G x G), skipping cells within min_cells (Chebyshev) of a
+    taken one: [(cell y, cell x)]."""
+    G = heat.shape[0]
+    order = np.argsort(-heat.ravel(), kind="stable")
     taken = []
     for c in order:
         cy, cx = divmod(int(c), G)
@@ -87,7 +87,14 @@ def propose(logits: np.ndarray, k: int, m: int, min_cells: int = 3):
             taken.append((cy, cx))
             if len(taken) == k:
                 break
-    return [((cy, cx), np.argsort(-logits[cy, cx, 1:], kind="stable")[:m]) for cy, cx in taken]
+    return taken
+
+
+def propose(logits: np.ndarray, k: int, m: int, min_cells: int = 3):
+    """Top-k cells by location logit, skipping cells within min_cells (Chebyshev) of a taken one,
+    each with its top-m orientation bins: [((cell y, cell x), bins)]."""
+    cells = propose_cells(logits[..., 0], k, min_cells)
+    return [((cy, cx), np.argsort(-logits[cy, cx, 1:], kind="stable")[:m]) for cy, cx in cells]
 
 
 @dataclass
@@ -291,8 +298,12 @@ class ProposalStudy:
         "network" (its top-m bins), "coarse" (top-m bins of a coarse local search) or "union"."""
         t0 = time.perf_counter()
         if proposals is None:
-            logits = net.forward(micrograph[None])[0]
-            proposals = self.propose(logits, k, m)
+            # the location logits, then the orientation logits of the chosen cells only
+            net.forward(micrograph[None], want_logits=False)
+            cells = propose_cells(net.heat()[0], k)
+            G = net.grid
+            lg = net.cell_logits([cy * G + cx for cy, cx in cells]) if cells else np.zeros((0, 1 + N_BINS))
+            proposals = [(c, np.argsort(-lg[q, 1:], kind="stable")[:m]) for q, c in enumerate(cells)]
         t1 = time.perf_counter()
         refiner = self._refiner
         if refiner is None:
@@ -301,7 +312,7 @@ class ProposalStudy:
                                                                      padding=1.0, max_in_flight=2048),
                                                       template_spectrum=self.tpl_spectrum, device=self.device)
         else:
-            refiner.set_image(micrograph, device=self.device)
+            refiner.set_micrograph(micrograph)  # prepared on the device
         t2 = time.perf_counter()
         n0 = 0
         if bins in ("coarse", "union"):

@@ -24,7 +24,8 @@ OUTPUT_NAMES = ("best_mip", "best_scaled_mip", "best_psi", "best_theta", "best_p
                 "best_pixel_size")
 REFINE_EXPORTS = ["tm_prepare_refine_image", "tm_refine_create", "tm_refine_destroy", "tm_refine_init",
                   "tm_refine_run", "tm_refine_peak_list", "tm_refine_score", "tm_refine_set_timing",
-                  "tm_refine_get_timing", "tm_refine_last_error", "tm_refine_score_batch", "tm_refine_set_image"]
+                  "tm_refine_get_timing", "tm_refine_last_error", "tm_refine_score_batch", "tm_refine_set_image",
+                  "tm_refine_set_micrograph", "tm_refine_get_image"]
 
 
 class RefineParams(ctypes.Structure):
@@ -66,6 +67,8 @@ def _lib():
         L.tm_refine_score.argtypes = [vp, ctypes.c_int, ctypes.c_int] + [ctypes.c_float] * 6 + [F32P]
         L.tm_refine_score_batch.argtypes = [vp, ctypes.c_int, ip, F32P, ctypes.c_float, F32P]
         L.tm_refine_set_image.argtypes = [vp, F32P, F32P, F32P, ctypes.c_int]
+        L.tm_refine_set_micrograph.argtypes = [vp, F32P]
+        L.tm_refine_get_image.argtypes = [vp, F32P, F32P, F32P]
         L.tm_refine_set_timing.argtypes = [vp, ctypes.c_int]
         i64p = ctypes.POINTER(ctypes.c_longlong)
         L.tm_refine_get_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_double), i64p,
@@ -171,6 +174,21 @@ class TemplateRefiner:
         _check(_lib().tm_refine_set_image(self.h, _f(np.ascontiguousarray(img, np.float32)),
                                           _f(np.ascontiguousarray(cx, np.float32)),
                                           _f(np.ascontiguousarray(cy, np.float32)), int(cx.size)))
+
+    def set_micrograph(self, micrograph: np.ndarray) -> None:
+        """Replaces the micrograph (same size), prepared on the device: tm_refine_set_micrograph."""
+        m = np.ascontiguousarray(micrograph, np.float32)
+        if m.shape != (self.my, self.mx):
+            raise ValueError(f"micrograph shape {m.shape} != {(self.my, self.mx)}")
+        _check(_lib().tm_refine_set_micrograph(self.h, _f(m)))
+
+    def image(self):
+        """The handle's whitened image and whitening curve (x, y): tm_refine_get_image."""
+        img = np.zeros((self.my, self.mx), np.float32)
+        cx = np.zeros(self.curve_x.size, np.float32)
+        cy = np.zeros(self.curve_x.size, np.float32)
+        _check(_lib().tm_refine_get_image(self.h, _f(img), _f(cx), _f(cy)))
+        return img, cx, cy
 
     def peak_list(self, scaled_mip: np.ndarray, cap: int = 100000) -> np.ndarray:
         sm = np.ascontiguousarray(scaled_mip, np.float32)

@@ -56,6 +56,12 @@ int tm_cnn_get_params(tm_cnn* h, float* params);
  * batch x G x G x (1 + n_bins) values, cell-major. */
 int tm_cnn_forward(tm_cnn* h, const float* images, int batch, float* logits);
 
+/* Location logits of the last forward pass (batch x G x G), and the 1 + n_bins logits of n chosen
+ * cells (flat indices b G^2 + y G + x) into out (n x (1 + n_bins)): what a proposal step reads
+ * instead of the whole logit volume. */
+int tm_cnn_heat(tm_cnn* h, float* heat);
+int tm_cnn_cell_logits(tm_cnn* h, const int* cells, int n, float* out);
+
 /* Loss and gradients of the last forward pass. heat_target: batch x G x G values in {0, 1};
  * bin_target: batch x G x G bins (-1: no orientation term at that cell). Loss = mean over cells of
  * the location logit's binary cross-entropy (positives weighted by pos_weight) + the mean over

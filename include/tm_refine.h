@@ -89,6 +89,18 @@ int tm_refine_init(tm_refine* h, const tm_refine_params* params, const float* wh
 int tm_refine_set_image(tm_refine* h, const float* whitened_image, const float* curve_x, const float* curve_y,
                         int n_curve);
 
+/* The per-micrograph preparation of refine_template.cpp:526-546 on the device, for a handle that
+ * scores many micrographs of one size: the raw micrograph (image_x x image_y dense) is cleaned of
+ * 5-sigma outliers, transformed, whitened by its own radial power spectrum and normalised, and
+ * becomes the handle's whitened image and curve, as tm_prepare_refine_image followed by
+ * tm_refine_set_image would give (the curve's bins are summed in FP64 rather than in float). No
+ * counterpart in the reference (its program prepares one image per run, on the CPU). */
+int tm_refine_set_micrograph(tm_refine* h, const float* micrograph);
+
+/* Test hook: the handle's whitened image (dense) and whitening curve (n_curve points each); any
+ * pointer may be NULL. */
+int tm_refine_get_image(tm_refine* h, float* whitened, float* curve_x, float* curve_y);
+
 /* Refines every peak of the scaled MIP. maps[7]: mip, scaled_mip, psi, theta, phi, defocus,
  * pixel_size, each image_x x image_y dense (the search's output maps). out_maps[7] (the array or
  * any entry may be NULL) receive best_mip, best_scaled_mip, best_psi, best_theta, best_phi,

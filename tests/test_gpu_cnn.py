@@ -3,7 +3,8 @@
 * bf16 MFMA product C = A . B^T: ragged M, N, K, a 601-column head shape and a split-K case, against
   float64 on the same bf16-rounded operands, within 1e-5 of sum |a| |b| per element
 * forward pass: every layer's activations and the logits within 1e-5 (relative L2) of the oracle
-  with the device's bf16 rounding points (the remainder is FP32 accumulation order)
+  with the device's bf16 rounding points (the remainder is FP32 accumulation order); the location
+  logits and the logits of chosen cells read back equal the full logit volume
 * backward pass: the loss terms within 1e-4 relative and every parameter tensor's gradient within
   1e-4 (relative L2) of the oracle; the oracle's own gradient is pinned by finite differences on the CPU
 * training: 40 Adam steps on a fixed batch lower the loss by more than half; parameters round-trip
@@ -72,6 +73,10 @@ def test_forward_matches_oracle():
         e = _rel(logits, acts[-1])
         print(f"logits: rel {e:.2e}", flush=True)
         assert e <= 1e-5
+        # the proposal step's reads: the location logits and the logits of chosen cells
+        assert np.array_equal(net.heat(3), logits[..., 0])
+        cells = [0, 5, 63, 64 + 17, 3 * 16 * 16 - 1]
+        assert np.array_equal(net.cell_logits(cells), logits.reshape(-1, logits.shape[-1])[cells])
     finally:
         net.close()
 
