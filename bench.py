@@ -454,9 +454,12 @@ def bench_proposal(args):
             for v in variants}
     t_ex, n_peaks = 0.0, 0
     misses = []
+    ex_split = {}
     for i, m in enumerate(evalset):
         peaks, dt = study.exhaustive(m)
         t_ex += dt
+        for key, val in study.last_exhaustive_times.items():
+            ex_split[key] = ex_split.get(key, 0.0) + val
         n_peaks += len(peaks)
         for v in variants:
             kind, k, mb = v
@@ -513,6 +516,7 @@ def bench_proposal(args):
                    "network_params": net.n_params, "grid": net.grid, "orientation_bins": PR.N_BINS},
         "chosen": pick,
         "exhaustive_s_per_micrograph": round(t_ex / ne, 5),
+        "exhaustive_split_ms": {key: round(1e3 * val / ne, 3) for key, val in ex_split.items()},
         "recall_at_chosen": pick["recall"],
         "speedup_vs_exhaustive": pick["speedup_vs_exhaustive"],
         "table": table,

@@ -187,9 +187,13 @@ class ProposalStudy:
             search.core.set_graph(True)
         else:
             search.set_micrograph(micrograph)
-        pk = self._peaks(search, search.run())
-        dt = time.perf_counter() - t0
-        return pk, dt
+        t1 = time.perf_counter()
+        n = search.run()
+        t2 = time.perf_counter()
+        pk = self._peaks(search, n)
+        t3 = time.perf_counter()
+        self.last_exhaustive_times = {"image_preparation": t1 - t0, "search": t2 - t1, "z_score_and_peaks": t3 - t2}
+        return pk, t3 - t0
 
     def labelled(self, seeds, workers: int = 4, log=None) -> list:
         """Micrographs of the seeds labelled by the exhaustive search, `workers` searches at a time
