@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtm_hip.so")
 # (source, C++ standard): the refinement's descent is written as C++20 coroutines
 SOURCES = [("tm_core.hip", "c++17"), ("tm_prepare.hip", "c++17"), ("tm_refine.hip", "c++20"), ("tm_cnn.hip", "c++17")]
-HEADERS = ["tm_fft.hpp", "tm_kernels.hpp", "tm_host.hpp", "tm_sizes.hpp"]
+HEADERS = ["tm_fft.hpp", "tm_kernels.hpp", "tm_host.hpp", "tm_sizes.hpp", "tm_prep_kernels.hpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

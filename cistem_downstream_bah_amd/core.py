@@ -26,7 +26,8 @@ I64P = ctypes.POINTER(ctypes.c_longlong)
 # Every symbol include/tm_hip.h declares.
 EXPORTS = ["tm_device_count", "tm_create", "tm_destroy", "tm_init", "tm_run_inner_loop", "tm_get_results", "tm_merge",
            "tm_set_template", "tm_reset", "tm_finalize", "tm_debug_projection", "tm_debug_ccf", "tm_set_timing", "tm_get_timing",
-           "tm_last_error", "tm_autotune_batch", "tm_set_graph", "tm_set_input"]
+           "tm_last_error", "tm_autotune_batch", "tm_set_graph", "tm_set_input", "tm_set_micrograph",
+           "tm_find_peaks"]
 
 PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.c_longlong, ctypes.c_void_p)
 
@@ -74,6 +75,9 @@ def lib():
         L.tm_set_timing.argtypes = [vp, ctypes.c_int]
         L.tm_set_graph.argtypes = [vp, ctypes.c_int]
         L.tm_set_input.argtypes = [vp, F32P]
+        L.tm_set_micrograph.argtypes = [vp, F32P, F32P, F32P]
+        L.tm_find_peaks.argtypes = [vp, ctypes.c_longlong, ctypes.c_float, ctypes.c_int, ctypes.c_float, F32P,
+                                    ctypes.c_int, I32P]
         L.tm_get_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_char_p), F64P, I64P, ctypes.c_int]
         L.tm_autotune_batch.argtypes = [ctypes.c_int, ctypes.POINTER(InitParams), F32P, F32P, F32P, F32P, F32P, I32P,
                                         ctypes.c_int, ctypes.c_int, I32P, F64P]
@@ -218,6 +222,29 @@ class TemplateMatchingCore:
         if inp.shape != (self.H, self.W // 2 + 1):
             raise ValueError(f"input spectrum shape {inp.shape} != {(self.H, self.W // 2 + 1)}")
         _check(lib().tm_set_input(self.h, _f32(inp.view(np.float32))))
+
+    def set_micrograph(self, micrograph: np.ndarray):
+        """Prepares a W x H micrograph on the device into the input spectrum (tm_set_micrograph);
+        returns the whitening curve (x, y)."""
+        m = np.ascontiguousarray(micrograph, np.float32)
+        if m.shape != (self.H, self.W):
+            raise ValueError(f"micrograph shape {m.shape} != {(self.H, self.W)}")
+        n = int((self.W / 2.0 + 1.0) * float(np.sqrt(np.float32(2.0))) + 1.0)
+        cx = np.zeros(n, np.float32)
+        cy = np.zeros(n, np.float32)
+        _check(lib().tm_set_micrograph(self.h, _f32(m), _f32(cx), _f32(cy)))
+        return cx, cy
+
+    def find_peaks(self, total_ccfs: int, threshold: float, edge: int, min_peak_radius: float = 10.0,
+                   capacity: int = 1000) -> list:
+        """Peaks of the z-score map found on the device (tm_find_peaks): dicts with x, y (pixels),
+        peak_height and the psi, theta, phi, defocus, pixel_size values at the peak."""
+        out = np.zeros((capacity, 8), np.float32)
+        n = ctypes.c_int(0)
+        _check(lib().tm_find_peaks(self.h, total_ccfs, threshold, edge, min_peak_radius, _f32(out), capacity,
+                                   ctypes.byref(n)))
+        keys = ("x", "y", "peak_height", "psi", "theta", "phi", "defocus", "pixel_size")
+        return [{k: float(v) for k, v in zip(keys, row)} for row in out[: n.value]]
 
     def reset(self) -> None:
         _check(lib().tm_reset(self.h))

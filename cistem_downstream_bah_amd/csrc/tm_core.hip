@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
+#include <climits>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -18,6 +19,7 @@
 #include <vector>
 
 #include "tm_host.hpp"
+#include "tm_prep_kernels.hpp"
 
 namespace {
 
@@ -81,6 +83,10 @@ struct tm_core {
   hipGraphExec_t graph_exec = nullptr;
   DevBuf<int> plane_base;
   int* h_plane_base = nullptr;  // pinned host staging of the offset
+  // device preparation of a micrograph (tm_set_micrograph)
+  DevBuf<float> pr_src, pr_pad, pr_st, pr_cx, pr_cy;
+  DevBuf<cf> pr_spec;
+  DevBuf<double> pr_part, pr_acc, pr_pc, pr_ps;
 
   void drop_graph() {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -371,6 +377,75 @@ void require_ready(tm_core* h) {
   TM_HIP(hipSetDevice(h->device));
 }
 
+// ------------------------------------------------------------------ device peak search
+// First maximum in row-major order of z over [edge, W - edge) x [edge, H - edge): per-block
+// (value, index) partials over rows, then one block combines them (ties keep the lower index).
+__device__ inline bool zbetter(float v, int i, float bv, int bi) { return v > bv || (v == bv && i < bi); }
+
+__global__ __launch_bounds__(256) void k_zmax_part(const float* __restrict__ z, int W, int H, int edge,
+                                                   float* __restrict__ pv, int* __restrict__ pi) {
+  __shared__ float sv[256];
+  __shared__ int si[256];
+  float bv = -INFINITY;
+  int bi = INT_MAX;
+  const int x0 = edge, x1 = W - edge, y1 = H - edge;
+  for (int y = edge + blockIdx.x; y < y1; y += gridDim.x)
+    for (int x = x0 + threadIdx.x; x < x1; x += 256) {
+      const int i = y * W + x;
+      const float v = z[i];
+      if (zbetter(v, i, bv, bi)) { bv = v; bi = i; }
+    }
+  sv[threadIdx.x] = bv;
+  si[threadIdx.x] = bi;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k && zbetter(sv[threadIdx.x + k], si[threadIdx.x + k], sv[threadIdx.x], si[threadIdx.x])) {
+      sv[threadIdx.x] = sv[threadIdx.x + k];
+      si[threadIdx.x] = si[threadIdx.x + k];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    pv[blockIdx.x] = sv[0];
+    pi[blockIdx.x] = si[0];
+  }
+}
+
+// the maximum of n partials, its pixel's best orientation index: out = {value bits, pixel, index}
+__global__ __launch_bounds__(256) void k_zmax_final(const float* __restrict__ pv, const int* __restrict__ pi, int n,
+                                                    const int* __restrict__ idx, int* __restrict__ out) {
+  __shared__ float sv[256];
+  __shared__ int si[256];
+  float bv = -INFINITY;
+  int bi = INT_MAX;
+  for (int q = threadIdx.x; q < n; q += 256)
+    if (zbetter(pv[q], pi[q], bv, bi)) { bv = pv[q]; bi = pi[q]; }
+  sv[threadIdx.x] = bv;
+  si[threadIdx.x] = bi;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k && zbetter(sv[threadIdx.x + k], si[threadIdx.x + k], sv[threadIdx.x], si[threadIdx.x])) {
+      sv[threadIdx.x] = sv[threadIdx.x + k];
+      si[threadIdx.x] = si[threadIdx.x + k];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = __float_as_int(sv[0]);
+    out[1] = si[0];
+    out[2] = si[0] == INT_MAX ? -1 : idx[si[0]];
+  }
+}
+
+// -FLT_MAX over the pixels of [by - r, by + r) x [bx - r, bx + r) whose squared distance is <= r2
+__global__ void k_zmask(float* __restrict__ z, int W, int H, int bx, int by, int r, float r2) {
+  const int x = max(bx - r, 0) + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int y = max(by - r, 0) + (int)blockIdx.y;
+  if (x >= min(bx + r, W) || y >= min(by + r, H)) return;
+  const float dx = float(x) - bx, dy = float(y) - by;
+  if (dx * dx + dy * dy <= r2) z[(size_t)y * W + x] = -FLT_MAX;
+}
+
 }  // namespace
 
 // ====================================================================== C ABI
@@ -528,6 +603,91 @@ int tm_set_input(tm_core* h, const float* input_spectrum) {
     upload_input(h, input_spectrum);
     reset_state(h);
     TM_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+int tm_set_micrograph(tm_core* h, const float* micrograph, float* curve_x, float* curve_y) {
+  return guarded([&] {
+    require_ready(h);
+    TM_REQUIRE(micrograph, TM_ERR_INVALID, "micrograph is NULL");
+    const int W = h->W, H = h->H, nh = W / 2 + 1;
+    const int n = int((W / 2.0 + 1.0) * sqrtf(2.0) + 1.0);
+    const long long npx = (long long)W * H, nspec = (long long)nh * H;
+    hipStream_t st = h->stream;
+    TM_HIP(hipStreamSynchronize(st));
+    if (h->pr_src.n != (size_t)npx) h->pr_src.alloc(npx);
+    if (h->pr_pad.n != (size_t)(W + 2) * H) h->pr_pad.alloc((size_t)(W + 2) * H);
+    if (h->pr_spec.n != (size_t)nspec) h->pr_spec.alloc(nspec);
+    if (!h->pr_part.p) h->pr_part.alloc(2 * 1024);
+    if (!h->pr_st.p) h->pr_st.alloc(3);
+    if (h->pr_acc.n != (size_t)2 * n) h->pr_acc.alloc(2 * (size_t)n);
+    if (h->pr_cx.n != (size_t)n) { h->pr_cx.alloc(n); h->pr_cy.alloc(n); }
+    TM_HIP(hipMemcpyAsync(h->pr_src.p, micrograph, npx * sizeof(float), hipMemcpyHostToDevice, st));
+    // ReplaceOutliersWithMean(5) (image.cpp:986-1100)
+    const long long chunk = std::max<long long>(256, (npx + 1023) / 1024);
+    const int np = (int)((npx + chunk - 1) / chunk);
+    hipLaunchKernelGGL(tmp::k_moments<256>, dim3(np), dim3(256), 0, st, h->pr_src.p, npx, chunk, h->pr_part.p);
+    check_launch("k_moments");
+    hipLaunchKernelGGL(tmp::k_outlier_bounds, dim3(1), dim3(1), 0, st, h->pr_part.p, np, npx, h->pr_st.p);
+    check_launch("k_outlier_bounds");
+    const unsigned g = (unsigned)std::min<long long>(65535, ((long long)(W + 2) * H + 255) / 256);
+    hipLaunchKernelGGL(tmp::k_outlier_pad, dim3(g), dim3(256), 0, st, h->pr_src.p, W, H, h->pr_st.p, h->pr_pad.p);
+    check_launch("k_outlier_pad");
+    // ForwardFFT (scaled 1/N), SwapRealSpaceQuadrants, ZeroCentralPixel (match_template.cpp:703-709)
+    rows_r2c(h, W, h->pr_pad.p, (size_t)(W + 2) * H, W + 2, 0, 0, W, 0, h->pr_spec.p, nspec, nh, H, 1, 1.0f);
+    cols_line(h, H, h->pr_spec.p, nh, nspec, 1, -1);
+    const float inv_n = 1. / (float)((long)W * H);
+    const unsigned gs = (unsigned)std::min<long long>(65535, (nspec + 255) / 256);
+    hipLaunchKernelGGL(tmp::k_scale_zero_dc, dim3(gs), dim3(256), 0, st, h->pr_spec.p, nspec, inv_n);
+    check_launch("k_scale_zero_dc");
+    hipLaunchKernelGGL(tmp::k_swap_phase_2d, dim3(gs), dim3(256), 0, st, h->pr_spec.p, W, H);
+    check_launch("k_swap_phase_2d");
+    // whitening curve (Curve::SetupXAxis(0, 0.5 sqrt 2, n), match_template.cpp:711-716)
+    std::vector<float> x(n);
+    const float lo = 0.0f, hi = 0.5 * sqrtf(2.0);
+    for (int c = 0; c < n; ++c) x[c] = c * (hi - lo) / float(n - 1) + lo;
+    TM_HIP(hipMemcpyAsync(h->pr_cx.p, x.data(), n * sizeof(float), hipMemcpyHostToDevice, st));
+    TM_HIP(hipMemsetAsync(h->pr_acc.p, 0, 2 * (size_t)n * sizeof(double), st));
+    hipLaunchKernelGGL(tmp::k_power_curve, dim3(gs), dim3(256), 0, st, h->pr_spec.p, W, H, h->pr_cx.p, n, h->pr_acc.p,
+                       h->pr_acc.p + n);
+    check_launch("k_power_curve");
+    hipLaunchKernelGGL(tmp::k_curve_finish, dim3(1), dim3(1), 0, st, h->pr_acc.p, h->pr_acc.p + n, n, h->pr_cy.p);
+    check_launch("k_curve_finish");
+    // ApplyCurveFilter, ZeroCentralPixel, 1 / sqrt(ReturnSumOfSquares()) (match_template.cpp:717-724)
+    hipLaunchKernelGGL(tmp::k_apply_curve, dim3(gs), dim3(256), 0, st, h->pr_spec.p, W, H, h->pr_cx.p, h->pr_cy.p, n);
+    check_launch("k_apply_curve");
+    const long long schunk = std::max<long long>(256, (nspec + 1023) / 1024);
+    const int nsp = (int)((nspec + schunk - 1) / schunk);
+    hipLaunchKernelGGL(tmp::k_spectrum_ssq<256>, dim3(nsp), dim3(256), 0, st, h->pr_spec.p, W, H, schunk,
+                       h->pr_part.p);
+    check_launch("k_spectrum_ssq");
+    hipLaunchKernelGGL(tmp::k_normalise_spectrum, dim3(gs), dim3(256), 0, st, h->pr_spec.p, nspec, h->pr_part.p, nsp,
+                       1);
+    check_launch("k_normalise_spectrum");
+    // the handle's input spectrum: transposed, with the centring phase at H = 4096 (as upload_input)
+    const double* pc = nullptr;
+    const double* ps = nullptr;
+    if (column_rows_at_zero(H)) {
+      if (h->pr_pc.n != (size_t)H) { h->pr_pc.alloc(H); h->pr_ps.alloc(H); }
+      std::vector<double> c(H), sn(H);
+      const long double two_pi = 2.0L * acosl(-1.0L);
+      const long long y0 = H / 2 - h->T / 2;
+      for (int y = 0; y < H; ++y) {
+        const long double a = two_pi * (long double)((y0 * y) % H) / (long double)H;
+        c[y] = (double)cosl(a);
+        sn[y] = (double)sinl(a);
+      }
+      TM_HIP(hipMemcpyAsync(h->pr_pc.p, c.data(), H * sizeof(double), hipMemcpyHostToDevice, st));
+      TM_HIP(hipMemcpyAsync(h->pr_ps.p, sn.data(), H * sizeof(double), hipMemcpyHostToDevice, st));
+      pc = h->pr_pc.p;
+      ps = h->pr_ps.p;
+    }
+    hipLaunchKernelGGL(tmp::k_transpose_phase, dim3(gs), dim3(256), 0, st, h->pr_spec.p, nh, H, pc, ps, h->inpT.p);
+    check_launch("k_transpose_phase");
+    if (curve_x) TM_HIP(hipMemcpyAsync(curve_x, h->pr_cx.p, n * sizeof(float), hipMemcpyDeviceToHost, st));
+    if (curve_y) TM_HIP(hipMemcpyAsync(curve_y, h->pr_cy.p, n * sizeof(float), hipMemcpyDeviceToHost, st));
+    reset_state(h);
+    TM_HIP(hipStreamSynchronize(st));
   });
 }
 
@@ -697,6 +857,74 @@ int tm_merge(tm_core* dst, tm_core* src) {
     check_launch("k_add_u64");
     TM_HIP(hipStreamSynchronize(dst->stream));
     dst->n_ccfs += src->n_ccfs;
+  });
+}
+
+int tm_find_peaks(tm_core* h, long long total_ccfs, float threshold, int edge, float min_peak_radius, float* out,
+                  int capacity, int* n_peaks) {
+  return guarded([&] {
+    require_ready(h);
+    TM_REQUIRE(total_ccfs > 0 && n_peaks && (out || capacity == 0) && capacity >= 0 && edge >= 0, TM_ERR_INVALID,
+               "invalid arguments");
+    const int W = h->W, H = h->H;
+    const size_t npx = (size_t)W * H;
+    hipStream_t st = h->stream;
+    DevBuf<float> fin, pv;
+    DevBuf<int> pi, res;
+    fin.alloc(4 * npx);
+    const int nparts = std::max(1, std::min(1024, H - 2 * edge));
+    pv.alloc(nparts);
+    pi.alloc(nparts);
+    res.alloc(3);
+    const double sqrt_px = sqrt((double)(W * H));
+    hipLaunchKernelGGL(tmk::k_finalize, dim3((npx + 255) / 256), dim3(256), 0, st, h->mip.p, h->sum.p, h->sumsq.p, npx,
+                       (float)total_ccfs, (float)sqrt_px, fin.p, fin.p + npx, fin.p + 2 * npx, fin.p + 3 * npx);
+    check_launch("k_finalize");
+    float* z = fin.p + npx;
+    const int r = (int)min_peak_radius;
+    const float r2 = powf(float(r), 2);
+    const int per_plane = h->prm.n_search_positions * h->n_psi;
+    int found = 0;
+    while (found < capacity && W > 2 * edge && H > 2 * edge) {
+      hipLaunchKernelGGL(k_zmax_part, dim3(nparts), dim3(256), 0, st, z, W, H, edge, pv.p, pi.p);
+      check_launch("k_zmax_part");
+      hipLaunchKernelGGL(k_zmax_final, dim3(1), dim3(256), 0, st, pv.p, pi.p, nparts, h->idx.p, res.p);
+      check_launch("k_zmax_final");
+      int hres[3];
+      TM_HIP(hipMemcpyAsync(hres, res.p, sizeof(hres), hipMemcpyDeviceToHost, st));
+      TM_HIP(hipStreamSynchronize(st));
+      float v;
+      std::memcpy(&v, &hres[0], sizeof(v));
+      if (hres[1] < 0 || hres[1] == INT_MAX || !(v >= threshold)) break;
+      const int bx = hres[1] % W, by = hres[1] / W, g = hres[2];
+      float a_psi = 0.f, a_theta = 0.f, a_phi = 0.f, a_def = 0.f, a_pix = 0.f;
+      if (g >= 0) {
+        const int plane = g / per_plane, rem = g % per_plane;
+        const int pos = rem / h->n_psi, k = rem % h->n_psi;
+        a_psi = h->psi[k];
+        a_theta = h->grid[2 * pos + 1];
+        a_phi = h->grid[2 * pos];
+        a_def = float(h->planes[plane].y) * h->prm.defocus_step;
+        a_pix = float(h->planes[plane].x) * h->prm.pixel_size_step;
+      }
+      float* o = out + 8 * (size_t)found;
+      o[0] = (float)bx;
+      o[1] = (float)by;
+      o[2] = v;
+      o[3] = a_psi;
+      o[4] = a_theta;
+      o[5] = a_phi;
+      o[6] = a_def;
+      o[7] = a_pix;
+      ++found;
+      const int x_lo = std::max(bx - r, 0), x_hi = std::min(bx + r, W), y_lo = std::max(by - r, 0), y_hi = std::min(by + r, H);
+      if (x_hi > x_lo && y_hi > y_lo) {
+        hipLaunchKernelGGL(k_zmask, dim3((x_hi - x_lo + 63) / 64, y_hi - y_lo), dim3(64), 0, st, z, W, H, bx, by, r, r2);
+        check_launch("k_zmask");
+      }
+    }
+    TM_HIP(hipStreamSynchronize(st));
+    *n_peaks = found;
   });
 }
 

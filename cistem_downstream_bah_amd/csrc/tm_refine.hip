@@ -27,6 +27,7 @@
 
 #include "../../include/tm_refine.h"
 #include "tm_host.hpp"
+#include "tm_prep_kernels.hpp"
 
 namespace rtk {
 
@@ -330,233 +331,6 @@ __global__ void k_map_mask(float* __restrict__ a, int mx, int x0, int y0, int w,
   if (sq <= radius) a[(size_t)j * mx + i] = -FLT_MAX;
 }
 
-
-// ---------------------------------------------------------------- image preparation on the device
-// The per-micrograph preparation of refine_template.cpp:526-546 (ReplaceOutliersWithMean(5),
-// ForwardFFT, ZeroCentralPixel, Compute1DPowerSpectrumCurve, SquareRoot, Reciprocal, normalise to a
-// maximum of 1, ApplyCurveFilter, ZeroCentralPixel, DivideByConstant(sqrt(ReturnSumOfSquares())),
-// BackwardFFT) with the reference's float expressions; sums run in FP64 (the reference sums the
-// curve bins in float, in scan order).
-
-// FP64 sum and sum of squares (of powf(v, 2) in float) of a dense image, per block of a fixed chunk.
-template <int NT>
-__global__ __launch_bounds__(NT) void k_moments(const float* __restrict__ img, long long n, long long chunk,
-                                                double* __restrict__ part) {
-#pragma clang fp contract(off)
-  __shared__ double r1[NT], r2[NT];
-  const long long lo = blockIdx.x * chunk, hi = min(n, lo + chunk);
-  double a = 0.0, b = 0.0;
-  for (long long e = lo + threadIdx.x; e < hi; e += NT) {
-    const float v = img[e];
-    a += v;
-    b += v * v;
-  }
-  r1[threadIdx.x] = a;
-  r2[threadIdx.x] = b;
-  __syncthreads();
-  for (int k = NT / 2; k > 0; k >>= 1) {
-    if (threadIdx.x < k) {
-      r1[threadIdx.x] += r1[threadIdx.x + k];
-      r2[threadIdx.x] += r2[threadIdx.x + k];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    part[2 * blockIdx.x] = r1[0];
-    part[2 * blockIdx.x + 1] = r2[0];
-  }
-}
-
-// mean and the 5-sigma bounds of ReplaceOutliersWithMean (the host expression order)
-__global__ void k_outlier_bounds(const double* __restrict__ part, int nparts, long long n, float* __restrict__ st) {
-  double s = 0.0, s2 = 0.0;
-  for (int q = 0; q < nparts; ++q) {
-    s += part[2 * q];
-    s2 += part[2 * q + 1];
-  }
-  const float m = (float)(s / n);
-  const float sigma = sqrtf(fabsf(float(s2 / n - (double)(m * m))));
-  st[0] = m;
-  st[1] = m - 5.0f * sigma;
-  st[2] = m + 5.0f * sigma;
-}
-
-// dense micrograph -> padded rows (mx + 2) with outliers replaced by the mean
-__global__ void k_outlier_pad(const float* __restrict__ src, int mx, int my, const float* __restrict__ st,
-                              float* __restrict__ dst) {
-  const long long n = (long long)(mx + 2) * my;
-  const float m = st[0], lo = st[1], hi = st[2];
-  for (long long e = blockIdx.x * 256LL + threadIdx.x; e < n; e += (long long)gridDim.x * 256) {
-    const int x = (int)(e % (mx + 2));
-    const long long y = e / (mx + 2);
-    float v = 0.f;
-    if (x < mx) {
-      v = src[y * mx + x];
-      if (v > hi || v < lo) v = m;
-    }
-    dst[e] = v;
-  }
-}
-
-__global__ void k_scale_zero_dc(cf* __restrict__ c, long long n, float s) {
-  for (long long e = blockIdx.x * 256LL + threadIdx.x; e < n; e += (long long)gridDim.x * 256) {
-    cf v = c[e];
-    v.x *= s;
-    v.y *= s;
-    if (e == 0) v = cmake(0.f, 0.f);
-    c[e] = v;
-  }
-}
-
-// Curve::AddValueAtXUsingLinearInterpolation weights of one value at wx (curve x[0..n-1])
-__device__ inline void curve_add(const float* __restrict__ cx, int n, float wx, float val, double* __restrict__ acc) {
-#pragma clang fp contract(off)
-  const int idx = (cx[0] == cx[1]) ? n - 2 : int((wx - cx[0]) / (cx[1] - cx[0]));
-  if (idx == n - 1) {
-    atomicAdd(acc + idx, (double)val);
-    return;
-  }
-  const float d = (wx - cx[idx]) / (cx[idx + 1] - cx[idx]);
-  const float vm = val * (1.0 - d), vn = val * d;
-  atomicAdd(acc + idx, (double)vm);
-  atomicAdd(acc + idx + 1, (double)vn);
-}
-
-// Compute1DPowerSpectrumCurve (image.cpp:7135-7205) on a W x H half spectrum: intensities and
-// counts into the curve's bins (Hermitian mates of the i = 0 column skipped).
-__global__ void k_power_curve(const cf* __restrict__ c, int W, int H, const float* __restrict__ cx, int n,
-                              double* __restrict__ acc_w, double* __restrict__ acc_n) {
-#pragma clang fp contract(off)
-  const int nh = W / 2 + 1, fny = H / 2;
-  const long long total = (long long)nh * H;
-  for (long long e = blockIdx.x * 256LL + threadIdx.x; e < total; e += (long long)gridDim.x * 256) {
-    const int i = (int)(e % nh), j = (int)(e / nh);
-    if (i == 0 && j >= fny) continue;
-    const float fy = (float)((j >= fny ? j - H : j) * (1.0 / double(H)));
-    const float fx = (float)(i * (1.0 / double(W)));
-    const float sf = sqrtf(fx * fx + fy * fy + 0.0f);
-    const cf v = c[e];
-    curve_add(cx, n, sf, v.x * v.x + v.y * v.y, acc_w);
-    curve_add(cx, n, sf, 1.0f, acc_n);
-  }
-}
-
-// whitening filter: mean power per bin, SquareRoot, Reciprocal, scaled to a maximum of 1 (one thread)
-__global__ void k_curve_finish(const double* __restrict__ acc_w, const double* __restrict__ acc_n, int n,
-                               float* __restrict__ cy) {
-  float mx = -FLT_MAX;
-  for (int q = 0; q < n; ++q) {
-    const float w = (float)acc_w[q], c = (float)acc_n[q];
-    float a = c > 0.0f ? w / c : 0.0f;
-    a = sqrtf(a);
-    if (a != 0.0f) a = (float)(1.0 / a);
-    cy[q] = a;
-    mx = fmaxf(mx, a);
-  }
-  const float k = 1.0f / mx;
-  for (int q = 0; q < n; ++q) cy[q] *= k;
-}
-
-// Curve::ReturnLinearInterpolationFromX
-__device__ inline float curve_interp(const float* __restrict__ cx, const float* __restrict__ cy, int n, float wx) {
-#pragma clang fp contract(off)
-  int i = n - 1;
-  if (wx < cx[0]) {
-    i = 0;
-  } else {
-    int c = int((wx - cx[0]) / (cx[1] - cx[0]));
-    c = max(0, min(c, n - 1));
-    while (c > 0 && wx < cx[c]) --c;
-    while (c < n - 1 && wx >= cx[c + 1]) ++c;
-    i = c;
-  }
-  if (i == n - 1) return cy[n - 1];
-  float r = 0.0f;
-  const float d = (wx - cx[i]) / (cx[i + 1] - cx[i]);
-  r += (float)(cy[i] * (1.0 - d));
-  r += cy[i + 1] * d;
-  return r;
-}
-
-// ApplyCurveFilter (image.cpp:7284-7313, resolution limit 1), then ZeroCentralPixel
-__global__ void k_apply_curve(cf* __restrict__ c, int W, int H, const float* __restrict__ cx,
-                              const float* __restrict__ cy, int n) {
-#pragma clang fp contract(off)
-  const int nh = W / 2 + 1;
-  const int fny = (H % 2 == 0) ? H / 2 : H / 2 + 1;
-  const long long total = (long long)nh * H;
-  for (long long e = blockIdx.x * 256LL + threadIdx.x; e < total; e += (long long)gridDim.x * 256) {
-    const int i = (int)(e % nh), j = (int)(e / nh);
-    // std::pow of the double frequency, then rounded to float (image.cpp:7298-7303)
-    const double fy = (j >= fny ? j - H : j) * (1.0 / double(H));
-    const double fx = i * (1.0 / double(W));
-    const float s2 = (float)(fx * fx) + (float)(fy * fy) + 0.0f;
-    const float sf = (float)sqrt((double)s2);
-    cf v = c[e];
-    if (e == 0) {
-      v = cmake(0.f, 0.f);
-    } else if (sf <= 1.0f) {
-      const float f = curve_interp(cx, cy, n, sf);
-      v.x *= f;
-      v.y *= f;
-    } else {
-      v = cmake(0.f, 0.f);
-    }
-    c[e] = v;
-  }
-}
-
-// ReturnSumOfSquares, Fourier branch (image.cpp:290-311) of a 2-D half spectrum: FP64 partials
-template <int NT>
-__global__ __launch_bounds__(NT) void k_spectrum_ssq(const cf* __restrict__ c, int W, int H, long long chunk,
-                                                     double* __restrict__ part) {
-  __shared__ double r[NT];
-  const int nh = W / 2 + 1;
-  const bool xe = W % 2 == 0, ye = H % 2 == 0;
-  const int fny = ye ? H / 2 : H / 2 + 1;
-  const int lo_y = ye ? -H / 2 : -(H - 1) / 2;
-  const long long total = (long long)nh * H;
-  const long long lo = blockIdx.x * chunk, hi = min(total, lo + chunk);
-  double a = 0.0;
-  for (long long e = lo + threadIdx.x; e < hi; e += NT) {
-    const int i = (int)(e % nh), j = (int)(e / nh);
-    const int jj = j >= fny ? j - H : j;
-    const cf v = c[e];
-    const double m = (double)hypotf(v.x, v.y);  // |c| in float, squared in double
-    const double a2 = m * m;
-    const bool edge_x = (i == 0 || (i == W / 2 && xe));
-    if (edge_x && (jj == 0 || (jj == lo_y && xe))) a += a2 * 0.5;
-    else if ((i != 0 && (i != W / 2 || !xe)) || jj >= 0) a += a2;
-  }
-  r[threadIdx.x] = a;
-  __syncthreads();
-  for (int k = NT / 2; k > 0; k >>= 1) {
-    if (threadIdx.x < k) r[threadIdx.x] += r[threadIdx.x + k];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) part[blockIdx.x] = r[0];
-}
-
-// c *= 1 / sqrt(sum of squares) (DivideByConstant of refine_template.cpp:544)
-__global__ void k_normalise_spectrum(cf* __restrict__ c, long long n, const double* __restrict__ part, int nparts) {
-  double s = 0.0;
-  for (int q = 0; q < nparts; ++q) s += part[q];
-  const float ssq = (float)(2.0 * s);
-  const float inv = (float)(1. / (double)(float)sqrt((double)ssq));
-  for (long long e = blockIdx.x * 256LL + threadIdx.x; e < n; e += (long long)gridDim.x * 256) {
-    cf v = c[e];
-    v.x *= inv;
-    v.y *= inv;
-    c[e] = v;
-  }
-}
-
-// padded rows (mx + 2) -> dense image
-__global__ void k_unpad(const float* __restrict__ src, int mx, int my, float* __restrict__ dst) {
-  const long long n = (long long)mx * my;
-  for (long long e = blockIdx.x * 256LL + threadIdx.x; e < n; e += (long long)gridDim.x * 256)
-    dst[e] = src[(e / mx) * (mx + 2) + e % mx];
-}
 
 }  // namespace rtk
 
@@ -1573,19 +1347,19 @@ int tm_refine_set_micrograph(tm_refine* h, const float* micrograph) {
     // ReplaceOutliersWithMean(5)
     const long long chunk = std::max<long long>(256, (npx + 1023) / 1024);
     const int np = (int)((npx + chunk - 1) / chunk);
-    hipLaunchKernelGGL(rtk::k_moments<256>, dim3(np), dim3(256), 0, st, h->pr_src.p, npx, chunk, h->pr_part.p);
+    hipLaunchKernelGGL(tmp::k_moments<256>, dim3(np), dim3(256), 0, st, h->pr_src.p, npx, chunk, h->pr_part.p);
     check_launch("k_moments");
-    hipLaunchKernelGGL(rtk::k_outlier_bounds, dim3(1), dim3(1), 0, st, h->pr_part.p, np, npx, h->pr_st.p);
+    hipLaunchKernelGGL(tmp::k_outlier_bounds, dim3(1), dim3(1), 0, st, h->pr_part.p, np, npx, h->pr_st.p);
     check_launch("k_outlier_bounds");
     const unsigned g = (unsigned)std::min<long long>(65535, ((long long)(mx + 2) * my + 255) / 256);
-    hipLaunchKernelGGL(rtk::k_outlier_pad, dim3(g), dim3(256), 0, st, h->pr_src.p, mx, my, h->pr_st.p, h->pr_pad.p);
+    hipLaunchKernelGGL(tmp::k_outlier_pad, dim3(g), dim3(256), 0, st, h->pr_src.p, mx, my, h->pr_st.p, h->pr_pad.p);
     check_launch("k_outlier_pad");
     // ForwardFFT (scaled 1/N), ZeroCentralPixel
     rows_r2c(h, mx, h->pr_pad.p, (size_t)(mx + 2) * my, mx + 2, 0, 0, mx, 0, h->pr_spec.p, nspec, nh, my, 1, 1.0f);
     cols_line(h, my, h->pr_spec.p, nh, nspec, 1, -1);
     const float inv_n = 1. / (float)((long)mx * my);
     const unsigned gs = (unsigned)std::min<long long>(65535, (nspec + 255) / 256);
-    hipLaunchKernelGGL(rtk::k_scale_zero_dc, dim3(gs), dim3(256), 0, st, h->pr_spec.p, nspec, inv_n);
+    hipLaunchKernelGGL(tmp::k_scale_zero_dc, dim3(gs), dim3(256), 0, st, h->pr_spec.p, nspec, inv_n);
     check_launch("k_scale_zero_dc");
     // the whitening curve on the curve's x axis (Curve::SetupXAxis(0, 0.5 sqrt 2, n))
     std::vector<float> x(n);
@@ -1593,26 +1367,27 @@ int tm_refine_set_micrograph(tm_refine* h, const float* micrograph) {
     for (int c = 0; c < n; ++c) x[c] = c * (hi - lo) / float(n - 1) + lo;
     TM_HIP(hipMemcpyAsync(h->curve_x.p, x.data(), n * sizeof(float), hipMemcpyHostToDevice, st));
     TM_HIP(hipMemsetAsync(h->pr_acc.p, 0, 2 * (size_t)n * sizeof(double), st));
-    hipLaunchKernelGGL(rtk::k_power_curve, dim3(gs), dim3(256), 0, st, h->pr_spec.p, mx, my, h->curve_x.p, n,
+    hipLaunchKernelGGL(tmp::k_power_curve, dim3(gs), dim3(256), 0, st, h->pr_spec.p, mx, my, h->curve_x.p, n,
                        h->pr_acc.p, h->pr_acc.p + n);
     check_launch("k_power_curve");
-    hipLaunchKernelGGL(rtk::k_curve_finish, dim3(1), dim3(1), 0, st, h->pr_acc.p, h->pr_acc.p + n, n, h->curve_y.p);
+    hipLaunchKernelGGL(tmp::k_curve_finish, dim3(1), dim3(1), 0, st, h->pr_acc.p, h->pr_acc.p + n, n, h->curve_y.p);
     check_launch("k_curve_finish");
     // ApplyCurveFilter, ZeroCentralPixel, DivideByConstant(sqrt(ReturnSumOfSquares()))
-    hipLaunchKernelGGL(rtk::k_apply_curve, dim3(gs), dim3(256), 0, st, h->pr_spec.p, mx, my, h->curve_x.p,
+    hipLaunchKernelGGL(tmp::k_apply_curve, dim3(gs), dim3(256), 0, st, h->pr_spec.p, mx, my, h->curve_x.p,
                        h->curve_y.p, n);
     check_launch("k_apply_curve");
     const long long schunk = std::max<long long>(256, (nspec + 1023) / 1024);
     const int nsp = (int)((nspec + schunk - 1) / schunk);
-    hipLaunchKernelGGL(rtk::k_spectrum_ssq<256>, dim3(nsp), dim3(256), 0, st, h->pr_spec.p, mx, my, schunk,
+    hipLaunchKernelGGL(tmp::k_spectrum_ssq<256>, dim3(nsp), dim3(256), 0, st, h->pr_spec.p, mx, my, schunk,
                        h->pr_part.p);
     check_launch("k_spectrum_ssq");
-    hipLaunchKernelGGL(rtk::k_normalise_spectrum, dim3(gs), dim3(256), 0, st, h->pr_spec.p, nspec, h->pr_part.p, nsp);
+    hipLaunchKernelGGL(tmp::k_normalise_spectrum, dim3(gs), dim3(256), 0, st, h->pr_spec.p, nspec, h->pr_part.p, nsp,
+                       0);
     check_launch("k_normalise_spectrum");
     // BackwardFFT (unscaled) into the handle's whitened image
     cols_line(h, my, h->pr_spec.p, nh, nspec, 1, +1);
     rows_c2r_line(h, mx, h->pr_spec.p, nspec, nh, h->pr_pad.p, (size_t)(mx + 2) * my, mx + 2, my, 1);
-    hipLaunchKernelGGL(rtk::k_unpad, dim3((unsigned)std::min<long long>(65535, (npx + 255) / 256)), dim3(256), 0, st,
+    hipLaunchKernelGGL(tmp::k_unpad, dim3((unsigned)std::min<long long>(65535, (npx + 255) / 256)), dim3(256), 0, st,
                        h->pr_pad.p, mx, my, h->img.p);
     check_launch("k_unpad");
     TM_HIP(hipStreamSynchronize(st));

@@ -184,9 +184,18 @@ class MatchTemplateSearch:
                        self.histogram_min_scaled, self.histogram_step_scaled, 512, self.max_padding,
                        self.defocus_step, self.pixel_size_step, p.batch_size, p.plane_storage)
 
-    def set_micrograph(self, micrograph: np.ndarray) -> None:
+    def set_micrograph(self, micrograph: np.ndarray, on_device: bool = False) -> None:
         """Searches another micrograph of the same size with this handle (tm_set_input): the template,
-        grid, work buffers and a captured graph are kept; the maps restart from zero."""
+        grid, work buffers and a captured graph are kept; the maps restart from zero. on_device:
+        prepare it on the device (tm_set_micrograph; the micrograph must already be W x H and need
+        no rotation); the host copy of the input spectrum is then not kept."""
+        if on_device:
+            if micrograph.shape != (self.H, self.W) or self.rotated:
+                raise ValueError("device preparation needs a micrograph at the search size without rotation")
+            self.curve_x, self.curve_y = self.core.set_micrograph(micrograph)
+            self.input_spectrum = None
+            self.original_shape = micrograph.shape
+            return
         spec, cx, cy, W, H, rotated, max_padding = prepare_input_image(micrograph, self.T, self.device)
         if (W, H, rotated) != (self.W, self.H, self.rotated):
             raise ValueError("the micrograph's search size differs from this search's")

@@ -168,11 +168,10 @@ class ProposalStudy:
                                       template_spectrum=self.tpl_spectrum)
 
     def _peaks(self, search, n: int) -> list:
-        fin = search.core.finalize(n)
-        r = search.core.results()
+        """The search's peaks above the threshold, found on the device (tm_find_peaks; the
+        micrographs here need no padding or rotation, so the search geometry is the image's)."""
         thr = self.cfg.threshold if self.cfg.threshold > 0 else PK.expected_threshold(search.W, search.H, n)
-        pk = PK.find_peaks(fin["zscore"], thr, self.cfg.template_size, self.cfg.min_peak_radius,
-                           maps={"phi": r.phi, "theta": r.theta, "psi": r.psi})
+        pk = search.core.find_peaks(n, thr, self.cfg.template_size // 4 + 1, float(self.cfg.min_peak_radius))
         return [dict(x=p["x"], y=p["y"], z=p["peak_height"], phi=p["phi"], theta=p["theta"], psi=p["psi"])
                 for p in pk]
 
@@ -186,7 +185,7 @@ class ProposalStudy:
             search = self._exhaustive_search = self._search(micrograph)
             search.core.set_graph(True)
         else:
-            search.set_micrograph(micrograph)
+            search.set_micrograph(micrograph, on_device=True)
         t1 = time.perf_counter()
         n = search.run()
         t2 = time.perf_counter()

@@ -92,6 +92,11 @@ class TemplateMatchingCore {
   // accumulating, as after the reference's GPU[tIDX].template_reconstruction.CopyFrom
   // (match_template.cpp:883-886).
   void SetTemplate(const float* template_spectrum) { check(tm_set_template(h_, template_spectrum)); }
+  // Prepares a micrograph at the search size on the device into the input spectrum, resets the maps
+  // (no counterpart in the reference).
+  void SetMicrograph(const float* micrograph, float* curve_x, float* curve_y) {
+    check(tm_set_micrograph(h_, micrograph, curve_x, curve_y));
+  }
   // Replaces the input spectrum and resets the maps (no counterpart in the reference).
   void SetInput(const float* input_spectrum) { check(tm_set_input(h_, input_spectrum)); }
   // Replays each plane from one captured hipGraph (no counterpart in the reference).

@@ -140,6 +140,26 @@ int tm_get_timing(tm_core* h, const char** names, double* ms, long long* launche
  * TemplateMatchingCore). */
 int tm_set_input(tm_core* h, const float* input_spectrum);
 
+/* The input preparation of match_template.cpp:703-724 on the device, for a micrograph already at
+ * the handle's image size W x H (after the size policy; dense, W x H floats): ReplaceOutliersWithMean(5),
+ * ForwardFFT, SwapRealSpaceQuadrants, the whitening curve and filter, normalisation; the result
+ * replaces the input spectrum (as tm_set_input does, maps reset) and curve_x/curve_y (may be NULL)
+ * receive the whitening curve for the projection filters (int((W/2+1) sqrt 2 + 1) points). The
+ * curve's bins are summed in FP64. Micrographs that need padding or rotation go through
+ * tm_prepare_input_image and tm_set_input. No counterpart in the reference. */
+int tm_set_micrograph(tm_core* h, const float* micrograph, float* curve_x, float* curve_y);
+
+/* The peak search of match_template.cpp:1839-1980 on the device, over the z-score map of the
+ * search geometry (W x H, which is the micrograph's when the size policy neither pads nor rotates
+ * it): the finalised z-score (as tm_finalize), then repeatedly the first maximum in row-major order
+ * over [edge, W - edge) x [edge, H - edge), stopping below threshold or at capacity, each peak masking
+ * the pixels of [y - r, y + r) x [x - r, x + r) whose squared distance is <= r^2 (r = the integer
+ * part of min_peak_radius). out receives 8 floats per peak: x, y (pixels), z, psi, theta, phi,
+ * defocus and pixel-size offsets. No counterpart in the reference's GPU path (the peak search runs
+ * on the host there). */
+int tm_find_peaks(tm_core* h, long long total_ccfs, float threshold, int edge, float min_peak_radius, float* out,
+                  int capacity, int* n_peaks);
+
 /* Graph mode, with no counterpart in the reference (its loop launches every kernel): when enabled,
  * the first tm_run_inner_loop after it captures the launches of one plane (all batches) into a
  * hipGraph and every later plane replays it; the plane's orientation-index offset is read from

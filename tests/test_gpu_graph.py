@@ -9,6 +9,8 @@ offset read from device memory.
 * a second tm_init drops the graph; the handle captures again and stays equal
 * tm_set_input: after searching one micrograph, a graph-mode handle searches another of the same
   size exactly as a handle made for that micrograph
+* tm_set_micrograph: the input preparation on the device gives the host path's whitening curve
+  (1e-5) and a search within 1e-5 of its MIP and sums, best index identical on >= 99% of pixels
 """
 import numpy as np
 import pytest
@@ -80,3 +82,32 @@ def test_set_input_equals_a_fresh_search(c0):
         reused.core.close()
         if fresh is not None:
             fresh.core.close()
+
+
+def test_device_preparation_search(c0):
+    """tm_set_micrograph: the input preparation on the device gives the host preparation's curve and
+    a search within the parity bars of a handle prepared on the host."""
+    print("case: set_micrograph on the device vs host preparation", flush=True)
+    other = synth.config0(7).micrograph
+    dev, host = _search(c0, False), None
+    try:
+        dev.set_micrograph(other, on_device=True)
+        host = _search(synth.Synthetic(other, c0.volume, []), False)
+        assert np.array_equal(dev.curve_x, host.curve_x)
+        e_curve = float(np.abs(dev.curve_y - host.curve_y).max() / np.abs(host.curve_y).max())
+        print(f"curve {e_curve:.2e}", flush=True)
+        assert e_curve <= 1e-5
+        for d in (0, 1):
+            dev.core.run_inner_loop(dev.filter_for(d), 0, d)
+            host.core.run_inner_loop(host.filter_for(d), 0, d)
+        ra, rb = dev.core.results(), host.core.results()
+        e_mip = float(np.abs(ra.mip - rb.mip).max() / np.abs(rb.mip).max())
+        same = float(np.mean(ra.best_index == rb.best_index))
+        e_sum = float(np.abs(ra.sum - rb.sum).max() / np.abs(rb.sum).max())
+        print(f"MIP {e_mip:.2e}, index identical {same:.4f}, sum {e_sum:.2e}", flush=True)
+        assert e_mip <= 1e-5 and same >= 0.99 and e_sum <= 1e-5
+        assert np.array_equal(ra.hist.sum(), rb.hist.sum())
+    finally:
+        dev.core.close()
+        if host is not None:
+            host.core.close()
