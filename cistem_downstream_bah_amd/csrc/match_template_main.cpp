@@ -239,7 +239,20 @@ int main(int argc, char** argv) {
     }
     // peaks (match_template.cpp:1839-1980): integer maximum of the z-score map away from a
     // T/4 + 1 border; mask a disc of min_peak_radius; stop below the threshold or at 1000 peaks
-    {
+    if (!r90 && W == ox && H == oy) {
+      // the search geometry is the image's: the peak search runs on the device
+      std::vector<float> pk(8 * 1000);
+      const int found = core.FindPeaks(total, thr, T / 4 + 1, min_peak_radius, pk.data(), 1000);
+      FILE* pf2 = std::fopen((out + "_peaks.txt").c_str(), "w");
+      std::fprintf(pf2, "# x_A y_A psi theta phi defocus pixel_size peak_height\n");
+      for (int q = 0; q < found; ++q) {
+        const float* r = &pk[8 * (size_t)q];
+        std::fprintf(pf2, "%f %f %f %f %f %f %f %f\n", r[0] * pixel_size, r[1] * pixel_size, r[3], r[4], r[5], r[6],
+                     r[7], r[2]);
+      }
+      std::fclose(pf2);
+      std::fprintf(stderr, "%d peaks above the expected threshold %.3f\n", found, thr);
+    } else {
       std::vector<float> zz = z_o;
       const int edge = T / 4 + 1;
       const int rad = (int)min_peak_radius;

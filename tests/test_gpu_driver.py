@@ -1,6 +1,7 @@
 """GPU: the C++ match_template driver (cistem_downstream_bah_amd/match_template_hip) end to end on the
-config-0 inputs written as MRC files: its maps equal the C-ABI path's and its peak list contains
-every inserted particle."""
+config-0 inputs written as MRC files: its maps equal the C-ABI path's, its peak list (found on the
+device) equals the host peak rule applied to its own output maps, and it contains every inserted
+particle above the threshold."""
 import os
 import subprocess
 
@@ -8,7 +9,7 @@ import numpy as np
 import pytest
 
 from cistem_downstream_bah_amd import match_template as MT
-from cistem_downstream_bah_amd import mrc, synth
+from cistem_downstream_bah_amd import mrc, peaks as PK, synth
 
 pytestmark = pytest.mark.gpu
 DRIVER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "cistem_downstream_bah_amd",
@@ -43,6 +44,15 @@ def test_driver_end_to_end(tmp_path):
         detectable += 1
         assert any(abs(px - pt.x) <= 2 and abs(py - pt.y) <= 2 for px, py in peaks[:, :2]), (pt.x, pt.y)
     print(f"{len(peaks)} peaks, {detectable} of {len(s.particles)} inserted copies above {thr}", flush=True)
+    # the driver found its peaks on the device (no padding or rotation here): the host rule on its
+    # own output maps gives the same list
+    maps = {k: mrc.read_mrc(out + f"_{k}.mrc") for k in ("psi", "theta", "phi", "defocus")}
+    host = PK.find_peaks(z, thr, 64, 10, maps=maps)
+    assert len(host) == len(peaks)
+    for row, h in zip(peaks, host):
+        assert (row[0], row[1]) == (h["x"], h["y"])
+        assert abs(row[7] - h["peak_height"]) <= 1e-5 * abs(h["peak_height"])
+        assert np.allclose(row[2:6], [h["psi"], h["theta"], h["phi"], h["defocus"]], rtol=0, atol=1e-5)
     assert detectable >= 8
     hist = np.loadtxt(out + "_histogram.txt")
     assert hist.shape == (512, 4) and hist[:, 1].sum() == n * 512 * 512
