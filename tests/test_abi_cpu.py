@@ -95,3 +95,29 @@ def test_synthetic_projection_convention():
     assert abs(ix - (32 + want[0])) <= 0.5 and abs(iy - (32 + want[1])) <= 0.5
     # integral of the projected atom: amplitude * sigma sqrt(2 pi) (along z) * 2 pi sigma^2 (in-plane)
     assert abs(img.sum() - 1.5 * np.sqrt(2 * np.pi) * 2 * np.pi * 1.5 ** 2) < 1e-3
+
+
+def test_new_entry_points_reject_bad_arguments():
+    """Entry points without a reference counterpart return an error code (never crash) on NULL
+    handles and invalid arguments; none of these calls reaches a GPU."""
+    L = core.lib()
+    n = ctypes.c_int(0)
+    assert L.tm_set_graph(None, 1) != 0
+    assert L.tm_set_input(None, None) != 0
+    assert L.tm_set_micrograph(None, None, None, None) != 0
+    assert L.tm_find_peaks(None, 1, 0.0, 0, 10.0, None, 0, ctypes.byref(n)) != 0
+    assert L.tm_autotune_batch(0, None, None, None, None, None, None, None, 0, 0, None, None) != 0
+    R = refine._lib()
+    assert R.tm_refine_set_micrograph(None, None) != 0
+    assert R.tm_refine_set_image(None, None, None, None, 0) != 0
+    assert R.tm_refine_score_batch(None, 0, None, None, 0.0, None) != 0
+    assert R.tm_refine_get_image(None, None, None, None) != 0
+    C = cnn._lib()
+    h = ctypes.c_void_p()
+    assert C.tm_cnn_create(0, None, ctypes.byref(h)) != 0
+    bad = cnn.CnnConfig(512, 0, (ctypes.c_int * cnn.MAX_LAYERS)(), (ctypes.c_int * cnn.MAX_LAYERS)(), 600, 1)
+    assert C.tm_cnn_create(0, ctypes.byref(bad), ctypes.byref(h)) != 0 and not h.value
+    assert C.tm_cnn_heat(None, None) != 0
+    assert C.tm_cnn_cell_logits(None, None, 0, None) != 0
+    assert C.tm_cnn_adam_step(None, 1e-3, 0.9, 0.999, 1e-8) != 0
+    assert C.tm_cnn_last_error()
