@@ -107,7 +107,8 @@ def main():
     ap.add_argument("--weights", type=str, default="",
                     help="proposal: load the network (raw float32 parameters) instead of training")
     ap.add_argument("--save-weights", type=str, default="", help="proposal: save the trained network as raw float32 parameters")
-    ap.add_argument("--workload", choices=["config1", "config2", "refine", "proposal", "rocfft"], default="config1",
+    ap.add_argument("--workload", choices=["config1", "config2", "refine", "proposal", "rocfft", "cnn"],
+                    default="config1",
                     help="config2: eight micrographs of mixed sizes (own JSON line, not the headline)")
     ap.add_argument("--concurrency", type=str, default="1,2,4",
                     help="config2: numbers of images searched concurrently (one stream each)")
@@ -147,6 +148,9 @@ def main():
         return
     if args.workload == "rocfft":
         bench_rocfft(args)
+        return
+    if args.workload == "cnn":
+        bench_cnn(args)
         return
 
     if args.small:
@@ -358,6 +362,63 @@ def planted_maps(s, off=(3.0, -3.0, 3.0), margin=60):
             maps["theta"][iy, ix] = q.theta + off[1]
             maps["psi"][iy, ix] = q.psi + off[2]
     return maps
+
+
+def cnn_step_flops(spec, batch: int) -> float:
+    """Multiply-add FLOPs (2 per MAC) of one training step: per 3x3 convolution the forward product
+    (pixels x cout x 9 cin), the weight gradient (the same) and the data gradient (the same, not for
+    the first layer); the same three for the 1 x 1 head."""
+    s, cin, total = spec.input_size, 1, 0.0
+    for l, (c, st) in enumerate(zip(spec.channels, spec.strides)):
+        s //= st
+        mac = batch * s * s * c * 9 * cin
+        total += 2.0 * mac * (3 if l > 0 else 2)
+        cin = c
+    total += 2.0 * batch * s * s * (1 + spec.n_bins) * cin * 3
+    return total
+
+
+def bench_cnn(args):
+    """Training steps of the proposal network (forward, backward, Adam; bf16 MFMA products) on
+    seeded synthetic 512^2 images and targets: images/s and achieved TFLOP/s of the products
+    against the bf16 dense MFMA peak."""
+    from cistem_downstream_bah_amd import cnn
+
+    spec = cnn.NetSpec()
+    B = 16
+    net = cnn.ProposalNet(spec, max_batch=B, seed=1)
+    rng = np.random.default_rng(0)
+    imgs = rng.standard_normal((B, spec.input_size, spec.input_size)).astype(np.float32)
+    G = net.grid
+    heat = (rng.random((B, G, G)) < 0.003).astype(np.float32)
+    bins = np.where(heat > 0, rng.integers(0, spec.n_bins, (B, G, G)), -1).astype(np.int32)
+
+    def step():
+        net.forward(imgs, want_logits=False)
+        net.backward(heat, bins, pos_weight=30.0)
+        net.adam_step(lr=1e-3)
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    dt = time.perf_counter() - t0
+    flops = cnn_step_flops(spec, B) * args.steps
+    tflops = flops / dt / 1e12
+    peak = 2500.0
+    out = {"metric": "proposal-network training images/sec (forward + backward + Adam), 512^2 input",
+           "value": round(B * args.steps / dt, 2), "unit": "images/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(1e3 * dt / args.steps, 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "bf16 products, fp32 accumulation and weights",
+           "data": "synthetic (seeded images and targets)",
+           "config": {"workload": "cnn_train_512sq_b16", "batch": B, "params": net.n_params, "grid": G},
+           "roofline": {"bound": "mfma", "achieved": round(tflops, 2), "peak": peak, "unit": "TFLOP/s",
+                        "frac": round(tflops / peak, 5), "traffic": None,
+                        "kernel": "whole step (every product of the step counted once)",
+                        "flops_per_step": cnn_step_flops(spec, B)}}
+    net.close()
+    print(json.dumps(out), flush=True)
 
 
 def bench_rocfft(args):

@@ -83,18 +83,19 @@ inline unsigned grid1(long long n, int bs = 256, long long cap = 1 << 20) {
 
 // ------------------------------------------------------------------ kernels
 // C (+ z * split_stride) = A[M][lda] . B[N][ldb]^T over k in [z * kchunk, min(K, (z + 1) * kchunk)),
-// K and kchunk multiples of 16, lda and ldb multiples of 4. Block (x: 64 rows, y: 64 columns);
-// wave w owns rows 32 (w / 2) .. and columns 32 (w % 2) .. as 2 x 2 MFMA tiles. MFMA 16x16x16
-// bf16 lane layout: A[m = lane % 16][k = 4 (lane / 16) + i], B[n = lane % 16][k = 4 (lane / 16) + i],
-// D[m = 4 (lane / 16) + i][n = lane % 16].
-template <bool BIAS, bool RELU>
+// K and kchunk multiples of 16, lda and ldb multiples of 4. A block of four waves covers
+// (128 / WN) rows x (32 WN) columns; wave w owns rows 32 (w / WN) .. and columns 32 (w % WN) .. as
+// 2 x 2 MFMA tiles (WN = 2: 64 x 64; WN = 1: 128 x 32 for N <= 32; WN = 4: 32 x 128 for M <= 32).
+// MFMA 16x16x16 bf16 lane layout: A[m = lane % 16][k = 4 (lane / 16) + i],
+// B[n = lane % 16][k = 4 (lane / 16) + i], D[m = 4 (lane / 16) + i][n = lane % 16].
+template <bool BIAS, bool RELU, int WN>
 __global__ __launch_bounds__(256) void k_gemm_nt(const bf16_t* __restrict__ A, int lda, const bf16_t* __restrict__ B,
                                                  int ldb, float* __restrict__ C, long long ldc, long long M, int N,
                                                  int K, int kchunk, long long split_stride,
                                                  const float* __restrict__ bias) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const long long m0 = (long long)blockIdx.x * 64 + (wave >> 1) * 32;
-  const int n0 = blockIdx.y * 64 + (wave & 1) * 32;
+  const long long m0 = (long long)blockIdx.x * (128 / WN) + (wave / WN) * 32;
+  const int n0 = blockIdx.y * (32 * WN) + (wave % WN) * 32;
   const int r = lane & 15, kq = (lane >> 4) * 4;
   const int k_lo = blockIdx.z * kchunk, k_hi = min(K, k_lo + kchunk);
   const bf16_t* ap[2];
@@ -145,55 +146,106 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const bf16_t* __restrict__ A, i
       }
 }
 
-// out[m][n] = sum over z of parts[z][m][n] in z order (+ bias, ReLU)
+// out[m][n] = sum over z of parts[z][m][n] (+ bias, ReLU): a block covers 32 consecutive outputs
+// with 8 interleaved z subsets each (z = q, q + 8, ...), added in subset order.
 template <bool BIAS, bool RELU>
-__global__ void k_reduce_splits(const float* __restrict__ parts, int nsplit, long long M, int N,
-                                long long split_stride, float* __restrict__ out, long long ldo,
-                                const float* __restrict__ bias) {
+__global__ __launch_bounds__(256) void k_reduce_splits(const float* __restrict__ parts, int nsplit, long long M, int N,
+                                                       long long split_stride, float* __restrict__ out,
+                                                       long long ldo, const float* __restrict__ bias) {
+  __shared__ float red[8][32];
   const long long total = M * N;
-  for (long long e = blockIdx.x * 256LL + threadIdx.x; e < total; e += (long long)gridDim.x * 256) {
-    const long long m = e / N;
-    const int n = (int)(e % N);
-    float s = 0.f;
-    for (int z = 0; z < nsplit; ++z) s += parts[z * split_stride + m * N + n];
-    if (BIAS) s += bias[n];
-    if (RELU) s = fmaxf(s, 0.f);
-    out[m * ldo + n] = s;
+  const int o = threadIdx.x & 31, q = threadIdx.x >> 5;
+  for (long long e0 = blockIdx.x * 32LL; e0 < total; e0 += (long long)gridDim.x * 32) {
+    const long long e = e0 + o;
+    float s0 = 0.f, s1 = 0.f;
+    if (e < total) {
+      int z = q;
+      for (; z + 8 < nsplit; z += 16) {
+        s0 += parts[z * split_stride + e];
+        s1 += parts[(z + 8) * split_stride + e];
+      }
+      if (z < nsplit) s0 += parts[z * split_stride + e];
+    }
+    red[q][o] = s0 + s1;
+    __syncthreads();
+    if (q == 0 && e < total) {
+      float s = 0.f;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) s += red[k][o];
+      const long long m = e / N;
+      const int n = (int)(e - m * N);
+      if (BIAS) s += bias[n];
+      if (RELU) s = fmaxf(s, 0.f);
+      out[m * ldo + n] = s;
+    }
+    __syncthreads();
   }
 }
 
 // col[row][k] (bf16, Kp per row): row = (b, oy, ox) of the So x So output, k = (ky * 3 + kx) * Cin + c
-// reads X[b][oy * stride + ky - 1][ox * stride + kx - 1][c] (0 outside the image and for k >= 9 Cin)
-__global__ void k_im2col(const float* __restrict__ X, int Si, int Cin, int So, int stride, long long rows, int Kp,
+// reads X[b][oy * stride + ky - 1][ox * stride + kx - 1][c] (0 outside the image and for k >= 9 Cin).
+// One thread per 4 consecutive k of a row (Kp % 4 == 0); when Cin % 4 == 0 the 4 share one tap and
+// are one 16-byte load. rows * Kp / 4 < 2^31 (checked by the caller).
+__global__ void k_im2col(const float* __restrict__ X, int Si, int Cin, int So, int stride, int rows, int Kp,
                          bf16_t* __restrict__ col) {
-  const long long total = rows * Kp;
-  const int K9 = 9 * Cin;
-  for (long long e = blockIdx.x * 256LL + threadIdx.x; e < total; e += (long long)gridDim.x * 256) {
-    const long long row = e / Kp;
-    const int k = (int)(e % Kp);
-    float v = 0.f;
-    if (k < K9) {
-      const int tap = k / Cin, c = k % Cin;
-      const int ky = tap / 3, kx = tap % 3;
-      const long long b = row / ((long long)So * So);
-      const int p = (int)(row % ((long long)So * So));
-      const int iy = (p / So) * stride + ky - 1, ix = (p % So) * stride + kx - 1;
-      if (iy >= 0 && iy < Si && ix >= 0 && ix < Si) v = X[((b * Si + iy) * Si + ix) * Cin + c];
+  const int Kq = Kp >> 2, K9 = 9 * Cin, SoSo = So * So;
+  const unsigned total = (unsigned)rows * (unsigned)Kq;
+  const bool vec = (Cin & 3) == 0;
+  for (unsigned e = blockIdx.x * 256u + threadIdx.x; e < total; e += gridDim.x * 256u) {
+    const int row = (int)(e / (unsigned)Kq);
+    const int k0 = (int)(e - (unsigned)row * (unsigned)Kq) * 4;
+    const int b = row / SoSo, pix = row - b * SoSo;
+    const int oy = pix / So, ox = pix - oy * So;
+    const float* xb = X + (size_t)b * Si * Si * Cin;
+    float v[4];
+    if (vec) {
+      v[0] = v[1] = v[2] = v[3] = 0.f;
+      if (k0 < K9) {
+        const int tap = k0 / Cin, c = k0 - tap * Cin;
+        const int ky = tap / 3, kx = tap - 3 * ky;
+        const int iy = oy * stride + ky - 1, ix = ox * stride + kx - 1;
+        if (iy >= 0 && iy < Si && ix >= 0 && ix < Si) {
+          const float4 q = *reinterpret_cast<const float4*>(xb + ((size_t)iy * Si + ix) * Cin + c);
+          v[0] = q.x;
+          v[1] = q.y;
+          v[2] = q.z;
+          v[3] = q.w;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int k = k0 + i;
+        v[i] = 0.f;
+        if (k < K9) {
+          const int tap = k / Cin, c = k - tap * Cin;
+          const int ky = tap / 3, kx = tap - 3 * ky;
+          const int iy = oy * stride + ky - 1, ix = ox * stride + kx - 1;
+          if (iy >= 0 && iy < Si && ix >= 0 && ix < Si) v[i] = xb[((size_t)iy * Si + ix) * Cin + c];
+        }
+      }
     }
-    col[e] = f2bf(v);
+    uint2 w;
+    w.x = (unsigned)f2bf(v[0]) | ((unsigned)f2bf(v[1]) << 16);
+    w.y = (unsigned)f2bf(v[2]) | ((unsigned)f2bf(v[3]) << 16);
+    *reinterpret_cast<uint2*>(col + (size_t)row * Kp + k0) = w;
   }
 }
 
-// dX[b][iy][ix][c] = sum of dcol[(b, oy, ox)][(ky * 3 + kx) * Cin + c] over the taps that read it
-__global__ void k_col2im(const float* __restrict__ dcol, int ldk, int Si, int Cin, int So, int stride, long long total,
+// dX[b][iy][ix][c] = sum of dcol[(b, oy, ox)][(ky * 3 + kx) * Cin + c] over the taps that read it,
+// in tap order. One thread per 4 consecutive channels of a pixel (Cin % 4 == 0, else one channel);
+// pixels * Cin < 2^31 (checked by the caller).
+__global__ void k_col2im(const float* __restrict__ dcol, int ldk, int Si, int Cin, int So, int stride, int total,
                          float* __restrict__ dX) {
-  for (long long e = blockIdx.x * 256LL + threadIdx.x; e < total; e += (long long)gridDim.x * 256) {
-    const int c = (int)(e % Cin);
-    const long long p = e / Cin;
-    const int ix = (int)(p % Si);
-    const int iy = (int)((p / Si) % Si);
-    const long long b = p / ((long long)Si * Si);
-    float s = 0.f;
+  const int cw = (Cin & 3) == 0 ? 4 : 1;
+  const int Cq = Cin / cw, SiSi = Si * Si;
+  const unsigned n = (unsigned)(total / cw);
+  for (unsigned e = blockIdx.x * 256u + threadIdx.x; e < n; e += gridDim.x * 256u) {
+    const int p = (int)(e / (unsigned)Cq);
+    const int c = (int)(e - (unsigned)p * (unsigned)Cq) * cw;
+    const int b = p / SiSi, pix = p - b * SiSi;
+    const int iy = pix / Si, ix = pix - iy * Si;
+    float4 s = {0.f, 0.f, 0.f, 0.f};
     for (int ky = 0; ky < 3; ++ky) {
       const int ty = iy + 1 - ky;
       if (ty < 0 || ty % stride) continue;
@@ -204,10 +256,23 @@ __global__ void k_col2im(const float* __restrict__ dcol, int ldk, int Si, int Ci
         if (tx < 0 || tx % stride) continue;
         const int ox = tx / stride;
         if (ox >= So) continue;
-        s += dcol[((b * So + oy) * So + ox) * ldk + (ky * 3 + kx) * Cin + c];
+        const float* src = dcol + ((size_t)(b * So + oy) * So + ox) * ldk + (ky * 3 + kx) * Cin + c;
+        if (cw == 4) {
+          const float4 q = *reinterpret_cast<const float4*>(src);
+          s.x += q.x;
+          s.y += q.y;
+          s.z += q.z;
+          s.w += q.w;
+        } else {
+          s.x += *src;
+        }
       }
     }
-    dX[e] = s;
+    float* dst = dX + (size_t)p * Cin + c;
+    if (cw == 4)
+      *reinterpret_cast<float4*>(dst) = s;
+    else
+      *dst = s.x;
   }
 }
 
@@ -250,8 +315,8 @@ __global__ void k_relu_bwd(float* __restrict__ dA, const float* __restrict__ A, 
 }
 
 // Column sums of X[R][C] over row chunks of `chunk` rows: part[chunk index][c]. Threads split the
-// chunk's rows into 256 / Cg interleaved subsets per column group of Cg = min(C, 256) columns and
-// add the subsets in a fixed order.
+// chunk's rows into 256 / Cg interleaved subsets per column group of Cg = min(C, 256) columns, each
+// thread with four accumulators over every fourth row of its subset, added in a fixed order.
 __global__ __launch_bounds__(256) void k_colsum_part(const float* __restrict__ X, long long R, int C, long long chunk,
                                                      float* __restrict__ part) {
   __shared__ float red[256];
@@ -260,10 +325,18 @@ __global__ __launch_bounds__(256) void k_colsum_part(const float* __restrict__ X
   const long long r_lo = blockIdx.x * chunk, r_hi = min(R, r_lo + chunk);
   for (int cb = 0; cb < C; cb += Cg) {
     const int c = cb + c_off;
-    float s = 0.f;
-    if (sub < per && c < C)
-      for (long long r = r_lo + sub; r < r_hi; r += per) s += X[r * C + c];
-    red[t] = s;
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+    if (sub < per && c < C) {
+      long long r = r_lo + sub;
+      for (; r + 3 * per < r_hi; r += 4 * per) {
+        s0 += X[r * C + c];
+        s1 += X[(r + per) * C + c];
+        s2 += X[(r + 2 * per) * C + c];
+        s3 += X[(r + 3 * per) * C + c];
+      }
+      for (; r < r_hi; r += per) s0 += X[r * C + c];
+    }
+    red[t] = (s0 + s1) + (s2 + s3);
     __syncthreads();
     if (t < Cg && c < C) {
       float a = 0.f;
@@ -273,20 +346,35 @@ __global__ __launch_bounds__(256) void k_colsum_part(const float* __restrict__ X
     __syncthreads();
   }
 }
-__global__ void k_colsum_final(const float* __restrict__ part, int nparts, int C, float* __restrict__ out) {
-  const int c = blockIdx.x * 256 + threadIdx.x;
-  if (c >= C) return;
+// out[c] = sum over q of part[q][c]: one block per column, 256 interleaved subsets then a tree
+__global__ __launch_bounds__(256) void k_colsum_final(const float* __restrict__ part, int nparts, int C,
+                                                      float* __restrict__ out) {
+  __shared__ float red[256];
+  const int c = blockIdx.x, t = threadIdx.x;
   float s = 0.f;
-  for (int q = 0; q < nparts; ++q) s += part[(long long)q * C + c];
-  out[c] = s;
+  for (int q = t; q < nparts; q += 256) s += part[(long long)q * C + c];
+  red[t] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (t < k) red[t] += red[t + k];
+    __syncthreads();
+  }
+  if (t == 0) out[c] = red[0];
 }
 
-// Per-image standardisation: (x - mean) / std over the S x S pixels (FP64 sums), one block per image.
-__global__ __launch_bounds__(256) void k_standardize(float* __restrict__ X, long long npx) {
+// Per-image standardisation, (x - mean) / std over the S x S pixels with FP64 sums: part[b][p] holds
+// the sums of the p-th of STD_PARTS contiguous segments of image b (blockIdx (p, b)), then
+// k_std_apply adds them in order and normalises.
+constexpr int STD_PARTS = 64;
+__global__ __launch_bounds__(256) void k_std_part(const float* __restrict__ X, long long npx,
+                                                  double* __restrict__ part) {
   __shared__ double s1[256], s2[256];
-  float* x = X + blockIdx.x * npx;
+  const int p = blockIdx.x, b = blockIdx.y;
+  const long long seg = (npx + STD_PARTS - 1) / STD_PARTS;
+  const long long lo = p * seg, hi = min(npx, lo + seg);
+  const float* x = X + b * npx;
   double a = 0.0, q = 0.0;
-  for (long long e = threadIdx.x; e < npx; e += 256) {
+  for (long long e = lo + threadIdx.x; e < hi; e += 256) {
     const double v = x[e];
     a += v;
     q += v * v;
@@ -301,10 +389,31 @@ __global__ __launch_bounds__(256) void k_standardize(float* __restrict__ X, long
     }
     __syncthreads();
   }
-  const double mean = s1[0] / (double)npx;
-  const double var = fmax(s2[0] / (double)npx - mean * mean, 1e-30);
-  const float m = (float)mean, inv = (float)(1.0 / sqrt(var));
-  for (long long e = threadIdx.x; e < npx; e += 256) x[e] = (x[e] - m) * inv;
+  if (threadIdx.x == 0) {
+    part[((long long)b * STD_PARTS + p) * 2] = s1[0];
+    part[((long long)b * STD_PARTS + p) * 2 + 1] = s2[0];
+  }
+}
+__global__ __launch_bounds__(256) void k_std_apply(float* __restrict__ X, long long npx,
+                                                   const double* __restrict__ part) {
+  __shared__ float mi[2];
+  const int b = blockIdx.y;
+  if (threadIdx.x == 0) {
+    double a = 0.0, q = 0.0;
+    for (int p = 0; p < STD_PARTS; ++p) {
+      a += part[((long long)b * STD_PARTS + p) * 2];
+      q += part[((long long)b * STD_PARTS + p) * 2 + 1];
+    }
+    const double mean = a / (double)npx;
+    const double var = fmax(q / (double)npx - mean * mean, 1e-30);
+    mi[0] = (float)mean;
+    mi[1] = (float)(1.0 / sqrt(var));
+  }
+  __syncthreads();
+  const float m = mi[0], inv = mi[1];
+  float* x = X + b * npx;
+  for (long long e = blockIdx.x * 256LL + threadIdx.x; e < npx; e += (long long)gridDim.x * 256)
+    x[e] = (x[e] - m) * inv;
 }
 
 __device__ inline float softplus(float x) { return fmaxf(x, 0.f) + log1pf(expf(-fabsf(x))); }
@@ -397,27 +506,31 @@ struct Layer {
 
 // C[M][N] (ldc) = A[M][lda] . B[N][ldb]^T (+ bias, ReLU), split over K when the output tile grid is
 // too small to fill the GPU.
-void gemm(hipStream_t st, Buf<float>& parts, const bf16_t* A, int lda, const bf16_t* B, int ldb, float* C,
-          long long ldc, long long M, int N, int K, const float* bias, bool relu) {
-  CNN_REQUIRE(K % 16 == 0 && lda % 4 == 0 && ldb % 4 == 0, TM_ERR_INVALID, "gemm: K, lda, ldb not padded");
-  const long long gm = (M + 63) / 64;
-  const int gn = (N + 63) / 64;
+template <int WN>
+void gemm_wn(hipStream_t st, Buf<float>& parts, const bf16_t* A, int lda, const bf16_t* B, int ldb, float* C,
+             long long ldc, long long M, int N, int K, const float* bias, bool relu) {
+  constexpr int BM = 128 / WN, BN = 32 * WN;
+  const long long gm = (M + BM - 1) / BM;
+  const int gn = (N + BN - 1) / BN;
   CNN_REQUIRE(gm < (1LL << 31) && gn <= 65535, TM_ERR_INVALID, "gemm: output too large");
   const long long tiles = gm * gn;
   int nsplit = 1;
   if (tiles < 1024 && K >= 2048) {
-    nsplit = (int)std::min<long long>(std::min<long long>(K / 1024, 2048), std::max<long long>(1, 4096 / tiles));
-    const long long cap = (64LL << 20) / std::max<long long>(1, M * N);  // partials <= 256 MB
+    // about 2048 blocks, at least 2048 of K per block, partials <= 256 MB
+    nsplit = (int)std::min<long long>(K / 2048, std::max<long long>(1, 2048 / tiles));
+    const long long cap = (64LL << 20) / std::max<long long>(1, M * N);
     nsplit = (int)std::max<long long>(1, std::min<long long>(nsplit, cap));
   }
   if (nsplit <= 1) {
     dim3 g((unsigned)gm, gn, 1);
     if (bias && relu)
-      hipLaunchKernelGGL((k_gemm_nt<true, true>), g, dim3(256), 0, st, A, lda, B, ldb, C, ldc, M, N, K, K, 0LL, bias);
+      hipLaunchKernelGGL((k_gemm_nt<true, true, WN>), g, dim3(256), 0, st, A, lda, B, ldb, C, ldc, M, N, K, K, 0LL,
+                         bias);
     else if (bias)
-      hipLaunchKernelGGL((k_gemm_nt<true, false>), g, dim3(256), 0, st, A, lda, B, ldb, C, ldc, M, N, K, K, 0LL, bias);
+      hipLaunchKernelGGL((k_gemm_nt<true, false, WN>), g, dim3(256), 0, st, A, lda, B, ldb, C, ldc, M, N, K, K, 0LL,
+                         bias);
     else
-      hipLaunchKernelGGL((k_gemm_nt<false, false>), g, dim3(256), 0, st, A, lda, B, ldb, C, ldc, M, N, K, K, 0LL,
+      hipLaunchKernelGGL((k_gemm_nt<false, false, WN>), g, dim3(256), 0, st, A, lda, B, ldb, C, ldc, M, N, K, K, 0LL,
                          nullptr);
     check_launch("k_gemm_nt");
     return;
@@ -426,10 +539,10 @@ void gemm(hipStream_t st, Buf<float>& parts, const bf16_t* A, int lda, const bf1
   nsplit = (K + kchunk - 1) / kchunk;
   const long long ss = M * N;
   parts.ensure((size_t)ss * nsplit);
-  hipLaunchKernelGGL((k_gemm_nt<false, false>), dim3((unsigned)gm, gn, nsplit), dim3(256), 0, st, A, lda, B, ldb, parts.p,
-                     (long long)N, M, N, K, kchunk, ss, nullptr);
+  hipLaunchKernelGGL((k_gemm_nt<false, false, WN>), dim3((unsigned)gm, gn, nsplit), dim3(256), 0, st, A, lda, B, ldb,
+                     parts.p, (long long)N, M, N, K, kchunk, ss, nullptr);
   check_launch("k_gemm_nt split");
-  const unsigned g = grid1(M * N);
+  const unsigned g = grid1((M * N + 31) / 32, 1, 1 << 16);
   if (bias && relu)
     hipLaunchKernelGGL((k_reduce_splits<true, true>), dim3(g), dim3(256), 0, st, parts.p, nsplit, M, N, ss, C, ldc, bias);
   else if (bias)
@@ -438,6 +551,18 @@ void gemm(hipStream_t st, Buf<float>& parts, const bf16_t* A, int lda, const bf1
     hipLaunchKernelGGL((k_reduce_splits<false, false>), dim3(g), dim3(256), 0, st, parts.p, nsplit, M, N, ss, C, ldc,
                        nullptr);
   check_launch("k_reduce_splits");
+}
+
+// Block shape by operand size: 128 x 32 when N <= 32, 32 x 128 when M <= 32, else 64 x 64.
+void gemm(hipStream_t st, Buf<float>& parts, const bf16_t* A, int lda, const bf16_t* B, int ldb, float* C,
+          long long ldc, long long M, int N, int K, const float* bias, bool relu) {
+  CNN_REQUIRE(K % 16 == 0 && lda % 4 == 0 && ldb % 4 == 0, TM_ERR_INVALID, "gemm: K, lda, ldb not padded");
+  if (N <= 32)
+    gemm_wn<1>(st, parts, A, lda, B, ldb, C, ldc, M, N, K, bias, relu);
+  else if (M <= 32)
+    gemm_wn<4>(st, parts, A, lda, B, ldb, C, ldc, M, N, K, bias, relu);
+  else
+    gemm_wn<2>(st, parts, A, lda, B, ldb, C, ldc, M, N, K, bias, relu);
 }
 
 void transpose_bf16(hipStream_t st, const float* X, long long R, int C, long long ldx, bf16_t* T, long long ldt, int Cp,
@@ -456,14 +581,23 @@ void to_bf16_pad(hipStream_t st, const float* X, long long R, int C, long long l
   hipLaunchKernelGGL(k_to_bf16_pad, dim3(grid1(R * Cp)), dim3(256), 0, st, X, R, C, ldx, Y, Cp);
   check_launch("k_to_bf16_pad");
 }
+// Column sums in two fixed-order passes: about 2048 row chunks, then one block per column.
 void colsum(hipStream_t st, Buf<float>& part, const float* X, long long R, int C, float* out) {
-  const long long chunk = 4096;
+  const long long per = 256 / std::min(C, 256);
+  const long long chunk = std::max<long long>(4 * per, (R + 2047) / 2048);
   const int np = (int)((R + chunk - 1) / chunk);
   part.ensure((size_t)np * C);
   hipLaunchKernelGGL(k_colsum_part, dim3(np), dim3(256), 0, st, X, R, C, chunk, part.p);
   check_launch("k_colsum_part");
-  hipLaunchKernelGGL(k_colsum_final, dim3((C + 255) / 256), dim3(256), 0, st, part.p, np, C, out);
+  hipLaunchKernelGGL(k_colsum_final, dim3(C), dim3(256), 0, st, part.p, np, C, out);
   check_launch("k_colsum_final");
+}
+
+void im2col(hipStream_t st, const float* X, const Layer& L, long long M, bf16_t* col) {
+  CNN_REQUIRE(M * (L.Kp / 4) < (1LL << 31), TM_ERR_INVALID, "im2col: layer too large for one launch");
+  hipLaunchKernelGGL(k_im2col, dim3(grid1(M * (L.Kp / 4))), dim3(256), 0, st, X, L.si, L.cin, L.so, L.stride, (int)M,
+                     L.Kp, col);
+  check_launch("k_im2col");
 }
 
 }  // namespace
@@ -483,6 +617,7 @@ struct tm_cnn {
   Buf<float> dA0, dA1, dcol, parts, csum, dlog, cell_loss, heat_t;
   Buf<int> bin_t, cells;
   Buf<float> gathered;
+  Buf<double> std_part;
   std::vector<float> h_loss;
 
   ~tm_cnn() {
@@ -511,9 +646,7 @@ void forward_layer(tm_cnn* h, int l) {
   if (head)
     to_bf16_pad(st, h->act[l].p, M, L.cin, L.cin, h->col.p, L.Kp);
   else {
-    hipLaunchKernelGGL(k_im2col, dim3(grid1(M * L.Kp)), dim3(256), 0, st, h->act[l].p, L.si, L.cin, L.so, L.stride, M,
-                       L.Kp, h->col.p);
-    check_launch("k_im2col");
+    im2col(st, h->act[l].p, L, M, h->col.p);
   }
   to_bf16_pad(st, h->w.p + L.w_off, L.cout, L.K, L.K, h->wb.p, L.Kp);
   gemm(st, h->parts, h->col.p, L.Kp, h->wb.p, L.Kp, h->act[l + 1].p, L.cout, M, L.cout, L.Kp, h->w.p + L.b_off, !head);
@@ -684,8 +817,12 @@ int tm_cnn_forward(tm_cnn* h, const float* images, int batch, float* logits) {
     h->batch = batch;
     const long long npx = (long long)h->cfg.input_size * h->cfg.input_size;
     CNN_HIP(hipMemcpyAsync(h->act[0].p, images, batch * npx * sizeof(float), hipMemcpyHostToDevice, h->stream));
-    hipLaunchKernelGGL(k_standardize, dim3(batch), dim3(256), 0, h->stream, h->act[0].p, npx);
-    check_launch("k_standardize");
+    h->std_part.ensure((size_t)batch * STD_PARTS * 2);
+    hipLaunchKernelGGL(k_std_part, dim3(STD_PARTS, batch), dim3(256), 0, h->stream, h->act[0].p, npx, h->std_part.p);
+    check_launch("k_std_part");
+    hipLaunchKernelGGL(k_std_apply, dim3((unsigned)std::min<long long>((npx + 1023) / 1024, 1024), batch), dim3(256), 0,
+                       h->stream, h->act[0].p, npx, h->std_part.p);
+    check_launch("k_std_apply");
     for (int l = 0; l < (int)h->layers.size(); ++l) forward_layer(h, l);
     if (logits) {
       const long long cells = (long long)batch * h->G * h->G;
@@ -732,9 +869,7 @@ int tm_cnn_backward(tm_cnn* h, const float* heat_target, const int* bin_target, 
       if (head)
         to_bf16_pad(st, h->act[l].p, M, L.cin, L.cin, h->col.p, L.Kp);
       else {
-        hipLaunchKernelGGL(k_im2col, dim3(grid1(M * L.Kp)), dim3(256), 0, st, h->act[l].p, L.si, L.cin, L.so, L.stride,
-                           M, L.Kp, h->col.p);
-        check_launch("k_im2col");
+        im2col(st, h->act[l].p, L, M, h->col.p);
       }
       transpose_bf16(st, h->col.p, M, L.Kp, L.Kp, h->colT.p, Mp, L.Kp, Mp);
       // dZ^T [cout][Mp] and dW[cout][K] = dZ^T . col^T^T
@@ -751,8 +886,10 @@ int tm_cnn_backward(tm_cnn* h, const float* heat_target, const int* bin_target, 
       } else {
         gemm(st, h->parts, h->dzb.p, c16, h->wbT.p, c16, h->dcol.p, L.K, M, L.K, c16, nullptr, false);
         const long long total = (long long)B * L.si * L.si * L.cin;
-        hipLaunchKernelGGL(k_col2im, dim3(grid1(total)), dim3(256), 0, st, h->dcol.p, L.K, L.si, L.cin, L.so, L.stride,
-                           total, dnext);
+        CNN_REQUIRE(total < (1LL << 31), TM_ERR_INVALID, "col2im: layer too large for one launch");
+        const long long threads = (L.cin % 4 == 0) ? total / 4 : total;
+        hipLaunchKernelGGL(k_col2im, dim3(grid1(threads)), dim3(256), 0, st, h->dcol.p, L.K, L.si, L.cin, L.so,
+                           L.stride, (int)total, dnext);
         check_launch("k_col2im");
       }
       dcur = dnext;

@@ -1,6 +1,7 @@
 """GPU: the proposal network's kernels (include/tm_cnn.h) against the oracle (oracle/cnn_oracle.py).
 
-* bf16 MFMA product C = A . B^T: ragged M, N, K, a 601-column head shape and a split-K case, against
+* bf16 MFMA product C = A . B^T: ragged M, N, K, a 601-column head shape, each block shape
+  (64 x 64, 128 x 32 for N <= 32, 32 x 128 for M <= 32) and split-K cases of 2 and 19 splits, against
   float64 on the same bf16-rounded operands, within 1e-5 of sum |a| |b| per element
 * forward pass: every layer's activations and the logits within 1e-5 (relative L2) of the oracle
   with the device's bf16 rounding points (the remainder is FP32 accumulation order); the location
@@ -23,7 +24,8 @@ def _bf(x):
     return C.bf16(x)
 
 
-@pytest.mark.parametrize("M,N,K", [(37, 50, 23), (130, 70, 64), (200, 601, 128), (5, 3, 5000), (64, 64, 16)])
+@pytest.mark.parametrize("M,N,K", [(37, 50, 23), (130, 70, 64), (200, 601, 128), (5, 3, 5000), (64, 64, 16),
+                                   (1000, 32, 300), (16, 200, 64), (20, 300, 4100), (32, 96, 40000)])
 def test_gemm(M, N, K):
     print(f"case: gemm {M}x{N}x{K}", flush=True)
     rng = np.random.default_rng(M * N + K)
