@@ -116,7 +116,27 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const bf16_t* __restrict__ A, i
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
   const short4_t zero = {0, 0, 0, 0};
-  for (int k = k_lo; k < k_hi; k += 16) {
+  // KU k steps per iteration: every load of the group is issued before its MFMAs
+  constexpr int KU = 4;
+  int k = k_lo;
+  for (; k + 16 * KU <= k_hi; k += 16 * KU) {
+    short4_t a[KU][2], b[KU][2];
+#pragma unroll
+    for (int u = 0; u < KU; ++u)
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        a[u][i] = aok[i] ? *reinterpret_cast<const short4_t*>(ap[i] + k + 16 * u) : zero;
+        b[u][i] = bok[i] ? *reinterpret_cast<const short4_t*>(bp[i] + k + 16 * u) : zero;
+      }
+#pragma unroll
+    for (int u = 0; u < KU; ++u)
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[u][i], b[u][j], acc[i][j], 0, 0, 0);
+  }
+  for (; k < k_hi; k += 16) {
     short4_t a[2], b[2];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
@@ -143,6 +163,121 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const bf16_t* __restrict__ A, i
           if (RELU) v = fmaxf(v, 0.f);
           Cz[m * ldc + n] = v;
         }
+      }
+}
+
+// 3x3 convolution as an implicit product, for Cin % 16 == 0: C[m][n] = relu(sum_k col(m, k) B[n][k] + bias[n])
+// with col(m, k) = bf16(X[b][oy * stride + ky - 1][ox * stride + kx - 1][c]) for m = (b, oy, ox),
+// k = (ky * 3 + kx) * Cin + c (0 outside the image), read from the FP32 activations instead of a
+// materialised im2col matrix. Every 16-wide k step lies in one tap, so each lane's four values are one
+// 16-byte load. Block and wave layout as k_gemm_nt; K = 9 Cin; Si^2 Cin < 2^31 (checked by the caller).
+__device__ inline short4_t bf16x4(float4 v) {
+  return short4_t{(short)f2bf(v.x), (short)f2bf(v.y), (short)f2bf(v.z), (short)f2bf(v.w)};
+}
+template <int WN>
+__global__ __launch_bounds__(256) void k_conv_fwd(const float* __restrict__ X, int Si, int Cin, int So, int stride,
+                                                  const bf16_t* __restrict__ B, int ldb, float* __restrict__ C,
+                                                  long long ldc, long long M, int N, const float* __restrict__ bias) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long long m0 = (long long)blockIdx.x * (128 / WN) + (wave / WN) * 32;
+  const int n0 = blockIdx.y * (32 * WN) + (wave % WN) * 32;
+  const int r = lane & 15, kq = (lane >> 4) * 4;
+  const int K = 9 * Cin, SoSo = So * So;
+  const float* xp[2];
+  int y0[2], x0[2];
+  bool aok[2];
+  const bf16_t* bp[2];
+  bool bok[2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const long long m = m0 + i * 16 + r;
+    aok[i] = m < M;
+    const int mm = aok[i] ? (int)m : 0;
+    const int b = mm / SoSo, pix = mm - b * SoSo;
+    const int oy = pix / So, ox = pix - oy * So;
+    y0[i] = oy * stride - 1;
+    x0[i] = ox * stride - 1;
+    xp[i] = X + (size_t)b * Si * Si * Cin + kq;
+    const int n = n0 + i * 16 + r;
+    bok[i] = n < N;
+    bp[i] = B + (size_t)(bok[i] ? n : 0) * ldb + kq;
+  }
+  float4_t acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
+  const short4_t zero = {0, 0, 0, 0};
+  // (ky, kx, c0): the tap and first channel of k step k; KU steps per iteration, loads first (K / 16 =
+  // 9 Cin / 16 is a multiple of 9, the tail loop takes the remainder)
+  constexpr int KU = 4;
+  int ky = 0, kx = 0, c0 = 0;
+  auto advance = [&]() {
+    c0 += 16;
+    if (c0 == Cin) {
+      c0 = 0;
+      if (++kx == 3) {
+        kx = 0;
+        ++ky;
+      }
+    }
+  };
+  auto load_a = [&](int i) -> short4_t {
+    const int iy = y0[i] + ky, ix = x0[i] + kx;
+    const bool ok = aok[i] && iy >= 0 && iy < Si && ix >= 0 && ix < Si;
+    return ok ? bf16x4(*reinterpret_cast<const float4*>(xp[i] + (iy * Si + ix) * Cin + c0)) : zero;
+  };
+  int k = 0;
+  for (; k + 16 * KU <= K; k += 16 * KU) {
+    float4 araw[KU][2];
+    bool aval[KU][2];
+    short4_t b[KU][2];
+#pragma unroll
+    for (int u = 0; u < KU; ++u) {
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const int iy = y0[i] + ky, ix = x0[i] + kx;
+        aval[u][i] = aok[i] && iy >= 0 && iy < Si && ix >= 0 && ix < Si;
+        araw[u][i] = aval[u][i] ? *reinterpret_cast<const float4*>(xp[i] + (iy * Si + ix) * Cin + c0)
+                                : float4{0.f, 0.f, 0.f, 0.f};
+        b[u][i] = bok[i] ? *reinterpret_cast<const short4_t*>(bp[i] + k + 16 * u) : zero;
+      }
+      advance();
+    }
+#pragma unroll
+    for (int u = 0; u < KU; ++u) {
+      short4_t a[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) a[i] = aval[u][i] ? bf16x4(araw[u][i]) : zero;
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[u][j], acc[i][j], 0, 0, 0);
+    }
+  }
+  for (; k < K; k += 16) {
+    short4_t a[2], b[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      a[i] = load_a(i);
+      b[i] = bok[i] ? *reinterpret_cast<const short4_t*>(bp[i] + k) : zero;
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+    advance();
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const long long m = m0 + i * 16 + kq + e;
+        const int n = n0 + j * 16 + r;
+        if (m < M && n < N) C[m * ldc + n] = fmaxf(acc[i][j][e] + bias[n], 0.f);
       }
 }
 
@@ -565,6 +700,28 @@ void gemm(hipStream_t st, Buf<float>& parts, const bf16_t* A, int lda, const bf1
     gemm_wn<2>(st, parts, A, lda, B, ldb, C, ldc, M, N, K, bias, relu);
 }
 
+template <int WN>
+void conv_fwd_wn(hipStream_t st, const float* X, int Si, int Cin, int So, int stride, const bf16_t* W, int ldw, float* Y,
+                 long long M, int N, const float* bias) {
+  constexpr int BM = 128 / WN, BN = 32 * WN;
+  const long long gm = (M + BM - 1) / BM;
+  const int gn = (N + BN - 1) / BN;
+  CNN_REQUIRE(gm < (1LL << 31) && gn <= 65535, TM_ERR_INVALID, "conv: output too large");
+  hipLaunchKernelGGL((k_conv_fwd<WN>), dim3((unsigned)gm, gn), dim3(256), 0, st, X, Si, Cin, So, stride, W, ldw, Y,
+                     (long long)N, M, N, bias);
+  check_launch("k_conv_fwd");
+}
+// Forward 3x3 convolution + bias + ReLU without an im2col matrix (Cin % 16 == 0).
+void conv_fwd(hipStream_t st, const float* X, int Si, int Cin, int So, int stride, const bf16_t* W, int ldw, float* Y,
+              long long M, int N, const float* bias) {
+  CNN_REQUIRE(Cin % 16 == 0 && ldw % 4 == 0, TM_ERR_INVALID, "conv: Cin must be a multiple of 16");
+  CNN_REQUIRE((long long)Si * Si * Cin < (1LL << 31) && M < (1LL << 31), TM_ERR_INVALID, "conv: image too large");
+  if (N <= 32)
+    conv_fwd_wn<1>(st, X, Si, Cin, So, stride, W, ldw, Y, M, N, bias);
+  else
+    conv_fwd_wn<2>(st, X, Si, Cin, So, stride, W, ldw, Y, M, N, bias);
+}
+
 void transpose_bf16(hipStream_t st, const float* X, long long R, int C, long long ldx, bf16_t* T, long long ldt, int Cp,
                     long long Rp) {
   hipLaunchKernelGGL(k_transpose_bf16<float>, dim3((unsigned)((Rp + 31) / 32), (Cp + 31) / 32), dim3(256), 0, st, X, R,
@@ -643,12 +800,16 @@ void forward_layer(tm_cnn* h, int l) {
   hipStream_t st = h->stream;
   const bool head = l == (int)h->layers.size() - 1;
   const long long M = rows_of(h, L);
+  to_bf16_pad(st, h->w.p + L.w_off, L.cout, L.K, L.K, h->wb.p, L.Kp);
+  if (!head && L.cin % 16 == 0) {
+    conv_fwd(st, h->act[l].p, L.si, L.cin, L.so, L.stride, h->wb.p, L.Kp, h->act[l + 1].p, M, L.cout,
+             h->w.p + L.b_off);
+    return;
+  }
   if (head)
     to_bf16_pad(st, h->act[l].p, M, L.cin, L.cin, h->col.p, L.Kp);
-  else {
+  else
     im2col(st, h->act[l].p, L, M, h->col.p);
-  }
-  to_bf16_pad(st, h->w.p + L.w_off, L.cout, L.K, L.K, h->wb.p, L.Kp);
   gemm(st, h->parts, h->col.p, L.Kp, h->wb.p, L.Kp, h->act[l + 1].p, L.cout, M, L.cout, L.Kp, h->w.p + L.b_off, !head);
 }
 

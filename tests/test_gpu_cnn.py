@@ -3,6 +3,9 @@
 * bf16 MFMA product C = A . B^T: ragged M, N, K, a 601-column head shape, each block shape
   (64 x 64, 128 x 32 for N <= 32, 32 x 128 for M <= 32) and split-K cases of 2 and 19 splits, against
   float64 on the same bf16-rounded operands, within 1e-5 of sum |a| |b| per element
+* two networks: channels 8, 16, 16 (strides 2, 2, 1) and 16, 32, 48 (strides 2, 1, 2), so that the
+  implicit-product convolution (Cin a multiple of 16) runs with one, two and three k steps per tap
+  beside the im2col path (Cin 1 and 8)
 * forward pass: every layer's activations and the logits within 1e-5 (relative L2) of the oracle
   with the device's bf16 rounding points (the remainder is FP32 accumulation order); the location
   logits and the logits of chosen cells read back equal the full logit volume
@@ -42,8 +45,12 @@ def _rel(a, b):
     return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
 
 
-def _case():
-    spec = cnn.NetSpec(input_size=64, channels=[8, 16, 16], strides=[2, 2, 1], n_bins=20)
+CASES = {"narrow": ([8, 16, 16], [2, 2, 1]), "wide": ([16, 32, 48], [2, 1, 2])}
+
+
+def _case(kind="narrow"):
+    ch, st = CASES[kind]
+    spec = cnn.NetSpec(input_size=64, channels=list(ch), strides=list(st), n_bins=20)
     cfg = C.Config(64, spec.channels, spec.strides, spec.n_bins)
     rng = np.random.default_rng(5)
     flat = C.init_params(cfg, 7)
@@ -60,9 +67,10 @@ def _case():
     return spec, cfg, flat, images, heat, bins
 
 
-def test_forward_matches_oracle():
-    print("case: cnn forward 64^2, 3 layers, 20 bins, batch 3", flush=True)
-    spec, cfg, flat, images, *_ = _case()
+@pytest.mark.parametrize("kind", list(CASES))
+def test_forward_matches_oracle(kind):
+    print(f"case: cnn forward 64^2, 3 layers {CASES[kind]}, 20 bins, batch 3", flush=True)
+    spec, cfg, flat, images, *_ = _case(kind)
     net = cnn.ProposalNet(spec, max_batch=4, seed=None)
     try:
         net.set_params(flat)
@@ -77,15 +85,17 @@ def test_forward_matches_oracle():
         assert e <= 1e-5
         # the proposal step's reads: the location logits and the logits of chosen cells
         assert np.array_equal(net.heat(3), logits[..., 0])
-        cells = [0, 5, 63, 64 + 17, 3 * 16 * 16 - 1]
+        G = net.grid
+        cells = [0, 5, G * G - 1, G * G + 17, 3 * G * G - 1]
         assert np.array_equal(net.cell_logits(cells), logits.reshape(-1, logits.shape[-1])[cells])
     finally:
         net.close()
 
 
-def test_backward_matches_oracle():
-    print("case: cnn backward 64^2, 3 layers, 20 bins, batch 3", flush=True)
-    spec, cfg, flat, images, heat, bins = _case()
+@pytest.mark.parametrize("kind", list(CASES))
+def test_backward_matches_oracle(kind):
+    print(f"case: cnn backward 64^2, 3 layers {CASES[kind]}, 20 bins, batch 3", flush=True)
+    spec, cfg, flat, images, heat, bins = _case(kind)
     net = cnn.ProposalNet(spec, max_batch=4, seed=None)
     try:
         net.set_params(flat)
