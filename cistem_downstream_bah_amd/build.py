@@ -13,6 +13,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtm_hip.so")
 # (source, C++ standard): the refinement's descent is written as C++20 coroutines
 SOURCES = [("tm_core.hip", "c++17"), ("tm_prepare.hip", "c++17"), ("tm_refine.hip", "c++20"), ("tm_cnn.hip", "c++17")]
+# host-only sources (no device code), compiled with g++
+HOST_SOURCES = ["tm_results.cpp"]
 HEADERS = ["tm_fft.hpp", "tm_kernels.hpp", "tm_host.hpp", "tm_sizes.hpp", "tm_prep_kernels.hpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -42,6 +44,13 @@ def build(verbose: bool = False) -> str:
         obj = os.path.join(objdir, src.replace(".hip", ".o"))
         objs.append(obj)
         cmd = [HIPCC, *FLAGS, f"-std={std}", "-c", "-o", obj, os.path.join(CSRC, src)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
+    for src in HOST_SOURCES:
+        obj = os.path.join(objdir, src.replace(".cpp", ".o"))
+        objs.append(obj)
+        cmd = ["g++", "-O2", "-fPIC", "-std=c++17", "-c", "-o", obj, os.path.join(CSRC, src)]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))

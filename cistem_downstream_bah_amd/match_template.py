@@ -17,7 +17,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from . import core
+from . import core, results
 
 F32P = core.F32P
 
@@ -246,6 +246,23 @@ class MatchTemplateSearch:
             for di in range(-self.n_def_half, self.n_def_half + 1):
                 n += self.core.run_inner_loop(self.filter_for(di), si, di, progress)
         return n
+
+    def worker_record(self, particle_radius_angstroms: float = 0.0, n_ccfs: float | None = None) -> np.ndarray:
+        """This search's maps as the record a worker sends to the master (tm_result_pack,
+        match_template.cpp:1371-1531): turned back to the micrograph's orientation
+        (Rotate2DInPlaceBy90Degrees(false), image.cpp:11127-11138), cut to the valid area and padded to
+        the micrograph's size with each map's central average. remove_npix_from_edge is
+        myroundint(particle radius / pixel size) in float (match_template.cpp:494); n_ccfs defaults to
+        the CCFs this handle computed."""
+        r = self.core.results()
+        maps = {k: getattr(r, k).astype(np.float32) for k in results.MAP_NAMES}
+        if self.rotated:
+            maps = {k: np.ascontiguousarray(np.rot90(v, 1)) for k, v in maps.items()}
+        q = float(np.float32(particle_radius_angstroms) / np.float32(self.p.pixel_size))
+        remove = int(q + 0.5) if q > 0 else int(q - 0.5)
+        oy, ox = self.original_shape
+        return results.pack(maps, ox, oy, remove, r.hist, float(r.nccf if n_ccfs is None else n_ccfs),
+                            math.sqrt(float(self.W * self.H)))
 
     @property
     def total_correlation_positions(self) -> int:

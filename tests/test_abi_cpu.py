@@ -6,7 +6,7 @@ import re
 
 import numpy as np
 
-from cistem_downstream_bah_amd import cnn, core, mrc, refine, synth
+from cistem_downstream_bah_amd import cnn, core, mrc, refine, results, synth
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -51,6 +51,17 @@ def test_cnn_exports_and_config_match_header():
     Lc = cnn._lib()
     assert Lc.tm_cnn_destroy(None) == 0
     assert Lc.tm_cnn_forward(None, None, 1, None) != 0 and Lc.tm_cnn_last_error()
+
+
+def test_result_record_exports_match_header():
+    header = open(os.path.join(ROOT, "include", "tm_results.h")).read()
+    declared = set(re.findall(r"^(?:int|long long|const char\*)\s+(tm_\w+)\(", header, re.M))
+    assert declared == set(results.RESULT_EXPORTS)
+    L = ctypes.CDLL(core.LIB_PATH)
+    for name in declared:
+        assert hasattr(L, name), name
+    assert "#define TM_RESULT_META 6" in header and "#define TM_RESULT_MAPS 8" in header
+    assert (results.META, results.MAPS) == (6, 8)
 
 
 def test_refine_errors_are_returned_not_fatal():
