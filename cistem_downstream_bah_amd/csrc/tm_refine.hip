@@ -332,6 +332,173 @@ __global__ void k_map_mask(float* __restrict__ a, int mx, int x0, int y0, int w,
 }
 
 
+// One TemplateScore per workgroup with every intermediate in LDS, for t = P (no padding) and no
+// pixel-size change: the steps of the multi-kernel path (k_extract_cols<t, 8, 1>, k_rows_c2r<t, 64>,
+// k_rows_r2c<t, 64> scaled, k_cols<t, 64, -1, 0>, k_score_product<256>, k_cols<t, 64, +1, 0>,
+// k_rows_c2r<t, 64>, k_score_peak_rows<256> + k_score_peak_final) on the same per-line FFT plans
+// (16 threads per line, 16 lines at a time: a butterfly's arithmetic does not depend on which thread
+// runs it) and the same 256-thread reductions (one part each), so the results are bit-identical to
+// that path. Lines: col[i] is column i of the t x (t/2+1) spectrum,
+// row[j] row j, re[j] real row j (in col's memory).
+template <int T>
+__global__ __launch_bounds__(256) void k_score_fused(const cf* __restrict__ tpl, const float* __restrict__ mats,
+                                                     const cf* __restrict__ phase, const cf* __restrict__ tw_t,
+                                                     const cf* __restrict__ tw_h, const cf* __restrict__ win,
+                                                     const int* __restrict__ slots, const float* __restrict__ dfs,
+                                                     CtfConst kc, const float* __restrict__ cx,
+                                                     const float* __restrict__ cy, int n_curve, float scale,
+                                                     float* __restrict__ out) {
+  constexpr int NT = 256, TH = T / 2 + 1, H = T / 2, LT = 16, LINES = NT / LT;
+  constexpr int NCOL = (TH + LINES - 1) / LINES * LINES;
+  // re shares col's memory: the real rows are written only after col has been copied into row, and
+  // read back into row before col is written again
+  __shared__ cf col[NCOL][tmk::lds_len(T)];
+  __shared__ cf row[T][tmk::lds_len(H + 1)];
+  static_assert(sizeof(float) * T * (T + 2) <= sizeof(cf) * NCOL * tmk::lds_len(T), "re must fit in col");
+  float(*re)[T + 2] = reinterpret_cast<float(*)[T + 2]>(&col[0][0]);
+  __shared__ double red[NT];
+  __shared__ float rv[NT];
+  __shared__ int ri[NT];
+  const int b = blockIdx.x, tid = threadIdx.x, lt = tid % LT, ln = tid / LT;
+  const float* m = mats + (size_t)b * 9;
+  // slice and its inverse column transforms
+  for (int e = tid; e < NCOL * T; e += NT) {
+    const int c = e % NCOL, j = e / NCOL;
+    col[c][tmk::lpad(j)] = c < TH ? tmk::slice_value<1>(tpl, T, m, c, j, phase, nullptr) : cmake(0.f, 0.f);
+  }
+  __syncthreads();
+  for (int c0 = 0; c0 < NCOL; c0 += LINES) tmk::fft_inplace<T, +1, LT>(col[c0 + ln], tw_t, lt);
+  // rows from columns, then C2R of each row into re (k_rows_c2r)
+  auto cols_to_rows = [&]() {
+    __syncthreads();
+    for (int e = tid; e < T * TH; e += NT) {
+      const int j = e / TH, i = e - j * TH;
+      row[j][tmk::lpad(i)] = col[i][tmk::lpad(j)];
+    }
+    __syncthreads();
+    for (int r0 = 0; r0 < T; r0 += LINES) {
+      cf* buf = row[r0 + ln];
+      for (int k = lt; k <= H / 2; k += LT) {
+        if (k == 0) {
+          const float x0 = buf[tmk::lpad(0)].x, xh = buf[tmk::lpad(H)].x;
+          buf[tmk::lpad(0)] = cmake(x0 + xh, x0 - xh);
+        } else {
+          const int kk = H - k;
+          const cf Xk = buf[tmk::lpad(k)], Xc = cconj(buf[tmk::lpad(kk)]);
+          const cf E = tmk::cadd(Xk, Xc);
+          const cf D = tmk::csub(Xk, Xc);
+          const cf w = tw_t[k];
+          const cf O = cmul(D, cconj(w));
+          const cf Ok = cmul(cconj(D), w);
+          buf[tmk::lpad(k)] = cmake(E.x - O.y, E.y + O.x);
+          buf[tmk::lpad(kk)] = cmake(E.x - Ok.y, -E.y + Ok.x);
+        }
+      }
+      __syncthreads();
+      tmk::fft_inplace<H, +1, LT>(buf, tw_h, lt);
+      for (int q = lt; q < H; q += LT) {
+        const cf z = buf[tmk::lpad(q)];
+        re[r0 + ln][2 * q] = z.x;
+        re[r0 + ln][2 * q + 1] = z.y;
+      }
+    }
+    __syncthreads();
+  };
+  cols_to_rows();
+  // forward R2C of each row, scaled (k_rows_r2c), then the forward column transforms
+  for (int r0 = 0; r0 < T; r0 += LINES) {
+    cf* buf = row[r0 + ln];
+    for (int q = lt; q < H; q += LT) buf[tmk::lpad(q)] = cmake(re[r0 + ln][2 * q], re[r0 + ln][2 * q + 1]);
+    __syncthreads();
+    tmk::fft_inplace<H, -1, LT>(buf, tw_h, lt);
+    for (int k = lt; k <= H / 2; k += LT) {
+      if (k == 0) {
+        const cf z0 = buf[tmk::lpad(0)];
+        buf[tmk::lpad(0)] = cmake((z0.x + z0.y) * scale, 0.f);
+        buf[tmk::lpad(H)] = cmake((z0.x - z0.y) * scale, 0.f);
+      } else {
+        const int kk = H - k;
+        const cf zk = buf[tmk::lpad(k)], zc = cconj(buf[tmk::lpad(kk)]);
+        const cf E = tmk::cscale(tmk::cadd(zk, zc), 0.5f);
+        const cf D = tmk::cscale(tmk::csub(zk, zc), 0.5f);
+        const cf O = cmake(D.y, -D.x);
+        const cf wO = cmul(tw_t[k], O);
+        const cf Xk = tmk::cadd(E, wO);
+        const cf Xkk = cconj(tmk::csub(E, wO));
+        buf[tmk::lpad(k)] = tmk::cscale(Xk, scale);
+        buf[tmk::lpad(kk)] = tmk::cscale(Xkk, scale);
+      }
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < T * TH; e += NT) {
+    const int i = e / T, j = e - i * T;
+    col[i][tmk::lpad(j)] = row[j][tmk::lpad(i)];
+  }
+  __syncthreads();
+  for (int c0 = 0; c0 < NCOL; c0 += LINES) tmk::fft_inplace<T, -1, LT>(col[c0 + ln], tw_t, lt);
+  __syncthreads();
+  // filter, sum of squares and conj(projection) x window (k_score_product, one part)
+  const cf* w = win + (size_t)slots[b] * T * TH;
+  const float d1 = dfs[2 * b], d2 = dfs[2 * b + 1];
+  double sum = 0.0;
+  for (int e = tid; e < T * TH; e += NT) {
+    const int j = e / TH, i = e - j * TH;
+    const float f = filter_value(i, j, T, d1, d2, kc, cx, cy, n_curve);
+    cf v = col[i][tmk::lpad(j)];
+    v = cmake(v.x * f, v.y * f);
+    if (e == 0) v = cmake(0.0f, 0.0f);
+    const int jj = j >= T / 2 ? j - T : j;
+    const bool edge_x = (i == 0 || i == T / 2);
+    const double a = (double)hypotf(v.x, v.y);
+    const double a2 = a * a;
+    if (edge_x && (jj == 0 || jj == -T / 2)) sum += a2 * 0.5;
+    else if (!edge_x || jj >= 0) sum += a2;
+    col[i][tmk::lpad(j)] = cmul(cconj(v), w[e]);
+  }
+  red[tid] = sum;
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (tid < s) red[tid] += red[tid + s];
+    __syncthreads();
+  }
+  // inverse column transforms, rows, C2R
+  for (int c0 = 0; c0 < NCOL; c0 += LINES) tmk::fft_inplace<T, +1, LT>(col[c0 + ln], tw_t, lt);
+  cols_to_rows();
+  // first maximum in scan order (k_score_peak_rows, one part) and the result (k_score_peak_final)
+  float best = -FLT_MAX;
+  int bi = 0x7fffffff;
+  for (int e = tid; e < T * T; e += NT) {
+    const int j = e / T, i = e - j * T;
+    const float v = re[j][i];
+    if (v > best) { best = v; bi = e; }
+  }
+  rv[tid] = best;
+  ri[tid] = bi;
+  __syncthreads();
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (tid < s) {
+      const float v = rv[tid + s];
+      const int k = ri[tid + s];
+      if (v > rv[tid] || (v == rv[tid] && k < ri[tid])) { rv[tid] = v; ri[tid] = k; }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    float pbest = -FLT_MAX;
+    int pbi = 0x7fffffff;
+    if (rv[0] > pbest || (rv[0] == pbest && ri[0] < pbi)) { pbest = rv[0]; pbi = ri[0]; }
+    double ss = 0.0;
+    ss += red[0];
+    const float sos = (float)(2.0 * ss);
+    const float inv = (float)(1. / sqrtf(sos));
+    const int e = pbi == 0x7fffffff ? 0 : pbi;
+    out[3 * b] = (float)(e % T - T / 2);
+    out[3 * b + 1] = (float)(e / T - T / 2);
+    out[3 * b + 2] = pbi == 0x7fffffff ? -FLT_MAX : pbest * inv;
+  }
+}
+
 }  // namespace rtk
 
 namespace {
@@ -645,6 +812,7 @@ struct tm_refine {
   PinnedBuf<int> h_slots;
   PinnedBuf<long long> h_shifts, h_arg_i;
   int score_parts = 1, peak_parts = 1;
+  bool fused = true;  // one-workgroup TemplateScore (k_score_fused) where it applies
   std::map<int, cf*> tw;
   // pixel-size resampling of projections (allocated at the first item that needs it)
   DevBuf<float> rs_pad, rs_big, rs_clip;
@@ -660,7 +828,7 @@ struct tm_refine {
   std::map<std::pair<int, int>, std::unique_ptr<BluesteinTables>> bluestein;
   // timing: event pairs around each stage of a TemplateScore batch, resolved after the batch's
   // synchronisation
-  enum { S_EXTRACT, S_SLICE_C2R, S_PROJ_R2C, S_PRODUCT, S_CCF_C2R, S_PEAK, S_COUNT };
+  enum { S_EXTRACT, S_SLICE_C2R, S_PROJ_R2C, S_PRODUCT, S_CCF_C2R, S_PEAK, S_FUSED, S_COUNT };
   bool timing = false;
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
@@ -709,7 +877,8 @@ struct tm_refine {
 namespace {
 
 const char* kStageNames[tm_refine::S_COUNT] = {"extract_slice_cols", "slice_rows_c2r", "projection_r2c",
-                                               "score_product",      "correlation_c2r", "score_peak"};
+                                               "score_product",      "correlation_c2r", "score_peak",
+                                               "score_fused"};
 
 // Records an event pair around one stage on the handle's stream when timing is on.
 struct Stage {
@@ -1039,6 +1208,25 @@ void run_scores(tm_refine* h, const std::vector<ScoreRequest>& reqs_in, const st
   const size_t proj_bs = (size_t)t * th;
   const float scale = (float)(1. / (double)(float)((long long)t * t));
   using R = tm_refine;
+  if (h->fused && P == t && t == 64 && n0 == n) {
+    {
+      Stage st(h, R::S_FUSED);
+      hipLaunchKernelGGL((rtk::k_score_fused<64>), dim3(n), dim3(256), 0, h->stream, h->tpl.p, h->mats.p, h->phaseT.p,
+                         h->twiddle(t), h->twiddle(t / 2), h->win.p, h->slots.p, h->dfs.p, h->ctf, h->curve_x.p,
+                         h->curve_y.p, h->n_curve, scale, h->results.p);
+      check_launch("k_score_fused");
+    }
+    TM_HIP(hipMemcpyAsync(h->h_results.p, h->results.p, 3 * n * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+    TM_HIP(hipStreamSynchronize(h->stream));
+    if (h->timing) {
+      h->batch_items = n;
+      h->resolve_timing();
+    }
+    out.resize(n);
+    for (int q = 0; q < n; ++q)
+      out[ord[q]] = ScoreResult{h->h_results.p[3 * q], h->h_results.p[3 * q + 1], h->h_results.p[3 * q + 2]};
+    return;
+  }
   if (P != t) {
     // ExtractSlice, SwapRealSpaceQuadrants, BackwardFFT (columns, then the t centre rows that
     // ClipInto keeps, or every row for a resampled item), ClipInto t^2 + ForwardFFT (rows, scaled
@@ -1491,6 +1679,13 @@ int tm_refine_score_batch(tm_refine* h, int n, const int* peak_xy, const float* 
         }
       flush();  // the next group of windows reuses the slots
     }
+  });
+}
+
+int tm_refine_set_fused(tm_refine* h, int enabled) {
+  return rguard([&] {
+    TM_REQUIRE(h, TM_ERR_INVALID, "handle is NULL");
+    h->fused = enabled != 0;
   });
 }
 

@@ -23,7 +23,7 @@ MAP_NAMES = ("mip", "scaled_mip", "psi", "theta", "phi", "defocus", "pixel_size"
 OUTPUT_NAMES = ("best_mip", "best_scaled_mip", "best_psi", "best_theta", "best_phi", "best_defocus",
                 "best_pixel_size")
 REFINE_EXPORTS = ["tm_prepare_refine_image", "tm_refine_create", "tm_refine_destroy", "tm_refine_init",
-                  "tm_refine_run", "tm_refine_peak_list", "tm_refine_score", "tm_refine_set_timing",
+                  "tm_refine_run", "tm_refine_peak_list", "tm_refine_score", "tm_refine_set_timing", "tm_refine_set_fused",
                   "tm_refine_get_timing", "tm_refine_last_error", "tm_refine_score_batch", "tm_refine_set_image",
                   "tm_refine_set_micrograph", "tm_refine_get_image"]
 
@@ -70,6 +70,7 @@ def _lib():
         L.tm_refine_set_micrograph.argtypes = [vp, F32P]
         L.tm_refine_get_image.argtypes = [vp, F32P, F32P, F32P]
         L.tm_refine_set_timing.argtypes = [vp, ctypes.c_int]
+        L.tm_refine_set_fused.argtypes = [vp, ctypes.c_int]
         i64p = ctypes.POINTER(ctypes.c_longlong)
         L.tm_refine_get_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_double), i64p,
                                            i64p, ctypes.c_int]
@@ -218,6 +219,10 @@ class TemplateRefiner:
 
     def set_timing(self, on: bool) -> None:
         _check(_lib().tm_refine_set_timing(self.h, int(on)))
+
+    def set_fused(self, on: bool) -> None:
+        """One-kernel TemplateScore batches for t = 64 (tm_refine_set_fused; on by default)."""
+        _check(_lib().tm_refine_set_fused(self.h, int(on)))
 
     def timing(self) -> dict:
         """{stage: (total ms, launches, items)} over the last run."""

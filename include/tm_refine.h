@@ -127,8 +127,13 @@ int tm_refine_score(tm_refine* h, int peak_x, int peak_y, float current_defocus,
 int tm_refine_score_batch(tm_refine* h, int n, const int* peak_xy, const float* angles, float current_defocus,
                           float* out);
 
+/* For t = 64 with P = t and no pixel-size change, a TemplateScore batch runs as one kernel with one
+ * workgroup per score and every intermediate in LDS (on by default); its results are bit-identical
+ * to the multi-kernel path, which enabled = 0 selects. No counterpart in the reference (a schedule). */
+int tm_refine_set_fused(tm_refine* h, int enabled);
+
 /* Per stage of the TemplateScore batches of the last tm_refine_run (extract_slice_cols,
- * slice_rows_c2r, projection_r2c, score_product, correlation_c2r, score_peak), measured with HIP
+ * slice_rows_c2r, projection_r2c, score_product, correlation_c2r, score_peak, score_fused), measured with HIP
  * events on the handle's stream when timing is enabled: total ms, number of launches and items
  * (TemplateScore calls) summed over the launches. Returns the number of entries written (at most cap).
  * No counterpart in the reference (diagnostics). */

@@ -3,6 +3,8 @@
 * tm_refine_score_batch equals tm_refine_score request by request (bit-identical), with more
   windows and requests than one batch holds; after tm_refine_set_image a handle scores exactly as a
   handle made for that micrograph
+* the one-kernel TemplateScore (k_score_fused, t = 64) equals the multi-kernel path bit for bit over
+  300 requests in two windows, and a single score equals its batched twin
 * with the exhaustive search's own peaks as proposals and every orientation bin, the local
   refinement finds every peak (recall 1 within 2 pixels) and mostly its orientation: the window
   correlation and the search's full-image correlation agree on position
@@ -40,6 +42,27 @@ def test_score_batch_equals_single_scores(study):
         for k in range(15):
             ref = r.score(xy[k][0], xy[k][1], *ang[k])
             assert tuple(got[k]) == tuple(np.float32(v) for v in ref), (k, got[k], ref)
+    finally:
+        r.close()
+
+
+def test_fused_scores_equal_the_multi_kernel_path(study):
+    print("case: k_score_fused vs the multi-kernel TemplateScore, 2 windows x 300 orientations", flush=True)
+    m = study.micrograph(4)
+    r = TemplateRefiner(m, study.volume, RefineSettings(padding=1.0, max_in_flight=256),
+                        template_spectrum=study.tpl_spectrum)
+    try:
+        rng = np.random.default_rng(2)
+        xy = np.array([(-37, 88), (140, -3)] * 150)
+        ang = study.orients[rng.integers(len(study.orients), size=300)]
+        fused = r.score_batch(xy, ang)
+        r.set_fused(False)
+        plain = r.score_batch(xy, ang)
+        r.set_fused(True)
+        assert np.array_equal(fused.view(np.uint32), plain.view(np.uint32))
+        # a single score (one item in the batch) through both paths
+        one = r.score(-37, 88, *ang[0])
+        assert tuple(np.float32(v) for v in one) == tuple(fused[0])
     finally:
         r.close()
 
