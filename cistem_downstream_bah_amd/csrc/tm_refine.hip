@@ -838,9 +838,13 @@ struct tm_refine {
   long long stage_items[S_COUNT] = {};
   int batch_items = 0;
 
+  hipEvent_t batch_done[2] = {nullptr, nullptr};  // the fused path's two batch halves
+
   ~tm_refine() {
     for (auto& kv : tw) (void)hipFree(kv.second);
     for (auto& e : ev_pool) (void)hipEventDestroy(e);
+    for (auto& e : batch_done)
+      if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
   }
   hipEvent_t next_event() {
@@ -1173,6 +1177,50 @@ void resample_projections(tm_refine* h, const float* src, size_t sstride, int a,
 // One batch of TemplateScore calls: item b correlates the projection at reqs[b] with window slots[b].
 // Items whose pixel-size factor is not within the tolerance of 1 go last, ordered by their resampled
 // edge, and take the resampling path after the slice's C2R.
+// The fused path split in two so that one batch's host work overlaps the previous batch on the
+// device: launch_fused fills half `half` of the pinned and device request buffers, enqueues the
+// copies, k_score_fused and the copy back, and records batch_done[half]; finish_fused waits for that
+// event and returns the results. Requests keep their order (no pixel-size change to group by).
+bool fused_applies(const tm_refine* h) { return h->fused && !h->timing && h->P == h->t && h->t == 64; }
+
+void launch_fused(tm_refine* h, const std::vector<ScoreRequest>& reqs, const std::vector<int>& slot_in, int half) {
+  const int n = (int)reqs.size(), t = h->t, K = h->K;
+  TM_REQUIRE(n <= K, TM_ERR_INVALID, "batch larger than max_in_flight");
+  float* hm = h->h_mats.p + (size_t)half * 9 * K;
+  float* hd = h->h_dfs.p + (size_t)half * 2 * K;
+  int* hs = h->h_slots.p + (size_t)half * K;
+  for (int q = 0; q < n; ++q) {
+    const ScoreRequest& r = reqs[q];
+    euler_matrix(r.phi, r.theta, r.psi, &hm[9 * q]);
+    hd[2 * q] = r.df1;
+    hd[2 * q + 1] = r.df2;
+    hs[q] = slot_in[q];
+  }
+  float* dm = h->mats.p + (size_t)half * 9 * K;
+  float* dd = h->dfs.p + (size_t)half * 2 * K;
+  int* ds = h->slots.p + (size_t)half * K;
+  float* dr = h->results.p + (size_t)half * 3 * K;
+  TM_HIP(hipMemcpyAsync(dm, hm, 9 * n * sizeof(float), hipMemcpyHostToDevice, h->stream));
+  TM_HIP(hipMemcpyAsync(dd, hd, 2 * n * sizeof(float), hipMemcpyHostToDevice, h->stream));
+  TM_HIP(hipMemcpyAsync(ds, hs, n * sizeof(int), hipMemcpyHostToDevice, h->stream));
+  const float scale = (float)(1. / (double)(float)((long long)t * t));
+  hipLaunchKernelGGL((rtk::k_score_fused<64>), dim3(n), dim3(256), 0, h->stream, h->tpl.p, dm, h->phaseT.p,
+                     h->twiddle(t), h->twiddle(t / 2), h->win.p, ds, dd, h->ctf, h->curve_x.p, h->curve_y.p,
+                     h->n_curve, scale, dr);
+  check_launch("k_score_fused");
+  TM_HIP(hipMemcpyAsync(h->h_results.p + (size_t)half * 3 * K, dr, 3 * n * sizeof(float), hipMemcpyDeviceToHost,
+                        h->stream));
+  if (!h->batch_done[half]) TM_HIP(hipEventCreateWithFlags(&h->batch_done[half], hipEventDisableTiming));
+  TM_HIP(hipEventRecord(h->batch_done[half], h->stream));
+}
+
+void finish_fused(tm_refine* h, int half, int n, std::vector<ScoreResult>& out) {
+  TM_HIP(hipEventSynchronize(h->batch_done[half]));
+  const float* r = h->h_results.p + (size_t)half * 3 * h->K;
+  out.resize(n);
+  for (int q = 0; q < n; ++q) out[q] = ScoreResult{r[3 * q], r[3 * q + 1], r[3 * q + 2]};
+}
+
 void run_scores(tm_refine* h, const std::vector<ScoreRequest>& reqs_in, const std::vector<int>& slot_in,
                 std::vector<ScoreResult>& out) {
   const int n = (int)reqs_in.size();
@@ -1455,10 +1503,10 @@ int tm_refine_init(tm_refine* h, const tm_refine_params* p, const float* whitene
     const int K = h->K;
     const size_t per = (size_t)t * h->th;
     h->score_parts = (int)std::max<size_t>(1, std::min<size_t>(32, per / 2048));
-    h->mats.alloc(9 * (size_t)K);
-    h->dfs.alloc(2 * (size_t)K);
-    h->results.alloc(3 * (size_t)K);
-    h->slots.alloc(K);
+    h->mats.alloc(2 * 9 * (size_t)K);  // two halves: the fused path overlaps batches
+    h->dfs.alloc(2 * 2 * (size_t)K);
+    h->results.alloc(2 * 3 * (size_t)K);
+    h->slots.alloc(2 * (size_t)K);
     h->shifts.alloc(K);
     h->partial.alloc((size_t)K * h->score_parts);
     h->peak_parts = std::max(1, std::min(t, t * t / 4096));
@@ -1487,10 +1535,10 @@ int tm_refine_init(tm_refine* h, const tm_refine_params* p, const float* whitene
     }
     h->arg_v.alloc(1024 + 1);
     h->arg_i.alloc(1024 + 1);
-    h->h_mats.alloc(9 * (size_t)K);
-    h->h_dfs.alloc(2 * (size_t)K);
-    h->h_results.alloc(3 * (size_t)K);
-    h->h_slots.alloc(K);
+    h->h_mats.alloc(2 * 9 * (size_t)K);
+    h->h_dfs.alloc(2 * 2 * (size_t)K);
+    h->h_results.alloc(2 * 3 * (size_t)K);
+    h->h_slots.alloc(2 * (size_t)K);
     h->h_shifts.alloc(K);
     h->h_arg_v.alloc(1);
     h->h_arg_i.alloc(1);
@@ -1649,13 +1697,34 @@ int tm_refine_score_batch(tm_refine* h, int n, const int* peak_xy, const float* 
     std::vector<ScoreRequest> reqs;
     std::vector<int> slots, which;
     std::vector<ScoreResult> res;
+    auto store = [&](const std::vector<int>& who) {
+      for (size_t q = 0; q < who.size(); ++q) {
+        out[3 * who[q]] = res[q].x;
+        out[3 * who[q] + 1] = res[q].y;
+        out[3 * who[q] + 2] = res[q].value;
+      }
+    };
+    // fused path: batch k is prepared and enqueued while batch k - 1 runs, then k - 1 is collected
+    const bool pipelined = fused_applies(h);
+    int half = 0, pending_half = -1;
+    std::vector<int> pending_which;
+    auto drain = [&] {
+      if (pending_half < 0) return;
+      finish_fused(h, pending_half, (int)pending_which.size(), res);
+      store(pending_which);
+      pending_half = -1;
+    };
     auto flush = [&] {
       if (reqs.empty()) return;
-      run_scores(h, reqs, slots, res);
-      for (size_t q = 0; q < reqs.size(); ++q) {
-        out[3 * which[q]] = res[q].x;
-        out[3 * which[q] + 1] = res[q].y;
-        out[3 * which[q] + 2] = res[q].value;
+      if (pipelined) {
+        launch_fused(h, reqs, slots, half);
+        drain();
+        pending_half = half;
+        pending_which.swap(which);
+        half ^= 1;
+      } else {
+        run_scores(h, reqs, slots, res);
+        store(which);
       }
       reqs.clear();
       slots.clear();
@@ -1669,6 +1738,7 @@ int tm_refine_score_batch(tm_refine* h, int n, const int* peak_xy, const float* 
         ys.push_back(wins[w].second);
         ss.push_back((int)(w - w0));
       }
+      drain();  // make_windows reuses the pinned slot buffer and the window slots
       make_windows(h, xs, ys, ss);
       for (size_t w = w0; w < w1; ++w)
         for (int i : reqs_of[w]) {
@@ -1679,6 +1749,7 @@ int tm_refine_score_batch(tm_refine* h, int n, const int* peak_xy, const float* 
         }
       flush();  // the next group of windows reuses the slots
     }
+    drain();
   });
 }
 
