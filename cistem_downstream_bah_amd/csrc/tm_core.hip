@@ -651,7 +651,7 @@ int tm_set_micrograph(tm_core* h, const float* micrograph, float* curve_x, float
     hipLaunchKernelGGL(tmp::k_power_curve, dim3(gs), dim3(256), 0, st, h->pr_spec.p, W, H, h->pr_cx.p, n, h->pr_acc.p,
                        h->pr_acc.p + n);
     check_launch("k_power_curve");
-    hipLaunchKernelGGL(tmp::k_curve_finish, dim3(1), dim3(1), 0, st, h->pr_acc.p, h->pr_acc.p + n, n, h->pr_cy.p);
+    hipLaunchKernelGGL(tmp::k_curve_finish, dim3(1), dim3(256), 0, st, h->pr_acc.p, h->pr_acc.p + n, n, h->pr_cy.p);
     check_launch("k_curve_finish");
     // ApplyCurveFilter, ZeroCentralPixel, 1 / sqrt(ReturnSumOfSquares()) (match_template.cpp:717-724)
     hipLaunchKernelGGL(tmp::k_apply_curve, dim3(gs), dim3(256), 0, st, h->pr_spec.p, W, H, h->pr_cx.p, h->pr_cy.p, n);

@@ -122,11 +122,14 @@ __global__ void k_power_curve(const cf* __restrict__ c, int W, int H, const floa
   }
 }
 
-// whitening filter: mean power per bin, SquareRoot, Reciprocal, scaled to a maximum of 1 (one thread)
-__global__ void k_curve_finish(const double* __restrict__ acc_w, const double* __restrict__ acc_n, int n,
-                               float* __restrict__ cy) {
+// whitening filter: mean power per bin, SquareRoot, Reciprocal, scaled to a maximum of 1. One block
+// of 256 threads: each bin's value is independent, and the maximum is exact in any order (fmaxf
+// skips NaN as the sequential loop does), so the result equals a one-thread loop bit for bit.
+__global__ __launch_bounds__(256) void k_curve_finish(const double* __restrict__ acc_w, const double* __restrict__ acc_n,
+                                                      int n, float* __restrict__ cy) {
+  __shared__ float red[256];
   float mx = -FLT_MAX;
-  for (int q = 0; q < n; ++q) {
+  for (int q = threadIdx.x; q < n; q += 256) {
     const float w = (float)acc_w[q], c = (float)acc_n[q];
     float a = c > 0.0f ? w / c : 0.0f;
     a = sqrtf(a);
@@ -134,8 +137,14 @@ __global__ void k_curve_finish(const double* __restrict__ acc_w, const double* _
     cy[q] = a;
     mx = fmaxf(mx, a);
   }
-  const float k = 1.0f / mx;
-  for (int q = 0; q < n; ++q) cy[q] *= k;
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  const float k = 1.0f / red[0];
+  for (int q = threadIdx.x; q < n; q += 256) cy[q] *= k;
 }
 
 // Curve::ReturnLinearInterpolationFromX

@@ -1606,7 +1606,7 @@ int tm_refine_set_micrograph(tm_refine* h, const float* micrograph) {
     hipLaunchKernelGGL(tmp::k_power_curve, dim3(gs), dim3(256), 0, st, h->pr_spec.p, mx, my, h->curve_x.p, n,
                        h->pr_acc.p, h->pr_acc.p + n);
     check_launch("k_power_curve");
-    hipLaunchKernelGGL(tmp::k_curve_finish, dim3(1), dim3(1), 0, st, h->pr_acc.p, h->pr_acc.p + n, n, h->curve_y.p);
+    hipLaunchKernelGGL(tmp::k_curve_finish, dim3(1), dim3(256), 0, st, h->pr_acc.p, h->pr_acc.p + n, n, h->curve_y.p);
     check_launch("k_curve_finish");
     // ApplyCurveFilter, ZeroCentralPixel, DivideByConstant(sqrt(ReturnSumOfSquares()))
     hipLaunchKernelGGL(tmp::k_apply_curve, dim3(gs), dim3(256), 0, st, h->pr_spec.p, mx, my, h->curve_x.p,
