@@ -128,8 +128,10 @@ int tm_refine_score_batch(tm_refine* h, int n, const int* peak_xy, const float* 
                           float* out);
 
 /* For t = 64 with P = t and no pixel-size change, a TemplateScore batch runs as one kernel with one
- * workgroup per score and every intermediate in LDS (on by default); its results are bit-identical
- * to the multi-kernel path, which enabled = 0 selects. No counterpart in the reference (a schedule). */
+ * workgroup per score and every intermediate in LDS (on by default), and the batches of
+ * tm_refine_score_batch overlap (the next batch's requests are prepared on the host while the device
+ * runs the current one); results are bit-identical to the multi-kernel path, which enabled = 0
+ * selects. No counterpart in the reference (a schedule). */
 int tm_refine_set_fused(tm_refine* h, int enabled);
 
 /* Per stage of the TemplateScore batches of the last tm_refine_run (extract_slice_cols,
