@@ -22,6 +22,7 @@
 
 #include "../../include/template_matching_core.hpp"
 #include "../../include/tm_prepare.h"
+#include "../../include/tm_results.h"
 
 namespace {
 
@@ -66,22 +67,33 @@ void write_mrc(const std::string& path, const float* data, int nx, int ny, float
   f.write(reinterpret_cast<const char*>(data), (size_t)nx * ny * sizeof(float));
 }
 
-// Padded-layout W x H map -> dense, un-rotated (Rotate2DInPlaceBy90Degrees(false),
-// image.cpp:11127-11138) and cropped/padded to the original size with a fill value
-// (Image::Resize -> ClipInto, image.cpp:8522-8575, 8825-8838).
-std::vector<float> to_original(const float* padded, int W, int H, bool rotated, int ox, int oy, bool edge_fill,
-                               float fill) {
+// Padded-layout W x H map -> dense and un-rotated (Rotate2DInPlaceBy90Degrees(false),
+// image.cpp:11127-11138); nx, ny receive the dense size.
+template <class T>
+std::vector<float> dense_unrotated(const T* padded, int W, int H, bool rotated, int* nx, int* ny) {
   std::vector<float> d((size_t)W * H);
-  for (int y = 0; y < H; ++y) std::memcpy(&d[(size_t)y * W], padded + (size_t)y * (W + 2), W * sizeof(float));
-  int nx = W, ny = H;
+  for (int y = 0; y < H; ++y)
+    for (int x = 0; x < W; ++x) d[(size_t)y * W + x] = (float)padded[(size_t)y * (W + 2) + x];
+  *nx = W;
+  *ny = H;
   if (rotated) {
     std::vector<float> r((size_t)W * H);
-    const int rx = H, ry = W;  // rotated image is H x W
+    const int rx = H;  // rotated image is H x W
     for (int yo = 0; yo < H; ++yo)
       for (int xo = 0; xo < W; ++xo) r[(size_t)(W - xo - 1) * rx + yo] = d[(size_t)yo * W + xo];
     d.swap(r);
-    nx = rx; ny = ry;
+    *nx = H;
+    *ny = W;
   }
+  return d;
+}
+
+// Padded-layout W x H map -> dense, un-rotated and cropped/padded to the original size with a fill
+// value (Image::Resize -> ClipInto, image.cpp:8522-8575, 8825-8838).
+std::vector<float> to_original(const float* padded, int W, int H, bool rotated, int ox, int oy, bool edge_fill,
+                               float fill) {
+  int nx, ny;
+  std::vector<float> d = dense_unrotated(padded, W, H, rotated, &nx, &ny);
   if (edge_fill) {
     double s = 0; long n = 0;
     for (int x = 0; x < nx; ++x) { s += d[x]; s += d[(size_t)(ny - 1) * nx + x]; n += 2; }
@@ -116,7 +128,7 @@ void usage() {
                "  [--cs 2.7] [--amplitude-contrast 0.07] [--defocus1 A] [--defocus2 A] [--defocus-angle 0]\n"
                "  [--phase-shift 0] [--angular-step 10] [--in-plane-step 10] [--defocus-range 0] [--defocus-step 0]\n"
                "  [--pixel-size-range 0] [--pixel-size-step 0] [--padding 1] [--symmetry C1] [--min-peak-radius 10]\n"
-               "  [--device 0] [--batch 0] [--graph 0|1]\n");
+               "  [--device 0] [--batch 0] [--graph 0|1] [--particle-radius 0] [--worker-record FILE]\n");
 }
 
 }  // namespace
@@ -200,6 +212,35 @@ int main(int argc, char** argv) {
     core.Finalize(total, mips.data(), z.data(), avg.data(), sd.data());
     const int ox = img.nx, oy = img.ny;
     const bool r90 = rot != 0;
+    if (a.count("worker-record")) {
+      // the record a worker sends its master (match_template.cpp:1371-1531): the 8 maps in the
+      // micrograph's orientation, remove_npix_from_edge = myroundint(particle radius / pixel size)
+      // (match_template.cpp:494), the CCFs this run computed, sqrt(W H)
+      std::vector<double> s1(np), s2(np);
+      std::vector<long> hist_unused(512, 0);
+      core.CopyResults(mip.data(), psi.data(), theta.data(), phi.data(), defo.data(), pix.data(), s1.data(), s2.data(),
+                       hist_unused.data());
+      int nx = 0, ny = 0;
+      std::vector<std::vector<float>> maps;
+      maps.push_back(dense_unrotated(mip.data(), W, H, r90, &nx, &ny));
+      for (const std::vector<float>* m : {&psi, &theta, &phi, &defo, &pix})
+        maps.push_back(dense_unrotated(m->data(), W, H, r90, &nx, &ny));
+      maps.push_back(dense_unrotated(s1.data(), W, H, r90, &nx, &ny));
+      maps.push_back(dense_unrotated(s2.data(), W, H, r90, &nx, &ny));
+      const float* ptrs[TM_RESULT_MAPS];
+      for (int k = 0; k < TM_RESULT_MAPS; ++k) ptrs[k] = maps[k].data();
+      const float q = (float)f("particle-radius", 0.0) / pixel_size;
+      const int remove = q > 0 ? (int)(q + 0.5f) : (int)(q - 0.5f);
+      std::vector<long long> h64(hist.begin(), hist.end());
+      const long long nf = tm_result_size(ox, oy, 512);
+      std::vector<float> rec((size_t)nf);
+      if (tm_result_pack(ptrs, nx, ny, ox, oy, remove, h64.data(), 512, (float)core.total_number_of_cccs_calculated,
+                         std::sqrt((double)W * (double)H), rec.data(), nf) != TM_OK)
+        throw std::runtime_error(tm_result_last_error());
+      std::ofstream wf(a["worker-record"], std::ios::binary);
+      wf.write(reinterpret_cast<const char*>(rec.data()), (std::streamsize)(rec.size() * sizeof(float)));
+      std::fprintf(stderr, "worker record: %lld floats -> %s\n", nf, a["worker-record"].c_str());
+    }
     auto mip_o = to_original(mips.data(), W, H, r90, ox, oy, true, 0.f);
     auto z_o = to_original(z.data(), W, H, r90, ox, oy, true, 0.f);
     auto avg_o = to_original(avg.data(), W, H, r90, ox, oy, true, 0.f);

@@ -1,7 +1,8 @@
 """GPU: the C++ match_template driver (cistem_downstream_bah_amd/match_template_hip) end to end on the
 config-0 inputs written as MRC files: its maps equal the C-ABI path's, its peak list (found on the
-device) equals the host peak rule applied to its own output maps, and it contains every inserted
-particle above the threshold."""
+device) equals the host peak rule applied to its own output maps, it contains every inserted
+particle above the threshold, and its worker record (--worker-record, include/tm_results.h) agrees
+with the Python mirror's record of the same search."""
 import os
 import subprocess
 
@@ -9,7 +10,7 @@ import numpy as np
 import pytest
 
 from cistem_downstream_bah_amd import match_template as MT
-from cistem_downstream_bah_amd import mrc, peaks as PK, synth
+from cistem_downstream_bah_amd import mrc, peaks as PK, results as R, synth
 
 pytestmark = pytest.mark.gpu
 DRIVER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "cistem_downstream_bah_amd",
@@ -23,7 +24,8 @@ def test_driver_end_to_end(tmp_path):
     mrc.write_mrc(vol, s.volume)
     out = str(tmp_path / "run")
     r = subprocess.run([DRIVER, "--image", img, "--template", vol, "--out", out, "--angular-step", "10",
-                        "--in-plane-step", "10", "--min-peak-radius", "10"], capture_output=True, text=True,
+                        "--in-plane-step", "10", "--min-peak-radius", "10", "--particle-radius", "16",
+                        "--worker-record", out + "_record.f32"], capture_output=True, text=True,
                        timeout=240)
     print(r.stderr, flush=True)
     assert r.returncode == 0
@@ -56,3 +58,11 @@ def test_driver_end_to_end(tmp_path):
     assert detectable >= 8
     hist = np.loadtxt(out + "_histogram.txt")
     assert hist.shape == (512, 4) and hist[:, 1].sum() == n * 512 * 512
+    # the driver's worker record (--worker-record) against the Python mirror's for the same search
+    u = R.unpack(np.fromfile(out + "_record.f32", np.float32))
+    py = R.unpack(search.worker_record(16.0))
+    assert (u["x"], u["y"], u["n"], u["bins"], u["n_ccfs"]) == (py["x"], py["y"], py["n"], py["bins"], py["n_ccfs"])
+    assert u["n_ccfs"] == n and u["sqrt_input_pixels"] == np.float32(512.0)
+    assert np.abs(u["mip"] - py["mip"]).max() <= 1e-5 * np.abs(py["mip"]).max()
+    assert (u["psi"] == py["psi"]).mean() > 0.99 and (u["theta"] == py["theta"]).mean() > 0.99
+    assert u["histogram"].sum() == py["histogram"].sum() == n * 512 * 512
