@@ -1730,6 +1730,14 @@ int tm_refine_score_batch(tm_refine* h, int n, const int* peak_xy, const float* 
       slots.clear();
       which.clear();
     };
+    // on an error with a batch in flight, let its copies finish before the pinned buffers are reused
+    struct Settle {
+      tm_refine* h;
+      const int* pending;
+      ~Settle() {
+        if (*pending >= 0) (void)hipStreamSynchronize(h->stream);
+      }
+    } settle{h, &pending_half};
     for (size_t w0 = 0; w0 < wins.size(); w0 += h->K) {
       const size_t w1 = std::min(wins.size(), w0 + (size_t)h->K);
       std::vector<int> xs, ys, ss;
