@@ -281,6 +281,92 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const float* __restrict__ X, i
       }
 }
 
+// Data gradient of a 3x3 convolution as an implicit product, for Cout % 16 == 0:
+// dX[m][c] = sum over (tap, co) of bf16(dZ[(b, oy, ox)][co]) Wd[c][tap Cout + co], m = (b, iy, ix),
+// over the taps whose output position reads the pixel (oy stride = iy + 1 - ky, ox likewise, in
+// range): the product form of dcol = dZ W followed by col2im, without the dcol matrix. Each 16-wide
+// k step lies in one tap; a lane's four dZ values are one 16-byte load. Block and wave layout as
+// k_gemm_nt; B: Wd rows of K = 9 Cout bf16 (k_weights_dgrad).
+template <int WN>
+__global__ __launch_bounds__(256) void k_conv_dgrad(const float* __restrict__ dZ, int Si, int Cout, int So, int stride,
+                                                    const bf16_t* __restrict__ B, float* __restrict__ dX, long long M,
+                                                    int N) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long long m0 = (long long)blockIdx.x * (128 / WN) + (wave / WN) * 32;
+  const int n0 = blockIdx.y * (32 * WN) + (wave % WN) * 32;
+  const int r = lane & 15, kq = (lane >> 4) * 4;
+  const int K = 9 * Cout, SiSi = Si * Si, ldb = K;
+  const float* zp[2];
+  int iy1[2], ix1[2];
+  bool aok[2];
+  const bf16_t* bp[2];
+  bool bok[2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const long long m = m0 + i * 16 + r;
+    aok[i] = m < M;
+    const int mm = aok[i] ? (int)m : 0;
+    const int b = mm / SiSi, pix = mm - b * SiSi;
+    const int iy = pix / Si, ix = pix - iy * Si;
+    iy1[i] = iy + 1;
+    ix1[i] = ix + 1;
+    zp[i] = dZ + (size_t)b * So * So * Cout + kq;
+    const int n = n0 + i * 16 + r;
+    bok[i] = n < N;
+    bp[i] = B + (size_t)(bok[i] ? n : 0) * ldb + kq;
+  }
+  float4_t acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
+  const short4_t zero = {0, 0, 0, 0};
+  int ky = 0, kx = 0, c0 = 0;
+  for (int k = 0; k < K; k += 16) {
+    short4_t a[2], b[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int ty = iy1[i] - ky, tx = ix1[i] - kx;
+      const int oy = ty / stride, ox = tx / stride;
+      const bool ok = aok[i] && ty >= 0 && tx >= 0 && oy * stride == ty && ox * stride == tx && oy < So && ox < So;
+      a[i] = ok ? bf16x4(*reinterpret_cast<const float4*>(zp[i] + (oy * So + ox) * Cout + c0)) : zero;
+      b[i] = bok[i] ? *reinterpret_cast<const short4_t*>(bp[i] + k) : zero;
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+    c0 += 16;
+    if (c0 == Cout) {
+      c0 = 0;
+      if (++kx == 3) {
+        kx = 0;
+        ++ky;
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const long long m = m0 + i * 16 + kq + e;
+        const int n = n0 + j * 16 + r;
+        if (m < M && n < N) dX[m * N + n] = acc[i][j][e];
+      }
+}
+
+// Wd[c][tap Cout + co] = bf16(W[co][tap Cin + c]): the weights as the data gradient's B operand
+__global__ void k_weights_dgrad(const float* __restrict__ W, int Cin, int Cout, bf16_t* __restrict__ Wd) {
+  const int total = Cin * 9 * Cout;
+  for (int e = blockIdx.x * 256 + threadIdx.x; e < total; e += gridDim.x * 256) {
+    const int c = e / (9 * Cout), rr = e - c * 9 * Cout;
+    const int tap = rr / Cout, co = rr - tap * Cout;
+    Wd[e] = f2bf(W[(size_t)co * 9 * Cin + tap * Cin + c]);
+  }
+}
+
 // out[m][n] = sum over z of parts[z][m][n] (+ bias, ReLU): a block covers 32 consecutive outputs
 // with 8 interleaved z subsets each (z = q, q + 8, ...), added in subset order.
 template <bool BIAS, bool RELU>
@@ -1038,6 +1124,28 @@ int tm_cnn_backward(tm_cnn* h, const float* heat_target, const int* bin_target, 
       gemm(st, h->parts, h->dzT.p, (int)Mp, h->colT.p, (int)Mp, h->g.p + L.w_off, L.K, L.cout, L.K, (int)Mp, nullptr,
            false);
       if (l == 0) break;
+      if (!head && L.cout % 16 == 0) {
+        // implicit transposed convolution straight into the input's gradient
+        CNN_REQUIRE((long long)L.so * L.so * L.cout < (1LL << 31) && (long long)B * L.si * L.si < (1LL << 31),
+                    TM_ERR_INVALID, "dgrad: layer too large");
+        hipLaunchKernelGGL(k_weights_dgrad, dim3(grid1((long long)L.cin * 9 * L.cout)),
+                           dim3(256), 0, st, h->w.p + L.w_off, L.cin, L.cout, h->wbT.p);
+        check_launch("k_weights_dgrad");
+        const long long Mi = (long long)B * L.si * L.si;
+        if (L.cin <= 32) {
+          const dim3 g((unsigned)((Mi + 127) / 128), (L.cin + 31) / 32);
+          hipLaunchKernelGGL((k_conv_dgrad<1>), g, dim3(256), 0, st, dcur, L.si, L.cout, L.so, L.stride, h->wbT.p,
+                             dnext, Mi, L.cin);
+        } else {
+          const dim3 g((unsigned)((Mi + 63) / 64), (L.cin + 63) / 64);
+          hipLaunchKernelGGL((k_conv_dgrad<2>), g, dim3(256), 0, st, dcur, L.si, L.cout, L.so, L.stride, h->wbT.p,
+                             dnext, Mi, L.cin);
+        }
+        check_launch("k_conv_dgrad");
+        dcur = dnext;
+        dnext = (dnext == h->dA0.p) ? h->dA1.p : h->dA0.p;
+        continue;
+      }
       // dcol[M][K] = dZ[M][cout16] . W^T[K][cout16]^T, then the gather into the input's gradient
       const int c16 = round16(L.cout);
       to_bf16_pad(st, dcur, M, L.cout, L.cout, h->dzb.p, c16);
