@@ -8,6 +8,11 @@ tm_run_inner_loop call (all orientations of the handle's position range for one 
 steps cycle through the defocus planes. Inputs are prepared on the GPU and resident in HBM before
 the timed region starts. Metric: cross-correlation maps (CCFs) per second.
 
+Other workloads (`--workload`, each its own JSON line, not the headline): config2 (eight
+micrographs of mixed sizes), refine (refine_template on the 4096^2 micrograph's peaks), proposal
+(learned proposals + local refinement against the exhaustive search at 512^2 / 64^3), rocfft (the
+column pass against the vendor FFT), cnn (proposal-network training steps, 16 images of 512^2).
+
 Prints one JSON line (rank 0, N = 1). `--gpus N` with N > 1 exits: this build runs on one GPU.
 """
 from __future__ import annotations
