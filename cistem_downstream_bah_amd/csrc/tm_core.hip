@@ -648,8 +648,7 @@ int tm_set_micrograph(tm_core* h, const float* micrograph, float* curve_x, float
     for (int c = 0; c < n; ++c) x[c] = c * (hi - lo) / float(n - 1) + lo;
     TM_HIP(hipMemcpyAsync(h->pr_cx.p, x.data(), n * sizeof(float), hipMemcpyHostToDevice, st));
     TM_HIP(hipMemsetAsync(h->pr_acc.p, 0, 2 * (size_t)n * sizeof(double), st));
-    hipLaunchKernelGGL(tmp::k_power_curve, dim3(gs), dim3(256), 0, st, h->pr_spec.p, W, H, h->pr_cx.p, n, h->pr_acc.p,
-                       h->pr_acc.p + n);
+    tmp::power_curve(st, h->pr_spec.p, W, H, h->pr_cx.p, n, h->pr_acc.p, h->pr_acc.p + n, gs);
     check_launch("k_power_curve");
     hipLaunchKernelGGL(tmp::k_curve_finish, dim3(1), dim3(256), 0, st, h->pr_acc.p, h->pr_acc.p + n, n, h->pr_cy.p);
     check_launch("k_curve_finish");

@@ -122,6 +122,45 @@ __global__ void k_power_curve(const cf* __restrict__ c, int W, int H, const floa
   }
 }
 
+// k_power_curve with each block's bins privatised in LDS (dynamic shared memory of 2 n doubles):
+// the block accumulates with LDS atomics, then adds each non-zero bin to the global curve once.
+__global__ __launch_bounds__(256) void k_power_curve_lds(const cf* __restrict__ c, int W, int H,
+                                                         const float* __restrict__ cx, int n,
+                                                         double* __restrict__ acc_w, double* __restrict__ acc_n) {
+#pragma clang fp contract(off)
+  extern __shared__ double bins[];
+  for (int q = threadIdx.x; q < 2 * n; q += 256) bins[q] = 0.0;
+  __syncthreads();
+  const int nh = W / 2 + 1, fny = H / 2;
+  const long long total = (long long)nh * H;
+  for (long long e = blockIdx.x * 256LL + threadIdx.x; e < total; e += (long long)gridDim.x * 256) {
+    const int i = (int)(e % nh), j = (int)(e / nh);
+    if (i == 0 && j >= fny) continue;
+    const float fy = (float)((j >= fny ? j - H : j) * (1.0 / double(H)));
+    const float fx = (float)(i * (1.0 / double(W)));
+    const float sf = sqrtf(fx * fx + fy * fy + 0.0f);
+    const cf v = c[e];
+    curve_add(cx, n, sf, v.x * v.x + v.y * v.y, bins);
+    curve_add(cx, n, sf, 1.0f, bins + n);
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < n; q += 256) {
+    if (bins[q] != 0.0) atomicAdd(acc_w + q, bins[q]);
+    if (bins[n + q] != 0.0) atomicAdd(acc_n + q, bins[n + q]);
+  }
+}
+
+// The power curve into acc_w / acc_n (zeroed by the caller): LDS-privatised bins up to 4000 bins
+// (64 KB), else one global atomic per contribution.
+inline void power_curve(hipStream_t st, const cf* spec, int W, int H, const float* cx, int n, double* acc_w,
+                        double* acc_n, int grid) {
+  if (n <= 4000)
+    hipLaunchKernelGGL(k_power_curve_lds, dim3(grid), dim3(256), 2 * (size_t)n * sizeof(double), st, spec, W, H, cx, n,
+                       acc_w, acc_n);
+  else
+    hipLaunchKernelGGL(k_power_curve, dim3(grid), dim3(256), 0, st, spec, W, H, cx, n, acc_w, acc_n);
+}
+
 // whitening filter: mean power per bin, SquareRoot, Reciprocal, scaled to a maximum of 1. One block
 // of 256 threads: each bin's value is independent, and the maximum is exact in any order (fmaxf
 // skips NaN as the sequential loop does), so the result equals a one-thread loop bit for bit.

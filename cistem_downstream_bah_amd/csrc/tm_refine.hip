@@ -1603,8 +1603,7 @@ int tm_refine_set_micrograph(tm_refine* h, const float* micrograph) {
     for (int c = 0; c < n; ++c) x[c] = c * (hi - lo) / float(n - 1) + lo;
     TM_HIP(hipMemcpyAsync(h->curve_x.p, x.data(), n * sizeof(float), hipMemcpyHostToDevice, st));
     TM_HIP(hipMemsetAsync(h->pr_acc.p, 0, 2 * (size_t)n * sizeof(double), st));
-    hipLaunchKernelGGL(tmp::k_power_curve, dim3(gs), dim3(256), 0, st, h->pr_spec.p, mx, my, h->curve_x.p, n,
-                       h->pr_acc.p, h->pr_acc.p + n);
+    tmp::power_curve(st, h->pr_spec.p, mx, my, h->curve_x.p, n, h->pr_acc.p, h->pr_acc.p + n, gs);
     check_launch("k_power_curve");
     hipLaunchKernelGGL(tmp::k_curve_finish, dim3(1), dim3(256), 0, st, h->pr_acc.p, h->pr_acc.p + n, n, h->curve_y.p);
     check_launch("k_curve_finish");
