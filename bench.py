@@ -69,6 +69,33 @@ def read_traffic(workload: str, kernel: str, batch: int):
     return None
 
 
+def host_cpu_info() -> dict:
+    """The host's CPU model, its logical CPU count (nproc), the CPUs this process may run on and the
+    cgroup CPU quota (cores' worth of time) that bounds them."""
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    nproc = os.cpu_count() or 1
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = nproc
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = float(q) / float(period)
+    except (OSError, ValueError):
+        pass
+    usable = affinity if quota is None else max(1, min(affinity, int(quota)))
+    return {"model": model, "nproc": nproc, "affinity": affinity, "cgroup_quota_cores": quota, "usable": usable}
+
+
 def cpu_baseline(mic, vol, params, threads: int, psi_per_pos: int):
     """The CPU oracle (the reference's CPU match_template path, one single-threaded worker per
     core) on positions [0, threads) x psi [0, psi_per_pos) at the same 4096^2 / 512^3 shape."""
@@ -255,20 +282,27 @@ def main():
         if not math.isnan(kb):
             per_kernel[name]["gbps"] = round(kb / (per_kernel[name]["avg_ms"] / 1e3) / 1e9, 1)
     traffic = read_traffic(workload, dominant, B) if args.plane_storage == "fp32" else None
+    traffic_src = (f"profiles/pmc_{dominant}_{workload}_b{B}.txt: rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE of this "
+                   f"command (committed profile, not measured in this run)") if traffic is not None else None
     roofline = {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic, "kernel": dominant,
+                "frac": round(achieved / HBM_PEAK, 4), "traffic": traffic, "traffic_source": traffic_src,
+                "kernel": dominant,
                 "algorithmic_bytes_per_launch": algo, "orientations_per_launch": round(nb_avg, 2)}
     pipe = pipeline_bytes_per_ccf(W, H, B, zs=zs)
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        log(f"CPU baseline: oracle on {threads} threads x {args.cpu_psi} psi ...")
+        hinfo = host_cpu_info()
+        # every core this process may use (the reference runs one single-threaded process per core)
+        threads = args.cpu_threads or hinfo["usable"]
+        log(f"CPU baseline: oracle on {threads} threads x {args.cpu_psi} psi ({hinfo}) ...")
         v, n, dt, setup = cpu_baseline(s.micrograph, s.volume, params, threads, args.cpu_psi)
         cpu = {"value": round(v, 3), "unit": "CCF/s", "cores": threads, "kind": "port",
                "sample": f"{n} CCFs at the same {W}x{H} / {P}^3 shape (positions 0-{threads - 1} x psi "
-                         f"0-{args.cpu_psi - 1}, one worker thread per core), {dt:.1f}s timed after "
-                         f"{setup:.1f}s setup",
+                         f"0-{args.cpu_psi - 1}, one worker thread per core on every core available to the "
+                         f"process), {dt:.1f}s timed after {setup:.1f}s setup",
+               "cpu_model": hinfo["model"], "nproc": hinfo["nproc"], "affinity_cpus": hinfo["affinity"],
+               "cgroup_quota_cores": hinfo["cgroup_quota_cores"],
                "gpu_over_cpu": round(value / v, 1)}
         log(f"CPU baseline {v:.2f} CCF/s ({n} CCFs in {dt:.1f}s)")
 

@@ -27,7 +27,7 @@ I64P = ctypes.POINTER(ctypes.c_longlong)
 EXPORTS = ["tm_device_count", "tm_create", "tm_destroy", "tm_init", "tm_run_inner_loop", "tm_get_results", "tm_merge",
            "tm_set_template", "tm_reset", "tm_finalize", "tm_debug_projection", "tm_debug_ccf", "tm_set_timing", "tm_get_timing",
            "tm_last_error", "tm_autotune_batch", "tm_set_graph", "tm_set_input", "tm_set_micrograph",
-           "tm_find_peaks"]
+           "tm_find_peaks", "tm_run_partial", "tm_state_size", "tm_export_state", "tm_import_state"]
 
 PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.c_longlong, ctypes.c_void_p)
 
@@ -46,7 +46,16 @@ class InitParams(ctypes.Structure):
                 ("histogram_min_scaled", ctypes.c_double), ("histogram_step_scaled", ctypes.c_double),
                 ("histogram_number_of_bins", ctypes.c_int), ("max_padding", ctypes.c_int),
                 ("defocus_step", ctypes.c_float), ("pixel_size_step", ctypes.c_float),
-                ("batch_size", ctypes.c_int), ("plane_storage", ctypes.c_int)]
+                ("batch_size", ctypes.c_int), ("plane_storage", ctypes.c_int),
+                ("orientation_limit", ctypes.c_int), ("device_memory_limit", ctypes.c_longlong)]
+
+
+class PeakInfo(ctypes.Structure):
+    """tm_peak_info: the fields of TemplateMatchFoundPeakInfo (src/core/template_matching.h:4-18)."""
+    _fields_ = [("peak_number", ctypes.c_int), ("x_pos", ctypes.c_float), ("y_pos", ctypes.c_float),
+                ("psi", ctypes.c_float), ("theta", ctypes.c_float), ("phi", ctypes.c_float),
+                ("defocus", ctypes.c_float), ("pixel_size", ctypes.c_float), ("peak_height", ctypes.c_float),
+                ("original_peak_number", ctypes.c_int), ("new_peak_number", ctypes.c_int)]
 
 PLANE_STORAGE = {"fp32": 0, "fp16": 1, "bf16": 2}
 
@@ -76,8 +85,12 @@ def lib():
         L.tm_set_graph.argtypes = [vp, ctypes.c_int]
         L.tm_set_input.argtypes = [vp, F32P]
         L.tm_set_micrograph.argtypes = [vp, F32P, F32P, F32P]
-        L.tm_find_peaks.argtypes = [vp, ctypes.c_longlong, ctypes.c_float, ctypes.c_int, ctypes.c_float, F32P,
-                                    ctypes.c_int, I32P]
+        L.tm_find_peaks.argtypes = [vp, ctypes.c_longlong, ctypes.c_float, ctypes.c_int, ctypes.c_float,
+                                    ctypes.c_float, ctypes.POINTER(PeakInfo), ctypes.c_int, I32P]
+        L.tm_run_partial.argtypes = [vp, F32P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, I64P]
+        L.tm_state_size.argtypes = [vp, I64P]
+        L.tm_export_state.argtypes = [vp, ctypes.c_void_p, ctypes.c_longlong]
+        L.tm_import_state.argtypes = [vp, ctypes.c_void_p, ctypes.c_longlong]
         L.tm_get_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_char_p), F64P, I64P, ctypes.c_int]
         L.tm_autotune_batch.argtypes = [ctypes.c_int, ctypes.POINTER(InitParams), F32P, F32P, F32P, F32P, F32P, I32P,
                                         ctypes.c_int, ctypes.c_int, I32P, F64P]
@@ -158,12 +171,15 @@ class TemplateMatchingCore:
              search_positions: np.ndarray, psi_list: np.ndarray, first_search_position: int,
              last_search_position: int, histogram_min_scaled: float, histogram_step_scaled: float,
              histogram_number_of_bins: int = 512, max_padding: int = 0, defocus_step: float = 0.0,
-             pixel_size_step: float = 0.0, batch_size: int = 0, plane_storage: str = "fp32") -> None:
+             pixel_size_step: float = 0.0, batch_size: int = 0, plane_storage: str = "fp32",
+             orientation_limit: int = 0, device_memory_limit: int = 0) -> None:
         """TemplateMatchingCore::Init (TemplateMatchingCore.cu:49-138).
 
         template_spectrum: (P, P, P/2+1) complex64; input_spectrum: (H, W/2+1) complex64;
         search_positions: (n, 2) float32 {phi, theta}; psi_list: float32 in-plane angles;
-        plane_storage: "fp32" (default), "fp16" or "bf16" storage of the inter-pass plane."""
+        plane_storage: "fp32" (default), "fp16" or "bf16" storage of the inter-pass plane;
+        orientation_limit: search only the first N orientations of the range (0: all);
+        device_memory_limit: fail (TM_ERR_HIP) rather than allocate more device bytes (0: none)."""
         tpl = np.ascontiguousarray(template_spectrum, np.complex64)
         inp = np.ascontiguousarray(input_spectrum, np.complex64)
         grid = np.ascontiguousarray(search_positions, np.float32)
@@ -173,12 +189,15 @@ class TemplateMatchingCore:
         W = 2 * (Wh - 1)
         p = InitParams(P, projection_size, W, H, grid.shape[0], psi.size, first_search_position,
                        last_search_position, histogram_min_scaled, histogram_step_scaled, histogram_number_of_bins,
-                       max_padding, defocus_step, pixel_size_step, batch_size, PLANE_STORAGE[plane_storage])
+                       max_padding, defocus_step, pixel_size_step, batch_size, PLANE_STORAGE[plane_storage],
+                       orientation_limit, device_memory_limit)
         _check(lib().tm_init(self.h, ctypes.byref(p), _f32(tpl.view(np.float32)), _f32(inp.view(np.float32)),
                              _f32(grid), _f32(psi)))
         self.W, self.H, self.T, self.P = W, H, projection_size, P
         self.n_psi = psi.size
         self.first, self.last = first_search_position, last_search_position
+        n = (last_search_position - first_search_position + 1) * psi.size
+        self.n_orientations = min(n, orientation_limit) if orientation_limit > 0 else n
 
     def run_inner_loop(self, projection_filter: np.ndarray, size_i: int = 0, defocus_i: int = 0,
                        progress=None) -> int:
@@ -236,15 +255,44 @@ class TemplateMatchingCore:
         return cx, cy
 
     def find_peaks(self, total_ccfs: int, threshold: float, edge: int, min_peak_radius: float = 10.0,
-                   capacity: int = 1000) -> list:
+                   capacity: int = 1000, pixel_size: float = 1.0) -> list:
         """Peaks of the z-score map found on the device (tm_find_peaks): dicts with x, y (pixels),
-        peak_height and the psi, theta, phi, defocus, pixel_size values at the peak."""
-        out = np.zeros((capacity, 8), np.float32)
+        x_pos, y_pos (Angstroms), peak_height (z) and the psi, theta, phi, defocus, pixel_size values at
+        the peak, and the peak numbers of TemplateMatchFoundPeakInfo."""
+        out = (PeakInfo * max(capacity, 1))()
         n = ctypes.c_int(0)
-        _check(lib().tm_find_peaks(self.h, total_ccfs, threshold, edge, min_peak_radius, _f32(out), capacity,
+        _check(lib().tm_find_peaks(self.h, total_ccfs, threshold, edge, min_peak_radius, pixel_size, out, capacity,
                                    ctypes.byref(n)))
-        keys = ("x", "y", "peak_height", "psi", "theta", "phi", "defocus", "pixel_size")
-        return [{k: float(v) for k, v in zip(keys, row)} for row in out[: n.value]]
+        res = []
+        for k in range(n.value):
+            p = out[k]
+            d = {f: getattr(p, f) for f, _ in PeakInfo._fields_}
+            d["x"] = float(round(p.x_pos / pixel_size))  # integer pixel coordinates
+            d["y"] = float(round(p.y_pos / pixel_size))
+            res.append(d)
+        return res
+
+    def run_partial(self, projection_filter: np.ndarray, size_i: int, defocus_i: int, first_orientation: int,
+                    n_orientations: int) -> int:
+        """tm_run_partial: orientations [first, first + n) of the range for one plane."""
+        f = np.ascontiguousarray(projection_filter, np.complex64)
+        n = ctypes.c_longlong(0)
+        _check(lib().tm_run_partial(self.h, _f32(f.view(np.float32)), size_i, defocus_i, first_orientation,
+                                    n_orientations, ctypes.byref(n)))
+        return n.value
+
+    def export_state(self) -> bytes:
+        """tm_export_state: the search state (maps, histogram, planes, next orientation) as bytes."""
+        size = ctypes.c_longlong(0)
+        _check(lib().tm_state_size(self.h, ctypes.byref(size)))
+        buf = ctypes.create_string_buffer(size.value)
+        _check(lib().tm_export_state(self.h, buf, size.value))
+        return buf.raw
+
+    def import_state(self, state: bytes) -> None:
+        """tm_import_state: continue the search from an exported state."""
+        buf = ctypes.create_string_buffer(state, len(state))
+        _check(lib().tm_import_state(self.h, buf, len(state)))
 
     def reset(self) -> None:
         _check(lib().tm_reset(self.h))

@@ -282,14 +282,14 @@ int main(int argc, char** argv) {
     // T/4 + 1 border; mask a disc of min_peak_radius; stop below the threshold or at 1000 peaks
     if (!r90 && W == ox && H == oy) {
       // the search geometry is the image's: the peak search runs on the device
-      std::vector<float> pk(8 * 1000);
-      const int found = core.FindPeaks(total, thr, T / 4 + 1, min_peak_radius, pk.data(), 1000);
+      std::vector<tm_peak_info> pk(1000);
+      const int found = core.FindPeaks(total, thr, T / 4 + 1, min_peak_radius, pixel_size, pk.data(), 1000);
       FILE* pf2 = std::fopen((out + "_peaks.txt").c_str(), "w");
       std::fprintf(pf2, "# x_A y_A psi theta phi defocus pixel_size peak_height\n");
       for (int q = 0; q < found; ++q) {
-        const float* r = &pk[8 * (size_t)q];
-        std::fprintf(pf2, "%f %f %f %f %f %f %f %f\n", r[0] * pixel_size, r[1] * pixel_size, r[3], r[4], r[5], r[6],
-                     r[7], r[2]);
+        const tm_peak_info& r = pk[q];
+        std::fprintf(pf2, "%f %f %f %f %f %f %f %f\n", r.x_pos, r.y_pos, r.psi, r.theta, r.phi, r.defocus,
+                     r.pixel_size, r.peak_height);
       }
       std::fclose(pf2);
       std::fprintf(stderr, "%d peaks above the expected threshold %.3f\n", found, thr);

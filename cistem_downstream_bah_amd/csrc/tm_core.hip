@@ -59,6 +59,7 @@ struct tm_core {
   int n_pos_local = 0, n_psi = 0, n_orient = 0;
   std::vector<float> grid, psi;      // host copies ({phi, theta} per position; psi list)
   std::vector<int2> planes;          // (size_i, defocus_i) per plane index, in call order
+  int next_orientation = 0;          // first unsearched orientation of the last plane (n_orient: done)
   long long n_ccfs = 0;
   // device data
   DevBuf<cf> tpl, inpT, phaseP, phaseT, filt, slice, projT, rows, Z;
@@ -128,10 +129,24 @@ struct tm_core {
     if (it != tw.end()) return it->second;
     auto host = twiddles(n);
     cf* d = nullptr;
-    TM_HIP(hipMalloc((void**)&d, sizeof(cf) * n));
-    TM_HIP(hipMemcpy(d, host.data(), sizeof(cf) * n, hipMemcpyHostToDevice));
+    device_alloc((void**)&d, sizeof(cf) * n);
     tw[n] = d;
+    TM_HIP(hipMemcpy(d, host.data(), sizeof(cf) * n, hipMemcpyHostToDevice));
     return d;
+  }
+
+  // Frees the search's device buffers (not the stream), as before a fresh tm_init.
+  void release_buffers() {
+    drop_graph();
+    for (DevBuf<cf>* b : {&tpl, &inpT, &phaseP, &phaseT, &filt, &slice, &projT, &rows, &Z}) b->release();
+    mats.release();
+    mip.release();
+    idx.release();
+    sum.release();
+    sumsq.release();
+    hist.release();
+    for (auto& kv : tw) (void)hipFree(kv.second);
+    tw.clear();
   }
 };
 
@@ -368,6 +383,7 @@ void reset_state(tm_core* h) {
   hipLaunchKernelGGL(tmk::k_zero_u64, dim3(2), dim3(256), 0, h->stream, h->hist.p, 512);
   check_launch("k_zero_u64");
   h->planes.clear();
+  h->next_orientation = 0;
   h->n_ccfs = 0;
 }
 
@@ -499,6 +515,17 @@ int tm_init(tm_core* h, const tm_init_params* p, const float* template_spectrum,
                "NULL input array");
     TM_HIP(hipSetDevice(h->device));
     h->initialised = false;
+    TM_REQUIRE(p->device_memory_limit >= 0 && p->orientation_limit >= 0, TM_ERR_INVALID,
+               "device_memory_limit and orientation_limit must be >= 0");
+    TM_HIP(hipStreamSynchronize(h->stream));
+    h->release_buffers();
+    // every allocation below counts against device_memory_limit; on failure the buffers are freed
+    ScopedAllocLimit limit(p->device_memory_limit);
+    struct FreeOnError {
+      tm_core* h;
+      bool ok = false;
+      ~FreeOnError() { if (!ok) h->release_buffers(); }
+    } free_on_error{h};
     const int P = p->template_size, T = p->projection_size, W = p->image_x, H = p->image_y;
     TM_REQUIRE(image_size_compiled(W) && image_size_compiled(H), TM_ERR_UNSUPPORTED,
                "image size " + std::to_string(W) + "x" + std::to_string(H) +
@@ -532,18 +559,19 @@ int tm_init(tm_core* h, const tm_init_params* p, const float* template_spectrum,
     TM_REQUIRE(h->B <= 256, TM_ERR_INVALID, "batch_size must be <= 256");
     h->n_pos_local = p->last_search_position - p->first_search_position + 1;
     h->n_psi = p->n_psi;
-    const long long n_orient = (long long)h->n_pos_local * h->n_psi;
+    long long n_orient = (long long)h->n_pos_local * h->n_psi;
+    if (p->orientation_limit > 0) n_orient = std::min<long long>(n_orient, p->orientation_limit);
     TM_REQUIRE(n_orient < (1LL << 30), TM_ERR_INVALID, "too many orientations for one handle");
     h->n_orient = (int)n_orient;
+    h->n_pos_local = (int)((n_orient + h->n_psi - 1) / h->n_psi);  // positions with at least one orientation
     h->grid.assign(search_positions, search_positions + 2 * (size_t)p->n_search_positions);
     h->psi.assign(psi_list, psi_list + p->n_psi);
 
     // orientation matrices in reference loop order: position-major, psi inner
     std::vector<float> m((size_t)h->n_orient * 9);
-    for (int pl = 0; pl < h->n_pos_local; ++pl) {
-      const int pos = p->first_search_position + pl;
-      for (int k = 0; k < h->n_psi; ++k)
-        euler_matrix(h->grid[2 * pos], h->grid[2 * pos + 1], h->psi[k], &m[((size_t)pl * h->n_psi + k) * 9]);
+    for (int o = 0; o < h->n_orient; ++o) {
+      const int pos = p->first_search_position + o / h->n_psi, k = o % h->n_psi;
+      euler_matrix(h->grid[2 * pos], h->grid[2 * pos + 1], h->psi[k], &m[(size_t)o * 9]);
     }
     h->mats.alloc(m.size());
     TM_HIP(hipMemcpy(h->mats.p, m.data(), m.size() * sizeof(float), hipMemcpyHostToDevice));
@@ -582,6 +610,7 @@ int tm_init(tm_core* h, const tm_init_params* p, const float* template_spectrum,
     for (int k : {P, P / 2, T, T / 2, W, W / 2, H}) h->twiddle(k);
     reset_state(h);
     TM_HIP(hipStreamSynchronize(h->stream));
+    free_on_error.ok = true;
     h->initialised = true;
   });
 }
@@ -757,7 +786,150 @@ int tm_run_inner_loop(tm_core* h, const float* projection_filter, int size_i, in
     TM_HIP(hipStreamSynchronize(h->stream));
     if (h->timing) h->resolve_timing();
     h->n_ccfs += done;
+    h->next_orientation = h->n_orient;
     if (ccfs) *ccfs = done;
+  });
+}
+
+int tm_run_partial(tm_core* h, const float* projection_filter, int size_i, int defocus_i, int first_orientation,
+                   int n_orientations, long long* ccfs) {
+  return guarded([&] {
+    require_ready(h);
+    TM_REQUIRE(first_orientation >= 0 && n_orientations >= 0 &&
+                   (long long)first_orientation + n_orientations <= h->n_orient,
+               TM_ERR_INVALID, "orientation range outside the handle's " + std::to_string(h->n_orient));
+    if (first_orientation == 0) {
+      const long long per_plane = (long long)h->prm.n_search_positions * h->n_psi;
+      TM_REQUIRE((long long)(h->planes.size() + 1) * per_plane < (1LL << 31), TM_ERR_INVALID,
+                 "orientation index space exhausted (too many planes for one handle)");
+      TM_REQUIRE(h->planes.empty() || h->next_orientation == h->n_orient, TM_ERR_STATE,
+                 "the previous plane is not finished (next orientation " + std::to_string(h->next_orientation) + ")");
+      h->planes.push_back(make_int2(size_i, defocus_i));
+      h->next_orientation = 0;
+    } else {
+      TM_REQUIRE(!h->planes.empty() && h->planes.back().x == size_i && h->planes.back().y == defocus_i,
+                 TM_ERR_STATE, "a continued range must belong to the current plane");
+      TM_REQUIRE(first_orientation == h->next_orientation, TM_ERR_STATE,
+                 "the current plane continues at orientation " + std::to_string(h->next_orientation));
+    }
+    upload_filter(h, projection_filter);
+    const int plane = (int)h->planes.size() - 1;
+    const long long per_plane = (long long)h->prm.n_search_positions * h->n_psi;
+    const long long first = (long long)h->prm.first_search_position * h->n_psi;
+    for (int k = 0; k < K_COUNT; ++k) { h->kernel_ms[k] = 0.0; h->kernel_launches[k] = 0; }
+    h->ev_marks.clear();
+    h->ev_used = 0;
+    const int end = first_orientation + n_orientations;
+    for (int o0 = first_orientation; o0 < end; o0 += h->B) {
+      const int nb = std::min(h->B, end - o0);
+      run_batch(h, o0, nb, (int)(plane * per_plane + first + o0));
+    }
+    TM_HIP(hipStreamSynchronize(h->stream));
+    if (h->timing) h->resolve_timing();
+    h->n_ccfs += n_orientations;
+    h->next_orientation = end;
+    if (ccfs) *ccfs = n_orientations;
+  });
+}
+
+namespace {
+// Serialised search state: header, planes, then the maps and the histogram.
+struct StateHeader {
+  char magic[8];
+  int W, H, n_orient, n_psi, n_search_positions, first_search_position, n_planes, next_orientation;
+  long long n_ccfs;
+};
+constexpr char kStateMagic[8] = {'T', 'M', 'S', 'T', 'A', 'T', 'E', '1'};
+
+long long state_bytes(const tm_core* h, size_t n_planes) {
+  const long long npx = (long long)h->W * h->H;
+  return (long long)sizeof(StateHeader) + (long long)n_planes * 2 * sizeof(int) + npx * (4 + 4 + 8 + 8) + 512 * 8;
+}
+}  // namespace
+
+int tm_state_size(tm_core* h, long long* bytes) {
+  return guarded([&] {
+    require_ready(h);
+    TM_REQUIRE(bytes, TM_ERR_INVALID, "bytes is NULL");
+    *bytes = state_bytes(h, h->planes.size());
+  });
+}
+
+int tm_export_state(tm_core* h, void* buffer, long long bytes) {
+  return guarded([&] {
+    require_ready(h);
+    TM_REQUIRE(buffer, TM_ERR_INVALID, "buffer is NULL");
+    TM_REQUIRE(bytes >= state_bytes(h, h->planes.size()), TM_ERR_INVALID, "buffer smaller than tm_state_size");
+    StateHeader hd{};
+    std::memcpy(hd.magic, kStateMagic, sizeof(hd.magic));
+    hd.W = h->W; hd.H = h->H; hd.n_orient = h->n_orient; hd.n_psi = h->n_psi;
+    hd.n_search_positions = h->prm.n_search_positions;
+    hd.first_search_position = h->prm.first_search_position;
+    hd.n_planes = (int)h->planes.size();
+    hd.next_orientation = h->next_orientation;
+    hd.n_ccfs = h->n_ccfs;
+    char* o = static_cast<char*>(buffer);
+    std::memcpy(o, &hd, sizeof(hd));
+    o += sizeof(hd);
+    for (const int2& pl : h->planes) {
+      const int v[2] = {pl.x, pl.y};
+      std::memcpy(o, v, sizeof(v));
+      o += sizeof(v);
+    }
+    const size_t npx = (size_t)h->W * h->H;
+    hipStream_t st = h->stream;
+    TM_HIP(hipMemcpyAsync(o, h->mip.p, npx * 4, hipMemcpyDeviceToHost, st));
+    o += npx * 4;
+    TM_HIP(hipMemcpyAsync(o, h->idx.p, npx * 4, hipMemcpyDeviceToHost, st));
+    o += npx * 4;
+    TM_HIP(hipMemcpyAsync(o, h->sum.p, npx * 8, hipMemcpyDeviceToHost, st));
+    o += npx * 8;
+    TM_HIP(hipMemcpyAsync(o, h->sumsq.p, npx * 8, hipMemcpyDeviceToHost, st));
+    o += npx * 8;
+    TM_HIP(hipMemcpyAsync(o, h->hist.p, 512 * 8, hipMemcpyDeviceToHost, st));
+    TM_HIP(hipStreamSynchronize(st));
+  });
+}
+
+int tm_import_state(tm_core* h, const void* buffer, long long bytes) {
+  return guarded([&] {
+    require_ready(h);
+    TM_REQUIRE(buffer && bytes >= (long long)sizeof(StateHeader), TM_ERR_INVALID, "buffer is NULL or too small");
+    StateHeader hd;
+    const char* in = static_cast<const char*>(buffer);
+    std::memcpy(&hd, in, sizeof(hd));
+    in += sizeof(hd);
+    TM_REQUIRE(std::memcmp(hd.magic, kStateMagic, sizeof(hd.magic)) == 0, TM_ERR_INVALID, "not a search state");
+    TM_REQUIRE(hd.W == h->W && hd.H == h->H && hd.n_orient == h->n_orient && hd.n_psi == h->n_psi &&
+                   hd.n_search_positions == h->prm.n_search_positions &&
+                   hd.first_search_position == h->prm.first_search_position,
+               TM_ERR_INVALID, "the state belongs to a different search (image, grid, psi or range)");
+    TM_REQUIRE(hd.n_planes >= 0 && hd.next_orientation >= 0 && hd.next_orientation <= hd.n_orient &&
+                   bytes >= state_bytes(h, (size_t)hd.n_planes),
+               TM_ERR_INVALID, "truncated or inconsistent search state");
+    std::vector<int2> planes(hd.n_planes);
+    for (int k = 0; k < hd.n_planes; ++k) {
+      int v[2];
+      std::memcpy(v, in, sizeof(v));
+      in += sizeof(v);
+      planes[k] = make_int2(v[0], v[1]);
+    }
+    const size_t npx = (size_t)h->W * h->H;
+    hipStream_t st = h->stream;
+    TM_HIP(hipStreamSynchronize(st));
+    TM_HIP(hipMemcpyAsync(h->mip.p, in, npx * 4, hipMemcpyHostToDevice, st));
+    in += npx * 4;
+    TM_HIP(hipMemcpyAsync(h->idx.p, in, npx * 4, hipMemcpyHostToDevice, st));
+    in += npx * 4;
+    TM_HIP(hipMemcpyAsync(h->sum.p, in, npx * 8, hipMemcpyHostToDevice, st));
+    in += npx * 8;
+    TM_HIP(hipMemcpyAsync(h->sumsq.p, in, npx * 8, hipMemcpyHostToDevice, st));
+    in += npx * 8;
+    TM_HIP(hipMemcpyAsync(h->hist.p, in, 512 * 8, hipMemcpyHostToDevice, st));
+    TM_HIP(hipStreamSynchronize(st));
+    h->planes = planes;
+    h->next_orientation = hd.next_orientation;
+    h->n_ccfs = hd.n_ccfs;
   });
 }
 
@@ -859,8 +1031,8 @@ int tm_merge(tm_core* dst, tm_core* src) {
   });
 }
 
-int tm_find_peaks(tm_core* h, long long total_ccfs, float threshold, int edge, float min_peak_radius, float* out,
-                  int capacity, int* n_peaks) {
+int tm_find_peaks(tm_core* h, long long total_ccfs, float threshold, int edge, float min_peak_radius, float pixel_size,
+                  tm_peak_info* out, int capacity, int* n_peaks) {
   return guarded([&] {
     require_ready(h);
     TM_REQUIRE(total_ccfs > 0 && n_peaks && (out || capacity == 0) && capacity >= 0 && edge >= 0, TM_ERR_INVALID,
@@ -906,16 +1078,17 @@ int tm_find_peaks(tm_core* h, long long total_ccfs, float threshold, int edge, f
         a_def = float(h->planes[plane].y) * h->prm.defocus_step;
         a_pix = float(h->planes[plane].x) * h->prm.pixel_size_step;
       }
-      float* o = out + 8 * (size_t)found;
-      o[0] = (float)bx;
-      o[1] = (float)by;
-      o[2] = v;
-      o[3] = a_psi;
-      o[4] = a_theta;
-      o[5] = a_phi;
-      o[6] = a_def;
-      o[7] = a_pix;
+      tm_peak_info& o = out[found];
       ++found;
+      o.peak_number = o.original_peak_number = o.new_peak_number = found;
+      o.x_pos = (float)bx * pixel_size;
+      o.y_pos = (float)by * pixel_size;
+      o.psi = a_psi;
+      o.theta = a_theta;
+      o.phi = a_phi;
+      o.defocus = a_def;
+      o.pixel_size = a_pix;
+      o.peak_height = v;
       const int x_lo = std::max(bx - r, 0), x_hi = std::min(bx + r, W), y_lo = std::max(by - r, 0), y_hi = std::min(by + r, H);
       if (x_hi > x_lo && y_hi > y_lo) {
         hipLaunchKernelGGL(k_zmask, dim3((x_hi - x_lo + 63) / 64, y_hi - y_lo), dim3(64), 0, st, z, W, H, bx, by, r, r2);
@@ -960,8 +1133,9 @@ int tm_debug_projection(tm_core* h, const float* projection_filter, int search_p
     require_ready(h);
     TM_REQUIRE(out, TM_ERR_INVALID, "out is NULL");
     const int pl = search_position - h->prm.first_search_position;
-    TM_REQUIRE(pl >= 0 && pl < h->n_pos_local && psi_index >= 0 && psi_index < h->n_psi, TM_ERR_INVALID,
-               "orientation outside the handle's range");
+    TM_REQUIRE(pl >= 0 && pl < h->n_pos_local && psi_index >= 0 && psi_index < h->n_psi &&
+                   pl * h->n_psi + psi_index < h->n_orient,
+               TM_ERR_INVALID, "orientation outside the handle's range");
     upload_filter(h, projection_filter);
     run_projections(h, pl * h->n_psi + psi_index, 1);
     TM_HIP(hipMemcpyAsync(out, h->projT.p, (size_t)h->T * (h->T + 2) * sizeof(float), hipMemcpyDeviceToHost,
@@ -975,8 +1149,9 @@ int tm_debug_ccf(tm_core* h, const float* projection_filter, int search_position
     require_ready(h);
     TM_REQUIRE(out, TM_ERR_INVALID, "out is NULL");
     const int pl = search_position - h->prm.first_search_position;
-    TM_REQUIRE(pl >= 0 && pl < h->n_pos_local && psi_index >= 0 && psi_index < h->n_psi, TM_ERR_INVALID,
-               "orientation outside the handle's range");
+    TM_REQUIRE(pl >= 0 && pl < h->n_pos_local && psi_index >= 0 && psi_index < h->n_psi &&
+                   pl * h->n_psi + psi_index < h->n_orient,
+               TM_ERR_INVALID, "orientation outside the handle's range");
     upload_filter(h, projection_filter);
     run_projections(h, pl * h->n_psi + psi_index, 1);
     run_forward_rows(h, 1);

@@ -106,6 +106,35 @@ void euler_matrix(float phi, float theta, float psi, float* m) {
   m[8] = ct;
 }
 
+// A limit on the device bytes the calling thread may allocate while it is active (tm_init's
+// device_memory_limit); exceeding it fails like hipMalloc running out of memory.
+struct AllocLimit {
+  long long limit = 0, used = 0;
+};
+thread_local AllocLimit* g_alloc_limit = nullptr;
+
+inline void device_alloc(void** p, size_t bytes) {
+  if (g_alloc_limit) {
+    if (g_alloc_limit->used + (long long)bytes > g_alloc_limit->limit)
+      throw TmError{TM_ERR_HIP, "device allocation of " + std::to_string(bytes) + " bytes would exceed the limit of " +
+                                    std::to_string(g_alloc_limit->limit) + " bytes (" +
+                                    std::to_string(g_alloc_limit->used) + " allocated): out of memory"};
+    g_alloc_limit->used += (long long)bytes;
+  }
+  TM_HIP(hipMalloc(p, bytes));
+}
+
+// Activates a limit for the current scope (no limit when bytes <= 0).
+struct ScopedAllocLimit {
+  AllocLimit lim;
+  AllocLimit* prev;
+  explicit ScopedAllocLimit(long long bytes) : prev(g_alloc_limit) {
+    lim.limit = bytes;
+    if (bytes > 0) g_alloc_limit = &lim;
+  }
+  ~ScopedAllocLimit() { g_alloc_limit = prev; }
+};
+
 template <class T>
 struct DevBuf {
   T* p = nullptr;
@@ -113,7 +142,7 @@ struct DevBuf {
   void alloc(size_t count) {
     release();
     if (count == 0) return;
-    TM_HIP(hipMalloc((void**)&p, count * sizeof(T)));
+    device_alloc((void**)&p, count * sizeof(T));
     n = count;
   }
   void release() {

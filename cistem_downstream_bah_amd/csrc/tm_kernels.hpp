@@ -833,6 +833,19 @@ struct EpiParams {
   float zinv;  // inverse of the scale of 16-bit Z storage (1 for FP32)
 };
 
+// Histogram bin of a correlation value as match_template.cpp:1124 computes it,
+// int(double(v - histogram_min_scaled) / histogram_step_scaled) with the min and step in double, or
+// -1 outside bins 0..nbins-1 (the reference's guard admits bin nbins, one past its array; here it is
+// dropped). The quotient is formed with the reciprocal of the step, which can differ from the
+// correctly rounded division by an ulp; within 1e-9 of an integer (where that ulp could cross a bin
+// edge) the division itself is done, so the bin always equals the reference expression's.
+__device__ __forceinline__ int hist_bin(float v, const EpiParams& ep) {
+  const double t = (double)v - ep.hmin;
+  double d = t * ep.hstep_inv;
+  if (__builtin_expect(fabs(d - rint(d)) < 1e-9, 0)) d = t / ep.hstep;  // rare: kept as a branch
+  return (d > -1.0 && d < (double)ep.nbins) ? (int)d : -1;
+}
+
 // Threads of a k_last block: QR orientations x (W/2) / R1 threads, R1 the first radix of the
 // W/2-point plan (radix 8 wherever W/2 has three factors of two).
 template <int W, int QR>
@@ -951,8 +964,8 @@ void k_last(const cf* __restrict__ Z, int H, int nb, EpiParams ep, const cf* __r
           const float vx = vv[c];
           const int p = 2 * u + c;
           if (vx > mip[p]) { mip[p] = vx; idx[p] = g; }
-          const double d = ((double)vx - ep.hmin) * ep.hstep_inv;
-          if (d > -1.0 && d < (double)ep.nbins) atomicAdd(&hist[(int)d * HC + tid % HC], 1u);
+          const int bin = hist_bin(vx, ep);
+          if (bin >= 0) atomicAdd(&hist[bin * HC + tid % HC], 1u);
           s1[p] += (double)vx;
           const float sq = vx * vx;
           s2[p] += (double)sq;
@@ -1126,8 +1139,8 @@ void k_last_dma(const cf* __restrict__ Z, int H, int nb, EpiParams ep, const cf*
           const float vx = vv[c];
           const int p = 2 * u + c;
           if (vx > mip[p]) { mip[p] = vx; idx[p] = g; }
-          const double d = ((double)vx - ep.hmin) * ep.hstep_inv;
-          if (d > -1.0 && d < (double)ep.nbins) lds_inc(&hist[(int)d * HC + tid % HC]);
+          const int bin = hist_bin(vx, ep);
+          if (bin >= 0) lds_inc(&hist[bin * HC + tid % HC]);
           s1[p] += (double)vx;
           const float sq = vx * vx;
           s2[p] += (double)sq;

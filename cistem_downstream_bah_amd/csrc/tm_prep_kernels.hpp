@@ -92,7 +92,8 @@ __global__ void k_scale_zero_dc(cf* __restrict__ c, long long n, float s) {
 // Curve::AddValueAtXUsingLinearInterpolation weights of one value at wx (curve x[0..n-1])
 __device__ inline void curve_add(const float* __restrict__ cx, int n, float wx, float val, double* __restrict__ acc) {
 #pragma clang fp contract(off)
-  const int idx = (cx[0] == cx[1]) ? n - 2 : int((wx - cx[0]) / (cx[1] - cx[0]));
+  // the reference asserts 0 <= index < n (curve.cpp:528); the clamp keeps a rounding edge in range
+  const int idx = min(max((cx[0] == cx[1]) ? n - 2 : int((wx - cx[0]) / (cx[1] - cx[0])), 0), n - 1);
   if (idx == n - 1) {
     atomicAdd(acc + idx, (double)val);
     return;
@@ -154,9 +155,11 @@ __global__ __launch_bounds__(256) void k_power_curve_lds(const cf* __restrict__ 
 // (64 KB), else one global atomic per contribution.
 inline void power_curve(hipStream_t st, const cf* spec, int W, int H, const float* cx, int n, double* acc_w,
                         double* acc_n, int grid) {
+  // the LDS variant zeroes and flushes 2 n bins per block: at most 1024 blocks (4 per CU), so that
+  // each block's grid-stride loop covers many spectrum elements
   if (n <= 4000)
-    hipLaunchKernelGGL(k_power_curve_lds, dim3(grid), dim3(256), 2 * (size_t)n * sizeof(double), st, spec, W, H, cx, n,
-                       acc_w, acc_n);
+    hipLaunchKernelGGL(k_power_curve_lds, dim3(std::min(grid, 1024)), dim3(256), 2 * (size_t)n * sizeof(double), st,
+                       spec, W, H, cx, n, acc_w, acc_n);
   else
     hipLaunchKernelGGL(k_power_curve, dim3(grid), dim3(256), 0, st, spec, W, H, cx, n, acc_w, acc_n);
 }

@@ -99,11 +99,12 @@ class TemplateMatchingCore {
   }
   // Replaces the input spectrum and resets the maps (no counterpart in the reference).
   void SetInput(const float* input_spectrum) { check(tm_set_input(h_, input_spectrum)); }
-  // Peaks of the z-score map on the device, 8 floats each (x, y, z, psi, theta, phi, defocus,
-  // pixel size); returns their number (no counterpart in the reference's GPU path).
-  int FindPeaks(long long total_ccfs, float threshold, int edge, float min_peak_radius, float* out, int capacity) {
+  // Peaks of the z-score map on the device, in the layout of TemplateMatchFoundPeakInfo
+  // (template_matching.h:4-18); returns their number (no counterpart in the reference's GPU path).
+  int FindPeaks(long long total_ccfs, float threshold, int edge, float min_peak_radius, float pixel_size,
+                tm_peak_info* out, int capacity) {
     int n = 0;
-    check(tm_find_peaks(h_, total_ccfs, threshold, edge, min_peak_radius, out, capacity, &n));
+    check(tm_find_peaks(h_, total_ccfs, threshold, edge, min_peak_radius, pixel_size, out, capacity, &n));
     return n;
   }
   // Replays each plane from one captured hipGraph (no counterpart in the reference).

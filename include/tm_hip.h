@@ -52,6 +52,14 @@ typedef struct tm_init_params {
                                     TM_PLANE_BF16 (half the traffic; image height 4096 and a
                                     power-of-two width >= 1024 only, else TM_ERR_UNSUPPORTED). No
                                     counterpart in the reference, whose planes are FP32.           */
+  int orientation_limit;         /* 0: every orientation of the position range; N > 0: only the first
+                                    N in loop order (position-major, psi inner), the last position
+                                    possibly cut short (a search truncated to its first N
+                                    orientations). No counterpart in the reference.                */
+  long long device_memory_limit; /* 0: none; else tm_init fails with TM_ERR_HIP, freeing what it
+                                    allocated and leaving the handle uninitialised, when its device
+                                    allocations would exceed this many bytes, as when hipMalloc runs
+                                    out of memory (the reference exits there, defines.h:75-80).     */
 } tm_init_params;
 
 #define TM_PLANE_FP32 0
@@ -149,16 +157,56 @@ int tm_set_input(tm_core* h, const float* input_spectrum);
  * tm_prepare_input_image and tm_set_input. No counterpart in the reference. */
 int tm_set_micrograph(tm_core* h, const float* micrograph, float* curve_x, float* curve_y);
 
+/* One found peak, with the fields of TemplateMatchFoundPeakInfo in its order
+ * (src/core/template_matching.h:4-18): positions in Angstroms (pixel coordinate x pixel size, as
+ * match_template.cpp:1883-1884 reports them), angles in degrees, the defocus and pixel-size offsets
+ * of the best plane, peak_height the z-score. peak_number, original_peak_number and new_peak_number
+ * are the 1-based order in which the peak was found. */
+typedef struct tm_peak_info {
+  int peak_number;
+  float x_pos;
+  float y_pos;
+  float psi;
+  float theta;
+  float phi;
+  float defocus;
+  float pixel_size;
+  float peak_height;
+  int original_peak_number;
+  int new_peak_number;
+} tm_peak_info;
+
 /* The peak search of match_template.cpp:1839-1980 on the device, over the z-score map of the
  * search geometry (W x H, which is the micrograph's when the size policy neither pads nor rotates
  * it): the finalised z-score (as tm_finalize), then repeatedly the first maximum in row-major order
  * over [edge, W - edge) x [edge, H - edge), stopping below threshold or at capacity, each peak masking
  * the pixels of [y - r, y + r) x [x - r, x + r) whose squared distance is <= r^2 (r = the integer
- * part of min_peak_radius). out receives 8 floats per peak: x, y (pixels), z, psi, theta, phi,
- * defocus and pixel-size offsets. No counterpart in the reference's GPU path (the peak search runs
- * on the host there). */
-int tm_find_peaks(tm_core* h, long long total_ccfs, float threshold, int edge, float min_peak_radius, float* out,
-                  int capacity, int* n_peaks);
+ * part of min_peak_radius). out receives one tm_peak_info per peak (pixel_size converts the pixel
+ * coordinates to x_pos / y_pos in Angstroms). No counterpart in the reference's GPU path (the peak
+ * search runs on the host there). */
+int tm_find_peaks(tm_core* h, long long total_ccfs, float threshold, int edge, float min_peak_radius, float pixel_size,
+                  tm_peak_info* out, int capacity, int* n_peaks);
+
+/* Checkpoint and resume of a search, with no counterpart in the reference (whose jobs are restarted
+ * from their position range, match_template.cpp:378-379, 735-739).
+ *
+ * tm_run_partial runs orientations [first_orientation, first_orientation + n_orientations) of the
+ * handle's range (indices in loop order, 0 .. number of orientations - 1) for one plane and folds
+ * them into the maps. first_orientation 0 starts a new plane (size_i, defocus_i), as
+ * tm_run_inner_loop does; a later first_orientation must equal the plane's next unsearched
+ * orientation and the same (size_i, defocus_i). tm_run_inner_loop of a plane equals tm_run_partial
+ * calls that cover it in order, bit for bit, however they split it.
+ *
+ * tm_state_size gives the bytes of the search state: MIP, best orientation index, FP64 sum and sum
+ * of squares (W x H each), the histogram, the CCF count, the planes run so far and the next
+ * orientation of the current plane. tm_export_state copies it to a caller buffer; tm_import_state
+ * restores it into a handle initialised for the same search (same tm_init_params and inputs), so
+ * that the search continues where the exporting handle stopped. */
+int tm_run_partial(tm_core* h, const float* projection_filter, int size_i, int defocus_i, int first_orientation,
+                   int n_orientations, long long* ccfs);
+int tm_state_size(tm_core* h, long long* bytes);
+int tm_export_state(tm_core* h, void* buffer, long long bytes);
+int tm_import_state(tm_core* h, const void* buffer, long long bytes);
 
 /* Graph mode, with no counterpart in the reference (its loop launches every kernel): when enabled,
  * the first tm_run_inner_loop after it captures the launches of one plane (all batches) into a

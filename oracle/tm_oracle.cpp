@@ -436,8 +436,9 @@ struct MatchTemplate {
   }
 
   // The hot loop over [first_pos, last_pos] x psi for all defocus planes (match_template.cpp:890-1169).
-  // Positions are split over `threads` contiguous slices per plane and merged in slice order,
-  // which keeps the sequential tie-break (earliest orientation wins).
+  // The plane's orientations, in loop order (position-major, psi inner), are split over `threads`
+  // contiguous slices and merged in slice order, which keeps the sequential tie-break (earliest
+  // orientation wins).
   void Run(MTResults& out, int threads, int pos_lo = -1, int pos_hi = -1, int def_lo = INT32_MIN, int def_hi = INT32_MAX,
            int psi_lo = 0, int psi_hi = INT32_MAX) const {
     psi_hi = std::min(psi_hi, NumPsi() - 1);
@@ -450,23 +451,21 @@ struct MatchTemplate {
       if (defocus_i < def_lo || defocus_i > def_hi) continue;
       Image<R> pf;
       ProjectionFilter(defocus_i, pf);
-      const int npos = pos_hi - pos_lo + 1;
-      const int nt = std::min(threads, std::max(1, npos));
+      const long npos = pos_hi - pos_lo + 1, nk = std::max(0, psi_hi - psi_lo + 1), n = npos * nk;
+      const int nt = (int)std::min<long>(threads, std::max(1L, n));
       std::vector<MTResults> part(nt);
       std::vector<std::thread> th;
       for (int t = 0; t < nt; ++t) {
         th.emplace_back([&, t]() {
-          int a = pos_lo + (int)((long)npos * t / nt);
-          int b = pos_lo + (int)((long)npos * (t + 1) / nt) - 1;
+          const long a = n * t / nt, b = n * (t + 1) / nt;
           InitResults(part[t]);
           Work w;
           InitWork(w);
-          for (int pos = a; pos <= b; ++pos) {
-            for (int k = psi_lo; k <= psi_hi; ++k) {
-              Projection(w, pf, grid[2 * pos], grid[2 * pos + 1], psi[k], size_i);
-              CCF(w);
-              Accumulate(w, part[t], pos, k, defocus_i, size_i, GlobalIndex(size_i, defocus_i, pos, k));
-            }
+          for (long o = a; o < b; ++o) {
+            const int pos = pos_lo + (int)(o / nk), k = psi_lo + (int)(o % nk);
+            Projection(w, pf, grid[2 * pos], grid[2 * pos + 1], psi[k], size_i);
+            CCF(w);
+            Accumulate(w, part[t], pos, k, defocus_i, size_i, GlobalIndex(size_i, defocus_i, pos, k));
           }
         });
       }

@@ -75,7 +75,7 @@ def test_refine_errors_are_returned_not_fatal():
 def test_init_params_struct_matches_header():
     header = open(os.path.join(ROOT, "include", "tm_hip.h")).read()
     body = header[header.index("typedef struct tm_init_params"):header.index("} tm_init_params;")]
-    fields = re.findall(r"^\s+(?:int|double|float)\s+([\w, ]+);", body, re.M)
+    fields = re.findall(r"^\s+(?:int|double|float|long long)\s+([\w, ]+);", body, re.M)
     names = [f.strip() for group in fields for f in group.split(",")]
     assert names == [f[0] for f in core.InitParams._fields_]
 
@@ -116,7 +116,12 @@ def test_new_entry_points_reject_bad_arguments():
     assert L.tm_set_graph(None, 1) != 0
     assert L.tm_set_input(None, None) != 0
     assert L.tm_set_micrograph(None, None, None, None) != 0
-    assert L.tm_find_peaks(None, 1, 0.0, 0, 10.0, None, 0, ctypes.byref(n)) != 0
+    assert L.tm_find_peaks(None, 1, 0.0, 0, 10.0, 1.0, None, 0, ctypes.byref(n)) != 0
+    assert L.tm_run_partial(None, None, 0, 0, 0, 1, None) != 0
+    size = ctypes.c_longlong(0)
+    assert L.tm_state_size(None, ctypes.byref(size)) != 0
+    assert L.tm_export_state(None, None, 0) != 0
+    assert L.tm_import_state(None, None, 0) != 0
     assert L.tm_autotune_batch(0, None, None, None, None, None, None, None, 0, 0, None, None) != 0
     R = refine._lib()
     assert R.tm_refine_set_micrograph(None, None) != 0

@@ -3,9 +3,12 @@
 * bf16 MFMA product C = A . B^T: ragged M, N, K, a 601-column head shape, each block shape
   (64 x 64, 128 x 32 for N <= 32, 32 x 128 for M <= 32) and split-K cases of 2 and 19 splits, against
   float64 on the same bf16-rounded operands, within 1e-5 of sum |a| |b| per element
-* two networks: channels 8, 16, 16 (strides 2, 2, 1) and 16, 32, 48 (strides 2, 1, 2), so that the
-  implicit-product convolution (Cin a multiple of 16) runs with one, two and three k steps per tap
-  beside the im2col path (Cin 1 and 8)
+* three networks: channels 8, 16, 16 (strides 2, 2, 1), 16, 32, 48 (strides 2, 1, 2) and 16, 48, 64, 32
+  (strides 2, 1, 2, 1), so that the implicit-product convolution (Cin a multiple of 16) runs with
+  one, two and three k steps per tap beside the im2col path (Cin 1 and 8), and the backward pass's
+  implicit transposed convolution (data gradient of a non-head layer with Cout a multiple of 16)
+  runs both for Cin <= 32 (32-channel tiles) and for Cin 48 and 64 (64-channel tiles, the shapes of
+  the default network's deeper layers; 48 leaves a partial tile)
 * forward pass: every layer's activations and the logits within 1e-5 (relative L2) of the oracle
   with the device's bf16 rounding points (the remainder is FP32 accumulation order); the location
   logits and the logits of chosen cells read back equal the full logit volume
@@ -45,7 +48,7 @@ def _rel(a, b):
     return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
 
 
-CASES = {"narrow": ([8, 16, 16], [2, 2, 1]), "wide": ([16, 32, 48], [2, 1, 2])}
+CASES = {"narrow": ([8, 16, 16], [2, 2, 1]), "wide": ([16, 32, 48], [2, 1, 2]), "deep": ([16, 48, 64, 32], [2, 1, 2, 1])}
 
 
 def _case(kind="narrow"):
@@ -69,7 +72,7 @@ def _case(kind="narrow"):
 
 @pytest.mark.parametrize("kind", list(CASES))
 def test_forward_matches_oracle(kind):
-    print(f"case: cnn forward 64^2, 3 layers {CASES[kind]}, 20 bins, batch 3", flush=True)
+    print(f"case: cnn forward 64^2, layers {CASES[kind]}, 20 bins, batch 3", flush=True)
     spec, cfg, flat, images, *_ = _case(kind)
     net = cnn.ProposalNet(spec, max_batch=4, seed=None)
     try:
@@ -94,7 +97,7 @@ def test_forward_matches_oracle(kind):
 
 @pytest.mark.parametrize("kind", list(CASES))
 def test_backward_matches_oracle(kind):
-    print(f"case: cnn backward 64^2, 3 layers {CASES[kind]}, 20 bins, batch 3", flush=True)
+    print(f"case: cnn backward 64^2, layers {CASES[kind]}, 20 bins, batch 3", flush=True)
     spec, cfg, flat, images, heat, bins = _case(kind)
     net = cnn.ProposalNet(spec, max_batch=4, seed=None)
     try:

@@ -44,6 +44,21 @@ def test_device_peaks_equal_host_peaks(searched, thr):
             assert a[k] == b[k], k
 
 
+def test_peak_info_fields(searched):
+    """tm_peak_info follows TemplateMatchFoundPeakInfo (template_matching.h:4-18): positions in
+    Angstroms (pixel x pixel size, match_template.cpp:1883-1884), peak numbers in finding order."""
+    se, n = searched
+    a = se.core.find_peaks(n, 4.5, se.T // 4 + 1, 10.0, capacity=20)
+    b = se.core.find_peaks(n, 4.5, se.T // 4 + 1, 10.0, capacity=20, pixel_size=1.3)
+    assert len(a) == len(b) == 20
+    for k, (p, q) in enumerate(zip(a, b)):
+        assert p["peak_number"] == p["original_peak_number"] == p["new_peak_number"] == k + 1
+        assert (p["x_pos"], p["y_pos"]) == (p["x"], p["y"])
+        assert q["x_pos"] == np.float32(np.float32(p["x"]) * np.float32(1.3))
+        assert q["y_pos"] == np.float32(np.float32(p["y"]) * np.float32(1.3))
+        assert (q["x"], q["y"], q["peak_height"], q["psi"]) == (p["x"], p["y"], p["peak_height"], p["psi"])
+
+
 def test_capacity_and_high_threshold(searched):
     se, n = searched
     assert len(se.core.find_peaks(n, 4.5, se.T // 4 + 1, 10.0, capacity=3)) == 3

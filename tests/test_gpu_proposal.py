@@ -1,7 +1,7 @@
 """GPU: learned proposals + local exhaustive refinement (cistem_downstream_bah_amd/proposal.py).
 
 * tm_refine_score_batch equals tm_refine_score request by request (bit-identical), with more
-  windows and requests than one batch holds; after tm_refine_set_image a handle scores exactly as a
+  windows and requests than one batch holds (3 and 7 windows over 4 slots); after tm_refine_set_image a handle scores exactly as a
   handle made for that micrograph
 * the one-kernel TemplateScore (k_score_fused, t = 64) equals the multi-kernel path bit for bit over
   300 requests in two windows, and a single score equals its batched twin
@@ -41,6 +41,26 @@ def test_score_batch_equals_single_scores(study):
         got = r.score_batch(np.array(xy), np.array(ang))
         for k in range(15):
             ref = r.score(xy[k][0], xy[k][1], *ang[k])
+            assert tuple(got[k]) == tuple(np.float32(v) for v in ref), (k, got[k], ref)
+    finally:
+        r.close()
+
+
+def test_score_batch_more_windows_than_in_flight(study):
+    """Seven distinct windows with four slots: batches still in flight must drain before their
+    window slots are refilled. Fused path (t = 64, padding 1) against one score per request."""
+    print("case: score_batch vs score, 7 windows x 3 orientations, max_in_flight 4", flush=True)
+    m = study.micrograph(6)
+    r = TemplateRefiner(m, study.volume, RefineSettings(padding=1.0, max_in_flight=4),
+                        template_spectrum=study.tpl_spectrum)
+    try:
+        rng = np.random.default_rng(9)
+        wins = [(-150, 40), (12, -7), (200, 190), (-60, -180), (90, 33), (-210, 120), (0, 0)]
+        xy = np.array([w for w in wins for _ in range(3)])
+        ang = study.orients[rng.integers(len(study.orients), size=len(xy))]
+        got = r.score_batch(xy, ang)
+        for k in range(len(xy)):
+            ref = r.score(int(xy[k][0]), int(xy[k][1]), *ang[k])
             assert tuple(got[k]) == tuple(np.float32(v) for v in ref), (k, got[k], ref)
     finally:
         r.close()
