@@ -123,7 +123,12 @@ def main():
     ap.add_argument("--graph", action="store_true",
                     help="also time the same steps with each plane replayed from a captured hipGraph")
     ap.add_argument("--autotune", type=str, default="",
-                    help="comma-separated batch candidates: tm_autotune_batch picks the batch before timing")
+                    help="comma-separated batch candidates: tm_autotune picks the batch (and, with "
+                         "--autotune-columns, the column pass) before timing")
+    ap.add_argument("--autotune-columns", type=str, default="",
+                    help="comma-separated column-pass candidates for --autotune (pairs,persistent,persistent4)")
+    ap.add_argument("--column-pass", choices=["auto", "pairs", "persistent", "persistent4"], default="auto",
+                    help="column-transform kernel at height 4096 (auto: the persistent pass with 4-column tiles)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-psi", type=int, default=8)
@@ -200,7 +205,7 @@ def main():
                          padding=params["padding"], defocus_search_range=params["def_range"],
                          defocus_step=params["def_step"], first_search_position=0,
                          last_search_position=args.positions - 1, batch_size=args.batch,
-                         plane_storage=args.plane_storage)
+                         plane_storage=args.plane_storage, column_pass=args.column_pass)
     t0 = time.time()
     search = MT.MatchTemplateSearch(s.micrograph, s.volume, sp)
     log(f"prepared on GPU in {time.time() - t0:.1f}s: W={search.W} H={search.H} T={search.T} "
@@ -209,11 +214,19 @@ def main():
     if args.autotune:
         cands = [int(c) for c in args.autotune.split(",")]
         t0 = time.time()
-        best, us = search.autotune_batch(cands, sample_orientations=4 * max(cands))
-        args.batch = best
-        tuned = {"chosen": best, "us_per_ccf": {str(k): round(v, 3) for k, v in us.items()},
-                 "sample_orientations": 4 * max(cands), "seconds": round(time.time() - t0, 1)}
-        log(f"autotune: batch {best} ({tuned['us_per_ccf']} us per CCF)")
+        if args.autotune_columns:
+            cols = args.autotune_columns.split(",")
+            best, best_c, table = search.autotune(cands, cols, sample_orientations=4 * max(cands))
+            args.batch, args.column_pass = best, best_c
+            tuned = {"chosen": {"batch": best, "column_pass": best_c},
+                     "us_per_ccf": {f"{c}/{b}": (None if v is None else round(v, 3)) for (c, b), v in table.items()},
+                     "sample_orientations": 4 * max(cands), "seconds": round(time.time() - t0, 1)}
+        else:
+            best, us = search.autotune_batch(cands, sample_orientations=4 * max(cands))
+            args.batch = best
+            tuned = {"chosen": best, "us_per_ccf": {str(k): round(v, 3) for k, v in us.items()},
+                     "sample_orientations": 4 * max(cands), "seconds": round(time.time() - t0, 1)}
+        log(f"autotune: {tuned['chosen']} ({tuned['us_per_ccf']} us per CCF)")
     g = search.core
     planes = list(range(-search.n_def_half, search.n_def_half + 1))
     filters = {d: search.filter_for(d) for d in planes}
@@ -322,7 +335,7 @@ def main():
         "config": {"workload": workload + ("" if args.plane_storage == "fp32" else "_" + args.plane_storage + "_plane"),
                    "image": [W, H], "template": T, "padded_template": P,
                    "orientations_per_step": search.core.n_psi * (search.last - search.first + 1),
-                   "defocus_planes": len(planes), "batch": B,
+                   "defocus_planes": len(planes), "batch": B, "column_pass": search.core.column_pass_name(),
                    "precision": f"{args.plane_storage} storage of the inter-pass plane, fp32 spectra and "
                                 "butterflies, fp64 sum/sum-of-squares"},
         "roofline": roofline,

@@ -26,7 +26,7 @@ I64P = ctypes.POINTER(ctypes.c_longlong)
 # Every symbol include/tm_hip.h declares.
 EXPORTS = ["tm_device_count", "tm_create", "tm_destroy", "tm_init", "tm_run_inner_loop", "tm_get_results", "tm_merge",
            "tm_set_template", "tm_reset", "tm_finalize", "tm_debug_projection", "tm_debug_ccf", "tm_set_timing", "tm_get_timing",
-           "tm_last_error", "tm_autotune_batch", "tm_set_graph", "tm_set_input", "tm_set_micrograph",
+           "tm_last_error", "tm_autotune", "tm_autotune_batch", "tm_get_launch_shape", "tm_set_graph", "tm_set_input", "tm_set_micrograph",
            "tm_find_peaks", "tm_run_partial", "tm_state_size", "tm_export_state", "tm_import_state"]
 
 PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.c_longlong, ctypes.c_void_p)
@@ -47,7 +47,10 @@ class InitParams(ctypes.Structure):
                 ("histogram_number_of_bins", ctypes.c_int), ("max_padding", ctypes.c_int),
                 ("defocus_step", ctypes.c_float), ("pixel_size_step", ctypes.c_float),
                 ("batch_size", ctypes.c_int), ("plane_storage", ctypes.c_int),
-                ("orientation_limit", ctypes.c_int), ("device_memory_limit", ctypes.c_longlong)]
+                ("orientation_limit", ctypes.c_int), ("column_pass", ctypes.c_int),
+                ("device_memory_limit", ctypes.c_longlong)]
+
+COLUMN_PASS = {"auto": 0, "pairs": 1, "persistent": 2, "persistent4": 3}
 
 
 class PeakInfo(ctypes.Structure):
@@ -92,6 +95,9 @@ def lib():
         L.tm_export_state.argtypes = [vp, ctypes.c_void_p, ctypes.c_longlong]
         L.tm_import_state.argtypes = [vp, ctypes.c_void_p, ctypes.c_longlong]
         L.tm_get_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_char_p), F64P, I64P, ctypes.c_int]
+        L.tm_get_launch_shape.argtypes = [vp, I32P, I32P, I32P]
+        L.tm_autotune.argtypes = [ctypes.c_int, ctypes.POINTER(InitParams), F32P, F32P, F32P, F32P, F32P, I32P,
+                                  ctypes.c_int, I32P, ctypes.c_int, ctypes.c_int, I32P, I32P, F64P]
         L.tm_autotune_batch.argtypes = [ctypes.c_int, ctypes.POINTER(InitParams), F32P, F32P, F32P, F32P, F32P, I32P,
                                         ctypes.c_int, ctypes.c_int, I32P, F64P]
         _lib = L
@@ -133,6 +139,32 @@ def autotune_batch(params: InitParams, template_spectrum: np.ndarray, input_spec
     return best.value, {int(c): float(u) for c, u in zip(cand, us)}
 
 
+def autotune(params: InitParams, template_spectrum: np.ndarray, input_spectrum: np.ndarray,
+             search_positions: np.ndarray, psi_list: np.ndarray, projection_filter: np.ndarray,
+             batches=(32, 64, 96), columns=("pairs", "persistent", "persistent4"), sample_orientations: int = 1024,
+             device: int = 0):
+    """tm_autotune: times one plane of a sample of the search for every (column pass, batch depth).
+    Returns (best batch, best column pass name, {(column pass, batch): microseconds per CCF or None
+    where the pass does not apply})."""
+    tpl = np.ascontiguousarray(template_spectrum, np.complex64)
+    inp = np.ascontiguousarray(input_spectrum, np.complex64)
+    grid = np.ascontiguousarray(search_positions, np.float32)
+    psi = np.ascontiguousarray(psi_list, np.float32)
+    f = np.ascontiguousarray(projection_filter, np.complex64)
+    bc = np.ascontiguousarray(batches, np.int32)
+    cc = np.ascontiguousarray([COLUMN_PASS[c] for c in columns], np.int32)
+    us = np.zeros(bc.size * cc.size, np.float64)
+    best_b, best_c = ctypes.c_int(0), ctypes.c_int(0)
+    _check(lib().tm_autotune(device, ctypes.byref(params), _f32(tpl.view(np.float32)), _f32(inp.view(np.float32)),
+                             _f32(grid), _f32(psi), _f32(f.view(np.float32)), bc.ctypes.data_as(I32P), int(bc.size),
+                             cc.ctypes.data_as(I32P), int(cc.size), int(sample_orientations), ctypes.byref(best_b),
+                             ctypes.byref(best_c), us.ctypes.data_as(F64P)))
+    names = {v: k for k, v in COLUMN_PASS.items()}
+    table = {(c, int(b)): (None if us[ci * bc.size + bi] < 0 else float(us[ci * bc.size + bi]))
+             for ci, c in enumerate(columns) for bi, b in enumerate(bc)}
+    return best_b.value, names[best_c.value], table
+
+
 @dataclass
 class Results:
     mip: np.ndarray
@@ -172,14 +204,16 @@ class TemplateMatchingCore:
              last_search_position: int, histogram_min_scaled: float, histogram_step_scaled: float,
              histogram_number_of_bins: int = 512, max_padding: int = 0, defocus_step: float = 0.0,
              pixel_size_step: float = 0.0, batch_size: int = 0, plane_storage: str = "fp32",
-             orientation_limit: int = 0, device_memory_limit: int = 0) -> None:
+             orientation_limit: int = 0, device_memory_limit: int = 0, column_pass: str = "auto") -> None:
         """TemplateMatchingCore::Init (TemplateMatchingCore.cu:49-138).
 
         template_spectrum: (P, P, P/2+1) complex64; input_spectrum: (H, W/2+1) complex64;
         search_positions: (n, 2) float32 {phi, theta}; psi_list: float32 in-plane angles;
         plane_storage: "fp32" (default), "fp16" or "bf16" storage of the inter-pass plane;
         orientation_limit: search only the first N orientations of the range (0: all);
-        device_memory_limit: fail (TM_ERR_HIP) rather than allocate more device bytes (0: none)."""
+        device_memory_limit: fail (TM_ERR_HIP) rather than allocate more device bytes (0: none);
+        column_pass: "auto", "pairs", "persistent" or "persistent4" kernel for the column transforms at
+        height 4096."""
         tpl = np.ascontiguousarray(template_spectrum, np.complex64)
         inp = np.ascontiguousarray(input_spectrum, np.complex64)
         grid = np.ascontiguousarray(search_positions, np.float32)
@@ -190,7 +224,7 @@ class TemplateMatchingCore:
         p = InitParams(P, projection_size, W, H, grid.shape[0], psi.size, first_search_position,
                        last_search_position, histogram_min_scaled, histogram_step_scaled, histogram_number_of_bins,
                        max_padding, defocus_step, pixel_size_step, batch_size, PLANE_STORAGE[plane_storage],
-                       orientation_limit, device_memory_limit)
+                       orientation_limit, COLUMN_PASS[column_pass], device_memory_limit)
         _check(lib().tm_init(self.h, ctypes.byref(p), _f32(tpl.view(np.float32)), _f32(inp.view(np.float32)),
                              _f32(grid), _f32(psi)))
         self.W, self.H, self.T, self.P = W, H, projection_size, P
@@ -321,6 +355,16 @@ class TemplateMatchingCore:
     def set_graph(self, on: bool) -> None:
         """Replays one captured hipGraph per plane instead of launching kernel by kernel (tm_set_graph)."""
         _check(lib().tm_set_graph(self.h, int(on)))
+
+    def launch_shape(self) -> dict:
+        """tm_get_launch_shape: batch depth, column pass and Z tile width in use."""
+        b, c, z = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+        _check(lib().tm_get_launch_shape(self.h, ctypes.byref(b), ctypes.byref(c), ctypes.byref(z)))
+        names = {v: k for k, v in COLUMN_PASS.items()}
+        return {"batch": b.value, "column_pass": names[c.value], "z_tile": z.value}
+
+    def column_pass_name(self) -> str:
+        return self.launch_shape()["column_pass"]
 
     def timing(self) -> dict:
         cap = 16

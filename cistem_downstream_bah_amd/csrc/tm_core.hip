@@ -55,6 +55,9 @@ struct tm_core {
   tm_init_params prm{};
   int W = 0, H = 0, Wh = 0, T = 0, Th = 0, P = 0, Ph = 0, B = 16;
   int zst = TM_PLANE_FP32;  // storage of the inter-pass plane Z
+  int column_pass = TM_COLUMNS_AUTO;  // column pass in use (TM_COLUMNS_*)
+  int zt = 4;                         // column-tile width of Z
+  int persistent_grid = 512;          // blocks of the persistent column pass (two per CU, a multiple of 8)
   float zscale = 1.0f;      // power of two applied to 16-bit Z values before storing
   int n_pos_local = 0, n_psi = 0, n_orient = 0;
   std::vector<float> grid, psi;      // host copies ({phi, theta} per position; psi list)
@@ -168,18 +171,58 @@ void launch_mid_t(tm_core* h, int nb) {
 }
 
 // Columns per k_mid block for image height H, and the column-tile width of the Z layout
-// Z[b][tile][y][z_tile]. At H = 4096 two 2-column blocks fill one 4-column tile, so that a row
-// read of k_last takes 32 bytes from each cache line instead of 16.
+// Z[b][tile][y][z_tile] (the handle's h->zt, set at init by column_pass_for):
+//   * persistent pass at H = 4096, T <= 256 (k_mid4096p): 4-column tiles (the default; the halves
+//     written by two blocks at once merge in L2) or 2-column tiles, each written whole by one
+//     wavefront (faster columns, slower row reads);
+//   * one block per column pair at H = 4096 (k_mid4096): two 2-column blocks fill one 4-column tile,
+//     so that a row read of k_last takes 32 bytes from each cache line instead of 16;
+//   * other heights (k_mid): one tile per block.
 int mid_kc(int H) { return H == 4096 ? 2 : tmk::mid_cols(H); }
 int mid_tiles(const tm_core* h) { return (h->Wh + mid_kc(h->H) - 1) / mid_kc(h->H); }
-int z_tile(int H) { return H >= 4096 ? 4 : mid_kc(H); }
 // Whether the column pass places the projection rows at y = 0 (with the centring phase folded into
 // the input spectrum) rather than at H/2 - T/2.
 bool column_rows_at_zero(int H) { return H == 4096; }
-int z_tiles(const tm_core* h) { return (h->Wh + z_tile(h->H) - 1) / z_tile(h->H); }
+int z_tiles(const tm_core* h) { return (h->Wh + h->zt - 1) / h->zt; }
+
+// The column pass of a handle: the persistent pass with 4-column tiles where it applies (H = 4096,
+// T <= 256, FP32 planes) unless the caller asks for another; the Z tile width follows.
+void column_pass_for(tm_core* h, int requested) {
+  // 16-bit planes are read four elements per 16-byte load by the row pass, so their tiles are 4 wide
+  const bool persistent_ok = h->H == 4096 && h->T <= 256 && h->zst == TM_PLANE_FP32;
+  TM_REQUIRE(requested != TM_COLUMNS_PERSISTENT || persistent_ok, TM_ERR_UNSUPPORTED,
+             "the persistent column pass needs image height 4096, projection size <= 256 and FP32 planes");
+  TM_REQUIRE(requested != TM_COLUMNS_PAIRS || h->H == 4096, TM_ERR_UNSUPPORTED,
+             "the column-pair pass needs image height 4096");
+  TM_REQUIRE(requested != TM_COLUMNS_PERSISTENT4 || persistent_ok, TM_ERR_UNSUPPORTED,
+             "the persistent column pass needs image height 4096, projection size <= 256 and FP32 planes");
+  if (h->H != 4096) {
+    h->column_pass = TM_COLUMNS_AUTO;
+    h->zt = mid_kc(h->H);
+  } else if (requested == TM_COLUMNS_PAIRS || !persistent_ok) {
+    h->column_pass = TM_COLUMNS_PAIRS;
+    h->zt = 4;
+  } else if (requested == TM_COLUMNS_PERSISTENT) {
+    h->column_pass = TM_COLUMNS_PERSISTENT;
+    h->zt = 2;
+  } else {  // TM_COLUMNS_AUTO or TM_COLUMNS_PERSISTENT4: 4-column tiles keep the row pass's reads wide
+    h->column_pass = TM_COLUMNS_PERSISTENT4;
+    h->zt = 4;
+  }
+}
 
 template <int ZST>
 void launch_mid4096(tm_core* h, int nb) {
+  if (h->column_pass == TM_COLUMNS_PERSISTENT || h->column_pass == TM_COLUMNS_PERSISTENT4) {
+    if (h->column_pass == TM_COLUMNS_PERSISTENT)
+      hipLaunchKernelGGL((tmk::k_mid4096p<ZST, 2>), dim3(h->persistent_grid), dim3(512), 0, h->stream, h->rows.p,
+                         h->inpT.p, h->Z.p, h->Wh, h->T, nb, (h->Wh + 1) / 2, h->twiddle(4096), h->zscale);
+    else
+      hipLaunchKernelGGL((tmk::k_mid4096p<ZST, 4>), dim3(h->persistent_grid), dim3(512), 0, h->stream, h->rows.p,
+                         h->inpT.p, h->Z.p, h->Wh, h->T, nb, (h->Wh + 1) / 2, h->twiddle(4096), h->zscale);
+    check_launch("k_mid4096p");
+    return;
+  }
   const int ntiles = mid_tiles(h);
   const int grid = ((ntiles + 7) / 8) * 8;
   if (h->T <= 256)
@@ -213,7 +256,7 @@ void launch_mid(tm_core* h, int nb) {
 // groups are whole wavefronts and the row gather runs as double-buffered direct LDS loads.
 template <int WW, int QR>
 void launch_last_t(tm_core* h, int nb, tmk::EpiParams ep) {
-  ep.ztile = z_tile(h->H);
+  ep.ztile = h->zt;
   ep.ntiles = z_tiles(h);
   ep.hstep_inv = 1.0 / ep.hstep;
   ep.zinv = 1.0f / h->zscale;  // a power of two: exact
@@ -557,6 +600,12 @@ int tm_init(tm_core* h, const tm_init_params* p, const float* template_spectrum,
     h->zscale = h->zst == TM_PLANE_FP32 ? 1.0f
                                        : ldexpf(1.0f, (int)lround(0.5 * std::log2((double)W * W * H)) - 2);
     TM_REQUIRE(h->B <= 256, TM_ERR_INVALID, "batch_size must be <= 256");
+    TM_REQUIRE(p->column_pass >= TM_COLUMNS_AUTO && p->column_pass <= TM_COLUMNS_PERSISTENT4, TM_ERR_INVALID,
+               "column_pass must be 0, 1, 2 or 3");
+    column_pass_for(h, p->column_pass);
+    int cus = 0;
+    TM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
+    h->persistent_grid = std::max(8, (2 * cus) / 8 * 8);
     h->n_pos_local = p->last_search_position - p->first_search_position + 1;
     h->n_psi = p->n_psi;
     long long n_orient = (long long)h->n_pos_local * h->n_psi;
@@ -597,7 +646,7 @@ int tm_init(tm_core* h, const tm_init_params* p, const float* template_spectrum,
     h->slice.alloc((size_t)B * P * h->Ph);
     h->projT.alloc((size_t)B * T * h->Th);
     h->rows.alloc((size_t)B * T * h->Wh);
-    const size_t z_elems = (size_t)B * z_tiles(h) * H * z_tile(H);
+    const size_t z_elems = (size_t)B * z_tiles(h) * H * h->zt;
     h->Z.alloc(h->zst == TM_PLANE_FP32 ? z_elems : (z_elems + 1) / 2);  // cf units; 16-bit: 4 bytes each
 
     // state
@@ -1199,6 +1248,15 @@ int tm_set_graph(tm_core* h, int enabled) {
   });
 }
 
+int tm_get_launch_shape(tm_core* h, int* batch_size, int* column_pass, int* z_tile) {
+  return guarded([&] {
+    require_ready(h);
+    if (batch_size) *batch_size = h->B;
+    if (column_pass) *column_pass = h->column_pass;
+    if (z_tile) *z_tile = h->zt;
+  });
+}
+
 int tm_set_timing(tm_core* h, int enabled) {
   return guarded([&] {
     TM_REQUIRE(h, TM_ERR_INVALID, "handle is NULL");
@@ -1217,53 +1275,80 @@ int tm_get_timing(tm_core* h, const char** names, double* ms, long long* launche
   return n;
 }
 
-int tm_autotune_batch(int device_ordinal, const tm_init_params* params, const float* template_spectrum,
-                      const float* input_spectrum, const float* search_positions, const float* psi_list,
-                      const float* projection_filter, const int* candidates, int n_candidates, int sample_orientations,
-                      int* best_batch, double* us_per_ccf) {
+int tm_autotune(int device_ordinal, const tm_init_params* params, const float* template_spectrum,
+                const float* input_spectrum, const float* search_positions, const float* psi_list,
+                const float* projection_filter, const int* batch_candidates, int n_batch, const int* column_candidates,
+                int n_column, int sample_orientations, int* best_batch, int* best_column, double* us_per_ccf) {
   return guarded([&] {
     TM_REQUIRE(params && template_spectrum && input_spectrum && search_positions && psi_list && projection_filter &&
-                   candidates && best_batch,
+                   batch_candidates && column_candidates && best_batch && best_column,
                TM_ERR_INVALID, "NULL argument");
-    TM_REQUIRE(n_candidates > 0 && sample_orientations > 0, TM_ERR_INVALID,
-               "n_candidates and sample_orientations must be positive");
-    for (int c = 0; c < n_candidates; ++c)
-      TM_REQUIRE(candidates[c] >= 1 && candidates[c] <= 256, TM_ERR_INVALID, "batch candidates must be in 1..256");
+    TM_REQUIRE(n_batch > 0 && n_column > 0 && sample_orientations > 0, TM_ERR_INVALID,
+               "n_batch, n_column and sample_orientations must be positive");
+    for (int c = 0; c < n_batch; ++c)
+      TM_REQUIRE(batch_candidates[c] >= 1 && batch_candidates[c] <= 256, TM_ERR_INVALID,
+                 "batch candidates must be in 1..256");
+    for (int c = 0; c < n_column; ++c)
+      TM_REQUIRE(column_candidates[c] >= TM_COLUMNS_AUTO && column_candidates[c] <= TM_COLUMNS_PERSISTENT4,
+                 TM_ERR_INVALID, "column-pass candidates must be TM_COLUMNS_*");
     TM_REQUIRE(params->n_psi > 0 && params->first_search_position >= 0 &&
                    params->first_search_position <= params->last_search_position,
                TM_ERR_INVALID, "empty or invalid search position range");
-    // the sample: the first search positions of the range, enough for sample_orientations
+    // the sample: the first orientations of the range
     tm_init_params p = *params;
     const int npos = (sample_orientations + p.n_psi - 1) / p.n_psi;
     p.last_search_position = std::min(params->last_search_position, params->first_search_position + npos - 1);
+    p.orientation_limit = sample_orientations;
     auto ok = [](int rc) {
       if (rc != TM_OK) throw TmError{rc, g_err};
     };
     double best = 0.0;
-    int best_b = 0;
-    for (int c = 0; c < n_candidates; ++c) {
-      p.batch_size = candidates[c];
-      tm_core* h = nullptr;
-      ok(tm_create(device_ordinal, &h));
-      struct Drop {
-        tm_core* h;
-        ~Drop() { tm_destroy(h); }
-      } drop{h};
-      ok(tm_init(h, &p, template_spectrum, input_spectrum, search_positions, psi_list));
-      ok(tm_run_inner_loop(h, projection_filter, 0, 0, nullptr, nullptr, nullptr));  // warm-up plane
-      long long n = 0;
-      const auto t0 = std::chrono::steady_clock::now();
-      ok(tm_run_inner_loop(h, projection_filter, 0, 1, nullptr, nullptr, &n));
-      const double us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
-      const double per = us / (double)std::max(1LL, n);
-      if (us_per_ccf) us_per_ccf[c] = per;
-      if (best_b == 0 || per < best) {
-        best = per;
-        best_b = candidates[c];
+    int bb = 0, bc = -1;
+    for (int ci = 0; ci < n_column; ++ci) {
+      for (int bi = 0; bi < n_batch; ++bi) {
+        p.batch_size = batch_candidates[bi];
+        p.column_pass = column_candidates[ci];
+        double* slot = us_per_ccf ? us_per_ccf + (size_t)ci * n_batch + bi : nullptr;
+        tm_core* h = nullptr;
+        ok(tm_create(device_ordinal, &h));
+        struct Drop {
+          tm_core* h;
+          ~Drop() { tm_destroy(h); }
+        } drop{h};
+        const int rc = tm_init(h, &p, template_spectrum, input_spectrum, search_positions, psi_list);
+        if (rc == TM_ERR_UNSUPPORTED) {  // a column pass this search cannot use
+          if (slot) *slot = -1.0;
+          continue;
+        }
+        ok(rc);
+        ok(tm_run_inner_loop(h, projection_filter, 0, 0, nullptr, nullptr, nullptr));  // warm-up plane
+        long long n = 0;
+        const auto t0 = std::chrono::steady_clock::now();
+        ok(tm_run_inner_loop(h, projection_filter, 0, 1, nullptr, nullptr, &n));
+        const double us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+        const double per = us / (double)std::max(1LL, n);
+        if (slot) *slot = per;
+        if (bc < 0 || per < best) {
+          best = per;
+          bb = batch_candidates[bi];
+          bc = column_candidates[ci];
+        }
       }
     }
-    *best_batch = best_b;
+    TM_REQUIRE(bc >= 0, TM_ERR_UNSUPPORTED, "no candidate column pass applies to this search");
+    *best_batch = bb;
+    *best_column = bc;
   });
+}
+
+int tm_autotune_batch(int device_ordinal, const tm_init_params* params, const float* template_spectrum,
+                      const float* input_spectrum, const float* search_positions, const float* psi_list,
+                      const float* projection_filter, const int* candidates, int n_candidates, int sample_orientations,
+                      int* best_batch, double* us_per_ccf) {
+  int column = params ? params->column_pass : 0, best_column = 0;
+  return tm_autotune(device_ordinal, params, template_spectrum, input_spectrum, search_positions, psi_list,
+                     projection_filter, candidates, n_candidates, &column, 1, sample_orientations, best_batch,
+                     &best_column, us_per_ccf);
 }
 
 }  // extern "C"

@@ -146,14 +146,14 @@ __device__ __forceinline__ void lds_dma_barrier() {
   asm volatile("s_waitcnt vmcnt(0) lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-// LDS atomic increment without return (ds_add_u32). Written out because the compiler's wait
+// LDS atomic add without return (ds_add_u32). Written out because the compiler's wait
 // insertion cannot tell an atomic apart from a pending direct global-to-LDS load and would wait for
 // every such load (vmcnt 0) before each histogram update. The instruction has no output register;
 // LDS operations complete in order, so the compiler's own LDS waits stay correct, and an
 // lds_barrier() completes it before the counts are read.
-__device__ __forceinline__ void lds_inc(unsigned int* p) {
+__device__ __forceinline__ void lds_add(unsigned int* p, unsigned int v) {
   const unsigned addr = (unsigned)(size_t)((__attribute__((address_space(3))) unsigned int*)p);
-  asm volatile("ds_add_u32 %0, %1" ::"v"(addr), "v"(1u) : "memory");
+  asm volatile("ds_add_u32 %0, %1" ::"v"(addr), "v"(v) : "memory");
 }
 
 // twiddle W^(idx) from the table (forward table); conjugated for the inverse.

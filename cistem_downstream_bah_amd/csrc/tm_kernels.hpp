@@ -678,14 +678,19 @@ __global__ __launch_bounds__(KC * 256, 4) void k_mid4096(const cf* __restrict__ 
   constexpr int CS = lds_len(H) + 8;  // column stride: shifts the columns by 16 banks
   static_assert(KC * M <= 1024, "KC columns of 256 butterflies per block");
   __shared__ cf lds[KC * CS];
-  // Twiddles from LDS: W4096^(16 m) = W256^m and W4096^(b q) for b, q < 16; the stage-3 twiddle
-  // W4096^(j q) with j = 16 a + b is W256^(a q) * W4096^(b q).
+  // Twiddles from LDS: W4096^(16 m) = W256^m; the stage-2 twiddle W256^(k q) and the second factor
+  // of the stage-3 twiddle W4096^(j q) = W256^(a q) * W4096^(b q) (j = 16 a + b) are stored q-major,
+  // [q][k] and [q][b], so that the 16 lanes of a wave that differ in k (or b) read 16 consecutive
+  // entries (distinct banks) rather than entries 16 apart (one bank).
   __shared__ cf tw256[256];
-  __shared__ cf twbq[256];
+  __shared__ cf tw2q[256];
+  __shared__ cf tw3q[256];
   const int tid = threadIdx.x;
   if (tid < 256) {
+    const int q = tid / 16, k = tid % 16;
     tw256[tid] = tw[16 * tid];
-    twbq[tid] = tw[(tid / 16) * (tid % 16)];
+    tw2q[tid] = tw[16 * ((k * q) % 256)];
+    tw3q[tid] = tw[k * q];
   }
   lds_barrier();
   const int c = tid % KC;
@@ -745,7 +750,7 @@ __global__ __launch_bounds__(KC * 256, 4) void k_mid4096(const cf* __restrict__ 
       lds_barrier();
     }
 #pragma unroll
-    for (int q = 1; q < 16; ++q) v[q] = cmul(v[q], twiddle<-1>(tw256, k2 * q));
+    for (int q = 1; q < 16; ++q) v[q] = cmul(v[q], twiddle<-1>(tw2q, 16 * q + k2));
     dft16<-1>(v);
 #pragma unroll
     for (int q = 0; q < 16; ++q) p_s2[17 * q] = v[q];
@@ -762,7 +767,7 @@ __global__ __launch_bounds__(KC * 256, 4) void k_mid4096(const cf* __restrict__ 
     for (int q = 0; q < 16; ++q) v[q] = p_rd[272 * q];
 #pragma unroll
     for (int q = 1; q < 16; ++q)
-      v[q] = cmul(v[q], twiddle<-1>(tw256, (j / 16) * q), twiddle<-1>(twbq, k2 * 16 + q));
+      v[q] = cmul(v[q], twiddle<-1>(tw256, (j / 16) * q), twiddle<-1>(tw3q, 16 * q + k2));
     dft16<-1>(v);
     // ZeroCentralPixel and conj(reference) * input
 #pragma unroll
@@ -782,7 +787,7 @@ __global__ __launch_bounds__(KC * 256, 4) void k_mid4096(const cf* __restrict__ 
     for (int q = 0; q < 16; ++q) v[q] = p_rd[272 * q];
     lds_barrier();
 #pragma unroll
-    for (int q = 1; q < 16; ++q) v[q] = cmul(v[q], twiddle<+1>(tw256, k2 * q));
+    for (int q = 1; q < 16; ++q) v[q] = cmul(v[q], twiddle<+1>(tw2q, 16 * q + k2));
     dft16<+1>(v);
 #pragma unroll
     for (int q = 0; q < 16; ++q) p_s2[17 * q] = v[q];
@@ -792,7 +797,7 @@ __global__ __launch_bounds__(KC * 256, 4) void k_mid4096(const cf* __restrict__ 
     for (int q = 0; q < 16; ++q) v[q] = p_rd[272 * q];
 #pragma unroll
     for (int q = 1; q < 16; ++q)
-      v[q] = cmul(v[q], twiddle<+1>(tw256, (j / 16) * q), twiddle<+1>(twbq, k2 * 16 + q));
+      v[q] = cmul(v[q], twiddle<+1>(tw256, (j / 16) * q), twiddle<+1>(tw3q, 16 * q + k2));
     dft16<+1>(v);
     const size_t zoff = b * z_bs + (size_t)(kx / ZT) * H * ZT + (size_t)j * ZT + kx % ZT;  // elements
     if constexpr (ZST == ZS_F32) {
@@ -817,6 +822,157 @@ __global__ __launch_bounds__(KC * 256, 4) void k_mid4096(const cf* __restrict__ 
   }
 }
 
+// Persistent column pass for H = 4096 and T <= 256 (the arithmetic of k_mid4096<2, *, *, *, true>,
+// bit for bit). The grid is the number of co-resident blocks (two per CU); the work is a list of
+// units (orientation b, pair of columns) in orientation-major order, cut into eight contiguous
+// ranges, one per XCD (workgroups are dispatched round-robin over the XCDs, so block bid runs on
+// XCD bid % 8), which its blocks take in turn. The blocks of an XCD therefore work on neighbouring
+// column pairs of one orientation at a time: they share that orientation's forward rows in L2,
+// and a unit's Z writes never wait on a slow block. Three orderings keep memory latency off the
+// critical path:
+//  * Z uses tiles of two columns (ZT = 2), so one wavefront's 16 stores each write 32 rows x 16 B,
+//    whole 128-byte lines, with no partner block to merge with;
+//  * the input-spectrum column is loaded at the top of the unit, beside the rows, and its latency
+//    hides behind forward stages 2 and 3;
+//  * a unit's Z stores are issued after the next unit's row and input-spectrum loads (ROT): vmcnt
+//    counts loads and stores in issue order, so a wait for loads issued after the stores would also
+//    wait for the stores to be acknowledged.
+// With ZT = 4 (the k_mid4096 layout) the two column pairs of a tile are neighbouring units, run at
+// the same time by two blocks of one XCD; their half rows are then stored through L2 (not
+// non-temporal), where they merge into whole lines, and are issued in place (ROT off).
+template <int ZST = ZS_F32, int ZT = 2>
+__global__ __launch_bounds__(512, 4) void k_mid4096p(const cf* __restrict__ rows, const cf* __restrict__ inpT,
+                                                     cf* __restrict__ Z, int Wh, int T, int nb, int npairs,
+                                                     const cf* __restrict__ tw, float zscale) {
+  constexpr int H = 4096, M = 256, KC = 2;
+  constexpr bool NTS = ZT == 2, ROT = ZT == 2;
+  static_assert(ZT == 2 || ZT == 4, "Z tiles of 2 or 4 columns");
+  constexpr int CS = lds_len(H) + 8;  // column stride: shifts the columns by 16 banks
+  __shared__ cf lds[KC * CS];
+  __shared__ cf tw256[256];
+  __shared__ cf tw2q[256];
+  __shared__ cf tw3q[256];
+  const int tid = threadIdx.x;
+  if (tid < 256) {
+    const int q = tid / 16, k = tid % 16;
+    tw256[tid] = tw[16 * tid];
+    tw2q[tid] = tw[16 * ((k * q) % 256)];
+    tw3q[tid] = tw[k * q];
+  }
+  lds_barrier();
+  const int c = tid % KC;
+  const long long U = (long long)npairs * nb;
+  const int x = blockIdx.x % 8, nbx = gridDim.x / 8, l = blockIdx.x / 8;
+  const long long u_lo = U * x / 8, u_hi = U * (x + 1) / 8;
+  const size_t z_bs = (size_t)((Wh + ZT - 1) / ZT) * H * ZT;
+  cf zp[16];  // the previous unit's Z column segment, stored once the next unit's loads are issued
+  size_t zoff_prev = 0;
+  bool have_prev = false;
+  auto store_z = [&](const cf* val, size_t zoff) {
+    if constexpr (ZST == ZS_F32) {
+      unsigned long long* zt = reinterpret_cast<unsigned long long*>(Z) + zoff;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        unsigned long long bits;
+        __builtin_memcpy(&bits, &val[q], sizeof(bits));
+        if constexpr (NTS) __builtin_nontemporal_store(bits, zt + (size_t)M * ZT * q);
+        else zt[(size_t)M * ZT * q] = bits;
+      }
+    } else {
+      unsigned* zt = reinterpret_cast<unsigned*>(Z) + zoff;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        if constexpr (NTS) __builtin_nontemporal_store(pack_half2<ZST>(val[q], zscale), zt + (size_t)M * ZT * q);
+        else zt[(size_t)M * ZT * q] = pack_half2<ZST>(val[q], zscale);
+      }
+    }
+  };
+  for (long long u = u_lo + l; u < u_hi; u += nbx) {
+    const int b = (int)(u / npairs), pair = (int)(u - (long long)b * npairs);
+    const int kx = pair * KC + c;
+    const bool valid = kx < Wh;
+    // re-derived every unit from a register the compiler cannot see through, so the address and
+    // twiddle computations are not hoisted out of the loop (they would spill)
+    int j = tid / KC;
+    asm volatile("" : "+v"(j));
+    const int k2 = j % 16;
+    cf* const colbuf = lds + c * CS;
+    cf* const p_s1 = colbuf + 17 * j;                         // stage-1 stores: + q
+    cf* const p_rd = colbuf + j + (j >> 4);                   // strided reads:  + 272 q
+    cf* const p_s2 = colbuf + (j / 16) * 272 + k2;            // stage-2 stores: + 17 q
+    const cf* ic = inpT + (size_t)(valid ? kx : 0) * H + j;   // input spectrum: + 256 q
+    const cf* rb = rows + (size_t)b * T * Wh + kx;
+    // forward stage 1 (Ns = 1) has the single non-zero input x[j] (T <= 256), so stage 2 butterfly j
+    // takes x[j / 16 + 16 q] straight from the rows (as k_mid4096 with DIRECT)
+    cf v[16], iv[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int r = j / 16 + 16 * q;
+      v[q] = (valid && r < T) ? rb[(size_t)r * Wh] : cmake(0.f, 0.f);
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) iv[q] = valid ? ic[M * q] : cmake(0.f, 0.f);
+    if constexpr (ROT) {
+      asm volatile("" ::: "memory");  // the loads above are issued before the stores below
+      if (have_prev) store_z(zp, zoff_prev);
+    }
+    // forward stage 2 (Ns = 16)
+#pragma unroll
+    for (int q = 1; q < 16; ++q) v[q] = cmul(v[q], twiddle<-1>(tw2q, 16 * q + k2));
+    dft16<-1>(v);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) p_s2[17 * q] = v[q];
+    lds_barrier();
+    // forward stage 3 (Ns = 256): X[j + 256 q]
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = p_rd[272 * q];
+#pragma unroll
+    for (int q = 1; q < 16; ++q)
+      v[q] = cmul(v[q], twiddle<-1>(tw256, (j / 16) * q), twiddle<-1>(tw3q, 16 * q + k2));
+    dft16<-1>(v);
+    // ZeroCentralPixel and conj(reference) * input
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const cf xx = (kx == 0 && j == 0 && q == 0) ? cmake(0.f, 0.f) : v[q];
+      v[q] = cconjmul(xx, iv[q]);
+    }
+    lds_barrier();
+    // inverse stage 1
+    dft16<+1>(v);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) p_s1[q] = v[q];
+    lds_barrier();
+    // inverse stage 2
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = p_rd[272 * q];
+    lds_barrier();
+#pragma unroll
+    for (int q = 1; q < 16; ++q) v[q] = cmul(v[q], twiddle<+1>(tw2q, 16 * q + k2));
+    dft16<+1>(v);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) p_s2[17 * q] = v[q];
+    lds_barrier();
+    // inverse stage 3: z[j + 256 q]
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = p_rd[272 * q];
+#pragma unroll
+    for (int q = 1; q < 16; ++q)
+      v[q] = cmul(v[q], twiddle<+1>(tw256, (j / 16) * q), twiddle<+1>(tw3q, 16 * q + k2));
+    dft16<+1>(v);
+    const size_t zoff = b * z_bs + (size_t)(kx / ZT) * H * ZT + (size_t)j * ZT + kx % ZT;
+    if constexpr (ROT) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) zp[q] = v[q];
+      zoff_prev = zoff;
+      have_prev = true;
+    } else {
+      store_z(v, zoff);
+    }
+    lds_barrier();  // LDS reads of this unit complete before the next one writes
+  }
+  if (ROT && have_prev) store_z(zp, zoff_prev);
+}
+
 // ---------------------------------------------------------------- row pass + epilogue
 struct EpiParams {
   float* mip;
@@ -833,17 +989,24 @@ struct EpiParams {
   float zinv;  // inverse of the scale of 16-bit Z storage (1 for FP32)
 };
 
-// Histogram bin of a correlation value as match_template.cpp:1124 computes it,
-// int(double(v - histogram_min_scaled) / histogram_step_scaled) with the min and step in double, or
-// -1 outside bins 0..nbins-1 (the reference's guard admits bin nbins, one past its array; here it is
-// dropped). The quotient is formed with the reciprocal of the step, which can differ from the
-// correctly rounded division by an ulp; within 1e-9 of an integer (where that ulp could cross a bin
-// edge) the division itself is done, so the bin always equals the reference expression's.
-__device__ __forceinline__ int hist_bin(float v, const EpiParams& ep) {
-  const double t = (double)v - ep.hmin;
-  double d = t * ep.hstep_inv;
-  if (__builtin_expect(fabs(d - rint(d)) < 1e-9, 0)) d = t / ep.hstep;  // rare: kept as a branch
-  return (d > -1.0 && d < (double)ep.nbins) ? (int)d : -1;
+// Histogram bins of correlation values as match_template.cpp:1124 computes them,
+// int(double(v - histogram_min_scaled) / histogram_step_scaled) with the min and step in double.
+// Bins outside 0..nbins-1 go to the spill bin nbins (NBIN_SPILL slots are reserved for it), which is
+// never flushed: the reference's guard admits bin nbins, one past its array; here it is dropped.
+// The quotient is formed with the reciprocal of the step, which can differ from the correctly
+// rounded division by an ulp; bin_near_edge flags a quotient within 1e-9 of an integer (where that
+// ulp could cross a bin edge), and the caller then moves the counts of values whose bin_exact
+// differs, so every bin equals the reference expression's. A quotient in (-1, 0) truncates to bin 0, as in the reference.
+__device__ __forceinline__ double bin_quotient(float v, const EpiParams& ep) {
+  return ((double)v - ep.hmin) * ep.hstep_inv;
+}
+__device__ __forceinline__ bool bin_near_edge(double d) { return fabs(d - rint(d)) < 1e-9; }
+__device__ __forceinline__ int bin_of(double d, const EpiParams& ep) {
+  const int k = (int)d;  // truncation, as int() in the reference
+  return (unsigned)k < (unsigned)ep.nbins ? k : ep.nbins;
+}
+__device__ __forceinline__ int bin_exact(float v, const EpiParams& ep) {
+  return bin_of(((double)v - ep.hmin) / ep.hstep, ep);
 }
 
 // Threads of a k_last block: QR orientations x (W/2) / R1 threads, R1 the first radix of the
@@ -855,7 +1018,7 @@ __host__ __device__ constexpr int last_threads() { return QR * ((W / 2) / plan_r
 // orientations of the batch at a time: C2R of row y of Z into W reals (an h = W/2 point complex
 // inverse FFT after the even/odd pre-twist), then every thread updates its 2*PAIRS pixels for
 // those orientations in batch order: strict-greater MIP/argmax, histogram bin
-// int((v - min) / step) over 0..nbins-1, FP64 sum += v and FP64 sum-of-squares += (v*v in float)
+// int(double(v - min) / step) over 0..nbins-1, FP64 sum += v and FP64 sum-of-squares += (v*v in float)
 // (match_template.cpp:1100-1149). The per-pixel state stays in registers for the whole batch.
 // Rows are assigned so that consecutive rows run on the same XCD and share Z's cache lines in L2.
 // Twiddles live in LDS; the block is sized so that two blocks share a CU and overlap each other's
@@ -874,12 +1037,12 @@ void k_last(const cf* __restrict__ Z, int H, int nb, EpiParams ep, const cf* __r
   __shared__ cf buf[QR][lds_len(h + 1)];
   __shared__ cf twh[h];
   __shared__ cf twn[h / 2 + 1];
-  __shared__ unsigned int hist[512 * HC];
+  __shared__ unsigned int hist[513 * HC];  // bins 0..511 and the spill bin
   const int tid0 = threadIdx.x;
   const int bid = blockIdx.x;
   const int y = xcd_row(bid, H);
   const int gbase = ep.g0 + (ep.plane_g0 ? *ep.plane_g0 : 0);
-  for (int i = tid0; i < 512 * HC; i += BT) hist[i] = 0u;
+  for (int i = tid0; i < 513 * HC; i += BT) hist[i] = 0u;
   for (int i = tid0; i < h; i += BT) twh[i] = tw_h[i];
   for (int i = tid0; i <= h / 2; i += BT) twn[i] = tw_n[i];
   float mip[2 * PAIRS];
@@ -955,6 +1118,7 @@ void k_last(const cf* __restrict__ Z, int H, int nb, EpiParams ep, const cf* __r
 #pragma unroll 1
     for (int q = 0; q < nbp; ++q) {
       const int g = gbase + b0 + q;
+      bool near = false;
 #pragma unroll
       for (int u = 0; u < PAIRS; ++u) {
         const cf z = buf[q][lpad(tid + u * BT)];
@@ -964,11 +1128,25 @@ void k_last(const cf* __restrict__ Z, int H, int nb, EpiParams ep, const cf* __r
           const float vx = vv[c];
           const int p = 2 * u + c;
           if (vx > mip[p]) { mip[p] = vx; idx[p] = g; }
-          const int bin = hist_bin(vx, ep);
-          if (bin >= 0) atomicAdd(&hist[bin * HC + tid % HC], 1u);
+          const double d = bin_quotient(vx, ep);
+          atomicAdd(&hist[bin_of(d, ep) * HC + tid % HC], 1u);
+          near |= bin_near_edge(d);
           s1[p] += (double)vx;
           const float sq = vx * vx;
           s2[p] += (double)sq;
+        }
+      }
+      if (__builtin_expect(near, 0)) {
+        // rare: move each value whose exact quotient falls in another bin
+#pragma unroll 1
+        for (int p = 0; p < 2 * PAIRS; ++p) {
+          const cf z = buf[q][lpad(tid + (p >> 1) * BT)];
+          const float vx = (p & 1) ? z.y : z.x;
+          const int a = bin_of(bin_quotient(vx, ep), ep), e = bin_exact(vx, ep);
+          if (a != e) {
+            atomicSub(&hist[a * HC + tid % HC], 1u);
+            atomicAdd(&hist[e * HC + tid % HC], 1u);
+          }
         }
       }
     }
@@ -1020,7 +1198,7 @@ void k_last_dma(const cf* __restrict__ Z, int H, int nb, EpiParams ep, const cf*
   __shared__ cf buf[QR][lds_len(h)];       // FFT work buffers (padded index)
   __shared__ cf fft_lo[64], fft_hi[h / 64];
   __shared__ cf pre_lo[64], pre_hi[h / 128 + 1];
-  __shared__ unsigned int hist[512 * HC];
+  __shared__ unsigned int hist[513 * HC];  // bins 0..511 and the spill bin
   const int tid0 = threadIdx.x;
   const int bid = blockIdx.x;
   const int y = xcd_row(bid, H);
@@ -1053,7 +1231,7 @@ void k_last_dma(const cf* __restrict__ Z, int H, int nb, EpiParams ep, const cf*
     else return unpack_half2<ZST>(reinterpret_cast<const unsigned*>(gb)[k], ep.zinv);
   };
   gather(0, tid0);
-  for (int i = tid0; i < 512 * HC; i += BT) hist[i] = 0u;
+  for (int i = tid0; i < 513 * HC; i += BT) hist[i] = 0u;
   for (int i = tid0; i < 64; i += BT) {
     fft_lo[i] = tw_h[i];
     pre_lo[i] = tw_n[i];
@@ -1130,6 +1308,7 @@ void k_last_dma(const cf* __restrict__ Z, int H, int nb, EpiParams ep, const cf*
 #pragma unroll 1
     for (int q = 0; q < nbp; ++q) {
       const int g = gbase + b0 + q;
+      bool near = false;
 #pragma unroll
       for (int u = 0; u < PAIRS; ++u) {
         const cf z = buf[q][lpad(tid + u * BT)];
@@ -1139,11 +1318,25 @@ void k_last_dma(const cf* __restrict__ Z, int H, int nb, EpiParams ep, const cf*
           const float vx = vv[c];
           const int p = 2 * u + c;
           if (vx > mip[p]) { mip[p] = vx; idx[p] = g; }
-          const int bin = hist_bin(vx, ep);
-          if (bin >= 0) lds_inc(&hist[bin * HC + tid % HC]);
+          const double d = bin_quotient(vx, ep);
+          lds_add(&hist[bin_of(d, ep) * HC + tid % HC], 1u);
+          near |= bin_near_edge(d);
           s1[p] += (double)vx;
           const float sq = vx * vx;
           s2[p] += (double)sq;
+        }
+      }
+      if (__builtin_expect(near, 0)) {
+        // rare: move each value whose exact quotient falls in another bin
+#pragma unroll 1
+        for (int p = 0; p < 2 * PAIRS; ++p) {
+          const cf z = buf[q][lpad(tid + (p >> 1) * BT)];
+          const float vx = (p & 1) ? z.y : z.x;
+          const int a = bin_of(bin_quotient(vx, ep), ep), e = bin_exact(vx, ep);
+          if (a != e) {
+            lds_add(&hist[a * HC + tid % HC], 0xFFFFFFFFu);  // minus one, modulo 2^32
+            lds_add(&hist[e * HC + tid % HC], 1u);
+          }
         }
       }
     }

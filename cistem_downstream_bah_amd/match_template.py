@@ -139,6 +139,7 @@ class SearchParams:
     last_search_position: int = -1
     batch_size: int = 0
     plane_storage: str = "fp32"  # "fp16"/"bf16": 16-bit inter-pass plane (image height 4096 only)
+    column_pass: str = "auto"    # column-transform kernel at image height 4096 (core.COLUMN_PASS)
 
 
 class MatchTemplateSearch:
@@ -182,7 +183,8 @@ class MatchTemplateSearch:
         self.core = core.TemplateMatchingCore(device)
         self.core.init(self.template_spectrum, self.input_spectrum, T, self.grid, self.psi, self.first, self.last,
                        self.histogram_min_scaled, self.histogram_step_scaled, 512, self.max_padding,
-                       self.defocus_step, self.pixel_size_step, p.batch_size, p.plane_storage)
+                       self.defocus_step, self.pixel_size_step, p.batch_size, p.plane_storage,
+                       column_pass=p.column_pass)
 
     def set_micrograph(self, micrograph: np.ndarray, on_device: bool = False) -> None:
         """Searches another micrograph of the same size with this handle (tm_set_input): the template,
@@ -210,7 +212,7 @@ class MatchTemplateSearch:
                                self.psi.size, self.first, self.last, self.histogram_min_scaled,
                                self.histogram_step_scaled, 512, self.max_padding, self.defocus_step,
                                self.pixel_size_step, p.batch_size if batch_size is None else batch_size,
-                               core.PLANE_STORAGE[p.plane_storage])
+                               core.PLANE_STORAGE[p.plane_storage], 0, core.COLUMN_PASS[p.column_pass], 0)
 
     def autotune_batch(self, candidates=(16, 32, 64, 128), sample_orientations: int = 1024):
         """Times a sample of this search at each candidate batch depth (tm_autotune_batch) and
@@ -220,8 +222,22 @@ class MatchTemplateSearch:
         self.p = dataclasses.replace(self.p, batch_size=best)
         self.core.init(self.template_spectrum, self.input_spectrum, self.T, self.grid, self.psi, self.first, self.last,
                        self.histogram_min_scaled, self.histogram_step_scaled, 512, self.max_padding,
-                       self.defocus_step, self.pixel_size_step, best, self.p.plane_storage)
+                       self.defocus_step, self.pixel_size_step, best, self.p.plane_storage,
+                       column_pass=self.p.column_pass)
         return best, us
+
+    def autotune(self, batches=(32, 64, 96), columns=("pairs", "persistent", "persistent4"),
+                 sample_orientations: int = 1024):
+        """Times a sample of this search at each (column pass, batch depth) (tm_autotune) and
+        re-initialises the search at the fastest. Returns (batch, column pass, table of us per CCF)."""
+        best_b, best_c, table = core.autotune(self.init_params(), self.template_spectrum, self.input_spectrum,
+                                              self.grid, self.psi, self.filter_for(0), batches, columns,
+                                              sample_orientations, self.device)
+        self.p = dataclasses.replace(self.p, batch_size=best_b, column_pass=best_c)
+        self.core.init(self.template_spectrum, self.input_spectrum, self.T, self.grid, self.psi, self.first, self.last,
+                       self.histogram_min_scaled, self.histogram_step_scaled, 512, self.max_padding,
+                       self.defocus_step, self.pixel_size_step, best_b, self.p.plane_storage, column_pass=best_c)
+        return best_b, best_c, table
 
     def filter_for(self, defocus_i: int) -> np.ndarray:
         p = self.p

@@ -56,6 +56,16 @@ typedef struct tm_init_params {
                                     N in loop order (position-major, psi inner), the last position
                                     possibly cut short (a search truncated to its first N
                                     orientations). No counterpart in the reference.                */
+  int column_pass;               /* kernel of the column transforms at image height 4096 (no
+                                    counterpart in the reference; results are bit-identical):
+                                    TM_COLUMNS_AUTO (0): the persistent pass with 4-column Z tiles
+                                    where it applies (projection size <= 256, FP32 planes), else
+                                    one block per column pair;
+                                    TM_COLUMNS_PAIRS (1): one block per column pair;
+                                    TM_COLUMNS_PERSISTENT (2): the persistent pass with 2-column Z
+                                    tiles; TM_COLUMNS_PERSISTENT4 (3): the persistent pass with the
+                                    4-column tiles of TM_COLUMNS_PAIRS (both TM_ERR_UNSUPPORTED
+                                    where they do not apply). Ignored at other heights.            */
   long long device_memory_limit; /* 0: none; else tm_init fails with TM_ERR_HIP, freeing what it
                                     allocated and leaving the handle uninitialised, when its device
                                     allocations would exceed this many bytes, as when hipMalloc runs
@@ -65,6 +75,11 @@ typedef struct tm_init_params {
 #define TM_PLANE_FP32 0
 #define TM_PLANE_FP16 1
 #define TM_PLANE_BF16 2
+
+#define TM_COLUMNS_AUTO 0
+#define TM_COLUMNS_PAIRS 1
+#define TM_COLUMNS_PERSISTENT 2
+#define TM_COLUMNS_PERSISTENT4 3
 
 /* Number of visible HIP devices. Replaces DeviceManager::Init's device query
  * (src/gpu/DeviceManager.cu:27-124). */
@@ -215,17 +230,30 @@ int tm_import_state(tm_core* h, const void* buffer, long long bytes);
  * Not used while timing is enabled. tm_init, and enabling or disabling, drop the captured graph. */
 int tm_set_graph(tm_core* h, int enabled);
 
+/* The launch shape tm_init chose: orientations per launch, the column pass in use (TM_COLUMNS_PAIRS
+ * or TM_COLUMNS_PERSISTENT* at height 4096, TM_COLUMNS_AUTO elsewhere) and the column-tile width of
+ * the inter-pass plane. Any pointer may be NULL. No counterpart in the reference. */
+int tm_get_launch_shape(tm_core* h, int* batch_size, int* column_pass, int* z_tile);
+
 /* Last error message of the calling thread. */
 const char* tm_last_error(void);
 
-/* Autotuner of the batch depth (orientations per launch), with no counterpart in the reference.
- * For each candidate batch size it builds a handle over a sample of the search (the first search
- * positions of params' range, at least sample_orientations orientations), runs one warm-up plane
- * and one timed plane with projection_filter, and writes the time per CCF in microseconds to
- * us_per_ccf[c] (may be NULL). *best_batch receives the fastest candidate (the earlier one on a
- * tie). The batch depth changes no result: each pixel's state is updated in orientation order
- * whatever the batch, so every candidate gives bit-identical maps, sums and histograms. Kernel tile
- * shapes are fixed per compiled size (they are template parameters), not tuned here. */
+/* Autotuner of the launch shape, with no counterpart in the reference: the batch depth
+ * (orientations per launch) and the column-pass kernel (TM_COLUMNS_*: one block per column pair with
+ * 4-column Z tiles, or the persistent pass with 2- or 4-column tiles). For each (column pass, batch)
+ * candidate it builds a handle over a sample of the search (its first sample_orientations
+ * orientations), runs one warm-up plane and one timed plane with projection_filter, and writes the
+ * time per CCF in microseconds to us_per_ccf[column * n_batch + batch] (-1 where the column pass
+ * does not apply to the search; us_per_ccf may be NULL). *best_batch and *best_column receive the
+ * fastest pair (the earlier one on a tie). No candidate changes a result: each pixel's state is
+ * updated in orientation order whatever the batch, and every column pass does the same arithmetic,
+ * so all candidates give bit-identical maps, sums and histograms. */
+int tm_autotune(int device_ordinal, const tm_init_params* params, const float* template_spectrum,
+                const float* input_spectrum, const float* search_positions, const float* psi_list,
+                const float* projection_filter, const int* batch_candidates, int n_batch, const int* column_candidates,
+                int n_column, int sample_orientations, int* best_batch, int* best_column, double* us_per_ccf);
+
+/* tm_autotune over batch candidates only, with params->column_pass as the one column pass. */
 int tm_autotune_batch(int device_ordinal, const tm_init_params* params, const float* template_spectrum,
                       const float* input_spectrum, const float* search_positions, const float* psi_list,
                       const float* projection_filter, const int* candidates, int n_candidates, int sample_orientations,

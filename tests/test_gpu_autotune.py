@@ -1,4 +1,4 @@
-"""GPU: the batch-depth autotuner (tm_autotune_batch; no counterpart in the reference).
+"""GPU: the launch-shape autotuner (tm_autotune, tm_autotune_batch; no counterpart in the reference).
 
 * the batch depth changes no result: config 0 searched at batch 1, 7, 64 and 256 gives bit-identical
   MIP, best index, FP64 sum and sum of squares, and histogram (each pixel's state is updated in
@@ -6,6 +6,11 @@
 * the tuner times every candidate, returns the fastest, and re-initialises the search at it; the
   tuned search then gives the same bit-identical results
 * invalid candidates are rejected with an error, not clamped
+* at 4096^2 (config-1 micrograph and template, position 3, 241 psi) the three column passes (one
+  block per column pair with 4-column Z tiles; the persistent pass with 2- and 4-column tiles) at
+  batch 64 and 41 give bit-identical MIP, index, sums and histogram; tm_autotune over (column pass,
+  batch) times every pair, picks the fastest and the search re-initialised at it gives the same maps;
+  a column pass that does not apply (persistent with 16-bit planes) is reported as such
 """
 import numpy as np
 import pytest
@@ -71,3 +76,56 @@ def test_autotune_rejects_bad_candidates(c0):
         search.autotune_batch((64, 300))
     with pytest.raises(core.TMError):
         search.autotune_batch((0,))
+
+
+@pytest.fixture(scope="module")
+def c1():
+    return synth.config1(1000)
+
+
+def _search4096(s, batch, column_pass, plane_storage="fp32"):
+    return MT.MatchTemplateSearch(s.micrograph, s.volume,
+                                  MT.SearchParams(angular_step=2.5, in_plane_angular_step=1.5, padding=2.0,
+                                                  first_search_position=3, last_search_position=3,
+                                                  batch_size=batch, column_pass=column_pass,
+                                                  plane_storage=plane_storage))
+
+
+def test_column_passes_change_no_result(c1):
+    ref = None
+    for cp, b in (("persistent", 64), ("pairs", 64), ("persistent4", 64), ("persistent", 41), ("pairs", 41)):
+        print(f"case: 4096^2 position 3, column pass {cp}, batch {b}", flush=True)
+        se = _search4096(c1, b, cp)
+        shape = se.core.launch_shape()
+        assert shape["column_pass"] == cp and shape["batch"] == b
+        assert shape["z_tile"] == (2 if cp == "persistent" else 4)
+        r = _results(se)
+        se.core.close()
+        if ref is None:
+            ref = r
+            continue
+        for f in ("mip", "best_index", "sum", "sumsq", "hist"):
+            assert np.array_equal(getattr(r, f), getattr(ref, f)), (cp, b, f)
+
+
+def test_autotune_over_column_passes(c1):
+    print("case: autotune 4096^2 over column passes x batch 32, 64", flush=True)
+    search = _search4096(c1, 64, "auto")
+    base = _results(search)
+    best_b, best_c, table = search.autotune((32, 64), ("pairs", "persistent", "persistent4"), sample_orientations=192)
+    print(f"autotune: {best_c} / {best_b} from {table}", flush=True)
+    assert all(v is not None and v > 0 for v in table.values()) and len(table) == 6
+    assert table[(best_c, best_b)] == min(table.values())
+    shape = search.core.launch_shape()
+    assert (shape["column_pass"], shape["batch"]) == (best_c, best_b)
+    r = _results(search)
+    assert np.array_equal(r.mip, base.mip) and np.array_equal(r.best_index, base.best_index)
+    assert np.array_equal(r.sum, base.sum) and np.array_equal(r.hist, base.hist)
+    search.core.close()
+    fp16 = _search4096(c1, 64, "auto", "fp16")
+    assert fp16.core.launch_shape()["column_pass"] == "pairs"
+    _, c16, t16 = fp16.autotune((64,), ("persistent", "pairs"), sample_orientations=128)
+    assert c16 == "pairs" and t16[("persistent", 64)] is None and t16[("pairs", 64)] > 0
+    with pytest.raises(core.TMError):
+        _search4096(c1, 64, "persistent", "fp16")
+    fp16.core.close()

@@ -9,8 +9,11 @@
   implicit transposed convolution (data gradient of a non-head layer with Cout a multiple of 16)
   runs both for Cin <= 32 (32-channel tiles) and for Cin 48 and 64 (64-channel tiles, the shapes of
   the default network's deeper layers; 48 leaves a partial tile)
-* forward pass: every layer's activations and the logits within 1e-5 (relative L2) of the oracle
-  with the device's bf16 rounding points (the remainder is FP32 accumulation order); the location
+* forward pass: every layer's activations and the logits within 2e-4 (relative L2) of the oracle
+  with the device's bf16 rounding points (the remainder is FP32 accumulation order, and the odd
+  input that lies within that noise of a bf16 rounding edge and rounds the other way, one bf16 ulp
+  of 2^-8 on that element; the 48- and 64-channel layers have the most such elements), and each
+  layer within 1e-6 of the oracle's layer applied to the device's own input; the location
   logits and the logits of chosen cells read back equal the full logit volume
 * backward pass: the loss terms within 1e-4 relative and every parameter tensor's gradient within
   1e-4 (relative L2) of the oracle; the oracle's own gradient is pinned by finite differences on the CPU
@@ -79,13 +82,23 @@ def test_forward_matches_oracle(kind):
         net.set_params(flat)
         logits = net.forward(images)
         acts = C.forward(cfg, flat, images, use_bf16=True)
+        P = C.unflatten(cfg, flat)
+        prev = acts[0]
         for l in range(len(spec.channels)):
-            e = _rel(net.activation(l, 3), acts[l + 1])
-            print(f"layer {l}: rel {e:.2e}", flush=True)
-            assert e <= 1e-5
+            dev = net.activation(l, 3)
+            e = _rel(dev, acts[l + 1])
+            # the layer alone, on the device's own input: no rounding flips carried in from before
+            w, b = P[l]
+            col = C.im2col(prev, spec.strides[l])
+            Bn, So = col.shape[0], col.shape[1]
+            one = np.maximum(C.bf16(col.reshape(Bn * So * So, -1)) @ C.bf16(w.reshape(w.shape[0], -1)).T + b, 0.0)
+            e1 = _rel(dev, one.reshape(dev.shape))
+            print(f"layer {l}: rel {e:.2e} end to end, {e1:.2e} on the device's own input", flush=True)
+            assert e <= 2e-4 and e1 <= 1e-6
+            prev = dev.astype(np.float64)
         e = _rel(logits, acts[-1])
         print(f"logits: rel {e:.2e}", flush=True)
-        assert e <= 1e-5
+        assert e <= 2e-4
         # the proposal step's reads: the location logits and the logits of chosen cells
         assert np.array_equal(net.heat(3), logits[..., 0])
         G = net.grid
