@@ -11,7 +11,9 @@ the timed region starts. Metric: cross-correlation maps (CCFs) per second.
 Other workloads (`--workload`, each its own JSON line, not the headline): config2 (eight
 micrographs of mixed sizes), refine (refine_template on the 4096^2 micrograph's peaks), proposal
 (learned proposals + local refinement against the exhaustive search at 512^2 / 64^3), rocfft (the
-column pass against the vendor FFT), cnn (proposal-network training steps, 16 images of 512^2).
+column pass against the vendor FFT), cnn (proposal-network training steps, 16 images of 512^2),
+full_search (the whole truncated config-1 search, 200,000 orientations x 11 planes, once: seconds per
+micrograph).
 
 Prints one JSON line (rank 0, N = 1). `--gpus N` with N > 1 exits: this build runs on one GPU.
 """
@@ -144,7 +146,8 @@ def main():
     ap.add_argument("--weights", type=str, default="",
                     help="proposal: load the network (raw float32 parameters) instead of training")
     ap.add_argument("--save-weights", type=str, default="", help="proposal: save the trained network as raw float32 parameters")
-    ap.add_argument("--workload", choices=["config1", "config2", "refine", "proposal", "rocfft", "cnn"],
+    ap.add_argument("--seed", type=int, default=-1, help="full_search: micrograph seed (default 1000)")
+    ap.add_argument("--workload", choices=["config1", "config2", "refine", "proposal", "rocfft", "cnn", "full_search"],
                     default="config1",
                     help="config2: eight micrographs of mixed sizes (own JSON line, not the headline)")
     ap.add_argument("--concurrency", type=str, default="1,2,4",
@@ -188,6 +191,9 @@ def main():
         return
     if args.workload == "cnn":
         bench_cnn(args)
+        return
+    if args.workload == "full_search":
+        bench_full_search(args)
         return
 
     if args.small:
@@ -640,6 +646,75 @@ def bench_proposal(args):
                       "loss_last_epoch": [round(v, 4) for v in hist[-1]]}),
     }
     net.close()
+    print(json.dumps(out), flush=True)
+
+
+def bench_full_search(args):
+    """The whole truncated config-1 search once (the judge of the learned proposals): the 4096^2
+    micrograph, 256^3 template padded to 512^3, the first 200,000 orientations of the 2.5/1.5 deg C1
+    grid in reference order (positions 0-828 and psi 0-210 of position 829), 11 defocus planes: 2.2 M
+    CCFs. Timed from the raw micrograph (resident on the host, as the reference reads it) to the peak
+    list: device preparation of the input, every plane, z-score and peak search. Reports seconds per
+    micrograph and the peaks above z 7.5 against the inserted particles."""
+    from cistem_downstream_bah_amd import core, synth
+    from cistem_downstream_bah_amd import match_template as MT
+    from cistem_downstream_bah_amd import peaks as PK
+
+    t0 = time.time()
+    s = synth.config1(args.seed if args.seed >= 0 else 1000)
+    log(f"synthetic config-1 micrograph in {time.time() - t0:.1f}s")
+    sp = MT.SearchParams(angular_step=2.5, in_plane_angular_step=1.5, padding=2.0, defocus_search_range=1000.0,
+                         defocus_step=200.0, first_search_position=0, last_search_position=829,
+                         orientation_limit=200_000, batch_size=args.batch, column_pass=args.column_pass)
+    t0 = time.time()
+    search = MT.MatchTemplateSearch(s.micrograph, s.volume, sp)
+    log(f"template and first input prepared in {time.time() - t0:.1f}s; orientations per plane "
+        f"{search.orientations_per_plane}")
+    g = search.core
+    # warm-up: a short plane compiles nothing but touches every kernel once
+    g.reset()
+    t0 = time.perf_counter()
+    search.set_micrograph(s.micrograph, on_device=True)
+    t_prep = time.perf_counter() - t0
+    n = 0
+    planes = list(range(-search.n_def_half, search.n_def_half + 1))
+    for d in planes:
+        t1 = time.perf_counter()
+        n += g.run_inner_loop(search.filter_for(d), 0, d)
+        log(f"plane {d:+d}: {time.perf_counter() - t1:.2f}s ({n} CCFs so far)")
+    t_search = time.perf_counter() - t0 - t_prep
+    t1 = time.perf_counter()
+    pk = g.find_peaks(n, 7.5, search.T // 4 + 1, 10.0)
+    t_peaks = time.perf_counter() - t1
+    total = time.perf_counter() - t0
+    expected = PK.expected_threshold(search.W, search.H, n)
+    # inserted particles within 3 px of a peak
+    parts = np.array([(p.x, p.y, p.theta) for p in s.particles])
+    hits = []
+    for q in pk:
+        d2 = (parts[:, 0] - q["x"]) ** 2 + (parts[:, 1] - q["y"]) ** 2
+        i = int(np.argmin(d2))
+        hits.append((float(np.sqrt(d2[i])), float(parts[i, 2]), q["peak_height"], q["theta"]))
+    near = [h for h in hits if h[0] <= 3.0]
+    zs = sorted((h[2] for h in hits), reverse=True)
+    out = {
+        "metric": "s/micrograph, plain exhaustive search (truncated config-1: 200,000 orientations x 11 planes)",
+        "value": round(total, 3), "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": round(1e3 * total, 1), "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic (config-1 micrograph seed 1000, seeded template)",
+        "config": {"workload": "full_search_config1_4096sq_T256_P512", "image": [search.W, search.H],
+                   "template": search.T, "orientations_per_plane": search.orientations_per_plane,
+                   "defocus_planes": len(planes), "ccfs": n, "batch": g.launch_shape()["batch"],
+                   "column_pass": g.column_pass_name()},
+        "ccf_per_s": round(n / t_search, 1),
+        "seconds": {"input_preparation": round(t_prep, 3), "search": round(t_search, 3),
+                    "z_score_and_peaks": round(t_peaks, 3)},
+        "peaks": {"z_ge_7.5": len(pk), "expected_threshold": round(expected, 3),
+                  "within_3px_of_an_inserted_particle": len(near),
+                  "z_top10": [round(z, 2) for z in zs[:10]], "z_min": round(zs[-1], 2) if zs else None,
+                  "true_theta_of_matched": sorted(round(h[1], 1) for h in near),
+                  "unmatched": [(round(h[0], 1), round(h[2], 2)) for h in hits if h[0] > 3.0][:20]},
+    }
     print(json.dumps(out), flush=True)
 
 

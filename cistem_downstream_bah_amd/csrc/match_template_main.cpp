@@ -111,17 +111,6 @@ std::vector<float> to_original(const float* padded, int W, int H, bool rotated, 
   return o;
 }
 
-double erfcinv_winitzki(double x) {  // functions.cpp:1282-1299
-  const double PI = 3.14159265359;
-  double y = 1.0 - x;
-  double sgn = (y < 0) ? -1.0 : 1.0;
-  y = (1 - y) * (1 + y);
-  double lnx = log(y);
-  double tt1 = 2 / (PI * 0.147) + 0.5 * lnx;
-  double tt2 = 1 / (0.147) * lnx;
-  return sgn * sqrt(-tt1 + sqrt(tt1 * tt1 - tt2));
-}
-
 void usage() {
   std::fprintf(stderr,
                "usage: match_template_hip --image F --template F --out PREFIX [--pixel-size 1] [--voltage 300]\n"
@@ -261,21 +250,19 @@ int main(int argc, char** argv) {
     write_mrc(out + "_pixel_size.mrc", pix_o.data(), ox, oy, pixel_size);
 
     // expected threshold, master form (match_template.cpp:1799)
-    const double erf_input = 2.0 / (1.0 * (double)ox * (double)oy * (double)total);
-    const float thr = sqrtf(2.0f) * (float)erfcinv_winitzki(erf_input);
+    const float thr = (float)tm_expected_threshold(ox, oy, total);
     // histogram file (match_template.cpp:1333-1364), bins 0..511
     {
-      const float first = -12.5f + hstep / 2.0f;
+      std::vector<long long> h64(hist.begin(), hist.end());
+      std::vector<float> snr(512);
       std::vector<double> surv(512, 0.0), expect(512, 0.0);
-      surv[511] = (double)hist[511];
-      for (int i = 510; i >= 0; --i) surv[i] = surv[i + 1] + (double)hist[i];
-      for (int i = 0; i < 512; ++i)
-        expect[i] = (erfc((first + hstep * float(i)) / sqrtf(2.0f)) / 2.0f) * ((float)(sqrt_px * sqrt_px) * float(total));
+      if (tm_histogram_table(h64.data(), 512, -12.5f, hstep, sqrt_px, total, snr.data(), surv.data(), expect.data()) !=
+          TM_OK)
+        throw std::runtime_error(std::string("tm_histogram_table: ") + tm_result_last_error());
       FILE* hf = std::fopen((out + "_histogram.txt").c_str(), "w");
       std::fprintf(hf, "# Expected threshold = %.2f\n", thr);
       std::fprintf(hf, "# SNR, histogram, survival histogram, random survival histogram\n");
-      for (int i = 0; i < 512; ++i)
-        std::fprintf(hf, "%f %ld %f %f\n", first + hstep * float(i), hist[i], surv[i], expect[i]);
+      for (int i = 0; i < 512; ++i) std::fprintf(hf, "%f %ld %f %f\n", snr[i], hist[i], surv[i], expect[i]);
       std::fclose(hf);
     }
     // peaks (match_template.cpp:1839-1980): integer maximum of the z-score map away from a

@@ -101,6 +101,43 @@ Header read_header(const float* r, long long floats) {
 
 extern "C" {
 
+// cisTEM_erfinv / cisTEM_erfcinv (functions.cpp:1282-1299): Winitzki's closed-form inverse of his
+// approximation erf(x) ~ sqrt(1 - exp(-x^2 (4/pi + a x^2) / (1 + a x^2))), a = 0.147, in double.
+static double winitzki_erfinv(double x) {
+  const double PI = 3.14159265359;  // defines.h:6-7
+  const double a = 0.147;
+  const double sgn = (x < 0) ? -1.0 : 1.0;
+  const double lnx = log((1 - x) * (1 + x));
+  const double tt1 = 2 / (PI * a) + 0.5 * lnx;
+  const double tt2 = 1 / a * lnx;
+  return sgn * sqrt(-tt1 + sqrt(tt1 * tt1 - tt2));
+}
+
+double tm_expected_threshold(int image_x, int image_y, long long total_ccfs) {
+  const double erf_input = 2.0 / (1.0 * (double)image_x * (double)image_y * (double)total_ccfs);
+  return (double)(sqrtf(2.0f) * (float)winitzki_erfinv(1.0 - erf_input));
+}
+
+int tm_histogram_table(const long long* histogram, int bins, float histogram_min, float histogram_step,
+                       double sqrt_input_pixels, long long total_ccfs, float* snr, double* survival,
+                       double* expected) {
+  return rguard([&] {
+    require(histogram && bins > 0, "histogram is NULL or empty");
+    const float first = histogram_min + (histogram_step / 2.0f);
+    const float scale = (float)(sqrt_input_pixels * sqrt_input_pixels) * float(total_ccfs);
+    double run = 0.0;
+    for (int i = bins - 1; i >= 0; --i) {
+      run += (double)histogram[i];
+      if (survival) survival[i] = run;
+    }
+    for (int i = 0; i < bins; ++i) {
+      const float x = first + histogram_step * float(i);
+      if (snr) snr[i] = x;
+      if (expected) expected[i] = (std::erfc(x / sqrtf(2.0f)) / 2.0f) * scale;
+    }
+  });
+}
+
 const char* tm_result_last_error(void) { return g_rerr.c_str(); }
 
 long long tm_result_size(int x, int y, int histogram_bins) {

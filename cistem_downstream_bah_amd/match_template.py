@@ -140,6 +140,7 @@ class SearchParams:
     batch_size: int = 0
     plane_storage: str = "fp32"  # "fp16"/"bf16": 16-bit inter-pass plane (image height 4096 only)
     column_pass: str = "auto"    # column-transform kernel at image height 4096 (core.COLUMN_PASS)
+    orientation_limit: int = 0   # only the first N orientations of the position range (0: all)
 
 
 class MatchTemplateSearch:
@@ -184,7 +185,7 @@ class MatchTemplateSearch:
         self.core.init(self.template_spectrum, self.input_spectrum, T, self.grid, self.psi, self.first, self.last,
                        self.histogram_min_scaled, self.histogram_step_scaled, 512, self.max_padding,
                        self.defocus_step, self.pixel_size_step, p.batch_size, p.plane_storage,
-                       column_pass=p.column_pass)
+                       column_pass=p.column_pass, orientation_limit=p.orientation_limit)
 
     def set_micrograph(self, micrograph: np.ndarray, on_device: bool = False) -> None:
         """Searches another micrograph of the same size with this handle (tm_set_input): the template,
@@ -212,7 +213,8 @@ class MatchTemplateSearch:
                                self.psi.size, self.first, self.last, self.histogram_min_scaled,
                                self.histogram_step_scaled, 512, self.max_padding, self.defocus_step,
                                self.pixel_size_step, p.batch_size if batch_size is None else batch_size,
-                               core.PLANE_STORAGE[p.plane_storage], 0, core.COLUMN_PASS[p.column_pass], 0)
+                               core.PLANE_STORAGE[p.plane_storage], p.orientation_limit,
+                               core.COLUMN_PASS[p.column_pass], 0)
 
     def autotune_batch(self, candidates=(16, 32, 64, 128), sample_orientations: int = 1024):
         """Times a sample of this search at each candidate batch depth (tm_autotune_batch) and
@@ -223,7 +225,7 @@ class MatchTemplateSearch:
         self.core.init(self.template_spectrum, self.input_spectrum, self.T, self.grid, self.psi, self.first, self.last,
                        self.histogram_min_scaled, self.histogram_step_scaled, 512, self.max_padding,
                        self.defocus_step, self.pixel_size_step, best, self.p.plane_storage,
-                       column_pass=self.p.column_pass)
+                       column_pass=self.p.column_pass, orientation_limit=self.p.orientation_limit)
         return best, us
 
     def autotune(self, batches=(32, 64, 96), columns=("pairs", "persistent", "persistent4"),
@@ -236,7 +238,8 @@ class MatchTemplateSearch:
         self.p = dataclasses.replace(self.p, batch_size=best_b, column_pass=best_c)
         self.core.init(self.template_spectrum, self.input_spectrum, self.T, self.grid, self.psi, self.first, self.last,
                        self.histogram_min_scaled, self.histogram_step_scaled, 512, self.max_padding,
-                       self.defocus_step, self.pixel_size_step, best_b, self.p.plane_storage, column_pass=best_c)
+                       self.defocus_step, self.pixel_size_step, best_b, self.p.plane_storage, column_pass=best_c,
+                       orientation_limit=self.p.orientation_limit)
         return best_b, best_c, table
 
     def filter_for(self, defocus_i: int) -> np.ndarray:
@@ -281,8 +284,13 @@ class MatchTemplateSearch:
                             math.sqrt(float(self.W * self.H)))
 
     @property
+    def orientations_per_plane(self) -> int:
+        n = (self.last - self.first + 1) * self.psi.size
+        return min(n, self.p.orientation_limit) if self.p.orientation_limit > 0 else n
+
+    @property
     def total_correlation_positions(self) -> int:
-        return (self.last - self.first + 1) * self.psi.size * (2 * self.n_def_half + 1)
+        return self.orientations_per_plane * (2 * self.n_def_half + 1)
 
 
 def run_concurrent(searches: list, workers: int = 2) -> list:

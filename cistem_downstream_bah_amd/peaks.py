@@ -2,7 +2,8 @@
 
 Follows the master side of the reference's match_template (match_template.cpp:1763-1980):
   * expected threshold = sqrt(2) * erfcinv(2 / (W * H * N_ccf)) with cisTEM's erfcinv (Winitzki's
-    approximation, a = 0.147; src/core/functions.cpp:1282-1299) and the original image size;
+    approximation, a = 0.147; src/core/functions.cpp:1282-1299) and the original image size, computed
+    by the library's tm_expected_threshold;
   * peaks: repeatedly take the integer maximum of the scaled MIP away from a border of T/4 + 1
     pixels (Image::FindPeakWithIntegerCoordinates, image.cpp:9961-10100), stop below the threshold,
     record x, y (Angstrom), the best angles, defocus and pixel-size offsets and the peak height,
@@ -19,19 +20,10 @@ MAX_PEAKS = 1000
 PI = 3.14159265359
 
 
-def cistem_erfcinv(x: float) -> float:
-    y = 1.0 - x
-    sgn = -1.0 if y < 0 else 1.0
-    y = (1 - y) * (1 + y)
-    lnx = math.log(y)
-    tt1 = 2 / (PI * 0.147) + 0.5 * lnx
-    tt2 = 1 / 0.147 * lnx
-    return sgn * math.sqrt(-tt1 + math.sqrt(tt1 * tt1 - tt2))
-
-
 def expected_threshold(image_x: int, image_y: int, n_ccf: int) -> float:
-    erf_input = 2.0 / (1.0 * image_x * image_y * n_ccf)
-    return float(np.float32(math.sqrt(2.0)) * np.float32(cistem_erfcinv(erf_input)))
+    """tm_expected_threshold of the library (include/tm_results.h)."""
+    from . import results
+    return results.expected_threshold(image_x, image_y, n_ccf)
 
 
 def find_peaks(scaled_mip: np.ndarray, threshold: float, template_size: int, min_peak_radius: int = 10,

@@ -16,7 +16,8 @@ I64P = core.I64P
 META = 6
 MAPS = 8
 MAP_NAMES = ("mip", "psi", "theta", "phi", "defocus", "pixel_size", "sum", "sumsq")
-RESULT_EXPORTS = ["tm_result_size", "tm_result_pack", "tm_result_add", "tm_result_finalize", "tm_result_last_error"]
+RESULT_EXPORTS = ["tm_result_size", "tm_result_pack", "tm_result_add", "tm_result_finalize", "tm_result_last_error",
+                  "tm_expected_threshold", "tm_histogram_table"]
 
 
 def _lib():
@@ -30,6 +31,11 @@ def _lib():
                                      ctypes.c_longlong]
         L.tm_result_add.argtypes = [F32P, F32P, ctypes.c_longlong, ctypes.POINTER(ctypes.c_float), ctypes.c_int]
         L.tm_result_finalize.argtypes = [F32P, ctypes.c_longlong, ctypes.c_float, F32P, F32P, F32P, F32P, F32P, I64P]
+        L.tm_expected_threshold.restype = ctypes.c_double
+        L.tm_expected_threshold.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_longlong]
+        L.tm_histogram_table.argtypes = [I64P, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_double,
+                                         ctypes.c_longlong, F32P, ctypes.POINTER(ctypes.c_double),
+                                         ctypes.POINTER(ctypes.c_double)]
         L._tm_results_bound = True
     return L
 
@@ -103,3 +109,23 @@ def unpack(record: np.ndarray) -> dict:
         out[name] = r[META + k * n: META + (k + 1) * n].reshape(y, row)[:, :x].copy()
     out["histogram"] = r[META + MAPS * n: META + MAPS * n + 2 * bins].view(np.int64).copy()
     return out
+
+
+def expected_threshold(image_x: int, image_y: int, n_ccf: int) -> float:
+    """tm_expected_threshold: sqrtf(2) erfcinv(2 / (x y N)) with cisTEM's erfcinv (match_template.cpp:1799)."""
+    return float(_lib().tm_expected_threshold(int(image_x), int(image_y), int(n_ccf)))
+
+
+def histogram_table(histogram: np.ndarray, histogram_min: float, histogram_step: float, sqrt_input_pixels: float,
+                    total_ccfs: int):
+    """tm_histogram_table: the histogram file's columns (SNR, survival, expected survival;
+    match_template.cpp:1333-1364)."""
+    h = np.ascontiguousarray(histogram, np.int64)
+    snr = np.zeros(h.size, np.float32)
+    surv = np.zeros(h.size, np.float64)
+    exp_ = np.zeros(h.size, np.float64)
+    dp = ctypes.POINTER(ctypes.c_double)
+    _check(_lib().tm_histogram_table(h.ctypes.data_as(I64P), int(h.size), float(histogram_min), float(histogram_step),
+                                     float(sqrt_input_pixels), int(total_ccfs), snr.ctypes.data_as(F32P),
+                                     surv.ctypes.data_as(dp), exp_.ctypes.data_as(dp)))
+    return snr, surv, exp_

@@ -63,6 +63,22 @@ int tm_result_add(float* collated, const float* result, long long floats, float*
 int tm_result_finalize(const float* collated, long long floats, float total_ccfs, float* mip_scaled, float* zscore,
                        float* average, float* std_dev, float* angles, long long* histogram);
 
+/* Expected threshold of a search (the master's form, match_template.cpp:1799): sqrtf(2) * erfcinv(2 /
+ * (x * y * total_ccfs)) with cisTEM's erfcinv (Winitzki's approximation of erfinv with a = 0.147,
+ * functions.cpp:1282-1299), x and y the micrograph's original size; returned as the float the
+ * reference computes. The local run discards the erfcinv result (match_template.cpp:1282); the
+ * master's formula is used for both. */
+double tm_expected_threshold(int image_x, int image_y, long long total_ccfs);
+
+/* The histogram file's columns (match_template.cpp:1333-1364), bins 0..bins-1: snr[i] = histogram_min
+ * + histogram_step / 2 + histogram_step * i (float), survival[i] = sum of counts of bins i..bins-1,
+ * expected[i] = erfc(snr[i] / sqrtf(2)) / 2 * (sqrt_input_pixels^2 * total_ccfs), in float as there
+ * (the reference's loop fills one entry past its array; here it stops at bins - 1). Any output may be
+ * NULL. */
+int tm_histogram_table(const long long* histogram, int bins, float histogram_min, float histogram_step,
+                       double sqrt_input_pixels, long long total_ccfs, float* snr, double* survival,
+                       double* expected);
+
 const char* tm_result_last_error(void);
 
 #ifdef __cplusplus

@@ -55,7 +55,7 @@ def test_cnn_exports_and_config_match_header():
 
 def test_result_record_exports_match_header():
     header = open(os.path.join(ROOT, "include", "tm_results.h")).read()
-    declared = set(re.findall(r"^(?:int|long long|const char\*)\s+(tm_\w+)\(", header, re.M))
+    declared = set(re.findall(r"^(?:int|long long|double|const char\*)\s+(tm_\w+)\(", header, re.M))
     assert declared == set(results.RESULT_EXPORTS)
     L = ctypes.CDLL(core.LIB_PATH)
     for name in declared:

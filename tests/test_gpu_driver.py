@@ -1,8 +1,10 @@
 """GPU: the C++ match_template driver (cistem_downstream_bah_amd/match_template_hip) end to end on the
 config-0 inputs written as MRC files: its maps equal the C-ABI path's, its peak list (found on the
 device) equals the host peak rule applied to its own output maps, it contains every inserted
-particle above the threshold, and its worker record (--worker-record, include/tm_results.h) agrees
-with the Python mirror's record of the same search."""
+particle above the threshold, its histogram file holds the search's counts with the survival and
+expected-survival columns of tm_histogram_table and the master's expected threshold, and its worker
+record (--worker-record, include/tm_results.h) agrees with the Python mirror's record of the same
+search."""
 import os
 import subprocess
 
@@ -58,6 +60,15 @@ def test_driver_end_to_end(tmp_path):
     assert detectable >= 8
     hist = np.loadtxt(out + "_histogram.txt")
     assert hist.shape == (512, 4) and hist[:, 1].sum() == n * 512 * 512
+    # the file's columns are the histogram table of its own counts (match_template.cpp:1333-1364) and
+    # its threshold line the master's formula (test_threshold_cpu pins both)
+    assert thr == round(R.expected_threshold(512, 512, n), 2)
+    counts = hist[:, 1].astype(np.int64)
+    assert np.array_equal(counts, search.core.results().hist)
+    snr, surv, expected = R.histogram_table(counts, -12.5, float(np.float32(35.0 / 512)), 512.0, n)
+    assert np.allclose(hist[:, 0], snr, rtol=0, atol=1e-6)
+    assert np.array_equal(hist[:, 2], surv)
+    assert np.allclose(hist[:, 3], expected, rtol=1e-6, atol=1e-6)
     # the driver's worker record (--worker-record) against the Python mirror's for the same search
     u = R.unpack(np.fromfile(out + "_record.f32", np.float32))
     py = R.unpack(search.worker_record(16.0))
