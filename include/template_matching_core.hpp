@@ -1,10 +1,13 @@
 // C++ drop-in for the reference's GPU class TemplateMatchingCore
 // (src/gpu/TemplateMatchingCore.h:55-179), header-only over the C ABI of tm_hip.h.
 //
-// The reference passes cisTEM `Image` objects; this class takes the same data as plain arrays in
-// the same memory layouts (FFTW half-complex spectra, padded real maps), so a caller holding
-// Images passes image.real_values / reinterpret_cast<float*>(image.complex_values). Method names
-// and argument meaning follow the reference:
+// Two forms. The reference's own signatures, templated on the caller's Image / EulerSearch types
+// (Init with the 25 arguments of TemplateMatchingCore.h:147-171, RunInnerLoop(Image&, float, float,
+// int, long&) of :173, and the members d_max_intensity_projection, d_best_psi/theta/phi, d_sum3,
+// d_sumSq3, histogram and total_number_of_cccs_calculated that match_template.cpp:918-985 reads), so
+// that caller compiles unchanged and merges planes on the host as it does now. And a plain-array
+// form in the same memory layouts (FFTW half-complex spectra, padded real maps) whose maps accumulate
+// over planes on the device. Method names and argument meaning follow the reference:
 //   Init(...)          TemplateMatchingCore.h:147-171 (search ranges, Euler grid, psi loop, histogram)
 //   RunInnerLoop(...)  TemplateMatchingCore.h:173 (one pixel-size/defocus plane; size_i and
 //                      defocus_i are loop indices, as in the reference)
@@ -124,15 +127,123 @@ class TemplateMatchingCore {
     check(tm_finalize(h_, total_correlation_positions, mip_scaled, zscore, average, std_dev));
   }
 
+  // ------------------------------------------------------------------------------------------
+  // The reference's own signatures (TemplateMatchingCore.h:147-173), templated on the caller's
+  // types so that match_template.cpp:858-985 compiles against this class unchanged. ImageT is the
+  // reference's Image: complex_values (std::complex<float>*, FFTW half-complex), real_values,
+  // logical_x/y_dimension, padding_jump_value. EulerT provides number_of_search_positions and
+  // list_of_search_parameters ({phi, theta} rows). parent_pointer, angles, my_progress,
+  // total_correlation_positions, is_running_locally and threadIDX carry the reference's progress and
+  // job plumbing and are not used; the template, input and current_projection sizes give P, W x H and T.
+  template <class AppT, class ImageT, class AnglesT, class EulerT, class ProgressT>
+  void Init(AppT* parent_pointer, ImageT& template_reconstruction, ImageT& input_image, ImageT& current_projection,
+            float pixel_size_search_range, float pixel_size_step, float pixel_size, float defocus_search_range,
+            float defocus_step, float defocus1, float defocus2, float psi_max, float psi_start, float psi_step,
+            AnglesT& angles, EulerT& global_euler_search, float histogram_min_scaled, float histogram_step_scaled,
+            int histogram_number_of_bins, int max_padding, int first_search_position, int last_search_position,
+            ProgressT* my_progress, long total_correlation_positions, bool is_running_locally) {
+    (void)parent_pointer; (void)pixel_size_search_range; (void)pixel_size; (void)defocus_search_range;
+    (void)defocus1; (void)defocus2; (void)angles; (void)my_progress; (void)total_correlation_positions;
+    (void)is_running_locally;
+    const int n = global_euler_search.number_of_search_positions;
+    std::vector<float> grid(2 * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+      grid[2 * (size_t)i] = global_euler_search.list_of_search_parameters[i][0];
+      grid[2 * (size_t)i + 1] = global_euler_search.list_of_search_parameters[i][1];
+    }
+    Init(reinterpret_cast<const float*>(template_reconstruction.complex_values),
+         template_reconstruction.logical_x_dimension, reinterpret_cast<const float*>(input_image.complex_values),
+         input_image.logical_x_dimension, input_image.logical_y_dimension, current_projection.logical_x_dimension,
+         pixel_size_step, defocus_step, psi_max, psi_start, psi_step, grid.data(), n, histogram_min_scaled,
+         histogram_step_scaled, histogram_number_of_bins, max_padding, first_search_position, last_search_position);
+  }
+
+  // One plane, as the reference's RunInnerLoop (TemplateMatchingCore.cu:140-368): the maps, sums and
+  // histogram start from zero (-FLT_MAX for the MIP, the CPU path's start) at every call, so the
+  // caller merges planes on the host (match_template.cpp:918-985) from the members below;
+  // total_number_of_cccs_calculated is this call's count and current_correlation_position grows by it.
+  template <class ImageT>
+  void RunInnerLoop(ImageT& projection_filter, float pixel_i, float defocus_i, int threadIDX,
+                    long& current_correlation_position) {
+    (void)threadIDX;
+    check(tm_reset(h_));
+    cache_valid_ = false;
+    long long n = 0;
+    check(tm_run_inner_loop(h_, reinterpret_cast<const float*>(projection_filter.complex_values), (int)pixel_i,
+                            (int)defocus_i, nullptr, nullptr, &n));
+    total_number_of_cccs_calculated = n;
+    current_correlation_position += (long)n;
+  }
+
+  // Stand-ins for the reference's GpuImage members that match_template.cpp:922-947 reads:
+  // CopyDeviceToHost(Image& dst, bool block, bool free) fills dst's padded real-space rows; Wait() is
+  // a no-op (copies complete before returning).
+  class DeviceMap {
+   public:
+    DeviceMap(TemplateMatchingCore* owner, int which) : owner_(owner), which_(which) {}
+    template <class ImageT>
+    void CopyDeviceToHost(ImageT& dst, bool should_block_until_complete = true, bool free_gpu_memory = false) {
+      (void)should_block_until_complete; (void)free_gpu_memory;
+      owner_->copy_map(which_, dst.real_values, dst.logical_x_dimension, dst.logical_y_dimension,
+                       dst.padding_jump_value);
+    }
+    void Wait() {}
+
+   private:
+    TemplateMatchingCore* owner_;
+    int which_;
+  };
+  // Stand-in for the reference's Histogram member (Histogram.cu:188-203).
+  class HistogramCopy {
+   public:
+    explicit HistogramCopy(TemplateMatchingCore* owner) : owner_(owner) {}
+    void CopyToHostAndAdd(long* histogram_data) { owner_->add_histogram(histogram_data); }
+
+   private:
+    TemplateMatchingCore* owner_;
+  };
+  DeviceMap d_max_intensity_projection{this, 0}, d_best_psi{this, 1}, d_best_theta{this, 2}, d_best_phi{this, 3},
+      d_sum3{this, 4}, d_sumSq3{this, 5};
+  HistogramCopy histogram{this};
+
   long long total_number_of_cccs_calculated = 0;
 
  private:
   static void check(int rc) {
     if (rc != TM_OK) throw std::runtime_error(std::string("TemplateMatchingCore: ") + tm_last_error());
   }
+  void fetch() {
+    if (cache_valid_) return;
+    const size_t n = (size_t)(image_x_ + 2) * image_y_;
+    for (auto& m : maps_) m.assign(n, 0.f);
+    sums_[0].assign(n, 0.0);
+    sums_[1].assign(n, 0.0);
+    hist_.assign(512, 0);
+    check(tm_get_results(h_, maps_[0].data(), maps_[1].data(), maps_[2].data(), maps_[3].data(), nullptr, nullptr,
+                         sums_[0].data(), sums_[1].data(), nullptr, hist_.data(), nullptr));
+    cache_valid_ = true;
+  }
+  // map 0 MIP, 1 psi, 2 theta, 3 phi, 4 sum, 5 sum of squares (as floats) into rows of x + jump floats
+  void copy_map(int which, float* dst, int x, int y, int jump) {
+    if (x != image_x_ || y != image_y_) throw std::runtime_error("TemplateMatchingCore: image size differs");
+    fetch();
+    for (int j = 0; j < y; ++j)
+      for (int i = 0; i < x; ++i) {
+        const size_t s = (size_t)j * (image_x_ + 2) + i;
+        dst[(size_t)j * (x + jump) + i] = which < 4 ? maps_[which][s] : (float)sums_[which - 4][s];
+      }
+  }
+  void add_histogram(long* histogram_data) {
+    fetch();
+    for (int b = 0; b < n_bins_; ++b) histogram_data[b] += (long)hist_[b];
+  }
   tm_core* h_ = nullptr;
   std::vector<float> psi_;
   int image_x_ = 0, image_y_ = 0, n_bins_ = 512;
+  bool cache_valid_ = false;
+  std::vector<float> maps_[4];
+  std::vector<double> sums_[2];
+  std::vector<long long> hist_;
 };
 
 }  // namespace cistem_amd
