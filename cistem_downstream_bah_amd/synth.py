@@ -182,6 +182,62 @@ def make_synthetic(seed: int, W: int, H: int, T: int, n_atoms: int, atom_radius:
     return Synthetic(noisy.astype(np.float32), vol, parts, noise_sigma, var, used, defocus)
 
 
+def make_synthetic_gpu(seed: int, W: int, H: int, T: int, n_atoms: int, atom_radius: float, n_copies: int,
+                       snr: float, defocus: float, pixel_size: float = 1.0, kv: float = 300.0, cs_mm: float = 2.7,
+                       amp_contrast: float = 0.07, sigma: float = 1.5, min_dist: float | None = None,
+                       template: tuple | None = None) -> Synthetic:
+    """make_synthetic with the copies' projections, the CTF and the noise scaling computed on the GPU in
+    float64 (torch): the same random draws in the same order (template, positions, orientations, then
+    the noise from the numpy generator), so the micrograph equals make_synthetic's up to float64
+    rounding. For the workloads that need hundreds of 4096^2 micrographs. Falls back to make_synthetic
+    when no GPU is visible."""
+    try:
+        import torch
+        if not torch.cuda.is_available():
+            raise ImportError
+    except ImportError:
+        return make_synthetic(seed, W, H, T, n_atoms, atom_radius, n_copies, snr, defocus, pixel_size, kv, cs_mm,
+                              amp_contrast, sigma, min_dist, template)
+    dev = torch.device("cuda")
+    rng = np.random.default_rng(seed)
+    if template is None:
+        pos, amp = make_atoms(rng, n_atoms, atom_radius)
+        vol = render_volume(T, pos, amp, sigma)
+    else:
+        pos, amp, vol = template
+    if min_dist is None:
+        min_dist = T
+    centres, used = place_particles(rng, n_copies, W, H, T // 2, min_dist)
+    box = int(2 * (atom_radius + 4 * sigma)) + 2
+    img = torch.zeros((H, W), dtype=torch.float64, device=dev)
+    s2 = 2.0 * sigma * sigma
+    w = torch.tensor(amp * sigma * np.sqrt(2.0 * np.pi), dtype=torch.float64, device=dev)
+    parts = []
+    for (cx, cy) in centres:
+        phi, theta, psi = random_euler(rng)
+        m = euler_matrix(phi, theta, psi)
+        proj = pos @ m
+        x0 = max(0, int(np.floor(cx)) - box // 2)
+        y0 = max(0, int(np.floor(cy)) - box // 2)
+        x1 = min(W, int(np.floor(cx)) + box // 2 + 1)
+        y1 = min(H, int(np.floor(cy)) + box // 2 + 1)
+        px = torch.tensor(cx + proj[:, 0], dtype=torch.float64, device=dev)
+        py = torch.tensor(cy + proj[:, 1], dtype=torch.float64, device=dev)
+        xs = torch.arange(x0, x1, dtype=torch.float64, device=dev)
+        ys = torch.arange(y0, y1, dtype=torch.float64, device=dev)
+        gx = torch.exp(-((xs[None, :] - px[:, None]) ** 2) / s2)
+        gy = torch.exp(-((ys[None, :] - py[:, None]) ** 2) / s2)
+        img[y0:y1, x0:x1] += (gy.T * w[None, :]).contiguous() @ gx.contiguous()
+        parts.append(Particle(float(cx), float(cy), phi, theta, psi))
+    ctf = torch.tensor(ctf_2d(H, W, pixel_size, kv, cs_mm, amp_contrast, defocus), dtype=torch.float64, device=dev)
+    signal = torch.fft.irfft2(torch.fft.rfft2(img) * ctf, s=(H, W))
+    var = float(signal.var(unbiased=False))
+    noise_sigma = float(np.sqrt(var / snr)) if snr > 0 else 0.0
+    noise = torch.from_numpy(rng.standard_normal((H, W))).to(dev)
+    noisy = (signal + noise_sigma * noise).to(torch.float32).cpu().numpy()
+    return Synthetic(noisy, vol, parts, noise_sigma, var, used, defocus)
+
+
 # Workload presets (sizes and counts of the benchmark configurations).
 def config0(seed: int = 0) -> Synthetic:
     """512^2 micrograph, 64^3 template (200 atoms, r=24), 10 copies, SNR 0.1, defocus 15000 A."""
