@@ -27,7 +27,7 @@ I64P = ctypes.POINTER(ctypes.c_longlong)
 EXPORTS = ["tm_device_count", "tm_create", "tm_destroy", "tm_init", "tm_run_inner_loop", "tm_get_results", "tm_merge",
            "tm_set_template", "tm_reset", "tm_finalize", "tm_debug_projection", "tm_debug_ccf", "tm_set_timing", "tm_get_timing",
            "tm_last_error", "tm_autotune", "tm_autotune_batch", "tm_get_launch_shape", "tm_set_graph", "tm_set_input", "tm_set_micrograph",
-           "tm_find_peaks", "tm_run_partial", "tm_state_size", "tm_export_state", "tm_import_state"]
+           "tm_find_peaks", "tm_find_peaks_noalloc", "tm_run_partial", "tm_state_size", "tm_export_state", "tm_import_state"]
 
 PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.c_longlong, ctypes.c_void_p)
 
@@ -90,6 +90,7 @@ def lib():
         L.tm_set_micrograph.argtypes = [vp, F32P, F32P, F32P]
         L.tm_find_peaks.argtypes = [vp, ctypes.c_longlong, ctypes.c_float, ctypes.c_int, ctypes.c_float,
                                     ctypes.c_float, ctypes.POINTER(PeakInfo), ctypes.c_int, I32P]
+        L.tm_find_peaks_noalloc.argtypes = L.tm_find_peaks.argtypes
         L.tm_run_partial.argtypes = [vp, F32P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, I64P]
         L.tm_state_size.argtypes = [vp, I64P]
         L.tm_export_state.argtypes = [vp, ctypes.c_void_p, ctypes.c_longlong]
@@ -289,14 +290,15 @@ class TemplateMatchingCore:
         return cx, cy
 
     def find_peaks(self, total_ccfs: int, threshold: float, edge: int, min_peak_radius: float = 10.0,
-                   capacity: int = 1000, pixel_size: float = 1.0) -> list:
-        """Peaks of the z-score map found on the device (tm_find_peaks): dicts with x, y (pixels),
-        x_pos, y_pos (Angstroms), peak_height (z) and the psi, theta, phi, defocus, pixel_size values at
-        the peak, and the peak numbers of TemplateMatchFoundPeakInfo."""
+                   capacity: int = 1000, pixel_size: float = 1.0, allocation_free: bool = False) -> list:
+        """Peaks of the z-score map found on the device (tm_find_peaks, or tm_find_peaks_noalloc with
+        allocation_free): dicts with x, y (pixels), x_pos, y_pos (Angstroms), peak_height (z) and the
+        psi, theta, phi, defocus, pixel_size values at the peak, and the peak numbers of
+        TemplateMatchFoundPeakInfo."""
         out = (PeakInfo * max(capacity, 1))()
         n = ctypes.c_int(0)
-        _check(lib().tm_find_peaks(self.h, total_ccfs, threshold, edge, min_peak_radius, pixel_size, out, capacity,
-                                   ctypes.byref(n)))
+        fn = lib().tm_find_peaks_noalloc if allocation_free else lib().tm_find_peaks
+        _check(fn(self.h, total_ccfs, threshold, edge, min_peak_radius, pixel_size, out, capacity, ctypes.byref(n)))
         res = []
         for k in range(n.value):
             p = out[k]

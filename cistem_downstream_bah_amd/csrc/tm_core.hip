@@ -15,6 +15,7 @@
 #include <cstdint>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -505,6 +506,144 @@ __global__ void k_zmask(float* __restrict__ z, int W, int H, int bx, int by, int
   if (dx * dx + dy * dy <= r2) z[(size_t)y * W + x] = -FLT_MAX;
 }
 
+// ------------------------------------------------------------------ allocation-free peak search
+// The same search as above with no z-score map: each pixel's z-score is computed where it is read
+// (finalize_pixel, the arithmetic of k_finalize) and the exclusion discs of the peaks found so far
+// are one bit per pixel. Its working memory is the file-scope array `scratch` below: the bits, then
+// the reduction partials and the result.
+__global__ __launch_bounds__(256) void k_fill_u32(unsigned* __restrict__ p, size_t n, unsigned v) {
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) p[i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_zmax_part_bits(const float* __restrict__ mip, const double* __restrict__ sum,
+                                                        const double* __restrict__ sumsq,
+                                                        const unsigned* __restrict__ excluded, int W, int H, int edge,
+                                                        float n_ccf, float sqrt_px, float* __restrict__ pv,
+                                                        int* __restrict__ pi) {
+  __shared__ float sv[256];
+  __shared__ int si[256];
+  float bv = -INFINITY;
+  int bi = INT_MAX;
+  const int x0 = edge, x1 = W - edge, y1 = H - edge;
+  for (int y = edge + blockIdx.x; y < y1; y += gridDim.x)
+    for (int x = x0 + threadIdx.x; x < x1; x += 256) {
+      const int i = y * W + x;
+      const float v = ((excluded[i >> 5] >> (i & 31)) & 1u)
+                          ? -FLT_MAX
+                          : tmk::finalize_pixel(mip[i], sum[i], sumsq[i], n_ccf, sqrt_px).z;
+      if (zbetter(v, i, bv, bi)) { bv = v; bi = i; }
+    }
+  sv[threadIdx.x] = bv;
+  si[threadIdx.x] = bi;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k && zbetter(sv[threadIdx.x + k], si[threadIdx.x + k], sv[threadIdx.x], si[threadIdx.x])) {
+      sv[threadIdx.x] = sv[threadIdx.x + k];
+      si[threadIdx.x] = si[threadIdx.x + k];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    pv[blockIdx.x] = sv[0];
+    pi[blockIdx.x] = si[0];
+  }
+}
+
+// k_zmask's disc, as set bits
+__global__ void k_zmask_bits(unsigned* __restrict__ excluded, int W, int H, int bx, int by, int r, float r2) {
+  const int x = max(bx - r, 0) + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int y = max(by - r, 0) + (int)blockIdx.y;
+  if (x >= min(bx + r, W) || y >= min(by + r, H)) return;
+  const float dx = float(x) - bx, dy = float(y) - by;
+  if (dx * dx + dy * dy <= r2) {
+    const int i = y * W + x;
+    atomicOr(&excluded[i >> 5], 1u << (i & 31));
+  }
+}
+
+}  // namespace
+
+// Working memory of tm_find_peaks_noalloc (TM_NOALLOC_SCRATCH_BYTES): every device that loads the
+// library holds its own copy until the process ends. Host code never passes `scratch` itself to a
+// kernel (on the host it is only a stand-in for the symbol): tm_create looks up each device's
+// address once, with that device current, and the entry point uses that address.
+static __device__ __align__(256) unsigned char scratch[TM_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxDevices] = {};
+
+// once per device, with that device current
+void scratch_setup(int device) {
+  TM_REQUIRE(device >= 0 && device < kMaxDevices, TM_ERR_INVALID, "device ordinal beyond the library's table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch[device]) return;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    throw TmError{TM_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e)};
+  g_scratch[device] = p;
+}
+
+void* scratch_address(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  void* p = (device >= 0 && device < kMaxDevices) ? g_scratch[device] : nullptr;
+  TM_REQUIRE(p, TM_ERR_STATE, "no scratch address for device " + std::to_string(device));
+  return p;
+}
+
+// The peak loop of tm_find_peaks and tm_find_peaks_noalloc: find_max() leaves {value bits, pixel,
+// orientation index} of the highest remaining z-score at res on the device; exclude() removes the
+// disc of min_peak_radius around a peak (box [bx - r, bx + r) x [by - r, by + r) clipped to the
+// image, one 64-thread row of blocks per image row). Returns the number of peaks written.
+template <class FindMax, class Exclude>
+int peak_loop(tm_core* h, float threshold, int edge, float min_peak_radius, float pixel_size, tm_peak_info* out,
+              int capacity, const int* res, FindMax find_max, Exclude exclude) {
+  const int W = h->W, H = h->H;
+  hipStream_t st = h->stream;
+  const int r = (int)min_peak_radius;
+  const float r2 = powf(float(r), 2);
+  const int per_plane = h->prm.n_search_positions * h->n_psi;
+  int found = 0;
+  while (found < capacity && W > 2 * edge && H > 2 * edge) {
+    find_max();
+    int hres[3];
+    TM_HIP(hipMemcpyAsync(hres, res, sizeof(hres), hipMemcpyDeviceToHost, st));
+    TM_HIP(hipStreamSynchronize(st));
+    float v;
+    std::memcpy(&v, &hres[0], sizeof(v));
+    if (hres[1] < 0 || hres[1] == INT_MAX || !(v >= threshold)) break;
+    const int bx = hres[1] % W, by = hres[1] / W, g = hres[2];
+    float a_psi = 0.f, a_theta = 0.f, a_phi = 0.f, a_def = 0.f, a_pix = 0.f;
+    if (g >= 0) {
+      const int plane = g / per_plane, rem = g % per_plane;
+      const int pos = rem / h->n_psi, k = rem % h->n_psi;
+      a_psi = h->psi[k];
+      a_theta = h->grid[2 * pos + 1];
+      a_phi = h->grid[2 * pos];
+      a_def = float(h->planes[plane].y) * h->prm.defocus_step;
+      a_pix = float(h->planes[plane].x) * h->prm.pixel_size_step;
+    }
+    tm_peak_info& o = out[found];
+    ++found;
+    o.peak_number = o.original_peak_number = o.new_peak_number = found;
+    o.x_pos = (float)bx * pixel_size;
+    o.y_pos = (float)by * pixel_size;
+    o.psi = a_psi;
+    o.theta = a_theta;
+    o.phi = a_phi;
+    o.defocus = a_def;
+    o.pixel_size = a_pix;
+    o.peak_height = v;
+    const int x_lo = std::max(bx - r, 0), x_hi = std::min(bx + r, W), y_lo = std::max(by - r, 0), y_hi = std::min(by + r, H);
+    if (x_hi > x_lo && y_hi > y_lo) exclude(bx, by, r, r2, dim3((x_hi - x_lo + 63) / 64, y_hi - y_lo));
+  }
+  TM_HIP(hipStreamSynchronize(st));
+  return found;
+}
+
 }  // namespace
 
 // ====================================================================== C ABI
@@ -530,6 +669,7 @@ int tm_create(int device_ordinal, tm_core** out) {
     TM_REQUIRE(device_ordinal >= 0 && device_ordinal < n, TM_ERR_INVALID,
                "device ordinal " + std::to_string(device_ordinal) + " out of range (" + std::to_string(n) + " devices)");
     TM_HIP(hipSetDevice(device_ordinal));
+    scratch_setup(device_ordinal);
     auto* h = new tm_core();
     h->device = device_ordinal;
     hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
@@ -1092,7 +1232,7 @@ int tm_find_peaks(tm_core* h, long long total_ccfs, float threshold, int edge, f
     DevBuf<float> fin, pv;
     DevBuf<int> pi, res;
     fin.alloc(4 * npx);
-    const int nparts = std::max(1, std::min(1024, H - 2 * edge));
+    const int nparts = std::max(1, std::min(TM_NOALLOC_PARTS, H - 2 * edge));
     pv.alloc(nparts);
     pi.alloc(nparts);
     res.alloc(3);
@@ -1101,51 +1241,58 @@ int tm_find_peaks(tm_core* h, long long total_ccfs, float threshold, int edge, f
                        (float)total_ccfs, (float)sqrt_px, fin.p, fin.p + npx, fin.p + 2 * npx, fin.p + 3 * npx);
     check_launch("k_finalize");
     float* z = fin.p + npx;
-    const int r = (int)min_peak_radius;
-    const float r2 = powf(float(r), 2);
-    const int per_plane = h->prm.n_search_positions * h->n_psi;
-    int found = 0;
-    while (found < capacity && W > 2 * edge && H > 2 * edge) {
-      hipLaunchKernelGGL(k_zmax_part, dim3(nparts), dim3(256), 0, st, z, W, H, edge, pv.p, pi.p);
-      check_launch("k_zmax_part");
-      hipLaunchKernelGGL(k_zmax_final, dim3(1), dim3(256), 0, st, pv.p, pi.p, nparts, h->idx.p, res.p);
-      check_launch("k_zmax_final");
-      int hres[3];
-      TM_HIP(hipMemcpyAsync(hres, res.p, sizeof(hres), hipMemcpyDeviceToHost, st));
-      TM_HIP(hipStreamSynchronize(st));
-      float v;
-      std::memcpy(&v, &hres[0], sizeof(v));
-      if (hres[1] < 0 || hres[1] == INT_MAX || !(v >= threshold)) break;
-      const int bx = hres[1] % W, by = hres[1] / W, g = hres[2];
-      float a_psi = 0.f, a_theta = 0.f, a_phi = 0.f, a_def = 0.f, a_pix = 0.f;
-      if (g >= 0) {
-        const int plane = g / per_plane, rem = g % per_plane;
-        const int pos = rem / h->n_psi, k = rem % h->n_psi;
-        a_psi = h->psi[k];
-        a_theta = h->grid[2 * pos + 1];
-        a_phi = h->grid[2 * pos];
-        a_def = float(h->planes[plane].y) * h->prm.defocus_step;
-        a_pix = float(h->planes[plane].x) * h->prm.pixel_size_step;
-      }
-      tm_peak_info& o = out[found];
-      ++found;
-      o.peak_number = o.original_peak_number = o.new_peak_number = found;
-      o.x_pos = (float)bx * pixel_size;
-      o.y_pos = (float)by * pixel_size;
-      o.psi = a_psi;
-      o.theta = a_theta;
-      o.phi = a_phi;
-      o.defocus = a_def;
-      o.pixel_size = a_pix;
-      o.peak_height = v;
-      const int x_lo = std::max(bx - r, 0), x_hi = std::min(bx + r, W), y_lo = std::max(by - r, 0), y_hi = std::min(by + r, H);
-      if (x_hi > x_lo && y_hi > y_lo) {
-        hipLaunchKernelGGL(k_zmask, dim3((x_hi - x_lo + 63) / 64, y_hi - y_lo), dim3(64), 0, st, z, W, H, bx, by, r, r2);
-        check_launch("k_zmask");
-      }
-    }
-    TM_HIP(hipStreamSynchronize(st));
-    *n_peaks = found;
+    *n_peaks = peak_loop(
+        h, threshold, edge, min_peak_radius, pixel_size, out, capacity, res.p,
+        [&] {
+          hipLaunchKernelGGL(k_zmax_part, dim3(nparts), dim3(256), 0, st, z, W, H, edge, pv.p, pi.p);
+          check_launch("k_zmax_part");
+          hipLaunchKernelGGL(k_zmax_final, dim3(1), dim3(256), 0, st, pv.p, pi.p, nparts, h->idx.p, res.p);
+          check_launch("k_zmax_final");
+        },
+        [&](int bx, int by, int r, float r2, dim3 grid) {
+          hipLaunchKernelGGL(k_zmask, grid, dim3(64), 0, st, z, W, H, bx, by, r, r2);
+          check_launch("k_zmask");
+        });
+  });
+}
+
+int tm_find_peaks_noalloc(tm_core* h, long long total_ccfs, float threshold, int edge, float min_peak_radius,
+                          float pixel_size, tm_peak_info* out, int capacity, int* n_peaks) {
+  return guarded([&] {
+    require_ready(h);
+    TM_REQUIRE(total_ccfs > 0 && n_peaks && (out || capacity == 0) && capacity >= 0 && edge >= 0, TM_ERR_INVALID,
+               "invalid arguments");
+    const int W = h->W, H = h->H;
+    const long long npx = (long long)W * H;
+    TM_REQUIRE(npx <= TM_NOALLOC_MAX_PIXELS, TM_ERR_INVALID,
+               "search image of " + std::to_string(W) + " x " + std::to_string(H) +
+                   " pixels exceeds the allocation-free peak search's limit of " +
+                   std::to_string(TM_NOALLOC_MAX_PIXELS) + " pixels");
+    unsigned* excluded = static_cast<unsigned*>(scratch_address(h->device));
+    float* pv = reinterpret_cast<float*>(excluded + TM_NOALLOC_MAX_PIXELS / 32);
+    int* pi = reinterpret_cast<int*>(pv + TM_NOALLOC_PARTS);
+    int* res = pi + TM_NOALLOC_PARTS;
+    hipStream_t st = h->stream;
+    const int nparts = std::max(1, std::min(TM_NOALLOC_PARTS, H - 2 * edge));
+    const size_t words = (size_t)(npx + 31) / 32;
+    // the bits of the previous call on this device are still there: clear this image's share
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)std::min<size_t>((words + 255) / 256, 2048)), dim3(256), 0, st,
+                       excluded, words, 0u);
+    check_launch("k_fill_u32");
+    const float n_ccf = (float)total_ccfs, sqrt_px = (float)sqrt((double)(W * H));
+    *n_peaks = peak_loop(
+        h, threshold, edge, min_peak_radius, pixel_size, out, capacity, res,
+        [&] {
+          hipLaunchKernelGGL(k_zmax_part_bits, dim3(nparts), dim3(256), 0, st, h->mip.p, h->sum.p, h->sumsq.p,
+                             excluded, W, H, edge, n_ccf, sqrt_px, pv, pi);
+          check_launch("k_zmax_part_bits");
+          hipLaunchKernelGGL(k_zmax_final, dim3(1), dim3(256), 0, st, pv, pi, nparts, h->idx.p, res);
+          check_launch("k_zmax_final");
+        },
+        [&](int bx, int by, int r, float r2, dim3 grid) {
+          hipLaunchKernelGGL(k_zmask_bits, grid, dim3(64), 0, st, excluded, W, H, bx, by, r, r2);
+          check_launch("k_zmask_bits");
+        });
   });
 }
 

@@ -1400,26 +1400,35 @@ static __global__ void k_add_u64(unsigned long long* __restrict__ a, const unsig
   if (i < n) a[i] += b[i];
 }
 
-// match_template.cpp:1221-1243, 1300-1310 with the reference's float/double mix.
+// One pixel of the finalisation: match_template.cpp:1221-1243, 1300-1310 with the reference's
+// float/double mix (scaled MIP, z-score, average and standard deviation).
+struct FinalPixel {
+  float m, z, avg, sd;
+};
+__device__ inline FinalPixel finalize_pixel(float mip, double sum, double sumsq, float n_ccf, float sqrt_px) {
+  double s = sum / n_ccf;
+  double q = sumsq / n_ccf - (double)powf((float)s, 2);
+  if (q > 0.0f) q = (double)(sqrtf((float)q) * sqrt_px);
+  else q = 0.0f;
+  s *= sqrt_px;
+  const float m = mip * sqrt_px;
+  float zz = (float)((double)m - s);
+  if (q > 0.0f) zz = (float)((double)zz / q);
+  else zz = 0.0f;
+  return {m, zz, (float)s, (float)q};
+}
+
 static __global__ void k_finalize(const float* __restrict__ mip, const double* __restrict__ sum,
                            const double* __restrict__ sumsq, size_t n, float n_ccf, float sqrt_px,
                            float* __restrict__ mip_scaled, float* __restrict__ z, float* __restrict__ avg,
                            float* __restrict__ sd) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double s = sum[i] / n_ccf;
-  double q = sumsq[i] / n_ccf - (double)powf((float)s, 2);
-  if (q > 0.0f) q = (double)(sqrtf((float)q) * sqrt_px);
-  else q = 0.0f;
-  s *= sqrt_px;
-  const float m = mip[i] * sqrt_px;
-  float zz = (float)((double)m - s);
-  if (q > 0.0f) zz = (float)((double)zz / q);
-  else zz = 0.0f;
-  mip_scaled[i] = m;
-  z[i] = zz;
-  avg[i] = (float)s;
-  sd[i] = (float)q;
+  const FinalPixel f = finalize_pixel(mip[i], sum[i], sumsq[i], n_ccf, sqrt_px);
+  mip_scaled[i] = f.m;
+  z[i] = f.z;
+  avg[i] = f.avg;
+  sd[i] = f.sd;
 }
 
 }  // namespace tmk

@@ -202,6 +202,21 @@ typedef struct tm_peak_info {
 int tm_find_peaks(tm_core* h, long long total_ccfs, float threshold, int edge, float min_peak_radius, float pixel_size,
                   tm_peak_info* out, int capacity, int* n_peaks);
 
+/* Allocation-free tm_find_peaks. No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments and the same peaks, bit for bit, as tm_find_peaks; it calls no hipMalloc,
+ * hipFree or allocating library routine. Its working memory is a static device array of
+ * TM_NOALLOC_SCRATCH_BYTES that each GPU the library runs on holds until the process ends (one bit
+ * per search pixel for the exclusion discs, then the reduction partials), so the search image may
+ * hold at most TM_NOALLOC_MAX_PIXELS pixels (4096 x 4096); a larger one gets TM_ERR_INVALID. Every
+ * call on one GPU shares that GPU's array: calls that overlap in time on one GPU (from several
+ * threads or handles) race, and the caller must serialise them. */
+#define TM_NOALLOC_MAX_PIXELS (4096LL * 4096LL)
+#define TM_NOALLOC_PARTS 1024
+#define TM_NOALLOC_SCRATCH_BYTES (TM_NOALLOC_MAX_PIXELS / 8 + TM_NOALLOC_PARTS * 8 + 256)
+int tm_find_peaks_noalloc(tm_core* h, long long total_ccfs, float threshold, int edge, float min_peak_radius,
+                          float pixel_size, tm_peak_info* out, int capacity, int* n_peaks);
+
 /* Checkpoint and resume of a search, with no counterpart in the reference (whose jobs are restarted
  * from their position range, match_template.cpp:378-379, 735-739).
  *
