@@ -147,7 +147,22 @@ def main():
                     help="proposal: load the network (raw float32 parameters) instead of training")
     ap.add_argument("--save-weights", type=str, default="", help="proposal: save the trained network as raw float32 parameters")
     ap.add_argument("--seed", type=int, default=-1, help="full_search: micrograph seed (default 1000)")
-    ap.add_argument("--workload", choices=["config1", "config2", "refine", "proposal", "rocfft", "cnn", "full_search"],
+    ap.add_argument("--records", type=str, default="",
+                    help="proposal4096: append one JSON record per held-out micrograph to this file")
+    ap.add_argument("--summarize", type=str, default="",
+                    help="proposal4096: only summarise the comma-separated record files (no GPU) and exit")
+    ap.add_argument("--eval-offset", type=int, default=0, help="proposal4096: first held-out micrograph index")
+    ap.add_argument("--judge-cache", type=str, default="",
+                    help="proposal4096: take the judge's peaks and time of a micrograph from an earlier record file")
+    ap.add_argument("--variants", type=str, default="",
+                    help="proposal4096: the learned-path variants to time, as kind:K:radius (network, binned_peaks, "
+                         "union, random) or network_bins:K:bins, comma-separated")
+    ap.add_argument("--union-network", type=int, default=32,
+                    help="proposal4096: network cells added to the binned search's peaks in the union variant")
+    ap.add_argument("--radii", type=str, default="7.5",
+                    help="proposal4096: rotation radii (degrees) of the local refinement around the coarse angles")
+    ap.add_argument("--workload", choices=["config1", "config2", "refine", "proposal", "proposal4096", "rocfft", "cnn",
+                                           "full_search"],
                     default="config1",
                     help="config2: eight micrographs of mixed sizes (own JSON line, not the headline)")
     ap.add_argument("--concurrency", type=str, default="1,2,4",
@@ -174,9 +189,16 @@ def main():
               "reduction of the per-pixel state).", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "proposal4096" and args.summarize:
+        summarize_proposal4096(args.summarize.split(","))
+        return
+
     from cistem_downstream_bah_amd import core, synth
     from cistem_downstream_bah_amd import match_template as MT
 
+    if args.workload == "proposal4096":
+        bench_proposal4096(args)
+        return
     if args.workload == "config2":
         bench_config2(args)
         return
@@ -647,6 +669,208 @@ def bench_proposal(args):
     }
     net.close()
     print(json.dumps(out), flush=True)
+
+
+def bench_proposal4096(args):
+    """Learned proposals at the scale of the 4096^2 search (cistem_downstream_bah_amd.proposal_large):
+    the network reads each micrograph binned 8 x 8 (512^2), proposes cells of 64 px with orientation
+    bins, and every orientation of the exhaustive grid in those bins is scored at those cells (local
+    exhaustive refinement); the judge is the truncated exhaustive search (200,000 orientations x 11
+    planes, z >= 7.5 peaks). Training micrographs are labelled by the exhaustive search on the
+    micrograph and template binned 4 x 4 (positions up to theta 50 deg at 5 deg). Both paths are timed
+    from the raw micrograph on the host to their peak lists."""
+    from cistem_downstream_bah_amd import cnn, proposal as PR, proposal_large as PL
+
+    study = PL.LargeStudy()
+    net = cnn.ProposalNet(PL.default_net(), max_batch=16, seed=1)
+    log(f"network: {net.n_params} parameters, grid {net.grid}; judge orientations {study.orients.shape[0]}, "
+        f"reachable bins {int(study.reachable.sum())}")
+    training = None
+    if args.weights:
+        net.set_params(np.fromfile(args.weights, np.float32))
+        training = {"loaded_from": os.path.basename(args.weights)}
+        log(f"network loaded from {args.weights}")
+    else:
+        t0 = time.time()
+        inputs, labels = [], []
+        t_gen = t_lab = 0.0
+        for i in range(args.train):
+            t1 = time.time()
+            m = study.micrograph(i).micrograph
+            t2 = time.time()
+            labels.append(study.label(m))
+            inputs.append(study.net_input(m))
+            t_gen += t2 - t1
+            t_lab += time.time() - t2
+            if (i + 1) % 25 == 0 or i + 1 == args.train:
+                log(f"labelled {i + 1} / {args.train}: {np.mean([len(p) for p in labels]):.1f} peaks per micrograph, "
+                    f"generation {t_gen / (i + 1):.2f}s, label {t_lab / (i + 1):.2f}s each")
+        t1 = time.time()
+        hist = study.train(net, inputs, labels, epochs=args.epochs, batch=16, lr=1e-3, log=log)
+        training = {"micrographs": args.train, "epochs": args.epochs,
+                    "label_peaks_per_micrograph": round(float(np.mean([len(p) for p in labels])), 2),
+                    "generate_and_label_seconds": round(t1 - t0, 1), "train_seconds": round(time.time() - t1, 1),
+                    "loss_first_epoch": [round(v, 4) for v in hist[0]],
+                    "loss_last_epoch": [round(v, 4) for v in hist[-1]]}
+        del inputs, labels
+        if args.save_weights:
+            net.params().tofile(args.save_weights)
+            log(f"weights saved to {args.save_weights}")
+    Ks = [int(k) for k in args.proposals.split(",")]
+    radii = [float(r) for r in args.radii.split(",")]
+    # (locations, K, radius in degrees or orientation bins per location)
+    variants = [("network", k, radii[0]) for k in Ks] + [("binned_peaks", k, r) for k in Ks for r in radii]
+    variants += [("union", k, r) for k in Ks for r in radii]
+    variants += [("random", Ks[-1], radii[0]), ("network_bins", Ks[-1], int(args.bins.split(",")[0]))]
+    if args.variants:  # kind:K:radius (or kind:K:bins for network_bins), comma-separated
+        variants = [(v.split(":")[0], int(v.split(":")[1]),
+                     int(v.split(":")[2]) if v.startswith("network_bins") else float(v.split(":")[2]))
+                    for v in args.variants.split(",")]
+    rng = np.random.default_rng(7 + args.eval_offset)
+    records = []
+    cached = {}
+    if args.judge_cache:
+        with open(args.judge_cache) as f:
+            cached = {r["seed"]: r for r in map(json.loads, f) if "judge_peaks" in r}
+    for j in range(args.eval):
+        seed = 500_000 + args.eval_offset + j
+        t1 = time.time()
+        gen_t = {}
+        syn = study.micrograph(seed, timing=gen_t)
+        m = syn.micrograph
+        log(f"micrograph {seed} generated in {time.time() - t1:.2f}s " +
+            ("on the device: " + ", ".join(f"{k} {v:.3f}s" for k, v in gen_t.items()) if gen_t else "on the host"))
+        parts = np.array([(p.x, p.y) for p in syn.particles])
+        if j == 0:  # one-off handle set-up of both learned paths, outside the timing
+            study.detect(net, m, Ks[0], 1)
+            study.detect_coarse(net, m, Ks[0], radii[0])
+        if seed in cached:  # a trial of the learned path against an earlier run's judge of this micrograph
+            c = cached[seed]
+            peaks, t_plain, n_ccf = c["judge_peaks"], c["plain_s"], c["ccfs"]
+            log(f"micrograph {seed}: judge peaks and time from {args.judge_cache}")
+        else:
+            peaks, t_plain, n_ccf = study.judge(m, log=log)
+        # a judge peak is true when an inserted particle lies within 3 px of it
+        true = [bool(((parts - (p["x"], p["y"])) ** 2).sum(1).min() <= 9.0) for p in peaks]
+        rec = {"seed": seed, "plain_s": round(t_plain, 3), "ccfs": n_ccf, "judge_cached": seed in cached,
+               "peaks": [[round(p["x"]), round(p["y"]), round(p["z"], 3), round(p["theta"], 2), t]
+                         for p, t in zip(peaks, true)],
+               "judge_peaks": peaks, "variants": []}
+        for kind, k, par in variants:
+            if kind == "network_bins":
+                dets, n, t, split = study.detect(net, m, k, par)
+                quality = None
+            else:
+                dets, n, t, split = study.detect_coarse(net, m, k, par, locations=kind, rng=rng,
+                                                        k_network=args.union_network)
+                quality = study.proposal_quality(peaks)
+            row = {"kind": kind, "k": k, "m": par, "s": round(t, 4), "scores": n,
+                   "split": {key: round(v, 4) for key, v in split.items()}}
+            for rad in (2.0, 3.0):
+                mm = PR.match(dets, peaks, rad)
+                row[f"hit{int(rad)}"] = int(sum(f for f, _ in mm))
+                if rad == 2.0:
+                    row["hit2_true"] = int(sum(f for (f, _), tr in zip(mm, true) if tr))
+                    row["same_orientation"] = int(sum(sm for f, sm in mm if f))
+                    row["misses"] = [[round(p["z"], 2), round(p["theta"], 1), tr] +
+                                     (list(quality[i]) if quality is not None else [])
+                                     for i, ((f, _), p, tr) in enumerate(zip(mm, peaks, true)) if not f]
+            if quality is not None:
+                row["located"] = int(sum(a for a, _ in quality))
+                row["orientation_covered"] = int(sum(b for _, b in quality))
+            rec["variants"].append(row)
+            log(f"micrograph {seed}: {kind} K={k} {par}: {row['hit2']}/{len(peaks)} within 2 px, "
+                f"{row['hit2_true']}/{sum(true)} true (located {row.get('located')}, covered "
+                f"{row.get('orientation_covered')}), {t:.3f}s, {n} scores, {row['split']}, misses {row['misses']}")
+        records.append(rec)
+        if args.records:
+            with open(args.records, "a") as f:
+                f.write(json.dumps(rec) + "\n")
+    net.close()
+    out = proposal4096_line(records)
+    out["training"] = training
+    print(json.dumps(out), flush=True)
+
+
+def proposal4096_line(records: list) -> dict:
+    """The JSON line of the proposal4096 workload over per-micrograph records. Learned rows use the
+    network's locations (network, union, network_bins); binned_peaks uses none (the coarse-to-fine
+    search alone); random is the ablation of the network's locations. The chosen row is the fastest
+    learned row whose recall of all z >= 7.5 peaks within 2 px is at least 0.99, else the learned row
+    with the best recall."""
+    ne = len(records)
+    n_peaks = sum(len(r["peaks"]) for r in records)
+    n_true = sum(sum(1 for p in r["peaks"] if p[4]) for r in records)
+    plain = sum(r["plain_s"] for r in records) / max(ne, 1)
+    cached_judges = sum(1 for r in records if r.get("judge_cached"))
+    table = []
+    keys = [(v["kind"], v["k"], v["m"]) for v in records[0]["variants"]] if records else []
+    for key in keys:
+        rows = [v for r in records for v in r["variants"] if (v["kind"], v["k"], v["m"]) == key]
+        s = sum(v["s"] for v in rows) / max(ne, 1)
+        table.append({"locations_from": key[0], "locations": key[1],
+                      ("orientation_bins" if key[0] == "network_bins" else "radius_deg"): key[2],
+                      "learned": key[0] in ("network", "union", "network_bins"),
+                      "s_per_micrograph": round(s, 4),
+                      "template_scores_per_micrograph": sum(v["scores"] for v in rows) // max(ne, 1),
+                      "recall_2px": round(sum(v["hit2"] for v in rows) / max(n_peaks, 1), 4),
+                      "recall_2px_true_peaks": round(sum(v["hit2_true"] for v in rows) / max(n_true, 1), 4),
+                      "recall_3px": round(sum(v["hit3"] for v in rows) / max(n_peaks, 1), 4),
+                      "same_orientation_at_2px": round(sum(v.get("same_orientation", 0) for v in rows)
+                                                       / max(sum(v["hit2"] for v in rows), 1), 4),
+                      "located": (round(sum(v["located"] for v in rows) / max(n_peaks, 1), 4)
+                                  if "located" in rows[0] else None),
+                      "orientation_covered": (round(sum(v["orientation_covered"] for v in rows) / max(n_peaks, 1), 4)
+                                              if "located" in rows[0] else None),
+                      "speedup_vs_plain": round(plain / s, 1) if s > 0 else None,
+                      "misses_2px [z, theta, true, located, covered]": [m for v in rows for m in v["misses"]],
+                      "split_s": {k: round(sum(v["split"][k] for v in rows) / max(ne, 1), 4)
+                                  for k in rows[0]["split"]}})
+    learned = [r for r in table if r["learned"]]
+    ok = [r for r in learned if r["recall_2px"] >= 0.99]
+    pick = (min(ok, key=lambda r: r["s_per_micrograph"]) if ok else
+            max(learned, key=lambda r: (r["recall_2px"], -r["s_per_micrograph"])) if learned else None)
+    plain_rows = [r for r in table if r["locations_from"] == "binned_peaks"]
+    ok = [r for r in plain_rows if r["recall_2px"] >= 0.99]
+    coarse = (min(ok, key=lambda r: r["s_per_micrograph"]) if ok else
+              max(plain_rows, key=lambda r: (r["recall_2px"], -r["s_per_micrograph"])) if plain_rows else None)
+    return {
+        "metric": "s/micrograph, learned proposals + local exhaustive refinement vs plain exhaustive search "
+                  "(truncated 4096^2 search), at recall of its z>=7.5 peaks",
+        "value": pick["s_per_micrograph"] if pick else None, "unit": "s/micrograph",
+        "n_gpus": 1, "steps": ne, "warmup": 1,
+        "ms_per_step": round(1e3 * pick["s_per_micrograph"], 1) if pick else None,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16 network, fp32 scores and search",
+        "data": "synthetic (seeded 4096^2 micrographs, 300 copies of one seeded 256^3 template, SNR 0.05; "
+                "training labels from the exhaustive search of the 4x4-binned micrograph and template)",
+        "config": {"workload": "proposal_config1_4096sq_T256", "image": [4096, 4096], "template": 256,
+                   "network_input": [512, 512], "grid_cell_px": 64,
+                   "judge": "first 200,000 orientations of 2.5/1.5 deg C1 x 11 planes, z >= 7.5",
+                   "eval_micrographs": ne, "peaks_judged": n_peaks, "true_peaks": n_true},
+        "plain_s_per_micrograph": round(plain, 3),
+        "recall_at_chosen": pick["recall_2px"] if pick else None,
+        "recall_true_peaks_at_chosen": pick["recall_2px_true_peaks"] if pick else None,
+        "learned_over_plain_time": round(pick["s_per_micrograph"] / plain, 5) if pick else None,
+        "speedup_vs_plain": pick["speedup_vs_plain"] if pick else None,
+        "chosen": pick, "coarse_to_fine_without_network": coarse, "table": table,
+        "eval_seeds": [r["seed"] for r in records],
+        "judges_taken_from_earlier_runs": cached_judges,
+    }
+
+
+def summarize_proposal4096(files: list):
+    records = []
+    for fn in files:
+        with open(fn) as f:
+            records += [json.loads(line) for line in f if line.strip()]
+    seen = set()
+    uniq = []
+    for r in records:
+        if r["seed"] not in seen:
+            seen.add(r["seed"])
+            uniq.append(r)
+    print(json.dumps(proposal4096_line(uniq)), flush=True)
 
 
 def bench_full_search(args):

@@ -16,6 +16,7 @@ exactly that, so the ground-truth angles are directly comparable with the search
 """
 from __future__ import annotations
 
+import time
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -95,13 +96,16 @@ def ctf_2d(ny: int, nx: int, pixel_size: float, kv: float, cs_mm: float, amp_con
 def place_particles(rng: np.random.Generator, n: int, W: int, H: int, margin: int, min_dist: float):
     """Random sequential placement with a minimum centre distance (relaxed by 5% when stuck)."""
     pts = []
+    acc = np.empty((n, 2))
     d = float(min_dist)
     tries = 0
     while len(pts) < n:
         x = rng.uniform(margin, W - margin)
         y = rng.uniform(margin, H - margin)
-        if all((x - px) ** 2 + (y - py) ** 2 >= d * d for px, py in pts):
+        k = len(pts)
+        if k == 0 or bool(np.all((x - acc[:k, 0]) ** 2 + (y - acc[:k, 1]) ** 2 >= d * d)):
             pts.append((x, y))
+            acc[k] = (x, y)
             tries = 0
         else:
             tries += 1
@@ -182,15 +186,19 @@ def make_synthetic(seed: int, W: int, H: int, T: int, n_atoms: int, atom_radius:
     return Synthetic(noisy.astype(np.float32), vol, parts, noise_sigma, var, used, defocus)
 
 
+_CTF_CACHE: dict = {}
+
+
 def make_synthetic_gpu(seed: int, W: int, H: int, T: int, n_atoms: int, atom_radius: float, n_copies: int,
                        snr: float, defocus: float, pixel_size: float = 1.0, kv: float = 300.0, cs_mm: float = 2.7,
                        amp_contrast: float = 0.07, sigma: float = 1.5, min_dist: float | None = None,
-                       template: tuple | None = None) -> Synthetic:
+                       template: tuple | None = None, timing: dict | None = None) -> Synthetic:
     """make_synthetic with the copies' projections, the CTF and the noise scaling computed on the GPU in
     float64 (torch): the same random draws in the same order (template, positions, orientations, then
     the noise from the numpy generator), so the micrograph equals make_synthetic's up to float64
     rounding. For the workloads that need hundreds of 4096^2 micrographs. Falls back to make_synthetic
-    when no GPU is visible."""
+    when no GPU is visible. timing: a dict that receives the seconds of each phase (the device is
+    synchronised after each phase)."""
     try:
         import torch
         if not torch.cuda.is_available():
@@ -199,6 +207,15 @@ def make_synthetic_gpu(seed: int, W: int, H: int, T: int, n_atoms: int, atom_rad
         return make_synthetic(seed, W, H, T, n_atoms, atom_radius, n_copies, snr, defocus, pixel_size, kv, cs_mm,
                               amp_contrast, sigma, min_dist, template)
     dev = torch.device("cuda")
+    clock = [time.perf_counter()]
+
+    def phase(name):
+        if timing is not None:
+            torch.cuda.synchronize()
+            now = time.perf_counter()
+            timing[name] = timing.get(name, 0.0) + now - clock[0]
+            clock[0] = now
+
     rng = np.random.default_rng(seed)
     if template is None:
         pos, amp = make_atoms(rng, n_atoms, atom_radius)
@@ -208,33 +225,57 @@ def make_synthetic_gpu(seed: int, W: int, H: int, T: int, n_atoms: int, atom_rad
     if min_dist is None:
         min_dist = T
     centres, used = place_particles(rng, n_copies, W, H, T // 2, min_dist)
+    phase("placement")
     box = int(2 * (atom_radius + 4 * sigma)) + 2
     img = torch.zeros((H, W), dtype=torch.float64, device=dev)
     s2 = 2.0 * sigma * sigma
     w = torch.tensor(amp * sigma * np.sqrt(2.0 * np.pi), dtype=torch.float64, device=dev)
-    parts = []
-    for (cx, cy) in centres:
-        phi, theta, psi = random_euler(rng)
-        m = euler_matrix(phi, theta, psi)
-        proj = pos @ m
-        x0 = max(0, int(np.floor(cx)) - box // 2)
-        y0 = max(0, int(np.floor(cy)) - box // 2)
-        x1 = min(W, int(np.floor(cx)) + box // 2 + 1)
-        y1 = min(H, int(np.floor(cy)) + box // 2 + 1)
-        px = torch.tensor(cx + proj[:, 0], dtype=torch.float64, device=dev)
-        py = torch.tensor(cy + proj[:, 1], dtype=torch.float64, device=dev)
-        xs = torch.arange(x0, x1, dtype=torch.float64, device=dev)
-        ys = torch.arange(y0, y1, dtype=torch.float64, device=dev)
-        gx = torch.exp(-((xs[None, :] - px[:, None]) ** 2) / s2)
-        gy = torch.exp(-((ys[None, :] - py[:, None]) ** 2) / s2)
-        img[y0:y1, x0:x1] += (gy.T * w[None, :]).contiguous() @ gx.contiguous()
-        parts.append(Particle(float(cx), float(cy), phi, theta, psi))
-    ctf = torch.tensor(ctf_2d(H, W, pixel_size, kv, cs_mm, amp_contrast, defocus), dtype=torch.float64, device=dev)
+    eulers = [random_euler(rng) for _ in range(len(centres))]  # the host generator's draw order
+    parts = [Particle(float(cx), float(cy), *e) for (cx, cy), e in zip(centres, eulers)]
+    lo = [(int(np.floor(cx)) - box // 2, int(np.floor(cy)) - box // 2) for cx, cy in centres]
+    inside = all(x0 >= 0 and y0 >= 0 and x0 + box + 1 <= W and y0 + box + 1 <= H for x0, y0 in lo)
+    chunk = 32 if inside else 1
+    nb = box + 1
+    for c0 in range(0, len(centres), chunk):
+        cs = range(c0, min(c0 + chunk, len(centres)))
+        proj = np.stack([pos @ euler_matrix(*eulers[i]) for i in cs])  # (copies, atoms, 3)
+        px = torch.tensor(np.array([centres[i][0] for i in cs])[:, None] + proj[:, :, 0], dtype=torch.float64, device=dev)
+        py = torch.tensor(np.array([centres[i][1] for i in cs])[:, None] + proj[:, :, 1], dtype=torch.float64, device=dev)
+        if inside:  # every box lies inside the image: one batched product per chunk of copies
+            x0 = torch.tensor([lo[i][0] for i in cs], dtype=torch.float64, device=dev)
+            y0 = torch.tensor([lo[i][1] for i in cs], dtype=torch.float64, device=dev)
+            r = torch.arange(nb, dtype=torch.float64, device=dev)
+            xs, ys = x0[:, None] + r[None, :], y0[:, None] + r[None, :]
+        else:
+            i = cs[0]
+            cx, cy = centres[i]
+            x0, y0 = max(0, lo[i][0]), max(0, lo[i][1])
+            x1 = min(W, int(np.floor(cx)) + box // 2 + 1)
+            y1 = min(H, int(np.floor(cy)) + box // 2 + 1)
+            xs = torch.arange(x0, x1, dtype=torch.float64, device=dev)[None]
+            ys = torch.arange(y0, y1, dtype=torch.float64, device=dev)[None]
+        gx = torch.exp(-((xs[:, None, :] - px[:, :, None]) ** 2) / s2)  # (copies, atoms, nx)
+        gy = torch.exp(-((ys[:, None, :] - py[:, :, None]) ** 2) / s2)
+        tiles = torch.bmm((gy * w[None, :, None]).transpose(1, 2).contiguous(), gx.contiguous())
+        for k, i in enumerate(cs):  # added copy by copy, in the host generator's order
+            xa, ya = max(0, lo[i][0]), max(0, lo[i][1])
+            img[ya:ya + tiles.shape[1], xa:xa + tiles.shape[2]] += tiles[k]
+    phase("projections")
+    key = (H, W, pixel_size, kv, cs_mm, amp_contrast, defocus)
+    ctf = _CTF_CACHE.get(key)
+    if ctf is None:
+        ctf = torch.tensor(ctf_2d(*key), dtype=torch.float64, device=dev)
+        _CTF_CACHE.clear()  # one micrograph shape at a time
+        _CTF_CACHE[key] = ctf
     signal = torch.fft.irfft2(torch.fft.rfft2(img) * ctf, s=(H, W))
     var = float(signal.var(unbiased=False))
     noise_sigma = float(np.sqrt(var / snr)) if snr > 0 else 0.0
-    noise = torch.from_numpy(rng.standard_normal((H, W))).to(dev)
+    phase("ctf")
+    noise = rng.standard_normal((H, W))
+    phase("noise_draw")
+    noise = torch.from_numpy(noise).to(dev)
     noisy = (signal + noise_sigma * noise).to(torch.float32).cpu().numpy()
+    phase("noise_add_and_copy")
     return Synthetic(noisy, vol, parts, noise_sigma, var, used, defocus)
 
 
