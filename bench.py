@@ -13,7 +13,8 @@ micrographs of mixed sizes), refine (refine_template on the 4096^2 micrograph's 
 (learned proposals + local refinement against the exhaustive search at 512^2 / 64^3), rocfft (the
 column pass against the vendor FFT), cnn (proposal-network training steps, 16 images of 512^2),
 full_search (the whole truncated config-1 search, 200,000 orientations x 11 planes, once: seconds per
-micrograph).
+micrograph), config3 (the headline workload with FP32, FP16 and BF16 planes: throughput and error vs
+FP32), proposal4096 (learned proposals at 4096^2 against the truncated exhaustive search).
 
 Prints one JSON line (rank 0, N = 1). `--gpus N` with N > 1 exits: this build runs on one GPU.
 """
@@ -161,8 +162,8 @@ def main():
                     help="proposal4096: network cells added to the binned search's peaks in the union variant")
     ap.add_argument("--radii", type=str, default="7.5",
                     help="proposal4096: rotation radii (degrees) of the local refinement around the coarse angles")
-    ap.add_argument("--workload", choices=["config1", "config2", "refine", "proposal", "proposal4096", "rocfft", "cnn",
-                                           "full_search"],
+    ap.add_argument("--workload", choices=["config1", "config2", "config3", "refine", "proposal", "proposal4096", "rocfft",
+                                           "cnn", "full_search"],
                     default="config1",
                     help="config2: eight micrographs of mixed sizes (own JSON line, not the headline)")
     ap.add_argument("--concurrency", type=str, default="1,2,4",
@@ -201,6 +202,9 @@ def main():
         return
     if args.workload == "config2":
         bench_config2(args)
+        return
+    if args.workload == "config3":
+        bench_config3(args)
         return
     if args.workload == "refine":
         bench_refine(args)
@@ -373,6 +377,82 @@ def main():
         "graph": graph,
         "pipeline": {"algorithmic_bytes_per_ccf": pipe,
                      "fraction_of_hbm_peak": round(pipe * value / HBM_PEAK, 4)},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_config3(args):
+    """Mixed precision at the 4096^2 shape: the headline workload (positions 0-7 x 241 psi, 11 defocus
+    planes, batch 64) with the inter-pass plane stored in FP32, FP16 and BF16. For each: throughput
+    over the timed steps, then one accumulation of all 11 planes whose MIP, z-score, best orientation
+    and peaks (z at or above the search's expected threshold, 10-pixel exclusion) are compared with
+    the FP32 run's."""
+    from cistem_downstream_bah_amd import synth
+    from cistem_downstream_bah_amd import match_template as MT
+    from cistem_downstream_bah_amd import peaks as PK
+
+    s = synth.config1(1000)
+    tpl = MT.prepare_template(s.volume, 2.0)
+    runs = {}
+    for st in ("fp32", "fp16", "bf16"):
+        sp = MT.SearchParams(angular_step=2.5, in_plane_angular_step=1.5, padding=2.0, defocus_search_range=1000.0,
+                             defocus_step=200.0, first_search_position=0, last_search_position=args.positions - 1,
+                             batch_size=args.batch, plane_storage=st)
+        search = MT.MatchTemplateSearch(s.micrograph, s.volume, sp, template_spectrum=tpl)
+        g = search.core
+        planes = list(range(-search.n_def_half, search.n_def_half + 1))
+        filters = {d: search.filter_for(d) for d in planes}
+        for w in range(args.warmup):
+            g.run_inner_loop(filters[planes[w % len(planes)]], 0, planes[w % len(planes)])
+        g.reset()
+        ccfs = 0
+        t0 = time.perf_counter()
+        for k in range(args.steps):
+            d = planes[k % len(planes)]
+            ccfs += g.run_inner_loop(filters[d], 0, d)
+        elapsed = time.perf_counter() - t0
+        g.reset()
+        n = sum(g.run_inner_loop(filters[d], 0, d) for d in planes)
+        r = g.results()
+        z = g.finalize(n)["zscore"]
+        thr = PK.expected_threshold(search.W, search.H, n)
+        pk = g.find_peaks(n, thr, search.T // 4 + 1, 10.0)
+        runs[st] = {"ccf_per_s": ccfs / elapsed, "ms_per_step": 1e3 * elapsed / args.steps, "r": r, "z": z,
+                    "peaks": pk, "threshold": thr, "column_pass": g.column_pass_name(), "n": n}
+        log(f"{st}: {ccfs / elapsed:.1f} CCF/s ({g.column_pass_name()}), {len(pk)} peaks at z >= {thr:.2f}")
+        g.close()
+    r32, z32 = runs["fp32"]["r"], runs["fp32"]["z"]
+    scale = float(np.abs(r32.mip).max())
+    table = {}
+    for st, v in runs.items():
+        row = {"ccf_per_s": round(v["ccf_per_s"], 2), "ms_per_step": round(v["ms_per_step"], 3),
+               "column_pass": v["column_pass"], "peaks": len(v["peaks"])}
+        if st != "fp32":
+            d = np.abs(v["r"].mip.astype(np.float64) - r32.mip) / scale
+            dz = np.abs(v["z"].astype(np.float64) - z32)
+            p32 = {(p["x"], p["y"]): (p["phi"], p["theta"], p["psi"]) for p in runs["fp32"]["peaks"]}
+            p16 = {(p["x"], p["y"]): (p["phi"], p["theta"], p["psi"]) for p in v["peaks"]}
+            common = set(p32) & set(p16)
+            row.update({"mip_err_max_of_max": float(f"{d.max():.3e}"),
+                        "mip_err_rms_of_max": float(f"{np.sqrt((d ** 2).mean()):.3e}"),
+                        "z_err_max": float(f"{dz.max():.3e}"), "z_err_rms": float(f"{np.sqrt((dz ** 2).mean()):.3e}"),
+                        "best_index_equal_fraction": round(float((v["r"].best_index == r32.best_index).mean()), 5),
+                        "peaks_also_in_fp32": len(common), "fp32_peaks_missing": len(set(p32) - set(p16)),
+                        "peaks_same_orientation": sum(p32[k] == p16[k] for k in common),
+                        "speedup_vs_fp32": round(v["ccf_per_s"] / runs["fp32"]["ccf_per_s"], 4)})
+        table[st] = row
+    best = max(("fp16", "bf16"), key=lambda st: runs[st]["ccf_per_s"])
+    out = {
+        "metric": "correlations/sec @4096^2 (CCF/s) with the inter-pass plane in 16 bits, and its error vs FP32",
+        "value": round(runs[best]["ccf_per_s"], 2), "unit": "CCF/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(runs[best]["ms_per_step"], 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": best + " plane, fp32 butterflies, fp64 sums",
+        "data": "synthetic (seeded Gaussian pseudo-atom template and micrograph)",
+        "config": {"workload": "config3_4096sq_T256_P512_16bit_plane", "image": [4096, 4096], "template": 256,
+                   "padded_template": 512, "orientations_per_step": 241 * args.positions, "defocus_planes": 11,
+                   "batch": args.batch, "accuracy_run_ccfs": runs["fp32"]["n"],
+                   "peak_threshold": round(runs["fp32"]["threshold"], 3)},
+        "table": table,
     }
     print(json.dumps(out), flush=True)
 
@@ -796,16 +876,25 @@ def proposal4096_line(records: list) -> dict:
     """The JSON line of the proposal4096 workload over per-micrograph records. Learned rows use the
     network's locations (network, union, network_bins); binned_peaks uses none (the coarse-to-fine
     search alone); random is the ablation of the network's locations. The chosen row is the fastest
-    learned row whose recall of all z >= 7.5 peaks within 2 px is at least 0.99, else the learned row
-    with the best recall."""
+    learned row whose recall of all z >= 7.5 peaks within 2 px is at least 0.99, else the fastest whose
+    recall of the judge's true peaks (those with an inserted particle within 3 px) is, else the learned
+    row with the best recall: the exhaustive search's own noise peaks at z >= 7.5 (about one in fifteen
+    here) cannot be recalled by a search that skips any position."""
     ne = len(records)
     n_peaks = sum(len(r["peaks"]) for r in records)
     n_true = sum(sum(1 for p in r["peaks"] if p[4]) for r in records)
     plain = sum(r["plain_s"] for r in records) / max(ne, 1)
     cached_judges = sum(1 for r in records if r.get("judge_cached"))
     table = []
-    keys = [(v["kind"], v["k"], v["m"]) for v in records[0]["variants"]] if records else []
+    keys = []
+    for r in records:
+        for v in r["variants"]:
+            if (v["kind"], v["k"], v["m"]) not in keys:
+                keys.append((v["kind"], v["k"], v["m"]))
     for key in keys:
+        recs = [r for r in records if any((v["kind"], v["k"], v["m"]) == key for v in r["variants"])]
+        if len(recs) < ne:
+            continue  # every row covers every micrograph of the line
         rows = [v for r in records for v in r["variants"] if (v["kind"], v["k"], v["m"]) == key]
         s = sum(v["s"] for v in rows) / max(ne, 1)
         table.append({"locations_from": key[0], "locations": key[1],
@@ -826,14 +915,19 @@ def proposal4096_line(records: list) -> dict:
                       "misses_2px [z, theta, true, located, covered]": [m for v in rows for m in v["misses"]],
                       "split_s": {k: round(sum(v["split"][k] for v in rows) / max(ne, 1), 4)
                                   for k in rows[0]["split"]}})
-    learned = [r for r in table if r["learned"]]
-    ok = [r for r in learned if r["recall_2px"] >= 0.99]
-    pick = (min(ok, key=lambda r: r["s_per_micrograph"]) if ok else
-            max(learned, key=lambda r: (r["recall_2px"], -r["s_per_micrograph"])) if learned else None)
-    plain_rows = [r for r in table if r["locations_from"] == "binned_peaks"]
-    ok = [r for r in plain_rows if r["recall_2px"] >= 0.99]
-    coarse = (min(ok, key=lambda r: r["s_per_micrograph"]) if ok else
-              max(plain_rows, key=lambda r: (r["recall_2px"], -r["s_per_micrograph"])) if plain_rows else None)
+    def choose(rows):
+        """The fastest row at recall >= 0.99 of all judge peaks; else of the true peaks (the judge's
+        peaks with an inserted particle within 3 px); else the row with the best recall."""
+        for basis in ("recall_2px", "recall_2px_true_peaks"):
+            ok = [r for r in rows if r[basis] >= 0.99]
+            if ok:
+                return min(ok, key=lambda r: r["s_per_micrograph"]), basis
+        if not rows:
+            return None, None
+        return max(rows, key=lambda r: (r["recall_2px_true_peaks"], -r["s_per_micrograph"])), "best available"
+
+    pick, basis = choose([r for r in table if r["learned"]])
+    coarse, coarse_basis = choose([r for r in table if r["locations_from"] == "binned_peaks"])
     return {
         "metric": "s/micrograph, learned proposals + local exhaustive refinement vs plain exhaustive search "
                   "(truncated 4096^2 search), at recall of its z>=7.5 peaks",
@@ -849,11 +943,13 @@ def proposal4096_line(records: list) -> dict:
                    "judge": "first 200,000 orientations of 2.5/1.5 deg C1 x 11 planes, z >= 7.5",
                    "eval_micrographs": ne, "peaks_judged": n_peaks, "true_peaks": n_true},
         "plain_s_per_micrograph": round(plain, 3),
+        "recall_basis": basis,
         "recall_at_chosen": pick["recall_2px"] if pick else None,
         "recall_true_peaks_at_chosen": pick["recall_2px_true_peaks"] if pick else None,
         "learned_over_plain_time": round(pick["s_per_micrograph"] / plain, 5) if pick else None,
         "speedup_vs_plain": pick["speedup_vs_plain"] if pick else None,
-        "chosen": pick, "coarse_to_fine_without_network": coarse, "table": table,
+        "chosen": pick, "coarse_to_fine_without_network": coarse,
+        "coarse_to_fine_without_network_basis": coarse_basis, "table": table,
         "eval_seeds": [r["seed"] for r in records],
         "judges_taken_from_earlier_runs": cached_judges,
     }

@@ -165,8 +165,16 @@ void launch_mid_t(tm_core* h, int nb) {
   const int grid = ((ntiles + 7) / 8) * 8;
   // orientations spread over grid rows until about 2048 blocks are in flight
   const int rows_y = std::max(1, std::min(nb, (2048 + grid - 1) / grid));
-  hipLaunchKernelGGL((tmk::k_mid<HH, KC, NTC>), dim3(grid, rows_y), dim3(KC * NTC), 0, h->stream, h->rows.p, h->inpT.p,
-                     h->Z.p, h->Wh, h->T, nb, ntiles, h->twiddle(HH));
+  const dim3 g(grid, rows_y), bl(KC * NTC);
+  if (h->zst == TM_PLANE_FP16)
+    hipLaunchKernelGGL((tmk::k_mid<HH, KC, NTC, tmk::ZS_F16>), g, bl, 0, h->stream, h->rows.p, h->inpT.p, h->Z.p, h->Wh,
+                       h->T, nb, ntiles, h->twiddle(HH), h->zscale);
+  else if (h->zst == TM_PLANE_BF16)
+    hipLaunchKernelGGL((tmk::k_mid<HH, KC, NTC, tmk::ZS_BF16>), g, bl, 0, h->stream, h->rows.p, h->inpT.p, h->Z.p, h->Wh,
+                       h->T, nb, ntiles, h->twiddle(HH), h->zscale);
+  else
+    hipLaunchKernelGGL((tmk::k_mid<HH, KC, NTC>), g, bl, 0, h->stream, h->rows.p, h->inpT.p, h->Z.p, h->Wh, h->T, nb,
+                       ntiles, h->twiddle(HH), h->zscale);
   check_launch("k_mid");
   }
 }
@@ -189,14 +197,16 @@ int z_tiles(const tm_core* h) { return (h->Wh + h->zt - 1) / h->zt; }
 // The column pass of a handle: the persistent pass with 4-column tiles where it applies (H = 4096,
 // T <= 256, FP32 planes) unless the caller asks for another; the Z tile width follows.
 void column_pass_for(tm_core* h, int requested) {
-  // 16-bit planes are read four elements per 16-byte load by the row pass, so their tiles are 4 wide
-  const bool persistent_ok = h->H == 4096 && h->T <= 256 && h->zst == TM_PLANE_FP32;
-  TM_REQUIRE(requested != TM_COLUMNS_PERSISTENT || persistent_ok, TM_ERR_UNSUPPORTED,
-             "the persistent column pass needs image height 4096, projection size <= 256 and FP32 planes");
+  // 16-bit planes are read four elements per 16-byte load by the row pass, so their tiles are 4 wide:
+  // the persistent pass with 2-column tiles takes FP32 planes only
+  const bool persistent_ok = h->H == 4096 && h->T <= 256;
+  TM_REQUIRE(requested != TM_COLUMNS_PERSISTENT || (persistent_ok && h->zst == TM_PLANE_FP32), TM_ERR_UNSUPPORTED,
+             "the persistent column pass with 2-column tiles needs image height 4096, projection size <= 256 "
+             "and FP32 planes");
   TM_REQUIRE(requested != TM_COLUMNS_PAIRS || h->H == 4096, TM_ERR_UNSUPPORTED,
              "the column-pair pass needs image height 4096");
   TM_REQUIRE(requested != TM_COLUMNS_PERSISTENT4 || persistent_ok, TM_ERR_UNSUPPORTED,
-             "the persistent column pass needs image height 4096, projection size <= 256 and FP32 planes");
+             "the persistent column pass needs image height 4096 and projection size <= 256");
   if (h->H != 4096) {
     h->column_pass = TM_COLUMNS_AUTO;
     h->zt = mid_kc(h->H);
@@ -254,27 +264,42 @@ void launch_mid(tm_core* h, int nb) {
 }
 
 // QR orientations per k_last block (tmk::last_qr). For power-of-two widths from 1024 on the
-// groups are whole wavefronts and the row gather runs as double-buffered direct LDS loads.
+// groups are whole wavefronts and the row gather runs as double-buffered direct LDS loads; those read
+// 16 bytes of one tile row at a time (four 16-bit elements), so 16-bit planes in tiles narrower than
+// 4 columns take the gathering row pass.
 template <int WW, int QR>
 void launch_last_t(tm_core* h, int nb, tmk::EpiParams ep) {
   ep.ztile = h->zt;
   ep.ntiles = z_tiles(h);
   ep.hstep_inv = 1.0 / ep.hstep;
   ep.zinv = 1.0f / h->zscale;  // a power of two: exact
+  const dim3 g(h->H);
   if constexpr (WW >= 1024 && (WW & (WW - 1)) == 0) {
-    if (h->zst == TM_PLANE_FP16)
-      hipLaunchKernelGGL((tmk::k_last_dma<WW, QR, tmk::ZS_F16>), dim3(h->H), dim3(QR * (WW / 16)), 0, h->stream,
-                         h->Z.p, h->H, nb, ep, h->twiddle(WW), h->twiddle(WW / 2));
-    else if (h->zst == TM_PLANE_BF16)
-      hipLaunchKernelGGL((tmk::k_last_dma<WW, QR, tmk::ZS_BF16>), dim3(h->H), dim3(QR * (WW / 16)), 0, h->stream,
-                         h->Z.p, h->H, nb, ep, h->twiddle(WW), h->twiddle(WW / 2));
-    else
-      hipLaunchKernelGGL((tmk::k_last_dma<WW, QR>), dim3(h->H), dim3(QR * (WW / 16)), 0, h->stream, h->Z.p, h->H,
-                         nb, ep, h->twiddle(WW), h->twiddle(WW / 2));
-  } else {
-    hipLaunchKernelGGL((tmk::k_last<WW, QR>), dim3(h->H), dim3(tmk::last_threads<WW, QR>()), 0, h->stream, h->Z.p,
-                       h->H, nb, ep, h->twiddle(WW), h->twiddle(WW / 2));
+    if (h->zst == TM_PLANE_FP32 || h->zt >= 4) {
+      const dim3 bl(QR * (WW / 16));
+      if (h->zst == TM_PLANE_FP16)
+        hipLaunchKernelGGL((tmk::k_last_dma<WW, QR, tmk::ZS_F16>), g, bl, 0, h->stream, h->Z.p, h->H, nb, ep,
+                           h->twiddle(WW), h->twiddle(WW / 2));
+      else if (h->zst == TM_PLANE_BF16)
+        hipLaunchKernelGGL((tmk::k_last_dma<WW, QR, tmk::ZS_BF16>), g, bl, 0, h->stream, h->Z.p, h->H, nb, ep,
+                           h->twiddle(WW), h->twiddle(WW / 2));
+      else
+        hipLaunchKernelGGL((tmk::k_last_dma<WW, QR>), g, bl, 0, h->stream, h->Z.p, h->H, nb, ep, h->twiddle(WW),
+                           h->twiddle(WW / 2));
+      check_launch("k_last_dma");
+      return;
+    }
   }
+  const dim3 bl(tmk::last_threads<WW, QR>());
+  if (h->zst == TM_PLANE_FP16)
+    hipLaunchKernelGGL((tmk::k_last<WW, QR, tmk::ZS_F16>), g, bl, 0, h->stream, h->Z.p, h->H, nb, ep, h->twiddle(WW),
+                       h->twiddle(WW / 2));
+  else if (h->zst == TM_PLANE_BF16)
+    hipLaunchKernelGGL((tmk::k_last<WW, QR, tmk::ZS_BF16>), g, bl, 0, h->stream, h->Z.p, h->H, nb, ep, h->twiddle(WW),
+                       h->twiddle(WW / 2));
+  else
+    hipLaunchKernelGGL((tmk::k_last<WW, QR>), g, bl, 0, h->stream, h->Z.p, h->H, nb, ep, h->twiddle(WW),
+                       h->twiddle(WW / 2));
   check_launch("k_last");
 }
 
@@ -732,8 +757,6 @@ int tm_init(tm_core* h, const tm_init_params* p, const float* template_spectrum,
     h->B = p->batch_size > 0 ? p->batch_size : 64;
     TM_REQUIRE(p->plane_storage == TM_PLANE_FP32 || p->plane_storage == TM_PLANE_FP16 ||
                    p->plane_storage == TM_PLANE_BF16, TM_ERR_INVALID, "plane_storage must be 0, 1 or 2");
-    TM_REQUIRE(p->plane_storage == TM_PLANE_FP32 || (H == 4096 && pow2(W) && W >= 1024), TM_ERR_UNSUPPORTED,
-               "16-bit plane storage needs image height 4096 and a power-of-two width >= 1024");
     h->zst = p->plane_storage;
     // 16-bit values: the plane's elements are ~ 1 / sqrt(W^2 H) (correlations ~ 1 / sqrt(W H) after
     // the row transform of W points); scale them to ~1/4, leaving FP16 headroom above and below

@@ -602,10 +602,10 @@ __device__ __forceinline__ cf unpack_half2(unsigned w, float inv_scale) {
 // round-robin over the 8 XCDs), letting partial cache lines of Z merge in that XCD's L2. The
 // orientations of the batch are spread over gridDim.y (block row y takes b = y, y + gridDim.y, ..),
 // so that small images, with few column tiles, still fill the GPU.
-template <int H, int KC, int NTC>
+template <int H, int KC, int NTC, int ZST = ZS_F32>
 __global__ __launch_bounds__(KC* NTC) void k_mid(const cf* __restrict__ rows, const cf* __restrict__ inpT,
                                                  cf* __restrict__ Z, int Wh, int T, int nb, int ntiles,
-                                                 const cf* __restrict__ tw) {
+                                                 const cf* __restrict__ tw, float zscale) {
   __shared__ cf buf[KC][lds_len(H)];
   const int tid = threadIdx.x;
   const int c = tid / NTC, ctid = tid % NTC;
@@ -641,13 +641,17 @@ __global__ __launch_bounds__(KC* NTC) void k_mid(const cf* __restrict__ rows, co
     __syncthreads();
     fft_inplace<H, +1, NTC>(buf[c], tw, ctid);
     // Z[b][tile][y][KC]: the tile's output is one contiguous block; streamed past L2
-    cf* zt = Z + b * z_bs + (size_t)tile * H * KC;
+    const size_t zo = b * z_bs + (size_t)tile * H * KC;
     for (int e = tid; e < KC * H; e += KC * NTC) {
       const int cc = e % KC, y = e / KC;
       const cf v = buf[cc][lpad(y)];
-      unsigned long long bits;
-      __builtin_memcpy(&bits, &v, sizeof(bits));
-      __builtin_nontemporal_store(bits, reinterpret_cast<unsigned long long*>(zt + e));
+      if constexpr (ZST == ZS_F32) {
+        unsigned long long bits;
+        __builtin_memcpy(&bits, &v, sizeof(bits));
+        __builtin_nontemporal_store(bits, reinterpret_cast<unsigned long long*>(Z + zo + e));
+      } else {
+        __builtin_nontemporal_store(pack_half2<ZST>(v, zscale), reinterpret_cast<unsigned*>(Z) + zo + e);
+      }
     }
     __syncthreads();
   }
@@ -1023,7 +1027,7 @@ __host__ __device__ constexpr int last_threads() { return QR * ((W / 2) / plan_r
 // Rows are assigned so that consecutive rows run on the same XCD and share Z's cache lines in L2.
 // Twiddles live in LDS; the block is sized so that two blocks share a CU and overlap each other's
 // Z gathers.
-template <int W, int QR>
+template <int W, int QR, int ZST = ZS_F32>
 __global__ __launch_bounds__((last_threads<W, QR>())) __attribute__((amdgpu_waves_per_eu(4)))
 void k_last(const cf* __restrict__ Z, int H, int nb, EpiParams ep, const cf* __restrict__ tw_n,
             const cf* __restrict__ tw_h) {
@@ -1074,8 +1078,15 @@ void k_last(const cf* __restrict__ Z, int H, int nb, EpiParams ep, const cf* __r
     cf* const bb = buf[grp];
     // gather row y of orientation b0 + grp (elements 0..h) from the column tiles
     if (active) {
-      const cf* zb = Z + (size_t)(b0 + grp) * z_bs + (size_t)y * ep.ztile;
-      for (int e = t; e <= h; e += NT) bb[lpad(e)] = zb[(size_t)(e >> sh) * H * ep.ztile + (e & (ep.ztile - 1))];
+      const size_t zo = (size_t)(b0 + grp) * z_bs + (size_t)y * ep.ztile;
+      if constexpr (ZST == ZS_F32) {
+        const cf* zb = Z + zo;
+        for (int e = t; e <= h; e += NT) bb[lpad(e)] = zb[(size_t)(e >> sh) * H * ep.ztile + (e & (ep.ztile - 1))];
+      } else {
+        const unsigned* zb = reinterpret_cast<const unsigned*>(Z) + zo;
+        for (int e = t; e <= h; e += NT)
+          bb[lpad(e)] = unpack_half2<ZST>(zb[(size_t)(e >> sh) * H * ep.ztile + (e & (ep.ztile - 1))], ep.zinv);
+      }
     }
     __syncthreads();
     // first radix-8 stage (Ns = 1) with the real-to-complex pre-twist folded in:

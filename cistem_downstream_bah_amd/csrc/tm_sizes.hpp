@@ -3,19 +3,21 @@
 // Image sides are what the reference's size policy produces (match_template.cpp:527-598 with
 // ReturnClosestFactorizedUpper/Lower, functions.cpp:529-627): a power of two when one lies within
 // 10% above, else the nearest 3-smooth (then 5-smooth) even size. Compiled here: powers of two from
-// 256 to 4096; 1536, 2592, 2916 and 3072 (2048 x 1536, 2560^2 -> 2592^2, 4096 x 2880 -> 4096 x 2916,
-// 3072 x 2048 rotated -> 2048 x 3072); 270, 324 and 384 (small 2/3/5-smooth sizes).
+// 256 to 4096; 1536, 2592, 2916, 3072 and 5832 (2048 x 1536, 2560^2 -> 2592^2, 4096 x 2880 -> 4096 x 2916,
+// 3072 x 2048 rotated -> 2048 x 3072, a K3 frame 5760 x 4092 -> 5832 x 4096); 270, 324 and 384 (small
+// 2/3/5-smooth sizes).
 // Template and projection sides are powers of two.
 #pragma once
 
 #define TM_IMAGE_SIZES(X) \
-  X(256) X(270) X(324) X(384) X(512) X(1024) X(1536) X(2048) X(2592) X(2916) X(3072) X(4096)
+  X(256) X(270) X(324) X(384) X(512) X(1024) X(1536) X(2048) X(2592) X(2916) X(3072) X(4096) X(5832)
 #define TM_TEMPLATE_SIZES(X) X(32) X(64) X(128) X(256) X(512) X(1024)
 // every line length a row or column transform of the input image, the template or the projection
 // takes (the union of the two lists), and 1000, the intermediate edge of a template's pixel-size change
 // (1 / tolerance 0.001)
 #define TM_LINE_SIZES(X) \
-  X(32) X(64) X(128) X(256) X(270) X(324) X(384) X(512) X(1000) X(1024) X(1536) X(2048) X(2592) X(2916) X(3072) X(4096)
+  X(32) X(64) X(128) X(256) X(270) X(324) X(384) X(512) X(1000) X(1024) X(1536) X(2048) X(2592) X(2916) X(3072) \
+  X(4096) X(5832)
 
 namespace tmk {
 
@@ -30,6 +32,7 @@ constexpr int mid_cols(int H) {
     case 2592: return 2;
     case 2916: return 2;
     case 3072: return 2;
+    case 5832: return 2;
     default: return 16;  // 256 .. 1024
   }
 }
@@ -46,6 +49,7 @@ constexpr int mid_threads(int H) {
     case 2592: return 324;  // 16 2 3 3 3 3
     case 2916: return 486;  // 4 3 3 3 3 3 3
     case 3072: return 384;  // 16 16 4 3
+    case 5832: return 486;  // 8 3 3 3 3 3 3
     default: return 0;
   }
 }
@@ -66,6 +70,7 @@ constexpr int last_qr(int W) {
     case 2916: return 1;  // R1 2, 729
     case 3072: return 2;  // R1 8, 192
     case 4096: return 2;  // R1 8, 256
+    case 5832: return 1;  // R1 4, 729
     default: return 0;
   }
 }

@@ -48,10 +48,11 @@ typedef struct tm_init_params {
   float pixel_size_step;         /* best_pixel_size map stores size_i * pixel_size_step              */
   int batch_size;                /* orientations per launch; 0 selects the default (64)              */
   int plane_storage;             /* storage of the plane between the column and row transforms:
-                                    TM_PLANE_FP32 (0, the reference's precision), TM_PLANE_FP16 or
-                                    TM_PLANE_BF16 (half the traffic; image height 4096 and a
-                                    power-of-two width >= 1024 only, else TM_ERR_UNSUPPORTED). No
-                                    counterpart in the reference, whose planes are FP32.           */
+                                    TM_PLANE_FP32 (0, the reference's CPU precision), TM_PLANE_FP16
+                                    or TM_PLANE_BF16 (half the plane traffic; every compiled size).
+                                    The reference's GPU path keeps its input image and padded
+                                    reference in FP16 (TemplateMatchingCore.cu:163-164); here the
+                                    MIP, sums and histogram stay FP32/FP64 whatever the plane.      */
   int orientation_limit;         /* 0: every orientation of the position range; N > 0: only the first
                                     N in loop order (position-major, psi inner), the last position
                                     possibly cut short (a search truncated to its first N
@@ -59,13 +60,14 @@ typedef struct tm_init_params {
   int column_pass;               /* kernel of the column transforms at image height 4096 (no
                                     counterpart in the reference; results are bit-identical):
                                     TM_COLUMNS_AUTO (0): the persistent pass with 4-column Z tiles
-                                    where it applies (projection size <= 256, FP32 planes), else
-                                    one block per column pair;
+                                    where it applies (projection size <= 256), else one block per
+                                    column pair;
                                     TM_COLUMNS_PAIRS (1): one block per column pair;
                                     TM_COLUMNS_PERSISTENT (2): the persistent pass with 2-column Z
-                                    tiles; TM_COLUMNS_PERSISTENT4 (3): the persistent pass with the
-                                    4-column tiles of TM_COLUMNS_PAIRS (both TM_ERR_UNSUPPORTED
-                                    where they do not apply). Ignored at other heights.            */
+                                    tiles (FP32 planes only); TM_COLUMNS_PERSISTENT4 (3): the
+                                    persistent pass with the 4-column tiles of TM_COLUMNS_PAIRS
+                                    (both TM_ERR_UNSUPPORTED where they do not apply). Ignored at
+                                    other heights.                                                 */
   long long device_memory_limit; /* 0: none; else tm_init fails with TM_ERR_HIP, freeing what it
                                     allocated and leaving the handle uninitialised, when its device
                                     allocations would exceed this many bytes, as when hipMalloc runs

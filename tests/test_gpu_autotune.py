@@ -10,7 +10,9 @@
   block per column pair with 4-column Z tiles; the persistent pass with 2- and 4-column tiles) at
   batch 64 and 41 give bit-identical MIP, index, sums and histogram; tm_autotune over (column pass,
   batch) times every pair, picks the fastest and the search re-initialised at it gives the same maps;
-  a column pass that does not apply (persistent with 16-bit planes) is reported as such
+  a column pass that does not apply (2-column tiles with 16-bit planes) is reported as such
+* with FP16 and BF16 planes the persistent pass (4-column tiles, the default) and the column-pair pass
+  give bit-identical results
 """
 import numpy as np
 import pytest
@@ -123,9 +125,27 @@ def test_autotune_over_column_passes(c1):
     assert np.array_equal(r.sum, base.sum) and np.array_equal(r.hist, base.hist)
     search.core.close()
     fp16 = _search4096(c1, 64, "auto", "fp16")
-    assert fp16.core.launch_shape()["column_pass"] == "pairs"
-    _, c16, t16 = fp16.autotune((64,), ("persistent", "pairs"), sample_orientations=128)
-    assert c16 == "pairs" and t16[("persistent", 64)] is None and t16[("pairs", 64)] > 0
+    assert fp16.core.launch_shape()["column_pass"] == "persistent4"
+    _, c16, t16 = fp16.autotune((64,), ("persistent", "pairs", "persistent4"), sample_orientations=128)
+    assert c16 in ("pairs", "persistent4") and t16[("persistent", 64)] is None
+    assert t16[("pairs", 64)] > 0 and t16[("persistent4", 64)] > 0
     with pytest.raises(core.TMError):
         _search4096(c1, 64, "persistent", "fp16")
     fp16.core.close()
+
+
+@pytest.mark.parametrize("st", ["fp16", "bf16"])
+def test_16bit_column_passes_change_no_result(c1, st):
+    """The persistent pass writes the 16-bit plane exactly as the column-pair pass does."""
+    ref = None
+    for cp in ("pairs", "persistent4"):
+        print(f"case: 4096^2 position 3, {st}, column pass {cp}", flush=True)
+        se = _search4096(c1, 64, cp, st)
+        assert se.core.launch_shape()["column_pass"] == cp
+        r = _results(se)
+        se.core.close()
+        if ref is None:
+            ref = r
+            continue
+        for f in ("mip", "best_index", "sum", "sumsq", "hist"):
+            assert np.array_equal(getattr(r, f), getattr(ref, f)), (cp, f)

@@ -77,3 +77,44 @@ def test_driver_end_to_end(tmp_path):
     assert np.abs(u["mip"] - py["mip"]).max() <= 1e-5 * np.abs(py["mip"]).max()
     assert (u["psi"] == py["psi"]).mean() > 0.99 and (u["theta"] == py["theta"]).mean() > 0.99
     assert u["histogram"].sum() == py["histogram"].sum() == n * 512 * 512
+
+
+def test_job_arguments_and_interactive_answers_give_the_named_run(tmp_path):
+    """The reference's argument forms (--job-args: the 42 job arguments; --interactive: its 32
+    questions on stdin) run the same search as the named options: identical maps and histogram file."""
+    s = synth.config0(0)
+    img, vol = str(tmp_path / "img.mrc"), str(tmp_path / "vol.mrc")
+    mrc.write_mrc(img, s.micrograph)
+    mrc.write_mrc(vol, s.volume)
+    named = str(tmp_path / "named")
+    print("case: driver named options, config 0", flush=True)
+    r = subprocess.run([DRIVER, "--image", img, "--template", vol, "--out", named, "--angular-step", "10",
+                        "--in-plane-step", "10", "--defocus1", "15000"], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr
+    kinds = ["mip", "scaled_mip", "psi", "theta", "phi", "defocus", "pixel_size", "avg", "std"]
+
+    def names(p):
+        return [f"{p}_{k}.mrc" for k in kinds]
+
+    ja = str(tmp_path / "ja")
+    m = names(ja)
+    job = [img, vol, "1.0", "300", "2.7", "0.07", "15000", "15000", "0", "300", "8", "10", "20", "0", "0", "0", "0",
+           "1.0", "No", "0", "0", m[0], m[2], m[3], m[4], m[5], m[6], m[1], m[7], "C1", "10", ja + "_histogram.txt",
+           "-1", "-1", "0", "0", m[8], "/dev/null", "/dev/null", "10", "Yes", "1"]
+    print("case: driver --job-args, config 0", flush=True)
+    r = subprocess.run([DRIVER, "--job-args", *job], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr
+    it = str(tmp_path / "it")
+    m = names(it)
+    answers = [img, vol, m[0], m[1], m[2], m[3], m[4], m[5], m[6], m[7], m[8], it + "_histogram.txt", "1.0", "300",
+               "2.7", "0.07", "15000", "15000", "0", "0", "8", "10", "10", "0", "0", "0", "0", "1", "0", "C1", "Yes", "1"]
+    print("case: driver --interactive, config 0", flush=True)
+    r = subprocess.run([DRIVER, "--interactive"], input="\n".join(answers) + "\n", capture_output=True, text=True,
+                       timeout=240)
+    assert r.returncode == 0, r.stderr
+    for k in kinds:
+        a = mrc.read_mrc(f"{named}_{k}.mrc")
+        for other in (ja, it):
+            assert np.array_equal(a, mrc.read_mrc(f"{other}_{k}.mrc")), (other, k)
+    h = open(named + "_histogram.txt").read()
+    assert h == open(ja + "_histogram.txt").read() == open(it + "_histogram.txt").read()

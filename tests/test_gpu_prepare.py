@@ -29,7 +29,7 @@ def test_grid_and_psi_identical():
 
 
 def test_sizes_match_factorisation():
-    for (mx, my) in [(4096, 4096), (3072, 2048), (4096, 2880), (3840, 3840), (2560, 2560)]:
+    for (mx, my) in [(4096, 4096), (3072, 2048), (4096, 2880), (3840, 3840), (2560, 2560), (5760, 4092), (5760, 5760)]:
         W, H, rot, n, mp = MT.prepare_sizes(mx, my, 256)
         fx, fy = O.factorize_axis(mx, 256), O.factorize_axis(my, 256)
         assert (W, H) == ((fy, fx) if rot else (fx, fy))

@@ -5,7 +5,9 @@ preparation, the column pass and the row pass, against the oracle on the same in
   (2 3^3 5, an odd half-length row transform): the input preparation and a search over 10 grid
   positions to the bars of parity_util
 * the non-power-of-two image sizes of the multi-micrograph workload after the reference's size
-  policy: 2048 x 3072 (a 3072 x 2048 micrograph, rotated), 4096 x 2916, 2048 x 1536 and 2592 x 2592:
+  policy: 2048 x 3072 (a 3072 x 2048 micrograph, rotated), 4096 x 2916, 2048 x 1536 and 2592 x 2592,
+  and the K3 sizes 4096 x 5832 (a 5760 x 4092 frame after the policy, match_template.cpp:537-538) and
+  5832 x 5832 (2^3 3^6: a radix-8 stage and six radix-3 stages per line):
   input preparation, single CCF planes (max |gpu - oracle| / max |oracle| <= 1e-4) and a search over
   one grid position
 Each micrograph holds copies of the 64^3 pseudo-atom template (seeded), so the oracle stays cheap.
@@ -67,7 +69,7 @@ def test_small_mixed_radix_search(mx, my, W, H):
         g.close()
 
 
-@pytest.mark.parametrize("W,H", [(2048, 3072), (4096, 2916), (2048, 1536), (2592, 2592)])
+@pytest.mark.parametrize("W,H", [(2048, 3072), (4096, 2916), (2048, 1536), (2592, 2592), (4096, 5832), (5832, 5832)])
 def test_multi_micrograph_sizes(W, H):
     s = _micrograph(W, H, 8000 + H, 12)
     mt, g = _setup(s, O.MTParams(angular_step=10.0, in_plane_angular_step=15.0, padding=1.0,
