@@ -461,8 +461,9 @@ def bench_config2(args):
     """Eight micrographs of mixed sizes (the multi-micrograph workload), one shared 256^3 template
     padded to 512^3, 2.5/1.5 deg grid, one defocus plane per micrograph at its own defocus; micrograph
     i searches positions 9i..9i+8 (2169 orientations). Times the whole set with 1, 2, 4 images in
-    flight and reports pixel-correlations per second (sum over images of W*H*orientations / time)."""
-    from cistem_downstream_bah_amd import synth
+    flight (one stream and host thread each) and as one hipGraph launch over all eight (tm_group_run),
+    and reports pixel-correlations per second (sum over images of W*H*orientations / time)."""
+    from cistem_downstream_bah_amd import core, synth
     from cistem_downstream_bah_amd import match_template as MT
 
     t0 = time.time()
@@ -478,6 +479,19 @@ def bench_config2(args):
         searches.append(MT.MatchTemplateSearch(s.micrograph, s.volume, sp, template_spectrum=tpl))
     sizes = [(m.W, m.H) for m in searches]
     log(f"prepared {len(searches)} searches in {time.time() - t0:.1f}s: sizes {sizes}")
+    tuned = None
+    if args.autotune:
+        # per micrograph: batch depth x column pass (tile width), each candidate timed on a sample
+        cands = [int(c) for c in args.autotune.split(",")]
+        tuned = []
+        t0 = time.time()
+        for m in searches:
+            b, c, table = m.autotune(cands, sample_orientations=4 * max(cands))
+            tuned.append({"image": [m.W, m.H], "batch": b, "column_pass": c,
+                          "us_per_ccf": {f"{k[0]}/{k[1]}": (None if v is None else round(v, 3))
+                                         for k, v in table.items()}})
+            log(f"autotune {m.W}x{m.H}: batch {b}, column pass {c}")
+        log(f"autotuned in {time.time() - t0:.1f}s")
     px_ccf = sum(m.W * m.H * m.total_correlation_positions for m in searches)
     n_ccf = sum(m.total_correlation_positions for m in searches)
     for _ in range(max(args.warmup, 0) and 1):
@@ -493,6 +507,25 @@ def bench_config2(args):
         runs[str(c)] = {"seconds": round(dt, 4), "pixel_correlations_per_s": round(px_ccf / dt, 1),
                         "ccf_per_s": round(n_ccf / dt, 2)}
         log(f"{c} in flight: {dt:.3f}s -> {px_ccf / dt / 1e9:.2f} G pixel-correlations/s")
+    # every micrograph's plane in one hipGraph launch (tm_group_run: one branch per micrograph); the
+    # first launch captures the graph, the timed one replays it
+    group = core.TemplateMatchingGroup([m.core for m in searches])
+    for m in searches:
+        m.core.reset()
+    MT.run_group(searches, group)
+    ref = [m.core.results() for m in searches]
+    for m in searches:
+        m.core.reset()
+    t0 = time.perf_counter()
+    done = MT.run_group(searches, group)
+    dt = time.perf_counter() - t0
+    assert sum(done) == n_ccf
+    same = all(np.array_equal(getattr(a, k), getattr(m.core.results(), k))
+               for a, m in zip(ref, searches) for k in ("mip", "best_index", "sum", "sumsq", "hist"))
+    runs["graph"] = {"seconds": round(dt, 4), "pixel_correlations_per_s": round(px_ccf / dt, 1),
+                     "ccf_per_s": round(n_ccf / dt, 2), "replay_equals_capture_run": bool(same)}
+    log(f"one graph over all micrographs: {dt:.3f}s -> {px_ccf / dt / 1e9:.2f} G pixel-correlations/s")
+    group.close()
     best = max(runs, key=lambda k: runs[k]["pixel_correlations_per_s"])
     out = {
         "metric": "pixel-correlations/sec over the multi-micrograph set (sum of W*H*orientations / time)",
@@ -501,7 +534,9 @@ def bench_config2(args):
         "dtype": "fp32", "data": "synthetic (seeded Gaussian pseudo-atom template and micrographs)",
         "config": {"workload": "config2_8_micrographs_T256_P512", "images": sizes,
                    "orientations_per_image": [m.total_correlation_positions for m in searches],
-                   "defocus_planes_per_image": 1, "batch": args.batch, "best_concurrency": int(best)},
+                   "defocus_planes_per_image": 1,
+                   "launch_shapes": [m.core.launch_shape() for m in searches], "best_mode": best},
+        "autotune": tuned,
         "runs": runs,
     }
     print(json.dumps(out), flush=True)
@@ -759,6 +794,11 @@ def bench_proposal4096(args):
     planes, z >= 7.5 peaks). Training micrographs are labelled by the exhaustive search on the
     micrograph and template binned 4 x 4 (positions up to theta 50 deg at 5 deg). Both paths are timed
     from the raw micrograph on the host to their peak lists."""
+    # torch (the device renderer of the synthetic micrographs) first, so that libtm_hip.so binds to the
+    # HIP runtime torch has already loaded
+    import torch
+    if torch.cuda.is_available():
+        torch.cuda.init()
     from cistem_downstream_bah_amd import cnn, proposal as PR, proposal_large as PL
 
     study = PL.LargeStudy()
@@ -771,20 +811,37 @@ def bench_proposal4096(args):
         training = {"loaded_from": os.path.basename(args.weights)}
         log(f"network loaded from {args.weights}")
     else:
+        from concurrent.futures import ThreadPoolExecutor
+
         t0 = time.time()
         inputs, labels = [], []
         t_gen = t_lab = 0.0
-        for i in range(args.train):
+
+        def generate(i):
+            gen_t = {}
             t1 = time.time()
-            m = study.micrograph(i).micrograph
-            t2 = time.time()
-            labels.append(study.label(m))
-            inputs.append(study.net_input(m))
-            t_gen += t2 - t1
-            t_lab += time.time() - t2
-            if (i + 1) % 25 == 0 or i + 1 == args.train:
-                log(f"labelled {i + 1} / {args.train}: {np.mean([len(p) for p in labels]):.1f} peaks per micrograph, "
-                    f"generation {t_gen / (i + 1):.2f}s, label {t_lab / (i + 1):.2f}s each")
+            m = study.micrograph(i, timing=gen_t).micrograph
+            return m, gen_t, time.time() - t1
+
+        # the next micrograph is generated (host placement and noise draw, device projections) on a
+        # second thread while the current one is labelled by the binned search
+        with ThreadPoolExecutor(max_workers=1) as pool:
+            nxt = pool.submit(generate, 0) if args.train else None
+            for i in range(args.train):
+                t1 = time.time()
+                m, gen_t, _ = nxt.result()
+                if i + 1 < args.train:
+                    nxt = pool.submit(generate, i + 1)
+                t2 = time.time()
+                if i == 0 and "host_fallback" in gen_t:
+                    log(f"micrographs generated on the host: {gen_t['host_fallback']}")
+                labels.append(study.label(m))
+                inputs.append(study.net_input(m))
+                t_gen += t2 - t1
+                t_lab += time.time() - t2
+                if (i + 1) % 25 == 0 or i + 1 == args.train:
+                    log(f"labelled {i + 1} / {args.train}: {np.mean([len(p) for p in labels]):.1f} peaks per "
+                        f"micrograph, waiting for generation {t_gen / (i + 1):.2f}s, label {t_lab / (i + 1):.2f}s each")
         t1 = time.time()
         hist = study.train(net, inputs, labels, epochs=args.epochs, batch=16, lr=1e-3, log=log)
         training = {"micrographs": args.train, "epochs": args.epochs,
@@ -819,7 +876,8 @@ def bench_proposal4096(args):
         syn = study.micrograph(seed, timing=gen_t)
         m = syn.micrograph
         log(f"micrograph {seed} generated in {time.time() - t1:.2f}s " +
-            ("on the device: " + ", ".join(f"{k} {v:.3f}s" for k, v in gen_t.items()) if gen_t else "on the host"))
+            (f"on the host ({gen_t['host_fallback']})" if "host_fallback" in gen_t else
+             "on the device: " + ", ".join(f"{k} {v:.3f}s" for k, v in gen_t.items())))
         parts = np.array([(p.x, p.y) for p in syn.particles])
         if j == 0:  # one-off handle set-up of both learned paths, outside the timing
             study.detect(net, m, Ks[0], 1)

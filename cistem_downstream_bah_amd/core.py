@@ -27,7 +27,8 @@ I64P = ctypes.POINTER(ctypes.c_longlong)
 EXPORTS = ["tm_device_count", "tm_create", "tm_destroy", "tm_init", "tm_run_inner_loop", "tm_get_results", "tm_merge",
            "tm_set_template", "tm_reset", "tm_finalize", "tm_debug_projection", "tm_debug_ccf", "tm_set_timing", "tm_get_timing",
            "tm_last_error", "tm_autotune", "tm_autotune_batch", "tm_get_launch_shape", "tm_set_graph", "tm_set_input", "tm_set_micrograph",
-           "tm_find_peaks", "tm_find_peaks_noalloc", "tm_run_partial", "tm_state_size", "tm_export_state", "tm_import_state"]
+           "tm_find_peaks", "tm_find_peaks_noalloc", "tm_run_partial", "tm_state_size", "tm_export_state", "tm_import_state",
+           "tm_group_create", "tm_group_run", "tm_group_destroy"]
 
 PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.c_longlong, ctypes.c_void_p)
 
@@ -50,7 +51,7 @@ class InitParams(ctypes.Structure):
                 ("orientation_limit", ctypes.c_int), ("column_pass", ctypes.c_int),
                 ("device_memory_limit", ctypes.c_longlong)]
 
-COLUMN_PASS = {"auto": 0, "pairs": 1, "persistent": 2, "persistent4": 3}
+COLUMN_PASS = {"auto": 0, "pairs": 1, "persistent": 2, "persistent4": 3, "narrow": 4}
 
 
 class PeakInfo(ctypes.Structure):
@@ -101,6 +102,9 @@ def lib():
                                   ctypes.c_int, I32P, ctypes.c_int, ctypes.c_int, I32P, I32P, F64P]
         L.tm_autotune_batch.argtypes = [ctypes.c_int, ctypes.POINTER(InitParams), F32P, F32P, F32P, F32P, F32P, I32P,
                                         ctypes.c_int, ctypes.c_int, I32P, F64P]
+        L.tm_group_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(vp)]
+        L.tm_group_run.argtypes = [vp, ctypes.POINTER(F32P), I32P, I32P, I64P]
+        L.tm_group_destroy.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -214,7 +218,7 @@ class TemplateMatchingCore:
         orientation_limit: search only the first N orientations of the range (0: all);
         device_memory_limit: fail (TM_ERR_HIP) rather than allocate more device bytes (0: none);
         column_pass: "auto", "pairs", "persistent" or "persistent4" kernel for the column transforms at
-        height 4096."""
+        height 4096; "auto" or "narrow" (column tiles half as wide) at other heights."""
         tpl = np.ascontiguousarray(template_spectrum, np.complex64)
         inp = np.ascontiguousarray(input_spectrum, np.complex64)
         grid = np.ascontiguousarray(search_positions, np.float32)
@@ -375,3 +379,45 @@ class TemplateMatchingCore:
         launches = np.zeros(cap, np.int64)
         n = lib().tm_get_timing(self.h, names, ms.ctypes.data_as(F64P), launches.ctypes.data_as(I64P), cap)
         return {names[i].decode(): (float(ms[i]), int(launches[i])) for i in range(n)}
+
+
+class TemplateMatchingGroup:
+    """Several TemplateMatchingCore handles on one device whose planes run as one hipGraph launch
+    (tm_group_*): one independent branch per handle. Each handle's maps equal those of its own
+    run_inner_loop with the same arguments, bit for bit. The handles must stay open while the group
+    is in use."""
+
+    def __init__(self, cores: list):
+        self.cores = list(cores)
+        self._h = ctypes.c_void_p()
+        arr = (ctypes.c_void_p * len(self.cores))(*[c.h for c in self.cores])
+        _check(lib().tm_group_create(arr, len(self.cores), ctypes.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().tm_group_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, projection_filters: list, size_i=None, defocus_i=None) -> list:
+        """One plane for every handle: projection_filters[i] (T x (T/2+1) complex64 of handle i) with
+        (size_i[i], defocus_i[i]) (default 0). Returns the CCFs computed per handle."""
+        n = len(self.cores)
+        if len(projection_filters) != n:
+            raise ValueError(f"{n} filters expected, got {len(projection_filters)}")
+        filt = [np.ascontiguousarray(f, dtype=np.complex64) for f in projection_filters]
+        for c, f in zip(self.cores, filt):
+            if f.size != c.T * (c.T // 2 + 1):
+                raise ValueError(f"projection filter of {f.size} elements, handle expects {c.T * (c.T // 2 + 1)}")
+        ptrs = (F32P * n)(*[f.ctypes.data_as(F32P) for f in filt])
+        si = np.ascontiguousarray(np.zeros(n) if size_i is None else size_i, dtype=np.int32)
+        di = np.ascontiguousarray(np.zeros(n) if defocus_i is None else defocus_i, dtype=np.int32)
+        out = np.zeros(n, np.int64)
+        _check(lib().tm_group_run(self._h, ptrs, si.ctypes.data_as(I32P), di.ctypes.data_as(I32P),
+                                  out.ctypes.data_as(I64P)))
+        return [int(v) for v in out]

@@ -15,6 +15,7 @@
 #include <cstdint>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -92,6 +93,9 @@ struct tm_core {
   DevBuf<float> pr_src, pr_pad, pr_st, pr_cx, pr_cy;
   DevBuf<cf> pr_spec;
   DevBuf<double> pr_part, pr_acc, pr_pc, pr_ps;
+  // bumped whenever the search buffers are freed (tm_init, tm_destroy): a group graph that captured
+  // this handle's buffer addresses under an older epoch is stale
+  unsigned long long epoch = 0;
 
   void drop_graph() {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
@@ -142,6 +146,7 @@ struct tm_core {
   // Frees the search's device buffers (not the stream), as before a fresh tm_init.
   void release_buffers() {
     drop_graph();
+    ++epoch;
     for (DevBuf<cf>* b : {&tpl, &inpT, &phaseP, &phaseT, &filt, &slice, &projT, &rows, &Z}) b->release();
     mats.release();
     mip.release();
@@ -186,13 +191,22 @@ void launch_mid_t(tm_core* h, int nb) {
 //     wavefront (faster columns, slower row reads);
 //   * one block per column pair at H = 4096 (k_mid4096): two 2-column blocks fill one 4-column tile,
 //     so that a row read of k_last takes 32 bytes from each cache line instead of 16;
-//   * other heights (k_mid): one tile per block.
+//   * other heights (k_mid): one tile per block, tmk::mid_cols(H) columns wide, or half as wide with
+//     TM_COLUMNS_NARROW (a launch-shape choice for the autotuner: more, smaller blocks per CU against
+//     shorter row-pass reads).
 int mid_kc(int H) { return H == 4096 ? 2 : tmk::mid_cols(H); }
 int mid_tiles(const tm_core* h) { return (h->Wh + mid_kc(h->H) - 1) / mid_kc(h->H); }
 // Whether the column pass places the projection rows at y = 0 (with the centring phase folded into
 // the input spectrum) rather than at H/2 - T/2.
 bool column_rows_at_zero(int H) { return H == 4096; }
 int z_tiles(const tm_core* h) { return (h->Wh + h->zt - 1) / h->zt; }
+// Whether the row pass of a handle with Z tiles zt columns wide is the direct-load k_last_dma (power-of-two
+// widths from 1024, tiles at least one 16-byte load wide: two FP32 or four 16-bit elements) rather
+// than the gathering k_last. The two agree to FP32 rounding, not bit for bit (k_last_dma takes its
+// twiddles from factored tables).
+bool row_pass_dma(const tm_core* h, int zt) {
+  return h->W >= 1024 && (h->W & (h->W - 1)) == 0 && zt >= (h->zst == TM_PLANE_FP32 ? 2 : 4);
+}
 
 // The column pass of a handle: the persistent pass with 4-column tiles where it applies (H = 4096,
 // T <= 256, FP32 planes) unless the caller asks for another; the Z tile width follows.
@@ -207,9 +221,16 @@ void column_pass_for(tm_core* h, int requested) {
              "the column-pair pass needs image height 4096");
   TM_REQUIRE(requested != TM_COLUMNS_PERSISTENT4 || persistent_ok, TM_ERR_UNSUPPORTED,
              "the persistent column pass needs image height 4096 and projection size <= 256");
+  // narrow tiles only where the row pass stays the same kernel, so that results stay bit-identical
+  TM_REQUIRE(requested != TM_COLUMNS_NARROW ||
+                 (h->H != 4096 && row_pass_dma(h, mid_kc(h->H)) == row_pass_dma(h, mid_kc(h->H) / 2)),
+             TM_ERR_UNSUPPORTED,
+             "the narrow column tiles apply to image heights other than 4096 where they keep the row pass "
+             "(not at " + std::to_string(h->W) + "x" + std::to_string(h->H) + " with this plane storage)");
   if (h->H != 4096) {
-    h->column_pass = TM_COLUMNS_AUTO;
-    h->zt = mid_kc(h->H);
+    // one column tile per block: tmk::mid_cols(H) columns, or half as many (narrow)
+    h->column_pass = requested == TM_COLUMNS_NARROW ? TM_COLUMNS_NARROW : TM_COLUMNS_AUTO;
+    h->zt = requested == TM_COLUMNS_NARROW ? mid_kc(h->H) / 2 : mid_kc(h->H);
   } else if (requested == TM_COLUMNS_PAIRS || !persistent_ok) {
     h->column_pass = TM_COLUMNS_PAIRS;
     h->zt = 4;
@@ -254,9 +275,13 @@ void launch_mid(tm_core* h, int nb) {
     }
     return;
   }
+  const bool narrow = h->column_pass == TM_COLUMNS_NARROW;
   switch (h->H) {
-#define TM_CASE(n) \
-  case n: launch_mid_t<n, tmk::mid_cols(n), tmk::mid_threads(n)>(h, nb); break;
+#define TM_CASE(n)                                                                      \
+  case n:                                                                               \
+    if (narrow) launch_mid_t<n, tmk::mid_cols(n) / 2, tmk::mid_threads(n)>(h, nb);     \
+    else launch_mid_t<n, tmk::mid_cols(n), tmk::mid_threads(n)>(h, nb);                \
+    break;
     TM_IMAGE_SIZES(TM_CASE)
 #undef TM_CASE
     default: throw TmError{TM_ERR_UNSUPPORTED, "unsupported image height " + std::to_string(h->H)};
@@ -265,8 +290,8 @@ void launch_mid(tm_core* h, int nb) {
 
 // QR orientations per k_last block (tmk::last_qr). For power-of-two widths from 1024 on the
 // groups are whole wavefronts and the row gather runs as double-buffered direct LDS loads; those read
-// 16 bytes of one tile row at a time (four 16-bit elements), so 16-bit planes in tiles narrower than
-// 4 columns take the gathering row pass.
+// 16 bytes of one tile row at a time (two FP32 or four 16-bit elements), so planes in tiles narrower
+// than that take the gathering row pass.
 template <int WW, int QR>
 void launch_last_t(tm_core* h, int nb, tmk::EpiParams ep) {
   ep.ztile = h->zt;
@@ -275,7 +300,7 @@ void launch_last_t(tm_core* h, int nb, tmk::EpiParams ep) {
   ep.zinv = 1.0f / h->zscale;  // a power of two: exact
   const dim3 g(h->H);
   if constexpr (WW >= 1024 && (WW & (WW - 1)) == 0) {
-    if (h->zst == TM_PLANE_FP32 || h->zt >= 4) {
+    if (row_pass_dma(h, h->zt)) {
       const dim3 bl(QR * (WW / 16));
       if (h->zst == TM_PLANE_FP16)
         hipLaunchKernelGGL((tmk::k_last_dma<WW, QR, tmk::ZS_F16>), g, bl, 0, h->stream, h->Z.p, h->H, nb, ep,
@@ -763,8 +788,8 @@ int tm_init(tm_core* h, const tm_init_params* p, const float* template_spectrum,
     h->zscale = h->zst == TM_PLANE_FP32 ? 1.0f
                                        : ldexpf(1.0f, (int)lround(0.5 * std::log2((double)W * W * H)) - 2);
     TM_REQUIRE(h->B <= 256, TM_ERR_INVALID, "batch_size must be <= 256");
-    TM_REQUIRE(p->column_pass >= TM_COLUMNS_AUTO && p->column_pass <= TM_COLUMNS_PERSISTENT4, TM_ERR_INVALID,
-               "column_pass must be 0, 1, 2 or 3");
+    TM_REQUIRE(p->column_pass >= TM_COLUMNS_AUTO && p->column_pass <= TM_COLUMNS_NARROW, TM_ERR_INVALID,
+               "column_pass must be 0, 1, 2, 3 or 4");
     column_pass_for(h, p->column_pass);
     int cus = 0;
     TM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, h->device));
@@ -1445,6 +1470,163 @@ int tm_get_timing(tm_core* h, const char** names, double* ms, long long* launche
   return n;
 }
 
+// ------------------------------------------------------------------ groups of handles
+}  // extern "C"
+
+// Several handles (typically one micrograph each, of any sizes) whose planes run as one hipGraph:
+// one branch per handle, each branch that handle's batches in its own stream order, the branches
+// independent of one another. Captured at the first tm_group_run and replayed after; a handle
+// re-initialised since (its buffers moved) makes the next run capture again.
+struct tm_group {
+  int device = 0;
+  std::vector<tm_core*> hs;
+  std::vector<unsigned long long> epochs;
+  std::vector<hipEvent_t> ready, join;  // per handle: inputs uploaded (outside the graph); branch end
+  hipEvent_t fork = nullptr;
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+
+  void drop() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+  ~tm_group() {
+    drop();
+    for (auto e : ready) if (e) (void)hipEventDestroy(e);
+    for (auto e : join) if (e) (void)hipEventDestroy(e);
+    if (fork) (void)hipEventDestroy(fork);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+// Captures one plane of every handle of the group: the group's stream forks to each handle's stream
+// and joins it again, so the graph holds one independent branch per handle.
+void capture_group(tm_group* g) {
+  TM_HIP(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+  size_t joined = 0;
+  try {
+    TM_HIP(hipEventRecord(g->fork, g->stream));
+    for (size_t i = 0; i < g->hs.size(); ++i) {
+      tm_core* h = g->hs[i];
+      TM_HIP(hipStreamWaitEvent(h->stream, g->fork, 0));
+      h->capturing = true;
+      const long long first = (long long)h->prm.first_search_position * h->n_psi;
+      for (int o0 = 0; o0 < h->n_orient; o0 += h->B)
+        run_batch(h, o0, std::min(h->B, h->n_orient - o0), (int)(first + o0));
+      h->capturing = false;
+      TM_HIP(hipEventRecord(g->join[i], h->stream));
+      TM_HIP(hipStreamWaitEvent(g->stream, g->join[i], 0));
+      joined = i + 1;
+    }
+  } catch (...) {
+    // join every stream that entered the capture, so that ending it returns them all to normal
+    for (size_t i = 0; i < g->hs.size(); ++i) {
+      g->hs[i]->capturing = false;
+      if (i >= joined && hipEventRecord(g->join[i], g->hs[i]->stream) == hipSuccess)
+        (void)hipStreamWaitEvent(g->stream, g->join[i], 0);
+    }
+    hipGraph_t dead = nullptr;
+    (void)hipStreamEndCapture(g->stream, &dead);
+    if (dead) (void)hipGraphDestroy(dead);
+    throw;
+  }
+  TM_HIP(hipStreamEndCapture(g->stream, &g->graph));
+  TM_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+  for (size_t i = 0; i < g->hs.size(); ++i) g->epochs[i] = g->hs[i]->epoch;
+}
+}  // namespace
+
+extern "C" {
+
+int tm_group_create(tm_core* const* handles, int n, tm_group** out) {
+  return guarded([&] {
+    TM_REQUIRE(out, TM_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    TM_REQUIRE(handles && n > 0, TM_ERR_INVALID, "a group needs at least one handle");
+    for (int i = 0; i < n; ++i) {
+      TM_REQUIRE(handles[i], TM_ERR_INVALID, "handle " + std::to_string(i) + " is NULL");
+      TM_REQUIRE(handles[i]->device == handles[0]->device, TM_ERR_INVALID, "the handles of a group share one device");
+      for (int j = 0; j < i; ++j)
+        TM_REQUIRE(handles[j] != handles[i], TM_ERR_INVALID, "a handle appears twice in the group");
+    }
+    TM_HIP(hipSetDevice(handles[0]->device));
+    auto g = std::make_unique<tm_group>();
+    g->device = handles[0]->device;
+    g->hs.assign(handles, handles + n);
+    g->epochs.assign(n, ~0ULL);
+    g->ready.assign(n, nullptr);
+    g->join.assign(n, nullptr);
+    TM_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+    TM_HIP(hipEventCreateWithFlags(&g->fork, hipEventDisableTiming));
+    for (int i = 0; i < n; ++i) {
+      TM_HIP(hipEventCreateWithFlags(&g->ready[i], hipEventDisableTiming));
+      TM_HIP(hipEventCreateWithFlags(&g->join[i], hipEventDisableTiming));
+    }
+    *out = g.release();
+  });
+}
+
+int tm_group_destroy(tm_group* g) {
+  return guarded([&] {
+    if (!g) return;
+    (void)hipSetDevice(g->device);
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    delete g;
+  });
+}
+
+int tm_group_run(tm_group* g, const float* const* projection_filters, const int* size_i, const int* defocus_i,
+                 long long* ccfs) {
+  return guarded([&] {
+    TM_REQUIRE(g, TM_ERR_INVALID, "group is NULL");
+    TM_REQUIRE(projection_filters && size_i && defocus_i, TM_ERR_INVALID, "NULL argument");
+    TM_HIP(hipSetDevice(g->device));
+    const size_t n = g->hs.size();
+    bool stale = g->exec == nullptr;
+    for (size_t i = 0; i < n; ++i) {
+      tm_core* h = g->hs[i];
+      require_ready(h);
+      TM_REQUIRE(projection_filters[i], TM_ERR_INVALID, "projection filter " + std::to_string(i) + " is NULL");
+      TM_REQUIRE(!h->timing, TM_ERR_STATE, "kernel timing is on for handle " + std::to_string(i));
+      TM_REQUIRE(h->planes.empty() || h->next_orientation == h->n_orient, TM_ERR_STATE,
+                 "handle " + std::to_string(i) + " has an unfinished plane");
+      const long long per_plane = (long long)h->prm.n_search_positions * h->n_psi;
+      TM_REQUIRE((long long)(h->planes.size() + 1) * per_plane < (1LL << 31), TM_ERR_INVALID,
+                 "orientation index space exhausted (too many planes for handle " + std::to_string(i) + ")");
+      if (g->epochs[i] != h->epoch) stale = true;
+    }
+    if (stale) g->drop();
+    // each handle's plane offset and filter, on its own stream, before the graph starts
+    for (size_t i = 0; i < n; ++i) {
+      tm_core* h = g->hs[i];
+      if (!h->plane_base.p) {
+        h->plane_base.alloc(1);
+        TM_HIP(hipHostMalloc((void**)&h->h_plane_base, sizeof(int), hipHostMallocDefault));
+      }
+      const long long per_plane = (long long)h->prm.n_search_positions * h->n_psi;
+      *h->h_plane_base = (int)((long long)h->planes.size() * per_plane);
+      TM_HIP(hipMemcpyAsync(h->plane_base.p, h->h_plane_base, sizeof(int), hipMemcpyHostToDevice, h->stream));
+      upload_filter(h, projection_filters[i]);
+      TM_HIP(hipEventRecord(g->ready[i], h->stream));
+      TM_HIP(hipStreamWaitEvent(g->stream, g->ready[i], 0));
+    }
+    if (!g->exec) capture_group(g);
+    TM_HIP(hipGraphLaunch(g->exec, g->stream));
+    TM_HIP(hipStreamSynchronize(g->stream));
+    for (size_t i = 0; i < n; ++i) {
+      tm_core* h = g->hs[i];
+      h->planes.push_back(make_int2(size_i[i], defocus_i[i]));
+      h->n_ccfs += h->n_orient;
+      h->next_orientation = h->n_orient;
+      if (ccfs) ccfs[i] = h->n_orient;
+    }
+  });
+}
+
 int tm_autotune(int device_ordinal, const tm_init_params* params, const float* template_spectrum,
                 const float* input_spectrum, const float* search_positions, const float* psi_list,
                 const float* projection_filter, const int* batch_candidates, int n_batch, const int* column_candidates,
@@ -1459,7 +1641,7 @@ int tm_autotune(int device_ordinal, const tm_init_params* params, const float* t
       TM_REQUIRE(batch_candidates[c] >= 1 && batch_candidates[c] <= 256, TM_ERR_INVALID,
                  "batch candidates must be in 1..256");
     for (int c = 0; c < n_column; ++c)
-      TM_REQUIRE(column_candidates[c] >= TM_COLUMNS_AUTO && column_candidates[c] <= TM_COLUMNS_PERSISTENT4,
+      TM_REQUIRE(column_candidates[c] >= TM_COLUMNS_AUTO && column_candidates[c] <= TM_COLUMNS_NARROW,
                  TM_ERR_INVALID, "column-pass candidates must be TM_COLUMNS_*");
     TM_REQUIRE(params->n_psi > 0 && params->first_search_position >= 0 &&
                    params->first_search_position <= params->last_search_position,

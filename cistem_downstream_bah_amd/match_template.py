@@ -138,7 +138,7 @@ class SearchParams:
     first_search_position: int = -1
     last_search_position: int = -1
     batch_size: int = 0
-    plane_storage: str = "fp32"  # "fp16"/"bf16": 16-bit inter-pass plane (image height 4096 only)
+    plane_storage: str = "fp32"  # "fp16"/"bf16": 16-bit inter-pass plane
     column_pass: str = "auto"    # column-transform kernel at image height 4096 (core.COLUMN_PASS)
     orientation_limit: int = 0   # only the first N orientations of the position range (0: all)
 
@@ -228,10 +228,13 @@ class MatchTemplateSearch:
                        column_pass=self.p.column_pass, orientation_limit=self.p.orientation_limit)
         return best, us
 
-    def autotune(self, batches=(32, 64, 96), columns=("pairs", "persistent", "persistent4"),
-                 sample_orientations: int = 1024):
+    def autotune(self, batches=(32, 64, 96), columns=None, sample_orientations: int = 1024):
         """Times a sample of this search at each (column pass, batch depth) (tm_autotune) and
-        re-initialises the search at the fastest. Returns (batch, column pass, table of us per CCF)."""
+        re-initialises the search at the fastest. columns default: the three column passes at image
+        height 4096, the compiled and the narrow column tiles elsewhere. Returns (batch, column pass,
+        table of us per CCF)."""
+        if columns is None:
+            columns = ("pairs", "persistent", "persistent4") if self.H == 4096 else ("auto", "narrow")
         best_b, best_c, table = core.autotune(self.init_params(), self.template_spectrum, self.input_spectrum,
                                               self.grid, self.psi, self.filter_for(0), batches, columns,
                                               sample_orientations, self.device)
@@ -304,3 +307,29 @@ def run_concurrent(searches: list, workers: int = 2) -> list:
         return [s.run() for s in searches]
     with ThreadPoolExecutor(max_workers=workers) as ex:
         return list(ex.map(lambda s: s.run(), searches))
+
+
+def run_group(searches: list, group: "core.TemplateMatchingGroup | None" = None) -> list:
+    """Runs prepared searches (one per image, any compiled sizes, on one device) with every defocus
+    plane of all of them as ONE hipGraph launch (tm_group_run: one graph branch per image); the
+    per-image results equal those of each search's own run(), bit for bit. The searches must have the
+    same number of defocus planes and no pixel-size search. group: a TemplateMatchingGroup over the
+    searches' cores to reuse (its graph is captured once and replayed). Returns the correlations
+    computed per search."""
+    if any(m.n_size_half for m in searches):
+        raise ValueError("run_group takes searches without a pixel-size search")
+    nd = {m.n_def_half for m in searches}
+    if len(nd) != 1:
+        raise ValueError("run_group takes searches with the same number of defocus planes")
+    n_def_half = nd.pop()
+    own = group is None
+    g = core.TemplateMatchingGroup([m.core for m in searches]) if own else group
+    try:
+        done = [0] * len(searches)
+        for di in range(-n_def_half, n_def_half + 1):
+            n = g.run([m.filter_for(di) for m in searches], [0] * len(searches), [di] * len(searches))
+            done = [a + b for a, b in zip(done, n)]
+        return done
+    finally:
+        if own:
+            g.close()

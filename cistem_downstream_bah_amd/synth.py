@@ -197,13 +197,16 @@ def make_synthetic_gpu(seed: int, W: int, H: int, T: int, n_atoms: int, atom_rad
     float64 (torch): the same random draws in the same order (template, positions, orientations, then
     the noise from the numpy generator), so the micrograph equals make_synthetic's up to float64
     rounding. For the workloads that need hundreds of 4096^2 micrographs. Falls back to make_synthetic
-    when no GPU is visible. timing: a dict that receives the seconds of each phase (the device is
-    synchronised after each phase)."""
+    when torch sees no GPU (timing["host_fallback"] then says why); torch must be imported before
+    libtm_hip.so is loaded, so that both use torch's HIP runtime. timing: a dict that receives the
+    seconds of each phase (the device is synchronised after each phase)."""
     try:
         import torch
         if not torch.cuda.is_available():
-            raise ImportError
-    except ImportError:
+            raise ImportError("torch sees no GPU")
+    except ImportError as e:
+        if timing is not None:
+            timing["host_fallback"] = str(e)
         return make_synthetic(seed, W, H, T, n_atoms, atom_radius, n_copies, snr, defocus, pixel_size, kv, cs_mm,
                               amp_contrast, sigma, min_dist, template)
     dev = torch.device("cuda")

@@ -66,8 +66,13 @@ typedef struct tm_init_params {
                                     TM_COLUMNS_PERSISTENT (2): the persistent pass with 2-column Z
                                     tiles (FP32 planes only); TM_COLUMNS_PERSISTENT4 (3): the
                                     persistent pass with the 4-column tiles of TM_COLUMNS_PAIRS
-                                    (both TM_ERR_UNSUPPORTED where they do not apply). Ignored at
-                                    other heights.                                                 */
+                                    (both TM_ERR_UNSUPPORTED where they do not apply).
+                                    At other heights: TM_COLUMNS_AUTO, one block per column tile of
+                                    the size's compiled width, or TM_COLUMNS_NARROW (4), tiles half
+                                    as wide (TM_ERR_UNSUPPORTED at height 4096, and where the narrow
+                                    tiles would change the row-pass kernel: one-column tiles at
+                                    widths 1024, 2048 and 4096 with FP32 planes, tiles under four
+                                    columns there with 16-bit planes).                             */
   long long device_memory_limit; /* 0: none; else tm_init fails with TM_ERR_HIP, freeing what it
                                     allocated and leaving the handle uninitialised, when its device
                                     allocations would exceed this many bytes, as when hipMalloc runs
@@ -82,6 +87,7 @@ typedef struct tm_init_params {
 #define TM_COLUMNS_PAIRS 1
 #define TM_COLUMNS_PERSISTENT 2
 #define TM_COLUMNS_PERSISTENT4 3
+#define TM_COLUMNS_NARROW 4
 
 /* Number of visible HIP devices. Replaces DeviceManager::Init's device query
  * (src/gpu/DeviceManager.cu:27-124). */
@@ -247,6 +253,26 @@ int tm_import_state(tm_core* h, const void* buffer, long long bytes);
  * Not used while timing is enabled. tm_init, and enabling or disabling, drop the captured graph. */
 int tm_set_graph(tm_core* h, int enabled);
 
+/* Groups: several handles on one device (typically one micrograph each, of any compiled sizes and
+ * with their own position ranges) whose planes run as ONE hipGraph launch, with no counterpart in
+ * the reference (which runs one image per process, src/gui/MatchTemplatePanel.cpp:674-778, and one
+ * TemplateMatchingCore per thread, match_template.cpp:908-988). The graph holds one independent
+ * branch per handle: that handle's batches for one plane, in its own order, on its own buffers. It is
+ * captured at the first tm_group_run and replayed by the later ones, each plane's orientation-index
+ * offset read from device memory; a handle re-initialised (tm_init) since the capture makes the next
+ * run capture again. Each handle's maps, sums and histogram equal what tm_run_inner_loop with the
+ * same filter, size_i and defocus_i gives, bit for bit.
+ *
+ * tm_group_create: handles[0..n) must be distinct and on one device, and must outlive the group.
+ * tm_group_run: one plane for every handle: projection_filters[i], size_i[i], defocus_i[i] as in
+ * tm_run_inner_loop; ccfs (may be NULL) receives each handle's CCF count. Handles with kernel timing
+ * on, or with a plane left unfinished by tm_run_partial, give TM_ERR_STATE. */
+typedef struct tm_group tm_group;
+int tm_group_create(tm_core* const* handles, int n, tm_group** out);
+int tm_group_run(tm_group* g, const float* const* projection_filters, const int* size_i, const int* defocus_i,
+                 long long* ccfs);
+int tm_group_destroy(tm_group* g);
+
 /* The launch shape tm_init chose: orientations per launch, the column pass in use (TM_COLUMNS_PAIRS
  * or TM_COLUMNS_PERSISTENT* at height 4096, TM_COLUMNS_AUTO elsewhere) and the column-tile width of
  * the inter-pass plane. Any pointer may be NULL. No counterpart in the reference. */
@@ -256,8 +282,9 @@ int tm_get_launch_shape(tm_core* h, int* batch_size, int* column_pass, int* z_ti
 const char* tm_last_error(void);
 
 /* Autotuner of the launch shape, with no counterpart in the reference: the batch depth
- * (orientations per launch) and the column-pass kernel (TM_COLUMNS_*: one block per column pair with
- * 4-column Z tiles, or the persistent pass with 2- or 4-column tiles). For each (column pass, batch)
+ * (orientations per launch) and the column-pass kernel (TM_COLUMNS_*: at height 4096 one block per
+ * column pair with 4-column Z tiles, or the persistent pass with 2- or 4-column tiles; at other
+ * heights column tiles of the compiled width or half of it). For each (column pass, batch)
  * candidate it builds a handle over a sample of the search (its first sample_orientations
  * orientations), runs one warm-up plane and one timed plane with projection_filter, and writes the
  * time per CCF in microseconds to us_per_ccf[column * n_batch + batch] (-1 where the column pass

@@ -149,3 +149,55 @@ def test_16bit_column_passes_change_no_result(c1, st):
             continue
         for f in ("mip", "best_index", "sum", "sumsq", "hist"):
             assert np.array_equal(getattr(r, f), getattr(ref, f)), (cp, f)
+
+
+@pytest.mark.parametrize("W,H,st", [(512, 512, "fp32"), (384, 324, "fp32"), (2048, 1536, "fp32"), (2048, 2048, "fp16"),
+                                    (2048, 3072, "fp16"), (4096, 2916, "bf16")])
+def test_narrow_column_tiles_change_no_result(W, H, st):
+    """At heights other than 4096 the column pass's tiles of the compiled width and of half that width
+    (TM_COLUMNS_NARROW) give bit-identical MIP, index, sums and histogram wherever both keep the same
+    row pass (2-column tiles read by the direct-load row pass at 2048 x 1536 FP32 and 4-column tiles at
+    2048^2 FP16; one-column tiles at 2916 and 3072 with 16-bit planes, whose row pass gathers in both
+    cases); the autotuner times both. Where narrow tiles would switch the row pass to the gathering
+    kernel (one-column tiles with FP32 planes at width 2048) they are refused."""
+    s = synth.make_synthetic(8800 + H, W, H, 64, 200, 24.0, 6, 0.1, 15000.0, min_dist=64)
+    tpl = MT.prepare_template(s.volume, 1.0)
+    ref = None
+    for cp in ("auto", "narrow"):
+        print(f"case: {W}x{H} {st}, column tiles {cp}", flush=True)
+        se = MT.MatchTemplateSearch(s.micrograph, s.volume,
+                                    MT.SearchParams(angular_step=10.0, in_plane_angular_step=15.0, padding=1.0,
+                                                    first_search_position=40, last_search_position=41,
+                                                    batch_size=16, column_pass=cp, plane_storage=st),
+                                    template_spectrum=tpl)
+        shape = se.core.launch_shape()
+        assert shape["column_pass"] == cp
+        if ref is None:
+            wide = shape["z_tile"]
+        else:
+            assert shape["z_tile"] == wide // 2
+        r = _results(se)
+        if ref is None:
+            ref = r
+            best_b, best_c, table = se.autotune((16, 32), sample_orientations=64)
+            print(f"autotune {W}x{H}: {best_c} / {best_b} from {table}", flush=True)
+            assert set(table) == {("auto", 16), ("auto", 32), ("narrow", 16), ("narrow", 32)}
+            assert all(v is not None and v > 0 for v in table.values())
+            assert table[(best_c, best_b)] == min(table.values())
+        else:
+            for f in ("mip", "best_index", "sum", "sumsq", "hist"):
+                assert np.array_equal(getattr(r, f), getattr(ref, f)), (W, H, st, f)
+        se.core.close()
+    with pytest.raises(core.TMError):
+        _ = MT.MatchTemplateSearch(s.micrograph, s.volume,
+                                   MT.SearchParams(angular_step=10.0, in_plane_angular_step=15.0, padding=1.0,
+                                                   first_search_position=40, last_search_position=40,
+                                                   column_pass="pairs"),
+                                   template_spectrum=tpl)
+    if (W, H) == (2048, 3072):
+        with pytest.raises(core.TMError):
+            _ = MT.MatchTemplateSearch(s.micrograph, s.volume,
+                                       MT.SearchParams(angular_step=10.0, in_plane_angular_step=15.0, padding=1.0,
+                                                       first_search_position=40, last_search_position=40,
+                                                       column_pass="narrow"),
+                                       template_spectrum=tpl)
