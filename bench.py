@@ -461,8 +461,9 @@ def bench_config2(args):
     """Eight micrographs of mixed sizes (the multi-micrograph workload), one shared 256^3 template
     padded to 512^3, 2.5/1.5 deg grid, one defocus plane per micrograph at its own defocus; micrograph
     i searches positions 9i..9i+8 (2169 orientations). Times the whole set with 1, 2, 4 images in
-    flight (one stream and host thread each) and as one hipGraph launch over all eight (tm_group_run),
-    and reports pixel-correlations per second (sum over images of W*H*orientations / time)."""
+    flight (one stream and host thread each) and as one hipGraph launch over all eight (tm_group_run)
+    whose chains run 1, 2, 4 or 8 micrographs at a time, and reports pixel-correlations per second
+    (sum over images of W*H*orientations / time)."""
     from cistem_downstream_bah_amd import core, synth
     from cistem_downstream_bah_amd import match_template as MT
 
@@ -507,25 +508,32 @@ def bench_config2(args):
         runs[str(c)] = {"seconds": round(dt, 4), "pixel_correlations_per_s": round(px_ccf / dt, 1),
                         "ccf_per_s": round(n_ccf / dt, 2)}
         log(f"{c} in flight: {dt:.3f}s -> {px_ccf / dt / 1e9:.2f} G pixel-correlations/s")
-    # every micrograph's plane in one hipGraph launch (tm_group_run: one branch per micrograph); the
-    # first launch captures the graph, the timed one replays it
-    group = core.TemplateMatchingGroup([m.core for m in searches])
-    for m in searches:
-        m.core.reset()
-    MT.run_group(searches, group)
-    ref = [m.core.results() for m in searches]
-    for m in searches:
-        m.core.reset()
-    t0 = time.perf_counter()
-    done = MT.run_group(searches, group)
-    dt = time.perf_counter() - t0
-    assert sum(done) == n_ccf
-    same = all(np.array_equal(getattr(a, k), getattr(m.core.results(), k))
-               for a, m in zip(ref, searches) for k in ("mip", "best_index", "sum", "sumsq", "hist"))
-    runs["graph"] = {"seconds": round(dt, 4), "pixel_correlations_per_s": round(px_ccf / dt, 1),
-                     "ccf_per_s": round(n_ccf / dt, 2), "replay_equals_capture_run": bool(same)}
-    log(f"one graph over all micrographs: {dt:.3f}s -> {px_ccf / dt / 1e9:.2f} G pixel-correlations/s")
-    group.close()
+    # every micrograph's plane in one hipGraph launch (tm_group_run: the micrographs dealt to `lanes`
+    # chains, the chains independent branches of the graph); the first launch captures the graph, the
+    # timed one replays it
+    ref = None
+    for lanes in [int(x) for x in args.concurrency.split(",") if x] + [len(searches)]:
+        if f"graph_{lanes}" in runs:
+            continue
+        group = core.TemplateMatchingGroup([m.core for m in searches], lanes)
+        for m in searches:
+            m.core.reset()
+        MT.run_group(searches, group)
+        if ref is None:
+            ref = [m.core.results() for m in searches]
+        for m in searches:
+            m.core.reset()
+        t0 = time.perf_counter()
+        done = MT.run_group(searches, group)
+        dt = time.perf_counter() - t0
+        assert sum(done) == n_ccf
+        same = all(np.array_equal(getattr(a, k), getattr(m.core.results(), k))
+                   for a, m in zip(ref, searches) for k in ("mip", "best_index", "sum", "sumsq", "hist"))
+        runs[f"graph_{lanes}"] = {"seconds": round(dt, 4), "pixel_correlations_per_s": round(px_ccf / dt, 1),
+                                  "ccf_per_s": round(n_ccf / dt, 2), "results_equal_first_graph_run": bool(same)}
+        log(f"one graph over all micrographs, {lanes} chains: {dt:.3f}s -> {px_ccf / dt / 1e9:.2f} G "
+            f"pixel-correlations/s")
+        group.close()
     best = max(runs, key=lambda k: runs[k]["pixel_correlations_per_s"])
     out = {
         "metric": "pixel-correlations/sec over the multi-micrograph set (sum of W*H*orientations / time)",

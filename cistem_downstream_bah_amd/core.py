@@ -102,7 +102,7 @@ def lib():
                                   ctypes.c_int, I32P, ctypes.c_int, ctypes.c_int, I32P, I32P, F64P]
         L.tm_autotune_batch.argtypes = [ctypes.c_int, ctypes.POINTER(InitParams), F32P, F32P, F32P, F32P, F32P, I32P,
                                         ctypes.c_int, ctypes.c_int, I32P, F64P]
-        L.tm_group_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(vp)]
+        L.tm_group_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
         L.tm_group_run.argtypes = [vp, ctypes.POINTER(F32P), I32P, I32P, I64P]
         L.tm_group_destroy.argtypes = [vp]
         _lib = L
@@ -383,15 +383,15 @@ class TemplateMatchingCore:
 
 class TemplateMatchingGroup:
     """Several TemplateMatchingCore handles on one device whose planes run as one hipGraph launch
-    (tm_group_*): one independent branch per handle. Each handle's maps equal those of its own
-    run_inner_loop with the same arguments, bit for bit. The handles must stay open while the group
-    is in use."""
+    (tm_group_*): the handles are dealt to `lanes` chains (0: two), the chains being independent
+    branches of the graph. Each handle's maps equal those of its own run_inner_loop with the same
+    arguments, bit for bit. The handles must stay open while the group is in use."""
 
-    def __init__(self, cores: list):
+    def __init__(self, cores: list, lanes: int = 0):
         self.cores = list(cores)
         self._h = ctypes.c_void_p()
         arr = (ctypes.c_void_p * len(self.cores))(*[c.h for c in self.cores])
-        _check(lib().tm_group_create(arr, len(self.cores), ctypes.byref(self._h)))
+        _check(lib().tm_group_create(arr, len(self.cores), int(lanes), ctypes.byref(self._h)))
 
     def close(self):
         if self._h:

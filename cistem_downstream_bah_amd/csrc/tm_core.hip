@@ -8,6 +8,7 @@
 // the CPU loop.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <climits>
 #include <chrono>
@@ -395,13 +396,22 @@ void run_projections(tm_core* h, int o0, int nb) {
 }
 
 // Forward row FFTs of the padded reference: projection row r sits at column W/2 - T/2 of
-// row H/2 - T/2 + r (ClipIntoLargerRealSpace2D); scaled by 1/(W H) (ForwardFFT).
+// row H/2 - T/2 + r (ClipIntoLargerRealSpace2D); scaled by 1/(W H) (ForwardFFT). With a 16-bit
+// plane the rows are stored in the same 16-bit format, times the plane's power-of-two scale (the
+// column pass removes it exactly when it widens them).
 void run_forward_rows(tm_core* h, int nb) {
   Timer t(h, K_FWD_ROWS);
   const int T = h->T;
   const float scale = (float)(1. / (double)(float)((long long)h->W * h->H));
-  rows_r2c(h, h->W, reinterpret_cast<const float*>(h->projT.p), 2 * (size_t)T * h->Th, T + 2, 0, 0, T, h->W / 2 - T / 2,
-           h->rows.p, (size_t)T * h->Wh, h->Wh, T, nb, scale);
+  const float* src = reinterpret_cast<const float*>(h->projT.p);
+  const size_t sbs = 2 * (size_t)T * h->Th, dbs = (size_t)T * h->Wh;
+  const int c0 = h->W / 2 - T / 2;
+  if (h->zst == TM_PLANE_FP16)
+    rows_r2c_16<tmk::ZS_F16>(h, h->W, src, sbs, T + 2, 0, 0, T, c0, h->rows.p, dbs, h->Wh, T, nb, scale * h->zscale);
+  else if (h->zst == TM_PLANE_BF16)
+    rows_r2c_16<tmk::ZS_BF16>(h, h->W, src, sbs, T + 2, 0, 0, T, c0, h->rows.p, dbs, h->Wh, T, nb, scale * h->zscale);
+  else
+    rows_r2c(h, h->W, src, sbs, T + 2, 0, 0, T, c0, h->rows.p, dbs, h->Wh, T, nb, scale);
 }
 
 void run_batch(tm_core* h, int o0, int nb, int g0) {
@@ -833,7 +843,8 @@ int tm_init(tm_core* h, const tm_init_params* p, const float* template_spectrum,
     const int B = h->B;
     h->slice.alloc((size_t)B * P * h->Ph);
     h->projT.alloc((size_t)B * T * h->Th);
-    h->rows.alloc((size_t)B * T * h->Wh);
+    const size_t row_elems = (size_t)B * T * h->Wh;  // forward rows: stored like the plane
+    h->rows.alloc(h->zst == TM_PLANE_FP32 ? row_elems : (row_elems + 1) / 2);
     const size_t z_elems = (size_t)B * z_tiles(h) * H * h->zt;
     h->Z.alloc(h->zst == TM_PLANE_FP32 ? z_elems : (z_elems + 1) / 2);  // cf units; 16-bit: 4 bytes each
 
@@ -1473,15 +1484,18 @@ int tm_get_timing(tm_core* h, const char** names, double* ms, long long* launche
 // ------------------------------------------------------------------ groups of handles
 }  // extern "C"
 
-// Several handles (typically one micrograph each, of any sizes) whose planes run as one hipGraph:
-// one branch per handle, each branch that handle's batches in its own stream order, the branches
-// independent of one another. Captured at the first tm_group_run and replayed after; a handle
-// re-initialised since (its buffers moved) makes the next run capture again.
+// Several handles (typically one micrograph each, of any sizes) whose planes run as one hipGraph. The
+// handles are dealt to `lanes` chains (largest work first, each to the least-loaded chain); a chain
+// runs its handles one after another, each in its own stream order, and the chains are independent
+// branches of the graph, so that `lanes` searches share the GPU at a time. Captured at the first
+// tm_group_run and replayed after; a handle re-initialised since (its buffers moved) makes the next
+// run capture again.
 struct tm_group {
   int device = 0;
+  int lanes = 1;
   std::vector<tm_core*> hs;
   std::vector<unsigned long long> epochs;
-  std::vector<hipEvent_t> ready, join;  // per handle: inputs uploaded (outside the graph); branch end
+  std::vector<hipEvent_t> ready, join;  // per handle: inputs uploaded (outside the graph); end of its work
   hipEvent_t fork = nullptr;
   hipStream_t stream = nullptr;
   hipGraph_t graph = nullptr;
@@ -1503,30 +1517,56 @@ struct tm_group {
 };
 
 namespace {
-// Captures one plane of every handle of the group: the group's stream forks to each handle's stream
-// and joins it again, so the graph holds one independent branch per handle.
+// The chains of a group: handle indices dealt, by decreasing work (pixels x orientations per plane),
+// to the chain with the least work so far.
+std::vector<std::vector<int>> group_chains(const tm_group* g) {
+  const int n = (int)g->hs.size();
+  std::vector<int> order(n);
+  for (int i = 0; i < n; ++i) order[i] = i;
+  auto work = [&](int i) { return (double)g->hs[i]->W * g->hs[i]->H * g->hs[i]->n_orient; };
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work(a) > work(b); });
+  std::vector<std::vector<int>> chains(std::min(g->lanes, n));
+  std::vector<double> load(chains.size(), 0.0);
+  for (int i : order) {
+    const size_t l = std::min_element(load.begin(), load.end()) - load.begin();
+    chains[l].push_back(i);
+    load[l] += work(i);
+  }
+  return chains;
+}
+
+// Captures one plane of every handle of the group: the group's stream forks to the first handle of
+// each chain, each handle's stream waits for the previous handle of its chain, and the last handle of
+// each chain joins the group's stream again.
 void capture_group(tm_group* g) {
+  const auto chains = group_chains(g);
+  const size_t n = g->hs.size();
+  std::vector<char> entered(n, 0), joined(n, 0);
   TM_HIP(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
-  size_t joined = 0;
   try {
     TM_HIP(hipEventRecord(g->fork, g->stream));
-    for (size_t i = 0; i < g->hs.size(); ++i) {
-      tm_core* h = g->hs[i];
-      TM_HIP(hipStreamWaitEvent(h->stream, g->fork, 0));
-      h->capturing = true;
-      const long long first = (long long)h->prm.first_search_position * h->n_psi;
-      for (int o0 = 0; o0 < h->n_orient; o0 += h->B)
-        run_batch(h, o0, std::min(h->B, h->n_orient - o0), (int)(first + o0));
-      h->capturing = false;
-      TM_HIP(hipEventRecord(g->join[i], h->stream));
-      TM_HIP(hipStreamWaitEvent(g->stream, g->join[i], 0));
-      joined = i + 1;
+    for (const auto& chain : chains) {
+      hipEvent_t prev = g->fork;
+      for (int i : chain) {
+        tm_core* h = g->hs[i];
+        TM_HIP(hipStreamWaitEvent(h->stream, prev, 0));
+        entered[i] = 1;
+        h->capturing = true;
+        const long long first = (long long)h->prm.first_search_position * h->n_psi;
+        for (int o0 = 0; o0 < h->n_orient; o0 += h->B)
+          run_batch(h, o0, std::min(h->B, h->n_orient - o0), (int)(first + o0));
+        h->capturing = false;
+        TM_HIP(hipEventRecord(g->join[i], h->stream));
+        prev = g->join[i];
+      }
+      TM_HIP(hipStreamWaitEvent(g->stream, prev, 0));
+      for (int i : chain) joined[i] = 1;
     }
   } catch (...) {
     // join every stream that entered the capture, so that ending it returns them all to normal
-    for (size_t i = 0; i < g->hs.size(); ++i) {
+    for (size_t i = 0; i < n; ++i) {
       g->hs[i]->capturing = false;
-      if (i >= joined && hipEventRecord(g->join[i], g->hs[i]->stream) == hipSuccess)
+      if (entered[i] && !joined[i] && hipEventRecord(g->join[i], g->hs[i]->stream) == hipSuccess)
         (void)hipStreamWaitEvent(g->stream, g->join[i], 0);
     }
     hipGraph_t dead = nullptr;
@@ -1536,17 +1576,18 @@ void capture_group(tm_group* g) {
   }
   TM_HIP(hipStreamEndCapture(g->stream, &g->graph));
   TM_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
-  for (size_t i = 0; i < g->hs.size(); ++i) g->epochs[i] = g->hs[i]->epoch;
+  for (size_t i = 0; i < n; ++i) g->epochs[i] = g->hs[i]->epoch;
 }
 }  // namespace
 
 extern "C" {
 
-int tm_group_create(tm_core* const* handles, int n, tm_group** out) {
+int tm_group_create(tm_core* const* handles, int n, int lanes, tm_group** out) {
   return guarded([&] {
     TM_REQUIRE(out, TM_ERR_INVALID, "out is NULL");
     *out = nullptr;
     TM_REQUIRE(handles && n > 0, TM_ERR_INVALID, "a group needs at least one handle");
+    TM_REQUIRE(lanes >= 0, TM_ERR_INVALID, "lanes must be >= 0");
     for (int i = 0; i < n; ++i) {
       TM_REQUIRE(handles[i], TM_ERR_INVALID, "handle " + std::to_string(i) + " is NULL");
       TM_REQUIRE(handles[i]->device == handles[0]->device, TM_ERR_INVALID, "the handles of a group share one device");
@@ -1556,6 +1597,7 @@ int tm_group_create(tm_core* const* handles, int n, tm_group** out) {
     TM_HIP(hipSetDevice(handles[0]->device));
     auto g = std::make_unique<tm_group>();
     g->device = handles[0]->device;
+    g->lanes = std::min(n, lanes > 0 ? lanes : 2);
     g->hs.assign(handles, handles + n);
     g->epochs.assign(n, ~0ULL);
     g->ready.assign(n, nullptr);

@@ -223,13 +223,27 @@ void rows_c2r_line(Ctx* h, int N, const cf* in, size_t ibs, int irs, float* out,
   }
 }
 
-template <int N, class Ctx>
+template <int N, class Ctx, int OS = tmk::ZS_F32>
 void launch_r2c(Ctx* h, const float* src, size_t sbs, int srs, int r0, int s0, int len, int c0, cf* dst,
                 size_t dbs, int drs, int nrows, int nb, float scale) {
   constexpr int NT = row_threads<N>();
-  hipLaunchKernelGGL((tmk::k_rows_r2c<N, NT>), dim3(nrows, nb), dim3(NT), 0, h->stream, src, sbs, srs, r0, s0, len, c0,
-                     dst, dbs, drs, h->twiddle(N), h->twiddle(N / 2), scale);
+  hipLaunchKernelGGL((tmk::k_rows_r2c<N, NT, OS>), dim3(nrows, nb), dim3(NT), 0, h->stream, src, sbs, srs, r0, s0, len,
+                     c0, dst, dbs, drs, h->twiddle(N), h->twiddle(N / 2), scale);
   check_launch("k_rows_r2c");
+}
+
+// Row R2C of an image width with 16-bit output (OS = tmk::ZS_F16 or ZS_BF16): the search's forward
+// rows when its plane is stored in 16 bits.
+template <int OS, class Ctx>
+void rows_r2c_16(Ctx* h, int N, const float* src, size_t sbs, int srs, int r0, int s0, int len, int c0, cf* dst,
+                 size_t dbs, int drs, int nrows, int nb, float scale) {
+  switch (N) {
+#define TM_CASE(n) \
+  case n: launch_r2c<n, Ctx, OS>(h, src, sbs, srs, r0, s0, len, c0, dst, dbs, drs, nrows, nb, scale); break;
+    TM_IMAGE_SIZES(TM_CASE)
+#undef TM_CASE
+    default: throw TmError{TM_ERR_UNSUPPORTED, "unsupported R2C row length " + std::to_string(N)};
+  }
 }
 
 template <class Ctx>

@@ -454,11 +454,49 @@ __global__ __launch_bounds__(NT) void k_rows_c2r(const cf* in, size_t ibs, int i
   }
 }
 
+// ---------------------------------------------------------------- inter-pass plane storage
+// Storage of the plane Z between the column and row transforms: ZS_F32 (FP32 complex, 8 bytes),
+// ZS_F16 or ZS_BF16 (two 16-bit halves packed in 4 bytes, real in the low half). 16-bit values are
+// stored multiplied by a power of two (exact) that brings the plane's typical magnitude near 1, so
+// FP16 keeps its full precision; the row pass multiplies by the inverse after loading.
+enum { ZS_F32 = 0, ZS_F16 = 1, ZS_BF16 = 2 };
+
+template <int ZST>
+__device__ __forceinline__ unsigned short to_half_bits(float x) {
+  if constexpr (ZST == ZS_F16) {
+    const _Float16 h = (_Float16)x;  // round to nearest even
+    unsigned short b;
+    __builtin_memcpy(&b, &h, sizeof(b));
+    return b;
+  } else {
+    // bfloat16, round to nearest even (finite values)
+    unsigned u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+  }
+}
+template <int ZST>
+__device__ __forceinline__ float from_half_bits(unsigned b) {
+  if constexpr (ZST == ZS_F16) {
+    const unsigned short s = (unsigned short)b;
+    _Float16 h;
+    __builtin_memcpy(&h, &s, sizeof(h));
+    return (float)h;
+  } else {
+    return __uint_as_float(b << 16);
+  }
+}
+template <int ZST>
+__device__ __forceinline__ unsigned pack_half2(cf v, float scale) {
+  return (unsigned)to_half_bits<ZST>(v.x * scale) | ((unsigned)to_half_bits<ZST>(v.y * scale) << 16);
+}
+
 // R2C of rows of length N (unscaled, then * scale) via an N/2 complex forward FFT.
 // The input row is zero except a window of `len` reals starting at column c0 of the N-row,
 // read from src + b*sbs + (r0 + row)*srs + s0 (float units). Output N/2+1 complex at
-// dst + b*dbs + row*drs (complex units).
-template <int N, int NT>
+// dst + b*dbs + row*drs (element units): FP32 complex, or with OS = ZS_F16 / ZS_BF16 one 32-bit
+// word per element holding the two 16-bit parts of the scaled value (pack_half2).
+template <int N, int NT, int OS = ZS_F32>
 __global__ __launch_bounds__(NT) void k_rows_r2c(const float* __restrict__ src, size_t sbs, int srs, int r0, int s0,
                                                  int len, int c0, cf* __restrict__ dst, size_t dbs, int drs,
                                                  const cf* __restrict__ tw_n, const cf* __restrict__ tw_h,
@@ -476,12 +514,16 @@ __global__ __launch_bounds__(NT) void k_rows_r2c(const float* __restrict__ src, 
   __syncthreads();
   fft_inplace<h, -1, NT>(buf, tw_h, tid);
   // X_k = E_k + W^k O_k, X_{h-k} = conj(E_k - W^k O_k); E = (Z_k + conj Z_{h-k})/2, O = (Z_k - conj Z_{h-k})/(2i)
-  cf* d = dst + (size_t)b * dbs + (size_t)row * drs;
+  const size_t d0 = (size_t)b * dbs + (size_t)row * drs;
+  auto put = [&](int k, cf v) {  // element k of the row: v * scale
+    if constexpr (OS == ZS_F32) dst[d0 + k] = cscale(v, scale);
+    else reinterpret_cast<unsigned*>(dst)[d0 + k] = pack_half2<OS>(v, scale);
+  };
   for (int k = tid; k <= h / 2; k += NT) {
     if (k == 0) {
       const cf z0 = buf[lpad(0)];
-      d[0] = cmake((z0.x + z0.y) * scale, 0.f);
-      d[h] = cmake((z0.x - z0.y) * scale, 0.f);
+      put(0, cmake(z0.x + z0.y, 0.f));
+      put(h, cmake(z0.x - z0.y, 0.f));
     } else {
       const int kk = h - k;
       const cf zk = buf[lpad(k)], zc = cconj(buf[lpad(kk)]);
@@ -491,8 +533,8 @@ __global__ __launch_bounds__(NT) void k_rows_r2c(const float* __restrict__ src, 
       const cf wO = cmul(tw_n[k], O);
       const cf Xk = cadd(E, wO);
       const cf Xkk = cconj(csub(E, wO));
-      d[k] = cscale(Xk, scale);
-      d[kk] = cscale(Xkk, scale);
+      put(k, Xk);
+      put(kk, Xkk);
     }
   }
 }
@@ -551,45 +593,16 @@ __global__ __launch_bounds__(NT) void k_normalize(float* __restrict__ proj, int 
   }
 }
 
-// ---------------------------------------------------------------- inter-pass plane storage
-// Storage of the plane Z between the column and row transforms: ZS_F32 (FP32 complex, 8 bytes),
-// ZS_F16 or ZS_BF16 (two 16-bit halves packed in 4 bytes, real in the low half). 16-bit values are
-// stored multiplied by a power of two (exact) that brings the plane's typical magnitude near 1, so
-// FP16 keeps its full precision; the row pass multiplies by the inverse after loading.
-enum { ZS_F32 = 0, ZS_F16 = 1, ZS_BF16 = 2 };
-
-template <int ZST>
-__device__ __forceinline__ unsigned short to_half_bits(float x) {
-  if constexpr (ZST == ZS_F16) {
-    const _Float16 h = (_Float16)x;  // round to nearest even
-    unsigned short b;
-    __builtin_memcpy(&b, &h, sizeof(b));
-    return b;
-  } else {
-    // bfloat16, round to nearest even (finite values)
-    unsigned u = __float_as_uint(x);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (unsigned short)(u >> 16);
-  }
-}
-template <int ZST>
-__device__ __forceinline__ float from_half_bits(unsigned b) {
-  if constexpr (ZST == ZS_F16) {
-    const unsigned short s = (unsigned short)b;
-    _Float16 h;
-    __builtin_memcpy(&h, &s, sizeof(h));
-    return (float)h;
-  } else {
-    return __uint_as_float(b << 16);
-  }
-}
-template <int ZST>
-__device__ __forceinline__ unsigned pack_half2(cf v, float scale) {
-  return (unsigned)to_half_bits<ZST>(v.x * scale) | ((unsigned)to_half_bits<ZST>(v.y * scale) << 16);
-}
 template <int ZST>
 __device__ __forceinline__ cf unpack_half2(unsigned w, float inv_scale) {
   return cmake(from_half_bits<ZST>(w & 0xFFFFu) * inv_scale, from_half_bits<ZST>(w >> 16) * inv_scale);
+}
+// Element i of the forward rows, stored like the plane: FP32 complex, or 16-bit pairs scaled by the
+// plane's power-of-two scale (k_rows_r2c<.., ZST>), widened and unscaled here (exact).
+template <int ZST>
+__device__ __forceinline__ cf row_elem(const cf* __restrict__ rows, size_t i, float inv_scale) {
+  if constexpr (ZST == ZS_F32) return rows[i];
+  else return unpack_half2<ZST>(reinterpret_cast<const unsigned*>(rows)[i], inv_scale);
 }
 
 // ---------------------------------------------------------------- column pass
@@ -619,14 +632,15 @@ __global__ __launch_bounds__(KC* NTC) void k_mid(const cf* __restrict__ rows, co
   const int jlo = H / 2 - T / 2;
   const size_t rows_bs = (size_t)T * Wh;
   const size_t z_bs = (size_t)ntiles * H * KC;
+  const float rinv = 1.0f / zscale;  // a power of two: exact
   for (int b = blockIdx.y; b < nb; b += gridDim.y) {
     for (int e = ctid; e < H; e += NTC) buf[c][lpad(e)] = cmake(0.f, 0.f);
     __syncthreads();
-    const cf* rb = rows + b * rows_bs;
+    const size_t rb = b * rows_bs;
     for (int e = tid; e < KC * T; e += KC * NTC) {
       const int cc = e % KC, r = e / KC;
       const int kxx = kx0 + cc;
-      if (kxx < Wh) buf[cc][lpad(jlo + r)] = rb[(size_t)r * Wh + kxx];
+      if (kxx < Wh) buf[cc][lpad(jlo + r)] = row_elem<ZST>(rows, rb + (size_t)r * Wh + kxx, rinv);
     }
     __syncthreads();
     fft_inplace<H, -1, NTC>(buf[c], tw, ctid);
@@ -681,6 +695,7 @@ __global__ __launch_bounds__(KC * 256, 4) void k_mid4096(const cf* __restrict__ 
   constexpr int H = 4096, M = 256;
   constexpr int CS = lds_len(H) + 8;  // column stride: shifts the columns by 16 banks
   static_assert(KC * M <= 1024, "KC columns of 256 butterflies per block");
+  const float rinv = 1.0f / zscale;  // 16-bit forward rows carry the plane's power-of-two scale
   __shared__ cf lds[KC * CS];
   // Twiddles from LDS: W4096^(16 m) = W256^m; the stage-2 twiddle W256^(k q) and the second factor
   // of the stage-3 twiddle W4096^(j q) = W256^(a q) * W4096^(b q) (j = 16 a + b) are stored q-major,
@@ -721,7 +736,7 @@ __global__ __launch_bounds__(KC * 256, 4) void k_mid4096(const cf* __restrict__ 
     cf* const p_s2 = colbuf + (j / 16) * 272 + k2;            // stage-2 stores: + 17 q
     const cf* ic = inpT + (size_t)(valid ? kx : 0) * H + j;   // input spectrum: + 256 q
     cf v[16];
-    const cf* rb = rows + (size_t)b * T * Wh + kx;
+    const size_t rb = (size_t)b * T * Wh + kx;
     if constexpr (DIRECT) {
       // T <= 256: forward stage 1 (Ns = 1) has the single non-zero input x[j] (q = 0), so its 16
       // outputs all equal x[j] and element 16 j' + k1 of its result is x[j']. Stage 2 butterfly j
@@ -729,7 +744,7 @@ __global__ __launch_bounds__(KC * 256, 4) void k_mid4096(const cf* __restrict__ 
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
         const int r = j / 16 + 16 * q;
-        v[q] = (valid && r < T) ? rb[(size_t)r * Wh] : cmake(0.f, 0.f);
+        v[q] = (valid && r < T) ? row_elem<ZST>(rows, rb + (size_t)r * Wh, rinv) : cmake(0.f, 0.f);
       }
     } else {
       // forward stage 1 (Ns = 1): inputs x[j + 256 q] from the T non-zero rows; only q <= (T-1)/256
@@ -740,8 +755,8 @@ __global__ __launch_bounds__(KC * 256, 4) void k_mid4096(const cf* __restrict__ 
         v[q] = cmake(0.f, 0.f);
         if (q <= qmax) {
           const int r = j + M * q;
-          const cf* rq = rb + (ptrdiff_t)(M * q) * Wh;  // block-uniform base
-          if (valid && r < T) v[q] = rq[j * Wh];
+          const size_t rq = rb + (size_t)(M * q) * Wh;  // block-uniform base
+          if (valid && r < T) v[q] = row_elem<ZST>(rows, rq + (size_t)j * Wh, rinv);
         }
       }
       dft16<-1>(v);
@@ -850,6 +865,7 @@ __global__ __launch_bounds__(512, 4) void k_mid4096p(const cf* __restrict__ rows
                                                      const cf* __restrict__ tw, float zscale) {
   constexpr int H = 4096, M = 256, KC = 2;
   constexpr bool NTS = ZT == 2, ROT = ZT == 2;
+  const float rinv = 1.0f / zscale;  // 16-bit forward rows carry the plane's power-of-two scale
   static_assert(ZT == 2 || ZT == 4, "Z tiles of 2 or 4 columns");
   constexpr int CS = lds_len(H) + 8;  // column stride: shifts the columns by 16 banks
   __shared__ cf lds[KC * CS];
@@ -905,14 +921,14 @@ __global__ __launch_bounds__(512, 4) void k_mid4096p(const cf* __restrict__ rows
     cf* const p_rd = colbuf + j + (j >> 4);                   // strided reads:  + 272 q
     cf* const p_s2 = colbuf + (j / 16) * 272 + k2;            // stage-2 stores: + 17 q
     const cf* ic = inpT + (size_t)(valid ? kx : 0) * H + j;   // input spectrum: + 256 q
-    const cf* rb = rows + (size_t)b * T * Wh + kx;
+    const size_t rb = (size_t)b * T * Wh + kx;
     // forward stage 1 (Ns = 1) has the single non-zero input x[j] (T <= 256), so stage 2 butterfly j
     // takes x[j / 16 + 16 q] straight from the rows (as k_mid4096 with DIRECT)
     cf v[16], iv[16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
       const int r = j / 16 + 16 * q;
-      v[q] = (valid && r < T) ? rb[(size_t)r * Wh] : cmake(0.f, 0.f);
+      v[q] = (valid && r < T) ? row_elem<ZST>(rows, rb + (size_t)r * Wh, rinv) : cmake(0.f, 0.f);
     }
 #pragma unroll
     for (int q = 0; q < 16; ++q) iv[q] = valid ? ic[M * q] : cmake(0.f, 0.f);

@@ -309,9 +309,10 @@ def run_concurrent(searches: list, workers: int = 2) -> list:
         return list(ex.map(lambda s: s.run(), searches))
 
 
-def run_group(searches: list, group: "core.TemplateMatchingGroup | None" = None) -> list:
+def run_group(searches: list, group: "core.TemplateMatchingGroup | None" = None, lanes: int = 0) -> list:
     """Runs prepared searches (one per image, any compiled sizes, on one device) with every defocus
-    plane of all of them as ONE hipGraph launch (tm_group_run: one graph branch per image); the
+    plane of all of them as ONE hipGraph launch (tm_group_run: the images dealt to `lanes` chains,
+    0: two, each chain an independent branch of the graph); the
     per-image results equal those of each search's own run(), bit for bit. The searches must have the
     same number of defocus planes and no pixel-size search. group: a TemplateMatchingGroup over the
     searches' cores to reuse (its graph is captured once and replayed). Returns the correlations
@@ -323,7 +324,7 @@ def run_group(searches: list, group: "core.TemplateMatchingGroup | None" = None)
         raise ValueError("run_group takes searches with the same number of defocus planes")
     n_def_half = nd.pop()
     own = group is None
-    g = core.TemplateMatchingGroup([m.core for m in searches]) if own else group
+    g = core.TemplateMatchingGroup([m.core for m in searches], lanes) if own else group
     try:
         done = [0] * len(searches)
         for di in range(-n_def_half, n_def_half + 1):

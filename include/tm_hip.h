@@ -256,19 +256,21 @@ int tm_set_graph(tm_core* h, int enabled);
 /* Groups: several handles on one device (typically one micrograph each, of any compiled sizes and
  * with their own position ranges) whose planes run as ONE hipGraph launch, with no counterpart in
  * the reference (which runs one image per process, src/gui/MatchTemplatePanel.cpp:674-778, and one
- * TemplateMatchingCore per thread, match_template.cpp:908-988). The graph holds one independent
- * branch per handle: that handle's batches for one plane, in its own order, on its own buffers. It is
- * captured at the first tm_group_run and replayed by the later ones, each plane's orientation-index
- * offset read from device memory; a handle re-initialised (tm_init) since the capture makes the next
- * run capture again. Each handle's maps, sums and histogram equal what tm_run_inner_loop with the
- * same filter, size_i and defocus_i gives, bit for bit.
+ * TemplateMatchingCore per thread, match_template.cpp:908-988). The handles are dealt to `lanes`
+ * chains (largest work first, each to the chain with the least work so far): a chain runs its
+ * handles one after another, and the chains are independent branches of the graph, so that `lanes`
+ * searches share the GPU at a time (0: two). The graph is captured at the first tm_group_run and
+ * replayed by the later ones, each plane's orientation-index offset read from device memory; a handle
+ * re-initialised (tm_init) since the capture makes the next run capture again. Each handle's maps,
+ * sums and histogram equal what tm_run_inner_loop with the same filter, size_i and defocus_i gives,
+ * bit for bit, whatever the lanes.
  *
  * tm_group_create: handles[0..n) must be distinct and on one device, and must outlive the group.
  * tm_group_run: one plane for every handle: projection_filters[i], size_i[i], defocus_i[i] as in
  * tm_run_inner_loop; ccfs (may be NULL) receives each handle's CCF count. Handles with kernel timing
  * on, or with a plane left unfinished by tm_run_partial, give TM_ERR_STATE. */
 typedef struct tm_group tm_group;
-int tm_group_create(tm_core* const* handles, int n, tm_group** out);
+int tm_group_create(tm_core* const* handles, int n, int lanes, tm_group** out);
 int tm_group_run(tm_group* g, const float* const* projection_filters, const int* size_i, const int* defocus_i,
                  long long* ccfs);
 int tm_group_destroy(tm_group* g);

@@ -1,8 +1,9 @@
-"""GPU: searches of images of different sizes run as one hipGraph launch per plane (tm_group_run, one
-graph branch per image) give each image the maps, sums and histogram of its own tm_run_inner_loop
-calls, bit for bit: every branch runs that image's kernels on its own buffers in its own order. The
-second plane replays the captured graph with a new plane offset; a handle re-initialised after the
-capture (its buffers moved) makes the group capture again."""
+"""GPU: searches of images of different sizes run as one hipGraph launch per plane (tm_group_run: the
+images dealt to chains, each chain an independent branch of the graph) give each image the maps, sums
+and histogram of its own tm_run_inner_loop calls, bit for bit, with one, two and three chains: every
+image's kernels run on its own buffers in its own order. The second plane replays the captured graph
+with a new plane offset; a handle re-initialised after the capture (its buffers moved) makes the group
+capture again."""
 import numpy as np
 import pytest
 
@@ -34,14 +35,15 @@ def _same(a, b, tag):
         assert np.array_equal(np.asarray(getattr(a, name)), np.asarray(getattr(b, name))), (tag, name)
 
 
-def test_group_equals_separate_runs():
+@pytest.mark.parametrize("lanes", [1, 2, 3])
+def test_group_equals_separate_runs(lanes):
     searches = _searches()
-    print("case: group of 3 images (512^2, 384x324, 270^2), 3 defocus planes", flush=True)
+    print(f"case: group of 3 images (512^2, 384x324, 270^2), 3 defocus planes, {lanes} chains", flush=True)
     n_sep = [m.run() for m in searches]
     sep = [m.core.results() for m in searches]
     for m in searches:
         m.core.reset()
-    g = core.TemplateMatchingGroup([m.core for m in searches])
+    g = core.TemplateMatchingGroup([m.core for m in searches], lanes)
     n_grp = MT.run_group(searches, g)
     assert n_grp == n_sep == [m.total_correlation_positions for m in searches]
     for m, a in zip(searches, sep):
@@ -72,6 +74,8 @@ def test_group_rejects_bad_arguments():
     cores = [m.core for m in searches]
     with pytest.raises(core.TMError):
         core.TemplateMatchingGroup([cores[0], cores[0]])
+    with pytest.raises(core.TMError):
+        core.TemplateMatchingGroup(cores, -1)
     g = core.TemplateMatchingGroup(cores)
     with pytest.raises(ValueError):
         g.run([searches[0].filter_for(0)])
