@@ -123,6 +123,35 @@ __device__ __forceinline__ void dft16(cf* v) {
   }
 }
 
+// radix-9: 3 x 3 with internal twiddles W9^(n1*k1); input n = n1 + 3 m, output k1 + 3 k2
+template <int SIGN>
+__device__ __forceinline__ void dft9(cf* v) {
+  cf t[3][3];
+#pragma unroll
+  for (int n1 = 0; n1 < 3; ++n1) {
+    cf s[3] = {v[n1], v[n1 + 3], v[n1 + 6]};
+    dft3<SIGN>(s);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) t[n1][k] = s[k];
+  }
+  // W9^m = cos(2 pi m/9) + SIGN i sin(2 pi m/9) for m = n1 * k1 = 1, 2, 4
+  const float c1 = 0.76604444311897803520f, s1 = 0.64278760968653932632f;
+  const float c2 = 0.17364817766693034885f, s2 = 0.98480775301220805936f;
+  const float c4 = -0.93969262078590838405f, s4 = 0.34202014332566873304f;
+  auto tw = [](cf a, float c, float s) { return cmake(a.x * c - a.y * (SIGN * s), a.x * (SIGN * s) + a.y * c); };
+  t[1][1] = tw(t[1][1], c1, s1);
+  t[1][2] = tw(t[1][2], c2, s2);
+  t[2][1] = tw(t[2][1], c2, s2);
+  t[2][2] = tw(t[2][2], c4, s4);
+#pragma unroll
+  for (int k1 = 0; k1 < 3; ++k1) {
+    cf s[3] = {t[0][k1], t[1][k1], t[2][k1]};
+    dft3<SIGN>(s);
+#pragma unroll
+    for (int k2 = 0; k2 < 3; ++k2) v[k1 + 3 * k2] = s[k2];
+  }
+}
+
 template <int R, int SIGN>
 __device__ __forceinline__ void dftR(cf* v) {
   if constexpr (R == 2) dft2<SIGN>(v[0], v[1]);
@@ -130,8 +159,9 @@ __device__ __forceinline__ void dftR(cf* v) {
   else if constexpr (R == 8) dft8<SIGN>(v);
   else if constexpr (R == 16) dft16<SIGN>(v);
   else if constexpr (R == 3) dft3<SIGN>(v);
+  else if constexpr (R == 9) dft9<SIGN>(v);
   else if constexpr (R == 5) dft5<SIGN>(v);
-  else static_assert(R == 2 || R == 3 || R == 4 || R == 5 || R == 8 || R == 16, "unsupported radix");
+  else static_assert(R == 2 || R == 3 || R == 4 || R == 5 || R == 8 || R == 9 || R == 16, "unsupported radix");
 }
 
 // Workgroup barrier that orders LDS traffic only: this wave's LDS operations complete, then
@@ -178,8 +208,9 @@ __device__ __forceinline__ cf twiddle(const TwSplit& t, int idx) {
 
 // ---------------------------------------------------------------- radix plans
 // N = 2^a 3^b 5^c. The power-of-two part runs first, largest radix first: radices 2^LR (LR = 4:
-// radix 16, LR = 3: radix 8) and one smaller radix when a is not a multiple of LR; then b radix-3
-// and c radix-5 stages. For powers of two this is the plain largest-radix-first plan.
+// radix 16, LR = 3: radix 8) and one smaller radix when a is not a multiple of LR; then the factors
+// of three as radix-9 stages (3 x 3 in registers) and a radix-3 stage when b is odd; then c radix-5
+// stages. For powers of two this is the plain largest-radix-first plan.
 __host__ __device__ constexpr int log2i(int n) { return n <= 1 ? 0 : 1 + log2i(n / 2); }
 __host__ __device__ constexpr int nfactor(int n, int f) { return (n > 0 && n % f == 0) ? 1 + nfactor(n / f, f) : 0; }
 __host__ __device__ constexpr int ipow(int b, int e) { return e == 0 ? 1 : b * ipow(b, e - 1); }
@@ -189,12 +220,14 @@ __host__ __device__ constexpr bool fft_size_ok(int n) {
 __host__ __device__ constexpr int plan_stages2(int N, int LR) {
   return nfactor(N, 2) / LR + ((nfactor(N, 2) % LR) ? 1 : 0);
 }
+__host__ __device__ constexpr int plan_stages3(int N) { return nfactor(N, 3) / 2 + nfactor(N, 3) % 2; }
 __host__ __device__ constexpr int plan_stages(int N, int LR = 4) {
-  return plan_stages2(N, LR) + nfactor(N, 3) + nfactor(N, 5);
+  return plan_stages2(N, LR) + plan_stages3(N) + nfactor(N, 5);
 }
 __host__ __device__ constexpr int plan_radix(int N, int s, int LR = 4) {
-  return s < plan_stages2(N, LR) ? ((s < nfactor(N, 2) / LR) ? (1 << LR) : (1 << (nfactor(N, 2) % LR)))
-                                 : (s - plan_stages2(N, LR) < nfactor(N, 3) ? 3 : 5);
+  return s < plan_stages2(N, LR)
+             ? ((s < nfactor(N, 2) / LR) ? (1 << LR) : (1 << (nfactor(N, 2) % LR)))
+             : (s - plan_stages2(N, LR) < nfactor(N, 3) / 2 ? 9 : (s - plan_stages2(N, LR) < plan_stages3(N) ? 3 : 5));
 }
 __host__ __device__ constexpr int plan_ns(int N, int s, int LR = 4) {  // sub-transform length before stage s
   int ns = 1;

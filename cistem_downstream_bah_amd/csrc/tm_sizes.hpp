@@ -21,10 +21,10 @@
 
 namespace tmk {
 
-// Column pass (k_mid, every height except 4096): columns per block and threads per column. The
-// threads per column cover the butterflies of the largest-radix stage; heights with radix-2/3/5
-// stages get more threads per column (fewer butterflies each, so the butterfly registers fit
-// without spilling) and fewer columns per block. LDS per block: columns x (H + H/16) x 8 bytes.
+// Column pass (k_mid, every height except 4096): columns per block and threads per column (the
+// comments give each height's radix plan, tm_fft.hpp). Heights with radix-2/3/5/9 stages get more
+// threads per column (fewer butterflies each, so the butterfly registers fit without spilling) and
+// fewer columns per block. LDS per block: columns x (H + H/16) x 8 bytes.
 constexpr int mid_cols(int H) {
   switch (H) {
     case 1536: return 4;
@@ -39,17 +39,17 @@ constexpr int mid_cols(int H) {
 constexpr int mid_threads(int H) {
   switch (H) {
     case 256: return 16;    // 16 x 16
-    case 270: return 64;    // 2 3 3 3 5
-    case 324: return 64;    // 4 3 3 3 3
+    case 270: return 64;    // 2 9 3 5
+    case 324: return 64;    // 4 9 9
     case 384: return 32;    // 16 8 3
     case 512: return 32;    // 16 16 2
     case 1024: return 64;   // 16 16 4
     case 1536: return 192;  // 16 16 2 3
     case 2048: return 128;  // 16 16 8
-    case 2592: return 324;  // 16 2 3 3 3 3
-    case 2916: return 486;  // 4 3 3 3 3 3 3
+    case 2592: return 324;  // 16 2 9 9
+    case 2916: return 486;  // 4 9 9 9
     case 3072: return 384;  // 16 16 4 3
-    case 5832: return 486;  // 8 3 3 3 3 3 3
+    case 5832: return 486;  // 8 9 9 9
     default: return 0;
   }
 }
