@@ -206,6 +206,27 @@ __device__ __forceinline__ cf twiddle(const TwSplit& t, int idx) {
   return SIGN < 0 ? w : cconj(w);
 }
 
+// The twiddle W^(k q N / (NS R)) of butterfly input q at sub-transform index k of a Stockham stage.
+template <int SIGN, int N, int NS, int R, typename TW>
+__device__ __forceinline__ cf stage_twiddle(TW tw, int k, int q) {
+  return twiddle<SIGN>(tw, k * q * (N / (NS * R)));
+}
+// Factored table plus a direct table of the 64th roots of unity (d64[m] = W^(m N / 64)): the stages
+// whose angles are all multiples of 2 pi / 64 (NS R <= 64) read their twiddles directly.
+struct TwHybrid {
+  const cf* d64;
+  TwSplit split;
+};
+template <int SIGN, int N, int NS, int R>
+__device__ __forceinline__ cf stage_twiddle(const TwHybrid& tw, int k, int q) {
+  if constexpr (NS * R <= 64 && 64 % (NS * R) == 0) {
+    const cf w = tw.d64[k * q * (64 / (NS * R))];
+    return SIGN < 0 ? w : cconj(w);
+  } else {
+    return twiddle<SIGN>(tw.split, k * q * (N / (NS * R)));
+  }
+}
+
 // ---------------------------------------------------------------- radix plans
 // N = 2^a 3^b 5^c. The power-of-two part runs first, largest radix first: radices 2^LR (LR = 4:
 // radix 16, LR = 3: radix 8) and one smaller radix when a is not a multiple of LR; then the factors

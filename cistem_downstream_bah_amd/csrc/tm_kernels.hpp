@@ -43,12 +43,20 @@ __device__ __forceinline__ void fft_inplace(cf* buf, TW tw, int tid) {
     constexpr int M = N / R;
     constexpr int CNT = (M + NT - 1) / NT;
     cf v[CNT][R];
+    // lpad(i + 16 m) = lpad(i) + 17 m for i >= 0: with M (reads) or NS (writes) a multiple of 16 the
+    // R addresses of a butterfly are one padded base plus compile-time offsets
 #pragma unroll
     for (int c = 0; c < CNT; ++c) {
       const int j = tid + c * NT;
       if (tid < NT && j < M) {
+        if constexpr (M % 16 == 0) {
+          const cf* rd = buf + lpad(j);
 #pragma unroll
-        for (int q = 0; q < R; ++q) v[c][q] = buf[lpad(j + q * M)];
+          for (int q = 0; q < R; ++q) v[c][q] = rd[q * (M + M / 16)];
+        } else {
+#pragma unroll
+          for (int q = 0; q < R; ++q) v[c][q] = buf[lpad(j + q * M)];
+        }
       }
     }
     lds_barrier();
@@ -58,14 +66,27 @@ __device__ __forceinline__ void fft_inplace(cf* buf, TW tw, int tid) {
       if (tid < NT && j < M) {
         const int k = j % NS;
         if constexpr (NS > 1) {
-          constexpr int step = N / (NS * R);
 #pragma unroll
-          for (int q = 1; q < R; ++q) v[c][q] = cmul(v[c][q], twiddle<SIGN>(tw, k * q * step));
+          for (int q = 1; q < R; ++q) v[c][q] = cmul(v[c][q], stage_twiddle<SIGN, N, NS, R>(tw, k, q));
         }
         dftR<R, SIGN>(v[c]);
         const int base = (j / NS) * NS * R + k;
+        if constexpr (NS % 16 == 0) {
+          cf* wr = buf + lpad(base);
 #pragma unroll
-        for (int q = 0; q < R; ++q) buf[lpad(base + q * NS)] = v[c][q];
+          for (int q = 0; q < R; ++q) wr[q * (NS + NS / 16)] = v[c][q];
+        } else if constexpr (16 % NS == 0 && R > 16 / NS) {
+          // q NS = (q mod PB) NS + 16 (q / PB) with PB = 16 / NS: PB padded bases, offsets 17 (q / PB)
+          constexpr int PB = 16 / NS;
+          cf* wr[PB];
+#pragma unroll
+          for (int r = 0; r < PB; ++r) wr[r] = buf + lpad(base + r * NS);
+#pragma unroll
+          for (int q = 0; q < R; ++q) wr[q % PB][17 * (q / PB)] = v[c][q];
+        } else {
+#pragma unroll
+          for (int q = 0; q < R; ++q) buf[lpad(base + q * NS)] = v[c][q];
+        }
       }
     }
     lds_barrier();
@@ -1202,8 +1223,10 @@ typedef const __attribute__((address_space(1))) void* global_vptr;
 // (global_load_lds_dwordx4, 16 bytes = two FP32 or four 16-bit adjacent elements of a tile row per
 // lane; 16-bit elements are widened and unscaled when the first stage reads them): while the block transforms
 // and accumulates orientations b0..b0+QR-1, row y of the next QR orientations streams into a second
-// LDS buffer. Arithmetic and pixel ownership are those of k_last; the pre-twist and FFT twiddles come
-// from factored tables so that two blocks still fit in one CU's LDS. Each group of NT = W/16 threads
+// LDS buffer. Pixel ownership is that of k_last; the pre-twist and FFT twiddles come from factored
+// tables so that two blocks still fit in one CU's LDS, except those of the stages whose angles are
+// multiples of 2 pi / 64, read from a direct 64-entry table (the values then agree with k_last's to
+// FP32 rounding, not bit for bit). Each group of NT = W/16 threads
 // must be whole wavefronts (W >= 1024), because a direct load writes one wavefront's 64 lanes to
 // consecutive LDS addresses.
 template <int W, int QR, int ZST = ZS_F32>
@@ -1223,7 +1246,7 @@ void k_last_dma(const cf* __restrict__ Z, int H, int nb, EpiParams ep, const cf*
   constexpr int HC = 4;
   __shared__ __align__(16) unsigned char gbuf[QR][16 * NL];  // gathered rows in element order
   __shared__ cf buf[QR][lds_len(h)];       // FFT work buffers (padded index)
-  __shared__ cf fft_lo[64], fft_hi[h / 64];
+  __shared__ cf fft_lo[64], fft_hi[h / 64], fft_d64[64];
   __shared__ cf pre_lo[64], pre_hi[h / 128 + 1];
   __shared__ unsigned int hist[513 * HC];  // bins 0..511 and the spill bin
   const int tid0 = threadIdx.x;
@@ -1264,6 +1287,7 @@ void k_last_dma(const cf* __restrict__ Z, int H, int nb, EpiParams ep, const cf*
     pre_lo[i] = tw_n[i];
   }
   for (int i = tid0; i < h / 64; i += BT) fft_hi[i] = tw_h[64 * i];
+  for (int i = tid0; i < 64; i += BT) fft_d64[i] = tw_h[(h / 64) * i];
   for (int i = tid0; i <= h / 128; i += BT) pre_hi[i] = tw_n[64 * i];
   float mip[2 * PAIRS];
   int idx[2 * PAIRS];
@@ -1281,7 +1305,7 @@ void k_last_dma(const cf* __restrict__ Z, int H, int nb, EpiParams ep, const cf*
     s1[2 * u] = sa.x; s1[2 * u + 1] = sa.y;
     s2[2 * u] = qa.x; s2[2 * u + 1] = qa.y;
   }
-  const TwSplit ftw{fft_lo, fft_hi};
+  const TwHybrid ftw{fft_d64, TwSplit{fft_lo, fft_hi}};
   const TwSplit ptw{pre_lo, pre_hi};
   for (int b0 = 0; b0 < nb; b0 += QR) {
     // re-derived every iteration from a register the compiler cannot see through, so address

@@ -118,3 +118,66 @@ def test_job_arguments_and_interactive_answers_give_the_named_run(tmp_path):
             assert np.array_equal(a, mrc.read_mrc(f"{other}_{k}.mrc")), (other, k)
     h = open(named + "_histogram.txt").read()
     assert h == open(ja + "_histogram.txt").read() == open(it + "_histogram.txt").read()
+
+
+@pytest.mark.parametrize("mx,my,copies", [(3072, 2048, 300), (470, 360, 5)])
+def test_driver_unrotates_and_crops_to_the_micrograph(tmp_path, mx, my, copies):
+    """The driver on micrographs the size policy rotates (3072 x 2048 -> 2048 x 3072) or pads
+    (470 x 360 -> 512 x 384, match_template.cpp:527-598): its output maps come back in the
+    micrograph's own size and orientation (Rotate2DInPlaceBy90Degrees(false) and Resize,
+    image.cpp:11104-11141, 8825-8838). The ground truth is the insertion itself: the z-score map is
+    higher at the inserted copies' true positions than where an un-rotation in the wrong sense
+    (180 degrees off) or a crop off by the padding would put them (there it is the MIP of noise,
+    z ~ 4), and the peaks the driver lists lie on copies. Both micrographs hold copies over about a
+    tenth of their area at SNR 1 (noise of the frame's signal variance), where copies of this template
+    reach z 6-8 on the 10 deg grid; every copy above the threshold is a listed peak, and at most a few
+    listed peaks (noise, or the offset peaks of touching copies) lie off the copies."""
+    s = synth.make_synthetic(6100 + mx, mx, my, 64, 200, 24.0, copies, 1.0, 15000.0, min_dist=64)
+    W, H, rotated, _, _ = MT.prepare_sizes(mx, my, 64)
+    assert rotated == (mx == 3072) and (W, H) != (mx, my)
+    img, vol = str(tmp_path / "img.mrc"), str(tmp_path / "vol.mrc")
+    mrc.write_mrc(img, s.micrograph)
+    mrc.write_mrc(vol, s.volume)
+    out = str(tmp_path / "run")
+    print(f"case: driver on {mx}x{my} (search size {W}x{H}, rotated {rotated})", flush=True)
+    r = subprocess.run([DRIVER, "--image", img, "--template", vol, "--out", out, "--angular-step", "10",
+                        "--in-plane-step", "10", "--min-peak-radius", "10"], capture_output=True, text=True,
+                       timeout=240)
+    print(r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    z = mrc.read_mrc(out + "_scaled_mip.mrc")
+    assert z.shape == (my, mx)
+    for name in ("psi", "theta", "phi", "defocus"):
+        assert mrc.read_mrc(out + f"_{name}.mrc").shape == (my, mx)
+
+    def z_near(x, y):
+        x, y = int(round(x)), int(round(y))
+        if not (2 <= x < mx - 2 and 2 <= y < my - 2):
+            return np.nan
+        return float(z[y - 2:y + 3, x - 2:x + 3].max())
+
+    truth = np.array([(p.x, p.y) for p in s.particles])
+    at_truth = np.array([z_near(x, y) for x, y in truth])
+    wrong = {"rotated 180 deg": [(mx - 1 - x, my - 1 - y) for x, y in truth],
+             "shifted by the padding": [(x + (W - mx) // 2, y + (H - my) // 2) for x, y in truth]}
+    print(f"z at the true positions: median {np.median(at_truth):.2f}, range {at_truth.min():.2f} - "
+          f"{at_truth.max():.2f}", flush=True)
+    for name, pts in wrong.items():
+        if rotated != (name == "rotated 180 deg"):
+            continue
+        at_wrong = np.array([z_near(x, y) for x, y in pts])
+        print(f"z {name}: median {np.nanmedian(at_wrong):.2f}", flush=True)
+        assert np.nanmedian(at_wrong) <= np.median(at_truth) - 1.0
+    peaks = np.loadtxt(out + "_peaks.txt", ndmin=2)
+    print(f"{len(peaks)} peaks above {open(out + '_histogram.txt').readline().strip()}", flush=True)
+    assert len(peaks) >= 1
+    on_copy = [bool((np.abs(truth - (px, py)).max(axis=1) <= 2).any()) for px, py in peaks[:, :2]]
+    print(f"{sum(on_copy)} of {len(peaks)} peaks within 2 px of a copy", flush=True)
+    # the expected threshold admits a few noise peaks per search, and among 300 copies some peaks of
+    # touching copies sit a few pixels off
+    assert len(on_copy) - sum(on_copy) <= max(3, 0.1 * len(on_copy)), peaks[:, :2]
+    # every copy whose z at its true position reaches the threshold is a listed peak
+    thr = float(open(out + "_histogram.txt").readline().split("=")[1])
+    for (x, y), zt in zip(truth, at_truth):
+        if zt >= thr:
+            assert (np.abs(peaks[:, :2] - (x, y)).max(axis=1) <= 2).any(), (x, y)

@@ -1,11 +1,12 @@
 """GPU parity for image sides that are not powers of two: the mixed radix-2/3/5 FFT plans of the
 preparation, the column pass and the row pass, against the oracle on the same inputs.
 
-* 384 x 324 (2^7 3 x 2^2 3^4), and a 260 x 260 micrograph that the size policy pads to 270 x 270
-  (2 3^3 5, an odd half-length row transform): the input preparation and a search over 10 grid
-  positions to the bars of parity_util
+* 384 x 324 (2^7 3 x 2^2 3^4), a 260 x 260 micrograph that the size policy pads to 270 x 270
+  (2 3^3 5, an odd half-length row transform), and a 384 x 256 micrograph that it rotates to
+  256 x 384 (Rotate2DInPlaceBy90Degrees(true), match_template.cpp:595): the input preparation
+  (rotation included) and a search over 10 grid positions to the bars of parity_util
 * the non-power-of-two image sizes of the multi-micrograph workload after the reference's size
-  policy: 2048 x 3072 (a 3072 x 2048 micrograph, rotated), 4096 x 2916, 2048 x 1536 and 2592 x 2592,
+  policy: 2048 x 3072, 4096 x 2916, 2048 x 1536 and 2592 x 2592,
   and the K3 sizes 4096 x 5832 (a 5760 x 4092 frame after the policy, match_template.cpp:537-538) and
   5832 x 5832 (2^3 3^6: a radix-8 stage and six radix-3 stages per line):
   input preparation, single CCF planes (max |gpu - oracle| / max |oracle| <= 1e-4) and a search over
@@ -51,14 +52,14 @@ def _check_preparation(s, mt, label):
     assert ec <= 1e-5 and ei <= 1e-5
 
 
-@pytest.mark.parametrize("mx,my,W,H", [(384, 324, 384, 324), (260, 260, 270, 270)])
+@pytest.mark.parametrize("mx,my,W,H", [(384, 324, 384, 324), (260, 260, 270, 270), (384, 256, 256, 384)])
 def test_small_mixed_radix_search(mx, my, W, H):
     s = _micrograph(mx, my, 7000 + mx, 3)
     mt, g = _setup(s, O.MTParams(angular_step=15.0, in_plane_angular_step=15.0, padding=1.0,
                                  first_search_position=0, last_search_position=9))
     try:
         I = mt.info
-        assert (I.W, I.H, I.T) == (W, H, 64)
+        assert (I.W, I.H, I.T) == (W, H, 64) and I.rotated == (mx == 384 and my == 256)
         _check_preparation(s, mt, f"{W}x{H}")
         ref = mt.run(threads=THREADS)
         n = g.run_inner_loop(mt.projection_filter(0), 0, 0)
