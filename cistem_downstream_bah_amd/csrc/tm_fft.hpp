@@ -227,6 +227,26 @@ __device__ __forceinline__ cf stage_twiddle(const TwHybrid& tw, int k, int q) {
   }
 }
 
+// All twiddles w[q] = W^(k q N / (NS R)), q = 1 .. R-1, of one butterfly (w[0] unused).
+template <int SIGN, int N, int NS, int R, typename TW>
+__device__ __forceinline__ void stage_twiddles(TW tw, int k, cf* w) {
+#pragma unroll
+  for (int q = 1; q < R; ++q) w[q] = stage_twiddle<SIGN, N, NS, R>(tw, k, q);
+}
+// TwHybrid: stages past the 64th roots look up w[1] only and form the other powers by products
+// (w[2q] = w[q]^2, w[2q+1] = w[2q] w[1]: at most three products deep for R <= 8).
+template <int SIGN, int N, int NS, int R>
+__device__ __forceinline__ void stage_twiddles(const TwHybrid& tw, int k, cf* w) {
+  if constexpr (NS * R <= 64 && 64 % (NS * R) == 0) {
+#pragma unroll
+    for (int q = 1; q < R; ++q) w[q] = stage_twiddle<SIGN, N, NS, R>(tw, k, q);
+  } else {
+    w[1] = stage_twiddle<SIGN, N, NS, R>(tw, k, 1);
+#pragma unroll
+    for (int q = 2; q < R; ++q) w[q] = (q & 1) ? cmul(w[q - 1], w[1]) : cmul(w[q / 2], w[q / 2]);
+  }
+}
+
 // ---------------------------------------------------------------- radix plans
 // N = 2^a 3^b 5^c. The power-of-two part runs first, largest radix first: radices 2^LR (LR = 4:
 // radix 16, LR = 3: radix 8) and one smaller radix when a is not a multiple of LR; then the factors

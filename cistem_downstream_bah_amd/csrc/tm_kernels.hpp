@@ -66,8 +66,10 @@ __device__ __forceinline__ void fft_inplace(cf* buf, TW tw, int tid) {
       if (tid < NT && j < M) {
         const int k = j % NS;
         if constexpr (NS > 1) {
+          cf w[R];
+          stage_twiddles<SIGN, N, NS, R>(tw, k, w);
 #pragma unroll
-          for (int q = 1; q < R; ++q) v[c][q] = cmul(v[c][q], stage_twiddle<SIGN, N, NS, R>(tw, k, q));
+          for (int q = 1; q < R; ++q) v[c][q] = cmul(v[c][q], w[q]);
         }
         dftR<R, SIGN>(v[c]);
         const int base = (j / NS) * NS * R + k;
@@ -1225,8 +1227,9 @@ typedef const __attribute__((address_space(1))) void* global_vptr;
 // and accumulates orientations b0..b0+QR-1, row y of the next QR orientations streams into a second
 // LDS buffer. Pixel ownership is that of k_last; the pre-twist and FFT twiddles come from factored
 // tables so that two blocks still fit in one CU's LDS, except those of the stages whose angles are
-// multiples of 2 pi / 64, read from a direct 64-entry table (the values then agree with k_last's to
-// FP32 rounding, not bit for bit). Each group of NT = W/16 threads
+// multiples of 2 pi / 64, read from a direct 64-entry table, and in the later stages each
+// butterfly looks up its first twiddle and forms the others as its powers (the values then agree
+// with k_last's to FP32 rounding, not bit for bit). Each group of NT = W/16 threads
 // must be whole wavefronts (W >= 1024), because a direct load writes one wavefront's 64 lanes to
 // consecutive LDS addresses.
 template <int W, int QR, int ZST = ZS_F32>
@@ -1323,6 +1326,13 @@ void k_last_dma(const cf* __restrict__ Z, int H, int nb, EpiParams ep, const cf*
     cf v[R1];
     if (active) {
       const unsigned char* gb = gbuf[grp];
+      // W_W^k for k = t + NT q: the table's W_W^t times W_W^(NT q) = W_16^q (NT = W / 16), a constant
+      static_assert(R1 == 8 && NT * 16 == W, "k = t + (W / 16) q");
+      constexpr float c16[8] = {1.0f, 0.92387953251128675613f, 0.70710678118654752440f, 0.38268343236508977173f,
+                                0.0f, -0.38268343236508977173f, -0.70710678118654752440f, -0.92387953251128675613f};
+      constexpr float s16[8] = {0.0f, 0.38268343236508977173f, 0.70710678118654752440f, 0.92387953251128675613f,
+                                1.0f, 0.92387953251128675613f, 0.70710678118654752440f, 0.38268343236508977173f};
+      const cf wt = twiddle<-1>(ptw, t);
 #pragma unroll
       for (int q = 0; q < R1; ++q) {
         const int k = t + NT * q;
@@ -1334,13 +1344,7 @@ void k_last_dma(const cf* __restrict__ Z, int H, int nb, EpiParams ep, const cf*
           const cf Xc = cconj(Xm);
           const cf E = cadd(Xk, Xc);
           const cf D = csub(Xk, Xc);
-          cf w;
-          if (k <= h / 2) {
-            w = twiddle<-1>(ptw, k);
-          } else {
-            const cf w2 = twiddle<-1>(ptw, h - k);
-            w = cmake(-w2.x, w2.y);
-          }
+          const cf w = q == 0 ? wt : cmul(wt, cmake(c16[q], -s16[q]));
           const cf O = cmul(D, cconj(w));
           v[q] = cmake(E.x - O.y, E.y + O.x);
         }
