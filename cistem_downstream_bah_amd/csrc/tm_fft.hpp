@@ -233,6 +233,18 @@ __device__ __forceinline__ void stage_twiddles(TW tw, int k, cf* w) {
 #pragma unroll
   for (int q = 1; q < R; ++q) w[q] = stage_twiddle<SIGN, N, NS, R>(tw, k, q);
 }
+// A full twiddle table in global memory, read once per butterfly: w[1] = W^(k N / (NS R)) is looked
+// up and the other twiddles of the butterfly are its powers (w[2q] = w[q]^2, w[2q+1] = w[2q] w[1]).
+struct TwPow {
+  const cf* t;
+};
+template <int SIGN, int N, int NS, int R>
+__device__ __forceinline__ void stage_twiddles(const TwPow& tw, int k, cf* w) {
+  w[1] = twiddle<SIGN>(tw.t, k * (N / (NS * R)));
+#pragma unroll
+  for (int q = 2; q < R; ++q) w[q] = (q & 1) ? cmul(w[q - 1], w[1]) : cmul(w[q / 2], w[q / 2]);
+}
+
 // TwHybrid: stages past the 64th roots look up w[1] only and form the other powers by products
 // (w[2q] = w[q]^2, w[2q+1] = w[2q] w[1]: at most three products deep for R <= 8).
 template <int SIGN, int N, int NS, int R>

@@ -637,7 +637,8 @@ __device__ __forceinline__ cf row_elem(const cf* __restrict__ rows, size_t i, fl
 // Tiles are mapped so that consecutive tiles run on the same XCD (workgroups are dispatched
 // round-robin over the 8 XCDs), letting partial cache lines of Z merge in that XCD's L2. The
 // orientations of the batch are spread over gridDim.y (block row y takes b = y, y + gridDim.y, ..),
-// so that small images, with few column tiles, still fill the GPU.
+// so that small images, with few column tiles, still fill the GPU. Each butterfly reads one twiddle
+// from the global table and forms the others of its stage as powers of it.
 template <int H, int KC, int NTC, int ZST = ZS_F32>
 __global__ __launch_bounds__(KC* NTC) void k_mid(const cf* __restrict__ rows, const cf* __restrict__ inpT,
                                                  cf* __restrict__ Z, int Wh, int T, int nb, int ntiles,
@@ -666,7 +667,7 @@ __global__ __launch_bounds__(KC* NTC) void k_mid(const cf* __restrict__ rows, co
       if (kxx < Wh) buf[cc][lpad(jlo + r)] = row_elem<ZST>(rows, rb + (size_t)r * Wh + kxx, rinv);
     }
     __syncthreads();
-    fft_inplace<H, -1, NTC>(buf[c], tw, ctid);
+    fft_inplace<H, -1, NTC>(buf[c], TwPow{tw}, ctid);
     if (valid) {
       const cf* ic = inpT + (size_t)kx * H;
       for (int e = ctid; e < H; e += NTC) {
@@ -676,7 +677,7 @@ __global__ __launch_bounds__(KC* NTC) void k_mid(const cf* __restrict__ rows, co
       }
     }
     __syncthreads();
-    fft_inplace<H, +1, NTC>(buf[c], tw, ctid);
+    fft_inplace<H, +1, NTC>(buf[c], TwPow{tw}, ctid);
     // Z[b][tile][y][KC]: the tile's output is one contiguous block; streamed past L2
     const size_t zo = b * z_bs + (size_t)tile * H * KC;
     for (int e = tid; e < KC * H; e += KC * NTC) {
