@@ -166,195 +166,335 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const bf16_t* __restrict__ A, i
       }
 }
 
-// 3x3 convolution as an implicit product, for Cin % 16 == 0: C[m][n] = relu(sum_k col(m, k) B[n][k] + bias[n])
-// with col(m, k) = bf16(X[b][oy * stride + ky - 1][ox * stride + kx - 1][c]) for m = (b, oy, ox),
-// k = (ky * 3 + kx) * Cin + c (0 outside the image), read from the FP32 activations instead of a
-// materialised im2col matrix. Every 16-wide k step lies in one tap, so each lane's four values are one
-// 16-byte load. Block and wave layout as k_gemm_nt; K = 9 Cin; Si^2 Cin < 2^31 (checked by the caller).
-__device__ inline short4_t bf16x4(float4 v) {
-  return short4_t{(short)f2bf(v.x), (short)f2bf(v.y), (short)f2bf(v.z), (short)f2bf(v.w)};
-}
-template <int WN>
-__global__ __launch_bounds__(256) void k_conv_fwd(const float* __restrict__ X, int Si, int Cin, int So, int stride,
-                                                  const bf16_t* __restrict__ B, int ldb, float* __restrict__ C,
-                                                  long long ldc, long long M, int N, const float* __restrict__ bias) {
+// The same product for M >= 128 and N >= 128 on a 128 x 128 block tile: each wave owns 64 x 64 as 4 x 4
+// MFMA tiles (every loaded operand feeds four MFMAs) and steps k by 32 on the 16x16x32 bf16 MFMA, a
+// lane's eight values of a row being one 16-byte load (A and B use the same lane-to-k assignment, so
+// the product pairs them whatever order the hardware sums its 32 k in); a 16-wide k remainder takes one
+// 16x16x16 step. K, kchunk multiples of 16; lda, ldb multiples of 8.
+typedef short short8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+template <bool BIAS, bool RELU>
+__global__ __launch_bounds__(256) void k_gemm_nt4(const bf16_t* __restrict__ A, int lda, const bf16_t* __restrict__ B,
+                                                  int ldb, float* __restrict__ C, long long ldc, long long M, int N,
+                                                  int K, int kchunk, long long split_stride,
+                                                  const float* __restrict__ bias) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const long long m0 = (long long)blockIdx.x * (128 / WN) + (wave / WN) * 32;
-  const int n0 = blockIdx.y * (32 * WN) + (wave % WN) * 32;
-  const int r = lane & 15, kq = (lane >> 4) * 4;
-  const int K = 9 * Cin, SoSo = So * So;
-  const float* xp[2];
-  int y0[2], x0[2];
-  bool aok[2];
-  const bf16_t* bp[2];
-  bool bok[2];
+  const long long m0 = (long long)blockIdx.x * 128 + (wave >> 1) * 64;
+  const int n0 = blockIdx.y * 128 + (wave & 1) * 64;
+  const int r = lane & 15, kg = lane >> 4;
+  const int k_lo = blockIdx.z * kchunk, k_hi = min(K, k_lo + kchunk);
+  const bf16_t* ap[4];
+  const bf16_t* bp[4];
+  bool aok[4], bok[4];
 #pragma unroll
-  for (int i = 0; i < 2; ++i) {
+  for (int i = 0; i < 4; ++i) {
     const long long m = m0 + i * 16 + r;
     aok[i] = m < M;
-    const int mm = aok[i] ? (int)m : 0;
-    const int b = mm / SoSo, pix = mm - b * SoSo;
-    const int oy = pix / So, ox = pix - oy * So;
-    y0[i] = oy * stride - 1;
-    x0[i] = ox * stride - 1;
-    xp[i] = X + (size_t)b * Si * Si * Cin + kq;
+    ap[i] = A + (aok[i] ? m : 0) * (long long)lda;
     const int n = n0 + i * 16 + r;
     bok[i] = n < N;
-    bp[i] = B + (size_t)(bok[i] ? n : 0) * ldb + kq;
+    bp[i] = B + (size_t)(bok[i] ? n : 0) * ldb;
   }
-  float4_t acc[2][2];
+  float4_t acc[4][4];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
-  const short4_t zero = {0, 0, 0, 0};
-  // (ky, kx, c0): the tap and first channel of k step k; KU steps per iteration, loads first (K / 16 =
-  // 9 Cin / 16 is a multiple of 9, the tail loop takes the remainder)
-  constexpr int KU = 4;
-  int ky = 0, kx = 0, c0 = 0;
-  auto advance = [&]() {
-    c0 += 16;
-    if (c0 == Cin) {
-      c0 = 0;
-      if (++kx == 3) {
-        kx = 0;
-        ++ky;
+    for (int j = 0; j < 4; ++j) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
+  const short8_t zero8 = {0, 0, 0, 0, 0, 0, 0, 0};
+  // two 32-wide k steps per iteration, their loads issued before their MFMAs
+  int k = k_lo;
+  for (; k + 64 <= k_hi; k += 64) {
+    short8_t a[2][4], b[2][4];
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int kk = k + 32 * u + 8 * kg;
+        a[u][i] = aok[i] ? *reinterpret_cast<const short8_t*>(ap[i] + kk) : zero8;
+        b[u][i] = bok[i] ? *reinterpret_cast<const short8_t*>(bp[i] + kk) : zero8;
       }
-    }
-  };
-  auto load_a = [&](int i) -> short4_t {
-    const int iy = y0[i] + ky, ix = x0[i] + kx;
-    const bool ok = aok[i] && iy >= 0 && iy < Si && ix >= 0 && ix < Si;
-    return ok ? bf16x4(*reinterpret_cast<const float4*>(xp[i] + (iy * Si + ix) * Cin + c0)) : zero;
-  };
-  int k = 0;
-  for (; k + 16 * KU <= K; k += 16 * KU) {
-    float4 araw[KU][2];
-    bool aval[KU][2];
-    short4_t b[KU][2];
 #pragma unroll
-    for (int u = 0; u < KU; ++u) {
+    for (int u = 0; u < 2; ++u)
 #pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        const int iy = y0[i] + ky, ix = x0[i] + kx;
-        aval[u][i] = aok[i] && iy >= 0 && iy < Si && ix >= 0 && ix < Si;
-        araw[u][i] = aval[u][i] ? *reinterpret_cast<const float4*>(xp[i] + (iy * Si + ix) * Cin + c0)
-                                : float4{0.f, 0.f, 0.f, 0.f};
-        b[u][i] = bok[i] ? *reinterpret_cast<const short4_t*>(bp[i] + k + 16 * u) : zero;
-      }
-      advance();
-    }
+      for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int u = 0; u < KU; ++u) {
-      short4_t a[2];
-#pragma unroll
-      for (int i = 0; i < 2; ++i) a[i] = aval[u][i] ? bf16x4(araw[u][i]) : zero;
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[u][j], acc[i][j], 0, 0, 0);
-    }
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a[u][i]),
+                                                              __builtin_bit_cast(bf16x8_t, b[u][j]), acc[i][j], 0, 0, 0);
   }
-  for (; k < K; k += 16) {
-    short4_t a[2], b[2];
+  for (; k + 32 <= k_hi; k += 32) {
+    short8_t a[4], b[4];
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      a[i] = load_a(i);
-      b[i] = bok[i] ? *reinterpret_cast<const short4_t*>(bp[i] + k) : zero;
+    for (int i = 0; i < 4; ++i) {
+      a[i] = aok[i] ? *reinterpret_cast<const short8_t*>(ap[i] + k + 8 * kg) : zero8;
+      b[i] = bok[i] ? *reinterpret_cast<const short8_t*>(bp[i] + k + 8 * kg) : zero8;
     }
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
-    advance();
+      for (int j = 0; j < 4; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a[i]),
+                                                            __builtin_bit_cast(bf16x8_t, b[j]), acc[i][j], 0, 0, 0);
   }
+  if (k < k_hi) {  // 16 remaining
+    const short4_t zero = {0, 0, 0, 0};
+    short4_t a[4], b[4];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < 4; ++i) {
+      a[i] = aok[i] ? *reinterpret_cast<const short4_t*>(ap[i] + k + 4 * kg) : zero;
+      b[i] = bok[i] ? *reinterpret_cast<const short4_t*>(bp[i] + k + 4 * kg) : zero;
+    }
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+  }
+  float* Cz = C + (long long)blockIdx.z * split_stride;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int n = n0 + j * 16 + r;
+      if (n >= N) continue;
+      const float bn = BIAS ? bias[n] : 0.f;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const long long m = m0 + i * 16 + kq + e;
-        const int n = n0 + j * 16 + r;
-        if (m < M && n < N) C[m * ldc + n] = fmaxf(acc[i][j][e] + bias[n], 0.f);
+        const long long m = m0 + i * 16 + 4 * kg + e;
+        if (m < M) {
+          float v = acc[i][j][e] + bn;
+          if (RELU) v = fmaxf(v, 0.f);
+          Cz[m * ldc + n] = v;
+        }
       }
+    }
 }
 
-// Data gradient of a 3x3 convolution as an implicit product, for Cout % 16 == 0:
-// dX[m][c] = sum over (tap, co) of bf16(dZ[(b, oy, ox)][co]) Wd[c][tap Cout + co], m = (b, iy, ix),
-// over the taps whose output position reads the pixel (oy stride = iy + 1 - ky, ox likewise, in
-// range): the product form of dcol = dZ W followed by col2im, without the dcol matrix. Each 16-wide
-// k step lies in one tap; a lane's four dZ values are one 16-byte load. Block and wave layout as
-// k_gemm_nt; B: Wd rows of K = 9 Cout bf16 (k_weights_dgrad).
-template <int WN>
-__global__ __launch_bounds__(256) void k_conv_dgrad(const float* __restrict__ dZ, int Si, int Cout, int So, int stride,
-                                                    const bf16_t* __restrict__ B, float* __restrict__ dX, long long M,
-                                                    int N) {
+// 3x3 convolution kernels as implicit products on MFMA 16x16x16 bf16 tiles. A wave owns 64 pixels (four
+// 16-row tiles) x 16 NT channels, so each loaded operand feeds NT or four MFMAs; a block is four waves
+// along the pixels. Every 16-wide k step lies in one tap, and a lane's four values of it are four
+// consecutive channels of one pixel: one 16-byte load, rounded to bf16 in registers.
+__device__ inline short4_t bf16x4_hw(float a, float b, float c, float d) {
+  typedef __bf16 bf16x4v __attribute__((ext_vector_type(4)));
+  const bf16x4v v = {(__bf16)a, (__bf16)b, (__bf16)c, (__bf16)d};
+  return __builtin_bit_cast(short4_t, v);
+}
+__device__ inline short4_t bf16x4_hw(float4 v) { return bf16x4_hw(v.x, v.y, v.z, v.w); }
+
+// Forward, for Cin % 16 == 0: Y[m][n] = relu(sum_k col(m, k) B[n][k] + bias[n]) with
+// col(m, k) = bf16(X[b][oy * stride + ky - 1][ox * stride + kx - 1][c]) for m = (b, oy, ox),
+// k = (ky * 3 + kx) * Cin + c (0 outside the image), read from the FP32 activations instead of a
+// materialised im2col matrix. B: bf16 weights, rows of ldb >= 9 Cin. Si^2 Cin < 2^31, M < 2^31.
+template <int NT>
+__global__ __launch_bounds__(256) void k_conv_fwd(const float* __restrict__ X, int Si, int Cin, int So, int stride,
+                                                  const bf16_t* __restrict__ B, int ldb, float* __restrict__ Y,
+                                                  int M, int N, const float* __restrict__ bias) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const long long m0 = (long long)blockIdx.x * (128 / WN) + (wave / WN) * 32;
-  const int n0 = blockIdx.y * (32 * WN) + (wave % WN) * 32;
+  const int m0 = blockIdx.x * 256 + wave * 64, n0 = blockIdx.y * 16 * NT;
   const int r = lane & 15, kq = (lane >> 4) * 4;
-  const int K = 9 * Cout, SiSi = Si * Si, ldb = K;
-  const float* zp[2];
-  int iy1[2], ix1[2];
-  bool aok[2];
-  const bf16_t* bp[2];
-  bool bok[2];
+  const int SoSo = So * So;
+  int y0[4], x0[4];
+  const float* xb[4];
 #pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    const long long m = m0 + i * 16 + r;
-    aok[i] = m < M;
-    const int mm = aok[i] ? (int)m : 0;
-    const int b = mm / SiSi, pix = mm - b * SiSi;
-    const int iy = pix / Si, ix = pix - iy * Si;
-    iy1[i] = iy + 1;
-    ix1[i] = ix + 1;
-    zp[i] = dZ + (size_t)b * So * So * Cout + kq;
-    const int n = n0 + i * 16 + r;
-    bok[i] = n < N;
-    bp[i] = B + (size_t)(bok[i] ? n : 0) * ldb + kq;
+  for (int i = 0; i < 4; ++i) {
+    const int m = m0 + 16 * i + r;
+    const int mm = m < M ? m : 0;
+    const int b = mm / SoSo, pix = mm - b * SoSo;
+    const int oy = pix / So, ox = pix - oy * So;
+    // rows past M get a start that no tap brings into the image
+    y0[i] = m < M ? oy * stride - 1 : -4;
+    x0[i] = ox * stride - 1;
+    xb[i] = X + (size_t)b * Si * Si * Cin + kq;
   }
-  float4_t acc[2][2];
+  const bf16_t* bp[NT];
+  bool bok[NT];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int j = 0; j < NT; ++j) {
+    const int n = n0 + 16 * j + r;
+    bok[j] = n < N;
+    bp[j] = B + (size_t)(bok[j] ? n : 0) * ldb + kq;
+  }
+  float4_t acc[4][NT];
 #pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
   const short4_t zero = {0, 0, 0, 0};
-  int ky = 0, kx = 0, c0 = 0;
-  for (int k = 0; k < K; k += 16) {
-    short4_t a[2], b[2];
+  for (int tap = 0; tap < 9; ++tap) {
+    const int ky = tap / 3, kx = tap - 3 * (tap / 3);
+    const float* ap[4];
+    bool aok[4];
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      const int ty = iy1[i] - ky, tx = ix1[i] - kx;
-      const int oy = ty / stride, ox = tx / stride;
-      const bool ok = aok[i] && ty >= 0 && tx >= 0 && oy * stride == ty && ox * stride == tx && oy < So && ox < So;
-      a[i] = ok ? bf16x4(*reinterpret_cast<const float4*>(zp[i] + (oy * So + ox) * Cout + c0)) : zero;
-      b[i] = bok[i] ? *reinterpret_cast<const short4_t*>(bp[i] + k) : zero;
+    for (int i = 0; i < 4; ++i) {
+      const int iy = y0[i] + ky, ix = x0[i] + kx;
+      aok[i] = iy >= 0 && iy < Si && ix >= 0 && ix < Si;
+      ap[i] = xb[i] + (aok[i] ? (iy * Si + ix) * Cin : 0);
     }
+    const int kt = tap * Cin;
+    // two channel steps per iteration, their loads issued before their MFMAs
+    int c0 = 0;
+    for (; c0 + 32 <= Cin; c0 += 32) {
+      float4 araw[2][4];
+      short4_t bq[2][NT];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+      for (int u = 0; u < 2; ++u) {
 #pragma unroll
-      for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
-    c0 += 16;
-    if (c0 == Cout) {
-      c0 = 0;
-      if (++kx == 3) {
-        kx = 0;
-        ++ky;
+        for (int i = 0; i < 4; ++i)
+          araw[u][i] = aok[i] ? *reinterpret_cast<const float4*>(ap[i] + c0 + 16 * u) : float4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int j = 0; j < NT; ++j)
+          bq[u][j] = bok[j] ? *reinterpret_cast<const short4_t*>(bp[j] + kt + c0 + 16 * u) : zero;
+      }
+#pragma unroll
+      for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const short4_t a = bf16x4_hw(araw[u][i]);
+#pragma unroll
+          for (int j = 0; j < NT; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bq[u][j], acc[i][j], 0, 0, 0);
+        }
+    }
+    if (c0 < Cin) {
+      short4_t bq[NT];
+#pragma unroll
+      for (int j = 0; j < NT; ++j) bq[j] = bok[j] ? *reinterpret_cast<const short4_t*>(bp[j] + kt + c0) : zero;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const short4_t a = aok[i] ? bf16x4_hw(*reinterpret_cast<const float4*>(ap[i] + c0)) : zero;
+#pragma unroll
+        for (int j = 0; j < NT; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bq[j], acc[i][j], 0, 0, 0);
       }
     }
   }
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < NT; ++j) {
+      const int n = n0 + 16 * j + r;
+      if (n >= N) continue;
+      const float bn = bias[n];
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const long long m = m0 + i * 16 + kq + e;
-        const int n = n0 + j * 16 + r;
-        if (m < M && n < N) dX[m * N + n] = acc[i][j][e];
+        const int m = m0 + 16 * i + kq + e;
+        if (m < M) Y[(size_t)m * N + n] = fmaxf(acc[i][j][e] + bn, 0.f);
       }
+    }
+}
+
+// Data gradient, for Cout % 16 == 0: dX[m][c] = sum over (tap, co) of bf16(dZ[(b, oy, ox)][co]) Wd[c][tap Cout + co]
+// for the input pixel m = (b, iy, ix), over the taps whose output position reads it (oy stride =
+// iy + 1 - ky, ox likewise, in range): dcol = dZ W followed by col2im, without the dcol matrix.
+// At stride 2 the pixels run in four parity classes (blockIdx.z = 2 (iy % 2) + ix % 2): within a class
+// the taps that reach a pixel are the same for every pixel (ky = 1 for even iy, ky = 0 and 2 for odd
+// iy; kx likewise), so a wave runs 1, 2 or 4 of the 9 taps and no product of a tap that cannot reach
+// it. B: Wd rows of K = 9 Cout bf16 (k_weights_dgrad). Mc = the pixels of one class (all pixels at
+// stride 1); Si even at stride 2. The layer's input is the previous layer's ReLU output A, so the
+// epilogue applies that ReLU's backward as well: dX = 0 where A <= 0.
+template <int NT>
+__global__ __launch_bounds__(256) void k_conv_dgrad(const float* __restrict__ dZ, int Si, int Cout, int So, int stride,
+                                                    const bf16_t* __restrict__ B, const float* __restrict__ A,
+                                                    float* __restrict__ dX, int Mc, int N) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m0 = blockIdx.x * 256 + wave * 64, n0 = blockIdx.y * 16 * NT;
+  const int r = lane & 15, kq = (lane >> 4) * 4;
+  const int ldb = 9 * Cout;
+  // class geometry: pixel (b, q, p) of the class's Sq x Sq grid sits at (q step + py, p step + px)
+  const int py = stride == 2 ? (int)(blockIdx.z >> 1) : 0, px = stride == 2 ? (int)(blockIdx.z & 1) : 0;
+  const int Sq = Si / stride, SqSq = Sq * Sq;
+  const int ky0 = stride == 2 ? 1 - py : 0, kyn = stride == 2 ? 2 + py : 3, kys = stride == 2 ? 2 : 1;
+  const int kx0 = stride == 2 ? 1 - px : 0, kxn = stride == 2 ? 2 + px : 3, kxs = stride == 2 ? 2 : 1;
+  int iy1[4], ix1[4], xo[4];
+  const float* zb[4];
+  bool mok[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int m = m0 + 16 * i + r;
+    mok[i] = m < Mc;
+    const int mm = mok[i] ? m : 0;
+    const int b = mm / SqSq, pix = mm - b * SqSq;
+    const int q = pix / Sq, p = pix - q * Sq;
+    const int iy = q * stride + py, ix = p * stride + px;
+    iy1[i] = iy + 1;
+    ix1[i] = ix + 1;
+    zb[i] = dZ + (size_t)b * So * So * Cout + kq;
+    xo[i] = mok[i] ? ((b * Si + iy) * Si + ix) * N : -1;
+  }
+  const bf16_t* bp[NT];
+  bool bok[NT];
+#pragma unroll
+  for (int j = 0; j < NT; ++j) {
+    const int n = n0 + 16 * j + r;
+    bok[j] = n < N;
+    bp[j] = B + (size_t)(bok[j] ? n : 0) * ldb + kq;
+  }
+  float4_t acc[4][NT];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
+  const short4_t zero = {0, 0, 0, 0};
+  for (int ky = ky0; ky < kyn; ky += kys)
+    for (int kx = kx0; kx < kxn; kx += kxs) {
+      const float* ap[4];
+      bool aok[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        // (iy + 1 - ky) is a multiple of the stride for every tap of the class
+        const int ty = iy1[i] - ky, tx = ix1[i] - kx;
+        const int oy = stride == 2 ? ty >> 1 : ty, ox = stride == 2 ? tx >> 1 : tx;
+        aok[i] = mok[i] && ty >= 0 && tx >= 0 && oy < So && ox < So;
+        ap[i] = zb[i] + (aok[i] ? (oy * So + ox) * Cout : 0);
+      }
+      const int kt = (ky * 3 + kx) * Cout;
+      int c0 = 0;
+      for (; c0 + 32 <= Cout; c0 += 32) {
+        float4 araw[2][4];
+        short4_t bq[2][NT];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            araw[u][i] = aok[i] ? *reinterpret_cast<const float4*>(ap[i] + c0 + 16 * u) : float4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int j = 0; j < NT; ++j)
+            bq[u][j] = bok[j] ? *reinterpret_cast<const short4_t*>(bp[j] + kt + c0 + 16 * u) : zero;
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const short4_t a = bf16x4_hw(araw[u][i]);
+#pragma unroll
+            for (int j = 0; j < NT; ++j)
+              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bq[u][j], acc[i][j], 0, 0, 0);
+          }
+      }
+      if (c0 < Cout) {
+        short4_t bq[NT];
+#pragma unroll
+        for (int j = 0; j < NT; ++j) bq[j] = bok[j] ? *reinterpret_cast<const short4_t*>(bp[j] + kt + c0) : zero;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const short4_t a = aok[i] ? bf16x4_hw(*reinterpret_cast<const float4*>(ap[i] + c0)) : zero;
+#pragma unroll
+          for (int j = 0; j < NT; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bq[j], acc[i][j], 0, 0, 0);
+        }
+      }
+    }
+  // D rows are the pixels m0 + 16 i + kq + e: their output offsets (-1 past Mc) come from lane kq + e,
+  // which decoded that pixel
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int o = __shfl(xo[i], kq + e);
+#pragma unroll
+      for (int j = 0; j < NT; ++j) {
+        const int n = n0 + 16 * j + r;
+        if (o >= 0 && n < N) dX[(size_t)o + n] = A[(size_t)o + n] > 0.f ? acc[i][j][e] : 0.f;
+      }
+    }
 }
 
 // Wd[c][tap Cout + co] = bf16(W[co][tap Cin + c]): the weights as the data gradient's B operand
@@ -365,6 +505,117 @@ __global__ void k_weights_dgrad(const float* __restrict__ W, int Cin, int Cout, 
     const int tap = rr / Cout, co = rr - tap * Cout;
     Wd[e] = f2bf(W[(size_t)co * 9 * Cin + tap * Cin + c]);
   }
+}
+
+// Weight gradient of a 3x3 convolution as an implicit product, for Cin % 16 == 0, Cout % 16 == 0 and
+// So % 16 == 0: parts[z][co][tap Cin + c] = sum over the output pixels p = (b, oy, ox) of split z of
+// bf16(dZ[p][co]) bf16(X[b][oy stride + ky - 1][ox stride + kx - 1][c]) (0 outside the image), with
+// tap = 3 ky + kx: dW = dZ^T col without the im2col matrix or either transpose. The product's k runs
+// over pixels, in 16-pixel segments of one output row: a lane's four pixels are consecutive, so its
+// four dZ values lie Cout floats apart and its four X values stride Cin floats apart, at constant
+// offsets from one address (only the image's left and right edges mask a pixel). A block covers one
+// tap, 16 MT output and 16 NT input channels of one split's rows; its four waves take every fourth
+// segment and add their tiles through LDS in wave order (results do not depend on scheduling).
+// Block b runs on XCD b % 8: the blocks of one split share an XCD, so the tiles that read the same
+// pixels meet in one L2. nsplit % 8 == 0; ntiles = 9 (Cin / 16 NT) (Cout / 16 MT).
+template <int MT, int NT>
+__global__ __launch_bounds__(256) void k_conv_wgrad(const float* __restrict__ X, int Si, int Cin, int So, int stride,
+                                                    const float* __restrict__ dZ, int Cout, int rows, int nsplit,
+                                                    int ntiles, float* __restrict__ parts) {
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int xcd = blockIdx.x & 7, jx = blockIdx.x >> 3;
+  const int z = xcd * (nsplit >> 3) + jx / ntiles;
+  int t = jx % ntiles;
+  const int nc = Cin / (16 * NT);
+  const int cm = t / (9 * nc);
+  t -= cm * 9 * nc;
+  const int tap = t / nc, cc = t - tap * nc;
+  const int ky = tap / 3, kx = tap - 3 * ky;
+  const int co0 = cm * 16 * MT, ci0 = cc * 16 * NT;
+  const int r = lane & 15, pq = (lane >> 4) * 4;
+  const int spr = So >> 4;
+  const int g_lo = (int)((long long)rows * z / nsplit) * spr, g_hi = (int)((long long)rows * (z + 1) / nsplit) * spr;
+  float4_t acc[MT][NT];
+#pragma unroll
+  for (int i = 0; i < MT; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
+  // loads of segment g into raw registers: dZ (MT x 4) and X (NT x 4), masked X values as 0
+  auto load = [&](int g, float (&za)[MT][4], float (&xa)[NT][4]) {
+    const int row = g / spr;
+    const int ox = (g - row * spr) * 16 + pq;
+    const int b = row / So, oy = row - b * So;
+    const float* zp = dZ + ((size_t)row * So + ox) * Cout + co0 + r;
+#pragma unroll
+    for (int i = 0; i < MT; ++i)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) za[i][e] = zp[e * Cout + 16 * i];
+    const int iy = oy * stride + ky - 1;
+    const int ix0 = ox * stride + kx - 1;
+    if (iy >= 0 && iy < Si) {
+      const long long base = (((long long)b * Si + iy) * Si + ix0) * Cin + ci0 + r;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int ix = ix0 + e * stride;
+        const bool ok = ix >= 0 && ix < Si;
+        const float* xp = X + (ok ? base + (long long)e * stride * Cin : 0);
+#pragma unroll
+        for (int j = 0; j < NT; ++j) xa[j][e] = ok ? xp[16 * j] : 0.f;
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < NT; ++j)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) xa[j][e] = 0.f;
+    }
+  };
+  auto mfma = [&](const float (&za)[MT][4], const float (&xa)[NT][4]) {
+    short4_t a[MT], bq[NT];
+#pragma unroll
+    for (int i = 0; i < MT; ++i) a[i] = bf16x4_hw(za[i][0], za[i][1], za[i][2], za[i][3]);
+#pragma unroll
+    for (int j = 0; j < NT; ++j) bq[j] = bf16x4_hw(xa[j][0], xa[j][1], xa[j][2], xa[j][3]);
+#pragma unroll
+    for (int i = 0; i < MT; ++i)
+#pragma unroll
+      for (int j = 0; j < NT; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], bq[j], acc[i][j], 0, 0, 0);
+  };
+  int g = g_lo + wave;
+  for (; g + 4 < g_hi; g += 8) {
+    float za0[MT][4], xa0[NT][4], za1[MT][4], xa1[NT][4];
+    load(g, za0, xa0);
+    load(g + 4, za1, xa1);
+    mfma(za0, xa0);
+    mfma(za1, xa1);
+  }
+  if (g < g_hi) {
+    float za0[MT][4], xa0[NT][4];
+    load(g, za0, xa0);
+    mfma(za0, xa0);
+  }
+  // waves 1..3 hand their tiles to wave 0, which adds them in wave order and writes the partial
+  __shared__ float4_t red[3][MT * NT][64];
+  if (wave > 0) {
+#pragma unroll
+    for (int i = 0; i < MT; ++i)
+#pragma unroll
+      for (int j = 0; j < NT; ++j) red[wave - 1][i * NT + j][lane] = acc[i][j];
+  }
+  __syncthreads();
+  if (wave != 0) return;
+  const int K = 9 * Cin;
+  float* out = parts + (size_t)z * Cout * K + (size_t)tap * Cin;
+#pragma unroll
+  for (int i = 0; i < MT; ++i)
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      float4_t s = acc[i][j];
+#pragma unroll
+      for (int w = 0; w < 3; ++w) s += red[w][i * NT + j][lane];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) out[(size_t)(co0 + 16 * i + pq + e) * K + ci0 + 16 * j + r] = s[e];
+    }
 }
 
 // out[m][n] = sum over z of parts[z][m][n] (+ bias, ReLU): a block covers 32 consecutive outputs
@@ -774,11 +1025,58 @@ void gemm_wn(hipStream_t st, Buf<float>& parts, const bf16_t* A, int lda, const 
   check_launch("k_reduce_splits");
 }
 
-// Block shape by operand size: 128 x 32 when N <= 32, 32 x 128 when M <= 32, else 64 x 64.
+// C = A . B^T on k_gemm_nt4's 128 x 128 blocks, split over K into fixed-order partials when the tile
+// grid is too small to fill the GPU (about 2048 blocks, at least 512 of K per block).
+void gemm4(hipStream_t st, Buf<float>& parts, const bf16_t* A, int lda, const bf16_t* B, int ldb, float* C,
+           long long ldc, long long M, int N, int K, const float* bias, bool relu) {
+  const long long gm = (M + 127) / 128;
+  const int gn = (N + 127) / 128;
+  CNN_REQUIRE(gm < (1LL << 31) && gn <= 65535, TM_ERR_INVALID, "gemm: output too large");
+  const long long tiles = gm * gn;
+  int nsplit = 1;
+  if (tiles < 1024 && K >= 1024) {
+    nsplit = (int)std::min<long long>(K / 512, std::max<long long>(1, 2048 / tiles));
+    const long long cap = (64LL << 20) / std::max<long long>(1, M * N);
+    nsplit = (int)std::max<long long>(1, std::min<long long>(nsplit, cap));
+  }
+  if (nsplit <= 1) {
+    const dim3 g((unsigned)gm, gn, 1);
+    if (bias && relu)
+      hipLaunchKernelGGL((k_gemm_nt4<true, true>), g, dim3(256), 0, st, A, lda, B, ldb, C, ldc, M, N, K, K, 0LL, bias);
+    else if (bias)
+      hipLaunchKernelGGL((k_gemm_nt4<true, false>), g, dim3(256), 0, st, A, lda, B, ldb, C, ldc, M, N, K, K, 0LL, bias);
+    else
+      hipLaunchKernelGGL((k_gemm_nt4<false, false>), g, dim3(256), 0, st, A, lda, B, ldb, C, ldc, M, N, K, K, 0LL,
+                         nullptr);
+    check_launch("k_gemm_nt4");
+    return;
+  }
+  const int kchunk = (K / 16 + nsplit - 1) / nsplit * 16;
+  nsplit = (K + kchunk - 1) / kchunk;
+  const long long ss = M * N;
+  parts.ensure((size_t)ss * nsplit);
+  hipLaunchKernelGGL((k_gemm_nt4<false, false>), dim3((unsigned)gm, gn, nsplit), dim3(256), 0, st, A, lda, B, ldb,
+                     parts.p, (long long)N, M, N, K, kchunk, ss, nullptr);
+  check_launch("k_gemm_nt4 split");
+  const unsigned g = grid1((M * N + 31) / 32, 1, 1 << 16);
+  if (bias && relu)
+    hipLaunchKernelGGL((k_reduce_splits<true, true>), dim3(g), dim3(256), 0, st, parts.p, nsplit, M, N, ss, C, ldc, bias);
+  else if (bias)
+    hipLaunchKernelGGL((k_reduce_splits<true, false>), dim3(g), dim3(256), 0, st, parts.p, nsplit, M, N, ss, C, ldc, bias);
+  else
+    hipLaunchKernelGGL((k_reduce_splits<false, false>), dim3(g), dim3(256), 0, st, parts.p, nsplit, M, N, ss, C, ldc,
+                       nullptr);
+  check_launch("k_reduce_splits");
+}
+
+// Block shape by operand size: 128 x 128 (k_gemm_nt4) when M and N are both 128 or more and the rows
+// allow 16-byte loads; else 128 x 32 when N <= 32, 32 x 128 when M <= 32, 64 x 64 otherwise.
 void gemm(hipStream_t st, Buf<float>& parts, const bf16_t* A, int lda, const bf16_t* B, int ldb, float* C,
           long long ldc, long long M, int N, int K, const float* bias, bool relu) {
   CNN_REQUIRE(K % 16 == 0 && lda % 4 == 0 && ldb % 4 == 0, TM_ERR_INVALID, "gemm: K, lda, ldb not padded");
-  if (N <= 32)
+  if (M >= 128 && N >= 128 && lda % 8 == 0 && ldb % 8 == 0)
+    gemm4(st, parts, A, lda, B, ldb, C, ldc, M, N, K, bias, relu);
+  else if (N <= 32)
     gemm_wn<1>(st, parts, A, lda, B, ldb, C, ldc, M, N, K, bias, relu);
   else if (M <= 32)
     gemm_wn<4>(st, parts, A, lda, B, ldb, C, ldc, M, N, K, bias, relu);
@@ -786,26 +1084,51 @@ void gemm(hipStream_t st, Buf<float>& parts, const bf16_t* A, int lda, const bf1
     gemm_wn<2>(st, parts, A, lda, B, ldb, C, ldc, M, N, K, bias, relu);
 }
 
-template <int WN>
-void conv_fwd_wn(hipStream_t st, const float* X, int Si, int Cin, int So, int stride, const bf16_t* W, int ldw, float* Y,
-                 long long M, int N, const float* bias) {
-  constexpr int BM = 128 / WN, BN = 32 * WN;
-  const long long gm = (M + BM - 1) / BM;
-  const int gn = (N + BN - 1) / BN;
-  CNN_REQUIRE(gm < (1LL << 31) && gn <= 65535, TM_ERR_INVALID, "conv: output too large");
-  hipLaunchKernelGGL((k_conv_fwd<WN>), dim3((unsigned)gm, gn), dim3(256), 0, st, X, Si, Cin, So, stride, W, ldw, Y,
-                     (long long)N, M, N, bias);
+// Output channels per wave tile: 16 NT, NT = 4 from 64 channels up.
+inline int conv_nt(int n) { return n >= 64 ? 4 : (n + 15) / 16; }
+template <int NT>
+void conv_fwd_nt(hipStream_t st, const float* X, int Si, int Cin, int So, int stride, const bf16_t* W, int ldw, float* Y,
+                 int M, int N, const float* bias) {
+  const dim3 g((unsigned)((M + 255) / 256), (unsigned)((N + 16 * NT - 1) / (16 * NT)));
+  hipLaunchKernelGGL((k_conv_fwd<NT>), g, dim3(256), 0, st, X, Si, Cin, So, stride, W, ldw, Y, M, N, bias);
   check_launch("k_conv_fwd");
 }
 // Forward 3x3 convolution + bias + ReLU without an im2col matrix (Cin % 16 == 0).
 void conv_fwd(hipStream_t st, const float* X, int Si, int Cin, int So, int stride, const bf16_t* W, int ldw, float* Y,
               long long M, int N, const float* bias) {
   CNN_REQUIRE(Cin % 16 == 0 && ldw % 4 == 0, TM_ERR_INVALID, "conv: Cin must be a multiple of 16");
-  CNN_REQUIRE((long long)Si * Si * Cin < (1LL << 31) && M < (1LL << 31), TM_ERR_INVALID, "conv: image too large");
-  if (N <= 32)
-    conv_fwd_wn<1>(st, X, Si, Cin, So, stride, W, ldw, Y, M, N, bias);
-  else
-    conv_fwd_wn<2>(st, X, Si, Cin, So, stride, W, ldw, Y, M, N, bias);
+  CNN_REQUIRE((long long)Si * Si * Cin < (1LL << 31) && M * N < (1LL << 31), TM_ERR_INVALID, "conv: image too large");
+  switch (conv_nt(N)) {
+    case 1: conv_fwd_nt<1>(st, X, Si, Cin, So, stride, W, ldw, Y, (int)M, N, bias); break;
+    case 2: conv_fwd_nt<2>(st, X, Si, Cin, So, stride, W, ldw, Y, (int)M, N, bias); break;
+    case 3: conv_fwd_nt<3>(st, X, Si, Cin, So, stride, W, ldw, Y, (int)M, N, bias); break;
+    default: conv_fwd_nt<4>(st, X, Si, Cin, So, stride, W, ldw, Y, (int)M, N, bias); break;
+  }
+}
+
+// Data gradient of a 3x3 layer into dX[B][Si][Si][Cin] (k_conv_dgrad; Cout % 16 == 0), the weights
+// already in Wd (k_weights_dgrad), masked by the ReLU that produced the layer's input A. At stride 2
+// one grid layer per pixel-parity class.
+template <int NT>
+void conv_dgrad_nt(hipStream_t st, const float* dZ, const Layer& L, int batch, const bf16_t* Wd, const float* A,
+                   float* dX) {
+  const int Sq = L.si / L.stride;
+  const int Mc = batch * Sq * Sq;
+  const dim3 g((unsigned)((Mc + 255) / 256), (unsigned)((L.cin + 16 * NT - 1) / (16 * NT)), L.stride == 2 ? 4 : 1);
+  hipLaunchKernelGGL((k_conv_dgrad<NT>), g, dim3(256), 0, st, dZ, L.si, L.cout, L.so, L.stride, Wd, A, dX, Mc, L.cin);
+  check_launch("k_conv_dgrad");
+}
+void conv_dgrad(hipStream_t st, const float* dZ, const Layer& L, int batch, const bf16_t* Wd, const float* A,
+                float* dX) {
+  CNN_REQUIRE(L.cout % 16 == 0 && L.si % L.stride == 0, TM_ERR_INVALID, "dgrad: Cout must be a multiple of 16");
+  CNN_REQUIRE((long long)L.so * L.so * L.cout < (1LL << 31) && (long long)batch * L.si * L.si * L.cin < (1LL << 31),
+              TM_ERR_INVALID, "dgrad: layer too large");
+  switch (conv_nt(L.cin)) {
+    case 1: conv_dgrad_nt<1>(st, dZ, L, batch, Wd, A, dX); break;
+    case 2: conv_dgrad_nt<2>(st, dZ, L, batch, Wd, A, dX); break;
+    case 3: conv_dgrad_nt<3>(st, dZ, L, batch, Wd, A, dX); break;
+    default: conv_dgrad_nt<4>(st, dZ, L, batch, Wd, A, dX); break;
+  }
 }
 
 void transpose_bf16(hipStream_t st, const float* X, long long R, int C, long long ldx, bf16_t* T, long long ldt, int Cp,
@@ -834,6 +1157,59 @@ void colsum(hipStream_t st, Buf<float>& part, const float* X, long long R, int C
   check_launch("k_colsum_part");
   hipLaunchKernelGGL(k_colsum_final, dim3(C), dim3(256), 0, st, part.p, np, C, out);
   check_launch("k_colsum_final");
+}
+
+// dW[cout][9 cin] of a 3x3 layer straight from its input X and output gradient dZ (k_conv_wgrad), the
+// pixel splits' partials added in split order: about 2048 blocks, a split holding at least one
+// output row, the partials at most 64 M floats.
+template <int MT, int NT>
+void conv_wgrad_t(hipStream_t st, Buf<float>& parts, const float* X, const Layer& L, int rows, const float* dZ,
+                  float* dW) {
+  const int ntiles = 9 * (L.cin / (16 * NT)) * (L.cout / (16 * MT));
+  const long long per = (long long)L.cout * L.K;
+  long long ns = std::max<long long>(1, 2048 / ntiles);
+  ns = std::min<long long>(ns, rows);
+  ns = std::min<long long>(ns, (64LL << 20) / per);
+  const int nsplit = (int)std::max<long long>(8, ns / 8 * 8);
+  CNN_REQUIRE((long long)nsplit * ntiles < (1LL << 31), TM_ERR_INVALID, "wgrad: grid too large");
+  parts.ensure((size_t)nsplit * per);
+  hipLaunchKernelGGL((k_conv_wgrad<MT, NT>), dim3((unsigned)(nsplit * ntiles)), dim3(256), 0, st, X, L.si, L.cin,
+                     L.so, L.stride, dZ, L.cout, rows, nsplit, ntiles, parts.p);
+  check_launch("k_conv_wgrad");
+  const unsigned g = grid1((per + 31) / 32, 1, 1 << 16);
+  hipLaunchKernelGGL((k_reduce_splits<false, false>), dim3(g), dim3(256), 0, st, parts.p, nsplit, (long long)L.cout,
+                     L.K, per, dW, (long long)L.K, nullptr);
+  check_launch("k_reduce_splits");
+}
+template <int MT>
+void conv_wgrad_m(hipStream_t st, Buf<float>& parts, const float* X, const Layer& L, int rows, const float* dZ,
+                  float* dW) {
+  const int q = L.cin / 16;
+  if (q % 4 == 0)
+    conv_wgrad_t<MT, 4>(st, parts, X, L, rows, dZ, dW);
+  else if (q % 3 == 0)
+    conv_wgrad_t<MT, 3>(st, parts, X, L, rows, dZ, dW);
+  else if (q % 2 == 0)
+    conv_wgrad_t<MT, 2>(st, parts, X, L, rows, dZ, dW);
+  else
+    conv_wgrad_t<MT, 1>(st, parts, X, L, rows, dZ, dW);
+}
+// Whether k_conv_wgrad applies: channels in multiples of 16, output rows in 16-pixel segments.
+bool wgrad_implicit(const Layer& L, long long M) {
+  return L.cin % 16 == 0 && L.cout % 16 == 0 && L.so % 16 == 0 && M < (1LL << 31) &&
+         (long long)L.si * L.si * L.cin < (1LL << 31);
+}
+void conv_wgrad(hipStream_t st, Buf<float>& parts, const float* X, const Layer& L, int batch, const float* dZ,
+                float* dW) {
+  const int rows = batch * L.so, q = L.cout / 16;
+  if (q % 4 == 0)
+    conv_wgrad_m<4>(st, parts, X, L, rows, dZ, dW);
+  else if (q % 3 == 0)
+    conv_wgrad_m<3>(st, parts, X, L, rows, dZ, dW);
+  else if (q % 2 == 0)
+    conv_wgrad_m<2>(st, parts, X, L, rows, dZ, dW);
+  else
+    conv_wgrad_m<1>(st, parts, X, L, rows, dZ, dW);
 }
 
 void im2col(hipStream_t st, const float* X, const Layer& L, long long M, bf16_t* col) {
@@ -1103,45 +1479,38 @@ int tm_cnn_backward(tm_cnn* h, const float* heat_target, const int* bin_target, 
     // walk the layers backwards; dcur holds dL/d(output of layer l) (pre-activation for the head)
     float* dcur = h->dlog.p;
     float* dnext = h->dA0.p;
+    bool masked = false;  // dcur already carries the layer's ReLU backward (k_conv_dgrad's epilogue)
     for (int l = (int)h->layers.size() - 1; l >= 0; --l) {
       const Layer& L = h->layers[l];
       const bool head = l == (int)h->layers.size() - 1;
       const long long M = rows_of(h, L), Mp = round16ll(M);
-      if (!head) {
+      if (!head && !masked) {
         hipLaunchKernelGGL(k_relu_bwd, dim3(grid1(M * L.cout)), dim3(256), 0, st, dcur, h->act[l + 1].p, M * L.cout);
         check_launch("k_relu_bwd");
       }
       colsum(st, h->csum, dcur, M, L.cout, h->g.p + L.b_off);
-      // the layer's input as the product operand (recomputed), then its transpose [Kp][Mp]
-      if (head)
-        to_bf16_pad(st, h->act[l].p, M, L.cin, L.cin, h->col.p, L.Kp);
-      else {
-        im2col(st, h->act[l].p, L, M, h->col.p);
+      if (!head && wgrad_implicit(L, M)) {
+        conv_wgrad(st, h->parts, h->act[l].p, L, B, dcur, h->g.p + L.w_off);
+      } else {
+        // the layer's input as the product operand (recomputed), then its transpose [Kp][Mp]
+        if (head)
+          to_bf16_pad(st, h->act[l].p, M, L.cin, L.cin, h->col.p, L.Kp);
+        else
+          im2col(st, h->act[l].p, L, M, h->col.p);
+        transpose_bf16(st, h->col.p, M, L.Kp, L.Kp, h->colT.p, Mp, L.Kp, Mp);
+        // dZ^T [cout][Mp] and dW[cout][K] = dZ^T . col^T^T
+        transpose_bf16(st, dcur, M, L.cout, L.cout, h->dzT.p, Mp, L.cout, Mp);
+        gemm(st, h->parts, h->dzT.p, (int)Mp, h->colT.p, (int)Mp, h->g.p + L.w_off, L.K, L.cout, L.K, (int)Mp,
+             nullptr, false);
       }
-      transpose_bf16(st, h->col.p, M, L.Kp, L.Kp, h->colT.p, Mp, L.Kp, Mp);
-      // dZ^T [cout][Mp] and dW[cout][K] = dZ^T . col^T^T
-      transpose_bf16(st, dcur, M, L.cout, L.cout, h->dzT.p, Mp, L.cout, Mp);
-      gemm(st, h->parts, h->dzT.p, (int)Mp, h->colT.p, (int)Mp, h->g.p + L.w_off, L.K, L.cout, L.K, (int)Mp, nullptr,
-           false);
       if (l == 0) break;
       if (!head && L.cout % 16 == 0) {
         // implicit transposed convolution straight into the input's gradient
-        CNN_REQUIRE((long long)L.so * L.so * L.cout < (1LL << 31) && (long long)B * L.si * L.si < (1LL << 31),
-                    TM_ERR_INVALID, "dgrad: layer too large");
         hipLaunchKernelGGL(k_weights_dgrad, dim3(grid1((long long)L.cin * 9 * L.cout)),
                            dim3(256), 0, st, h->w.p + L.w_off, L.cin, L.cout, h->wbT.p);
         check_launch("k_weights_dgrad");
-        const long long Mi = (long long)B * L.si * L.si;
-        if (L.cin <= 32) {
-          const dim3 g((unsigned)((Mi + 127) / 128), (L.cin + 31) / 32);
-          hipLaunchKernelGGL((k_conv_dgrad<1>), g, dim3(256), 0, st, dcur, L.si, L.cout, L.so, L.stride, h->wbT.p,
-                             dnext, Mi, L.cin);
-        } else {
-          const dim3 g((unsigned)((Mi + 63) / 64), (L.cin + 63) / 64);
-          hipLaunchKernelGGL((k_conv_dgrad<2>), g, dim3(256), 0, st, dcur, L.si, L.cout, L.so, L.stride, h->wbT.p,
-                             dnext, Mi, L.cin);
-        }
-        check_launch("k_conv_dgrad");
+        conv_dgrad(st, dcur, L, B, h->wbT.p, h->act[l].p, dnext);
+        masked = true;
         dcur = dnext;
         dnext = (dnext == h->dA0.p) ? h->dA1.p : h->dA0.p;
         continue;
@@ -1161,6 +1530,7 @@ int tm_cnn_backward(tm_cnn* h, const float* heat_target, const int* bin_target, 
                            L.stride, (int)total, dnext);
         check_launch("k_col2im");
       }
+      masked = false;
       dcur = dnext;
       dnext = (dnext == h->dA0.p) ? h->dA1.p : h->dA0.p;
     }

@@ -1,14 +1,17 @@
 """GPU: the proposal network's kernels (include/tm_cnn.h) against the oracle (oracle/cnn_oracle.py).
 
 * bf16 MFMA product C = A . B^T: ragged M, N, K, a 601-column head shape, each block shape
-  (64 x 64, 128 x 32 for N <= 32, 32 x 128 for M <= 32) and split-K cases of 2 and 19 splits, against
-  float64 on the same bf16-rounded operands, within 1e-5 of sum |a| |b| per element
+  (128 x 128 on 16x16x32 MFMAs for M, N >= 128, with a 16-wide k remainder and split over K; 64 x 64,
+  128 x 32 for N <= 32, 32 x 128 for M <= 32) and split-K cases of 2, 8 and 19 splits, against float64
+  on the same bf16-rounded operands, within 1e-5 of sum |a| |b| per element
 * three networks: channels 8, 16, 16 (strides 2, 2, 1), 16, 32, 48 (strides 2, 1, 2) and 16, 48, 64, 32
-  (strides 2, 1, 2, 1), so that the implicit-product convolution (Cin a multiple of 16) runs with
-  one, two and three k steps per tap beside the im2col path (Cin 1 and 8), and the backward pass's
-  implicit transposed convolution (data gradient of a non-head layer with Cout a multiple of 16)
-  runs both for Cin <= 32 (32-channel tiles) and for Cin 48 and 64 (64-channel tiles, the shapes of
-  the default network's deeper layers; 48 leaves a partial tile)
+  (strides 2, 1, 2, 1), so that the implicit-product forward convolution (Cin a multiple of 16) runs
+  with one, two and three k steps per tap and 16-, 32-, 48- and 64-channel wave tiles beside the
+  im2col path (Cin 1 and 8); the implicit data gradient (Cout a multiple of 16, the previous ReLU's
+  backward in its epilogue) runs at stride 1 and over the four pixel-parity classes at stride 2, into
+  8 (a partial tile), 16, 32, 48 and 64 channels; and the implicit weight gradient (Cin and Cout
+  multiples of 16) runs with output x input channel tiles of 16 x 16, 32 x 16, 48 x 32, 48 x 16,
+  64 x 48 and 32 x 64 at both strides
 * forward pass: every layer's activations and the logits within 2e-4 (relative L2) of the oracle
   with the device's bf16 rounding points (the remainder is FP32 accumulation order, and the odd
   input that lies within that noise of a bf16 rounding edge and rounds the other way, one bf16 ulp
@@ -34,7 +37,8 @@ def _bf(x):
 
 
 @pytest.mark.parametrize("M,N,K", [(37, 50, 23), (130, 70, 64), (200, 601, 128), (5, 3, 5000), (64, 64, 16),
-                                   (1000, 32, 300), (16, 200, 64), (20, 300, 4100), (32, 96, 40000)])
+                                   (1000, 32, 300), (16, 200, 64), (20, 300, 4100), (32, 96, 40000),
+                                   (300, 200, 4100), (1000, 130, 48), (129, 601, 128), (601, 128, 20000)])
 def test_gemm(M, N, K):
     print(f"case: gemm {M}x{N}x{K}", flush=True)
     rng = np.random.default_rng(M * N + K)
