@@ -71,6 +71,13 @@ __device__ inline bf16_t f2bf(float f) {
   return (bf16_t)(u >> 16);
 }
 __device__ inline float bf2f(bf16_t b) { return __uint_as_float((unsigned)b << 16); }
+// *p, or `zero` when !ok; p must be a valid address either way. Loading first and selecting after keeps
+// the compiler from wrapping each masked load in a branch of its own.
+template <class T>
+__device__ inline T load_or(const T* p, bool ok, T zero) {
+  const T v = *p;
+  return ok ? v : zero;
+}
 __device__ inline float load_f(const float* p) { return *p; }
 __device__ inline float load_f(const bf16_t* p) { return bf2f(*p); }
 
@@ -125,8 +132,8 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const bf16_t* __restrict__ A, i
     for (int u = 0; u < KU; ++u)
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
-        a[u][i] = aok[i] ? *reinterpret_cast<const short4_t*>(ap[i] + k + 16 * u) : zero;
-        b[u][i] = bok[i] ? *reinterpret_cast<const short4_t*>(bp[i] + k + 16 * u) : zero;
+        a[u][i] = load_or(reinterpret_cast<const short4_t*>(ap[i] + k + 16 * u), aok[i], zero);
+        b[u][i] = load_or(reinterpret_cast<const short4_t*>(bp[i] + k + 16 * u), bok[i], zero);
       }
 #pragma unroll
     for (int u = 0; u < KU; ++u)
@@ -140,8 +147,8 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const bf16_t* __restrict__ A, i
     short4_t a[2], b[2];
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
-      a[i] = aok[i] ? *reinterpret_cast<const short4_t*>(ap[i] + k) : zero;
-      b[i] = bok[i] ? *reinterpret_cast<const short4_t*>(bp[i] + k) : zero;
+      a[i] = load_or(reinterpret_cast<const short4_t*>(ap[i] + k), aok[i], zero);
+      b[i] = load_or(reinterpret_cast<const short4_t*>(bp[i] + k), bok[i], zero);
     }
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -210,8 +217,8 @@ __global__ __launch_bounds__(256) void k_gemm_nt4(const bf16_t* __restrict__ A, 
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int kk = k + 32 * u + 8 * kg;
-        a[u][i] = aok[i] ? *reinterpret_cast<const short8_t*>(ap[i] + kk) : zero8;
-        b[u][i] = bok[i] ? *reinterpret_cast<const short8_t*>(bp[i] + kk) : zero8;
+        a[u][i] = load_or(reinterpret_cast<const short8_t*>(ap[i] + kk), aok[i], zero8);
+        b[u][i] = load_or(reinterpret_cast<const short8_t*>(bp[i] + kk), bok[i], zero8);
       }
 #pragma unroll
     for (int u = 0; u < 2; ++u)
@@ -226,8 +233,8 @@ __global__ __launch_bounds__(256) void k_gemm_nt4(const bf16_t* __restrict__ A, 
     short8_t a[4], b[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      a[i] = aok[i] ? *reinterpret_cast<const short8_t*>(ap[i] + k + 8 * kg) : zero8;
-      b[i] = bok[i] ? *reinterpret_cast<const short8_t*>(bp[i] + k + 8 * kg) : zero8;
+      a[i] = load_or(reinterpret_cast<const short8_t*>(ap[i] + k + 8 * kg), aok[i], zero8);
+      b[i] = load_or(reinterpret_cast<const short8_t*>(bp[i] + k + 8 * kg), bok[i], zero8);
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -241,8 +248,8 @@ __global__ __launch_bounds__(256) void k_gemm_nt4(const bf16_t* __restrict__ A, 
     short4_t a[4], b[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      a[i] = aok[i] ? *reinterpret_cast<const short4_t*>(ap[i] + k + 4 * kg) : zero;
-      b[i] = bok[i] ? *reinterpret_cast<const short4_t*>(bp[i] + k + 4 * kg) : zero;
+      a[i] = load_or(reinterpret_cast<const short4_t*>(ap[i] + k + 4 * kg), aok[i], zero);
+      b[i] = load_or(reinterpret_cast<const short4_t*>(bp[i] + k + 4 * kg), bok[i], zero);
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i)
@@ -280,14 +287,19 @@ __device__ inline short4_t bf16x4_hw(float a, float b, float c, float d) {
 }
 __device__ inline short4_t bf16x4_hw(float4 v) { return bf16x4_hw(v.x, v.y, v.z, v.w); }
 
-// Forward, for Cin % 16 == 0: Y[m][n] = relu(sum_k col(m, k) B[n][k] + bias[n]) with
+// Forward, for Cin % 16 == 0: Y[m][n] = epi(sum_k col(m, k) B[n][k]) with
 // col(m, k) = bf16(X[b][oy * stride + ky - 1][ox * stride + kx - 1][c]) for m = (b, oy, ox),
 // k = (ky * 3 + kx) * Cin + c (0 outside the image), read from the FP32 activations instead of a
-// materialised im2col matrix. B: bf16 weights, rows of ldb >= 9 Cin. Si^2 Cin < 2^31, M < 2^31.
-template <int NT>
+// materialised im2col matrix; TAPS = 1 takes the centre tap alone (a 1x1 layer, k = c: the head, and
+// the head's data gradient with X = dL/dlogits). epi: EPI_BIAS_RELU relu(v + bias[n]), EPI_BIAS
+// v + bias[n], EPI_MASK (A[m][n] > 0 ? v : 0, the backward of the ReLU that produced A). B: bf16
+// rows of ldb >= TAPS Cin. Si^2 Cin < 2^31, M N < 2^31.
+enum { EPI_BIAS_RELU = 0, EPI_BIAS = 1, EPI_MASK = 2 };
+template <int NT, int TAPS, int EPI>
 __global__ __launch_bounds__(256) void k_conv_fwd(const float* __restrict__ X, int Si, int Cin, int So, int stride,
                                                   const bf16_t* __restrict__ B, int ldb, float* __restrict__ Y,
-                                                  int M, int N, const float* __restrict__ bias) {
+                                                  int M, int N, const float* __restrict__ bias,
+                                                  const float* __restrict__ A) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int m0 = blockIdx.x * 256 + wave * 64, n0 = blockIdx.y * 16 * NT;
   const int r = lane & 15, kq = (lane >> 4) * 4;
@@ -319,7 +331,8 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const float* __restrict__ X, i
 #pragma unroll
     for (int j = 0; j < NT; ++j) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
   const short4_t zero = {0, 0, 0, 0};
-  for (int tap = 0; tap < 9; ++tap) {
+  const float4 zf4 = {0.f, 0.f, 0.f, 0.f};
+  for (int tap = TAPS == 9 ? 0 : 4; tap < (TAPS == 9 ? 9 : 5); ++tap) {
     const int ky = tap / 3, kx = tap - 3 * (tap / 3);
     const float* ap[4];
     bool aok[4];
@@ -327,9 +340,10 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const float* __restrict__ X, i
     for (int i = 0; i < 4; ++i) {
       const int iy = y0[i] + ky, ix = x0[i] + kx;
       aok[i] = iy >= 0 && iy < Si && ix >= 0 && ix < Si;
-      ap[i] = xb[i] + (aok[i] ? (iy * Si + ix) * Cin : 0);
+      // a pixel clamped into the image: a valid address without a branch, its value then dropped
+      ap[i] = xb[i] + (min(max(iy, 0), Si - 1) * Si + min(max(ix, 0), Si - 1)) * Cin;
     }
-    const int kt = tap * Cin;
+    const int kt = TAPS == 9 ? tap * Cin : 0;
     // two channel steps per iteration, their loads issued before their MFMAs
     int c0 = 0;
     for (; c0 + 32 <= Cin; c0 += 32) {
@@ -339,10 +353,10 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const float* __restrict__ X, i
       for (int u = 0; u < 2; ++u) {
 #pragma unroll
         for (int i = 0; i < 4; ++i)
-          araw[u][i] = aok[i] ? *reinterpret_cast<const float4*>(ap[i] + c0 + 16 * u) : float4{0.f, 0.f, 0.f, 0.f};
+          araw[u][i] = load_or(reinterpret_cast<const float4*>(ap[i] + c0 + 16 * u), aok[i], zf4);
 #pragma unroll
         for (int j = 0; j < NT; ++j)
-          bq[u][j] = bok[j] ? *reinterpret_cast<const short4_t*>(bp[j] + kt + c0 + 16 * u) : zero;
+          bq[u][j] = load_or(reinterpret_cast<const short4_t*>(bp[j] + kt + c0 + 16 * u), bok[j], zero);
       }
 #pragma unroll
       for (int u = 0; u < 2; ++u)
@@ -357,10 +371,10 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const float* __restrict__ X, i
     if (c0 < Cin) {
       short4_t bq[NT];
 #pragma unroll
-      for (int j = 0; j < NT; ++j) bq[j] = bok[j] ? *reinterpret_cast<const short4_t*>(bp[j] + kt + c0) : zero;
+      for (int j = 0; j < NT; ++j) bq[j] = load_or(reinterpret_cast<const short4_t*>(bp[j] + kt + c0), bok[j], zero);
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const short4_t a = aok[i] ? bf16x4_hw(*reinterpret_cast<const float4*>(ap[i] + c0)) : zero;
+        const short4_t a = bf16x4_hw(load_or(reinterpret_cast<const float4*>(ap[i] + c0), aok[i], zf4));
 #pragma unroll
         for (int j = 0; j < NT; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bq[j], acc[i][j], 0, 0, 0);
       }
@@ -372,11 +386,15 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const float* __restrict__ X, i
     for (int j = 0; j < NT; ++j) {
       const int n = n0 + 16 * j + r;
       if (n >= N) continue;
-      const float bn = bias[n];
+      const float bn = EPI == EPI_MASK ? 0.f : bias[n];
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int m = m0 + 16 * i + kq + e;
-        if (m < M) Y[(size_t)m * N + n] = fmaxf(acc[i][j][e] + bn, 0.f);
+        if (m >= M) continue;
+        const size_t o = (size_t)m * N + n;
+        if (EPI == EPI_BIAS_RELU) Y[o] = fmaxf(acc[i][j][e] + bn, 0.f);
+        if (EPI == EPI_BIAS) Y[o] = acc[i][j][e] + bn;
+        if (EPI == EPI_MASK) Y[o] = A[o] > 0.f ? acc[i][j][e] : 0.f;
       }
     }
 }
@@ -433,6 +451,7 @@ __global__ __launch_bounds__(256) void k_conv_dgrad(const float* __restrict__ dZ
 #pragma unroll
     for (int j = 0; j < NT; ++j) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
   const short4_t zero = {0, 0, 0, 0};
+  const float4 zf4 = {0.f, 0.f, 0.f, 0.f};
   for (int ky = ky0; ky < kyn; ky += kys)
     for (int kx = kx0; kx < kxn; kx += kxs) {
       const float* ap[4];
@@ -443,7 +462,7 @@ __global__ __launch_bounds__(256) void k_conv_dgrad(const float* __restrict__ dZ
         const int ty = iy1[i] - ky, tx = ix1[i] - kx;
         const int oy = stride == 2 ? ty >> 1 : ty, ox = stride == 2 ? tx >> 1 : tx;
         aok[i] = mok[i] && ty >= 0 && tx >= 0 && oy < So && ox < So;
-        ap[i] = zb[i] + (aok[i] ? (oy * So + ox) * Cout : 0);
+        ap[i] = zb[i] + (min(max(oy, 0), So - 1) * So + min(max(ox, 0), So - 1)) * Cout;  // clamped, as above
       }
       const int kt = (ky * 3 + kx) * Cout;
       int c0 = 0;
@@ -454,10 +473,10 @@ __global__ __launch_bounds__(256) void k_conv_dgrad(const float* __restrict__ dZ
         for (int u = 0; u < 2; ++u) {
 #pragma unroll
           for (int i = 0; i < 4; ++i)
-            araw[u][i] = aok[i] ? *reinterpret_cast<const float4*>(ap[i] + c0 + 16 * u) : float4{0.f, 0.f, 0.f, 0.f};
+            araw[u][i] = load_or(reinterpret_cast<const float4*>(ap[i] + c0 + 16 * u), aok[i], zf4);
 #pragma unroll
           for (int j = 0; j < NT; ++j)
-            bq[u][j] = bok[j] ? *reinterpret_cast<const short4_t*>(bp[j] + kt + c0 + 16 * u) : zero;
+            bq[u][j] = load_or(reinterpret_cast<const short4_t*>(bp[j] + kt + c0 + 16 * u), bok[j], zero);
         }
 #pragma unroll
         for (int u = 0; u < 2; ++u)
@@ -472,10 +491,10 @@ __global__ __launch_bounds__(256) void k_conv_dgrad(const float* __restrict__ dZ
       if (c0 < Cout) {
         short4_t bq[NT];
 #pragma unroll
-        for (int j = 0; j < NT; ++j) bq[j] = bok[j] ? *reinterpret_cast<const short4_t*>(bp[j] + kt + c0) : zero;
+        for (int j = 0; j < NT; ++j) bq[j] = load_or(reinterpret_cast<const short4_t*>(bp[j] + kt + c0), bok[j], zero);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          const short4_t a = aok[i] ? bf16x4_hw(*reinterpret_cast<const float4*>(ap[i] + c0)) : zero;
+            const short4_t a = bf16x4_hw(load_or(reinterpret_cast<const float4*>(ap[i] + c0), aok[i], zf4));
 #pragma unroll
           for (int j = 0; j < NT; ++j)
             acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bq[j], acc[i][j], 0, 0, 0);
@@ -507,30 +526,35 @@ __global__ void k_weights_dgrad(const float* __restrict__ W, int Cin, int Cout, 
   }
 }
 
-// Weight gradient of a 3x3 convolution as an implicit product, for Cin % 16 == 0, Cout % 16 == 0 and
-// So % 16 == 0: parts[z][co][tap Cin + c] = sum over the output pixels p = (b, oy, ox) of split z of
-// bf16(dZ[p][co]) bf16(X[b][oy stride + ky - 1][ox stride + kx - 1][c]) (0 outside the image), with
-// tap = 3 ky + kx: dW = dZ^T col without the im2col matrix or either transpose. The product's k runs
+// Weight gradient of a 3x3 convolution (TAPS = 9) or of the 1x1 head (TAPS = 1: the centre tap, stride
+// 1) as an implicit product, for Cin % 16 == 0 and So % 16 == 0: parts[z][co][tap Cin + c] = sum over
+// the output pixels p = (b, oy, ox) of split z of bf16(dZ[p][co]) bf16(X[b][oy stride + ky - 1]
+// [ox stride + kx - 1][c]) (0 outside the image), with tap = 3 ky + kx (0 for the head): dW = dZ^T col
+// without the im2col matrix or either transpose. dZ rows are ldz floats apart, ldz a multiple of
+// 16 MT: Cout need not be a multiple of 16 when columns Cout .. ldz of dZ are zero (the head's padded
+// dL/dlogits), the last tile then writing only its channels below Cout. Element offsets into X and
+// dZ are 32-bit (the host checks both sizes). The product's k runs
 // over pixels, in 16-pixel segments of one output row: a lane's four pixels are consecutive, so its
 // four dZ values lie Cout floats apart and its four X values stride Cin floats apart, at constant
 // offsets from one address (only the image's left and right edges mask a pixel). A block covers one
 // tap, 16 MT output and 16 NT input channels of one split's rows; its four waves take every fourth
 // segment and add their tiles through LDS in wave order (results do not depend on scheduling).
 // Block b runs on XCD b % 8: the blocks of one split share an XCD, so the tiles that read the same
-// pixels meet in one L2. nsplit % 8 == 0; ntiles = 9 (Cin / 16 NT) (Cout / 16 MT).
-template <int MT, int NT>
+// pixels meet in one L2. nsplit % 8 == 0; ntiles = TAPS (Cin / 16 NT) ceil(Cout / 16 MT).
+template <int MT, int NT, int TAPS>
 __global__ __launch_bounds__(256) void k_conv_wgrad(const float* __restrict__ X, int Si, int Cin, int So, int stride,
-                                                    const float* __restrict__ dZ, int Cout, int rows, int nsplit,
-                                                    int ntiles, float* __restrict__ parts) {
+                                                    const float* __restrict__ dZ, int ldz, int Cout, int rows,
+                                                    int nsplit, int ntiles, float* __restrict__ parts) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int xcd = blockIdx.x & 7, jx = blockIdx.x >> 3;
   const int z = xcd * (nsplit >> 3) + jx / ntiles;
   int t = jx % ntiles;
   const int nc = Cin / (16 * NT);
-  const int cm = t / (9 * nc);
-  t -= cm * 9 * nc;
-  const int tap = t / nc, cc = t - tap * nc;
+  const int cm = t / (TAPS * nc);
+  t -= cm * TAPS * nc;
+  const int tapi = t / nc, cc = t - tapi * nc;
+  const int tap = TAPS == 9 ? tapi : 4;
   const int ky = tap / 3, kx = tap - 3 * ky;
   const int co0 = cm * 16 * MT, ci0 = cc * 16 * NT;
   const int r = lane & 15, pq = (lane >> 4) * 4;
@@ -546,22 +570,25 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const float* __restrict__ X,
     const int row = g / spr;
     const int ox = (g - row * spr) * 16 + pq;
     const int b = row / So, oy = row - b * So;
-    const float* zp = dZ + ((size_t)row * So + ox) * Cout + co0 + r;
+    const int zo = (row * So + ox) * ldz + co0 + r;
 #pragma unroll
     for (int i = 0; i < MT; ++i)
 #pragma unroll
-      for (int e = 0; e < 4; ++e) za[i][e] = zp[e * Cout + 16 * i];
+      for (int e = 0; e < 4; ++e) za[i][e] = dZ[zo + e * ldz + 16 * i];
     const int iy = oy * stride + ky - 1;
     const int ix0 = ox * stride + kx - 1;
     if (iy >= 0 && iy < Si) {
-      const long long base = (((long long)b * Si + iy) * Si + ix0) * Cin + ci0 + r;
+      const int base = ((b * Si + iy) * Si + ix0) * Cin + ci0 + r;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int ix = ix0 + e * stride;
         const bool ok = ix >= 0 && ix < Si;
-        const float* xp = X + (ok ? base + (long long)e * stride * Cin : 0);
+        const int xo = ok ? base + e * stride * Cin : 0;  // X itself when masked: loaded, then dropped
 #pragma unroll
-        for (int j = 0; j < NT; ++j) xa[j][e] = ok ? xp[16 * j] : 0.f;
+        for (int j = 0; j < NT; ++j) {
+          const float v = X[xo + 16 * j];
+          xa[j][e] = ok ? v : 0.f;
+        }
       }
     } else {
 #pragma unroll
@@ -604,8 +631,8 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const float* __restrict__ X,
   }
   __syncthreads();
   if (wave != 0) return;
-  const int K = 9 * Cin;
-  float* out = parts + (size_t)z * Cout * K + (size_t)tap * Cin;
+  const int K = TAPS * Cin;
+  float* out = parts + (size_t)z * Cout * K + (size_t)tapi * Cin;
 #pragma unroll
   for (int i = 0; i < MT; ++i)
 #pragma unroll
@@ -614,7 +641,10 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const float* __restrict__ X,
 #pragma unroll
       for (int w = 0; w < 3; ++w) s += red[w][i * NT + j][lane];
 #pragma unroll
-      for (int e = 0; e < 4; ++e) out[(size_t)(co0 + 16 * i + pq + e) * K + ci0 + 16 * j + r] = s[e];
+      for (int e = 0; e < 4; ++e) {
+        const int co = co0 + 16 * i + pq + e;
+        if (co < Cout) out[(size_t)co * K + ci0 + 16 * j + r] = s[e];
+      }
     }
 }
 
@@ -789,8 +819,8 @@ __global__ void k_relu_bwd(float* __restrict__ dA, const float* __restrict__ A, 
 // Column sums of X[R][C] over row chunks of `chunk` rows: part[chunk index][c]. Threads split the
 // chunk's rows into 256 / Cg interleaved subsets per column group of Cg = min(C, 256) columns, each
 // thread with four accumulators over every fourth row of its subset, added in a fixed order.
-__global__ __launch_bounds__(256) void k_colsum_part(const float* __restrict__ X, long long R, int C, long long chunk,
-                                                     float* __restrict__ part) {
+__global__ __launch_bounds__(256) void k_colsum_part(const float* __restrict__ X, long long R, int C, long long ldx,
+                                                     long long chunk, float* __restrict__ part) {
   __shared__ float red[256];
   const int Cg = min(C, 256), per = 256 / Cg;
   const int t = threadIdx.x, c_off = t % Cg, sub = t / Cg;
@@ -801,12 +831,12 @@ __global__ __launch_bounds__(256) void k_colsum_part(const float* __restrict__ X
     if (sub < per && c < C) {
       long long r = r_lo + sub;
       for (; r + 3 * per < r_hi; r += 4 * per) {
-        s0 += X[r * C + c];
-        s1 += X[(r + per) * C + c];
-        s2 += X[(r + 2 * per) * C + c];
-        s3 += X[(r + 3 * per) * C + c];
+        s0 += X[r * ldx + c];
+        s1 += X[(r + per) * ldx + c];
+        s2 += X[(r + 2 * per) * ldx + c];
+        s3 += X[(r + 3 * per) * ldx + c];
       }
-      for (; r < r_hi; r += per) s0 += X[r * C + c];
+      for (; r < r_hi; r += per) s0 += X[r * ldx + c];
     }
     red[t] = (s0 + s1) + (s2 + s3);
     __syncthreads();
@@ -896,11 +926,13 @@ __device__ inline float sigmoid(float x) { return 1.f / (1.f + expf(-x)); }
 // scaled by heat_scale and bin_scale (the means' divisors); cell_loss receives the unscaled terms.
 __global__ __launch_bounds__(64) void k_loss(const float* __restrict__ logits, int Nh, const float* __restrict__ heat_t,
                                              const int* __restrict__ bin_t, float pos_w, float heat_scale,
-                                             float bin_scale, float* __restrict__ dlog, float* __restrict__ cell_loss) {
+                                             float bin_scale, float* __restrict__ dlog, int ldd,
+                                             float* __restrict__ cell_loss) {
   const long long cell = blockIdx.x;
   const float* o = logits + cell * Nh;
-  float* d = dlog + cell * Nh;
+  float* d = dlog + cell * ldd;
   const int lane = threadIdx.x;
+  for (int k = Nh + lane; k < ldd; k += 64) d[k] = 0.f;  // the row's padding
   const int nb = Nh - 1;
   const int tb = bin_t[cell];
   if (lane == 0) {
@@ -1086,23 +1118,26 @@ void gemm(hipStream_t st, Buf<float>& parts, const bf16_t* A, int lda, const bf1
 
 // Output channels per wave tile: 16 NT, NT = 4 from 64 channels up.
 inline int conv_nt(int n) { return n >= 64 ? 4 : (n + 15) / 16; }
-template <int NT>
+template <int NT, int TAPS, int EPI>
 void conv_fwd_nt(hipStream_t st, const float* X, int Si, int Cin, int So, int stride, const bf16_t* W, int ldw, float* Y,
-                 int M, int N, const float* bias) {
+                 int M, int N, const float* bias, const float* A) {
   const dim3 g((unsigned)((M + 255) / 256), (unsigned)((N + 16 * NT - 1) / (16 * NT)));
-  hipLaunchKernelGGL((k_conv_fwd<NT>), g, dim3(256), 0, st, X, Si, Cin, So, stride, W, ldw, Y, M, N, bias);
+  hipLaunchKernelGGL((k_conv_fwd<NT, TAPS, EPI>), g, dim3(256), 0, st, X, Si, Cin, So, stride, W, ldw, Y, M, N, bias,
+                     A);
   check_launch("k_conv_fwd");
 }
-// Forward 3x3 convolution + bias + ReLU without an im2col matrix (Cin % 16 == 0).
+// A 3x3 (TAPS 9) or 1x1 (TAPS 1) layer as an implicit product without an im2col matrix (Cin % 16 == 0),
+// with the epilogue EPI (bias and ReLU, bias, or the mask of A's ReLU).
+template <int TAPS, int EPI>
 void conv_fwd(hipStream_t st, const float* X, int Si, int Cin, int So, int stride, const bf16_t* W, int ldw, float* Y,
-              long long M, int N, const float* bias) {
+              long long M, int N, const float* bias, const float* A = nullptr) {
   CNN_REQUIRE(Cin % 16 == 0 && ldw % 4 == 0, TM_ERR_INVALID, "conv: Cin must be a multiple of 16");
   CNN_REQUIRE((long long)Si * Si * Cin < (1LL << 31) && M * N < (1LL << 31), TM_ERR_INVALID, "conv: image too large");
   switch (conv_nt(N)) {
-    case 1: conv_fwd_nt<1>(st, X, Si, Cin, So, stride, W, ldw, Y, (int)M, N, bias); break;
-    case 2: conv_fwd_nt<2>(st, X, Si, Cin, So, stride, W, ldw, Y, (int)M, N, bias); break;
-    case 3: conv_fwd_nt<3>(st, X, Si, Cin, So, stride, W, ldw, Y, (int)M, N, bias); break;
-    default: conv_fwd_nt<4>(st, X, Si, Cin, So, stride, W, ldw, Y, (int)M, N, bias); break;
+    case 1: conv_fwd_nt<1, TAPS, EPI>(st, X, Si, Cin, So, stride, W, ldw, Y, (int)M, N, bias, A); break;
+    case 2: conv_fwd_nt<2, TAPS, EPI>(st, X, Si, Cin, So, stride, W, ldw, Y, (int)M, N, bias, A); break;
+    case 3: conv_fwd_nt<3, TAPS, EPI>(st, X, Si, Cin, So, stride, W, ldw, Y, (int)M, N, bias, A); break;
+    default: conv_fwd_nt<4, TAPS, EPI>(st, X, Si, Cin, So, stride, W, ldw, Y, (int)M, N, bias, A); break;
   }
 }
 
@@ -1147,13 +1182,14 @@ void to_bf16_pad(hipStream_t st, const float* X, long long R, int C, long long l
   hipLaunchKernelGGL(k_to_bf16_pad, dim3(grid1(R * Cp)), dim3(256), 0, st, X, R, C, ldx, Y, Cp);
   check_launch("k_to_bf16_pad");
 }
-// Column sums in two fixed-order passes: about 2048 row chunks, then one block per column.
-void colsum(hipStream_t st, Buf<float>& part, const float* X, long long R, int C, float* out) {
+// Column sums of X[R][C] (rows ldx apart) in two fixed-order passes: about 2048 row chunks, then one
+// block per column.
+void colsum(hipStream_t st, Buf<float>& part, const float* X, long long R, int C, long long ldx, float* out) {
   const long long per = 256 / std::min(C, 256);
   const long long chunk = std::max<long long>(4 * per, (R + 2047) / 2048);
   const int np = (int)((R + chunk - 1) / chunk);
   part.ensure((size_t)np * C);
-  hipLaunchKernelGGL(k_colsum_part, dim3(np), dim3(256), 0, st, X, R, C, chunk, part.p);
+  hipLaunchKernelGGL(k_colsum_part, dim3(np), dim3(256), 0, st, X, R, C, ldx, chunk, part.p);
   check_launch("k_colsum_part");
   hipLaunchKernelGGL(k_colsum_final, dim3(C), dim3(256), 0, st, part.p, np, C, out);
   check_launch("k_colsum_final");
@@ -1162,10 +1198,10 @@ void colsum(hipStream_t st, Buf<float>& part, const float* X, long long R, int C
 // dW[cout][9 cin] of a 3x3 layer straight from its input X and output gradient dZ (k_conv_wgrad), the
 // pixel splits' partials added in split order: about 2048 blocks, a split holding at least one
 // output row, the partials at most 64 M floats.
-template <int MT, int NT>
+template <int MT, int NT, int TAPS>
 void conv_wgrad_t(hipStream_t st, Buf<float>& parts, const float* X, const Layer& L, int rows, const float* dZ,
-                  float* dW) {
-  const int ntiles = 9 * (L.cin / (16 * NT)) * (L.cout / (16 * MT));
+                  int ldz, float* dW) {
+  const int ntiles = TAPS * (L.cin / (16 * NT)) * ((L.cout + 16 * MT - 1) / (16 * MT));
   const long long per = (long long)L.cout * L.K;
   long long ns = std::max<long long>(1, 2048 / ntiles);
   ns = std::min<long long>(ns, rows);
@@ -1173,43 +1209,48 @@ void conv_wgrad_t(hipStream_t st, Buf<float>& parts, const float* X, const Layer
   const int nsplit = (int)std::max<long long>(8, ns / 8 * 8);
   CNN_REQUIRE((long long)nsplit * ntiles < (1LL << 31), TM_ERR_INVALID, "wgrad: grid too large");
   parts.ensure((size_t)nsplit * per);
-  hipLaunchKernelGGL((k_conv_wgrad<MT, NT>), dim3((unsigned)(nsplit * ntiles)), dim3(256), 0, st, X, L.si, L.cin,
-                     L.so, L.stride, dZ, L.cout, rows, nsplit, ntiles, parts.p);
+  hipLaunchKernelGGL((k_conv_wgrad<MT, NT, TAPS>), dim3((unsigned)(nsplit * ntiles)), dim3(256), 0, st, X, L.si,
+                     L.cin, L.so, L.stride, dZ, ldz, L.cout, rows, nsplit, ntiles, parts.p);
   check_launch("k_conv_wgrad");
   const unsigned g = grid1((per + 31) / 32, 1, 1 << 16);
   hipLaunchKernelGGL((k_reduce_splits<false, false>), dim3(g), dim3(256), 0, st, parts.p, nsplit, (long long)L.cout,
                      L.K, per, dW, (long long)L.K, nullptr);
   check_launch("k_reduce_splits");
 }
-template <int MT>
+template <int MT, int TAPS>
 void conv_wgrad_m(hipStream_t st, Buf<float>& parts, const float* X, const Layer& L, int rows, const float* dZ,
-                  float* dW) {
+                  int ldz, float* dW) {
   const int q = L.cin / 16;
   if (q % 4 == 0)
-    conv_wgrad_t<MT, 4>(st, parts, X, L, rows, dZ, dW);
+    conv_wgrad_t<MT, 4, TAPS>(st, parts, X, L, rows, dZ, ldz, dW);
   else if (q % 3 == 0)
-    conv_wgrad_t<MT, 3>(st, parts, X, L, rows, dZ, dW);
+    conv_wgrad_t<MT, 3, TAPS>(st, parts, X, L, rows, dZ, ldz, dW);
   else if (q % 2 == 0)
-    conv_wgrad_t<MT, 2>(st, parts, X, L, rows, dZ, dW);
+    conv_wgrad_t<MT, 2, TAPS>(st, parts, X, L, rows, dZ, ldz, dW);
   else
-    conv_wgrad_t<MT, 1>(st, parts, X, L, rows, dZ, dW);
+    conv_wgrad_t<MT, 1, TAPS>(st, parts, X, L, rows, dZ, ldz, dW);
 }
-// Whether k_conv_wgrad applies: channels in multiples of 16, output rows in 16-pixel segments.
-bool wgrad_implicit(const Layer& L, long long M) {
-  return L.cin % 16 == 0 && L.cout % 16 == 0 && L.so % 16 == 0 && M < (1LL << 31) &&
-         (long long)L.si * L.si * L.cin < (1LL << 31);
+// Whether k_conv_wgrad applies: input channels in multiples of 16 (and output channels for a 3x3
+// layer), output rows in 16-pixel segments, X and dZ (ldz floats per row) under 2^31 elements.
+bool wgrad_implicit(const Layer& L, int batch, int ldz, bool head) {
+  const long long M = (long long)batch * L.so * L.so;
+  return L.cin % 16 == 0 && (head || L.cout % 16 == 0) && ldz % 16 == 0 && L.so % 16 == 0 &&
+         M * ldz < (1LL << 31) && (long long)batch * L.si * L.si * L.cin < (1LL << 31);
 }
-void conv_wgrad(hipStream_t st, Buf<float>& parts, const float* X, const Layer& L, int batch, const float* dZ,
+// dZ rows ldz floats apart (ldz a multiple of 16, zeros past cout); tiles of 16 MT output channels
+// with MT dividing ldz / 16, so that no tile reads past a row.
+template <int TAPS>
+void conv_wgrad(hipStream_t st, Buf<float>& parts, const float* X, const Layer& L, int batch, const float* dZ, int ldz,
                 float* dW) {
-  const int rows = batch * L.so, q = L.cout / 16;
+  const int rows = batch * L.so, q = ldz / 16;
   if (q % 4 == 0)
-    conv_wgrad_m<4>(st, parts, X, L, rows, dZ, dW);
+    conv_wgrad_m<4, TAPS>(st, parts, X, L, rows, dZ, ldz, dW);
   else if (q % 3 == 0)
-    conv_wgrad_m<3>(st, parts, X, L, rows, dZ, dW);
+    conv_wgrad_m<3, TAPS>(st, parts, X, L, rows, dZ, ldz, dW);
   else if (q % 2 == 0)
-    conv_wgrad_m<2>(st, parts, X, L, rows, dZ, dW);
+    conv_wgrad_m<2, TAPS>(st, parts, X, L, rows, dZ, ldz, dW);
   else
-    conv_wgrad_m<1>(st, parts, X, L, rows, dZ, dW);
+    conv_wgrad_m<1, TAPS>(st, parts, X, L, rows, dZ, ldz, dW);
 }
 
 void im2col(hipStream_t st, const float* X, const Layer& L, long long M, bf16_t* col) {
@@ -1263,9 +1304,13 @@ void forward_layer(tm_cnn* h, int l) {
   const bool head = l == (int)h->layers.size() - 1;
   const long long M = rows_of(h, L);
   to_bf16_pad(st, h->w.p + L.w_off, L.cout, L.K, L.K, h->wb.p, L.Kp);
-  if (!head && L.cin % 16 == 0) {
-    conv_fwd(st, h->act[l].p, L.si, L.cin, L.so, L.stride, h->wb.p, L.Kp, h->act[l + 1].p, M, L.cout,
-             h->w.p + L.b_off);
+  if (L.cin % 16 == 0) {
+    if (head)
+      conv_fwd<1, EPI_BIAS>(st, h->act[l].p, L.si, L.cin, L.so, 1, h->wb.p, L.Kp, h->act[l + 1].p, M, L.cout,
+                            h->w.p + L.b_off);
+    else
+      conv_fwd<9, EPI_BIAS_RELU>(st, h->act[l].p, L.si, L.cin, L.so, L.stride, h->wb.p, L.Kp, h->act[l + 1].p, M,
+                                 L.cout, h->w.p + L.b_off);
     return;
   }
   if (head)
@@ -1303,7 +1348,7 @@ void alloc_for_batch(tm_cnn* h, int batch) {
   h->wb.ensure(wmax);
   h->wbT.ensure(wmax);
   const long long cells = (long long)batch * h->G * h->G;
-  h->dlog.ensure((size_t)cells * h->Nh);
+  h->dlog.ensure((size_t)cells * round16(h->Nh));
   h->cell_loss.ensure((size_t)cells * 2);
   h->heat_t.ensure((size_t)cells);
   h->bin_t.ensure((size_t)cells);
@@ -1474,7 +1519,7 @@ int tm_cnn_backward(tm_cnn* h, const float* heat_target, const int* bin_target, 
     CNN_HIP(hipMemcpyAsync(h->bin_t.p, bin_target, cells * sizeof(int), hipMemcpyHostToDevice, st));
     const float heat_scale = 1.f / (float)cells, bin_scale = 1.f / (float)std::max(1LL, n_bin);
     hipLaunchKernelGGL(k_loss, dim3((unsigned)cells), dim3(64), 0, st, h->act.back().p, Nh, h->heat_t.p, h->bin_t.p,
-                       pos_weight, heat_scale, bin_scale, h->dlog.p, h->cell_loss.p);
+                       pos_weight, heat_scale, bin_scale, h->dlog.p, round16(Nh), h->cell_loss.p);
     check_launch("k_loss");
     // walk the layers backwards; dcur holds dL/d(output of layer l) (pre-activation for the head)
     float* dcur = h->dlog.p;
@@ -1488,9 +1533,14 @@ int tm_cnn_backward(tm_cnn* h, const float* heat_target, const int* bin_target, 
         hipLaunchKernelGGL(k_relu_bwd, dim3(grid1(M * L.cout)), dim3(256), 0, st, dcur, h->act[l + 1].p, M * L.cout);
         check_launch("k_relu_bwd");
       }
-      colsum(st, h->csum, dcur, M, L.cout, h->g.p + L.b_off);
-      if (!head && wgrad_implicit(L, M)) {
-        conv_wgrad(st, h->parts, h->act[l].p, L, B, dcur, h->g.p + L.w_off);
+      // dcur rows: the head's gradient dL/dlogits is padded to a multiple of 16 columns (zeros)
+      const int ldz = head ? round16(L.cout) : L.cout;
+      colsum(st, h->csum, dcur, M, L.cout, ldz, h->g.p + L.b_off);
+      if (wgrad_implicit(L, B, ldz, head)) {
+        if (head)
+          conv_wgrad<1>(st, h->parts, h->act[l].p, L, B, dcur, ldz, h->g.p + L.w_off);
+        else
+          conv_wgrad<9>(st, h->parts, h->act[l].p, L, B, dcur, ldz, h->g.p + L.w_off);
       } else {
         // the layer's input as the product operand (recomputed), then its transpose [Kp][Mp]
         if (head)
@@ -1499,7 +1549,7 @@ int tm_cnn_backward(tm_cnn* h, const float* heat_target, const int* bin_target, 
           im2col(st, h->act[l].p, L, M, h->col.p);
         transpose_bf16(st, h->col.p, M, L.Kp, L.Kp, h->colT.p, Mp, L.Kp, Mp);
         // dZ^T [cout][Mp] and dW[cout][K] = dZ^T . col^T^T
-        transpose_bf16(st, dcur, M, L.cout, L.cout, h->dzT.p, Mp, L.cout, Mp);
+        transpose_bf16(st, dcur, M, L.cout, ldz, h->dzT.p, Mp, L.cout, Mp);
         gemm(st, h->parts, h->dzT.p, (int)Mp, h->colT.p, (int)Mp, h->g.p + L.w_off, L.K, L.cout, L.K, (int)Mp,
              nullptr, false);
       }
@@ -1517,8 +1567,17 @@ int tm_cnn_backward(tm_cnn* h, const float* heat_target, const int* bin_target, 
       }
       // dcol[M][K] = dZ[M][cout16] . W^T[K][cout16]^T, then the gather into the input's gradient
       const int c16 = round16(L.cout);
-      to_bf16_pad(st, dcur, M, L.cout, L.cout, h->dzb.p, c16);
       transpose_bf16(st, h->w.p + L.w_off, L.cout, L.K, L.K, h->wbT.p, c16, L.K, c16);
+      if (head && L.cin % 16 == 0) {
+        // the head's data gradient as a 1x1 implicit product over its padded dL/dlogits rows, with
+        // the last convolution's ReLU backward in the epilogue
+        conv_fwd<1, EPI_MASK>(st, dcur, L.si, ldz, L.so, 1, h->wbT.p, c16, dnext, M, L.cin, nullptr, h->act[l].p);
+        masked = true;
+        dcur = dnext;
+        dnext = (dnext == h->dA0.p) ? h->dA1.p : h->dA0.p;
+        continue;
+      }
+      to_bf16_pad(st, dcur, M, L.cout, ldz, h->dzb.p, c16);
       if (head) {
         gemm(st, h->parts, h->dzb.p, c16, h->wbT.p, c16, dnext, L.K, M, L.K, c16, nullptr, false);
       } else {

@@ -4,14 +4,17 @@
   (128 x 128 on 16x16x32 MFMAs for M, N >= 128, with a 16-wide k remainder and split over K; 64 x 64,
   128 x 32 for N <= 32, 32 x 128 for M <= 32) and split-K cases of 2, 8 and 19 splits, against float64
   on the same bf16-rounded operands, within 1e-5 of sum |a| |b| per element
-* three networks: channels 8, 16, 16 (strides 2, 2, 1), 16, 32, 48 (strides 2, 1, 2) and 16, 48, 64, 32
-  (strides 2, 1, 2, 1), so that the implicit-product forward convolution (Cin a multiple of 16) runs
+* four networks: channels 8, 16, 16 (strides 2, 2, 1), 16, 32, 48 (strides 2, 1, 2), 16, 48, 64, 32
+  (strides 2, 1, 2, 1) and 8, 24 (strides 2, 2), so that the implicit-product forward convolution (Cin a multiple of 16) runs
   with one, two and three k steps per tap and 16-, 32-, 48- and 64-channel wave tiles beside the
   im2col path (Cin 1 and 8); the implicit data gradient (Cout a multiple of 16, the previous ReLU's
   backward in its epilogue) runs at stride 1 and over the four pixel-parity classes at stride 2, into
   8 (a partial tile), 16, 32, 48 and 64 channels; and the implicit weight gradient (Cin and Cout
   multiples of 16) runs with output x input channel tiles of 16 x 16, 32 x 16, 48 x 32, 48 x 16,
-  64 x 48 and 32 x 64 at both strides
+  64 x 48 and 32 x 64 at both strides; the head (21 outputs, a ragged 16-channel tile) runs as 1x1
+  implicit products (forward, weight gradient over the zero-padded dL/dlogits rows, data gradient
+  with the last ReLU's backward), except after the 24-channel layer, where the head and that layer
+  take the general paths (im2col, transposes, split products, col2im)
 * forward pass: every layer's activations and the logits within 2e-4 (relative L2) of the oracle
   with the device's bf16 rounding points (the remainder is FP32 accumulation order, and the odd
   input that lies within that noise of a bf16 rounding edge and rounds the other way, one bf16 ulp
@@ -55,7 +58,8 @@ def _rel(a, b):
     return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
 
 
-CASES = {"narrow": ([8, 16, 16], [2, 2, 1]), "wide": ([16, 32, 48], [2, 1, 2]), "deep": ([16, 48, 64, 32], [2, 1, 2, 1])}
+CASES = {"narrow": ([8, 16, 16], [2, 2, 1]), "wide": ([16, 32, 48], [2, 1, 2]), "deep": ([16, 48, 64, 32], [2, 1, 2, 1]),
+         "odd": ([8, 24], [2, 2])}
 
 
 def _case(kind="narrow"):
