@@ -276,6 +276,14 @@ __global__ __launch_bounds__(256) void k_gemm_nt4(const bf16_t* __restrict__ A, 
     }
 }
 
+// Block x index dealt to the XCDs in contiguous ranges: block b runs on XCD b % 8, so when the x
+// extent is a multiple of 8 the blocks of XCD q take pixel blocks q n / 8 .. (q + 1) n / 8 - 1 (all
+// their n tiles too), and the nine taps' rereads of neighbouring pixels meet in that XCD's L2.
+__device__ inline int xcd_block_x() {
+  const int n = gridDim.x, b = blockIdx.x;
+  return (n & 7) ? b : (b & 7) * (n >> 3) + (b >> 3);
+}
+
 // 3x3 convolution kernels as implicit products on MFMA 16x16x16 bf16 tiles. A wave owns 64 pixels (four
 // 16-row tiles) x 16 NT channels, so each loaded operand feeds NT or four MFMAs; a block is four waves
 // along the pixels. Every 16-wide k step lies in one tap, and a lane's four values of it are four
@@ -287,55 +295,92 @@ __device__ inline short4_t bf16x4_hw(float a, float b, float c, float d) {
 }
 __device__ inline short4_t bf16x4_hw(float4 v) { return bf16x4_hw(v.x, v.y, v.z, v.w); }
 
+// Weights packed in MFMA operand order: Bp[j][s][lane][e] = bf16(B(n = 16 j + lane % 16,
+// k = 16 s + 4 (lane / 16) + e)), zero for n >= N, for j < NBp (a multiple of 4 blocks, so that a
+// wave's four 16-row tiles always lie inside) and s < K / 16: the 64 lanes of one operand load read
+// 512 contiguous bytes. B(n, k) by MODE: PACK_ROWS W[n ldw + k] (k < Kw, else 0); PACK_DGRAD
+// W[co 9 Cin + tap Cin + n] for k = tap Cout + co (the data gradient's B, the weights transposed per
+// tap); PACK_COLS W[k ldw + n] (k < Kw, else 0; the head's data gradient).
+enum { PACK_ROWS = 0, PACK_DGRAD = 1, PACK_COLS = 2 };
+template <int MODE>
+__global__ void k_pack_b(const float* __restrict__ W, int N, int K, int ldw, int Kw, int Cin, int Cout, int NBp,
+                         bf16_t* __restrict__ Bp) {
+  const int S = K / 16;
+  const long long total = (long long)NBp * S * 256;
+  for (long long e = blockIdx.x * 256LL + threadIdx.x; e < total; e += (long long)gridDim.x * 256) {
+    const int q = (int)(e & 3), lane = (int)((e >> 2) & 63);
+    const long long js = e >> 8;
+    const int s = (int)(js % S), j = (int)(js / S);
+    const int n = 16 * j + (lane & 15), k = 16 * s + 4 * (lane >> 4) + q;
+    float v = 0.f;
+    if (n < N) {
+      if (MODE == PACK_ROWS && k < Kw) v = W[(size_t)n * ldw + k];
+      if (MODE == PACK_COLS && k < Kw) v = W[(size_t)k * ldw + n];
+      if (MODE == PACK_DGRAD) {
+        const int tap = k / Cout, co = k - tap * Cout;
+        v = W[(size_t)co * 9 * Cin + tap * Cin + n];
+      }
+    }
+    Bp[e] = f2bf(v);
+  }
+}
+
+// The A operand of the convolutions: lane L loads four channels (4 (L % 4) ..) of pixel L / 4 of a
+// 16-pixel tile, so four consecutive lanes read one pixel's 64 contiguous bytes, and ds_bpermute then
+// hands MFMA lane (r, g) = r + 16 g the values of pixel r, channels 4 g ..: those of lane 4 r + g.
+__device__ inline short4_t to_mfma_lanes(float4 v, int src_addr) {
+  const short4_t h = bf16x4_hw(v);
+  int2 w = __builtin_bit_cast(int2, h);
+  w.x = __builtin_amdgcn_ds_bpermute(src_addr, w.x);
+  w.y = __builtin_amdgcn_ds_bpermute(src_addr, w.y);
+  return __builtin_bit_cast(short4_t, w);
+}
+
 // Forward, for Cin % 16 == 0: Y[m][n] = epi(sum_k col(m, k) B[n][k]) with
 // col(m, k) = bf16(X[b][oy * stride + ky - 1][ox * stride + kx - 1][c]) for m = (b, oy, ox),
 // k = (ky * 3 + kx) * Cin + c (0 outside the image), read from the FP32 activations instead of a
 // materialised im2col matrix; TAPS = 1 takes the centre tap alone (a 1x1 layer, k = c: the head, and
 // the head's data gradient with X = dL/dlogits). epi: EPI_BIAS_RELU relu(v + bias[n]), EPI_BIAS
-// v + bias[n], EPI_MASK (A[m][n] > 0 ? v : 0, the backward of the ReLU that produced A). B: bf16
-// rows of ldb >= TAPS Cin. Si^2 Cin < 2^31, M N < 2^31.
+// v + bias[n], EPI_MASK (A[m][n] > 0 ? v : 0, the backward of the ReLU that produced A). Bp: B packed
+// by k_pack_b (K = TAPS Cin). Si^2 Cin < 2^31, M N < 2^31.
 enum { EPI_BIAS_RELU = 0, EPI_BIAS = 1, EPI_MASK = 2 };
 template <int NT, int TAPS, int EPI>
 __global__ __launch_bounds__(256) void k_conv_fwd(const float* __restrict__ X, int Si, int Cin, int So, int stride,
-                                                  const bf16_t* __restrict__ B, int ldb, float* __restrict__ Y,
-                                                  int M, int N, const float* __restrict__ bias,
-                                                  const float* __restrict__ A) {
+                                                  const bf16_t* __restrict__ Bp, float* __restrict__ Y, int M, int N,
+                                                  const float* __restrict__ bias, const float* __restrict__ A) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int m0 = blockIdx.x * 256 + wave * 64, n0 = blockIdx.y * 16 * NT;
+  const int m0 = xcd_block_x() * 256 + wave * 64, n0 = blockIdx.y * 16 * NT;
   const int r = lane & 15, kq = (lane >> 4) * 4;
+  const int pl = lane >> 2, cq = (lane & 3) * 4, src = (4 * r + (lane >> 4)) * 4;
   const int SoSo = So * So;
   int y0[4], x0[4];
   const float* xb[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    const int m = m0 + 16 * i + r;
+    const int m = m0 + 16 * i + pl;
     const int mm = m < M ? m : 0;
     const int b = mm / SoSo, pix = mm - b * SoSo;
     const int oy = pix / So, ox = pix - oy * So;
     // rows past M get a start that no tap brings into the image
     y0[i] = m < M ? oy * stride - 1 : -4;
     x0[i] = ox * stride - 1;
-    xb[i] = X + (size_t)b * Si * Si * Cin + kq;
+    xb[i] = X + (size_t)b * Si * Si * Cin + cq;
   }
-  const bf16_t* bp[NT];
-  bool bok[NT];
-#pragma unroll
-  for (int j = 0; j < NT; ++j) {
-    const int n = n0 + 16 * j + r;
-    bok[j] = n < N;
-    bp[j] = B + (size_t)(bok[j] ? n : 0) * ldb + kq;
-  }
+  const int S = TAPS * (Cin / 16);
+  const bf16_t* bp = Bp + ((size_t)(n0 / 16) * S * 64 + lane) * 4;
   float4_t acc[4][NT];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < NT; ++j) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
-  const short4_t zero = {0, 0, 0, 0};
   const float4 zf4 = {0.f, 0.f, 0.f, 0.f};
-  for (int tap = TAPS == 9 ? 0 : 4; tap < (TAPS == 9 ? 9 : 5); ++tap) {
+  // k steps of 16 channels over (tap, c0), flattened: the loads of step s + 1 are in flight while the
+  // MFMAs of step s run (two register buffers)
+  int tap = TAPS == 9 ? 0 : 4, c0 = 0, ks = 0;
+  const float* ap[4];
+  bool aok[4];
+  auto setup = [&]() {
     const int ky = tap / 3, kx = tap - 3 * (tap / 3);
-    const float* ap[4];
-    bool aok[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int iy = y0[i] + ky, ix = x0[i] + kx;
@@ -343,42 +388,45 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const float* __restrict__ X, i
       // a pixel clamped into the image: a valid address without a branch, its value then dropped
       ap[i] = xb[i] + (min(max(iy, 0), Si - 1) * Si + min(max(ix, 0), Si - 1)) * Cin;
     }
-    const int kt = TAPS == 9 ? tap * Cin : 0;
-    // two channel steps per iteration, their loads issued before their MFMAs
-    int c0 = 0;
-    for (; c0 + 32 <= Cin; c0 += 32) {
-      float4 araw[2][4];
-      short4_t bq[2][NT];
+  };
+  auto load = [&](float4 (&ar)[4], short4_t (&br)[NT]) {
 #pragma unroll
-      for (int u = 0; u < 2; ++u) {
+    for (int i = 0; i < 4; ++i) ar[i] = load_or(reinterpret_cast<const float4*>(ap[i] + c0), aok[i], zf4);
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-          araw[u][i] = load_or(reinterpret_cast<const float4*>(ap[i] + c0 + 16 * u), aok[i], zf4);
-#pragma unroll
-        for (int j = 0; j < NT; ++j)
-          bq[u][j] = load_or(reinterpret_cast<const short4_t*>(bp[j] + kt + c0 + 16 * u), bok[j], zero);
-      }
-#pragma unroll
-      for (int u = 0; u < 2; ++u)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const short4_t a = bf16x4_hw(araw[u][i]);
-#pragma unroll
-          for (int j = 0; j < NT; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bq[u][j], acc[i][j], 0, 0, 0);
-        }
+    for (int j = 0; j < NT; ++j) br[j] = *reinterpret_cast<const short4_t*>(bp + ((size_t)j * S + ks) * 256);
+    ++ks;
+    c0 += 16;
+    if (c0 == Cin) {  // next tap (past the last one the addresses are computed and never used)
+      c0 = 0;
+      ++tap;
+      setup();
     }
-    if (c0 < Cin) {
-      short4_t bq[NT];
+  };
+  auto mma = [&](const float4 (&ar)[4], const short4_t (&br)[NT]) {
 #pragma unroll
-      for (int j = 0; j < NT; ++j) bq[j] = load_or(reinterpret_cast<const short4_t*>(bp[j] + kt + c0), bok[j], zero);
+    for (int i = 0; i < 4; ++i) {
+      const short4_t a = to_mfma_lanes(ar[i], src);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const short4_t a = bf16x4_hw(load_or(reinterpret_cast<const float4*>(ap[i] + c0), aok[i], zf4));
-#pragma unroll
-        for (int j = 0; j < NT; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bq[j], acc[i][j], 0, 0, 0);
-      }
+      for (int j = 0; j < NT; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, br[j], acc[i][j], 0, 0, 0);
     }
+  };
+  setup();
+  float4 a0[4], a1[4];
+  short4_t b0[NT], b1[NT];
+  load(a0, b0);
+  int s = 1;  // the next step to load; a0 holds step s - 1
+  for (; s + 1 < S; s += 2) {
+    load(a1, b1);
+    mma(a0, b0);
+    load(a0, b0);
+    mma(a1, b1);
+  }
+  if (s < S) {
+    load(a1, b1);
+    mma(a0, b0);
+    mma(a1, b1);
+  } else {
+    mma(a0, b0);
   }
 #pragma unroll
   for (int i = 0; i < 4; ++i)
@@ -405,17 +453,17 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const float* __restrict__ X, i
 // At stride 2 the pixels run in four parity classes (blockIdx.z = 2 (iy % 2) + ix % 2): within a class
 // the taps that reach a pixel are the same for every pixel (ky = 1 for even iy, ky = 0 and 2 for odd
 // iy; kx likewise), so a wave runs 1, 2 or 4 of the 9 taps and no product of a tap that cannot reach
-// it. B: Wd rows of K = 9 Cout bf16 (k_weights_dgrad). Mc = the pixels of one class (all pixels at
+// it. Bp: Wd packed by k_pack_b (PACK_DGRAD, K = 9 Cout). Mc = the pixels of one class (all pixels at
 // stride 1); Si even at stride 2. The layer's input is the previous layer's ReLU output A, so the
-// epilogue applies that ReLU's backward as well: dX = 0 where A <= 0.
+// epilogue applies that ReLU's backward as well: dX = 0 where A <= 0. A operand loads as k_conv_fwd.
 template <int NT>
 __global__ __launch_bounds__(256) void k_conv_dgrad(const float* __restrict__ dZ, int Si, int Cout, int So, int stride,
-                                                    const bf16_t* __restrict__ B, const float* __restrict__ A,
+                                                    const bf16_t* __restrict__ Bp, const float* __restrict__ A,
                                                     float* __restrict__ dX, int Mc, int N) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int m0 = blockIdx.x * 256 + wave * 64, n0 = blockIdx.y * 16 * NT;
+  const int m0 = xcd_block_x() * 256 + wave * 64, n0 = blockIdx.y * 16 * NT;
   const int r = lane & 15, kq = (lane >> 4) * 4;
-  const int ldb = 9 * Cout;
+  const int pl = lane >> 2, cq = (lane & 3) * 4, src = (4 * r + (lane >> 4)) * 4;
   // class geometry: pixel (b, q, p) of the class's Sq x Sq grid sits at (q step + py, p step + px)
   const int py = stride == 2 ? (int)(blockIdx.z >> 1) : 0, px = stride == 2 ? (int)(blockIdx.z & 1) : 0;
   const int Sq = Si / stride, SqSq = Sq * Sq;
@@ -426,7 +474,7 @@ __global__ __launch_bounds__(256) void k_conv_dgrad(const float* __restrict__ dZ
   bool mok[4];
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    const int m = m0 + 16 * i + r;
+    const int m = m0 + 16 * i + pl;
     mok[i] = m < Mc;
     const int mm = mok[i] ? m : 0;
     const int b = mm / SqSq, pix = mm - b * SqSq;
@@ -434,96 +482,89 @@ __global__ __launch_bounds__(256) void k_conv_dgrad(const float* __restrict__ dZ
     const int iy = q * stride + py, ix = p * stride + px;
     iy1[i] = iy + 1;
     ix1[i] = ix + 1;
-    zb[i] = dZ + (size_t)b * So * So * Cout + kq;
+    zb[i] = dZ + (size_t)b * So * So * Cout + cq;
     xo[i] = mok[i] ? ((b * Si + iy) * Si + ix) * N : -1;
   }
-  const bf16_t* bp[NT];
-  bool bok[NT];
-#pragma unroll
-  for (int j = 0; j < NT; ++j) {
-    const int n = n0 + 16 * j + r;
-    bok[j] = n < N;
-    bp[j] = B + (size_t)(bok[j] ? n : 0) * ldb + kq;
-  }
+  const int S = 9 * (Cout / 16);
+  const bf16_t* bp = Bp + ((size_t)(n0 / 16) * S * 64 + lane) * 4;
   float4_t acc[4][NT];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < NT; ++j) acc[i][j] = float4_t{0.f, 0.f, 0.f, 0.f};
-  const short4_t zero = {0, 0, 0, 0};
   const float4 zf4 = {0.f, 0.f, 0.f, 0.f};
-  for (int ky = ky0; ky < kyn; ky += kys)
-    for (int kx = kx0; kx < kxn; kx += kxs) {
-      const float* ap[4];
-      bool aok[4];
+  // the class's taps (nty x ntx of them) and their 16-channel k steps, flattened and pipelined as in
+  // k_conv_fwd; ks is the step's index in the packed weights (tap (ky, kx) starts at (3 ky + kx) Cout / 16)
+  const int ntx = (kxn - kx0 + kxs - 1) / kxs, ntaps = ((kyn - ky0 + kys - 1) / kys) * ntx;
+  int tq = 0, c0 = 0, ks = 0;
+  const float* ap[4];
+  bool aok[4];
+  auto setup = [&]() {
+    const int ky = ky0 + kys * (tq / ntx), kx = kx0 + kxs * (tq % ntx);
+    ks = (ky * 3 + kx) * (Cout / 16);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        // (iy + 1 - ky) is a multiple of the stride for every tap of the class
-        const int ty = iy1[i] - ky, tx = ix1[i] - kx;
-        const int oy = stride == 2 ? ty >> 1 : ty, ox = stride == 2 ? tx >> 1 : tx;
-        aok[i] = mok[i] && ty >= 0 && tx >= 0 && oy < So && ox < So;
-        ap[i] = zb[i] + (min(max(oy, 0), So - 1) * So + min(max(ox, 0), So - 1)) * Cout;  // clamped, as above
-      }
-      const int kt = (ky * 3 + kx) * Cout;
-      int c0 = 0;
-      for (; c0 + 32 <= Cout; c0 += 32) {
-        float4 araw[2][4];
-        short4_t bq[2][NT];
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-#pragma unroll
-          for (int i = 0; i < 4; ++i)
-            araw[u][i] = load_or(reinterpret_cast<const float4*>(ap[i] + c0 + 16 * u), aok[i], zf4);
-#pragma unroll
-          for (int j = 0; j < NT; ++j)
-            bq[u][j] = load_or(reinterpret_cast<const short4_t*>(bp[j] + kt + c0 + 16 * u), bok[j], zero);
-        }
-#pragma unroll
-        for (int u = 0; u < 2; ++u)
-#pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const short4_t a = bf16x4_hw(araw[u][i]);
-#pragma unroll
-            for (int j = 0; j < NT; ++j)
-              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bq[u][j], acc[i][j], 0, 0, 0);
-          }
-      }
-      if (c0 < Cout) {
-        short4_t bq[NT];
-#pragma unroll
-        for (int j = 0; j < NT; ++j) bq[j] = load_or(reinterpret_cast<const short4_t*>(bp[j] + kt + c0), bok[j], zero);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const short4_t a = bf16x4_hw(load_or(reinterpret_cast<const float4*>(ap[i] + c0), aok[i], zf4));
-#pragma unroll
-          for (int j = 0; j < NT; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bq[j], acc[i][j], 0, 0, 0);
-        }
-      }
+    for (int i = 0; i < 4; ++i) {
+      // (iy + 1 - ky) is a multiple of the stride for every tap of the class
+      const int ty = iy1[i] - ky, tx = ix1[i] - kx;
+      const int oy = stride == 2 ? ty >> 1 : ty, ox = stride == 2 ? tx >> 1 : tx;
+      aok[i] = mok[i] && ty >= 0 && tx >= 0 && oy < So && ox < So;
+      // an output pixel clamped into the image: a valid address without a branch, its value then dropped
+      ap[i] = zb[i] + (min(max(oy, 0), So - 1) * So + min(max(ox, 0), So - 1)) * Cout;
     }
-  // D rows are the pixels m0 + 16 i + kq + e: their output offsets (-1 past Mc) come from lane kq + e,
-  // which decoded that pixel
+  };
+  auto load = [&](float4 (&ar)[4], short4_t (&br)[NT]) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) ar[i] = load_or(reinterpret_cast<const float4*>(ap[i] + c0), aok[i], zf4);
+#pragma unroll
+    for (int j = 0; j < NT; ++j) br[j] = *reinterpret_cast<const short4_t*>(bp + ((size_t)j * S + ks) * 256);
+    ++ks;
+    c0 += 16;
+    if (c0 == Cout) {  // next tap (past the last one the addresses are computed and never used)
+      c0 = 0;
+      ++tq;
+      setup();
+    }
+  };
+  auto mma = [&](const float4 (&ar)[4], const short4_t (&br)[NT]) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const short4_t a = to_mfma_lanes(ar[i], src);
+#pragma unroll
+      for (int j = 0; j < NT; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, br[j], acc[i][j], 0, 0, 0);
+    }
+  };
+  const int steps = ntaps * (Cout / 16);
+  setup();
+  float4 a0[4], a1[4];
+  short4_t b0[NT], b1[NT];
+  load(a0, b0);
+  int s = 1;  // the next step to load; a0 holds step s - 1
+  for (; s + 1 < steps; s += 2) {
+    load(a1, b1);
+    mma(a0, b0);
+    load(a0, b0);
+    mma(a1, b1);
+  }
+  if (s < steps) {
+    load(a1, b1);
+    mma(a0, b0);
+    mma(a1, b1);
+  } else {
+    mma(a0, b0);
+  }
+  // D rows are the pixels m0 + 16 i + kq + e: their output offsets (-1 past Mc) come from lane
+  // 4 (kq + e), which decoded that pixel
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      const int o = __shfl(xo[i], kq + e);
+      const int o = __shfl(xo[i], 4 * (kq + e));
 #pragma unroll
       for (int j = 0; j < NT; ++j) {
         const int n = n0 + 16 * j + r;
         if (o >= 0 && n < N) dX[(size_t)o + n] = A[(size_t)o + n] > 0.f ? acc[i][j][e] : 0.f;
       }
     }
-}
-
-// Wd[c][tap Cout + co] = bf16(W[co][tap Cin + c]): the weights as the data gradient's B operand
-__global__ void k_weights_dgrad(const float* __restrict__ W, int Cin, int Cout, bf16_t* __restrict__ Wd) {
-  const int total = Cin * 9 * Cout;
-  for (int e = blockIdx.x * 256 + threadIdx.x; e < total; e += gridDim.x * 256) {
-    const int c = e / (9 * Cout), rr = e - c * 9 * Cout;
-    const int tap = rr / Cout, co = rr - tap * Cout;
-    Wd[e] = f2bf(W[(size_t)co * 9 * Cin + tap * Cin + c]);
-  }
 }
 
 // Weight gradient of a 3x3 convolution (TAPS = 9) or of the 1x1 head (TAPS = 1: the centre tap, stride
@@ -1118,51 +1159,61 @@ void gemm(hipStream_t st, Buf<float>& parts, const bf16_t* A, int lda, const bf1
 
 // Output channels per wave tile: 16 NT, NT = 4 from 64 channels up.
 inline int conv_nt(int n) { return n >= 64 ? 4 : (n + 15) / 16; }
+// 16-row blocks of a packed operand with N rows: a multiple of 4, so a wave's tiles stay inside.
+inline int packed_blocks(int N) { return ((N + 15) / 16 + 3) / 4 * 4; }
+inline size_t packed_size(int N, int K) { return (size_t)packed_blocks(N) * 16 * K; }
+// B packed in MFMA operand order (k_pack_b) into Bp, packed_size(N, K) elements, K % 16 == 0.
+template <int MODE>
+void pack_b(hipStream_t st, const float* W, int N, int K, int ldw, int Kw, int Cin, int Cout, bf16_t* Bp) {
+  hipLaunchKernelGGL((k_pack_b<MODE>), dim3(grid1((long long)packed_size(N, K))), dim3(256), 0, st, W, N, K, ldw, Kw,
+                     Cin, Cout, packed_blocks(N), Bp);
+  check_launch("k_pack_b");
+}
 template <int NT, int TAPS, int EPI>
-void conv_fwd_nt(hipStream_t st, const float* X, int Si, int Cin, int So, int stride, const bf16_t* W, int ldw, float* Y,
+void conv_fwd_nt(hipStream_t st, const float* X, int Si, int Cin, int So, int stride, const bf16_t* Bp, float* Y,
                  int M, int N, const float* bias, const float* A) {
   const dim3 g((unsigned)((M + 255) / 256), (unsigned)((N + 16 * NT - 1) / (16 * NT)));
-  hipLaunchKernelGGL((k_conv_fwd<NT, TAPS, EPI>), g, dim3(256), 0, st, X, Si, Cin, So, stride, W, ldw, Y, M, N, bias,
-                     A);
+  hipLaunchKernelGGL((k_conv_fwd<NT, TAPS, EPI>), g, dim3(256), 0, st, X, Si, Cin, So, stride, Bp, Y, M, N, bias, A);
   check_launch("k_conv_fwd");
 }
 // A 3x3 (TAPS 9) or 1x1 (TAPS 1) layer as an implicit product without an im2col matrix (Cin % 16 == 0),
-// with the epilogue EPI (bias and ReLU, bias, or the mask of A's ReLU).
+// B packed by pack_b<PACK_ROWS or PACK_COLS>(N rows, K = TAPS Cin), with the epilogue EPI (bias and
+// ReLU, bias, or the mask of A's ReLU).
 template <int TAPS, int EPI>
-void conv_fwd(hipStream_t st, const float* X, int Si, int Cin, int So, int stride, const bf16_t* W, int ldw, float* Y,
+void conv_fwd(hipStream_t st, const float* X, int Si, int Cin, int So, int stride, const bf16_t* Bp, float* Y,
               long long M, int N, const float* bias, const float* A = nullptr) {
-  CNN_REQUIRE(Cin % 16 == 0 && ldw % 4 == 0, TM_ERR_INVALID, "conv: Cin must be a multiple of 16");
+  CNN_REQUIRE(Cin % 16 == 0, TM_ERR_INVALID, "conv: Cin must be a multiple of 16");
   CNN_REQUIRE((long long)Si * Si * Cin < (1LL << 31) && M * N < (1LL << 31), TM_ERR_INVALID, "conv: image too large");
   switch (conv_nt(N)) {
-    case 1: conv_fwd_nt<1, TAPS, EPI>(st, X, Si, Cin, So, stride, W, ldw, Y, (int)M, N, bias, A); break;
-    case 2: conv_fwd_nt<2, TAPS, EPI>(st, X, Si, Cin, So, stride, W, ldw, Y, (int)M, N, bias, A); break;
-    case 3: conv_fwd_nt<3, TAPS, EPI>(st, X, Si, Cin, So, stride, W, ldw, Y, (int)M, N, bias, A); break;
-    default: conv_fwd_nt<4, TAPS, EPI>(st, X, Si, Cin, So, stride, W, ldw, Y, (int)M, N, bias, A); break;
+    case 1: conv_fwd_nt<1, TAPS, EPI>(st, X, Si, Cin, So, stride, Bp, Y, (int)M, N, bias, A); break;
+    case 2: conv_fwd_nt<2, TAPS, EPI>(st, X, Si, Cin, So, stride, Bp, Y, (int)M, N, bias, A); break;
+    case 3: conv_fwd_nt<3, TAPS, EPI>(st, X, Si, Cin, So, stride, Bp, Y, (int)M, N, bias, A); break;
+    default: conv_fwd_nt<4, TAPS, EPI>(st, X, Si, Cin, So, stride, Bp, Y, (int)M, N, bias, A); break;
   }
 }
 
-// Data gradient of a 3x3 layer into dX[B][Si][Si][Cin] (k_conv_dgrad; Cout % 16 == 0), the weights
-// already in Wd (k_weights_dgrad), masked by the ReLU that produced the layer's input A. At stride 2
-// one grid layer per pixel-parity class.
+// Data gradient of a 3x3 layer into dX[B][Si][Si][Cin] (k_conv_dgrad; Cout % 16 == 0), its weights
+// packed by pack_b<PACK_DGRAD>, masked by the ReLU that produced the layer's input A. At stride 2 one
+// grid layer per pixel-parity class.
 template <int NT>
-void conv_dgrad_nt(hipStream_t st, const float* dZ, const Layer& L, int batch, const bf16_t* Wd, const float* A,
+void conv_dgrad_nt(hipStream_t st, const float* dZ, const Layer& L, int batch, const bf16_t* Bp, const float* A,
                    float* dX) {
   const int Sq = L.si / L.stride;
   const int Mc = batch * Sq * Sq;
   const dim3 g((unsigned)((Mc + 255) / 256), (unsigned)((L.cin + 16 * NT - 1) / (16 * NT)), L.stride == 2 ? 4 : 1);
-  hipLaunchKernelGGL((k_conv_dgrad<NT>), g, dim3(256), 0, st, dZ, L.si, L.cout, L.so, L.stride, Wd, A, dX, Mc, L.cin);
+  hipLaunchKernelGGL((k_conv_dgrad<NT>), g, dim3(256), 0, st, dZ, L.si, L.cout, L.so, L.stride, Bp, A, dX, Mc, L.cin);
   check_launch("k_conv_dgrad");
 }
-void conv_dgrad(hipStream_t st, const float* dZ, const Layer& L, int batch, const bf16_t* Wd, const float* A,
+void conv_dgrad(hipStream_t st, const float* dZ, const Layer& L, int batch, const bf16_t* Bp, const float* A,
                 float* dX) {
   CNN_REQUIRE(L.cout % 16 == 0 && L.si % L.stride == 0, TM_ERR_INVALID, "dgrad: Cout must be a multiple of 16");
   CNN_REQUIRE((long long)L.so * L.so * L.cout < (1LL << 31) && (long long)batch * L.si * L.si * L.cin < (1LL << 31),
               TM_ERR_INVALID, "dgrad: layer too large");
   switch (conv_nt(L.cin)) {
-    case 1: conv_dgrad_nt<1>(st, dZ, L, batch, Wd, A, dX); break;
-    case 2: conv_dgrad_nt<2>(st, dZ, L, batch, Wd, A, dX); break;
-    case 3: conv_dgrad_nt<3>(st, dZ, L, batch, Wd, A, dX); break;
-    default: conv_dgrad_nt<4>(st, dZ, L, batch, Wd, A, dX); break;
+    case 1: conv_dgrad_nt<1>(st, dZ, L, batch, Bp, A, dX); break;
+    case 2: conv_dgrad_nt<2>(st, dZ, L, batch, Bp, A, dX); break;
+    case 3: conv_dgrad_nt<3>(st, dZ, L, batch, Bp, A, dX); break;
+    default: conv_dgrad_nt<4>(st, dZ, L, batch, Bp, A, dX); break;
   }
 }
 
@@ -1303,16 +1354,17 @@ void forward_layer(tm_cnn* h, int l) {
   hipStream_t st = h->stream;
   const bool head = l == (int)h->layers.size() - 1;
   const long long M = rows_of(h, L);
-  to_bf16_pad(st, h->w.p + L.w_off, L.cout, L.K, L.K, h->wb.p, L.Kp);
   if (L.cin % 16 == 0) {
+    pack_b<PACK_ROWS>(st, h->w.p + L.w_off, L.cout, L.K, L.K, L.K, 0, 0, h->wb.p);
     if (head)
-      conv_fwd<1, EPI_BIAS>(st, h->act[l].p, L.si, L.cin, L.so, 1, h->wb.p, L.Kp, h->act[l + 1].p, M, L.cout,
+      conv_fwd<1, EPI_BIAS>(st, h->act[l].p, L.si, L.cin, L.so, 1, h->wb.p, h->act[l + 1].p, M, L.cout,
                             h->w.p + L.b_off);
     else
-      conv_fwd<9, EPI_BIAS_RELU>(st, h->act[l].p, L.si, L.cin, L.so, L.stride, h->wb.p, L.Kp, h->act[l + 1].p, M,
-                                 L.cout, h->w.p + L.b_off);
+      conv_fwd<9, EPI_BIAS_RELU>(st, h->act[l].p, L.si, L.cin, L.so, L.stride, h->wb.p, h->act[l + 1].p, M, L.cout,
+                                 h->w.p + L.b_off);
     return;
   }
+  to_bf16_pad(st, h->w.p + L.w_off, L.cout, L.K, L.K, h->wb.p, L.Kp);
   if (head)
     to_bf16_pad(st, h->act[l].p, M, L.cin, L.cin, h->col.p, L.Kp);
   else
@@ -1334,6 +1386,10 @@ void alloc_for_batch(tm_cnn* h, int batch) {
     grad = std::max(grad, (size_t)((long long)batch * L.si * L.si * L.cin));
     dcol = std::max(dcol, (size_t)(M * L.K));
     wmax = std::max(wmax, (size_t)round16(L.cout) * L.Kp);
+    // packed operands: the forward's weights, the data gradient's (3x3: 9 Cout per input channel;
+    // head: its padded outputs per input channel)
+    wmax = std::max(wmax, packed_size(L.cout, L.Kp));
+    wmax = std::max(wmax, packed_size(L.cin, 9 * round16(L.cout)));
     h->act[l + 1].ensure((size_t)(M * L.cout));
   }
   const Layer& L0 = h->layers[0];
@@ -1556,9 +1612,7 @@ int tm_cnn_backward(tm_cnn* h, const float* heat_target, const int* bin_target, 
       if (l == 0) break;
       if (!head && L.cout % 16 == 0) {
         // implicit transposed convolution straight into the input's gradient
-        hipLaunchKernelGGL(k_weights_dgrad, dim3(grid1((long long)L.cin * 9 * L.cout)),
-                           dim3(256), 0, st, h->w.p + L.w_off, L.cin, L.cout, h->wbT.p);
-        check_launch("k_weights_dgrad");
+        pack_b<PACK_DGRAD>(st, h->w.p + L.w_off, L.cin, 9 * L.cout, 0, 0, L.cin, L.cout, h->wbT.p);
         conv_dgrad(st, dcur, L, B, h->wbT.p, h->act[l].p, dnext);
         masked = true;
         dcur = dnext;
@@ -1567,16 +1621,17 @@ int tm_cnn_backward(tm_cnn* h, const float* heat_target, const int* bin_target, 
       }
       // dcol[M][K] = dZ[M][cout16] . W^T[K][cout16]^T, then the gather into the input's gradient
       const int c16 = round16(L.cout);
-      transpose_bf16(st, h->w.p + L.w_off, L.cout, L.K, L.K, h->wbT.p, c16, L.K, c16);
       if (head && L.cin % 16 == 0) {
-        // the head's data gradient as a 1x1 implicit product over its padded dL/dlogits rows, with
-        // the last convolution's ReLU backward in the epilogue
-        conv_fwd<1, EPI_MASK>(st, dcur, L.si, ldz, L.so, 1, h->wbT.p, c16, dnext, M, L.cin, nullptr, h->act[l].p);
+        // the head's data gradient as a 1x1 implicit product over its padded dL/dlogits rows (B = W^T,
+        // zero past the head's outputs), with the last convolution's ReLU backward in the epilogue
+        pack_b<PACK_COLS>(st, h->w.p + L.w_off, L.cin, ldz, L.cin, L.cout, 0, 0, h->wbT.p);
+        conv_fwd<1, EPI_MASK>(st, dcur, L.si, ldz, L.so, 1, h->wbT.p, dnext, M, L.cin, nullptr, h->act[l].p);
         masked = true;
         dcur = dnext;
         dnext = (dnext == h->dA0.p) ? h->dA1.p : h->dA0.p;
         continue;
       }
+      transpose_bf16(st, h->w.p + L.w_off, L.cout, L.K, L.K, h->wbT.p, c16, L.K, c16);
       to_bf16_pad(st, dcur, M, L.cout, ldz, h->dzb.p, c16);
       if (head) {
         gemm(st, h->parts, h->dzb.p, c16, h->wbT.p, c16, dnext, L.K, M, L.K, c16, nullptr, false);
