@@ -574,7 +574,9 @@ __global__ __launch_bounds__(256) void k_conv_dgrad(const float* __restrict__ dZ
 // without the im2col matrix or either transpose. dZ rows are ldz floats apart, ldz a multiple of
 // 16 MT: Cout need not be a multiple of 16 when columns Cout .. ldz of dZ are zero (the head's padded
 // dL/dlogits), the last tile then writing only its channels below Cout. Element offsets into X and
-// dZ are 32-bit (the host checks both sizes). The product's k runs
+// dZ are 32-bit (the host checks both sizes). TAPS = 0 is the first layer (Cin = 1): the nine taps
+// are the product's n (one 16-wide tile, lane r holding tap r < 9 and reading its own shifted pixels),
+// parts[z][co][tap]. The product's k runs
 // over pixels, in 16-pixel segments of one output row: a lane's four pixels are consecutive, so its
 // four dZ values lie Cout floats apart and its four X values stride Cin floats apart, at constant
 // offsets from one address (only the image's left and right edges mask a pixel). A block covers one
@@ -591,11 +593,11 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const float* __restrict__ X,
   const int xcd = blockIdx.x & 7, jx = blockIdx.x >> 3;
   const int z = xcd * (nsplit >> 3) + jx / ntiles;
   int t = jx % ntiles;
-  const int nc = Cin / (16 * NT);
-  const int cm = t / (TAPS * nc);
-  t -= cm * TAPS * nc;
+  const int nc = TAPS == 0 ? 1 : Cin / (16 * NT);
+  const int cm = TAPS == 0 ? t : t / (TAPS * nc);
+  t -= TAPS == 0 ? t : cm * TAPS * nc;
   const int tapi = t / nc, cc = t - tapi * nc;
-  const int tap = TAPS == 9 ? tapi : 4;
+  const int tap = TAPS == 9 ? tapi : TAPS == 1 ? 4 : min(lane & 15, 8);
   const int ky = tap / 3, kx = tap - 3 * ky;
   const int co0 = cm * 16 * MT, ci0 = cc * 16 * NT;
   const int r = lane & 15, pq = (lane >> 4) * 4;
@@ -618,7 +620,17 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const float* __restrict__ X,
       for (int e = 0; e < 4; ++e) za[i][e] = dZ[zo + e * ldz + 16 * i];
     const int iy = oy * stride + ky - 1;
     const int ix0 = ox * stride + kx - 1;
-    if (iy >= 0 && iy < Si) {
+    if constexpr (TAPS == 0) {
+      // lane r's tap: its row is its own (clamped into the image, the value then dropped)
+      const bool rok = r < 9 && iy >= 0 && iy < Si;
+      const int base = (b * Si + min(max(iy, 0), Si - 1)) * Si;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int ix = ix0 + e * stride;
+        const float v = X[base + min(max(ix, 0), Si - 1)];
+        xa[0][e] = rok && ix >= 0 && ix < Si ? v : 0.f;
+      }
+    } else if (iy >= 0 && iy < Si) {
       const int base = ((b * Si + iy) * Si + ix0) * Cin + ci0 + r;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
@@ -672,7 +684,7 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const float* __restrict__ X,
   }
   __syncthreads();
   if (wave != 0) return;
-  const int K = TAPS * Cin;
+  const int K = TAPS == 0 ? 9 : TAPS * Cin;
   float* out = parts + (size_t)z * Cout * K + (size_t)tapi * Cin;
 #pragma unroll
   for (int i = 0; i < MT; ++i)
@@ -684,7 +696,7 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const float* __restrict__ X,
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int co = co0 + 16 * i + pq + e;
-        if (co < Cout) out[(size_t)co * K + ci0 + 16 * j + r] = s[e];
+        if (co < Cout && (TAPS != 0 || r < 9)) out[(size_t)co * K + ci0 + 16 * j + r] = s[e];
       }
     }
 }
@@ -1252,7 +1264,7 @@ void colsum(hipStream_t st, Buf<float>& part, const float* X, long long R, int C
 template <int MT, int NT, int TAPS>
 void conv_wgrad_t(hipStream_t st, Buf<float>& parts, const float* X, const Layer& L, int rows, const float* dZ,
                   int ldz, float* dW) {
-  const int ntiles = TAPS * (L.cin / (16 * NT)) * ((L.cout + 16 * MT - 1) / (16 * MT));
+  const int ntiles = (TAPS == 0 ? 1 : TAPS * (L.cin / (16 * NT))) * ((L.cout + 16 * MT - 1) / (16 * MT));
   const long long per = (long long)L.cout * L.K;
   long long ns = std::max<long long>(1, 2048 / ntiles);
   ns = std::min<long long>(ns, rows);
@@ -1272,7 +1284,9 @@ template <int MT, int TAPS>
 void conv_wgrad_m(hipStream_t st, Buf<float>& parts, const float* X, const Layer& L, int rows, const float* dZ,
                   int ldz, float* dW) {
   const int q = L.cin / 16;
-  if (q % 4 == 0)
+  if constexpr (TAPS == 0)
+    conv_wgrad_t<MT, 1, 0>(st, parts, X, L, rows, dZ, ldz, dW);
+  else if (q % 4 == 0)
     conv_wgrad_t<MT, 4, TAPS>(st, parts, X, L, rows, dZ, ldz, dW);
   else if (q % 3 == 0)
     conv_wgrad_t<MT, 3, TAPS>(st, parts, X, L, rows, dZ, ldz, dW);
@@ -1285,8 +1299,8 @@ void conv_wgrad_m(hipStream_t st, Buf<float>& parts, const float* X, const Layer
 // layer), output rows in 16-pixel segments, X and dZ (ldz floats per row) under 2^31 elements.
 bool wgrad_implicit(const Layer& L, int batch, int ldz, bool head) {
   const long long M = (long long)batch * L.so * L.so;
-  return L.cin % 16 == 0 && (head || L.cout % 16 == 0) && ldz % 16 == 0 && L.so % 16 == 0 &&
-         M * ldz < (1LL << 31) && (long long)batch * L.si * L.si * L.cin < (1LL << 31);
+  return (L.cin % 16 == 0 || (L.cin == 1 && !head)) && (head || L.cout % 16 == 0) && ldz % 16 == 0 &&
+         L.so % 16 == 0 && M * ldz < (1LL << 31) && (long long)batch * L.si * L.si * L.cin < (1LL << 31);
 }
 // dZ rows ldz floats apart (ldz a multiple of 16, zeros past cout); tiles of 16 MT output channels
 // with MT dividing ldz / 16, so that no tile reads past a row.
@@ -1595,6 +1609,8 @@ int tm_cnn_backward(tm_cnn* h, const float* heat_target, const int* bin_target, 
       if (wgrad_implicit(L, B, ldz, head)) {
         if (head)
           conv_wgrad<1>(st, h->parts, h->act[l].p, L, B, dcur, ldz, h->g.p + L.w_off);
+        else if (L.cin == 1)
+          conv_wgrad<0>(st, h->parts, h->act[l].p, L, B, dcur, ldz, h->g.p + L.w_off);
         else
           conv_wgrad<9>(st, h->parts, h->act[l].p, L, B, dcur, ldz, h->g.p + L.w_off);
       } else {
