@@ -596,17 +596,49 @@ def bench_cnn(args):
     heat = (rng.random((B, G, G)) < 0.003).astype(np.float32)
     bins = np.where(heat > 0, rng.integers(0, spec.n_bins, (B, G, G)), -1).astype(np.int32)
 
+    # the batch and its targets resident in device memory (HIP runtime calls through ctypes)
+    import ctypes
+    hip = ctypes.CDLL(os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib", "libamdhip64.so"))
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    dev = []
+    for a in (imgs, heat, bins):
+        ptr = ctypes.c_void_p()
+        if hip.hipMalloc(ctypes.byref(ptr), a.nbytes) != 0 or hip.hipMemcpy(ptr, a.ctypes.data, a.nbytes, 1) != 0:
+            raise RuntimeError("hipMalloc / hipMemcpy failed")
+        dev.append(ptr.value)
+    n_lab = int((bins >= 0).sum())
+
     def step():
+        net.forward_device(dev[0], B)
+        net.backward_device(dev[1], dev[2], n_lab, pos_weight=30.0)
+        net.adam_step(lr=1e-3)
+
+    def step_host():
         net.forward(imgs, want_logits=False)
         net.backward(heat, bins, pos_weight=30.0)
         net.adam_step(lr=1e-3)
 
     for _ in range(max(1, args.warmup)):
         step()
+    net.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
+    net.synchronize()
     dt = time.perf_counter() - t0
+    # the same steps through the host entry points (a PCIe copy of the batch and targets each step,
+    # and a wait after each call): reported beside the device-resident rate, never as the value
+    step_host()
+    n_host = max(1, args.steps // 2)
+    t1 = time.perf_counter()
+    for _ in range(n_host):
+        step_host()
+    net.synchronize()
+    dt_host = time.perf_counter() - t1
+    for ptr in dev:
+        hip.hipFree(ctypes.c_void_p(ptr))
     flops = cnn_step_flops(spec, B) * args.steps
     tflops = flops / dt / 1e12
     peak = 2500.0
@@ -614,8 +646,10 @@ def bench_cnn(args):
            "value": round(B * args.steps / dt, 2), "unit": "images/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": round(1e3 * dt / args.steps, 3), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "bf16 products, fp32 accumulation and weights",
-           "data": "synthetic (seeded images and targets)",
+           "data": "synthetic (seeded images and targets, resident in device memory)",
            "config": {"workload": "cnn_train_512sq_b16", "batch": B, "params": net.n_params, "grid": G},
+           "host_input": {"images_per_s": round(B * n_host / dt_host, 2),
+                          "ms_per_step": round(1e3 * dt_host / n_host, 3)},
            "roofline": {"bound": "mfma", "achieved": round(tflops, 2), "peak": peak, "unit": "TFLOP/s",
                         "frac": round(tflops / peak, 5), "traffic": None,
                         "kernel": "whole step (every product of the step counted once)",

@@ -16,7 +16,8 @@ I32P = core.I32P
 MAX_LAYERS = 16
 CNN_EXPORTS = ["tm_cnn_create", "tm_cnn_destroy", "tm_cnn_shape", "tm_cnn_init_params", "tm_cnn_set_params",
                "tm_cnn_get_params", "tm_cnn_forward", "tm_cnn_backward", "tm_cnn_adam_step", "tm_cnn_get_activation",
-               "tm_cnn_debug_gemm", "tm_cnn_last_error", "tm_cnn_heat", "tm_cnn_cell_logits"]
+               "tm_cnn_debug_gemm", "tm_cnn_last_error", "tm_cnn_heat", "tm_cnn_cell_logits", "tm_cnn_forward_device",
+               "tm_cnn_backward_device", "tm_cnn_loss", "tm_cnn_synchronize"]
 
 
 class CnnConfig(ctypes.Structure):
@@ -39,6 +40,10 @@ def _lib():
         L.tm_cnn_forward.argtypes = [vp, F32P, ctypes.c_int, F32P]
         L.tm_cnn_backward.argtypes = [vp, F32P, I32P, ctypes.c_float, F32P, F32P]
         L.tm_cnn_adam_step.argtypes = [vp] + [ctypes.c_float] * 4
+        L.tm_cnn_forward_device.argtypes = [vp, vp, ctypes.c_int]
+        L.tm_cnn_backward_device.argtypes = [vp, vp, vp, ctypes.c_longlong, ctypes.c_float]
+        L.tm_cnn_loss.argtypes = [vp, F32P]
+        L.tm_cnn_synchronize.argtypes = [vp]
         L.tm_cnn_heat.argtypes = [vp, F32P]
         L.tm_cnn_cell_logits.argtypes = [vp, I32P, ctypes.c_int, F32P]
         L.tm_cnn_get_activation.argtypes = [vp, ctypes.c_int, F32P]
@@ -128,6 +133,11 @@ class ProposalNet:
         _check(_lib().tm_cnn_forward(self.h, _f(x), B, _f(out)))
         return out
 
+    def forward_device(self, images_ptr: int, batch: int) -> None:
+        """Forward pass over batch images already in device memory (a device address of batch x S x S
+        FP32 values), enqueued without a copy or a wait (tm_cnn_forward_device)."""
+        _check(_lib().tm_cnn_forward_device(self.h, ctypes.c_void_p(images_ptr), batch))
+
     def heat(self, batch: int = 1) -> np.ndarray:
         """Location logits of the last forward pass, (batch, grid, grid)."""
         out = np.zeros((batch, self.grid, self.grid), np.float32)
@@ -149,6 +159,21 @@ class ProposalNet:
         g = np.zeros(self.n_params, np.float32) if want_grads else None
         _check(_lib().tm_cnn_backward(self.h, _f(ht), bt.ctypes.data_as(I32P), pos_weight, _f(loss), _f(g)))
         return (float(loss[0]), float(loss[1])), g
+
+    def backward_device(self, heat_ptr: int, bins_ptr: int, n_labelled: int, pos_weight: float = 1.0) -> None:
+        """Backward pass from targets in device memory (FP32 heat and int32 bins, batch x grid x grid),
+        enqueued without a wait; n_labelled = the number of cells with a bin (tm_cnn_backward_device)."""
+        _check(_lib().tm_cnn_backward_device(self.h, ctypes.c_void_p(heat_ptr), ctypes.c_void_p(bins_ptr), n_labelled,
+                                             pos_weight))
+
+    def loss(self):
+        """Both loss terms of the last backward pass (waits for it)."""
+        out = np.zeros(2, np.float32)
+        _check(_lib().tm_cnn_loss(self.h, _f(out)))
+        return float(out[0]), float(out[1])
+
+    def synchronize(self) -> None:
+        _check(_lib().tm_cnn_synchronize(self.h))
 
     def adam_step(self, lr: float = 1e-3, beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-8) -> None:
         _check(_lib().tm_cnn_adam_step(self.h, lr, beta1, beta2, eps))

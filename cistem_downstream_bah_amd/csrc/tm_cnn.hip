@@ -949,7 +949,8 @@ __global__ __launch_bounds__(256) void k_std_part(const float* __restrict__ X, l
     part[((long long)b * STD_PARTS + p) * 2 + 1] = s2[0];
   }
 }
-__global__ __launch_bounds__(256) void k_std_apply(float* __restrict__ X, long long npx,
+// Y = (X - mean) / std per image; X and Y may be the same buffer.
+__global__ __launch_bounds__(256) void k_std_apply(const float* X, float* Y, long long npx,
                                                    const double* __restrict__ part) {
   __shared__ float mi[2];
   const int b = blockIdx.y;
@@ -966,9 +967,10 @@ __global__ __launch_bounds__(256) void k_std_apply(float* __restrict__ X, long l
   }
   __syncthreads();
   const float m = mi[0], inv = mi[1];
-  float* x = X + b * npx;
+  const float* x = X + b * npx;
+  float* y = Y + b * npx;
   for (long long e = blockIdx.x * 256LL + threadIdx.x; e < npx; e += (long long)gridDim.x * 256)
-    x[e] = (x[e] - m) * inv;
+    y[e] = (x[e] - m) * inv;
 }
 
 __device__ inline float softplus(float x) { return fmaxf(x, 0.f) + log1pf(expf(-fabsf(x))); }
@@ -987,7 +989,7 @@ __global__ __launch_bounds__(64) void k_loss(const float* __restrict__ logits, i
   const int lane = threadIdx.x;
   for (int k = Nh + lane; k < ldd; k += 64) d[k] = 0.f;  // the row's padding
   const int nb = Nh - 1;
-  const int tb = bin_t[cell];
+  const int tb = bin_t[cell] < nb ? bin_t[cell] : -1;  // a bin past the last counts as none
   if (lane == 0) {
     const float z = o[0], y = heat_t[cell];
     cell_loss[2 * cell] = pos_w * y * softplus(-z) + (1.f - y) * softplus(z);
@@ -1344,6 +1346,7 @@ struct tm_cnn {
   Buf<float> gathered;
   Buf<double> std_part;
   std::vector<float> h_loss;
+  long long loss_cells = 0, loss_bins = 0;  // cells and labelled cells of the last backward pass
 
   ~tm_cnn() {
     if (stream) {
@@ -1423,6 +1426,11 @@ void alloc_for_batch(tm_cnn* h, int batch) {
   h->heat_t.ensure((size_t)cells);
   h->bin_t.ensure((size_t)cells);
 }
+
+// defined after the entry points that use them
+void read_loss(tm_cnn* h, float* loss);
+void forward_impl(tm_cnn* h, const float* src, int batch);
+void backward_impl(tm_cnn* h, const float* heat_t, const int* bin_t, long long n_bin, float pos_weight);
 
 }  // namespace
 
@@ -1552,22 +1560,24 @@ int tm_cnn_forward(tm_cnn* h, const float* images, int batch, float* logits) {
     require(h);
     CNN_REQUIRE(images, TM_ERR_INVALID, "images is NULL");
     CNN_REQUIRE(batch >= 1 && batch <= h->cfg.max_batch, TM_ERR_INVALID, "batch must be 1..max_batch");
-    h->batch = batch;
     const long long npx = (long long)h->cfg.input_size * h->cfg.input_size;
     CNN_HIP(hipMemcpyAsync(h->act[0].p, images, batch * npx * sizeof(float), hipMemcpyHostToDevice, h->stream));
-    h->std_part.ensure((size_t)batch * STD_PARTS * 2);
-    hipLaunchKernelGGL(k_std_part, dim3(STD_PARTS, batch), dim3(256), 0, h->stream, h->act[0].p, npx, h->std_part.p);
-    check_launch("k_std_part");
-    hipLaunchKernelGGL(k_std_apply, dim3((unsigned)std::min<long long>((npx + 1023) / 1024, 1024), batch), dim3(256), 0,
-                       h->stream, h->act[0].p, npx, h->std_part.p);
-    check_launch("k_std_apply");
-    for (int l = 0; l < (int)h->layers.size(); ++l) forward_layer(h, l);
+    forward_impl(h, h->act[0].p, batch);
     if (logits) {
       const long long cells = (long long)batch * h->G * h->G;
       CNN_HIP(hipMemcpyAsync(logits, h->act.back().p, cells * h->Nh * sizeof(float), hipMemcpyDeviceToHost,
                              h->stream));
     }
     CNN_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+
+int tm_cnn_forward_device(tm_cnn* h, const float* images, int batch) {
+  return cguard([&] {
+    require(h);
+    CNN_REQUIRE(images, TM_ERR_INVALID, "images is NULL");
+    CNN_REQUIRE(batch >= 1 && batch <= h->cfg.max_batch, TM_ERR_INVALID, "batch must be 1..max_batch");
+    forward_impl(h, images, batch);
   });
 }
 
@@ -1578,8 +1588,7 @@ int tm_cnn_backward(tm_cnn* h, const float* heat_target, const int* bin_target, 
     CNN_REQUIRE(heat_target && bin_target, TM_ERR_INVALID, "NULL target");
     CNN_REQUIRE(h->batch > 0, TM_ERR_STATE, "no forward pass to differentiate");
     hipStream_t st = h->stream;
-    const int B = h->batch, Nh = h->Nh;
-    const long long cells = (long long)B * h->G * h->G;
+    const long long cells = (long long)h->batch * h->G * h->G;
     long long n_bin = 0;
     for (long long c = 0; c < cells; ++c) {
       CNN_REQUIRE(bin_target[c] < h->cfg.n_bins, TM_ERR_INVALID, "bin target out of range");
@@ -1587,98 +1596,160 @@ int tm_cnn_backward(tm_cnn* h, const float* heat_target, const int* bin_target, 
     }
     CNN_HIP(hipMemcpyAsync(h->heat_t.p, heat_target, cells * sizeof(float), hipMemcpyHostToDevice, st));
     CNN_HIP(hipMemcpyAsync(h->bin_t.p, bin_target, cells * sizeof(int), hipMemcpyHostToDevice, st));
-    const float heat_scale = 1.f / (float)cells, bin_scale = 1.f / (float)std::max(1LL, n_bin);
-    hipLaunchKernelGGL(k_loss, dim3((unsigned)cells), dim3(64), 0, st, h->act.back().p, Nh, h->heat_t.p, h->bin_t.p,
-                       pos_weight, heat_scale, bin_scale, h->dlog.p, round16(Nh), h->cell_loss.p);
-    check_launch("k_loss");
-    // walk the layers backwards; dcur holds dL/d(output of layer l) (pre-activation for the head)
-    float* dcur = h->dlog.p;
-    float* dnext = h->dA0.p;
-    bool masked = false;  // dcur already carries the layer's ReLU backward (k_conv_dgrad's epilogue)
-    for (int l = (int)h->layers.size() - 1; l >= 0; --l) {
-      const Layer& L = h->layers[l];
-      const bool head = l == (int)h->layers.size() - 1;
-      const long long M = rows_of(h, L), Mp = round16ll(M);
-      if (!head && !masked) {
-        hipLaunchKernelGGL(k_relu_bwd, dim3(grid1(M * L.cout)), dim3(256), 0, st, dcur, h->act[l + 1].p, M * L.cout);
-        check_launch("k_relu_bwd");
-      }
-      // dcur rows: the head's gradient dL/dlogits is padded to a multiple of 16 columns (zeros)
-      const int ldz = head ? round16(L.cout) : L.cout;
-      colsum(st, h->csum, dcur, M, L.cout, ldz, h->g.p + L.b_off);
-      if (wgrad_implicit(L, B, ldz, head)) {
-        if (head)
-          conv_wgrad<1>(st, h->parts, h->act[l].p, L, B, dcur, ldz, h->g.p + L.w_off);
-        else if (L.cin == 1)
-          conv_wgrad<0>(st, h->parts, h->act[l].p, L, B, dcur, ldz, h->g.p + L.w_off);
-        else
-          conv_wgrad<9>(st, h->parts, h->act[l].p, L, B, dcur, ldz, h->g.p + L.w_off);
-      } else {
-        // the layer's input as the product operand (recomputed), then its transpose [Kp][Mp]
-        if (head)
-          to_bf16_pad(st, h->act[l].p, M, L.cin, L.cin, h->col.p, L.Kp);
-        else
-          im2col(st, h->act[l].p, L, M, h->col.p);
-        transpose_bf16(st, h->col.p, M, L.Kp, L.Kp, h->colT.p, Mp, L.Kp, Mp);
-        // dZ^T [cout][Mp] and dW[cout][K] = dZ^T . col^T^T
-        transpose_bf16(st, dcur, M, L.cout, ldz, h->dzT.p, Mp, L.cout, Mp);
-        gemm(st, h->parts, h->dzT.p, (int)Mp, h->colT.p, (int)Mp, h->g.p + L.w_off, L.K, L.cout, L.K, (int)Mp,
-             nullptr, false);
-      }
-      if (l == 0) break;
-      if (!head && L.cout % 16 == 0) {
-        // implicit transposed convolution straight into the input's gradient
-        pack_b<PACK_DGRAD>(st, h->w.p + L.w_off, L.cin, 9 * L.cout, 0, 0, L.cin, L.cout, h->wbT.p);
-        conv_dgrad(st, dcur, L, B, h->wbT.p, h->act[l].p, dnext);
-        masked = true;
-        dcur = dnext;
-        dnext = (dnext == h->dA0.p) ? h->dA1.p : h->dA0.p;
-        continue;
-      }
-      // dcol[M][K] = dZ[M][cout16] . W^T[K][cout16]^T, then the gather into the input's gradient
-      const int c16 = round16(L.cout);
-      if (head && L.cin % 16 == 0) {
-        // the head's data gradient as a 1x1 implicit product over its padded dL/dlogits rows (B = W^T,
-        // zero past the head's outputs), with the last convolution's ReLU backward in the epilogue
-        pack_b<PACK_COLS>(st, h->w.p + L.w_off, L.cin, ldz, L.cin, L.cout, 0, 0, h->wbT.p);
-        conv_fwd<1, EPI_MASK>(st, dcur, L.si, ldz, L.so, 1, h->wbT.p, dnext, M, L.cin, nullptr, h->act[l].p);
-        masked = true;
-        dcur = dnext;
-        dnext = (dnext == h->dA0.p) ? h->dA1.p : h->dA0.p;
-        continue;
-      }
-      transpose_bf16(st, h->w.p + L.w_off, L.cout, L.K, L.K, h->wbT.p, c16, L.K, c16);
-      to_bf16_pad(st, dcur, M, L.cout, ldz, h->dzb.p, c16);
-      if (head) {
-        gemm(st, h->parts, h->dzb.p, c16, h->wbT.p, c16, dnext, L.K, M, L.K, c16, nullptr, false);
-      } else {
-        gemm(st, h->parts, h->dzb.p, c16, h->wbT.p, c16, h->dcol.p, L.K, M, L.K, c16, nullptr, false);
-        const long long total = (long long)B * L.si * L.si * L.cin;
-        CNN_REQUIRE(total < (1LL << 31), TM_ERR_INVALID, "col2im: layer too large for one launch");
-        const long long threads = (L.cin % 4 == 0) ? total / 4 : total;
-        hipLaunchKernelGGL(k_col2im, dim3(grid1(threads)), dim3(256), 0, st, h->dcol.p, L.K, L.si, L.cin, L.so,
-                           L.stride, (int)total, dnext);
-        check_launch("k_col2im");
-      }
-      masked = false;
-      dcur = dnext;
-      dnext = (dnext == h->dA0.p) ? h->dA1.p : h->dA0.p;
-    }
-    h->h_loss.resize((size_t)cells * 2);
-    CNN_HIP(hipMemcpyAsync(h->h_loss.data(), h->cell_loss.p, cells * 2 * sizeof(float), hipMemcpyDeviceToHost, st));
+    backward_impl(h, h->heat_t.p, h->bin_t.p, n_bin, pos_weight);
     if (grads) CNN_HIP(hipMemcpyAsync(grads, h->g.p, h->n_params * sizeof(float), hipMemcpyDeviceToHost, st));
+    if (loss) read_loss(h, loss);
     CNN_HIP(hipStreamSynchronize(st));
-    if (loss) {
-      double lh = 0.0, lb = 0.0;
-      for (long long c = 0; c < cells; ++c) {
-        lh += h->h_loss[2 * c];
-        lb += h->h_loss[2 * c + 1];
-      }
-      loss[0] = (float)(lh / (double)cells);
-      loss[1] = (float)(lb / (double)std::max(1LL, n_bin));
-    }
   });
 }
+
+int tm_cnn_backward_device(tm_cnn* h, const float* heat_target, const int* bin_target, long long n_labelled,
+                           float pos_weight) {
+  return cguard([&] {
+    require(h);
+    CNN_REQUIRE(heat_target && bin_target, TM_ERR_INVALID, "NULL target");
+    CNN_REQUIRE(h->batch > 0, TM_ERR_STATE, "no forward pass to differentiate");
+    CNN_REQUIRE(n_labelled >= 0, TM_ERR_INVALID, "n_labelled must be >= 0");
+    backward_impl(h, heat_target, bin_target, n_labelled, pos_weight);
+  });
+}
+
+int tm_cnn_loss(tm_cnn* h, float* loss) {
+  return cguard([&] {
+    require(h);
+    CNN_REQUIRE(loss, TM_ERR_INVALID, "loss is NULL");
+    CNN_REQUIRE(h->loss_cells > 0, TM_ERR_STATE, "no backward pass");
+    read_loss(h, loss);
+  });
+}
+
+int tm_cnn_synchronize(tm_cnn* h) {
+  return cguard([&] {
+    require(h);
+    CNN_HIP(hipStreamSynchronize(h->stream));
+  });
+}
+}  // extern "C"
+
+namespace {
+// the two loss terms of the last backward pass (waits for it): means of the per-cell terms over all
+// cells and over the cells with a bin
+void read_loss(tm_cnn* h, float* loss) {
+  const long long cells = h->loss_cells;
+  h->h_loss.resize((size_t)cells * 2);
+  CNN_HIP(hipMemcpyAsync(h->h_loss.data(), h->cell_loss.p, cells * 2 * sizeof(float), hipMemcpyDeviceToHost,
+                         h->stream));
+  CNN_HIP(hipStreamSynchronize(h->stream));
+  double lh = 0.0, lb = 0.0;
+  for (long long c = 0; c < cells; ++c) {
+    lh += h->h_loss[2 * c];
+    lb += h->h_loss[2 * c + 1];
+  }
+  loss[0] = (float)(lh / (double)cells);
+  loss[1] = (float)(lb / (double)std::max(1LL, h->loss_bins));
+}
+
+// forward from the raw images at src (device memory; act[0] itself for the host entry point)
+void forward_impl(tm_cnn* h, const float* src, int batch) {
+  h->batch = batch;
+  const long long npx = (long long)h->cfg.input_size * h->cfg.input_size;
+  h->std_part.ensure((size_t)batch * STD_PARTS * 2);
+  hipLaunchKernelGGL(k_std_part, dim3(STD_PARTS, batch), dim3(256), 0, h->stream, src, npx, h->std_part.p);
+  check_launch("k_std_part");
+  hipLaunchKernelGGL(k_std_apply, dim3((unsigned)std::min<long long>((npx + 1023) / 1024, 1024), batch), dim3(256), 0,
+                     h->stream, src, h->act[0].p, npx, h->std_part.p);
+  check_launch("k_std_apply");
+  for (int l = 0; l < (int)h->layers.size(); ++l) forward_layer(h, l);
+}
+
+// loss and gradients of the last forward pass from device targets; n_bin cells carry a bin
+void backward_impl(tm_cnn* h, const float* heat_t, const int* bin_t, long long n_bin, float pos_weight) {
+  hipStream_t st = h->stream;
+  const int B = h->batch, Nh = h->Nh;
+  const long long cells = (long long)B * h->G * h->G;
+  h->loss_cells = cells;
+  h->loss_bins = n_bin;
+  const float heat_scale = 1.f / (float)cells, bin_scale = 1.f / (float)std::max(1LL, n_bin);
+  hipLaunchKernelGGL(k_loss, dim3((unsigned)cells), dim3(64), 0, st, h->act.back().p, Nh, heat_t, bin_t,
+                     pos_weight, heat_scale, bin_scale, h->dlog.p, round16(Nh), h->cell_loss.p);
+  check_launch("k_loss");
+  // walk the layers backwards; dcur holds dL/d(output of layer l) (pre-activation for the head)
+  float* dcur = h->dlog.p;
+  float* dnext = h->dA0.p;
+  bool masked = false;  // dcur already carries the layer's ReLU backward (k_conv_dgrad's epilogue)
+  for (int l = (int)h->layers.size() - 1; l >= 0; --l) {
+    const Layer& L = h->layers[l];
+    const bool head = l == (int)h->layers.size() - 1;
+    const long long M = rows_of(h, L), Mp = round16ll(M);
+    if (!head && !masked) {
+      hipLaunchKernelGGL(k_relu_bwd, dim3(grid1(M * L.cout)), dim3(256), 0, st, dcur, h->act[l + 1].p, M * L.cout);
+      check_launch("k_relu_bwd");
+    }
+    // dcur rows: the head's gradient dL/dlogits is padded to a multiple of 16 columns (zeros)
+    const int ldz = head ? round16(L.cout) : L.cout;
+    colsum(st, h->csum, dcur, M, L.cout, ldz, h->g.p + L.b_off);
+    if (wgrad_implicit(L, B, ldz, head)) {
+      if (head)
+        conv_wgrad<1>(st, h->parts, h->act[l].p, L, B, dcur, ldz, h->g.p + L.w_off);
+      else if (L.cin == 1)
+        conv_wgrad<0>(st, h->parts, h->act[l].p, L, B, dcur, ldz, h->g.p + L.w_off);
+      else
+        conv_wgrad<9>(st, h->parts, h->act[l].p, L, B, dcur, ldz, h->g.p + L.w_off);
+    } else {
+      // the layer's input as the product operand (recomputed), then its transpose [Kp][Mp]
+      if (head)
+        to_bf16_pad(st, h->act[l].p, M, L.cin, L.cin, h->col.p, L.Kp);
+      else
+        im2col(st, h->act[l].p, L, M, h->col.p);
+      transpose_bf16(st, h->col.p, M, L.Kp, L.Kp, h->colT.p, Mp, L.Kp, Mp);
+      // dZ^T [cout][Mp] and dW[cout][K] = dZ^T . col^T^T
+      transpose_bf16(st, dcur, M, L.cout, ldz, h->dzT.p, Mp, L.cout, Mp);
+      gemm(st, h->parts, h->dzT.p, (int)Mp, h->colT.p, (int)Mp, h->g.p + L.w_off, L.K, L.cout, L.K, (int)Mp,
+           nullptr, false);
+    }
+    if (l == 0) break;
+    if (!head && L.cout % 16 == 0) {
+      // implicit transposed convolution straight into the input's gradient
+      pack_b<PACK_DGRAD>(st, h->w.p + L.w_off, L.cin, 9 * L.cout, 0, 0, L.cin, L.cout, h->wbT.p);
+      conv_dgrad(st, dcur, L, B, h->wbT.p, h->act[l].p, dnext);
+      masked = true;
+      dcur = dnext;
+      dnext = (dnext == h->dA0.p) ? h->dA1.p : h->dA0.p;
+      continue;
+    }
+    // dcol[M][K] = dZ[M][cout16] . W^T[K][cout16]^T, then the gather into the input's gradient
+    const int c16 = round16(L.cout);
+    if (head && L.cin % 16 == 0) {
+      // the head's data gradient as a 1x1 implicit product over its padded dL/dlogits rows (B = W^T,
+      // zero past the head's outputs), with the last convolution's ReLU backward in the epilogue
+      pack_b<PACK_COLS>(st, h->w.p + L.w_off, L.cin, ldz, L.cin, L.cout, 0, 0, h->wbT.p);
+      conv_fwd<1, EPI_MASK>(st, dcur, L.si, ldz, L.so, 1, h->wbT.p, dnext, M, L.cin, nullptr, h->act[l].p);
+      masked = true;
+      dcur = dnext;
+      dnext = (dnext == h->dA0.p) ? h->dA1.p : h->dA0.p;
+      continue;
+    }
+    transpose_bf16(st, h->w.p + L.w_off, L.cout, L.K, L.K, h->wbT.p, c16, L.K, c16);
+    to_bf16_pad(st, dcur, M, L.cout, ldz, h->dzb.p, c16);
+    if (head) {
+      gemm(st, h->parts, h->dzb.p, c16, h->wbT.p, c16, dnext, L.K, M, L.K, c16, nullptr, false);
+    } else {
+      gemm(st, h->parts, h->dzb.p, c16, h->wbT.p, c16, h->dcol.p, L.K, M, L.K, c16, nullptr, false);
+      const long long total = (long long)B * L.si * L.si * L.cin;
+      CNN_REQUIRE(total < (1LL << 31), TM_ERR_INVALID, "col2im: layer too large for one launch");
+      const long long threads = (L.cin % 4 == 0) ? total / 4 : total;
+      hipLaunchKernelGGL(k_col2im, dim3(grid1(threads)), dim3(256), 0, st, h->dcol.p, L.K, L.si, L.cin, L.so,
+                         L.stride, (int)total, dnext);
+      check_launch("k_col2im");
+    }
+    masked = false;
+    dcur = dnext;
+    dnext = (dnext == h->dA0.p) ? h->dA1.p : h->dA0.p;
+  }
+}
+}  // namespace
+
+extern "C" {
 
 int tm_cnn_heat(tm_cnn* h, float* heat) {
   return cguard([&] {
@@ -1727,7 +1798,6 @@ int tm_cnn_adam_step(tm_cnn* h, float lr, float beta1, float beta2, float eps) {
     hipLaunchKernelGGL(k_adam, dim3(grid1(h->n_params)), dim3(256), 0, h->stream, h->w.p, h->g.p, h->m.p, h->v.p,
                        h->n_params, lr_t, beta1, beta2, eps);
     check_launch("k_adam");
-    CNN_HIP(hipStreamSynchronize(h->stream));
   });
 }
 

@@ -56,6 +56,11 @@ int tm_cnn_get_params(tm_cnn* h, float* params);
  * batch x G x G x (1 + n_bins) values, cell-major. */
 int tm_cnn_forward(tm_cnn* h, const float* images, int batch, float* logits);
 
+/* The same forward pass over batch images already in device memory (batch x S x S FP32 on the
+ * handle's device, not modified), enqueued on the handle's stream with no copy and no wait: for
+ * training loops whose inputs stay resident on the GPU. */
+int tm_cnn_forward_device(tm_cnn* h, const float* images, int batch);
+
 /* Location logits of the last forward pass (batch x G x G), and the 1 + n_bins logits of n chosen
  * cells (flat indices b G^2 + y G + x) into out (n x (1 + n_bins)): what a proposal step reads
  * instead of the whole logit volume. */
@@ -70,7 +75,18 @@ int tm_cnn_cell_logits(tm_cnn* h, const int* cells, int n, float* out);
 int tm_cnn_backward(tm_cnn* h, const float* heat_target, const int* bin_target, float pos_weight, float* loss,
                     float* grads);
 
-/* One Adam step with the gradients of the last backward pass (bias-corrected moments). */
+/* The same backward pass from targets in device memory (heat_target FP32, bin_target int32, batch x
+ * G x G), enqueued with no wait. n_labelled: the number of cells with a bin >= 0 (the orientation
+ * term is their mean); bins >= n_bins count as none here (tm_cnn_backward rejects them).
+ * tm_cnn_loss returns the two loss terms of the last backward pass, waiting for it;
+ * tm_cnn_synchronize waits for all enqueued work of the handle. */
+int tm_cnn_backward_device(tm_cnn* h, const float* heat_target, const int* bin_target, long long n_labelled,
+                           float pos_weight);
+int tm_cnn_loss(tm_cnn* h, float* loss);
+int tm_cnn_synchronize(tm_cnn* h);
+
+/* One Adam step with the gradients of the last backward pass (bias-corrected moments), enqueued on
+ * the handle's stream: every later call on the handle sees the updated parameters. */
 int tm_cnn_adam_step(tm_cnn* h, float lr, float beta1, float beta2, float eps);
 
 /* Test hook: the post-ReLU output of conv layer l (0-based; l = n_layers gives the head's logits)

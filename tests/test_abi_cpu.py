@@ -136,4 +136,7 @@ def test_new_entry_points_reject_bad_arguments():
     assert C.tm_cnn_heat(None, None) != 0
     assert C.tm_cnn_cell_logits(None, None, 0, None) != 0
     assert C.tm_cnn_adam_step(None, 1e-3, 0.9, 0.999, 1e-8) != 0
+    assert C.tm_cnn_forward_device(None, None, 1) != 0
+    assert C.tm_cnn_backward_device(None, None, None, 0, 1.0) != 0
+    assert C.tm_cnn_loss(None, None) != 0 and C.tm_cnn_synchronize(None) != 0
     assert C.tm_cnn_last_error()

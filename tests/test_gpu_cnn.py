@@ -25,6 +25,9 @@
   1e-4 (relative L2) of the oracle; the oracle's own gradient is pinned by finite differences on the CPU
 * training: 40 Adam steps on a fixed batch lower the loss by more than half; parameters round-trip
   through set/get; the seeded initialisation is deterministic
+* the device-input entry points (tm_cnn_forward_device, tm_cnn_backward_device, tm_cnn_loss) give
+  bit-identical location logits, loss terms and (after one Adam step each) parameters to the host
+  entry points on the same images and targets; a bin past the last counts as none there
 """
 import numpy as np
 import pytest
@@ -164,3 +167,62 @@ def test_training_lowers_the_loss_and_params_round_trip():
     finally:
         net.close()
         other.close()
+
+
+def _hip():
+    import ctypes
+    import os
+    H = ctypes.CDLL(os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib", "libamdhip64.so"))
+    H.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    H.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    H.hipFree.argtypes = [ctypes.c_void_p]
+    return H
+
+
+def _upload(H, a):
+    import ctypes
+    p = ctypes.c_void_p()
+    assert H.hipMalloc(ctypes.byref(p), a.nbytes) == 0
+    assert H.hipMemcpy(p, a.ctypes.data, a.nbytes, 1) == 0  # hipMemcpyHostToDevice
+    return p.value
+
+
+def test_device_entry_points_match_host_path():
+    print("case: cnn device-input forward/backward against the host entry points, wide network, batch 3", flush=True)
+    spec, cfg, flat, images, heat, bins = _case("wide")
+    H = _hip()
+    host = cnn.ProposalNet(spec, max_batch=4, seed=None)
+    dev = cnn.ProposalNet(spec, max_batch=4, seed=None)
+    bufs = []
+    try:
+        host.set_params(flat)
+        dev.set_params(flat)
+        logits = host.forward(images)
+        (lh, lb), _ = host.backward(heat, bins, pos_weight=5.0)
+        host.adam_step(lr=3e-3)
+        bufs = [_upload(H, np.ascontiguousarray(a)) for a in (images.astype(np.float32), heat, bins)]
+        dev.forward_device(bufs[0], 3)
+        assert np.array_equal(dev.heat(3), logits[..., 0])
+        dev.backward_device(bufs[1], bufs[2], int((bins >= 0).sum()), pos_weight=5.0)
+        assert dev.loss() == (lh, lb)
+        dev.adam_step(lr=3e-3)
+        dev.synchronize()
+        assert np.array_equal(dev.params(), host.params())
+        # a bin past the last one carries no orientation term on the device path
+        wild = bins.copy()
+        wild[0, 3, 5] = spec.n_bins + 7
+        tame = bins.copy()
+        tame[0, 3, 5] = -1
+        bufs.append(_upload(H, wild))
+        dev.set_params(flat)
+        host.set_params(flat)
+        dev.forward_device(bufs[0], 3)
+        dev.backward_device(bufs[1], bufs[3], int((tame >= 0).sum()), pos_weight=5.0)
+        host.forward(images, want_logits=False)
+        (th, tb), _ = host.backward(heat, tame, pos_weight=5.0)
+        assert dev.loss() == (th, tb)
+    finally:
+        for b in bufs:
+            H.hipFree(b)
+        host.close()
+        dev.close()
