@@ -139,6 +139,9 @@ def main():
     ap.add_argument("--plane-storage", choices=["fp32", "fp16", "bf16"], default="fp32",
                     help="storage of the inter-pass plane (fp32: the reference's precision, the headline)")
     ap.add_argument("--in-flight", type=int, default=0, help="refine: peaks refined together (0 = 256)")
+    ap.add_argument("--cnn-net", choices=["default", "large"], default="default",
+                    help="cnn: the 512^2 proposal network (511,545 parameters) or the config-1-scale one "
+                         "(14 convolutions, 1.57 M parameters)")
     ap.add_argument("--train", type=int, default=2000, help="proposal: training micrographs")
     ap.add_argument("--epochs", type=int, default=20, help="proposal: training epochs")
     ap.add_argument("--eval", type=int, default=40, help="proposal: held-out micrographs")
@@ -587,7 +590,11 @@ def bench_cnn(args):
     against the bf16 dense MFMA peak."""
     from cistem_downstream_bah_amd import cnn
 
-    spec = cnn.NetSpec()
+    if args.cnn_net == "large":
+        from cistem_downstream_bah_amd import proposal_large as PL
+        spec = PL.default_net()
+    else:
+        spec = cnn.NetSpec()
     B = 16
     net = cnn.ProposalNet(spec, max_batch=B, seed=1)
     rng = np.random.default_rng(0)
@@ -647,7 +654,8 @@ def bench_cnn(args):
            "warmup": args.warmup, "ms_per_step": round(1e3 * dt / args.steps, 3), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None, "dtype": "bf16 products, fp32 accumulation and weights",
            "data": "synthetic (seeded images and targets, resident in device memory)",
-           "config": {"workload": "cnn_train_512sq_b16", "batch": B, "params": net.n_params, "grid": G},
+           "config": {"workload": "cnn_train_512sq_b16" + ("_large" if args.cnn_net == "large" else ""), "batch": B,
+                      "params": net.n_params, "grid": G, "layers": len(spec.channels)},
            "host_input": {"images_per_s": round(B * n_host / dt_host, 2),
                           "ms_per_step": round(1e3 * dt_host / n_host, 3)},
            "roofline": {"bound": "mfma", "achieved": round(tflops, 2), "peak": peak, "unit": "TFLOP/s",
