@@ -576,7 +576,10 @@ __global__ __launch_bounds__(256) void k_conv_dgrad(const float* __restrict__ dZ
 // dL/dlogits), the last tile then writing only its channels below Cout. Element offsets into X and
 // dZ are 32-bit (the host checks both sizes). TAPS = 0 is the first layer (Cin = 1): the nine taps
 // are the product's n (one 16-wide tile, lane r holding tap r < 9 and reading its own shifted pixels),
-// parts[z][co][tap]. The product's k runs
+// parts[z][co][tap]. The bias gradient comes with it: the block of tap 0 and input-channel tile 0 of
+// each (split, output-channel tile) also sums its FP32 dZ values per output channel (in segment,
+// lane-group and wave order) into parts[z][Cout K + co], so a split's partials are the layer's
+// [weights | biases], as in the flat parameter array. The product's k runs
 // over pixels, in 16-pixel segments of one output row: a lane's four pixels are consecutive, so its
 // four dZ values lie Cout floats apart and its four X values stride Cin floats apart, at constant
 // offsets from one address (only the image's left and right edges mask a pixel). A block covers one
@@ -602,6 +605,10 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const float* __restrict__ X,
   const int co0 = cm * 16 * MT, ci0 = cc * 16 * NT;
   const int r = lane & 15, pq = (lane >> 4) * 4;
   const int spr = So >> 4;
+  const bool bias_blk = tapi == 0 && cc == 0;  // uniform over the block
+  float bsum[MT];
+#pragma unroll
+  for (int i = 0; i < MT; ++i) bsum[i] = 0.f;
   const int g_lo = (int)((long long)rows * z / nsplit) * spr, g_hi = (int)((long long)rows * (z + 1) / nsplit) * spr;
   float4_t acc[MT][NT];
 #pragma unroll
@@ -651,6 +658,10 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const float* __restrict__ X,
     }
   };
   auto mfma = [&](const float (&za)[MT][4], const float (&xa)[NT][4]) {
+    if (bias_blk) {
+#pragma unroll
+      for (int i = 0; i < MT; ++i) bsum[i] += ((za[i][0] + za[i][1]) + za[i][2]) + za[i][3];
+    }
     short4_t a[MT], bq[NT];
 #pragma unroll
     for (int i = 0; i < MT; ++i) a[i] = bf16x4_hw(za[i][0], za[i][1], za[i][2], za[i][3]);
@@ -674,18 +685,40 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const float* __restrict__ X,
     load(g, za0, xa0);
     mfma(za0, xa0);
   }
+  // the bias sums of the four lane groups (the same value in every group: fixed pairing order)
+  if (bias_blk) {
+#pragma unroll
+    for (int i = 0; i < MT; ++i) {
+      bsum[i] += __shfl_xor(bsum[i], 16);
+      bsum[i] += __shfl_xor(bsum[i], 32);
+    }
+  }
   // waves 1..3 hand their tiles to wave 0, which adds them in wave order and writes the partial
   __shared__ float4_t red[3][MT * NT][64];
+  __shared__ float bred[3][MT][16];
   if (wave > 0) {
 #pragma unroll
     for (int i = 0; i < MT; ++i)
 #pragma unroll
       for (int j = 0; j < NT; ++j) red[wave - 1][i * NT + j][lane] = acc[i][j];
+    if (bias_blk && lane < 16) {
+#pragma unroll
+      for (int i = 0; i < MT; ++i) bred[wave - 1][i][lane] = bsum[i];
+    }
   }
   __syncthreads();
   if (wave != 0) return;
   const int K = TAPS == 0 ? 9 : TAPS * Cin;
-  float* out = parts + (size_t)z * Cout * K + (size_t)tapi * Cin;
+  float* const zpart = parts + (size_t)z * Cout * (K + 1);
+  if (bias_blk && lane < 16) {
+#pragma unroll
+    for (int i = 0; i < MT; ++i) {
+      const int co = co0 + 16 * i + r;
+      const float s = ((bsum[i] + bred[0][i][r]) + bred[1][i][r]) + bred[2][i][r];
+      if (co < Cout) zpart[(size_t)Cout * K + co] = s;
+    }
+  }
+  float* out = zpart + (size_t)tapi * Cin;
 #pragma unroll
   for (int i = 0; i < MT; ++i)
 #pragma unroll
@@ -1260,14 +1293,15 @@ void colsum(hipStream_t st, Buf<float>& part, const float* X, long long R, int C
   check_launch("k_colsum_final");
 }
 
-// dW[cout][9 cin] of a 3x3 layer straight from its input X and output gradient dZ (k_conv_wgrad), the
-// pixel splits' partials added in split order: about 2048 blocks, a split holding at least one
+// dW[cout][9 cin] of a 3x3 layer and its bias gradient (written after it: dW points at the layer's
+// weights in the flat gradient, the biases follow them) straight from its input X and output gradient
+// dZ (k_conv_wgrad), the pixel splits' partials added in split order: about 2048 blocks, a split holding at least one
 // output row, the partials at most 64 M floats.
 template <int MT, int NT, int TAPS>
 void conv_wgrad_t(hipStream_t st, Buf<float>& parts, const float* X, const Layer& L, int rows, const float* dZ,
                   int ldz, float* dW) {
   const int ntiles = (TAPS == 0 ? 1 : TAPS * (L.cin / (16 * NT))) * ((L.cout + 16 * MT - 1) / (16 * MT));
-  const long long per = (long long)L.cout * L.K;
+  const long long per = (long long)L.cout * (L.K + 1);  // weights, then biases
   long long ns = std::max<long long>(1, 2048 / ntiles);
   ns = std::min<long long>(ns, rows);
   ns = std::min<long long>(ns, (64LL << 20) / per);
@@ -1278,8 +1312,8 @@ void conv_wgrad_t(hipStream_t st, Buf<float>& parts, const float* X, const Layer
                      L.cin, L.so, L.stride, dZ, ldz, L.cout, rows, nsplit, ntiles, parts.p);
   check_launch("k_conv_wgrad");
   const unsigned g = grid1((per + 31) / 32, 1, 1 << 16);
-  hipLaunchKernelGGL((k_reduce_splits<false, false>), dim3(g), dim3(256), 0, st, parts.p, nsplit, (long long)L.cout,
-                     L.K, per, dW, (long long)L.K, nullptr);
+  hipLaunchKernelGGL((k_reduce_splits<false, false>), dim3(g), dim3(256), 0, st, parts.p, nsplit, 1LL, (int)per, per,
+                     dW, per, nullptr);
   check_launch("k_reduce_splits");
 }
 template <int MT, int TAPS>
@@ -1687,8 +1721,8 @@ void backward_impl(tm_cnn* h, const float* heat_t, const int* bin_t, long long n
     }
     // dcur rows: the head's gradient dL/dlogits is padded to a multiple of 16 columns (zeros)
     const int ldz = head ? round16(L.cout) : L.cout;
-    colsum(st, h->csum, dcur, M, L.cout, ldz, h->g.p + L.b_off);
     if (wgrad_implicit(L, B, ldz, head)) {
+      // weights and biases together (the biases follow the weights in the flat gradient)
       if (head)
         conv_wgrad<1>(st, h->parts, h->act[l].p, L, B, dcur, ldz, h->g.p + L.w_off);
       else if (L.cin == 1)
@@ -1696,6 +1730,7 @@ void backward_impl(tm_cnn* h, const float* heat_t, const int* bin_t, long long n
       else
         conv_wgrad<9>(st, h->parts, h->act[l].p, L, B, dcur, ldz, h->g.p + L.w_off);
     } else {
+      colsum(st, h->csum, dcur, M, L.cout, ldz, h->g.p + L.b_off);
       // the layer's input as the product operand (recomputed), then its transpose [Kp][Mp]
       if (head)
         to_bf16_pad(st, h->act[l].p, M, L.cin, L.cin, h->col.p, L.Kp);
