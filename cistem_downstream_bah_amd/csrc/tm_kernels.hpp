@@ -589,12 +589,25 @@ __global__ __launch_bounds__(NT) void k_normalize(float* __restrict__ proj, int 
   const float edge_mean = (float)(red[0][0] / (double)(float)(4 * T - 4));
   __syncthreads();
   double s1 = 0.0, s2 = 0.0;
-  for (int i = tid; i < T * T; i += NT) {
-    const int y = i / T, x = i - y * T;
-    const float v = p[y * rs + x] + (-edge_mean);
-    p[y * rs + x] = v;
-    s1 += (double)v;
-    s2 += (double)v * (double)v;
+  // element i = tid + k NT: when T divides NT (every compiled projection size) the thread's column is
+  // fixed and its rows step by NT / T, the same elements in the same order without a division each
+  const bool fixed_x = NT % T == 0;
+  if (fixed_x) {
+    const int x = tid % T;
+    for (int y = tid / T; y < T; y += NT / T) {
+      const float v = p[y * rs + x] + (-edge_mean);
+      p[y * rs + x] = v;
+      s1 += (double)v;
+      s2 += (double)v * (double)v;
+    }
+  } else {
+    for (int i = tid; i < T * T; i += NT) {
+      const int y = i / T, x = i - y * T;
+      const float v = p[y * rs + x] + (-edge_mean);
+      p[y * rs + x] = v;
+      s1 += (double)v;
+      s2 += (double)v * (double)v;
+    }
   }
   red[1][tid] = s1;
   red[2][tid] = s2;
@@ -610,9 +623,14 @@ __global__ __launch_bounds__(NT) void k_normalize(float* __restrict__ proj, int 
   const float t2 = mean * nT / nP;
   const float variance = t1 - t2 * t2;
   const float inverse = (float)(1. / (double)sqrtf(variance));       // DivideByConstant (image.cpp:5118-5127)
-  for (int i = tid; i < T * T; i += NT) {
-    const int y = i / T, x = i - y * T;
-    p[y * rs + x] = p[y * rs + x] * inverse;
+  if (fixed_x) {
+    const int x = tid % T;
+    for (int y = tid / T; y < T; y += NT / T) p[y * rs + x] = p[y * rs + x] * inverse;
+  } else {
+    for (int i = tid; i < T * T; i += NT) {
+      const int y = i / T, x = i - y * T;
+      p[y * rs + x] = p[y * rs + x] * inverse;
+    }
   }
 }
 
