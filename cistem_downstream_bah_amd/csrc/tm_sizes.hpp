@@ -22,9 +22,10 @@
 namespace tmk {
 
 // Column pass (k_mid, every height except 4096): columns per block and threads per column (the
-// comments give each height's radix plan, tm_fft.hpp). Heights with radix-2/3/5/9 stages get more
-// threads per column (fewer butterflies each, so the butterfly registers fit without spilling) and
-// fewer columns per block. LDS per block: columns x (H + H/16) x 8 bytes.
+// comments give each height's radix plan, tm_fft.hpp). Heights with radix-2/3/5/9 stages get fewer
+// columns per block. The threads per column match the plan's butterfly counts (H / R per stage), so
+// that no stage leaves half of them idle (3072 = 16 16 4 3 has 192 radix-16 butterflies: 192 threads,
+// not 384) and more columns fit a CU. LDS per block: columns x (H + H/16) x 8 bytes.
 constexpr int mid_cols(int H) {
   switch (H) {
     case 1536: return 4;
@@ -44,11 +45,11 @@ constexpr int mid_threads(int H) {
     case 384: return 32;    // 16 8 3
     case 512: return 32;    // 16 16 2
     case 1024: return 64;   // 16 16 4
-    case 1536: return 192;  // 16 16 2 3
+    case 1536: return 96;   // 16 16 2 3
     case 2048: return 128;  // 16 16 8
     case 2592: return 324;  // 16 2 9 9
-    case 2916: return 486;  // 4 9 9 9
-    case 3072: return 384;  // 16 16 4 3
+    case 2916: return 324;  // 4 9 9 9
+    case 3072: return 192;  // 16 16 4 3
     case 5832: return 486;  // 8 9 9 9
     default: return 0;
   }
