@@ -1,14 +1,17 @@
 // Proposal network (include/tm_cnn.h): 3x3 convolutions with ReLU and a 1x1 head, forward,
 // backward and Adam on the device. No counterpart in the reference.
 //
-// Every product is one kernel, k_gemm_nt: C[m][n] = sum_k A[m][k] B[n][k] on bf16 operands with
-// FP32 accumulation on MFMA 16x16x16 tiles (a 64 x 64 block tile, four waves of 32 x 32), both
-// operands K-contiguous, optionally split over K into FP32 partials that a second kernel adds in
-// a fixed order (results do not depend on scheduling). With channels-last activations:
-//   forward        Y[pixel][cout]  = col[pixel][k] . W[cout][k]^T          (col: im2col, bf16)
-//   weight grad    dW[cout][k]     = dZ^T[cout][pixel] . col^T[k][pixel]^T (split over pixels)
-//   data grad      dcol[pixel][k]  = dZ[pixel][cout] . W^T[k][cout]^T, then a gather (col2im)
-// im2col is recomputed in the backward pass instead of being kept per layer.
+// Products run on bf16 MFMA tiles with FP32 accumulation, channels-last FP32 activations and
+// gradients read straight into the product (no im2col matrix, no transposes) wherever the channel
+// counts allow:
+//   forward        Y[pixel][cout] = relu(col(pixel, k) . W[cout][k] + b)   k_conv_fwd (implicit col)
+//   weight grad    dW[cout][k] = sum over pixels of dZ[pixel][cout] col(pixel, k)   k_conv_wgrad
+//   data grad      dX[pixel][cin] = sum over the taps reaching it of dZ W          k_conv_dgrad
+// the 1x1 head with the same kernels (one tap), the first layer's weight gradient with its nine taps
+// as the product's n. Weights go to the products packed in MFMA operand order (k_pack_b). Layers
+// whose channel counts the implicit kernels do not take use explicit products (k_gemm_nt / k_gemm_nt4
+// over im2col and transposed operands, col2im). Every split reduction adds its partials in a fixed
+// order, so results do not depend on scheduling.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
