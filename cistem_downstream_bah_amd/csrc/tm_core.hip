@@ -26,6 +26,9 @@
 
 namespace {
 
+// Row bands per orientation of the projection normalisation (k_norm_stats / k_norm_scale blocks).
+constexpr int NORM_BANDS = 8;
+
 bool pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
 bool image_size_compiled(int n) {
@@ -73,6 +76,7 @@ struct tm_core {
   DevBuf<float> mip;
   DevBuf<int> idx;
   DevBuf<double> sum, sumsq;
+  DevBuf<double> norm_part;  // projection normalisation: NORM_BANDS band sums per orientation
   DevBuf<unsigned long long> hist;
   std::map<int, cf*> tw;
   // timing: event pairs recorded around each kernel group on the handle's stream, resolved after
@@ -150,6 +154,7 @@ struct tm_core {
     ++epoch;
     for (DevBuf<cf>* b : {&tpl, &inpT, &phaseP, &phaseT, &filt, &slice, &projT, &rows, &Z}) b->release();
     mats.release();
+    norm_part.release();
     mip.release();
     idx.release();
     sum.release();
@@ -389,9 +394,12 @@ void run_projections(tm_core* h, int o0, int nb) {
   }
   {
     Timer t(h, K_NORMALIZE);
-    hipLaunchKernelGGL((tmk::k_normalize<1024>), dim3(nb), dim3(1024), 0, h->stream, reinterpret_cast<float*>(h->projT.p),
-                       T, 2 * proj_bs, (long long)h->W * h->H);
-    check_launch("k_normalize");
+    hipLaunchKernelGGL((tmk::k_norm_stats<256>), dim3(NORM_BANDS, nb), dim3(256), 0, h->stream,
+                       reinterpret_cast<const float*>(h->projT.p), T, 2 * proj_bs, h->norm_part.p);
+    check_launch("k_norm_stats");
+    hipLaunchKernelGGL((tmk::k_norm_scale<256>), dim3(NORM_BANDS, nb), dim3(256), 0, h->stream,
+                       reinterpret_cast<float*>(h->projT.p), T, 2 * proj_bs, h->norm_part.p, (long long)h->W * h->H);
+    check_launch("k_norm_scale");
   }
 }
 
@@ -855,6 +863,7 @@ int tm_init(tm_core* h, const tm_init_params* p, const float* template_spectrum,
     h->sum.alloc(npx);
     h->sumsq.alloc(npx);
     h->hist.alloc(512);
+    h->norm_part.alloc((size_t)B * NORM_BANDS * 3);
     for (int k : {P, P / 2, T, T / 2, W, W / 2, H}) h->twiddle(k);
     reset_state(h);
     TM_HIP(hipStreamSynchronize(h->stream));

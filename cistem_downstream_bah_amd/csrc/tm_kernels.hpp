@@ -8,7 +8,7 @@
 //   k_cols / k_rows_c2r / k_rows_r2c
 //                  the small 2-D transforms of the projection (C2R P^2, clip T^2, R2C T^2,
 //                  x filter, C2R T^2; match_template.cpp:1009-1030)
-//   k_normalize    edge mean, variance to the padded size, 1/sqrt(var) (match_template.cpp:1060-1068)
+//   k_norm_stats, k_norm_scale  edge mean, variance to the padded size, 1/sqrt(var) (match_template.cpp:1060-1068)
 //   k_rows_r2c     forward row FFT of the T non-zero rows of the W x H padded reference
 //                  (ClipIntoLargerRealSpace2D + first half of the R2C, image.cpp:8398-8464)
 //   k_mid          per column tile: forward column FFT, DC zero, conj(ref) * input, inverse
@@ -563,14 +563,26 @@ __global__ __launch_bounds__(NT) void k_rows_r2c(const float* __restrict__ src, 
 }
 
 // ---------------------------------------------------------------- projection normalisation
-// One block per orientation on a T x T real image in the padded layout (row stride T+2).
-// Follows match_template.cpp:1060-1068: subtract the edge mean (image.cpp:5907-6000), then
-// variance = <x^2> * nT/nP - (<x> * nT/nP)^2 in float from FP64 sums, then x * (1/sqrt(var)).
+// A T x T real image in the padded layout (row stride T+2), normalised as match_template.cpp:1060-1068:
+// subtract the edge mean (image.cpp:5907-6000), then variance = <x^2> * nT/nP - (<x> * nT/nP)^2 in
+// float from FP64 sums, then x * (1/sqrt(var)). S blocks per orientation (blockIdx.x = band of T/S
+// rows, blockIdx.y = orientation): k_norm_stats forms the edge mean (every block the same FP64 tree
+// sum) and its band's FP64 sums of x - mean without writing the image (other blocks still read its
+// edges); k_norm_scale adds the S band sums in band order and writes (x - mean) * (1/sqrt(var)).
+// part[b][s] = {sum, sum of squares, edge mean}.
 template <int NT>
-__global__ __launch_bounds__(NT) void k_normalize(float* __restrict__ proj, int T, size_t bstride, long long n_pad) {
-  __shared__ double red[3][NT];
-  const int tid = threadIdx.x;
-  float* p = proj + (size_t)blockIdx.x * bstride;
+__device__ void norm_band(int T, int s, int S, int& lo, int& n, int& sh) {
+  const int y0 = T * s / S, y1 = T * (s + 1) / S;
+  lo = y0;
+  n = (y1 - y0) * T;
+  sh = (T & (T - 1)) == 0 ? __ffs(T) - 1 : -1;  // element i of the band: row i >> sh, column i & (T - 1)
+}
+template <int NT>
+__global__ __launch_bounds__(NT) void k_norm_stats(const float* __restrict__ proj, int T, size_t bstride,
+                                                   double* __restrict__ part) {
+  __shared__ double red[2][NT];
+  const int tid = threadIdx.x, s = blockIdx.x, S = gridDim.x, b = blockIdx.y;
+  const float* p = proj + (size_t)b * bstride;
   const int rs = T + 2;
   double e = 0.0;
   for (int i = tid; i < 4 * T - 4; i += NT) {
@@ -582,55 +594,67 @@ __global__ __launch_bounds__(NT) void k_normalize(float* __restrict__ proj, int 
   }
   red[0][tid] = e;
   __syncthreads();
-  for (int s = NT / 2; s > 0; s >>= 1) {
-    if (tid < s) red[0][tid] += red[0][tid + s];
+  for (int k = NT / 2; k > 0; k >>= 1) {
+    if (tid < k) red[0][tid] += red[0][tid + k];
     __syncthreads();
   }
   const float edge_mean = (float)(red[0][0] / (double)(float)(4 * T - 4));
   __syncthreads();
+  int y0, n, sh;
+  norm_band<NT>(T, s, S, y0, n, sh);
   double s1 = 0.0, s2 = 0.0;
-  // element i = tid + k NT: when T divides NT (every compiled projection size) the thread's column is
-  // fixed and its rows step by NT / T, the same elements in the same order without a division each
-  const bool fixed_x = NT % T == 0;
-  if (fixed_x) {
-    const int x = tid % T;
-    for (int y = tid / T; y < T; y += NT / T) {
-      const float v = p[y * rs + x] + (-edge_mean);
-      p[y * rs + x] = v;
-      s1 += (double)v;
-      s2 += (double)v * (double)v;
-    }
-  } else {
-    for (int i = tid; i < T * T; i += NT) {
-      const int y = i / T, x = i - y * T;
-      const float v = p[y * rs + x] + (-edge_mean);
-      p[y * rs + x] = v;
-      s1 += (double)v;
-      s2 += (double)v * (double)v;
-    }
+  for (int i = tid; i < n; i += NT) {
+    const int dy = sh >= 0 ? i >> sh : i / T, x = i - dy * T;
+    const float v = p[(y0 + dy) * rs + x] + (-edge_mean);
+    s1 += (double)v;
+    s2 += (double)v * (double)v;
   }
-  red[1][tid] = s1;
-  red[2][tid] = s2;
+  red[0][tid] = s1;
+  red[1][tid] = s2;
   __syncthreads();
-  for (int s = NT / 2; s > 0; s >>= 1) {
-    if (tid < s) { red[1][tid] += red[1][tid + s]; red[2][tid] += red[2][tid + s]; }
+  for (int k = NT / 2; k > 0; k >>= 1) {
+    if (tid < k) { red[0][tid] += red[0][tid + k]; red[1][tid] += red[1][tid + k]; }
     __syncthreads();
   }
-  const float mean_sq = (float)(red[2][0] / T / T);                 // ReturnSumOfSquares (image.cpp:294-301)
-  const float mean = (float)(red[1][0] / ((long long)T * T));       // ReturnAverageOfRealValues (image.cpp:6327-6340)
-  const float nT = (float)((long long)T * T), nP = (float)n_pad;
-  const float t1 = mean_sq * nT / nP;
-  const float t2 = mean * nT / nP;
-  const float variance = t1 - t2 * t2;
-  const float inverse = (float)(1. / (double)sqrtf(variance));       // DivideByConstant (image.cpp:5118-5127)
-  if (fixed_x) {
-    const int x = tid % T;
-    for (int y = tid / T; y < T; y += NT / T) p[y * rs + x] = p[y * rs + x] * inverse;
-  } else {
-    for (int i = tid; i < T * T; i += NT) {
-      const int y = i / T, x = i - y * T;
-      p[y * rs + x] = p[y * rs + x] * inverse;
+  if (tid == 0) {
+    double* q = part + ((size_t)b * S + s) * 3;
+    q[0] = red[0][0];
+    q[1] = red[1][0];
+    q[2] = (double)edge_mean;
+  }
+}
+template <int NT>
+__global__ __launch_bounds__(NT) void k_norm_scale(float* __restrict__ proj, int T, size_t bstride,
+                                                   const double* __restrict__ part, long long n_pad) {
+  __shared__ float mi[2];
+  const int tid = threadIdx.x, s = blockIdx.x, S = gridDim.x, b = blockIdx.y;
+  if (tid == 0) {
+    const double* q = part + (size_t)b * S * 3;
+    double s1 = 0.0, s2 = 0.0;
+    for (int k = 0; k < S; ++k) {
+      s1 += q[3 * k];
+      s2 += q[3 * k + 1];
     }
+    const float mean_sq = (float)(s2 / T / T);                        // ReturnSumOfSquares (image.cpp:294-301)
+    const float mean = (float)(s1 / ((long long)T * T));              // ReturnAverageOfRealValues (image.cpp:6327-6340)
+    const float nT = (float)((long long)T * T), nP = (float)n_pad;
+    const float t1 = mean_sq * nT / nP;
+    const float t2 = mean * nT / nP;
+    const float variance = t1 - t2 * t2;
+    mi[0] = (float)q[2];
+    mi[1] = (float)(1. / (double)sqrtf(variance));                   // DivideByConstant (image.cpp:5118-5127)
+  }
+  __syncthreads();
+  const float edge_mean = mi[0], inverse = mi[1];
+  float* p = proj + (size_t)b * bstride;
+  const int rs = T + 2;
+  int y0, n, sh;
+  norm_band<NT>(T, s, S, y0, n, sh);
+  for (int i = tid; i < n; i += NT) {
+    const int dy = sh >= 0 ? i >> sh : i / T, x = i - dy * T;
+    float* e = p + (y0 + dy) * rs + x;
+    const float v = *e + (-edge_mean);
+    *e = v * inverse;
   }
 }
 
