@@ -4,8 +4,10 @@
   (128 x 128 on 16x16x32 MFMAs for M, N >= 128, with a 16-wide k remainder and split over K; 64 x 64,
   128 x 32 for N <= 32, 32 x 128 for M <= 32) and split-K cases of 2, 8 and 19 splits, against float64
   on the same bf16-rounded operands, within 1e-5 of sum |a| |b| per element
-* four networks: channels 8, 16, 16 (strides 2, 2, 1), 16, 32, 48 (strides 2, 1, 2), 16, 48, 64, 32
-  (strides 2, 1, 2, 1) and 8, 24 (strides 2, 2), so that the implicit-product forward convolution (Cin a multiple of 16) runs
+* five networks: channels 8, 16, 16 (strides 2, 2, 1), 16, 32, 48 (strides 2, 1, 2), 16, 48, 64, 32
+  (strides 2, 1, 2, 1), 8, 24 (strides 2, 2) and 16, 160, 32 (strides 2, 1, 2: the 160-channel
+  layers of the config-1-scale network, a partial 64-channel tile in the forward and data-gradient
+  products and 32-channel weight-gradient tiles), so that the implicit-product forward convolution (Cin a multiple of 16) runs
   with one, two and three k steps per tap and 16-, 32-, 48- and 64-channel wave tiles beside the
   im2col path (Cin 1 and 8); the implicit data gradient (Cout a multiple of 16, the previous ReLU's
   backward in its epilogue) runs at stride 1 and over the four pixel-parity classes at stride 2, into
@@ -62,7 +64,7 @@ def _rel(a, b):
 
 
 CASES = {"narrow": ([8, 16, 16], [2, 2, 1]), "wide": ([16, 32, 48], [2, 1, 2]), "deep": ([16, 48, 64, 32], [2, 1, 2, 1]),
-         "odd": ([8, 24], [2, 2])}
+         "odd": ([8, 24], [2, 2]), "c160": ([16, 160, 32], [2, 1, 2])}
 
 
 def _case(kind="narrow"):
