@@ -1464,6 +1464,18 @@ void alloc_for_batch(tm_cnn* h, int batch) {
   h->bin_t.ensure((size_t)cells);
 }
 
+// New parameters (host) with zero Adam moments, on the handle's stream: the enqueued work that still
+// reads or writes them (an Adam step does not wait) finishes first, and the copies complete before
+// the call returns.
+void reset_params(tm_cnn* h, const float* params) {
+  CNN_HIP(hipStreamSynchronize(h->stream));
+  CNN_HIP(hipMemcpyAsync(h->w.p, params, h->n_params * sizeof(float), hipMemcpyHostToDevice, h->stream));
+  CNN_HIP(hipMemsetAsync(h->m.p, 0, h->n_params * sizeof(float), h->stream));
+  CNN_HIP(hipMemsetAsync(h->v.p, 0, h->n_params * sizeof(float), h->stream));
+  CNN_HIP(hipStreamSynchronize(h->stream));
+  h->adam_t = 0;
+}
+
 // defined after the entry points that use them
 void read_loss(tm_cnn* h, float* loss);
 void forward_impl(tm_cnn* h, const float* src, int batch);
@@ -1560,10 +1572,7 @@ int tm_cnn_set_params(tm_cnn* h, const float* params) {
   return cguard([&] {
     require(h);
     CNN_REQUIRE(params, TM_ERR_INVALID, "params is NULL");
-    CNN_HIP(hipMemcpy(h->w.p, params, h->n_params * sizeof(float), hipMemcpyHostToDevice));
-    CNN_HIP(hipMemset(h->m.p, 0, h->n_params * sizeof(float)));
-    CNN_HIP(hipMemset(h->v.p, 0, h->n_params * sizeof(float)));
-    h->adam_t = 0;
+    reset_params(h, params);
   });
 }
 
@@ -1585,10 +1594,7 @@ int tm_cnn_init_params(tm_cnn* h, unsigned long long seed) {
       std::normal_distribution<double> nd(0.0, std::sqrt(2.0 / (double)L.K));
       for (long long e = 0; e < (long long)L.cout * L.K; ++e) p[L.w_off + e] = (float)nd(rng);
     }
-    CNN_HIP(hipMemcpy(h->w.p, p.data(), p.size() * sizeof(float), hipMemcpyHostToDevice));
-    CNN_HIP(hipMemset(h->m.p, 0, h->n_params * sizeof(float)));
-    CNN_HIP(hipMemset(h->v.p, 0, h->n_params * sizeof(float)));
-    h->adam_t = 0;
+    reset_params(h, p.data());
   });
 }
 

@@ -26,7 +26,8 @@
 * backward pass: the loss terms within 1e-4 relative and every parameter tensor's gradient within
   1e-4 (relative L2) of the oracle; the oracle's own gradient is pinned by finite differences on the CPU
 * training: 40 Adam steps on a fixed batch lower the loss by more than half; parameters round-trip
-  through set/get; the seeded initialisation is deterministic
+  through set/get; the seeded initialisation is deterministic; set_params right after an enqueued
+  Adam step takes effect with fresh moments
 * the device-input entry points (tm_cnn_forward_device, tm_cnn_backward_device, tm_cnn_loss) give
   bit-identical location logits, loss terms and (after one Adam step each) parameters to the host
   entry points on the same images and targets; a bin past the last counts as none there
@@ -228,3 +229,27 @@ def test_device_entry_points_match_host_path():
             H.hipFree(b)
         host.close()
         dev.close()
+
+
+def test_set_params_after_an_enqueued_adam_step():
+    """tm_cnn_adam_step only enqueues its update: tm_cnn_set_params right after it leaves exactly the
+    given parameters with fresh moments (the next step equals a new handle's first step bit for bit)."""
+    print("case: cnn set_params right after an enqueued Adam step", flush=True)
+    spec, cfg, flat, images, heat, bins = _case()
+    net = cnn.ProposalNet(spec, max_batch=4, seed=3)
+    other = cnn.ProposalNet(spec, max_batch=4, seed=None)
+    try:
+        net.forward(images, want_logits=False)
+        net.backward(heat, bins, pos_weight=5.0)
+        net.adam_step(lr=1e-2)
+        net.set_params(flat)
+        assert np.array_equal(net.params(), flat)
+        other.set_params(flat)
+        for n in (net, other):
+            n.forward(images, want_logits=False)
+            n.backward(heat, bins, pos_weight=5.0)
+            n.adam_step(lr=1e-2)
+        assert np.array_equal(net.params(), other.params())
+    finally:
+        net.close()
+        other.close()
