@@ -54,8 +54,10 @@ def kernel_bytes(name: str, nb: int, W: int, H: int, T: int, P: int, s: int = 8,
 
 
 def pipeline_bytes_per_ccf(W: int, H: int, B: int, s: int = 8, zs: int = 8) -> float:
-    """Survey formula 5 s Nc + 48 Npix / B bytes per CCF for the whole inner loop, with the
-    inter-pass plane (written and read once) at zs bytes per element: (3 s + 2 zs) Nc + 48 Npix / B."""
+    """Byte model of the whole inner loop per CCF, 5 s Nc + 48 Npix / B (s-byte complex spectra of Nc
+    elements: the input spectrum read, the product formed and transformed; the per-pixel state read
+    and written once per batch of B), with the inter-pass plane (written and read once) at zs bytes
+    per element: (3 s + 2 zs) Nc + 48 Npix / B."""
     Nc = (W // 2 + 1) * H
     return (3 * s + 2 * zs) * Nc + 48.0 * W * H / B
 
