@@ -681,6 +681,10 @@ __device__ __forceinline__ cf row_elem(const cf* __restrict__ rows, size_t i, fl
 // orientations of the batch are spread over gridDim.y (block row y takes b = y, y + gridDim.y, ..),
 // so that small images, with few column tiles, still fill the GPU. Each butterfly reads one twiddle
 // from the global table and forms the others of its stage as powers of it.
+// Largest power-of-two radix of the column pass's plan (the remaining factors of two form one smaller
+// stage): 2592 = 2^5 3^4 runs 8 4 9 9 (324, 648, 288, 288 butterflies for its 324 threads per column)
+// rather than 16 2 9 9 (162 butterflies in the radix-16 stage).
+constexpr int mid_lr(int H) { return H == 2592 ? 3 : 4; }
 template <int H, int KC, int NTC, int ZST = ZS_F32>
 __global__ __launch_bounds__(KC* NTC) void k_mid(const cf* __restrict__ rows, const cf* __restrict__ inpT,
                                                  cf* __restrict__ Z, int Wh, int T, int nb, int ntiles,
@@ -709,7 +713,7 @@ __global__ __launch_bounds__(KC* NTC) void k_mid(const cf* __restrict__ rows, co
       if (kxx < Wh) buf[cc][lpad(jlo + r)] = row_elem<ZST>(rows, rb + (size_t)r * Wh + kxx, rinv);
     }
     __syncthreads();
-    fft_inplace<H, -1, NTC>(buf[c], TwPow{tw}, ctid);
+    fft_inplace<H, -1, NTC, mid_lr(H)>(buf[c], TwPow{tw}, ctid);
     if (valid) {
       const cf* ic = inpT + (size_t)kx * H;
       for (int e = ctid; e < H; e += NTC) {
@@ -719,7 +723,7 @@ __global__ __launch_bounds__(KC* NTC) void k_mid(const cf* __restrict__ rows, co
       }
     }
     __syncthreads();
-    fft_inplace<H, +1, NTC>(buf[c], TwPow{tw}, ctid);
+    fft_inplace<H, +1, NTC, mid_lr(H)>(buf[c], TwPow{tw}, ctid);
     // Z[b][tile][y][KC]: the tile's output is one contiguous block; streamed past L2
     const size_t zo = b * z_bs + (size_t)tile * H * KC;
     for (int e = tid; e < KC * H; e += KC * NTC) {
