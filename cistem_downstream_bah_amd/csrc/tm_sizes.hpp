@@ -47,7 +47,7 @@ constexpr int mid_threads(int H) {
     case 1024: return 64;   // 16 16 4
     case 1536: return 96;   // 16 16 2 3
     case 2048: return 128;  // 16 16 8
-    case 2592: return 324;  // 16 2 9 9
+    case 2592: return 324;  // 8 4 9 9 (mid_lr, tm_kernels.hpp)
     case 2916: return 324;  // 4 9 9 9
     case 3072: return 192;  // 16 16 4 3
     case 5832: return 486;  // 8 9 9 9
